@@ -1,0 +1,209 @@
+/*
+ * libvdvc -- gfx950 kernels for the global Gauss-Newton DVC solve.
+ *
+ * C ABI: plain pointers and sizes, `int` status (0 = ok, < 0 = error, message
+ * from vdvc_last_error()).  Every pointer named d_* is device memory on the
+ * context's device; h_* pointers are host memory.  All work is enqueued on
+ * the stream given at context creation; functions returning host values
+ * synchronise that stream.
+ *
+ * Layouts (the reference's conventions, src/voxeldvc/engine/kernels_dvc.py:9-21):
+ *   voxel volumes: x fastest, index x + y*Nvx + z*Nvx*Nvy, Nv* = N*·h + 1
+ *                  (the reference's ravel(order='F') of an [x][y][z] array);
+ *   gradients:     three planes (gx, gy, gz), each a voxel volume;
+ *   nodal vectors: DOF 3*(i + j*(Nx+1) + k*(Nx+1)*(Ny+1)) + c;
+ *   deformed image g for sampling: x fastest, dims (gnx, gny, gnz).
+ * Element types: VDVC_F32 float32, VDVC_BF16 bfloat16, VDVC_F16 float16.
+ *
+ * The reference has no FFI for this path; its boundary is the Python
+ * function API of voxeldvc.engine with array-module injection.  Each entry
+ * below names the reference function it replaces; voxeldvc_amd.engine binds
+ * them through ctypes with the reference signatures.
+ */
+#ifndef VDVC_H
+#define VDVC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VDVC_ABI_VERSION 1
+
+enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2 };
+enum { VDVC_REG_NONE = 0, VDVC_REG_K = 1, VDVC_REG_EQGAP = 2 };
+
+typedef struct vdvc_ctx vdvc_ctx;
+
+int vdvc_abi_version(void);
+const char* vdvc_last_error(void);
+
+/* Context: one device, one stream (a hipStream_t passed as an integer; 0 is
+ * the null stream), library-owned scratch.  One host thread per context.
+ * (The reference runs on CuPy's null stream, run_dvc.py:134.) */
+int vdvc_ctx_create(int device, uintptr_t stream, vdvc_ctx** out);
+int vdvc_ctx_destroy(vdvc_ctx* ctx);
+int vdvc_synchronize(vdvc_ctx* ctx);
+
+/* Measurement hook (no reference counterpart): between begin and end, every
+ * launch of the voxel-streaming kernel (Hessian matvec / diagonal / RHS,
+ * including those inside vdvc_pcg) is bracketed by HIP events on the context
+ * stream; end synchronises and returns the summed kernel time and the number
+ * of launches. */
+int vdvc_timing_begin(vdvc_ctx* ctx);
+int vdvc_timing_end(vdvc_ctx* ctx, double* h_total_ms, int* h_launches);
+
+/* Reference-image setup: np.gradient per axis (edge_order=1), mask0 = f > glt,
+ * gradient zeroed where the 6-cross erosion of mask0 is false.
+ * Replaces image_grad_and_values + erode_mask0_vox + the zeroing in
+ * correlate_gpu (engine/correlate_gpu.py:49-94, 951-953).
+ * d_f: ftype (F32/F16); gradient planes: gtype (F32/BF16); d_mask0: uint8. */
+int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
+                     float glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
+                     uint8_t* d_mask0);
+
+/* Multiplicity-weighted masked (count, sum, sum of squares) of a voxel field
+ * in FP64.  Mask: d_mask (uint8) or, if use_thresh, field > glt; d_mask NULL
+ * and !use_thresh means all voxels.  Replaces the reductions of
+ * masked_quadpoint_mean_std (engine/correlate_gpu.py:113-157). */
+int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_field,
+                      int ftype, const uint8_t* d_mask, int use_thresh, float glt,
+                      double* h_out3);
+
+/* g(x + u(x)) at every voxel, active mask = in_bounds & mask0 & (g > glt), and
+ * the multiplicity-weighted (count, sum, sum sq) of g over d_bmask (or over
+ * the new mask when d_bmask is NULL).  Replaces sample_deformed_field
+ * (engine/correlate_gpu.py:201-281) fused with the b-mask statistics of
+ * correlate_gpu (:1091).  d_U float32 nodal; d_g gitype (F32/F16). */
+int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, const void* d_g,
+                int gitype, int gnx, int gny, int gnz, float gox, float goy, float goz,
+                float glt, const uint8_t* d_mask0, const uint8_t* d_bmask, float* d_gvals,
+                uint8_t* d_mask_out, double* h_stats3);
+
+/* Residual res = ((g - gmean) * kscale + (f - mean0)) zeroed outside d_bmask
+ * (float32, the reference's operation order), RHS b = sum W bmask res v_eq,
+ * and (weighted count, sum, sum sq, plain voxel count) of res over d_live,
+ * multiplicity-weighted except the last.  If d_fold_x is non-NULL
+ * also writes bfloat16 planes g * bmask.  Replaces correlate_gpu.py:1092-1109
+ * + compute_b_gpu (:341-378) + the display statistics (:1133). */
+int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                      const void* d_gy, const void* d_gz, int gtype, const void* d_f, int ftype,
+                      float mean0, const float* d_gvals, float gmean, float kscale,
+                      const uint8_t* d_bmask, const uint8_t* d_live, float* d_res, float* d_b,
+                      uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
+                      double* h_stats4);
+
+/* out = H v, H = sum_e sum_q w_q m_q v_eq v_eq^T.  d_mask may be NULL (mask
+ * folded into the gradients).  Replaces matvec_H (engine/kernels_dvc.py:452-506). */
+int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                  const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
+                  const float* d_v, float* d_out);
+
+/* out = diag(H).  Replaces build_diagonal_H (engine/kernels_dvc.py:509-543). */
+int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                const void* d_gz, int gtype, const uint8_t* d_mask, float* d_out);
+
+/* out = sum_e E_e P_e^T K P_e v with a 24x24 element matrix (h_K, float64
+ * row-major, applied in float32) and E_e = E or d_E[e].  Replaces
+ * matvec_K_ref (engine/kernels_dvc.py:349-374). */
+int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
+                  const float* d_E, const float* d_v, float* d_out);
+
+/* out = diagonal of the assembled K.  Replaces build_diagonal_K_ref
+ * (engine/kernels_dvc.py:377-392). */
+int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
+                const float* d_E, float* d_out);
+
+/* out = K (P_i (K v)), P_i the interior-node DOF projector.  Replaces
+ * matvec_equilibrium_gap (engine/kernels_dvc.py:402-417). */
+int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
+                      const float* d_E, const float* d_v, float* d_out);
+
+/* Hutchinson estimate diag(R_m) ~ mean_p z_p * (R_m z_p) over n_probes
+ * Rademacher probes d_probes (int8, n_probes x Ndof), accumulated in FP64,
+ * clipped below at 1e-12 * max|d|.  Replaces build_diagonal_equilibrium_gap
+ * (engine/kernels_dvc.py:420-437); the caller supplies the reference's probe
+ * stream. */
+int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
+                          const int8_t* d_probes, int n_probes, float* d_out);
+
+/* Regularizer used by vdvc_jacobi / vdvc_pcg:
+ *   VDVC_REG_NONE:  R = 0
+ *   VDVC_REG_K:     R = assembled E*K1            (Laplacian: K1 = K_ref_laplacian)
+ *   VDVC_REG_EQGAP: R = K1 P_i K1 + eps_reg*K2    (K1 elastic, K2 Laplacian)
+ * (engine/correlate_gpu.py:974-988). */
+
+/* diag = diag(H) + ll*diag(R) (+ ll*d_extra for REG_EQGAP, the precomputed
+ * Hutchinson diagonal of K1 P_i K1) and M_inv = 1/(diag + 1e-5*min diag), on
+ * device.  Replaces the Jacobi set-up of correlate_gpu.py:1042-1047,
+ * 1113-1120. */
+int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+                const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+                const float* d_extra, float* d_diag, float* d_minv);
+
+/* Jacobi-PCG for (H + ll R) x = b from x0 = 0, FP64 dots, stop when
+ * sqrt(r.r/(b.b + 1e-30)) < tol, device-resident loop (the host polls the
+ * stop flag every check_every iterations).  Replaces pcg
+ * (engine/kernels_dvc.py:550-583) with the matvec_full closure of
+ * correlate_gpu.py:1122-1126. */
+int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+             const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+             const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+             const float* d_b, const float* d_minv, float* d_x, float tol, int max_iter,
+             int check_every, int* h_iters, double* h_relres);
+
+/* out = H v + ll R v (the PCG operator).  Replaces matvec_full
+ * (engine/correlate_gpu.py:1122-1126). */
+int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                 const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+                 const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+                 const float* d_v, float* d_out);
+
+/* FP64 dot product of two float32 vectors.  Replaces xp.dot / linalg.norm
+ * (engine/kernels_dvc.py:558-578, engine/correlate_gpu.py:1131). */
+int vdvc_dot(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, double* h_out);
+
+/* y = a*x + b*y (float32). */
+int vdvc_axpby(vdvc_ctx* ctx, int64_t n, float a, const float* d_x, float b, float* d_y);
+
+/* out = a * b elementwise (float32); the Jacobi application z = M_inv r of
+ * pcg (engine/kernels_dvc.py:556, 577). */
+int vdvc_mul(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, float* d_out);
+
+/* Folded bfloat16 gradient planes o = g * mask (raw bf16 bits), the storage
+ * the Hessian kernels read when d_mask is NULL.  No reference counterpart:
+ * the reference keeps gradient and mask apart (engine/kernels_dvc.py:490-500);
+ * H uses m g g^T with m in {0,1}, so folding is exact. */
+int vdvc_fold_grad(vdvc_ctx* ctx, int64_t nvox, const uint16_t* d_gx, const uint16_t* d_gy,
+                   const uint16_t* d_gz, const uint8_t* d_mask, uint16_t* d_ox, uint16_t* d_oy,
+                   uint16_t* d_oz);
+
+/* 6-cross erosion of a uint8 volume (x fastest) with edge replication.
+ * Replaces erode_mask0_vox (engine/correlate_gpu.py:75-94). */
+int vdvc_erode6(vdvc_ctx* ctx, int nx, int ny, int nz, const uint8_t* d_in, uint8_t* d_out);
+
+/* Stride-s decimation of a volume, dst dims ((n-1)/s + 1 per axis).
+ * Replaces f_pix[::s, ::s, ::s] (engine/correlate_gpu.py:1257-1258). */
+int vdvc_stride(vdvc_ctx* ctx, const void* d_src, int type, int nx, int ny, int nz, int s,
+                void* d_dst);
+
+/* Phase correlation of two float32 volumes of equal dims (x fastest):
+ * returns in *h_cidx the argmax of ifftn(F conj(G)/(|.|+1e-12)).real as a
+ * C-order index ((x*ny + y)*nz + z).  Replaces the FFT part of
+ * fft_rigid_shift (engine/correlate_gpu.py:393-405). */
+int vdvc_phase_corr(vdvc_ctx* ctx, const float* d_f, const float* d_g, int nx, int ny, int nz,
+                    int64_t* h_cidx);
+
+/* Affine warp A(x - c) + c + t of every voxel of f, order-1 sample of g, and
+ * the FP64 sums (n, sum f, sum g, sum f^2, sum g^2, sum f g) over in-bounds
+ * samples.  Replaces warp_and_cost / _zncc_masked
+ * (engine/correlate_gpu.py:417-433, 523-536). */
+int vdvc_affine_sums(vdvc_ctx* ctx, const float* d_f, int nx, int ny, int nz, const float* d_g,
+                     int gnx, int gny, int gnz, const double* h_A9, const double* h_t3,
+                     const double* h_c3, double* h_out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VDVC_H */

@@ -1,0 +1,115 @@
+"""Write golden vectors from the reference engine (run once, in the build
+container, where the reference source tree is mounted).
+
+Only the composites that no closed form or brute-force check pins are
+recorded here: the end-to-end single-level and multiscale Gauss-Newton solves
+(normalisation, mask freeze, ll calibration and warm-start conventions), the
+equilibrium-gap solve and its Hutchinson diagonal (RNG stream and probe
+order), and the affine pre-alignment (SciPy Powell trajectory).
+
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+
+Inputs are either stored in the fixture (small) or regenerated from the seeded
+generator in voxeldvc_amd/synth.py and checked by SHA-256.
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, '..', '..')))
+sys.path.insert(0, '/root/reference/src')
+sys.dont_write_bytecode = True
+
+from voxeldvc.engine import correlate_gpu as rc  # noqa: E402
+from voxeldvc.engine import geometry_dvc as rg  # noqa: E402
+from voxeldvc.engine import kernels_dvc as rk  # noqa: E402
+
+from voxeldvc_amd import synth  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def problem17():
+    """The reference test's 17^3 sinusoid texture, g = f shifted by one voxel
+    in x (cubic spline) -- tests/test_dvc_gpu_multiscale.py:24-40."""
+    from scipy.ndimage import shift
+    n = 17
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    f = (np.sin(2 * np.pi * X / n) + np.sin(2 * np.pi * Y / n) + np.sin(2 * np.pi * Z / n)
+         + 0.5 * np.cos(2 * np.pi * (X + Y) / n))
+    g = shift(f, (1.0, 0, 0), order=3, mode='nearest')
+    return f, g
+
+
+def speckle33(seed=7):
+    """33^3 speckle (4 x 4 x 4 elements at h=8), translation + strain."""
+    f = synth.speckle_numpy((33, 33, 33), 600, seed)
+    g = synth.warp_backward(f, synth.translation_strain_field((1.3, -0.6, 0.4), 0.01, 16.0),
+                            1.0, seed + 1000)
+    return f, g
+
+
+def main():
+    out = {}
+    K = rg.build_K_ref_laplacian()
+    f17, g17 = problem17()
+
+    kw = dict(K_ref_laplacian=K, l0=36.0, maxiter=20, eps=1e-3, disp=False)
+    for name, dt in (('corr17_f32', np.float32), ('corr17_f64', np.float64)):
+        U, res, dU = rc.correlate_gpu(f17, g17, 2, 2, 2, 8, np, dtype=dt, **kw)
+        out[name] = dict(f=f17, g=g17, U=U, res=res, dU=dU)
+
+    U, res, dU = rc.multiscale_correlate_gpu(f17, g17, 2, 2, 2, 8, np, scales=(2, 1, 0),
+                                             K_ref_laplacian=K, l0=36.0, maxiter=30,
+                                             eps=1e-3, disp=False)
+    out['multi17_fft'] = dict(f=f17, g=g17, U=U, res=res, dU=dU)
+
+    f33, g33 = speckle33()
+    kw33 = dict(K_ref_laplacian=K, l0=36.0, glt=0.0, dynamic_mask=True,
+                freeze_mask_after=4, disp=False)
+    U, res, dU = rc.multiscale_correlate_gpu(f33, g33, 4, 4, 4, 8, np, scales=(2, 1, 0),
+                                             fft_prealign=True, **kw33)
+    out['multi33_dyn'] = dict(f=f33.astype(np.float32), g=g33.astype(np.float32),
+                              U=U, res=res, dU=dU)
+    # the same solve, fed the float32-rounded inputs stored above
+    U, res, dU = rc.multiscale_correlate_gpu(f33.astype(np.float32), g33.astype(np.float32),
+                                             4, 4, 4, 8, np, scales=(2, 1, 0),
+                                             fft_prealign=True, **kw33)
+    out['multi33_dyn_f32in'] = dict(U=U, res=res, dU=dU)
+
+    Kel = rg.build_K_ref(0.3)
+    U, res, dU = rc.correlate_gpu(f33.astype(np.float32), g33.astype(np.float32), 4, 4, 4, 8,
+                                  np, reg_type='equilibrium_gap', K_ref_elastic=Kel,
+                                  K_ref_laplacian=K, l0=36.0, eps_reg=1e-3, maxiter=10,
+                                  U0=None, disp=False)
+    out['eqgap33'] = dict(U=U, res=res, dU=dU)
+
+    interior = rg.build_interior_dof_mask(3, 3, 3).astype(np.float32)
+    d = rk.build_diagonal_equilibrium_gap(Kel, interior, 8, 3, 3, 3, 3 * 64, np)
+    out['eqgap_diag3'] = dict(diag=d)
+
+    A, t, c, info = rc.affine_prealign(f33.astype(np.float32), g33.astype(np.float32), np,
+                                       decimate=2)
+    out['affine33'] = dict(A=A, t=t, c=c, cost_final=np.float64(info['cost_final']),
+                           cost_identity=np.float64(info['cost_identity']),
+                           nfev=np.int64(info['nfev']))
+
+    d = rc.fft_rigid_shift(f33, g33, np)
+    out['fft33'] = dict(d=d)
+
+    for name, arrs in out.items():
+        np.savez_compressed(os.path.join(HERE, name + '.npz'), **arrs)
+    with open(os.path.join(HERE, 'INPUT_SHA256.txt'), 'w') as fh:
+        fh.write('speckle33_f %s\nspeckle33_g %s\n' % (sha(f33), sha(g33)))
+        fh.write('problem17_f %s\nproblem17_g %s\n' % (sha(f17), sha(g17)))
+    print('wrote', sorted(out))
+
+
+if __name__ == '__main__':
+    main()

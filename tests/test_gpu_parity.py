@@ -1,0 +1,297 @@
+"""GPU parity: every libvdvc entry point against the oracle on seeded inputs
+(float32 results within the tolerance written in each test; integer / mask
+results exact)."""
+import numpy as np
+import pytest
+
+from conftest import golden, problem17, rand_problem
+from oracle import dvc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, 2, 2, 8), (3, 2, 4, 8), (1, 1, 1, 1), (9, 2, 3, 8), (17, 2, 2, 4), (5, 3, 2, 3),
+          (4, 2, 3, 5), (70, 1, 2, 1), (2, 1, 3, 16), (3, 2, 2, 32), (13, 3, 3, 6), (8, 8, 8, 4)]
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(1e-30, np.abs(b).max()))
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+def test_matvec_H(gpu, shape):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz, h = shape
+    print('matvec_H', shape, flush=True)
+    g, m, v = rand_problem(1, Nx, Ny, Nz, h)
+    ref = O.matvec_H(v.astype(np.float64), g.astype(np.float64), m, h, Nx, Ny, Nz)
+    got = K.matvec_H(v, g, m, None, None, h, Nx, Ny, Nz)
+    assert rel(got, ref) < 2e-5
+    # mask folded into the gradient: same operator
+    got2 = K.matvec_H(v, g * m[:, None], None, None, None, h, Nx, Ny, Nz)
+    assert rel(got2, ref) < 2e-5
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+def test_diag_H(gpu, shape):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz, h = shape
+    print('diag_H', shape, flush=True)
+    g, m, _ = rand_problem(2, Nx, Ny, Nz, h)
+    ref = O.build_diagonal_H(g.astype(np.float64), m, h, Nx, Ny, Nz)
+    got = K.build_diagonal_H(g, m, None, None, h, Nx, Ny, Nz, ref.size)
+    assert rel(got, ref) < 2e-5
+
+
+@pytest.mark.parametrize('shape', SHAPES[:8])
+def test_compute_b(gpu, shape):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    Nx, Ny, Nz, h = shape
+    print('compute_b', shape, flush=True)
+    g, m, _ = rand_problem(3, Nx, Ny, Nz, h)
+    res = np.random.default_rng(4).normal(size=m.size).astype(np.float32)
+    ref = O.compute_b(res.astype(np.float64), m, g.astype(np.float64), h, Nx, Ny, Nz)
+    got = C.compute_b_gpu(res, m, g, None, None, h, Nx, Ny, Nz, ref.size)
+    assert rel(got, ref) < 2e-5
+
+
+@pytest.mark.parametrize('shape', [(2, 2, 2, 1), (3, 4, 2, 1), (5, 5, 5, 1)])
+def test_matvec_K_ref(gpu, shape):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz, _ = shape
+    print('matvec_K', shape, flush=True)
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    v = np.random.default_rng(5).normal(size=Ndof).astype(np.float32)
+    for Kr in (O.build_K_ref_laplacian(), O.build_K_ref(0.3)):
+        ref = O.matvec_K_ref(v.astype(np.float64), Kr, 8, Nx, Ny, Nz)
+        assert rel(K.matvec_K_ref(v, Kr, 8, Nx, Ny, Nz), ref) < 1e-5
+        dref = O.build_diagonal_K_ref(Kr, 8, Nx, Ny, Nz)
+        assert rel(K.build_diagonal_K_ref(Kr, 8, Nx, Ny, Nz, Ndof), dref) < 1e-6
+        E = np.random.default_rng(6).uniform(1, 3, Nx * Ny * Nz).astype(np.float32)
+        ref = O.matvec_K_ref(v.astype(np.float64), Kr, E.astype(np.float64), Nx, Ny, Nz)
+        assert rel(K.matvec_K_ref(v, Kr, E, Nx, Ny, Nz), ref) < 1e-5
+
+
+def test_equilibrium_gap(gpu):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz = 3, 4, 5
+    Kel = O.build_K_ref(0.3)
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    im = O.build_interior_dof_mask(Nx, Ny, Nz)
+    v = np.random.default_rng(7).normal(size=Ndof).astype(np.float32)
+    ref = O.matvec_equilibrium_gap(v.astype(np.float64), Kel, im, 8, Nx, Ny, Nz)
+    assert rel(K.matvec_equilibrium_gap(v, Kel, im, 8, Nx, Ny, Nz), ref) < 1e-5
+    # Hutchinson diagonal against the reference's own values (golden)
+    z = golden('eqgap_diag3')
+    d = K.build_diagonal_equilibrium_gap(Kel, O.build_interior_dof_mask(3, 3, 3), 8, 3, 3, 3,
+                                         192)
+    assert rel(d, z['diag']) < 1e-5
+
+
+def test_masked_stats_and_setup(gpu):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    Nx, Ny, Nz, h = 3, 2, 4, 4
+    rng = np.random.default_rng(8)
+    f = rng.integers(0, 256, size=(Nx * h + 1, Ny * h + 1, Nz * h + 1)).astype(np.float32)
+    f_vox, grad = C.image_grad_and_values(f)
+    rf, rg = O.image_grad_and_values(f)
+    assert np.array_equal(f_vox, rf)
+    assert np.array_equal(grad, rg)          # bit-exact float32 np.gradient
+    m = rng.random(f_vox.size) < 0.7
+    mean, std = C.masked_quadpoint_mean_std(f_vox, m, h, Nx, Ny, Nz)
+    rmean, rstd = O.masked_quadpoint_mean_std(f_vox, m, h, Nx, Ny, Nz)
+    assert abs(mean - rmean) < 1e-9 * abs(rmean) and abs(std - rstd) < 1e-9 * rstd
+    er = C.erode_mask0_vox(m, Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    assert np.array_equal(er, O.erode_mask_6(m, Nx * h + 1, Ny * h + 1, Nz * h + 1))
+
+
+@pytest.mark.parametrize('go', [(0, 0, 0), (2, -1, 1)])
+def test_sample_deformed_field(gpu, go):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    Nx, Ny, Nz, h = 3, 2, 2, 8
+    rng = np.random.default_rng(9)
+    gimg = rng.uniform(0, 255, size=(Nx * h + 3, Ny * h + 1, Nz * h + 2)).astype(np.float32)
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    U = rng.normal(scale=2.0, size=Ndof).astype(np.float32)
+    m0 = rng.random((Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)) < 0.9
+    rv, rm = O.sample_deformed_field(U, gimg, h, Nx, Ny, Nz, go, m0, glt=30.0)
+    gv, gm = C.sample_deformed_field(U, gimg, None, h, Nx, Ny, Nz, g_origin=go, mask0_vox=m0,
+                                     glt=30.0)
+    # u at a voxel is formed by a different float32 summation order; allow a
+    # few in-bounds / threshold flips at the boundary of the box
+    assert (gm != rm).mean() < 2e-3
+    ok = gm == rm
+    assert np.abs(gv[ok] - rv[ok]).max() < 2e-3
+
+
+def test_stride_and_phase_corr(gpu):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    rng = np.random.default_rng(42)
+    f = rng.uniform(0, 1, size=(32, 32, 32))
+    g = np.roll(f, (3, -2, 1), axis=(0, 1, 2))
+    assert np.allclose(C.fft_rigid_shift(f, g), (3, -2, 1))
+    # non-cubic, unequal shapes (common corner crop)
+    f2 = rng.uniform(0, 1, size=(20, 18, 15))
+    g2 = np.roll(f2, (-4, 5, 2), axis=(0, 1, 2))
+    g2 = np.pad(g2, ((0, 3), (0, 0), (0, 2)))
+    assert np.allclose(C.fft_rigid_shift(f2, g2), O.fft_rigid_shift(f2, g2))
+
+
+def test_affine_cost(gpu):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('multi33_dyn')
+    f = z['f'][::2, ::2, ::2]
+    g = z['g'][::2, ::2, ::2]
+    A = O.build_affine_matrix([0.02, -0.01, 0.03, 1.01, 0.99, 1.0])
+    t = np.array([0.5, -0.3, 0.2])
+    c = np.array([(n - 1) / 2.0 for n in f.shape])
+    ref = O.affine_cost(f, g, A, t, c)
+    from voxeldvc_amd import _lib as L
+    fd, fs = L.to_dev_volume(f)
+    gd, gs = L.to_dev_volume(g)
+    out = (L._d * 6)()
+    L.call('vdvc_affine_sums', L.ctx(), L.ptr(fd), *fs, L.ptr(gd), *gs, L.dbl(*A.ravel()),
+           L.dbl(*t), L.dbl(*c), out)
+    assert abs(C._zncc_cost(list(out)) - ref) < 1e-6
+
+
+def test_pcg_matches_oracle(gpu):
+    """Device-resident PCG on (H + ll L) against the oracle's PCG."""
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = 4, 3, 3, 4
+    g, m, b = rand_problem(10, Nx, Ny, Nz, h)
+    Klap = O.build_K_ref_laplacian()
+    ll = 0.7
+    A = lambda x: (O.matvec_H(x, g.astype(np.float64), m, h, Nx, Ny, Nz)
+                   + ll * O.matvec_K_ref(x, Klap, h, Nx, Ny, Nz))
+    d = O.build_diagonal_H(g.astype(np.float64), m, h, Nx, Ny, Nz) + ll * O.build_diagonal_K_ref(
+        Klap, h, Nx, Ny, Nz)
+    Minv = 1.0 / (d + 1e-5 * d.min())
+    xr, itr = O.pcg(A, b.astype(np.float64), Minv, max_iter=500, tol=1e-6)
+    gx, gy, gz = (L.to_dev_flat(np.ascontiguousarray(g[:, c])) for c in range(3))
+    md = L.to_dev_flat(m, 'uint8')
+    diag = L.empty(b.size)
+    minv = L.empty(b.size)
+    Kp = np.ascontiguousarray(Klap)
+    L.call('vdvc_jacobi', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+           L.ptr(md), L.REG_K, Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, None,
+           L.ptr(diag), L.ptr(minv))
+    assert rel(L.to_host(minv), Minv) < 1e-5
+    bd = L.to_dev_flat(b)
+    x = L.empty(b.size)
+    it = (L._i * 1)()
+    rr = (L._d * 1)()
+    L.call('vdvc_pcg', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32, L.ptr(md),
+           L.REG_K, Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, L.ptr(bd), L.ptr(minv),
+           L.ptr(x), 1e-6, 500, 7, it, rr)
+    assert rr[0] < 1e-6
+    assert abs(it[0] - itr) <= 2
+    assert rel(L.to_host(x), xr) < 1e-4
+
+
+def _eqgap_problem():
+    Nx, Ny, Nz, h = 3, 3, 2, 4
+    g, m, v = rand_problem(11, Nx, Ny, Nz, h)
+    return Nx, Ny, Nz, h, g, m, v
+
+
+@pytest.mark.parametrize('kind', ['laplacian', 'equilibrium_gap'])
+def test_apply_A_and_jacobi(gpu, kind):
+    """The PCG operator H + ll R and its Jacobi preconditioner for both
+    regularizers against the oracle (correlate_gpu.py:974-988, 1042-1047)."""
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    Nx, Ny, Nz, h, g, m, v = _eqgap_problem()
+    Ndof = v.size
+    Klap = O.build_K_ref_laplacian()
+    Kel = O.build_K_ref(0.3)
+    ll, eps = 0.37, 1e-3
+    g64 = g.astype(np.float64)
+    im = O.build_interior_dof_mask(Nx, Ny, Nz)
+    if kind == 'laplacian':
+        R = lambda x: O.matvec_K_ref(x, Klap, h, Nx, Ny, Nz)
+        dR = O.build_diagonal_K_ref(Klap, h, Nx, Ny, Nz)
+        rk, K1, K2, extra = L.REG_K, Klap, None, None
+    else:
+        R = lambda x: (O.matvec_equilibrium_gap(x, Kel, im, h, Nx, Ny, Nz)
+                       + eps * O.matvec_K_ref(x, Klap, h, Nx, Ny, Nz))
+        hutch = O.build_diagonal_equilibrium_gap(Kel, im, h, Nx, Ny, Nz)
+        dR = hutch + eps * O.build_diagonal_K_ref(Klap, h, Nx, Ny, Nz)
+        rk, K1, K2 = L.REG_EQGAP, Kel, Klap
+        extra = L.to_dev_flat(Kd.build_diagonal_equilibrium_gap(Kel, im, h, Nx, Ny, Nz, Ndof))
+        assert rel(L.to_host(extra), hutch) < 1e-5
+    ref = O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz) + ll * R(v.astype(np.float64))
+    gx, gy, gz = (L.to_dev_flat(np.ascontiguousarray(g[:, c])) for c in range(3))
+    md = L.to_dev_flat(m, 'uint8')
+    K1p = np.ascontiguousarray(K1)
+    K2p = None if K2 is None else np.ascontiguousarray(K2)
+    kp = lambda a: None if a is None else a.ctypes.data_as(L._dp)
+    vd = L.to_dev_flat(v)
+    out = L.empty(Ndof)
+    L.call('vdvc_apply_A', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+           L.ptr(md), rk, kp(K1p), kp(K2p), float(h), eps, ll, L.ptr(vd), L.ptr(out))
+    assert rel(L.to_host(out), ref) < 2e-5
+    d = O.build_diagonal_H(g64, m, h, Nx, Ny, Nz) + ll * dR
+    Minv = 1.0 / (d + 1e-5 * d.min())
+    diag = L.empty(Ndof)
+    minv = L.empty(Ndof)
+    L.call('vdvc_jacobi', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+           L.ptr(md), rk, kp(K1p), kp(K2p), float(h), eps, ll, L.ptr(extra), L.ptr(diag),
+           L.ptr(minv))
+    assert rel(L.to_host(diag), d) < 2e-5
+    assert rel(L.to_host(minv), Minv) < 2e-5
+    # PCG on this operator
+    A = lambda x: O.matvec_H(x, g64, m, h, Nx, Ny, Nz) + ll * R(x)
+    xr, itr = O.pcg(A, v.astype(np.float64), Minv, max_iter=400, tol=1e-6)
+    x = L.empty(Ndof)
+    it = (L._i * 1)()
+    rr = (L._d * 1)()
+    L.call('vdvc_pcg', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32, L.ptr(md),
+           rk, kp(K1p), kp(K2p), float(h), eps, ll, L.ptr(vd), L.ptr(minv), L.ptr(x), 1e-6, 400, 5,
+           it, rr)
+    print(kind, 'pcg iters gpu', it[0], 'oracle', itr, flush=True)
+    assert rr[0] < 1e-6
+    assert rel(L.to_host(x), xr) < 1e-3
+
+
+def test_correlate_golden_17(gpu):
+    """End-to-end single-level solve on the reference test problem against the
+    reference's own output (1e-3 voxel max-abs at FP32)."""
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('corr17_f32')
+    K = O.build_K_ref_laplacian()
+    U, res, dU = C.correlate_gpu(z['f'], z['g'], 2, 2, 2, 8, np, K_ref_laplacian=K, l0=36.0,
+                                 maxiter=20, eps=1e-3, disp=False)
+    assert np.abs(U - z['U']).max() < 1e-3
+    # residuals agree except where an in-bounds test flips at the box edge
+    both = (res != 0) & (z['res'] != 0)
+    assert both.mean() > 0.9
+    assert np.abs(res[both] - z['res'][both]).max() < 2e-3
+
+
+def test_multiscale_golden_33(gpu):
+    """Pyramid + FFT prealign + dynamic mask + freeze on 33^3 speckle against the
+    reference's own output."""
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('multi33_dyn')
+    zr = golden('multi33_dyn_f32in')
+    K = O.build_K_ref_laplacian()
+    U, res, dU = C.multiscale_correlate_gpu(z['f'], z['g'], 4, 4, 4, 8, np, scales=(2, 1, 0),
+                                            fft_prealign=True, K_ref_laplacian=K, l0=36.0,
+                                            glt=0.0, dynamic_mask=True, freeze_mask_after=4,
+                                            disp=False)
+    assert np.abs(U - zr['U']).max() < 1e-3
+
+
+def test_eqgap_golden_33(gpu):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('multi33_dyn')
+    zr = golden('eqgap33')
+    log, logr = [], []
+    kw = dict(reg_type='equilibrium_gap', K_ref_elastic=O.build_K_ref(0.3),
+              K_ref_laplacian=O.build_K_ref_laplacian(), l0=36.0, eps_reg=1e-3, maxiter=10)
+    U, res, dU = C.correlate_gpu(z['f'], z['g'], 4, 4, 4, 8, np, disp=False, log=log, **kw)
+    O.correlate(z['f'], z['g'], 4, 4, 4, 8, log=logr, **kw)
+    for a, b in zip(log, logr):
+        print('gpu', a, flush=True)
+        print('ora', b, flush=True)
+    assert np.abs(U - zr['U']).max() < 1e-3

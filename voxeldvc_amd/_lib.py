@@ -1,0 +1,206 @@
+"""ctypes binding of libvdvc.so (include/vdvc.h) and device plumbing.
+
+The library is the only compute path: if it is missing or no GPU is visible,
+every entry point raises.  torch is used only to own device memory and to
+provide the stream handle the library enqueues on.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libvdvc.so')
+
+F32, BF16, F16 = 0, 1, 2
+REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# name -> argument types (all return int status)
+SIGNATURES = {
+    'vdvc_abi_version': [],
+    'vdvc_ctx_create': [_i, ctypes.c_size_t, ctypes.POINTER(_vp)],
+    'vdvc_ctx_destroy': [_vp],
+    'vdvc_synchronize': [_vp],
+    'vdvc_timing_begin': [_vp],
+    'vdvc_timing_end': [_vp, _dp, _ip],
+    'vdvc_image_setup': [_vp, _i, _i, _i, _i, _vp, _i, _f, _vp, _vp, _vp, _i, _vp],
+    'vdvc_masked_stats': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _f, _dp],
+    'vdvc_sample': [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _vp, _vp,
+                    _vp, _vp, _dp],
+    'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _f, _vp, _f, _f, _vp,
+                          _vp, _vp, _vp, _vp, _vp, _vp, _dp],
+    'vdvc_matvec_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
+    'vdvc_diag_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
+    'vdvc_matvec_K': [_vp, _i, _i, _i, _dp, _f, _vp, _vp, _vp],
+    'vdvc_diag_K': [_vp, _i, _i, _i, _dp, _f, _vp, _vp],
+    'vdvc_matvec_eqgap': [_vp, _i, _i, _i, _dp, _f, _vp, _vp, _vp],
+    'vdvc_hutchinson_eqgap': [_vp, _i, _i, _i, _dp, _f, _vp, _i, _vp],
+    'vdvc_jacobi': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp,
+                    _vp, _vp],
+    'vdvc_pcg': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp, _vp,
+                 _vp, _f, _i, _i, _ip, _dp],
+    'vdvc_apply_A': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp,
+                     _vp],
+    'vdvc_dot': [_vp, _i64, _vp, _vp, _dp],
+    'vdvc_axpby': [_vp, _i64, _f, _vp, _f, _vp],
+    'vdvc_mul': [_vp, _i64, _vp, _vp, _vp],
+    'vdvc_fold_grad': [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'vdvc_erode6': [_vp, _i, _i, _i, _vp, _vp],
+    'vdvc_stride': [_vp, _vp, _i, _i, _i, _i, _i, _vp],
+    'vdvc_phase_corr': [_vp, _vp, _vp, _i, _i, _i, _i64p],
+    'vdvc_affine_sums': [_vp, _vp, _i, _i, _i, _vp, _i, _i, _i, _dp, _dp, _dp, _dp],
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load libvdvc.so (raises LibraryMissing when it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing('%s not built: run `python -m voxeldvc_amd.build`' % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        lib.vdvc_last_error.argtypes = []
+        lib.vdvc_last_error.restype = ctypes.c_char_p
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    """Map a library status to the reference's exception types."""
+    if rc == 0:
+        return
+    msg = _lib.vdvc_last_error().decode(errors='replace')
+    if rc == -1:
+        raise ValueError(msg)
+    raise RuntimeError('libvdvc error %d: %s' % (rc, msg))
+
+
+def call(name, *args):
+    lib = load()
+    check(getattr(lib, name)(*args))
+
+
+# ---------------------------------------------------------------------------
+# device context and arrays
+# ---------------------------------------------------------------------------
+
+_ctx = {}
+
+
+def torch():
+    import torch as _t
+    return _t
+
+
+def require_gpu():
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError('voxeldvc_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+    return t
+
+
+def ctx():
+    """The library context bound to torch's current device and stream."""
+    t = require_gpu()
+    load()
+    dev = t.cuda.current_device()
+    st = t.cuda.current_stream(dev).cuda_stream
+    key = (dev, st)
+    c = _ctx.get(key)
+    if c is None:
+        h = _vp()
+        check(_lib.vdvc_ctx_create(dev, st, ctypes.byref(h)))
+        c = h
+        _ctx[key] = c
+    return c
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else _vp(t.data_ptr())
+
+
+def dbl(*vals):
+    arr = (ctypes.c_double * len(vals))(*vals)
+    return arr
+
+
+def empty(n, dtype='float32'):
+    t = require_gpu()
+    return t.empty(int(n), dtype=getattr(t, dtype), device='cuda')
+
+
+def zeros(n, dtype='float32'):
+    t = require_gpu()
+    return t.zeros(int(n), dtype=getattr(t, dtype), device='cuda')
+
+
+def is_tensor(a):
+    try:
+        t = torch()
+    except ImportError:
+        return False
+    return isinstance(a, t.Tensor)
+
+
+def to_dev_flat(a, dtype='float32'):
+    """1-D array (numpy or tensor) -> contiguous device tensor of `dtype`."""
+    t = require_gpu()
+    tt = getattr(t, dtype)
+    if is_tensor(a):
+        return a.detach().to(device='cuda', dtype=tt).contiguous().view(-1)
+    arr = np.ascontiguousarray(np.asarray(a).reshape(-1))
+    if dtype == 'uint8':
+        arr = arr.astype(np.uint8)
+    return t.from_numpy(arr).to(device='cuda', dtype=tt)
+
+
+def to_dev_volume(a, dtype='float32'):
+    """(nx, ny, nz) volume (numpy C-order or tensor) -> device tensor laid out x
+    fastest (flat index x + y*nx + z*nx*ny), plus its shape."""
+    t = require_gpu()
+    tt = getattr(t, dtype)
+    if is_tensor(a):
+        if a.dim() != 3:
+            raise ValueError('volume must be 3-D')
+        shp = tuple(int(s) for s in a.shape)
+        d = a.detach().to(device='cuda', dtype=tt).permute(2, 1, 0).contiguous().view(-1)
+        return d, shp
+    arr = np.asarray(a)
+    if arr.ndim != 3:
+        raise ValueError('volume must be 3-D')
+    shp = arr.shape
+    if dtype == 'float32':
+        host = np.ravel(arr.astype(np.float32, copy=False), order='F')
+    elif dtype == 'float16':
+        host = np.ravel(arr.astype(np.float16, copy=False), order='F')
+    else:
+        host = np.ravel(arr, order='F')
+    return t.from_numpy(np.ascontiguousarray(host)).to('cuda'), shp
+
+
+def to_host(x):
+    return x.detach().cpu().numpy()

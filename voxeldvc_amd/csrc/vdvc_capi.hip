@@ -1,0 +1,756 @@
+// C-ABI entry points (declared in include/vdvc.h): argument checks, scratch
+// management, tile planning and kernel sequencing.  No compute happens on the
+// host; every numeric step is one of the gfx950 kernels in this directory.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vdvc.h"
+#include "vdvc_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define VDVC_HIP(call)                                                                    \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return fail(-2, "%s: %s", #call, hipGetErrorString(e_));         \
+  } while (0)
+#define VDVC_LAUNCH(call)                                                                 \
+  do {                                                                                    \
+    int e_ = (call);                                                                      \
+    if (e_ != 0) return fail(-2, "%s: %s", #call, hipGetErrorString((hipError_t)e_));     \
+  } while (0)
+
+// grow-only device scratch buffer
+struct Buf {
+  void* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= n) return 0;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      n = 0;
+      if (e != hipSuccess) return fail(-2, "hipFree: %s", hipGetErrorString(e));
+    }
+    size_t want = bytes + bytes / 8 + 256;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(-3, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    n = want;
+    return 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+  ~Buf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct vdvc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Buf P;          // voxel-kernel node partials
+  Buf stats;      // per-wave / per-block FP64 partials
+  Buf red;        // reduced FP64 results
+  Buf nodeA, nodeB, nodeC;  // node-vector temporaries
+  Buf pr, pz, pp, pAp;      // PCG vectors
+  Buf dparts, dparts2;      // FP64 dot partials
+  Buf pstate;               // PcgState
+  Buf fmin;                 // per-block minima
+  Buf K1, K2;               // element matrices (float32 24x24)
+  Buf hacc;                 // Hutchinson FP64 accumulator
+  Buf dft;                  // DFT work volumes
+  double* hpin = nullptr;   // pinned host staging (64 doubles)
+  std::vector<double> k1_cache, k2_cache;
+  // optional per-launch timing of the voxel-streaming kernels (HIP events
+  // recorded on the context stream around each launch)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int check_mesh(int Nx, int Ny, int Nz, int h) {
+  if (Nx < 1 || Ny < 1 || Nz < 1 || h < 1)
+    return fail(-1, "invalid mesh Nx=%d Ny=%d Nz=%d h=%d", Nx, Ny, Nz, h);
+  const long long nvox = (long long)(Nx * (long long)h + 1) * (Ny * (long long)h + 1) *
+                         (Nz * (long long)h + 1);
+  const long long ndof = 3LL * (Nx + 1) * (Ny + 1) * (Nz + 1);
+  if (nvox > 0x7fffffffLL || ndof > 0x7fffffffLL)
+    return fail(-1, "Nvox=%lld / Ndof=%lld exceed int32 range; matrix-free index arrays "
+                    "assume int32 indices", nvox, ndof);
+  return 0;
+}
+
+MeshDev make_mesh(int Nx, int Ny, int Nz, int h) {
+  MeshDev M;
+  M.Nx = Nx; M.Ny = Ny; M.Nz = Nz; M.h = h;
+  M.Nvx = Nx * h + 1; M.Nvy = Ny * h + 1; M.Nvz = Nz * h + 1;
+  M.nnx = Nx + 1; M.nny = Ny + 1; M.nnz = Nz + 1;
+  M.Nvox = (long long)M.Nvx * M.Nvy * M.Nvz;
+  M.Nnodes = (long long)M.nnx * M.nny * M.nnz;
+  M.inv_h = 1.0f / (float)h;
+  return M;
+}
+
+// wave decomposition: 64-voxel x strips x element rows x z tiles, with the z
+// tile chosen so that the launch has enough wavefronts to fill 256 CUs
+TilesDev make_tiles(const MeshDev& M) {
+  TilesDev T;
+  T.nsx = (M.Nvx + 63) / 64;
+  const long long base = (long long)T.nsx * M.Ny;
+  const long long target = 16384;
+  long long nzt_t = (target + base - 1) / base;
+  if (nzt_t < 1) nzt_t = 1;
+  int ez = (int)((M.Nz + nzt_t - 1) / nzt_t);
+  if (ez < 1) ez = 1;
+  T.ez = ez;
+  T.nzt = (M.Nz + ez - 1) / ez;
+  T.px = 63 / M.h + 3;
+  T.wave_stride = (long long)(ez + 1) * 2 * T.px * 3;
+  T.nwaves = base * T.nzt;
+  return T;
+}
+
+int gtype_ok(int t) { return t == VDVC_F32 || t == VDVC_BF16; }
+int ftype_ok(int t) { return t == VDVC_F32 || t == VDVC_F16; }
+
+int upload_K(vdvc_ctx* ctx, Buf& dst, std::vector<double>& cache, const double* hK,
+             int* blockdiag) {
+  if (!hK) return fail(-1, "element matrix is NULL");
+  bool same = cache.size() == 576 && std::memcmp(cache.data(), hK, 576 * sizeof(double)) == 0;
+  bool bd = true;
+  for (int a = 0; a < 8 && bd; ++a)
+    for (int b = 0; b < 8 && bd; ++b)
+      for (int c = 0; c < 3 && bd; ++c)
+        for (int d = 0; d < 3 && bd; ++d) {
+          const double v = hK[(3 * a + c) * 24 + 3 * b + d];
+          if (c != d && v != 0.0) bd = false;
+          if (c == d && v != hK[(3 * a) * 24 + 3 * b]) bd = false;
+        }
+  *blockdiag = bd ? 1 : 0;
+  if (same && dst.p) return 0;
+  int rc = dst.ensure(576 * sizeof(float));
+  if (rc) return rc;
+  float tmp[576];
+  for (int q = 0; q < 576; ++q) tmp[q] = (float)hK[q];
+  VDVC_HIP(hipMemcpyAsync(dst.p, tmp, sizeof(tmp), hipMemcpyHostToDevice, ctx->stream));
+  VDVC_HIP(hipStreamSynchronize(ctx->stream));
+  cache.assign(hK, hK + 576);
+  return 0;
+}
+
+int fetch(vdvc_ctx* ctx, const void* d, size_t bytes, void* h) {
+  if (bytes > 64 * sizeof(double)) return fail(-1, "fetch too large");
+  VDVC_HIP(hipMemcpyAsync(ctx->hpin, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  VDVC_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(h, ctx->hpin, bytes);
+  return 0;
+}
+
+VoxIn vox_in(const void* gx, const void* gy, const void* gz, const uint8_t* mask) {
+  VoxIn in;
+  std::memset(&in, 0, sizeof(in));
+  in.g[0] = gx; in.g[1] = gy; in.g[2] = gz;
+  in.mask = mask;
+  return in;
+}
+
+int timed_event(vdvc_ctx* ctx) {
+  if (!ctx->timing) return 0;
+  if (ctx->ev_used == ctx->ev.size()) {
+    hipEvent_t e;
+    VDVC_HIP(hipEventCreate(&e));
+    ctx->ev.push_back(e);
+  }
+  VDVC_HIP(hipEventRecord(ctx->ev[ctx->ev_used++], ctx->stream));
+  return 0;
+}
+
+int launch_restrict_timed(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, const TilesDev& T,
+                          const VoxIn& in, int ftype, bool fold, const float* v) {
+  int rc = timed_event(ctx);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_restrict(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
+                                    ctx->P.as<float>(), ctx->stream));
+  return timed_event(ctx);
+}
+
+int run_restrict(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, const TilesDev& T,
+                 const VoxIn& in, int ftype, bool fold, const float* v) {
+  int rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * sizeof(float));
+  if (rc) return rc;
+  return launch_restrict_timed(ctx, mode, gtype, M, T, in, ftype, fold, v);
+}
+
+NodeOp node_op0() {
+  NodeOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.cH = 1.0f;
+  return op;
+}
+
+KOp kop(const Buf& b, int blockdiag, float E, const float* Earr) {
+  KOp k;
+  k.K = b.as<const float>();
+  k.blockdiag = blockdiag;
+  k.E = E;
+  k.Earr = Earr;
+  return k;
+}
+
+// out = Hpartials (if P) + ll * R v, with R per reg_kind; eq-gap uses tmp
+// for the inner P_i K v; optional FP64 dot partials of (dotv . out)
+int apply_reg_sum(vdvc_ctx* ctx, const MeshDev& M, const TilesDev* T, bool useP, int reg_kind,
+                  int bd1, int bd2, float E, float eps_reg, float ll, const float* v, float* out,
+                  const float* dotv, double* dot_part, const int* done) {
+  NodeOp op = node_op0();
+  if (useP) { op.P = ctx->P.as<const float>(); op.T = *T; }
+  op.done = done;
+  if (reg_kind == VDVC_REG_K && ll != 0.0f) {
+    op.k1 = kop(ctx->K1, bd1, E, nullptr); op.v1 = v; op.c1 = ll; op.k1_on = 1;
+  }
+  if (reg_kind != VDVC_REG_EQGAP || ll == 0.0f) {
+    op.out = out; op.dotv = dotv; op.dot_part = dot_part;
+    VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+    return 0;
+  }
+  // equilibrium gap: out = H v (partials); w = P_i (K1 v); out += ll (K1 w + eps K2 v)
+  int rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(float));
+  if (rc) return rc;
+  float* w = ctx->nodeC.as<float>();
+  op.out = out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  NodeOp o2 = node_op0();
+  o2.done = done;
+  o2.k1 = kop(ctx->K1, bd1, E, nullptr); o2.v1 = v; o2.c1 = 1.0f; o2.k1_on = 1;
+  o2.interior = 1;
+  o2.out = w;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
+  NodeOp o3 = node_op0();
+  o3.done = done;
+  o3.addin = out; o3.cadd = 1.0f;
+  o3.k1 = kop(ctx->K1, bd1, E, nullptr); o3.v1 = w; o3.c1 = ll; o3.k1_on = 1;
+  o3.k2 = kop(ctx->K2, bd2, E, nullptr); o3.v2 = v; o3.c2 = ll * eps_reg; o3.k2_on = 1;
+  o3.out = out; o3.dotv = dotv; o3.dot_part = dot_part;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o3, ctx->stream));
+  return 0;
+}
+
+int prep_reg(vdvc_ctx* ctx, int reg_kind, const double* hK1, const double* hK2, int* bd1,
+             int* bd2) {
+  *bd1 = *bd2 = 0;
+  if (reg_kind == VDVC_REG_NONE) return 0;
+  if (reg_kind != VDVC_REG_K && reg_kind != VDVC_REG_EQGAP)
+    return fail(-1, "reg_kind must be VDVC_REG_NONE, VDVC_REG_K or VDVC_REG_EQGAP");
+  int rc = upload_K(ctx, ctx->K1, ctx->k1_cache, hK1, bd1);
+  if (rc) return rc;
+  if (reg_kind == VDVC_REG_EQGAP) return upload_K(ctx, ctx->K2, ctx->k2_cache, hK2, bd2);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vdvc_abi_version(void) { return VDVC_ABI_VERSION; }
+
+const char* vdvc_last_error(void) { return g_err.c_str(); }
+
+int vdvc_ctx_create(int device, uintptr_t stream, vdvc_ctx** out) {
+  if (!out) return fail(-1, "out is NULL");
+  int ndev = 0;
+  VDVC_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(-1, "device %d not present (%d devices)", device, ndev);
+  VDVC_HIP(hipSetDevice(device));
+  vdvc_ctx* c = new vdvc_ctx();
+  c->device = device;
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipHostMalloc((void**)&c->hpin, 64 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(-3, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return 0;
+}
+
+int vdvc_ctx_destroy(vdvc_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  if (ctx->hpin) (void)hipHostFree(ctx->hpin);
+  delete ctx;
+  return 0;
+}
+
+int vdvc_timing_begin(vdvc_ctx* ctx) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  ctx->timing = true;
+  ctx->ev_used = 0;
+  return 0;
+}
+
+int vdvc_timing_end(vdvc_ctx* ctx, double* h_total_ms, int* h_launches) {
+  if (!ctx || !h_total_ms || !h_launches) return fail(-1, "NULL argument");
+  VDVC_HIP(hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  for (size_t q = 0; q + 1 < ctx->ev_used; q += 2) {
+    float ms = 0.0f;
+    VDVC_HIP(hipEventElapsedTime(&ms, ctx->ev[q], ctx->ev[q + 1]));
+    tot += ms;
+  }
+  *h_total_ms = tot;
+  *h_launches = (int)(ctx->ev_used / 2);
+  ctx->timing = false;
+  ctx->ev_used = 0;
+  return 0;
+}
+
+int vdvc_synchronize(vdvc_ctx* ctx) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  VDVC_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
+                     float glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
+                     uint8_t* d_mask0) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!ftype_ok(ftype) || !gtype_ok(gtype)) return fail(-1, "unsupported element type");
+  if (!d_f || !d_gx || !d_gy || !d_gz || !d_mask0) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  VDVC_LAUNCH(vdvc::launch_setup(ftype, gtype, M, d_f, glt, d_gx, d_gy, d_gz, d_mask0, ctx->stream));
+  return 0;
+}
+
+int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_field,
+                      int ftype, const uint8_t* d_mask, int use_thresh, float glt,
+                      double* h_out3) {
+  if (!ctx || !h_out3 || !d_field) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!ftype_ok(ftype)) return fail(-1, "unsupported field type");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  const int nb = 2048;
+  if ((rc = ctx->stats.ensure((size_t)nb * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_stats(ftype, M, d_field, d_mask, use_thresh, glt,
+                                 ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 3, ctx->red.as<double>(), ctx->stream));
+  return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_out3);
+}
+
+int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, const void* d_g,
+                int gitype, int gnx, int gny, int gnz, float gox, float goy, float goz,
+                float glt, const uint8_t* d_mask0, const uint8_t* d_bmask, float* d_gvals,
+                uint8_t* d_mask_out, double* h_stats3) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!ftype_ok(gitype)) return fail(-1, "unsupported image type");
+  if (gnx < 1 || gny < 1 || gnz < 1) return fail(-1, "invalid g dims");
+  if (!d_U || !d_g || !d_gvals || !d_mask_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = ctx->stats.ensure((size_t)T.nwaves * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  SampleIn S;
+  S.gimg = d_g; S.gnx = gnx; S.gny = gny; S.gnz = gnz;
+  S.gox = gox; S.goy = goy; S.goz = goz; S.glt = glt;
+  S.mask0 = d_mask0; S.bmask = d_bmask; S.gvals = d_gvals; S.mask_out = d_mask_out;
+  S.stats = ctx->stats.as<double>();
+  VDVC_LAUNCH(vdvc::launch_sample(gitype, M, T, S, d_U, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 3, ctx->red.as<double>(),
+                                  ctx->stream));
+  if (h_stats3) return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_stats3);
+  return 0;
+}
+
+int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                      const void* d_gy, const void* d_gz, int gtype, const void* d_f, int ftype,
+                      float mean0, const float* d_gvals, float gmean, float kscale,
+                      const uint8_t* d_bmask, const uint8_t* d_live, float* d_res, float* d_b,
+                      uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
+                      double* h_stats4) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype) || !ftype_ok(ftype)) return fail(-1, "unsupported element type");
+  if (!d_gx || !d_gy || !d_gz || !d_f || !d_gvals || !d_bmask || !d_live || !d_res || !d_b)
+    return fail(-1, "NULL buffer");
+  const bool fold = d_fold_x != nullptr;
+  if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VoxIn in = vox_in(d_gx, d_gy, d_gz, d_bmask);
+  in.fimg = d_f; in.gvals = d_gvals; in.live = d_live;
+  in.mean0 = mean0; in.gmean = gmean; in.kscale = kscale;
+  in.res = d_res;
+  in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
+  in.stats = ctx->stats.as<double>();
+  if ((rc = run_restrict(ctx, MODE_RHS, gtype, M, T, in, ftype, fold, nullptr))) return rc;
+  NodeOp op = node_op0();
+  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_b;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 4, ctx->red.as<double>(),
+                                  ctx->stream));
+  if (h_stats4) return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
+  return 0;
+}
+
+int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                  const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
+                  const float* d_v, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
+                         VDVC_F32, false, d_v)))
+    return rc;
+  NodeOp op = node_op0();
+  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  return 0;
+}
+
+int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                const void* d_gz, int gtype, const uint8_t* d_mask, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
+                         false, nullptr)))
+    return rc;
+  DiagOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_out;
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  return 0;
+}
+
+int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
+                  const float* d_E, const float* d_v, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, 1);
+  if (rc) return rc;
+  if (!d_v || !d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, 1);
+  int bd = 0;
+  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_K, &bd))) return rc;
+  NodeOp op = node_op0();
+  op.k1 = kop(ctx->K1, bd, E, d_E); op.v1 = d_v; op.c1 = 1.0f; op.k1_on = 1;
+  op.out = d_out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  return 0;
+}
+
+int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
+                const float* d_E, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, 1);
+  if (rc) return rc;
+  if (!d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, 1);
+  int bd = 0;
+  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_K, &bd))) return rc;
+  DiagOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.k1 = kop(ctx->K1, bd, E, d_E); op.c1 = 1.0f; op.k1_on = 1;
+  op.out = d_out;
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  return 0;
+}
+
+int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
+                      const float* d_E, const float* d_v, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, 1);
+  if (rc) return rc;
+  if (!d_v || !d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, 1);
+  int bd = 0;
+  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_Kel, &bd))) return rc;
+  if ((rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(float)))) return rc;
+  float* w = ctx->nodeC.as<float>();
+  NodeOp o1 = node_op0();
+  o1.k1 = kop(ctx->K1, bd, E, d_E); o1.v1 = d_v; o1.c1 = 1.0f; o1.k1_on = 1;
+  o1.interior = 1; o1.out = w;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o1, ctx->stream));
+  NodeOp o2 = node_op0();
+  o2.k1 = kop(ctx->K1, bd, E, d_E); o2.v1 = w; o2.c1 = 1.0f; o2.k1_on = 1;
+  o2.out = d_out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
+  return 0;
+}
+
+int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
+                          const int8_t* d_probes, int n_probes, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, 1);
+  if (rc) return rc;
+  if (!d_probes || !d_out || n_probes < 1) return fail(-1, "invalid probes / output");
+  MeshDev M = make_mesh(Nx, Ny, Nz, 1);
+  const long long ndof = 3 * M.Nnodes;
+  if ((rc = ctx->hacc.ensure((size_t)ndof * sizeof(double)))) return rc;
+  if ((rc = ctx->nodeA.ensure((size_t)ndof * sizeof(float)))) return rc;
+  if ((rc = ctx->nodeB.ensure((size_t)ndof * sizeof(float)))) return rc;
+  if ((rc = ctx->dparts.ensure(1024 * sizeof(double)))) return rc;
+  VDVC_HIP(hipMemsetAsync(ctx->hacc.p, 0, (size_t)ndof * sizeof(double), ctx->stream));
+  for (int p = 0; p < n_probes; ++p) {
+    const int8_t* z = d_probes + (size_t)p * ndof;
+    VDVC_LAUNCH(vdvc::launch_expand_i8(ndof, z, ctx->nodeA.as<float>(), ctx->stream));
+    if ((rc = vdvc_matvec_eqgap(ctx, Nx, Ny, Nz, h_Kel, E, nullptr, ctx->nodeA.as<float>(),
+                                ctx->nodeB.as<float>())))
+      return rc;
+    VDVC_LAUNCH(vdvc::launch_hutch_acc(ndof, z, ctx->nodeB.as<float>(), ctx->hacc.as<double>(),
+                                       ctx->stream));
+  }
+  VDVC_LAUNCH(vdvc::launch_hutch_final(ndof, ctx->hacc.as<double>(), n_probes, d_out,
+                                       ctx->dparts.as<double>(), ctx->stream));
+  return 0;
+}
+
+int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+                const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+                const float* d_extra, float* d_diag, float* d_minv) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_diag || !d_minv) return fail(-1, "NULL buffer");
+  int bd1 = 0, bd2 = 0;
+  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
+                         false, nullptr)))
+    return rc;
+  const int nb = vdvc::node_blocks(M);
+  if ((rc = ctx->fmin.ensure((size_t)nb * sizeof(float)))) return rc;
+  DiagOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<float>();
+  if (reg_kind == VDVC_REG_K) {
+    op.k1 = kop(ctx->K1, bd1, E, nullptr); op.c1 = ll; op.k1_on = 1;
+  } else if (reg_kind == VDVC_REG_EQGAP) {
+    // diag R = hutchinson(K1 P_i K1) + eps * diag(K2): K2 part here, the
+    // Hutchinson part via d_extra
+    if (!d_extra) return fail(-1, "REG_EQGAP needs the Hutchinson diagonal in d_extra");
+    op.k1 = kop(ctx->K2, bd2, E, nullptr); op.c1 = ll * eps_reg; op.k1_on = 1;
+    op.extra = d_extra; op.cextra = ll;
+  }
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_jacobi(d_diag, ctx->fmin.as<float>(), nb, 3 * M.Nnodes, d_minv,
+                                  ctx->stream));
+  return 0;
+}
+
+int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+                 const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+                 const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+                 const float* d_v, float* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
+  int bd1 = 0, bd2 = 0;
+  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
+                         VDVC_F32, false, d_v)))
+    return rc;
+  return apply_reg_sum(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, d_v, d_out, nullptr,
+                       nullptr, nullptr);
+}
+
+int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
+             const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
+             const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
+             const float* d_b, const float* d_minv, float* d_x, float tol, int max_iter,
+             int check_every, int* h_iters, double* h_relres) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_b || !d_minv || !d_x) return fail(-1, "NULL buffer");
+  if (max_iter < 0) return fail(-1, "max_iter < 0");
+  if (check_every < 1) check_every = 1;
+  int bd1 = 0, bd2 = 0;
+  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M);
+  const long long ndof = 3 * M.Nnodes;
+  const size_t vb = (size_t)ndof * sizeof(float);
+  if ((rc = ctx->pr.ensure(vb)) || (rc = ctx->pz.ensure(vb)) || (rc = ctx->pp.ensure(vb)) ||
+      (rc = ctx->pAp.ensure(vb)))
+    return rc;
+  const int nvb = vdvc::vec_blocks(ndof);
+  const int nnb = vdvc::node_blocks(M);
+  if ((rc = ctx->dparts.ensure((size_t)2 * nvb * sizeof(double)))) return rc;
+  if ((rc = ctx->dparts2.ensure((size_t)nnb * sizeof(double)))) return rc;
+  if ((rc = ctx->pstate.ensure(sizeof(PcgState)))) return rc;
+  if ((rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * sizeof(float)))) return rc;
+  float* r = ctx->pr.as<float>();
+  float* z = ctx->pz.as<float>();
+  float* p = ctx->pp.as<float>();
+  float* Ap = ctx->pAp.as<float>();
+  PcgState* S = ctx->pstate.as<PcgState>();
+  const int* done = &S->done;
+  VDVC_LAUNCH(vdvc::launch_pcg_init(ndof, d_b, d_minv, d_x, r, z, p, ctx->dparts.as<double>(), S,
+                                    ctx->stream));
+  VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  in.done = done;
+  int it = 0;
+  while (it < max_iter) {
+    const int nrun = (max_iter - it) < check_every ? (max_iter - it) : check_every;
+    for (int q = 0; q < nrun; ++q, ++it) {
+      if ((rc = launch_restrict_timed(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, p)))
+        return rc;
+      if ((rc = apply_reg_sum(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, p, Ap, p,
+                              ctx->dparts2.as<double>(), done)))
+        return rc;
+      VDVC_LAUNCH(vdvc::launch_pcg_update(ndof, it, ctx->dparts2.as<double>(), nnb, S, d_minv, p,
+                                          Ap, d_x, r, z, ctx->dparts.as<double>(), ctx->stream));
+      VDVC_LAUNCH(vdvc::launch_pcg_pupdate(ndof, it, tol, ctx->dparts.as<double>(), S, z, p,
+                                           ctx->stream));
+    }
+    PcgState hs;
+    if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
+    if (hs.done) break;
+  }
+  PcgState hs;
+  if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
+  if (h_iters) *h_iters = hs.iters;
+  if (h_relres) *h_relres = hs.relres;
+  return 0;
+}
+
+int vdvc_dot(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, double* h_out) {
+  if (!ctx || !d_a || !d_b || !h_out) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  int rc;
+  const int nb = 1024;
+  if ((rc = ctx->stats.ensure((size_t)nb * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_dot(n, d_a, d_b, ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 1, ctx->red.as<double>(), ctx->stream));
+  return fetch(ctx, ctx->red.p, sizeof(double), h_out);
+}
+
+int vdvc_axpby(vdvc_ctx* ctx, int64_t n, float a, const float* d_x, float b, float* d_y) {
+  if (!ctx || !d_x || !d_y) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (n == 0) return 0;
+  VDVC_LAUNCH(vdvc::launch_axpby(n, a, d_x, b, d_y, ctx->stream));
+  return 0;
+}
+
+int vdvc_fold_grad(vdvc_ctx* ctx, int64_t nvox, const uint16_t* d_gx, const uint16_t* d_gy,
+                   const uint16_t* d_gz, const uint8_t* d_mask, uint16_t* d_ox, uint16_t* d_oy,
+                   uint16_t* d_oz) {
+  if (!ctx || !d_gx || !d_gy || !d_gz || !d_mask || !d_ox || !d_oy || !d_oz)
+    return fail(-1, "NULL argument");
+  if (nvox < 0) return fail(-1, "nvox < 0");
+  if (nvox == 0) return 0;
+  VDVC_LAUNCH(vdvc::launch_fold(nvox, d_gx, d_gy, d_gz, d_mask, d_ox, d_oy, d_oz, ctx->stream));
+  return 0;
+}
+
+int vdvc_erode6(vdvc_ctx* ctx, int nx, int ny, int nz, const uint8_t* d_in, uint8_t* d_out) {
+  if (!ctx || !d_in || !d_out) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(-1, "invalid dims");
+  VDVC_LAUNCH(vdvc::launch_erode6(nx, ny, nz, d_in, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_mul(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, float* d_out) {
+  if (!ctx || !d_a || !d_b || !d_out) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (n == 0) return 0;
+  VDVC_LAUNCH(vdvc::launch_mul(n, d_a, d_b, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_stride(vdvc_ctx* ctx, const void* d_src, int type, int nx, int ny, int nz, int s,
+                void* d_dst) {
+  if (!ctx || !d_src || !d_dst) return fail(-1, "NULL argument");
+  if (!ftype_ok(type)) return fail(-1, "unsupported element type");
+  if (nx < 1 || ny < 1 || nz < 1 || s < 1) return fail(-1, "invalid dims / stride");
+  const int mx = (nx - 1) / s + 1, my = (ny - 1) / s + 1, mz = (nz - 1) / s + 1;
+  VDVC_LAUNCH(vdvc::launch_stride(type, d_src, nx, ny, nz, s, d_dst, mx, my, mz, ctx->stream));
+  return 0;
+}
+
+int vdvc_phase_corr(vdvc_ctx* ctx, const float* d_f, const float* d_g, int nx, int ny, int nz,
+                    int64_t* h_cidx) {
+  if (!ctx || !d_f || !d_g || !h_cidx) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(-1, "invalid dims");
+  const long long n = (long long)nx * ny * nz;
+  if (n > (1LL << 26)) return fail(-1, "phase correlation volume too large (%lld voxels)", n);
+  int rc;
+  if ((rc = ctx->dft.ensure((size_t)3 * n * sizeof(double2)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_phase_corr(d_f, d_g, nx, ny, nz, ctx->dft.as<double2>(),
+                                      ctx->red.as<long long>(), ctx->stream));
+  long long idx = 0;
+  if ((rc = fetch(ctx, ctx->red.p, sizeof(long long), &idx))) return rc;
+  *h_cidx = idx;
+  return 0;
+}
+
+int vdvc_affine_sums(vdvc_ctx* ctx, const float* d_f, int nx, int ny, int nz, const float* d_g,
+                     int gnx, int gny, int gnz, const double* h_A9, const double* h_t3,
+                     const double* h_c3, double* h_out6) {
+  if (!ctx || !d_f || !d_g || !h_A9 || !h_t3 || !h_c3 || !h_out6) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1 || gnx < 1 || gny < 1 || gnz < 1) return fail(-1, "invalid dims");
+  AffineIn P;
+  for (int q = 0; q < 9; ++q) P.A[q] = h_A9[q];
+  for (int q = 0; q < 3; ++q) { P.t[q] = h_t3[q]; P.c[q] = h_c3[q]; }
+  const int nb = 1024;
+  int rc;
+  if ((rc = ctx->stats.ensure((size_t)nb * 6 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_affine_cost(d_f, nx, ny, nz, d_g, gnx, gny, gnz, P,
+                                       ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 6, ctx->red.as<double>(), ctx->stream));
+  return fetch(ctx, ctx->red.p, 6 * sizeof(double), h_out6);
+}
+
+}  // extern "C"

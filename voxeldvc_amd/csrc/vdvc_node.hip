@@ -1,0 +1,491 @@
+// Node-level kernels: gather of the voxel kernels' node partials, the
+// element-stiffness regularizer applied as a node stencil, Jacobi
+// preconditioner, the device-resident PCG vector updates with FP64 dots, and
+// small vector utilities.
+//
+// Reference semantics followed:
+//   regularizer   engine/kernels_dvc.py:349-437 (matvec_K_ref, diagonals,
+//                 equilibrium gap)
+//   Jacobi        engine/correlate_gpu.py:1042-1047, 1113-1120
+//   PCG           engine/kernels_dvc.py:550-583
+#include "vdvc_kernels.h"
+
+// corner index of local node offset (dx,dy,dz) in the reference Hex8 order
+__device__ __forceinline__ int corner_of(int dx, int dy, int dz) {
+  return dz * 4 + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
+}
+__constant__ int c_cdx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int c_cdy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__constant__ int c_cdz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+
+__device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp& op,
+                                          const float* __restrict__ v, int i, int j, int k,
+                                          float (&out)[3]) {
+  out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + (e >> 2);
+    if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
+    const int la = corner_of(i - ei, j - ej, k - ek);
+    const float Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
+                             : op.E;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int lb = 0; lb < 8; ++lb) {
+      const long long nb = node_index(M, ei + c_cdx[lb], ej + c_cdy[lb], ek + c_cdz[lb]);
+      const float vb0 = v[3 * nb], vb1 = v[3 * nb + 1], vb2 = v[3 * nb + 2];
+      if (op.blockdiag) {
+        const float l = op.K[(3 * la) * 24 + 3 * lb];
+        acc[0] += l * vb0; acc[1] += l * vb1; acc[2] += l * vb2;
+      } else {
+        const float* Kr = op.K + (3 * la) * 24 + 3 * lb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          acc[c] += Kr[c * 24 + 0] * vb0 + Kr[c * 24 + 1] * vb1 + Kr[c * 24 + 2] * vb2;
+      }
+    }
+    out[0] += Ee * acc[0]; out[1] += Ee * acc[1]; out[2] += Ee * acc[2];
+  }
+}
+
+__device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp& op, int i, int j, int k,
+                                         float (&out)[3]) {
+  out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + (e >> 2);
+    if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
+    const int la = corner_of(i - ei, j - ej, k - ek);
+    const float Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
+                             : op.E;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] += Ee * op.K[(3 * la + c) * 24 + 3 * la + c];
+  }
+}
+
+// sum of the node partials written by the voxel kernels for node (i,j,k)
+__device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev& T,
+                                                const float* __restrict__ P, int i, int j, int k,
+                                                float (&s)[3]) {
+  s[0] = s[1] = s[2] = 0.0f;
+  const int h = M.h;
+  const int za = max(0, k / T.ez - 1), zb = min(T.nzt - 1, k / T.ez);
+  const int xa = max(0, (i - 1) * h) / 64;
+  const int xb = min(M.Nvx - 1, (i + 1) * h) / 64;
+  for (int zt = za; zt <= zb; ++zt) {
+    const int kb = zt * T.ez;
+    if (k < kb || k > min(M.Nz, kb + T.ez)) continue;
+    const int kzl = k - kb;
+    for (int cy = 1; cy >= 0; --cy) {
+      const int ey = j - cy;
+      if (ey < 0 || ey >= M.Ny) continue;
+      for (int sx = xa; sx <= xb; ++sx) {
+        const int x0 = sx * 64;
+        const int nact = min(64, M.Nvx - x0);
+        const int i_lo = x0 / h;
+        const int nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
+        const int ln = i - i_lo;
+        if (ln < 0 || ln >= nn) continue;
+        const long long w = (long long)sx + (long long)T.nsx * (ey + (long long)M.Ny * zt);
+        const float* p = P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln) * 3;
+        s[0] += p[0]; s[1] += p[1]; s[2] += p[2];
+      }
+    }
+  }
+}
+
+
+__global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp op) {
+  __shared__ double lds[4];
+  if (op.done && *op.done) return;
+  const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  double d = 0.0;
+  if (n < M.Nnodes) {
+    const int i = (int)(n % M.nnx);
+    const long long r = n / M.nnx;
+    const int j = (int)(r % M.nny);
+    const int k = (int)(r / M.nny);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    if (op.P) {
+      float s[3];
+      gather_partials(M, op.T, op.P, i, j, k, s);
+      acc[0] = op.cH * s[0]; acc[1] = op.cH * s[1]; acc[2] = op.cH * s[2];
+    }
+    if (op.addin) {
+      acc[0] += op.cadd * op.addin[3 * n];
+      acc[1] += op.cadd * op.addin[3 * n + 1];
+      acc[2] += op.cadd * op.addin[3 * n + 2];
+    }
+    if (op.k1_on) {
+      float t[3];
+      kop_apply(M, op.k1, op.v1, i, j, k, t);
+      acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
+    }
+    if (op.k2_on) {
+      float t[3];
+      kop_apply(M, op.k2, op.v2, i, j, k, t);
+      acc[0] += op.c2 * t[0]; acc[1] += op.c2 * t[1]; acc[2] += op.c2 * t[2];
+    }
+    if (op.interior) {
+      // interior nodes: strictly inside the mesh bounding box
+      // (engine/geometry_dvc.py:201-217)
+      const bool in = i > 0 && i < M.Nx && j > 0 && j < M.Ny && k > 0 && k < M.Nz;
+      if (!in) { acc[0] = 0.0f; acc[1] = 0.0f; acc[2] = 0.0f; }
+    }
+    op.out[3 * n] = acc[0];
+    op.out[3 * n + 1] = acc[1];
+    op.out[3 * n + 2] = acc[2];
+    if (op.dot_part) {
+      d = (double)op.dotv[3 * n] * acc[0] + (double)op.dotv[3 * n + 1] * acc[1] +
+          (double)op.dotv[3 * n + 2] * acc[2];
+    }
+  }
+  if (op.dot_part) {
+    double v[1] = {d};
+    block_sum_f64<1>(v, lds);
+    if (threadIdx.x == 0) op.dot_part[blockIdx.x] = v[0];
+  }
+}
+
+
+__global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp op) {
+  __shared__ float lds[VDVC_BLOCK];
+  const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  float mn = INFINITY;
+  if (n < M.Nnodes) {
+    const int i = (int)(n % M.nnx);
+    const long long r = n / M.nnx;
+    const int j = (int)(r % M.nny);
+    const int k = (int)(r / M.nny);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    if (op.P) gather_partials(M, op.T, op.P, i, j, k, acc);
+    if (op.k1_on) {
+      float t[3];
+      kop_diag(M, op.k1, i, j, k, t);
+      acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
+    }
+    if (op.extra) {
+      acc[0] += op.cextra * op.extra[3 * n];
+      acc[1] += op.cextra * op.extra[3 * n + 1];
+      acc[2] += op.cextra * op.extra[3 * n + 2];
+    }
+    op.out[3 * n] = acc[0];
+    op.out[3 * n + 1] = acc[1];
+    op.out[3 * n + 2] = acc[2];
+    mn = fminf(acc[0], fminf(acc[1], acc[2]));
+  }
+  if (op.min_part) {
+    lds[threadIdx.x] = mn;
+    __syncthreads();
+    for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) lds[threadIdx.x] = fminf(lds[threadIdx.x], lds[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) op.min_part[blockIdx.x] = lds[0];
+  }
+}
+
+// M_inv = 1 / (diag + 1e-5 * min(diag)); every block reduces the partial
+// minima itself (fixed order), so no host round trip is needed
+__global__ __launch_bounds__(VDVC_BLOCK) void jacobi_kernel(const float* __restrict__ diag,
+                                                            const float* __restrict__ min_part,
+                                                            int nparts, long long ndof,
+                                                            float* __restrict__ minv) {
+  __shared__ float lds[VDVC_BLOCK];
+  float mn = INFINITY;
+  for (int q = threadIdx.x; q < nparts; q += VDVC_BLOCK) mn = fminf(mn, min_part[q]);
+  lds[threadIdx.x] = mn;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] = fminf(lds[threadIdx.x], lds[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float eps0 = (float)(1e-5 * (double)lds[0]);
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
+       q += (long long)gridDim.x * VDVC_BLOCK)
+    minv[q] = 1.0f / (diag[q] + eps0);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (x0 = 0) with FP64 dot products; state lives in device memory
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int n, int stride,
+                                            int off, double* lds) {
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += VDVC_BLOCK) s += part[(size_t)q * stride + off];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_init_kernel(long long ndof,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ minv,
+                                                              float* __restrict__ x,
+                                                              float* __restrict__ r,
+                                                              float* __restrict__ z,
+                                                              float* __restrict__ p,
+                                                              double* __restrict__ part) {
+  __shared__ double lds[8];
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (q < ndof) {
+    const float bq = b[q];
+    const float zq = minv[q] * bq;
+    x[q] = 0.0f; r[q] = bq; z[q] = zq; p[q] = zq;
+    v[0] = (double)bq * zq;
+    v[1] = (double)bq * bq;
+  }
+  block_sum_f64<2>(v, lds);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_init_state_kernel(const double* __restrict__ part,
+                                                                    int nparts,
+                                                                    PcgState* __restrict__ S) {
+  __shared__ double lds[VDVC_BLOCK];
+  const double rz = sum_parts(part, nparts, 2, 0, lds);
+  const double bb = sum_parts(part, nparts, 2, 1, lds);
+  if (threadIdx.x == 0) {
+    S->rz[0] = rz; S->rz[1] = 0.0;
+    S->r0 = bb + 1e-30;
+    S->relres = 1.0; S->done = 0; S->iters = 0;
+  }
+}
+
+// alpha = rz / (p.Ap + 1e-30); x += alpha p; r -= alpha Ap; z = M r;
+// partial dots (r.r, r.z)
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_kernel(
+    long long ndof, int it, const double* __restrict__ pap_part, int npap,
+    PcgState* __restrict__ S, const float* __restrict__ minv, const float* __restrict__ p,
+    const float* __restrict__ Ap, float* __restrict__ x, float* __restrict__ r,
+    float* __restrict__ z, double* __restrict__ part) {
+  __shared__ double lds[VDVC_BLOCK];
+  if (S->done) return;
+  const double pAp = sum_parts(pap_part, npap, 1, 0, lds);
+  const double alpha = S->rz[it & 1] / (pAp + 1e-30);
+  const float af = (float)alpha;
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (q < ndof) {
+    const float pq = p[q];
+    x[q] = x[q] + af * pq;
+    const float rq = r[q] - af * Ap[q];
+    r[q] = rq;
+    const float zq = minv[q] * rq;
+    z[q] = zq;
+    v[0] = (double)rq * rq;
+    v[1] = (double)rq * zq;
+  }
+  block_sum_f64<2>(v, lds);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// convergence test sqrt(r.r / r0) < tol (then stop: x is final), else
+// beta = rz_new / (rz + 1e-30); p = z + beta p
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
+    long long ndof, int it, float tol, const double* __restrict__ part, int nparts,
+    PcgState* __restrict__ S, const float* __restrict__ z, float* __restrict__ p) {
+  __shared__ double lds[VDVC_BLOCK];
+  if (S->done) return;
+  const double rr = sum_parts(part, nparts, 2, 0, lds);
+  const double rzn = sum_parts(part, nparts, 2, 1, lds);
+  const double res = sqrt(rr / S->r0);
+  const bool conv = res < (double)tol;
+  if (conv) {
+    // every block sees the same sums; block 0 records the stop
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S->relres = res;
+      S->iters = it + 1;
+      __threadfence();
+      S->done = 1;
+    }
+    return;
+  }
+  const double beta = rzn / (S->rz[it & 1] + 1e-30);
+  const float bf = (float)beta;
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q < ndof) p[q] = z[q] + bf * p[q];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->rz[(it + 1) & 1] = rzn;
+    S->relres = res;
+    S->iters = it + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small vector utilities
+// ---------------------------------------------------------------------------
+// y = a*x + b*y (float), optional partial FP64 dot(y, y)
+__global__ __launch_bounds__(VDVC_BLOCK) void axpby_kernel(long long n, float a,
+                                                           const float* __restrict__ x, float b,
+                                                           float* __restrict__ y) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q < n) y[q] = a * x[q] + b * y[q];
+}
+
+// partial FP64 dot products of two vectors
+__global__ __launch_bounds__(VDVC_BLOCK) void dot_kernel(long long n, const float* __restrict__ a,
+                                                         const float* __restrict__ b,
+                                                         double* __restrict__ part) {
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK)
+    v[0] += (double)a[q] * b[q];
+  block_sum_f64<1>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Hutchinson accumulation acc += z * Rz (FP64), z a Rademacher probe
+__global__ __launch_bounds__(VDVC_BLOCK) void hutch_acc_kernel(long long n,
+                                                               const int8_t* __restrict__ z,
+                                                               const float* __restrict__ Rz,
+                                                               double* __restrict__ acc) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q < n) acc[q] += (double)z[q] * (double)Rz[q];
+}
+
+// float copy of an int8 probe vector
+__global__ __launch_bounds__(VDVC_BLOCK) void expand_i8_kernel(long long n,
+                                                               const int8_t* __restrict__ z,
+                                                               float* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q < n) out[q] = (float)z[q];
+}
+
+// max |acc| / nprobes per block
+__global__ __launch_bounds__(VDVC_BLOCK) void hutch_max_kernel(long long n,
+                                                               const double* __restrict__ acc,
+                                                               double* __restrict__ part) {
+  __shared__ double lds[VDVC_BLOCK];
+  double m = 0.0;
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK)
+    m = fmax(m, fabs(acc[q]));
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
+}
+
+// d = acc / nprobes clipped below at 1e-12 * max|d| (every block reduces
+// the block maxima itself)
+__global__ __launch_bounds__(VDVC_BLOCK) void hutch_final_kernel(long long n,
+                                                                 const double* __restrict__ acc,
+                                                                 int nprobes,
+                                                                 const double* __restrict__ part,
+                                                                 int nparts,
+                                                                 float* __restrict__ out) {
+  __shared__ double lds[VDVC_BLOCK];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += VDVC_BLOCK) m = fmax(m, part[q]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double lo = 1e-12 * (lds[0] / nprobes);
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
+    const double d = acc[q] / nprobes;
+    out[q] = (float)(d < lo ? lo : d);
+  }
+}
+
+namespace vdvc {
+
+int vec_blocks(long long n) { return (int)((n + VDVC_BLOCK - 1) / VDVC_BLOCK); }
+
+int launch_hutch_acc(long long ndof, const int8_t* z, const float* Rz, double* acc,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(hutch_acc_kernel, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof, z,
+                     Rz, acc);
+  return (int)hipGetLastError();
+}
+
+int launch_expand_i8(long long n, const int8_t* z, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(expand_i8_kernel, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, z, out);
+  return (int)hipGetLastError();
+}
+
+int launch_hutch_final(long long ndof, const double* acc, int nprobes, float* out,
+                       double* scratch, hipStream_t st) {
+  int nb = vec_blocks(ndof);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(hutch_max_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, acc, scratch);
+  hipLaunchKernelGGL(hutch_final_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, acc, nprobes,
+                     scratch, nb, out);
+  return (int)hipGetLastError();
+}
+
+int launch_node_op(const MeshDev& M, const NodeOp& op, hipStream_t st) {
+  const long long nb = (M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  hipLaunchKernelGGL(node_op_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, M, op);
+  return (int)hipGetLastError();
+}
+
+int node_blocks(const MeshDev& M) { return (int)((M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK); }
+
+int launch_diag_op(const MeshDev& M, const DiagOp& op, hipStream_t st) {
+  hipLaunchKernelGGL(diag_op_kernel, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
+  return (int)hipGetLastError();
+}
+
+int launch_jacobi(const float* diag, const float* min_part, int nparts, long long ndof,
+                  float* minv, hipStream_t st) {
+  long long nb = (ndof + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, diag, min_part,
+                     nparts, ndof, minv);
+  return (int)hipGetLastError();
+}
+
+int launch_pcg_init(long long ndof, const float* b, const float* minv, float* x, float* r,
+                    float* z, float* p, double* part, PcgState* S, hipStream_t st) {
+  const int nb = vec_blocks(ndof);
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, b, minv, x, r, z,
+                     p, part);
+  hipLaunchKernelGGL(pcg_init_state_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, part, nb, S);
+  return (int)hipGetLastError();
+}
+
+int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
+                      const float* minv, const float* p, const float* Ap, float* x, float* r,
+                      float* z, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
+                     it, pap_part, npap, S, minv, p, Ap, x, r, z, part);
+  return (int)hipGetLastError();
+}
+
+int launch_pcg_pupdate(long long ndof, int it, float tol, const double* part, PcgState* S,
+                       const float* z, float* p, hipStream_t st) {
+  const int nb = vec_blocks(ndof);
+  hipLaunchKernelGGL(pcg_pupdate_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol, part,
+                     nb, S, z, p);
+  return (int)hipGetLastError();
+}
+
+int launch_axpby(long long n, float a, const float* x, float b, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, a, x, b, y);
+  return (int)hipGetLastError();
+}
+
+int launch_dot(long long n, const float* a, const float* b, double* part, int nparts,
+               hipStream_t st) {
+  hipLaunchKernelGGL(dot_kernel, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, a, b, part);
+  return (int)hipGetLastError();
+}
+
+}  // namespace vdvc

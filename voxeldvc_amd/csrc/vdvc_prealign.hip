@@ -1,0 +1,209 @@
+// Pyramid restriction and pre-alignment kernels.
+//
+//   stride restriction   engine/correlate_gpu.py:1256-1258 (f[::s, ::s, ::s])
+//   FFT phase shift      engine/correlate_gpu.py:381-414   (fft_rigid_shift)
+//   affine warp + ZNCC   engine/correlate_gpu.py:417-433, 505-536
+//
+// The phase correlation runs on the coarsest pyramid level only (tens to a
+// few hundred voxels per axis), so it uses separable direct DFTs in FP64
+// (exact twiddles from sincospi) instead of an FFT library: per axis each
+// output sample is one dot product over the line.
+#include "vdvc_kernels.h"
+
+template <typename T>
+__global__ __launch_bounds__(VDVC_BLOCK) void stride_kernel(const T* __restrict__ src, int nx,
+                                                            int ny, int nz, int s,
+                                                            T* __restrict__ dst, int mx, int my,
+                                                            int mz) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  const long long n = (long long)mx * my * mz;
+  if (q >= n) return;
+  const int x = (int)(q % mx);
+  const long long r = q / mx;
+  const int y = (int)(r % my);
+  const int z = (int)(r / my);
+  dst[q] = src[(long long)x * s + (long long)nx * ((long long)y * s + (long long)ny * ((long long)z * s))];
+}
+
+// one DFT pass along `axis` of a complex (interleaved double2) volume with
+// dims (nx,ny,nz) x fastest; real input when `src_real` is set
+__global__ __launch_bounds__(VDVC_BLOCK) void dft_pass_kernel(const double2* __restrict__ src,
+                                                              const float* __restrict__ src_real,
+                                                              double2* __restrict__ dst, int nx,
+                                                              int ny, int nz, int axis,
+                                                              double sign) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  const long long ntot = (long long)nx * ny * nz;
+  if (q >= ntot) return;
+  const int x = (int)(q % nx);
+  const long long r = q / nx;
+  const int y = (int)(r % ny);
+  const int z = (int)(r / ny);
+  int n, kk;
+  long long stride, base;
+  if (axis == 0) { n = nx; kk = x; stride = 1; base = q - x; }
+  else if (axis == 1) { n = ny; kk = y; stride = nx; base = q - (long long)y * nx; }
+  else { n = nz; kk = z; stride = (long long)nx * ny; base = q - (long long)z * nx * ny; }
+  double re = 0.0, im = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const long long m = ((long long)kk * t) % n;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+    sn *= sign;
+    if (src_real) {
+      const double a = (double)src_real[base + t * stride];
+      re += a * cs;
+      im += a * sn;
+    } else {
+      const double2 a = src[base + t * stride];
+      re += a.x * cs - a.y * sn;
+      im += a.x * sn + a.y * cs;
+    }
+  }
+  dst[q] = make_double2(re, im);
+}
+
+// R = F conj(G) / (|F conj(G)| + 1e-12), in place into F
+__global__ __launch_bounds__(VDVC_BLOCK) void cross_power_kernel(double2* __restrict__ F,
+                                                                 const double2* __restrict__ G,
+                                                                 long long n) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const double2 a = F[q], b = G[q];
+  const double re = a.x * b.x + a.y * b.y;
+  const double im = a.y * b.x - a.x * b.y;
+  const double mag = sqrt(re * re + im * im) + 1e-12;
+  F[q] = make_double2(re / mag, im / mag);
+}
+
+// argmax of the real part; ties resolve to the smallest C-order index
+// ((x*ny + y)*nz + z), as NumPy's argmax over a C-ordered (x,y,z) array
+__global__ __launch_bounds__(VDVC_BLOCK) void argmax_kernel(const double2* __restrict__ R, int nx,
+                                                            int ny, int nz,
+                                                            long long* __restrict__ out) {
+  __shared__ double lv[VDVC_BLOCK];
+  __shared__ long long li[VDVC_BLOCK];
+  const long long n = (long long)nx * ny * nz;
+  double best = -INFINITY;
+  long long bi = 0x7fffffffffffffffLL;
+  for (long long q = threadIdx.x; q < n; q += VDVC_BLOCK) {
+    const double v = R[q].x;
+    const int x = (int)(q % nx);
+    const long long r = q / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const long long ci = ((long long)x * ny + y) * nz + z;
+    if (v > best || (v == best && ci < bi)) { best = v; bi = ci; }
+  }
+  lv[threadIdx.x] = best;
+  li[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double v2 = lv[threadIdx.x + o];
+      const long long i2 = li[threadIdx.x + o];
+      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 < li[threadIdx.x])) {
+        lv[threadIdx.x] = v2;
+        li[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = li[0];
+}
+
+// affine warp x' = A(x - c) + c + t of every voxel of f_d, order-1 sample of
+// g_d (double weights, float result), FP64 partial sums over in-bounds
+// samples: n, sum f, sum g, sum f^2, sum g^2, sum f g
+
+__global__ __launch_bounds__(VDVC_BLOCK) void affine_cost_kernel(
+    const float* __restrict__ f, int nx, int ny, int nz, const float* __restrict__ g, int gnx,
+    int gny, int gnz, AffineIn P, double* __restrict__ part) {
+  __shared__ double lds[4 * 6];
+  const long long n = (long long)nx * ny * nz;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
+    const int x = (int)(q % nx);
+    const long long r = q / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const double dx = x - P.c[0], dy = y - P.c[1], dz = z - P.c[2];
+    const double X = P.A[0] * dx + P.A[1] * dy + P.A[2] * dz + P.c[0] + P.t[0];
+    const double Y = P.A[3] * dx + P.A[4] * dy + P.A[5] * dz + P.c[1] + P.t[1];
+    const double Z = P.A[6] * dx + P.A[7] * dy + P.A[8] * dz + P.c[2] + P.t[2];
+    if (!(X >= 0.0 && X <= gnx - 1 && Y >= 0.0 && Y <= gny - 1 && Z >= 0.0 && Z <= gnz - 1))
+      continue;
+    const int ix = (int)floor(X), iy = (int)floor(Y), iz = (int)floor(Z);
+    const int jx = min(ix + 1, gnx - 1), jy = min(iy + 1, gny - 1), jz = min(iz + 1, gnz - 1);
+    const double fx = X - ix, fy = Y - iy, fz = Z - iz;
+    const long long sy = gnx, sz = (long long)gnx * gny;
+    auto G = [&](int a, int b, int cc) -> double { return (double)g[a + b * sy + cc * sz]; };
+    const double gv = (1 - fz) * ((1 - fy) * ((1 - fx) * G(ix, iy, iz) + fx * G(jx, iy, iz)) +
+                                  fy * ((1 - fx) * G(ix, jy, iz) + fx * G(jx, jy, iz))) +
+                      fz * ((1 - fy) * ((1 - fx) * G(ix, iy, jz) + fx * G(jx, iy, jz)) +
+                            fy * ((1 - fx) * G(ix, jy, jz) + fx * G(jx, jy, jz)));
+    const double gs = (double)(float)gv;
+    const double fv = (double)f[q];
+    acc[0] += 1.0;
+    acc[1] += fv;
+    acc[2] += gs;
+    acc[3] += fv * fv;
+    acc[4] += gs * gs;
+    acc[5] += fv * gs;
+  }
+  block_sum_f64<6>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
+}
+
+namespace vdvc {
+
+int launch_stride(int type, const void* src, int nx, int ny, int nz, int s, void* dst, int mx,
+                  int my, int mz, hipStream_t st) {
+  const long long n = (long long)mx * my * mz;
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  if (type == GT_F32)
+    hipLaunchKernelGGL(stride_kernel<float>, dim3(nb), dim3(VDVC_BLOCK), 0, st,
+                       (const float*)src, nx, ny, nz, s, (float*)dst, mx, my, mz);
+  else
+    hipLaunchKernelGGL(stride_kernel<__half>, dim3(nb), dim3(VDVC_BLOCK), 0, st,
+                       (const __half*)src, nx, ny, nz, s, (__half*)dst, mx, my, mz);
+  return (int)hipGetLastError();
+}
+
+// phase correlation: f, g float32 volumes (x fastest, same dims); work holds
+// 3 complex volumes; out_idx receives the C-order argmax index
+int launch_phase_corr(const float* f, const float* g, int nx, int ny, int nz, double2* work,
+                      long long* out_idx, hipStream_t st) {
+  const long long n = (long long)nx * ny * nz;
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  double2* A = work;
+  double2* B = work + n;
+  double2* C = work + 2 * n;
+  dim3 g3(nb), b3(VDVC_BLOCK);
+  // F = DFT(f): x (real) -> B, y -> C, z -> A
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, nullptr, f, B, nx, ny, nz, 0, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, A, nx, ny, nz, 2, -1.0);
+  // G = DFT(g) into C (via B)
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, nullptr, g, B, nx, ny, nz, 0, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, B, nx, ny, nz, 2, -1.0);
+  hipLaunchKernelGGL(cross_power_kernel, g3, b3, 0, st, A, B, n);
+  // inverse DFT (positive sign; the 1/N scale does not move the argmax)
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, A, nullptr, B, nx, ny, nz, 0, 1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, 1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, A, nx, ny, nz, 2, 1.0);
+  hipLaunchKernelGGL(argmax_kernel, dim3(1), b3, 0, st, A, nx, ny, nz, out_idx);
+  return (int)hipGetLastError();
+}
+
+int launch_affine_cost(const float* f, int nx, int ny, int nz, const float* g, int gnx, int gny,
+                       int gnz, const AffineIn& P, double* part, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(affine_cost_kernel, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, f, nx, ny, nz,
+                     g, gnx, gny, gnz, P, part);
+  return (int)hipGetLastError();
+}
+
+}  // namespace vdvc

@@ -1,0 +1,718 @@
+// Voxel-streaming kernels: correlation-Hessian matvec, its diagonal, the
+// fused residual + right-hand side, deformed-image sampling, reference-image
+// setup (gradient + erosion) and the multiplicity-weighted masked statistics.
+//
+// Reference semantics followed:
+//   matvec      engine/kernels_dvc.py:452-506   (matvec_H)
+//   diagonal    engine/kernels_dvc.py:509-543   (build_diagonal_H)
+//   rhs         engine/correlate_gpu.py:341-378, 1091-1107
+//   sampling    engine/correlate_gpu.py:201-281 (sample_deformed_field)
+//   setup       engine/correlate_gpu.py:49-94, 951-953
+//   statistics  engine/correlate_gpu.py:113-157
+//
+// All restriction kernels share one decomposition (TilesDev): a wavefront
+// owns a 64-voxel x strip (lanes along x, so every load instruction reads a
+// contiguous 64-voxel run), one element row in y and `ez` elements in z, and
+// walks rows then planes.  Per voxel it forms the scalar s (W m g.u, W m res,
+// or W m g_k^2 for the diagonal), restricts s*g along y inside the lane
+// (running sums), along z by carrying two node planes in registers, and
+// along x by a segmented reduction across the lanes of one element at the
+// end of every element layer.  Node partials go to the wave's slot block.
+#include "vdvc_kernels.h"
+
+
+
+// Emit one node plane of a wave: c[cy*3+k] is the lane's z-complete
+// y-restricted sum for node row cy; a0/a1 are the lane's x weights for the
+// owning element's lower/upper node.  Writes nn node slots.
+template <bool POW2>
+__device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float a1,
+                                           int lane, int h, int x0, int nact, int i_lo,
+                                           int nn, float* __restrict__ slot, int px,
+                                           float* lds /* 64*13 floats */) {
+  if (POW2) {
+    float s0[6], s1[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) { s0[q] = a0 * c[q]; s1[q] = a1 * c[q]; }
+    for (int o = 1; o < h; o <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        s0[q] += __shfl_xor(s0[q], o, 64);
+        s1[q] += __shfl_xor(s1[q], o, 64);
+      }
+    }
+    float pv[6];
+    const int src = lane > 0 ? lane - 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pv[q] = __shfl(s1[q], src, 64);
+    if ((lane % h) == 0) {
+      const int ln = lane / h;
+      if (ln < nn) {
+        float* d = slot + (size_t)ln * 3;
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            d[(size_t)cy * px * 3 + k] = s0[cy * 3 + k] + (ln > 0 ? pv[cy * 3 + k] : 0.0f);
+      }
+    }
+    if (lane == 63) {
+      const int ln = 64 / h;
+      if (ln < nn) {
+        float* d = slot + (size_t)ln * 3;
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) d[(size_t)cy * px * 3 + k] = s1[cy * 3 + k];
+      }
+    }
+  } else {
+    // generic element size: stage lane values in LDS, node lanes sum them
+    float* L = lds;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      L[lane * 13 + q] = a0 * c[q];
+      L[lane * 13 + 6 + q] = a1 * c[q];
+    }
+    const int x = x0 + lane;
+    L[lane * 13 + 12] = __int_as_float(x / h - i_lo);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int ln = lane; ln < nn; ln += 64) {
+      const int i = i_lo + ln;
+      int lo = (i - 1) * h - x0;
+      int hi = (i + 1) * h - x0;
+      lo = lo < 0 ? 0 : lo;
+      hi = hi > nact - 1 ? nact - 1 : hi;
+      float v[6] = {0, 0, 0, 0, 0, 0};
+      for (int l = lo; l <= hi; ++l) {
+        const int el = __float_as_int(L[l * 13 + 12]);
+        if (el == ln) {
+#pragma unroll
+          for (int q = 0; q < 6; ++q) v[q] += L[l * 13 + q];
+        } else if (el == ln - 1) {
+#pragma unroll
+          for (int q = 0; q < 6; ++q) v[q] += L[l * 13 + 6 + q];
+        }
+      }
+      float* d = slot + (size_t)ln * 3;
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[(size_t)cy * px * 3 + k] = v[cy * 3 + k];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <int MODE, typename GT, bool HASMASK, typename FT, bool POW2, bool FOLD>
+__global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
+    MeshDev M, TilesDev T, VoxIn in, const float* __restrict__ v, float* __restrict__ P) {
+  __shared__ float lds_all[POW2 ? 1 : VDVC_WAVES_PER_BLOCK * 64 * 13];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const long long w = (long long)blk * VDVC_WAVES_PER_BLOCK + wib;
+  if (w >= T.nwaves) return;
+  if (in.done && *in.done) return;
+  float* lds = POW2 ? nullptr : lds_all + wib * 64 * 13;
+
+  const int h = M.h;
+  const float inv_h = M.inv_h;
+  const int sx = (int)(w % T.nsx);
+  const long long r = w / T.nsx;
+  const int ey = (int)(r % M.Ny);
+  const int zt = (int)(r / M.Ny);
+
+  const int x0 = sx * 64;
+  const int nact = min(64, M.Nvx - x0);
+  const int x = x0 + lane;
+  const bool act = lane < nact;
+  const int xe = act ? x : x0 + nact - 1;
+  const int ex = xe / h;               // owning element (== Nx for the last plane)
+  const int tx = xe - ex * h;
+  const float X1 = tx * inv_h;
+  const float X0 = 1.0f - X1;
+  const float Wx = act ? axis_weight(xe, M.Nvx - 1) : 0.0f;
+  const int i0 = min(ex, M.Nx);
+  const int i1 = min(ex + 1, M.Nx);
+  const int i_lo = x0 / h;
+  const int x_last = x0 + nact - 1;
+  const int nn = min(M.Nx, x_last / h + 1) - i_lo + 1;
+  const int mx = axis_mult(xe, h, M.Nvx - 1);
+
+  const int ez0 = zt * T.ez;
+  const int ez1 = min(M.Nz, ez0 + T.ez);
+  const int y0 = ey * h;
+  const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
+
+  float a0w = X0, a1w = X1;
+  if (MODE == MODE_DIAG) { a0w = X0 * X0; a1w = X1 * X1; }
+
+  float C[2][2][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) C[a][b][k] = 0.0f;
+
+  double st_c = 0.0, st_1 = 0.0, st_2 = 0.0, st_n = 0.0;
+
+  const GT* g0 = reinterpret_cast<const GT*>(in.g[0]);
+  const GT* g1 = reinterpret_cast<const GT*>(in.g[1]);
+  const GT* g2 = reinterpret_cast<const GT*>(in.g[2]);
+  float* Pw = P + (size_t)w * T.wave_stride;
+
+  for (int ez = ez0; ez < ez1; ++ez) {
+    const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+    float Ux[2][2][3];
+    if (MODE == MODE_MATVEC) {
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+          const long long n0 = node_index(M, i0, ey + cy, ez + cz);
+          const long long n1 = node_index(M, i1, ey + cy, ez + cz);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            Ux[cy][cz][k] = X0 * v[3 * n0 + k] + X1 * v[3 * n1 + k];
+        }
+    }
+    for (int tz = 0; tz < nzp; ++tz) {
+      const int z = ez * h + tz;
+      const float Z1 = tz * inv_h;
+      const float Z0 = 1.0f - Z1;
+      const float Wxz = Wx * axis_weight(z, M.Nvz - 1);
+      const int mxz = mx * axis_mult(z, h, M.Nvz - 1);
+      float u0[3] = {0, 0, 0}, du[3] = {0, 0, 0};
+      if (MODE == MODE_MATVEC) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          u0[k] = Z0 * Ux[0][0][k] + Z1 * Ux[0][1][k];
+          const float u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
+          du[k] = (u1 - u0[k]) * inv_h;
+        }
+      }
+      float A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+      const long long base = ((long long)z * M.Nvy + y0) * M.Nvx + xe;
+#pragma unroll 4
+      for (int ty = 0; ty < nrows; ++ty) {
+        const long long vi = base + (long long)ty * M.Nvx;
+        const int y = y0 + ty;
+        const float W = Wxz * axis_weight(y, M.Nvy - 1);
+        const float gx = ld_val(g0, vi);
+        const float gy = ld_val(g1, vi);
+        const float gz = ld_val(g2, vi);
+        float m = 1.0f;
+        if (HASMASK) m = in.mask[vi] ? 1.0f : 0.0f;
+        const float fty = (float)ty;
+        if (MODE == MODE_MATVEC) {
+          const float ux = u0[0] + fty * du[0];
+          const float uy = u0[1] + fty * du[1];
+          const float uz = u0[2] + fty * du[2];
+          const float s = W * m * (gx * ux + gy * uy + gz * uz);
+          const float st = s * fty;
+          A[0] += s * gx; A[1] += s * gy; A[2] += s * gz;
+          B[0] += st * gx; B[1] += st * gy; B[2] += st * gz;
+        } else if (MODE == MODE_RHS) {
+          const uint8_t bm = in.mask[vi];
+          float rv = 0.0f;
+          if (act) {
+            const float fval = ld_val(reinterpret_cast<const FT*>(in.fimg), vi);
+            const float gv = in.gvals[vi];
+            // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
+            // the reference's order, no contraction
+            rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
+                           __fsub_rn(fval, in.mean0));
+            if (!bm) rv = 0.0f;
+            in.res[vi] = rv;
+            if (in.live[vi]) {
+              const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
+              const double rd = (double)rv;
+              st_c += mlt;
+              st_1 += mlt * rd;
+              st_2 += mlt * rd * rd;
+              st_n += 1.0;
+            }
+            if (FOLD) {
+              const float fm = bm ? 1.0f : 0.0f;
+              in.fold_out[0][vi] = __bfloat16_as_ushort(__float2bfloat16(gx * fm));
+              in.fold_out[1][vi] = __bfloat16_as_ushort(__float2bfloat16(gy * fm));
+              in.fold_out[2][vi] = __bfloat16_as_ushort(__float2bfloat16(gz * fm));
+            }
+          }
+          const float s = W * (bm ? 1.0f : 0.0f) * rv;
+          const float st = s * fty;
+          A[0] += s * gx; A[1] += s * gy; A[2] += s * gz;
+          B[0] += st * gx; B[1] += st * gy; B[2] += st * gz;
+        } else {  // MODE_DIAG
+          const float Y1 = fty * inv_h;
+          const float Y0 = 1.0f - Y1;
+          const float wm = W * m;
+          const float q0 = wm * gx * gx, q1 = wm * gy * gy, q2 = wm * gz * gz;
+          const float y0w = Y0 * Y0, y1w = Y1 * Y1;
+          A[0] += y0w * q0; A[1] += y0w * q1; A[2] += y0w * q2;
+          B[0] += y1w * q0; B[1] += y1w * q1; B[2] += y1w * q2;
+        }
+      }
+      float R[2][3];
+      if (MODE == MODE_DIAG) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { R[0][k] = A[k]; R[1][k] = B[k]; }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          R[1][k] = B[k] * inv_h;
+          R[0][k] = A[k] - R[1][k];
+        }
+      }
+      float zw0 = Z0, zw1 = Z1;
+      if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          C[cy][0][k] += zw0 * R[cy][k];
+          C[cy][1][k] += zw1 * R[cy][k];
+        }
+    }
+    // node plane kz = ez is complete for this wave: emit and shift
+    {
+      float c[6];
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[cy * 3 + k] = C[cy][0][k];
+      emit_plane<POW2>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
+                       Pw + (size_t)(ez - ez0) * 2 * T.px * 3, T.px, lds);
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { C[cy][0][k] = C[cy][1][k]; C[cy][1][k] = 0.0f; }
+    }
+  }
+  {
+    float c[6];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[cy * 3 + k] = C[cy][0][k];
+    emit_plane<POW2>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
+                     Pw + (size_t)(ez1 - ez0) * 2 * T.px * 3, T.px, lds);
+  }
+  if (MODE == MODE_RHS) {
+    st_c = wave_sum(st_c);
+    st_1 = wave_sum(st_1);
+    st_2 = wave_sum(st_2);
+    st_n = wave_sum(st_n);
+    if (lane == 0) {
+      in.stats[4 * w + 0] = st_c;
+      in.stats[4 * w + 1] = st_1;
+      in.stats[4 * w + 2] = st_2;
+      in.stats[4 * w + 3] = st_n;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// deformed-image sampling g(x + u(x)) + active mask + statistics over b_mask
+// ---------------------------------------------------------------------------
+
+template <typename GI>
+__device__ __forceinline__ double sample_tri(const GI* __restrict__ img, int nx, int ny, int nz,
+                                             float cx, float cy, float cz) {
+  // order-1 interpolation at coordinates inside [0, n-1], upper neighbour
+  // clamped ('nearest'), weights in double
+  const int ix = (int)floorf(cx), iy = (int)floorf(cy), iz = (int)floorf(cz);
+  const int jx = min(ix + 1, nx - 1), jy = min(iy + 1, ny - 1), jz = min(iz + 1, nz - 1);
+  const double fx = (double)cx - ix, fy = (double)cy - iy, fz = (double)cz - iz;
+  const long long sy = nx, sz = (long long)nx * ny;
+  const long long b00 = iy * sy + iz * sz, b10 = jy * sy + iz * sz;
+  const long long b01 = iy * sy + jz * sz, b11 = jy * sy + jz * sz;
+  const double v000 = ld_val(img, b00 + ix), v100 = ld_val(img, b00 + jx);
+  const double v010 = ld_val(img, b10 + ix), v110 = ld_val(img, b10 + jx);
+  const double v001 = ld_val(img, b01 + ix), v101 = ld_val(img, b01 + jx);
+  const double v011 = ld_val(img, b11 + ix), v111 = ld_val(img, b11 + jx);
+  const double gx0 = 1.0 - fx, gy0 = 1.0 - fy, gz0 = 1.0 - fz;
+  return gz0 * (gy0 * (gx0 * v000 + fx * v100) + fy * (gx0 * v010 + fx * v110)) +
+         fz * (gy0 * (gx0 * v001 + fx * v101) + fy * (gx0 * v011 + fx * v111));
+}
+
+template <typename GI>
+__global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev T, SampleIn S,
+                                                            const float* __restrict__ U) {
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const long long w = (long long)blk * VDVC_WAVES_PER_BLOCK + wib;
+  if (w >= T.nwaves) return;
+  const int h = M.h;
+  const float inv_h = M.inv_h;
+  const int sx = (int)(w % T.nsx);
+  const long long r = w / T.nsx;
+  const int ey = (int)(r % M.Ny);
+  const int zt = (int)(r / M.Ny);
+  const int x0 = sx * 64;
+  const int nact = min(64, M.Nvx - x0);
+  const bool act = lane < nact;
+  const int xe = act ? x0 + lane : x0 + nact - 1;
+  const int ex = xe / h;
+  const int tx = xe - ex * h;
+  const float X1 = tx * inv_h, X0 = 1.0f - X1;
+  const int i0 = min(ex, M.Nx), i1 = min(ex + 1, M.Nx);
+  const int mx = axis_mult(xe, h, M.Nvx - 1);
+  const int ez0 = zt * T.ez, ez1 = min(M.Nz, ez0 + T.ez);
+  const int y0 = ey * h;
+  const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
+  const float fgnx = (float)(S.gnx - 1), fgny = (float)(S.gny - 1), fgnz = (float)(S.gnz - 1);
+  const GI* img = reinterpret_cast<const GI*>(S.gimg);
+  const float vxf = (float)xe;
+  double sc = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int ez = ez0; ez < ez1; ++ez) {
+    const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+    float Ux[2][2][3];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int cz = 0; cz < 2; ++cz) {
+        const long long n0 = node_index(M, i0, ey + cy, ez + cz);
+        const long long n1 = node_index(M, i1, ey + cy, ez + cz);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Ux[cy][cz][k] = X0 * U[3 * n0 + k] + X1 * U[3 * n1 + k];
+      }
+    for (int tz = 0; tz < nzp; ++tz) {
+      const int z = ez * h + tz;
+      const float Z1 = tz * inv_h, Z0 = 1.0f - Z1;
+      const int mxz = mx * axis_mult(z, h, M.Nvz - 1);
+      float u0[3], du[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        u0[k] = Z0 * Ux[0][0][k] + Z1 * Ux[0][1][k];
+        const float u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
+        du[k] = (u1 - u0[k]) * inv_h;
+      }
+      const long long base = ((long long)z * M.Nvy + y0) * M.Nvx + xe;
+      for (int ty = 0; ty < nrows; ++ty) {
+        const long long vi = base + (long long)ty * M.Nvx;
+        const int y = y0 + ty;
+        const float fty = (float)ty;
+        const float ux = u0[0] + fty * du[0];
+        const float uy = u0[1] + fty * du[1];
+        const float uz = u0[2] + fty * du[2];
+        // (u + x) - origin, float32, as the reference composes it
+        float cx = __fsub_rn(__fadd_rn(ux, vxf), S.gox);
+        float cy = __fsub_rn(__fadd_rn(uy, (float)y), S.goy);
+        float cz = __fsub_rn(__fadd_rn(uz, (float)z), S.goz);
+        const bool inb = (cx >= 0.0f) && (cx <= fgnx) && (cy >= 0.0f) && (cy <= fgny) &&
+                         (cz >= 0.0f) && (cz <= fgnz);
+        cx = fminf(fmaxf(cx, 0.0f), fgnx);
+        cy = fminf(fmaxf(cy, 0.0f), fgny);
+        cz = fminf(fmaxf(cz, 0.0f), fgnz);
+        // a NaN coordinate clamps to 0 here (NumPy would sample NaN); it is
+        // out of bounds either way, so the mask is false
+        const float gv = (float)sample_tri(img, S.gnx, S.gny, S.gnz, cx, cy, cz);
+        const bool m0 = S.mask0 ? (S.mask0[vi] != 0) : true;
+        const bool mk = inb && m0 && (gv > S.glt);
+        if (act) {
+          S.gvals[vi] = gv;
+          S.mask_out[vi] = mk ? 1 : 0;
+          const bool bm = S.bmask ? (S.bmask[vi] != 0) : mk;
+          if (bm) {
+            const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
+            const double gd = (double)gv;
+            sc += mlt;
+            s1 += mlt * gd;
+            s2 += mlt * gd * gd;
+          }
+        }
+      }
+    }
+  }
+  sc = wave_sum(sc);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane == 0) {
+    S.stats[3 * w + 0] = sc;
+    S.stats[3 * w + 1] = s1;
+    S.stats[3 * w + 2] = s2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reference-image setup: np.gradient (edge_order=1) per axis, mask0 = f > glt,
+// gradient zeroed where the 6-cross erosion of mask0 ('nearest') is false
+// ---------------------------------------------------------------------------
+template <typename FT, typename GO>
+__global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* __restrict__ f,
+                                                           float glt, GO* __restrict__ gxo,
+                                                           GO* __restrict__ gyo,
+                                                           GO* __restrict__ gzo,
+                                                           uint8_t* __restrict__ mask0) {
+  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  const long long row = blockIdx.x / nxb;
+  const int x = (blockIdx.x % nxb) * VDVC_BLOCK + threadIdx.x;
+  if (x >= M.Nvx || row >= (long long)M.Nvy * M.Nvz) return;
+  const int y = (int)(row % M.Nvy);
+  const int z = (int)(row / M.Nvy);
+  const long long vi = row * M.Nvx + x;
+  const long long sy = M.Nvx, sz = (long long)M.Nvx * M.Nvy;
+  const float fc = ld_val(f, vi);
+  const int xm = x > 0 ? x - 1 : 0, xp = x < M.Nvx - 1 ? x + 1 : x;
+  const int ym = y > 0 ? y - 1 : 0, yp = y < M.Nvy - 1 ? y + 1 : y;
+  const int zm = z > 0 ? z - 1 : 0, zp = z < M.Nvz - 1 ? z + 1 : z;
+  const float fxm = ld_val(f, vi + (xm - x)), fxp = ld_val(f, vi + (xp - x));
+  const float fym = ld_val(f, vi + (long long)(ym - y) * sy), fyp = ld_val(f, vi + (long long)(yp - y) * sy);
+  const float fzm = ld_val(f, vi + (long long)(zm - z) * sz), fzp = ld_val(f, vi + (long long)(zp - z) * sz);
+  auto grad1 = [](float fm, float fp, int t, int n) -> float {
+    if (t == 0 || t == n - 1) return __fsub_rn(fp, fm);  // one-sided, spacing 1
+    return __fsub_rn(fp, fm) * 0.5f;                    // central
+  };
+  float gx = grad1(fxm, fxp, x, M.Nvx);
+  float gy = grad1(fym, fyp, y, M.Nvy);
+  float gz = grad1(fzm, fzp, z, M.Nvz);
+  const bool m0 = fc > glt;
+  const bool er = m0 && (fxm > glt) && (fxp > glt) && (fym > glt) && (fyp > glt) && (fzm > glt) &&
+                  (fzp > glt);
+  if (!er) { gx = 0.0f; gy = 0.0f; gz = 0.0f; }
+  mask0[vi] = m0 ? 1 : 0;
+  if constexpr (sizeof(GO) == 4) {
+    gxo[vi] = gx; gyo[vi] = gy; gzo[vi] = gz;
+  } else {
+    gxo[vi] = __float2bfloat16(gx);
+    gyo[vi] = __float2bfloat16(gy);
+    gzo[vi] = __float2bfloat16(gz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// multiplicity-weighted masked statistics of a voxel field, per-block FP64
+// partials (count, sum, sum of squares)
+// ---------------------------------------------------------------------------
+template <typename FT>
+__global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* __restrict__ fld,
+                                                           const uint8_t* __restrict__ mask,
+                                                           int use_thresh, float glt,
+                                                           double* __restrict__ part) {
+  __shared__ double lds[4 * 3];
+  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  const long long nrows = (long long)M.Nvy * M.Nvz;
+  double acc[3] = {0.0, 0.0, 0.0};
+  // grid-stride over (row, x-block) pairs
+  for (long long it = blockIdx.x; it < nrows * nxb; it += gridDim.x) {
+    const long long row = it / nxb;
+    const int x = (int)(it % nxb) * VDVC_BLOCK + threadIdx.x;
+    if (x < M.Nvx) {
+      const int y = (int)(row % M.Nvy);
+      const int z = (int)(row / M.Nvy);
+      const long long vi = row * M.Nvx + x;
+      const float val = ld_val(fld, vi);
+      const bool m = use_thresh ? (val > glt) : (mask ? mask[vi] != 0 : true);
+      if (m) {
+        const double mlt = (double)(axis_mult(x, M.h, M.Nvx - 1) * axis_mult(y, M.h, M.Nvy - 1) *
+                                    axis_mult(z, M.h, M.Nvz - 1));
+        const double d = (double)val;
+        acc[0] += mlt;
+        acc[1] += mlt * d;
+        acc[2] += mlt * d * d;
+      }
+    }
+  }
+  block_sum_f64<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x + 0] = acc[0];
+    part[3 * blockIdx.x + 1] = acc[1];
+    part[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
+// fixed-order reduction of n partial records of width NV (deterministic)
+__global__ __launch_bounds__(VDVC_BLOCK) void reduce_partials_kernel(const double* __restrict__ part,
+                                                                     long long n, int nv,
+                                                                     double* __restrict__ out) {
+  __shared__ double lds[VDVC_BLOCK];
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += VDVC_BLOCK) s += part[i * nv + q];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = lds[0];
+    __syncthreads();
+  }
+}
+
+// bfloat16 gradient planes multiplied by a 0/1 voxel mask (the folded
+// storage read by the Hessian kernels when no separate mask is given)
+__global__ __launch_bounds__(VDVC_BLOCK) void fold_kernel(long long n,
+                                                          const uint16_t* __restrict__ gx,
+                                                          const uint16_t* __restrict__ gy,
+                                                          const uint16_t* __restrict__ gz,
+                                                          const uint8_t* __restrict__ m,
+                                                          uint16_t* __restrict__ ox,
+                                                          uint16_t* __restrict__ oy,
+                                                          uint16_t* __restrict__ oz) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const bool k = m[q] != 0;
+  ox[q] = k ? gx[q] : (uint16_t)0;
+  oy[q] = k ? gy[q] : (uint16_t)0;
+  oz[q] = k ? gz[q] : (uint16_t)0;
+}
+
+// 6-cross erosion of a uint8 mask with edge replication
+// (minimum_filter(mode='nearest'), engine/correlate_gpu.py:75-94)
+__global__ __launch_bounds__(VDVC_BLOCK) void erode6_kernel(int nx, int ny, int nz,
+                                                            const uint8_t* __restrict__ in,
+                                                            uint8_t* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  const long long n = (long long)nx * ny * nz;
+  if (q >= n) return;
+  const int x = (int)(q % nx);
+  const long long r = q / nx;
+  const int y = (int)(r % ny);
+  const int z = (int)(r / ny);
+  const long long sy = nx, sz = (long long)nx * ny;
+  bool v = in[q] != 0;
+  v = v && in[q + (x > 0 ? -1 : 0)] && in[q + (x < nx - 1 ? 1 : 0)];
+  v = v && in[q + (y > 0 ? -sy : 0)] && in[q + (y < ny - 1 ? sy : 0)];
+  v = v && in[q + (z > 0 ? -sz : 0)] && in[q + (z < nz - 1 ? sz : 0)];
+  out[q] = v ? 1 : 0;
+}
+
+// out = a * b elementwise (float32)
+__global__ __launch_bounds__(VDVC_BLOCK) void mul_kernel(long long n, const float* __restrict__ a,
+                                                         const float* __restrict__ b,
+                                                         float* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q < n) out[q] = a[q] * b[q];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (C++ linkage, used by the C-ABI layer)
+// ---------------------------------------------------------------------------
+namespace vdvc {
+
+int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
+                    const TilesDev& T, const VoxIn& in, const float* v, float* P,
+                    hipStream_t st) {
+  const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
+  const bool pow2 = (M.h <= 64) && ((64 % M.h) == 0);
+  dim3 g(nblk), b(VDVC_BLOCK);
+#define VDVC_L(MODE, GT, HM, FT, P2, FO) \
+  hipLaunchKernelGGL((vox_restrict_kernel<MODE, GT, HM, FT, P2, FO>), g, b, 0, st, M, T, in, v, P)
+#define VDVC_P2(MODE, GT, HM, FT, FO) \
+  do { if (pow2) VDVC_L(MODE, GT, HM, FT, true, FO); else VDVC_L(MODE, GT, HM, FT, false, FO); } while (0)
+#define VDVC_HM(MODE, GT, FT, FO) \
+  do { if (hasmask) VDVC_P2(MODE, GT, true, FT, FO); else VDVC_P2(MODE, GT, false, FT, FO); } while (0)
+  if (mode == MODE_MATVEC || mode == MODE_DIAG) {
+    if (mode == MODE_MATVEC) {
+      if (gtype == GT_F32) VDVC_HM(MODE_MATVEC, float, float, false);
+      else VDVC_HM(MODE_MATVEC, __hip_bfloat16, float, false);
+    } else {
+      if (gtype == GT_F32) VDVC_HM(MODE_DIAG, float, float, false);
+      else VDVC_HM(MODE_DIAG, __hip_bfloat16, float, false);
+    }
+  } else {
+    // RHS always reads the b_mask explicitly
+    if (gtype == GT_F32) {
+      if (ftype == GT_F32) {
+        if (fold) VDVC_P2(MODE_RHS, float, true, float, true);
+        else VDVC_P2(MODE_RHS, float, true, float, false);
+      } else {
+        if (fold) VDVC_P2(MODE_RHS, float, true, __half, true);
+        else VDVC_P2(MODE_RHS, float, true, __half, false);
+      }
+    } else {
+      if (ftype == GT_F32) {
+        if (fold) VDVC_P2(MODE_RHS, __hip_bfloat16, true, float, true);
+        else VDVC_P2(MODE_RHS, __hip_bfloat16, true, float, false);
+      } else {
+        if (fold) VDVC_P2(MODE_RHS, __hip_bfloat16, true, __half, true);
+        else VDVC_P2(MODE_RHS, __hip_bfloat16, true, __half, false);
+      }
+    }
+  }
+#undef VDVC_HM
+#undef VDVC_P2
+#undef VDVC_L
+  return (int)hipGetLastError();
+}
+
+int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,
+                  const float* U, hipStream_t st) {
+  const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
+  if (gitype == GT_F32)
+    hipLaunchKernelGGL(sample_kernel<float>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S, U);
+  else
+    hipLaunchKernelGGL(sample_kernel<__half>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S, U);
+  return (int)hipGetLastError();
+}
+
+int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, float glt, void* gx,
+                 void* gy, void* gz, uint8_t* mask0, hipStream_t st) {
+  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  const long long nb = (long long)nxb * M.Nvy * M.Nvz;
+  if (nb > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+  dim3 g((unsigned)nb), b(VDVC_BLOCK);
+  if (ftype == GT_F32) {
+    if (gtype == GT_F32)
+      hipLaunchKernelGGL((setup_kernel<float, float>), g, b, 0, st, M, (const float*)f, glt,
+                         (float*)gx, (float*)gy, (float*)gz, mask0);
+    else
+      hipLaunchKernelGGL((setup_kernel<float, __hip_bfloat16>), g, b, 0, st, M, (const float*)f,
+                         glt, (__hip_bfloat16*)gx, (__hip_bfloat16*)gy, (__hip_bfloat16*)gz, mask0);
+  } else {
+    if (gtype == GT_F32)
+      hipLaunchKernelGGL((setup_kernel<__half, float>), g, b, 0, st, M, (const __half*)f, glt,
+                         (float*)gx, (float*)gy, (float*)gz, mask0);
+    else
+      hipLaunchKernelGGL((setup_kernel<__half, __hip_bfloat16>), g, b, 0, st, M, (const __half*)f,
+                         glt, (__hip_bfloat16*)gx, (__hip_bfloat16*)gy, (__hip_bfloat16*)gz, mask0);
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
+                 float glt, double* part, int nblocks, hipStream_t st) {
+  if (ftype == GT_F32)
+    hipLaunchKernelGGL(stats_kernel<float>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
+                       (const float*)fld, mask, use_thresh, glt, part);
+  else
+    hipLaunchKernelGGL(stats_kernel<__half>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
+                       (const __half*)fld, mask, use_thresh, glt, part);
+  return (int)hipGetLastError();
+}
+
+int launch_fold(long long n, const uint16_t* gx, const uint16_t* gy, const uint16_t* gz,
+                const uint8_t* m, uint16_t* ox, uint16_t* oy, uint16_t* oz, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(fold_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, gx, gy, gz, m, ox, oy, oz);
+  return (int)hipGetLastError();
+}
+
+int launch_erode6(int nx, int ny, int nz, const uint8_t* in, uint8_t* out, hipStream_t st) {
+  const long long n = (long long)nx * ny * nz;
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(erode6_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, nx, ny, nz, in, out);
+  return (int)hipGetLastError();
+}
+
+int launch_mul(long long n, const float* a, const float* b, float* out, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(mul_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, a, b, out);
+  return (int)hipGetLastError();
+}
+
+int launch_reduce(const double* part, long long n, int nv, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, part, n, nv, out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace vdvc

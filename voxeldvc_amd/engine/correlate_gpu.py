@@ -1,0 +1,594 @@
+"""Global Gauss-Newton DVC solve: drop-in equivalents of the reference's
+src/voxeldvc/engine/correlate_gpu.py (same names, arguments, return values
+and errors), with every voxel and node computation in libvdvc.so.
+
+Host code here only sequences library calls and reads back the scalars the
+reference also brings to the host (statistics, norms, convergence tests).
+Device arrays are torch tensors (memory ownership only).
+
+Precision (the `storage` keyword, an extension):
+  'fp32'  float32 images and gradients plus a uint8 mask (13 B/voxel in the
+          Hessian matvec), float32 matvec arithmetic, FP64 dots and norms;
+  'bf16'  float16 images (exact for 8-bit data), bfloat16 gradients with the
+          mask folded in (6 B/voxel), same arithmetic; exact for 8-bit
+          integer images, whose gradients are multiples of 1/2 below 256.
+"""
+import math
+
+import numpy as np
+
+from .. import _lib as L
+from . import kernels_dvc as K
+from .geometry_dvc import build_N_stencil, build_K_ref_laplacian  # noqa: F401
+
+
+def _as_out(xp, t):
+    """Return a device tensor as the caller's array type."""
+    if xp is not None and getattr(xp, '__name__', '') == 'torch':
+        return t
+    return L.to_host(t)
+
+
+def _stats_mean_std(s):
+    cnt, s1, s2 = s[0], s[1], s[2]
+    if cnt < 1:
+        return 0.0, 0.0
+    mean = s1 / cnt
+    var = s2 / cnt - mean * mean
+    return mean, (var if var > 0.0 else 0.0) ** 0.5
+
+
+# ---------------------------------------------------------------------------
+# per-voxel helpers (correlate_gpu.py:49-378)
+# ---------------------------------------------------------------------------
+
+def image_grad_and_values(pix, xp=np, dtype=np.float32):
+    """(f_vox (Nvox,), grad_f_vox (Nvox, 3)) in the reference layout
+    (correlate_gpu.py:49-72).  No mask: glt = -inf keeps every gradient."""
+    if np.dtype(dtype) != np.float32:
+        raise NotImplementedError('device path computes float32 gradients')
+    t = L.require_gpu()
+    f, shp = L.to_dev_volume(pix)
+    Nvox = f.numel()
+    g = [L.empty(Nvox) for _ in range(3)]
+    m = L.empty(Nvox, 'uint8')
+    # image_setup works on any (nx,ny,nz) volume through a 1-element mesh per
+    # axis of size n-1 with h = 1
+    L.call('vdvc_image_setup', L.ctx(), shp[0] - 1, shp[1] - 1, shp[2] - 1, 1, L.ptr(f), L.F32,
+           float('-inf'), L.ptr(g[0]), L.ptr(g[1]), L.ptr(g[2]), L.F32, L.ptr(m))
+    grad = t.stack(g, dim=1)
+    return _as_out(xp, f), _as_out(xp, grad)
+
+
+def erode_mask0_vox(mask0_vox, Nvx, Nvy, Nvz, xp=np):
+    """6-cross erosion with 'nearest' edges (correlate_gpu.py:75-94)."""
+    m = L.to_dev_flat(mask0_vox, 'uint8')
+    out = L.empty(m.numel(), 'uint8')
+    L.call('vdvc_erode6', L.ctx(), Nvx, Nvy, Nvz, L.ptr(m), L.ptr(out))
+    return out.bool() if L.is_tensor(mask0_vox) else L.to_host(out).astype(bool)
+
+
+def masked_quadpoint_mean_std(field_vox, mask_vox, h, Nx, Ny, Nz, xp=np, chunk_size=None):
+    """Multiplicity-weighted masked mean / population std, FP64
+    (correlate_gpu.py:113-157)."""
+    f = L.to_dev_flat(field_vox)
+    m = L.to_dev_flat(mask_vox, 'uint8')
+    s = (L._d * 3)()
+    L.call('vdvc_masked_stats', L.ctx(), Nx, Ny, Nz, h, L.ptr(f), L.F32, L.ptr(m), 0, 0.0, s)
+    return _stats_mean_std(s)
+
+
+def sample_deformed_field(U, g_pix_x, N_stencil, h, Nx, Ny, Nz, xp=np, g_origin=(0, 0, 0),
+                          mask0_vox=None, chunk_size=None, glt=0.0):
+    """(g_vals, mask_vox): g(x+u(x)) and in_bounds & mask0 & (g > glt) at every
+    voxel (correlate_gpu.py:201-281)."""
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    dU = L.to_dev_flat(U)
+    g, gshp = L.to_dev_volume(g_pix_x)
+    m0 = None if mask0_vox is None else L.to_dev_flat(mask0_vox, 'uint8')
+    gv = L.empty(Nvox)
+    mk = L.empty(Nvox, 'uint8')
+    s = (L._d * 3)()
+    L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), L.F32, gshp[0], gshp[1],
+           gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]), float(glt),
+           L.ptr(m0), None, L.ptr(gv), L.ptr(mk), s)
+    dev = L.is_tensor(U) or L.is_tensor(g_pix_x)
+    return (gv, mk.bool()) if dev else (L.to_host(gv), L.to_host(mk).astype(bool))
+
+
+def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny, Nz, Ndof,
+                  xp=np, chunk_size=None):
+    """b = sum_q w_q m_q res_q v_eq (correlate_gpu.py:341-378)."""
+    t = L.require_gpu()
+    dev = L.is_tensor(res_vox) or L.is_tensor(grad_f_vox)
+    gx, gy, gz = K.grad_planes(grad_f_vox)
+    r = L.to_dev_flat(res_vox)
+    m = L.to_dev_flat(mask_vox, 'uint8')
+    zero = t.zeros_like(r)
+    rout = t.empty_like(r)
+    b = L.empty(Ndof)
+    s = (L._d * 4)()
+    # res = ((0 - 0) * 0 + (res - 0)) reproduces res exactly
+    L.call('vdvc_residual_rhs', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+           L.ptr(r), L.F32, 0.0, L.ptr(zero), 0.0, 0.0, L.ptr(m), L.ptr(m), L.ptr(rout), L.ptr(b),
+           None, None, None, s)
+    return b if dev else L.to_host(b)
+
+
+# ---------------------------------------------------------------------------
+# pre-alignment (correlate_gpu.py:381-567)
+# ---------------------------------------------------------------------------
+
+def _unravel_c(idx, shape):
+    nx, ny, nz = shape
+    return idx // (ny * nz), (idx // nz) % ny, idx % nz
+
+
+def _crop_dev(vol, shp, crop):
+    """Crop an x-fastest device volume of dims shp to its first crop voxels."""
+    if tuple(shp) == tuple(crop):
+        return vol
+    v = vol.view(shp[2], shp[1], shp[0])[:crop[2], :crop[1], :crop[0]]
+    return v.contiguous().view(-1)
+
+
+def _fft_shift_dev(f, fshp, g, gshp):
+    shape = tuple(min(a, b) for a, b in zip(fshp, gshp))
+    fc = _crop_dev(f, fshp, shape)
+    gc = _crop_dev(g, gshp, shape)
+    idx = (L._i64 * 1)()
+    L.call('vdvc_phase_corr', L.ctx(), L.ptr(fc), L.ptr(gc), shape[0], shape[1], shape[2], idx)
+    peak = _unravel_c(int(idx[0]), shape)
+    d = []
+    for p, n in zip(peak, shape):
+        p = int(p)
+        if p > n // 2:
+            p -= n
+        d.append(-float(p))
+    return np.array(d, dtype=np.float64)
+
+
+def fft_rigid_shift(f_pix, g_pix, xp=np):
+    """Integer shift d with g(x) ~ f(x - d), FFT phase correlation on the
+    common corner crop (correlate_gpu.py:381-414)."""
+    f, fs = L.to_dev_volume(f_pix)
+    g, gs = L.to_dev_volume(g_pix)
+    return _fft_shift_dev(f, fs, g, gs)
+
+
+def _build_affine_matrix(p):
+    """A = Rz Ry Rx diag(s) (correlate_gpu.py:436-448)."""
+    rx, ry, rz, sx, sy, sz = (float(q) for q in p[:6])
+    Rx = np.array([[1, 0, 0], [0, math.cos(rx), -math.sin(rx)], [0, math.sin(rx), math.cos(rx)]])
+    Ry = np.array([[math.cos(ry), 0, math.sin(ry)], [0, 1, 0], [-math.sin(ry), 0, math.cos(ry)]])
+    Rz = np.array([[math.cos(rz), -math.sin(rz), 0], [math.sin(rz), math.cos(rz), 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx @ np.diag([sx, sy, sz])
+
+
+def _zncc_cost(sums):
+    """1 - ZNCC from the device sums (n, sf, sg, sff, sgg, sfg), 2.0 when
+    fewer than 1000 samples or a degenerate variance (correlate_gpu.py:417-433)."""
+    n, sf, sg, sff, sgg, sfg = sums
+    if n < 1000:
+        return 2.0
+    vf = sff - sf * sf / n
+    vg = sgg - sg * sg / n
+    den = math.sqrt(max(vf, 0.0) * max(vg, 0.0))
+    if den < 1e-8:
+        return 2.0
+    return 1.0 - (sfg - sf * sg / n) / den
+
+
+def affine_prealign(f_pix, g_pix, xp=np, decimate=4, t0=None, angle_bound=0.3,
+                    scale_bound=0.15, maxiter=200, disp=False):
+    """9-DOF affine (Euler angles, axis scales, translation) minimising
+    1 - ZNCC on stride-decimated volumes (correlate_gpu.py:451-567).  Powell
+    runs on the host (SciPy, as in the reference); each cost evaluation is one
+    device warp + FP64 reduction.  Returns (A, t_full, c_full, info)."""
+    from scipy.optimize import minimize
+    f, fs = L.to_dev_volume(f_pix)
+    g, gs = L.to_dev_volume(g_pix)
+    fd, fds = _stride_dev(f, fs, decimate)
+    gd, gds = _stride_dev(g, gs, decimate)
+    c = np.array([(n - 1) / 2.0 for n in fds], dtype=np.float64)
+    t0d = _fft_shift_dev(fd, fds, gd, gds) if t0 is None else np.asarray(t0, float) / decimate
+    out6 = (L._d * 6)()
+
+    def cost(p):
+        A = _build_affine_matrix(p)
+        tt = np.asarray(p[6:9], dtype=np.float64)
+        L.call('vdvc_affine_sums', L.ctx(), L.ptr(fd), fds[0], fds[1], fds[2], L.ptr(gd),
+               gds[0], gds[1], gds[2], L.dbl(*A.ravel()), L.dbl(*tt), L.dbl(*c), out6)
+        return _zncc_cost(list(out6))
+
+    p_id = np.array([0., 0., 0., 1., 1., 1., 0., 0., 0.])
+    p_tr = p_id.copy()
+    p_tr[6:9] = t0d
+    cost_id = cost(p_id)
+    cost_tr = cost(p_tr)
+    bounds = ([(-angle_bound, angle_bound)] * 3 + [(1 - scale_bound, 1 + scale_bound)] * 3
+              + [(None, None)] * 3)
+    res = minimize(cost, p_tr, method='Powell', bounds=bounds,
+                   options={'maxiter': maxiter, 'disp': disp})
+    A = _build_affine_matrix(res.x)
+    info = {'cost_identity': cost_id, 'cost_translation_only': cost_tr,
+            'cost_final': float(res.fun), 'nfev': res.nfev, 'p': res.x}
+    return A, res.x[6:9] * decimate, c * decimate, info
+
+
+def _stride_dev(vol, shp, s, type_=L.F32):
+    """Stride-s decimation on the device; returns (tensor, shape)."""
+    if s == 1:
+        return vol, tuple(shp)
+    out_shp = tuple((n - 1) // s + 1 for n in shp)
+    out = L.empty(out_shp[0] * out_shp[1] * out_shp[2], 'float32' if type_ == L.F32 else 'float16')
+    L.call('vdvc_stride', L.ctx(), L.ptr(vol), type_, shp[0], shp[1], shp[2], s, L.ptr(out))
+    return out, out_shp
+
+
+# ---------------------------------------------------------------------------
+# single-level GN solve (correlate_gpu.py:828-1140)
+# ---------------------------------------------------------------------------
+
+class _Reg:
+    """Regularizer set-up for the library calls (correlate_gpu.py:966-1031)."""
+
+    def __init__(self, reg_type, K_lap, K_el, h, eps_reg):
+        self.kind = L.REG_NONE
+        self.E = float(h)
+        self.eps = float(eps_reg)
+        self.K1 = self.K2 = None
+        if reg_type == 'laplacian':
+            if K_lap is not None:
+                self.kind = L.REG_K
+                self.K1 = np.ascontiguousarray(np.asarray(K_lap, dtype=np.float64))
+        else:
+            if K_el is not None and K_lap is not None:
+                self.kind = L.REG_EQGAP
+                self.K1 = np.ascontiguousarray(np.asarray(K_el, dtype=np.float64))
+                self.K2 = np.ascontiguousarray(np.asarray(K_lap, dtype=np.float64))
+
+    def kp(self, K):
+        return None if K is None else K.ctypes.data_as(L._dp)
+
+    def args(self):
+        return (self.kind, self.kp(self.K1), self.kp(self.K2), self.E, self.eps)
+
+    def matvec(self, M, v, out):
+        """out = R v (without ll)."""
+        c = L.ctx()
+        if self.kind == L.REG_K:
+            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], self.kp(self.K1), self.E, None,
+                   L.ptr(v), L.ptr(out))
+        else:
+            tmp = L.empty(out.numel())
+            L.call('vdvc_matvec_eqgap', c, M[0], M[1], M[2], self.kp(self.K1), self.E, None,
+                   L.ptr(v), L.ptr(out))
+            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], self.kp(self.K2), self.E, None,
+                   L.ptr(v), L.ptr(tmp))
+            K.axpby(self.eps, tmp, 1.0, out)
+        return out
+
+
+def _grad_store(storage):
+    if storage == 'fp32':
+        return L.F32, 'float32', L.F32, 'float32'
+    if storage == 'bf16':
+        return L.BF16, 'bfloat16', L.F16, 'float16'
+    raise ValueError("storage must be 'fp32' or 'bf16'")
+
+
+def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.0, l0=None,
+                  K_ref_laplacian=None, reg_exponent=2, c_reg=1.0, reg_type='laplacian',
+                  K_ref_elastic=None, eps_reg=1e-3, dynamic_mask=False, freeze_mask_after=None,
+                  U0=None, maxiter=30, eps=1e-3, pcg_tol=1e-6, pcg_max_iter=2000, disp=True,
+                  dtype=np.float32, storage='fp32', log=None, pcg_check_every=8):
+    """Matrix-free GN DVC correlation on one level; returns (U, res, dU)
+    (correlate_gpu.py:828-1140).  Extensions: `storage` (see module doc),
+    `log` (list receiving one dict per GN iteration) and `pcg_check_every`
+    (PCG iterations between host polls of the device stop flag)."""
+    Nvx, Nvy, Nvz = Nx * h + 1, Ny * h + 1, Nz * h + 1
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    Nvox = Nvx * Nvy * Nvz
+    if Nvox > np.iinfo(np.int32).max or Ndof > np.iinfo(np.int32).max:
+        raise ValueError("Nvox=%d / Ndof=%d exceed int32 range; matrix-free index "
+                         "arrays assume int32 indices" % (Nvox, Ndof))
+    if np.dtype(dtype) != np.float32:
+        raise NotImplementedError('the device solve computes in float32 with FP64 reductions; '
+                                  'use dtype=np.float32')
+    f, fshp = L.to_dev_volume(f_pix, 'float32')
+    g, gshp = L.to_dev_volume(g_pix, 'float32')
+    if tuple(fshp) != (Nvx, Nvy, Nvz):
+        raise ValueError('f_pix shape %r != (Nx*h+1, Ny*h+1, Nz*h+1) = %r'
+                         % (tuple(fshp), (Nvx, Nvy, Nvz)))
+    st = _Level(f, g, gshp, Nx, Ny, Nz, h, storage)
+    U, res, dU = st.solve(g_origin=g_origin, glt=glt, l0=l0, K_ref_laplacian=K_ref_laplacian,
+                          reg_exponent=reg_exponent, c_reg=c_reg, reg_type=reg_type,
+                          K_ref_elastic=K_ref_elastic, eps_reg=eps_reg,
+                          dynamic_mask=dynamic_mask, freeze_mask_after=freeze_mask_after,
+                          U0=None if U0 is None else L.to_dev_flat(U0), maxiter=maxiter, eps=eps,
+                          pcg_tol=pcg_tol, pcg_max_iter=pcg_max_iter, disp=disp, log=log,
+                          pcg_check_every=pcg_check_every)
+    return _as_out(xp, U), _as_out(xp, res), _as_out(xp, dU)
+
+
+class _Level:
+    """Device state of one pyramid level: images, gradients, masks."""
+
+    def __init__(self, f, g, gshp, Nx, Ny, Nz, h, storage='fp32'):
+        self.M = (Nx, Ny, Nz, h)
+        self.Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+        self.Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+        self.storage = storage
+        self.gtype, gname, self.itype, iname = _grad_store(storage)
+        t = L.require_gpu()
+        # images in the storage type (float16 is exact for 8-bit data)
+        self.f = f if iname == 'float32' else f.to(t.float16)
+        self.g = g if iname == 'float32' else g.to(t.float16)
+        self.gshp = tuple(gshp)
+        self.gname = gname
+
+    def solve(self, g_origin, glt, l0, K_ref_laplacian, reg_exponent, c_reg, reg_type,
+              K_ref_elastic, eps_reg, dynamic_mask, freeze_mask_after, U0, maxiter, eps,
+              pcg_tol, pcg_max_iter, disp, log, pcg_check_every):
+        t = L.require_gpu()
+        c = L.ctx()
+        Nx, Ny, Nz, h = self.M
+        Nvox, Ndof = self.Nvox, self.Ndof
+        gt = self.gtype
+        # reference-image set-up: gradients (+ erosion zeroing) and mask0
+        G = [L.empty(Nvox, self.gname) for _ in range(3)]
+        mask0 = L.empty(Nvox, 'uint8')
+        L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, float(glt),
+               L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]), gt, L.ptr(mask0))
+        s3 = (L._d * 3)()
+        L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, None, 1,
+               float(glt), s3)
+        mean0, std0 = _stats_mean_std(s3)
+        mean0_f = float(np.float32(mean0))
+
+        if reg_type not in ('laplacian', 'equilibrium_gap'):
+            raise ValueError("reg_type must be 'laplacian' or 'equilibrium_gap'")
+        reg = _Reg(reg_type, K_ref_laplacian, K_ref_elastic, h, eps_reg)
+        folded = gt == L.BF16
+        # folded storage: H and b read g * mask; the working planes are
+        # re-folded whenever the active mask changes
+        Gw = [L.empty(Nvox, 'bfloat16') for _ in range(3)] if folded else G
+
+        def fold(mask):
+            L.call('vdvc_fold_grad', c, Nvox, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
+                   L.ptr(mask), L.ptr(Gw[0]), L.ptr(Gw[1]), L.ptr(Gw[2]))
+
+        def hmask(mask):
+            """(gradient planes, mask pointer) for H with `mask`."""
+            return (Gw, None) if folded else (G, mask)
+
+        hutch = None
+        if reg.kind == L.REG_EQGAP:
+            hutch = L.empty(Ndof)
+            pr = L.to_dev_flat(K.rademacher_probes(Ndof).ravel(), 'int8')
+            L.call('vdvc_hutchinson_eqgap', c, Nx, Ny, Nz, reg.kp(reg.K1), reg.E, L.ptr(pr), 32,
+                   L.ptr(hutch))
+            del pr
+
+        ll = 0.0
+        if l0 is not None:
+            if K_ref_laplacian is None:
+                raise ValueError("K_ref_laplacian is required when l0 is given")
+            if reg_type == 'equilibrium_gap' and K_ref_elastic is None:
+                raise ValueError("K_ref_elastic is required when l0 is given and "
+                                 "reg_type='equilibrium_gap'")
+            T = 10 * h
+            if folded:
+                fold(mask0)
+            Gh, mh = hmask(mask0)
+            if reg_type == 'equilibrium_gap':
+                d = L.empty(Ndof)
+                L.call('vdvc_diag_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]),
+                       gt, L.ptr(mh), L.ptr(d))
+                ones = t.ones(Ndof, dtype=t.float32, device='cuda')
+                H0 = K.dot(d, ones)
+                dl = L.empty(Ndof)
+                L.call('vdvc_diag_K', c, Nx, Ny, Nz, reg.kp(reg.K2), reg.E, None, L.ptr(dl))
+                L0 = K.dot(hutch, ones) + reg.eps * K.dot(dl, ones)
+            else:
+                nodes = np.arange((Nx + 1) * (Ny + 1) * (Nz + 1))
+                jj = (nodes // (Nx + 1)) % (Ny + 1)
+                V = np.zeros(Ndof, dtype=np.float32)
+                V[0::3] = np.cos(jj.astype(np.float32) * h / T * 2 * np.pi)
+                Vd = L.to_dev_flat(V)
+                Hv = L.empty(Ndof)
+                L.call('vdvc_matvec_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]),
+                       L.ptr(Gh[2]), gt, L.ptr(mh), L.ptr(Vd), L.ptr(Hv))
+                Lv = reg.matvec(self.M, Vd, L.empty(Ndof))
+                H0 = K.dot(Vd, Hv)
+                L0 = K.dot(Vd, Lv)
+            ll = c_reg * (l0 / T) ** reg_exponent * H0 / L0
+            if disp:
+                print('Regularization: %s with lambda = %2.3e' % (reg_type, ll))
+        elif disp:
+            print('Regularization: None')
+        llf = float(ll)
+        rk = reg.args() if l0 is not None else (L.REG_NONE, None, None, reg.E, reg.eps)
+
+        U = t.zeros(Ndof, dtype=t.float32, device='cuda') if U0 is None else U0.clone()
+        res = t.zeros(Nvox, dtype=t.float32, device='cuda')
+        dU = t.zeros(Ndof, dtype=t.float32, device='cuda')
+        diag = L.empty(Ndof)
+        Minv = L.empty(Ndof)
+        b = L.empty(Ndof)
+        RU = L.empty(Ndof)
+        gvals = L.empty(Nvox)
+        mask = L.empty(Nvox, 'uint8')
+
+        def jacobi(bm):
+            Gh, mh = hmask(bm)
+            L.call('vdvc_jacobi', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]), gt,
+                   L.ptr(mh), rk[0], rk[1], rk[2], rk[3], rk[4], llf, L.ptr(hutch),
+                   L.ptr(diag), L.ptr(Minv))
+
+        if not dynamic_mask:
+            if folded:
+                fold(mask0)
+            jacobi(mask0)
+        frozen = None
+        s4 = (L._d * 4)()
+        it_ = (L._i * 1)()
+        rr_ = (L._d * 1)()
+        for ik in range(maxiter):
+            L.call('vdvc_sample', c, Nx, Ny, Nz, h, L.ptr(U), L.ptr(self.g), self.itype,
+                   self.gshp[0], self.gshp[1], self.gshp[2], float(g_origin[0]),
+                   float(g_origin[1]), float(g_origin[2]), float(glt), L.ptr(mask0),
+                   L.ptr(frozen), L.ptr(gvals), L.ptr(mask), s3)
+            if freeze_mask_after is not None and frozen is None and ik >= freeze_mask_after:
+                frozen = mask.clone()
+            bmask = frozen if frozen is not None else mask
+            g_mean, std1 = _stats_mean_std(s3)
+            kscale = -(std0 / std1)  # ZeroDivisionError on an empty mask, as the reference
+            # folded storage with a dynamic mask: the residual kernel writes
+            # g * b_mask for this iteration's H
+            refold = folded and dynamic_mask
+            L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
+                   gt, L.ptr(self.f), self.itype, mean0_f, L.ptr(gvals),
+                   float(np.float32(g_mean)), float(np.float32(kscale)), L.ptr(bmask),
+                   L.ptr(mask), L.ptr(res), L.ptr(b),
+                   L.ptr(Gw[0]) if refold else None, L.ptr(Gw[1]) if refold else None,
+                   L.ptr(Gw[2]) if refold else None, s4)
+            if l0 is not None:
+                reg.matvec(self.M, U, RU)
+                K.axpby(-llf, RU, 1.0, b)
+            # H uses the b-mask only with a dynamic mask; otherwise the
+            # frozen-H mask0 (correlate_gpu.py:1042-1053, 1113-1126)
+            hm = bmask if dynamic_mask else mask0
+            if dynamic_mask:
+                jacobi(bmask)
+            Gh, mh = hmask(hm)
+            L.call('vdvc_pcg', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]), gt,
+                   L.ptr(mh),
+                   rk[0], rk[1], rk[2], rk[3], rk[4], llf, L.ptr(b), L.ptr(Minv), L.ptr(dU),
+                   float(pcg_tol), int(pcg_max_iter), int(pcg_check_every), it_, rr_)
+            K.axpby(1.0, dU, 1.0, U)
+            nd = K.dot(dU, dU)
+            nu = K.dot(U, U)
+            err = math.sqrt(nd) / math.sqrt(nu) if nu > 0 else (math.nan if nd == 0 else math.inf)
+            if disp or log is not None:
+                _, stdr = _stats_mean_std(s4)
+                n_active = int(round(s4[3]))
+                if log is not None:
+                    log.append(dict(iter=ik + 1, std_res=stdr, err=err, active=n_active,
+                                    pcg_iters=int(it_[0]), pcg_relres=float(rr_[0]), ll=ll))
+                if disp:
+                    print("Iter # %2d | std(res)=%2.2f gl | dU/U=%1.2e | active pts=%d/%d (%.1f%%)"
+                          % (ik + 1, stdr, err, n_active, Nvox, 100 * n_active / Nvox))
+            if err < eps:
+                break
+        return U, res, dU
+
+
+# ---------------------------------------------------------------------------
+# multiscale driver (correlate_gpu.py:1143-1305)
+# ---------------------------------------------------------------------------
+
+def _affine_U(A, t, c, s, h_i, Nx, Ny, Nz):
+    c_i = np.asarray(c, float) / s
+    t_i = np.asarray(t, float) / s
+    n = np.arange((Nx + 1) * (Ny + 1) * (Nz + 1))
+    X = np.stack([n % (Nx + 1), (n // (Nx + 1)) % (Ny + 1), n // ((Nx + 1) * (Ny + 1))],
+                 axis=1).astype(np.float64) * h_i
+    u = (X - c_i) @ np.asarray(A, float).T + c_i + t_i - X
+    U = np.zeros(3 * n.size, dtype=np.float32)
+    for k in range(3):
+        U[k::3] = u[:, k].astype(np.float32)
+    return U
+
+
+def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0),
+                             g_origin=(0, 0, 0), l0=None, fft_prealign=True,
+                             prealign_affine=False, ext_affine=None, **kwargs):
+    """Coarse-to-fine stride-decimation pyramid warm start of correlate_gpu
+    (correlate_gpu.py:1143-1305).  Images are uploaded once and decimated on
+    the device."""
+    scales = sorted(set(scales), reverse=True)
+    storage = kwargs.pop('storage', 'fp32')
+    dtype = kwargs.pop('dtype', np.float32)
+    if np.dtype(dtype) != np.float32:
+        raise NotImplementedError('the device solve computes in float32 with FP64 reductions')
+    disp = kwargs.get('disp', True)
+    A_aff = t_aff = c_aff = None
+    if ext_affine is not None:
+        A_aff, t_aff, c_aff = (np.asarray(a, dtype=np.float64) for a in ext_affine)
+        if disp:
+            print("External affine: t = (%.1f, %.1f, %.1f) voxels, "
+                  "c = (%.1f, %.1f, %.1f)" % (*t_aff, *c_aff))
+    elif prealign_affine:
+        A_aff, t_aff, c_aff, info = affine_prealign(f_pix, g_pix, xp)
+        if disp:
+            rx, ry, rz = (180 / np.pi) * np.array([
+                np.arctan2(A_aff[2, 1], A_aff[2, 2]),
+                np.arctan2(-A_aff[2, 0], np.hypot(A_aff[2, 1], A_aff[2, 2])),
+                np.arctan2(A_aff[1, 0], A_aff[0, 0])])
+            sx, sy, sz = np.linalg.norm(A_aff, axis=0)
+            print("Affine pre-alignment (full res): rotation (deg) = (%.3f, %.3f, %.3f), "
+                  "scale = (%.4f, %.4f, %.4f), t = (%.1f, %.1f, %.1f) voxels "
+                  "(1-ZNCC %.4f -> %.4f)"
+                  % (rx, ry, rz, sx, sy, sz, t_aff[0], t_aff[1], t_aff[2],
+                     info['cost_translation_only'], info['cost_final']))
+    for isc in scales:
+        s = 2 ** isc
+        if h % s != 0:
+            raise ValueError("h=%d not divisible by 2**%d=%d" % (h, isc, s))
+        if any(o % s != 0 for o in g_origin):
+            raise ValueError("g_origin=%r not divisible by 2**%d=%d for all "
+                             "requested scales" % (g_origin, isc, s))
+    f, fs = L.to_dev_volume(f_pix, 'float32')
+    g, gs = L.to_dev_volume(g_pix, 'float32')
+    U = None
+    prev = None
+    res = dU = None
+    log = kwargs.pop('log', None)
+    for isc in scales:
+        s = 2 ** isc
+        h_i = h // s
+        fi, fis = _stride_dev(f, fs, s)
+        gi, gis = _stride_dev(g, gs, s)
+        go_i = tuple(o // s for o in g_origin)
+        l0_i = None if l0 is None else l0 / s
+        if U is not None:
+            K.axpby(prev / s, U, 0.0, U)
+        elif A_aff is not None:
+            U = L.to_dev_flat(_affine_U(A_aff, t_aff, c_aff, s, h_i, Nx, Ny, Nz))
+        elif fft_prealign:
+            d = _fft_shift_dev(fi, fis, gi, gis)
+            Un = np.zeros(3 * (Nx + 1) * (Ny + 1) * (Nz + 1), dtype=np.float32)
+            for k in range(3):
+                Un[k::3] = d[k]
+            U = L.to_dev_flat(Un)
+            if disp:
+                print("FFT pre-alignment: rigid shift d = (%.1f, %.1f, %.1f) voxels"
+                      % (d[0] * s, d[1] * s, d[2] * s))
+        if disp:
+            print("=== MULTISCALE: scale 2^%d (image decimated by %d, h_i=%d/%d=%d) ==="
+                  % (isc, s, h, s, h_i))
+        if tuple(fis) != (Nx * h_i + 1, Ny * h_i + 1, Nz * h_i + 1):
+            raise ValueError('f_pix shape does not match the mesh at scale 2^%d' % isc)
+        lv = _Level(fi, gi, gis, Nx, Ny, Nz, h_i, storage)
+        sub = None if log is None else []
+        U, res, dU = lv.solve(g_origin=go_i, l0=l0_i, U0=U, log=sub,
+                              **_solve_kwargs(kwargs))
+        if log is not None:
+            log.append(dict(scale=isc, iters=sub))
+        prev = s
+    return _as_out(xp, U), _as_out(xp, res), _as_out(xp, dU)
+
+
+def _solve_kwargs(kw):
+    keys = dict(glt=0.0, K_ref_laplacian=None, reg_exponent=2, c_reg=1.0, reg_type='laplacian',
+                K_ref_elastic=None, eps_reg=1e-3, dynamic_mask=False, freeze_mask_after=None,
+                maxiter=30, eps=1e-3, pcg_tol=1e-6, pcg_max_iter=2000, disp=True,
+                pcg_check_every=8)
+    unknown = set(kw) - set(keys)
+    if unknown:
+        raise TypeError('unexpected keyword arguments %s' % sorted(unknown))
+    out = dict(keys)
+    out.update(kw)
+    return out

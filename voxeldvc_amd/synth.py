@@ -1,0 +1,177 @@
+"""Seeded synthetic DVC workloads: speckle reference volumes and known warps.
+
+Speckle:  f = background + sum_i A_i exp(-|x - c_i|^2 / (2 sigma^2)),
+c_i ~ U[0, n)^3, A_i ~ U(50, 255), truncated at 4 sigma.
+Warp (backward, as the reference's ground-truth generator
+ground_truth/prepare_ground_truth.py:63-86):  g(y) = f(y - u(y)) by
+trilinear interpolation with linear extrapolation at the border, then noise.
+
+Generation runs in NumPy for small volumes and in torch on the GPU for large
+ones (input preparation only; it is not part of the timed solve).
+"""
+import numpy as np
+
+
+def speckle_numpy(shape, n_blobs, seed, sigma=1.5, amp=(50.0, 255.0), background=20.0):
+    """float64 speckle volume of `shape` (x, y, z), C-ordered [x, y, z]."""
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(s) for s in shape)
+    c = rng.uniform(0.0, 1.0, size=(n_blobs, 3)) * np.array(shape, dtype=np.float64)
+    a = rng.uniform(amp[0], amp[1], size=n_blobs)
+    r = int(np.ceil(4 * sigma))
+    off = np.arange(-r, r + 1)
+    f = np.full(shape, background, dtype=np.float64)
+    chunk = 4096
+    for s in range(0, n_blobs, chunk):
+        cc = c[s:s + chunk]
+        aa = a[s:s + chunk]
+        base = np.floor(cc).astype(np.int64)
+        ix = base[:, 0:1] + off[None, :]
+        iy = base[:, 1:2] + off[None, :]
+        iz = base[:, 2:3] + off[None, :]
+        wx = np.exp(-((ix - cc[:, 0:1]) ** 2) / (2 * sigma ** 2))
+        wy = np.exp(-((iy - cc[:, 1:2]) ** 2) / (2 * sigma ** 2))
+        wz = np.exp(-((iz - cc[:, 2:3]) ** 2) / (2 * sigma ** 2))
+        vx = (ix >= 0) & (ix < shape[0])
+        vy = (iy >= 0) & (iy < shape[1])
+        vz = (iz >= 0) & (iz < shape[2])
+        val = (aa[:, None, None, None] * wx[:, :, None, None] * wy[:, None, :, None]
+               * wz[:, None, None, :])
+        ok = vx[:, :, None, None] & vy[:, None, :, None] & vz[:, None, None, :]
+        IX = np.broadcast_to(ix[:, :, None, None], val.shape)[ok]
+        IY = np.broadcast_to(iy[:, None, :, None], val.shape)[ok]
+        IZ = np.broadcast_to(iz[:, None, None, :], val.shape)[ok]
+        np.add.at(f, (IX, IY, IZ), val[ok])
+    return f
+
+
+def trilinear_extrap(img, px, py, pz):
+    """Trilinear sample of img at float coordinates with linear extrapolation
+    outside [0, n-1] (RegularGridInterpolator(fill_value=None) semantics)."""
+    out = np.zeros(px.shape, dtype=np.float64)
+    idx = []
+    for p, n in zip((px, py, pz), img.shape):
+        i0 = np.clip(np.floor(p).astype(np.int64), 0, n - 2)
+        idx.append((i0, p - i0))
+    (x0, fx), (y0, fy), (z0, fz) = idx
+    for dx, wx in ((0, 1 - fx), (1, fx)):
+        for dy, wy in ((0, 1 - fy), (1, fy)):
+            for dz, wz in ((0, 1 - fz), (1, fz)):
+                out += wx * wy * wz * img[x0 + dx, y0 + dy, z0 + dz]
+    return out
+
+
+def warp_backward(f, ufield, noise_sigma, seed):
+    """g(y) = f(y - u(y)) + N(0, noise_sigma); `ufield(X, Y, Z)` returns the
+    three displacement components at voxel coordinates."""
+    nx, ny, nz = f.shape
+    X, Y, Z = np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nz, dtype=np.float64), indexing='ij')
+    ux, uy, uz = ufield(X, Y, Z)
+    g = trilinear_extrap(f, X - ux, Y - uy, Z - uz)
+    if noise_sigma > 0:
+        g = g + np.random.default_rng(seed).normal(0.0, noise_sigma, size=g.shape)
+    return g
+
+
+def translation_strain_field(t=(2.3, -1.1, 0.7), ez=0.01, zc=32.0):
+    """Translation plus uniform z-strain about plane z = zc."""
+    def u(X, Y, Z):
+        return (np.full_like(X, t[0]), np.full_like(Y, t[1]), t[2] + ez * (Z - zc))
+    return u
+
+
+def necking_field(center, ez=0.02, nu_r=0.3, amp=0.5, width=40.0):
+    """Tensile necking about `center`: u_z = ez (z - cz) e(r),
+    u_r = -nu_r ez r e(r), e(r) = 1 + amp exp(-r^2 / (2 width^2))."""
+    cx, cy, cz = center
+
+    def u(X, Y, Z):
+        dx, dy = X - cx, Y - cy
+        r2 = dx * dx + dy * dy
+        e = 1.0 + amp * np.exp(-r2 / (2 * width ** 2))
+        return (-nu_r * ez * dx * e, -nu_r * ez * dy * e, ez * (Z - cz) * e)
+    return u
+
+
+def fourier_field(n_modes, wl_range, peak, seed, extent):
+    """Sum of random plane-wave modes, rescaled so that max |u| over the
+    `extent` grid is `peak` voxels."""
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=(n_modes, 3))
+    k /= np.linalg.norm(k, axis=1, keepdims=True)
+    wl = rng.uniform(wl_range[0], wl_range[1], size=n_modes)
+    kv = k * (2 * np.pi / wl)[:, None]
+    ph = rng.uniform(0, 2 * np.pi, size=n_modes)
+    av = rng.normal(size=(n_modes, 3))
+
+    def raw(X, Y, Z):
+        out = [np.zeros_like(X) for _ in range(3)]
+        for m in range(n_modes):
+            s = np.sin(kv[m, 0] * X + kv[m, 1] * Y + kv[m, 2] * Z + ph[m])
+            for c in range(3):
+                out[c] += av[m, c] * s
+        return out
+
+    g = [np.arange(0, n, max(1, n // 32), dtype=np.float64) for n in extent]
+    Xs, Ys, Zs = np.meshgrid(*g, indexing='ij')
+    r = raw(Xs, Ys, Zs)
+    scale = peak / max(1e-12, float(np.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2).max()))
+
+    def u(X, Y, Z):
+        return tuple(scale * c for c in raw(X, Y, Z))
+    return u
+
+
+def blobs_for(shape, density_ref=(250_000, 257 ** 3)):
+    """Blob count at the density of 250k blobs per 257^3."""
+    return int(round(density_ref[0] * np.prod(shape) / density_ref[1]))
+
+
+def c1_pair(seed=1, n=65, n_blobs=4000):
+    """65^3 speckle, translation (2.3,-1.1,0.7) + 1% z-strain, N(0,1)."""
+    f = speckle_numpy((n, n, n), n_blobs, seed)
+    g = warp_backward(f, translation_strain_field(zc=(n - 1) / 2.0), 1.0, seed + 1000)
+    return f, g
+
+
+def c2_pair(seed=2, n=257, n_blobs=None):
+    """257^3 speckle with the tensile-necking field, N(0,1)."""
+    n_blobs = blobs_for((n, n, n)) if n_blobs is None else n_blobs
+    f = speckle_numpy((n, n, n), n_blobs, seed)
+    c = (n - 1) / 2.0
+    g = warp_backward(f, necking_field((c, c, c)), 1.0, seed + 1000)
+    return f, g
+
+
+def speckle_torch(shape, n_blobs, seed, sigma=1.5, amp=(50.0, 255.0), background=20.0,
+                  device='cuda', chunk=8192):
+    """Same generator as speckle_numpy, evaluated in float32 on the GPU with
+    torch (for volumes too large for the host); returns a flat x-fastest
+    float32 tensor (index x + y*nx + z*nx*ny)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(s) for s in shape)
+    c = rng.uniform(0.0, 1.0, size=(n_blobs, 3)) * np.array(shape, dtype=np.float64)
+    a = rng.uniform(amp[0], amp[1], size=n_blobs)
+    r = int(np.ceil(4 * sigma))
+    nx, ny, nz = shape
+    f = torch.full((nx * ny * nz,), float(background), dtype=torch.float32, device=device)
+    off = torch.arange(-r, r + 1, device=device, dtype=torch.int64)
+    inv = 1.0 / (2 * sigma ** 2)
+    for s in range(0, n_blobs, chunk):
+        cc = torch.from_numpy(c[s:s + chunk]).to(device)
+        aa = torch.from_numpy(a[s:s + chunk]).to(device=device, dtype=torch.float32)
+        base = torch.floor(cc).to(torch.int64)
+        idx = [base[:, d:d + 1] + off[None, :] for d in range(3)]
+        w = [torch.exp(-((idx[d].to(torch.float64) - cc[:, d:d + 1]) ** 2) * inv).to(torch.float32)
+             for d in range(3)]
+        ok = [(idx[d] >= 0) & (idx[d] < shape[d]) for d in range(3)]
+        w = [w[d] * ok[d] for d in range(3)]
+        idx = [idx[d].clamp(0, shape[d] - 1) for d in range(3)]
+        val = (aa[:, None, None, None] * w[0][:, :, None, None] * w[1][:, None, :, None]
+               * w[2][:, None, None, :])
+        lin = (idx[0][:, :, None, None] + nx * (idx[1][:, None, :, None]
+                                                 + ny * idx[2][:, None, None, :]))
+        f.index_add_(0, lin.reshape(-1), val.reshape(-1))
+    return f
