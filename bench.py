@@ -77,14 +77,15 @@ def main():
     f = synth.speckle_torch((Nv, Nv, Nv), synth.blobs_for((Nv, Nv, Nv)), seed=3)
     f = f.round().clamp_(0, 255)
     c = L.ctx()
-    f16 = f.to(torch.float16)
+    f16 = L.pitch_dense(f, (Nv, Nv, Nv), 'float16')   # pitched level layout
     del f
-    Gb = [L.empty(Nvox, 'bfloat16') for _ in range(3)]
-    mask0 = L.empty(Nvox, 'uint8')
+    Np = f16.numel()
+    Gb = [L.empty(Np, 'bfloat16') for _ in range(3)]
+    mask0 = L.empty(Np, 'uint8')
     L.call('vdvc_image_setup', c, n_el, n_el, n_el, h, L.ptr(f16), L.F16, 0.0, *[L.ptr(q) for q in Gb],
            L.BF16, L.ptr(mask0))
-    Gw = [L.empty(Nvox, 'bfloat16') for _ in range(3)]
-    L.call('vdvc_fold_grad', c, Nvox, *[L.ptr(q) for q in Gb], L.ptr(mask0), *[L.ptr(q) for q in Gw])
+    Gw = [L.empty(Np, 'bfloat16') for _ in range(3)]
+    L.call('vdvc_fold_grad', c, Np, *[L.ptr(q) for q in Gb], L.ptr(mask0), *[L.ptr(q) for q in Gw])
     del Gb
     v = torch.randn(Ndof, device='cuda', dtype=torch.float32)
     out = L.empty(Ndof)
@@ -110,7 +111,7 @@ def main():
     ms_b, kms_b = run(Gw, L.BF16, None, a.steps, a.warmup)
     # float32 + uint8 mask form
     del Gw
-    G32 = [L.empty(Nvox) for _ in range(3)]
+    G32 = [L.empty(Np) for _ in range(3)]
     L.call('vdvc_image_setup', c, n_el, n_el, n_el, h, L.ptr(f16), L.F16, 0.0,
            *[L.ptr(q) for q in G32], L.F32, L.ptr(mask0))
     print('timing fp32+mask matvec', file=sys.stderr, flush=True)

@@ -7,9 +7,14 @@
  * the stream given at context creation; functions returning host values
  * synchronise that stream.
  *
- * Layouts (the reference's conventions, src/voxeldvc/engine/kernels_dvc.py:9-21):
- *   voxel volumes: x fastest, index x + y*Nvx + z*Nvx*Nvy, Nv* = N*·h + 1
- *                  (the reference's ravel(order='F') of an [x][y][z] array);
+ * Layouts (the reference's conventions, src/voxeldvc/engine/kernels_dvc.py:9-21,
+ * with a padded row pitch):
+ *   voxel volumes of a mesh level (images f, gradients, masks, sampled g,
+ *                  residual): x fastest, index x + pitch*(y + Nvy*z),
+ *                  Nv* = N*·h + 1, pitch = vdvc_voxel_pitch(Nvx) (a multiple
+ *                  of 16 >= Nvx); the voxel order is the reference's
+ *                  ravel(order='F') of an [x][y][z] array; padding voxels are
+ *                  zero and never read as data;
  *   gradients:     three planes (gx, gy, gz), each a voxel volume;
  *   nodal vectors: DOF 3*(i + j*(Nx+1) + k*(Nx+1)*(Ny+1)) + c;
  *   deformed image g for sampling: x fastest, dims (gnx, gny, gnz).
@@ -37,6 +42,10 @@ typedef struct vdvc_ctx vdvc_ctx;
 
 int vdvc_abi_version(void);
 const char* vdvc_last_error(void);
+
+/* Row pitch (elements) of the voxel volumes of a mesh level with Nvx voxels
+ * per row. */
+int vdvc_voxel_pitch(int nvx);
 
 /* Context: one device, one stream (a hipStream_t passed as an integer; 0 is
  * the null stream), library-owned scratch.  One host thread per context.

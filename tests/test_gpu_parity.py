@@ -167,8 +167,10 @@ def test_pcg_matches_oracle(gpu):
         Klap, h, Nx, Ny, Nz)
     Minv = 1.0 / (d + 1e-5 * d.min())
     xr, itr = O.pcg(A, b.astype(np.float64), Minv, max_iter=500, tol=1e-6)
-    gx, gy, gz = (L.to_dev_flat(np.ascontiguousarray(g[:, c])) for c in range(3))
-    md = L.to_dev_flat(m, 'uint8')
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = Kd.grad_planes(g, shp)
+    md = Kd._mask_u8(m, shp)
     diag = L.empty(b.size)
     minv = L.empty(b.size)
     Kp = np.ascontiguousarray(Klap)
@@ -220,8 +222,9 @@ def test_apply_A_and_jacobi(gpu, kind):
         extra = L.to_dev_flat(Kd.build_diagonal_equilibrium_gap(Kel, im, h, Nx, Ny, Nz, Ndof))
         assert rel(L.to_host(extra), hutch) < 1e-5
     ref = O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz) + ll * R(v.astype(np.float64))
-    gx, gy, gz = (L.to_dev_flat(np.ascontiguousarray(g[:, c])) for c in range(3))
-    md = L.to_dev_flat(m, 'uint8')
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = Kd.grad_planes(g, shp)
+    md = Kd._mask_u8(m, shp)
     K1p = np.ascontiguousarray(K1)
     K2p = None if K2 is None else np.ascontiguousarray(K2)
     kp = lambda a: None if a is None else a.ctypes.data_as(L._dp)
@@ -264,7 +267,7 @@ def test_correlate_golden_17(gpu):
     assert np.abs(U - z['U']).max() < 1e-3
     # residuals agree except where an in-bounds test flips at the box edge
     both = (res != 0) & (z['res'] != 0)
-    assert both.mean() > 0.9
+    assert ((res != 0) == (z['res'] != 0)).mean() > 0.99
     assert np.abs(res[both] - z['res'][both]).max() < 2e-3
 
 

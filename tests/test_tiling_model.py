@@ -1,0 +1,240 @@
+"""Lane-exact NumPy model of the voxel-streaming kernels' decomposition.
+
+The Hessian kernels split the voxel grid over wavefronts (x strips x element
+rows x z tiles), restrict per lane, reduce across the lanes of one element
+and hand node partials to a gather kernel through per-wave slot blocks.  This
+model replays that index arithmetic lane by lane (64 lanes per wave) for both
+decompositions -- the vectorised one (V voxels per lane, element-aligned
+strips, last voxel plane as a per-row tail) and the generic one (one voxel per
+lane, 64-voxel strips) -- and checks on CPU that every voxel is visited exactly
+once, every slot the gather reads was written, and the result equals the
+oracle's matvec_H / build_diagonal_H.  It mirrors vdvc_hessian.hip,
+vdvc_voxel.hip (generic kernel) and gather_partials in vdvc_node.hip.
+"""
+import numpy as np
+import pytest
+
+from oracle import dvc_oracle as O
+
+
+def aw(t, nl):
+    return np.where((t == 0) | (t == nl), 0.5, 1.0)
+
+
+def plan(Nx, Ny, Nz, h, V, target):
+    if V:
+        sw = 64 * V
+        nsx = -(-(Nx * h) // sw)
+        px = 64 * V // h + 2
+    else:
+        sw = 64
+        nsx = -(-(Nx * h + 1) // 64)
+        px = 63 // h + 3
+    base = nsx * Ny
+    nzt_t = max(1, -(-target // base))
+    ez = max(1, -(-Nz // nzt_t))
+    nzt = -(-Nz // ez)
+    return dict(V=V, sw=sw, nsx=nsx, px=px, ez=ez, nzt=nzt, nwaves=base * nzt,
+                stride=(ez + 1) * 2 * px * 3)
+
+
+def vec_width(h, itemsize):
+    if h < 1 or h > 64 or (h & (h - 1)):
+        return 0
+    vmax = 16 // itemsize
+    return min(h, vmax)
+
+
+def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
+    V = T['V']
+    Nvx, Nvy, Nvz = Nx * h + 1, Ny * h + 1, Nz * h + 1
+    nnx, nny = Nx + 1, Ny + 1
+    nid = lambda i, j, k: i + nnx * (j + nny * k)
+    P = np.full((T['nwaves'], T['stride']), np.nan)
+    lane = np.arange(64)
+    xend = Nx * h
+    G = h // V
+    for w in range(T['nwaves']):
+        sx = w % T['nsx']
+        r = w // T['nsx']
+        ey, zt = r % Ny, r // Ny
+        x0 = sx * T['sw']
+        xb = x0 + lane * V
+        act = xb < xend
+        xbc = np.where(act, xb, x0)
+        ex = xbc // h
+        tb = xbc - ex * h
+        tail = act & (xb + V == xend)
+        X1b = tb / h
+        wx0 = np.where(xb == 0, 0.5, 1.0)
+        e_lo = x0 // h
+        e_hi = min(Nx, (x0 + T['sw']) // h)
+        nn = e_hi - e_lo + 1
+        ez0 = zt * T['ez']
+        ez1 = min(Nz, ez0 + T['ez'])
+        y0 = ey * h
+        nrows = h + (1 if ey == Ny - 1 else 0)
+        C = np.zeros((2, 2, 2, 3, 64))
+
+        def emit(kzl):
+            a = C[:, :, 0].copy()                     # (cx, cy, k, lane)
+            grp = a.reshape(2, 2, 3, 64 // G, G).sum(-1)
+            base = kzl * 2 * T['px'] * 3
+            for ln in range(64 // G + 1):
+                if ln >= nn:
+                    continue
+                val = np.zeros((2, 3))
+                if ln < 64 // G:
+                    val += grp[0, :, :, ln]
+                if ln >= 1:
+                    val += grp[1, :, :, ln - 1]
+                for cy in range(2):
+                    o = base + (cy * T['px'] + ln) * 3
+                    P[w, o:o + 3] = val[cy]
+
+        for ez in range(ez0, ez1):
+            nzp = h + (1 if ez == Nz - 1 else 0)
+            for tz in range(nzp):
+                z = ez * h + tz
+                Z1 = tz / h
+                Z0 = 1 - Z1
+                Wz = float(aw(z, Nvz - 1))
+                if mode == 'mv':
+                    def nv(i, j, k):
+                        n = nid(i, j, k)
+                        return np.stack([v[3 * n + c] for c in range(3)])
+                    l0 = Z0 * nv(ex, ey, ez) + Z1 * nv(ex, ey, ez + 1)
+                    l1 = Z0 * nv(ex, ey + 1, ez) + Z1 * nv(ex, ey + 1, ez + 1)
+                    r0 = Z0 * nv(ex + 1, ey, ez) + Z1 * nv(ex + 1, ey, ez + 1)
+                    r1 = Z0 * nv(ex + 1, ey + 1, ez) + Z1 * nv(ex + 1, ey + 1, ez + 1)
+                    duL = (l1 - l0) / h
+                    duR = (r1 - r0) / h
+                A = np.zeros((2, 2, 3, 64))
+                for ty in range(nrows):
+                    y = y0 + ty
+                    Wr = Wz * float(aw(y, Nvy - 1))
+                    S = np.zeros((3, 64))
+                    Tt = np.zeros((3, 64))
+                    if mode == 'mv':
+                        uL = l0 + ty * duL
+                        d = (r0 + ty * duR) - uL
+                    for j in range(V):
+                        x = xbc + j
+                        vi = x + Nvx * (y + Nvy * z)
+                        covered[vi[act]] += 1
+                        X1 = X1b + j / h
+                        wm = (wx0 if j == 0 else 1.0) * m[vi]
+                        gg = g[vi].T
+                        if mode == 'mv':
+                            u = uL + X1 * d
+                            s = wm * (gg * u).sum(0)
+                            S += s * gg
+                            Tt += X1 * s * gg
+                        else:
+                            S += (1 - X1) ** 2 * wm * gg ** 2
+                            Tt += X1 ** 2 * wm * gg ** 2
+                    if tail.any():
+                        vi = xend + Nvx * (y + Nvy * z)
+                        covered[vi] += 1
+                        gg = g[vi][:, None]
+                        wm = 0.5 * m[vi]
+                        if mode == 'mv':
+                            s = wm * (gg * (uL + d)).sum(0)          # t = h: X1 = 1
+                            add = (s * gg)[:, tail]
+                            S[:, tail] += add
+                        else:
+                            add = wm * gg ** 2
+                        Tt[:, tail] += add
+                    S[:, ~act] = 0
+                    Tt[:, ~act] = 0
+                    R = np.stack([S - Tt, Tt]) if mode == 'mv' else np.stack([S, Tt])
+                    Y1 = ty / h
+                    wy = (Wr * (1 - Y1), Wr * Y1) if mode == 'mv' else (Wr * (1 - Y1) ** 2,
+                                                                        Wr * Y1 ** 2)
+                    A[:, 0] += wy[0] * R
+                    A[:, 1] += wy[1] * R
+                zw = (Z0, Z1) if mode == 'mv' else (Z0 ** 2, Z1 ** 2)
+                C[:, :, 0] += zw[0] * A
+                C[:, :, 1] += zw[1] * A
+            emit(ez - ez0)
+            C[:, :, 0] = C[:, :, 1]
+            C[:, :, 1] = 0
+        emit(ez1 - ez0)
+    return P
+
+
+def gather(P, T, Nx, Ny, Nz, h):
+    Nvx = Nx * h + 1
+    nnx, nny = Nx + 1, Ny + 1
+    out = np.zeros(3 * nnx * nny * (Nz + 1))
+    for k in range(Nz + 1):
+        for j in range(Ny + 1):
+            for i in range(Nx + 1):
+                s = np.zeros(3)
+                za = max(0, k // T['ez'] - 1)
+                zb = min(T['nzt'] - 1, k // T['ez'])
+                if T['V']:
+                    xa = (max(i - 1, 0) * h) // T['sw']
+                    xb = (min(i, Nx - 1) * h) // T['sw']
+                else:
+                    xa = max(0, (i - 1) * h) // 64
+                    xb = min(Nvx - 1, (i + 1) * h) // 64
+                for zt in range(za, zb + 1):
+                    kb = zt * T['ez']
+                    if k < kb or k > min(Nz, kb + T['ez']):
+                        continue
+                    for cy in (1, 0):
+                        ey = j - cy
+                        if ey < 0 or ey >= Ny:
+                            continue
+                        for sx in range(xa, xb + 1):
+                            x0 = sx * T['sw']
+                            i_lo = x0 // h
+                            if T['V']:
+                                nn = min(Nx, (x0 + T['sw']) // h) - i_lo + 1
+                            else:
+                                nact = min(64, Nvx - x0)
+                                nn = min(Nx, (x0 + nact - 1) // h + 1) - i_lo + 1
+                            ln = i - i_lo
+                            if ln < 0 or ln >= nn:
+                                continue
+                            w = sx + T['nsx'] * (ey + Ny * zt)
+                            o = ((k - kb) * 2 + cy) * T['px'] * 3 + ln * 3
+                            blk = P[w, o:o + 3]
+                            assert not np.isnan(blk).any(), ('unwritten slot', i, j, k)
+                            s += blk
+                n = i + nnx * (j + nny * k)
+                out[3 * n:3 * n + 3] = s
+    return out
+
+
+CASES = [  # (Nx, Ny, Nz, h, itemsize, target)
+    (2, 2, 2, 8, 2, 8192), (3, 2, 3, 8, 4, 4), (70, 1, 2, 8, 2, 3), (9, 2, 2, 16, 2, 5),
+    (130, 1, 1, 4, 2, 2), (65, 2, 1, 2, 2, 3), (3, 1, 2, 1, 2, 1), (2, 1, 2, 32, 2, 1),
+    (1, 1, 1, 64, 4, 1), (17, 2, 2, 4, 4, 7), (64, 1, 1, 8, 2, 1),
+]
+
+
+@pytest.mark.parametrize('case', CASES)
+def test_vectorised_decomposition(case):
+    Nx, Ny, Nz, h, isz, tg = case
+    V = vec_width(h, isz)
+    assert V and h % V == 0
+    T = plan(Nx, Ny, Nz, h, V, tg)
+    rng = np.random.default_rng(sum(case))
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    g = rng.normal(size=(Nvox, 3))
+    m = rng.random(Nvox) < 0.8
+    v = rng.normal(size=3 * (Nx + 1) * (Ny + 1) * (Nz + 1))
+    for mode, ref in (('mv', O.matvec_H(v, g, m, h, Nx, Ny, Nz)),
+                      ('dg', O.build_diagonal_H(g, m, h, Nx, Ny, Nz))):
+        cov = np.zeros(Nvox, int)
+        P = run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, cov)
+        assert (cov == 1).all(), np.unique(cov)
+        got = gather(P, T, Nx, Ny, Nz, h)
+        assert np.abs(got - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_vec_width_rule():
+    assert [vec_width(h, 2) for h in (1, 2, 4, 8, 16, 64, 3, 6, 128)] == [1, 2, 4, 8, 8, 8, 0, 0, 0]
+    assert [vec_width(h, 4) for h in (1, 2, 4, 8, 16)] == [1, 2, 4, 4, 4]

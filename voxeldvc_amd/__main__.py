@@ -1,0 +1,21 @@
+"""Command-line entry: `python -m voxeldvc_amd run WORK_DIR [--l0-factor F]`,
+the `run` subcommand of the reference's CLI (src/voxeldvc/cli.py:13-61)."""
+import sys
+
+
+def main(argv=None):
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if not argv or argv[0] in ('-h', '--help'):
+        print(__doc__)
+        return 0
+    cmd, rest = argv[0], argv[1:]
+    if cmd == 'run':
+        from .run_dvc import main as run_main
+        run_main(rest)
+        return 0
+    print('unknown subcommand %r (this package provides: run)' % cmd, file=sys.stderr)
+    return 2
+
+
+if __name__ == '__main__':
+    sys.exit(main())

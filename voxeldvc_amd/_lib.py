@@ -28,6 +28,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 # name -> argument types (all return int status)
 SIGNATURES = {
     'vdvc_abi_version': [],
+    'vdvc_voxel_pitch': [_i],
     'vdvc_ctx_create': [_i, ctypes.c_size_t, ctypes.POINTER(_vp)],
     'vdvc_ctx_destroy': [_vp],
     'vdvc_synchronize': [_vp],
@@ -204,3 +205,41 @@ def to_dev_volume(a, dtype='float32'):
 
 def to_host(x):
     return x.detach().cpu().numpy()
+
+
+def pitch_of(nvx):
+    """Row pitch of level voxel volumes (include/vdvc.h: vdvc_voxel_pitch)."""
+    return (int(nvx) + 15) & ~15
+
+
+def pitch_dense(d, shape, dtype=None):
+    """Dense x-fastest device volume of `shape` (nx, ny, nz) -> zero-padded
+    pitched volume (index x + pitch*(y + ny*z))."""
+    t = require_gpu()
+    nx, ny, nz = shape
+    p = pitch_of(nx)
+    out = t.zeros((nz, ny, p), dtype=d.dtype if dtype is None else getattr(t, dtype),
+                  device=d.device)
+    out[:, :, :nx] = d.view(nz, ny, nx)
+    return out.view(-1)
+
+
+def unpitch(d, shape):
+    """Pitched device volume -> dense x-fastest (the reference's voxel order)."""
+    nx, ny, nz = shape
+    return d.view(nz, ny, pitch_of(nx))[:, :, :nx].contiguous().view(-1)
+
+
+def to_dev_level(a, shape, dtype='float32'):
+    """Reference-order voxel array (flat (Nvox,), or (Nvox, k) columns are not
+    accepted here) or (nx, ny, nz) volume -> pitched device volume."""
+    if (is_tensor(a) and a.dim() == 3) or (not is_tensor(a) and np.ndim(a) == 3):
+        d, shp = to_dev_volume(a, dtype)
+        if tuple(shp) != tuple(shape):
+            raise ValueError('volume shape %r != %r' % (tuple(shp), tuple(shape)))
+        return pitch_dense(d, shape)
+    d = to_dev_flat(a, dtype)
+    if d.numel() != shape[0] * shape[1] * shape[2]:
+        raise ValueError('voxel array has %d entries, expected %d'
+                         % (d.numel(), shape[0] * shape[1] * shape[2]))
+    return pitch_dense(d, shape)

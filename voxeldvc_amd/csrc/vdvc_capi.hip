@@ -106,29 +106,50 @@ MeshDev make_mesh(int Nx, int Ny, int Nz, int h) {
   MeshDev M;
   M.Nx = Nx; M.Ny = Ny; M.Nz = Nz; M.h = h;
   M.Nvx = Nx * h + 1; M.Nvy = Ny * h + 1; M.Nvz = Nz * h + 1;
+  M.pitch = vdvc_pitch_of(M.Nvx);
   M.nnx = Nx + 1; M.nny = Ny + 1; M.nnz = Nz + 1;
   M.Nvox = (long long)M.Nvx * M.Nvy * M.Nvz;
+  M.Nvox_p = (long long)M.pitch * M.Nvy * M.Nvz;
   M.Nnodes = (long long)M.nnx * M.nny * M.nnz;
   M.inv_h = 1.0f / (float)h;
   return M;
 }
 
-// wave decomposition: 64-voxel x strips x element rows x z tiles, with the z
-// tile chosen so that the launch has enough wavefronts to fill 256 CUs
-TilesDev make_tiles(const MeshDev& M) {
-  TilesDev T;
-  T.nsx = (M.Nvx + 63) / 64;
+// z tiling shared by both decompositions: enough wavefronts to fill 256 CUs
+void plan_z(const MeshDev& M, TilesDev& T, long long target) {
   const long long base = (long long)T.nsx * M.Ny;
-  const long long target = 16384;
   long long nzt_t = (target + base - 1) / base;
   if (nzt_t < 1) nzt_t = 1;
   int ez = (int)((M.Nz + nzt_t - 1) / nzt_t);
   if (ez < 1) ez = 1;
   T.ez = ez;
   T.nzt = (M.Nz + ez - 1) / ez;
-  T.px = 63 / M.h + 3;
   T.wave_stride = (long long)(ez + 1) * 2 * T.px * 3;
   T.nwaves = base * T.nzt;
+}
+
+// generic decomposition: 64-voxel x strips over [0, Nvx), one voxel per lane
+TilesDev make_tiles_generic(const MeshDev& M) {
+  TilesDev T;
+  T.vec = 0;
+  T.sw = 64;
+  T.nsx = (M.Nvx + 63) / 64;
+  T.px = 63 / M.h + 3;
+  plan_z(M, T, 16384);
+  return T;
+}
+
+// decomposition of the voxel-streaming Hessian kernels: vectorised strips of
+// 64*V voxels over [0, Nx*h) when h is a power of two, else generic
+TilesDev make_tiles(const MeshDev& M, int gtype) {
+  const int V = vdvc::vec_width(M.h, gtype);
+  if (V == 0) return make_tiles_generic(M);
+  TilesDev T;
+  T.vec = V;
+  T.sw = 64 * V;
+  T.nsx = (M.Nx * M.h + T.sw - 1) / T.sw;
+  T.px = 64 * V / M.h + 2;
+  plan_z(M, T, 8192);
   return T;
 }
 
@@ -191,8 +212,12 @@ int launch_restrict_timed(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, 
                           const VoxIn& in, int ftype, bool fold, const float* v) {
   int rc = timed_event(ctx);
   if (rc) return rc;
-  VDVC_LAUNCH(vdvc::launch_restrict(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
-                                    ctx->P.as<float>(), ctx->stream));
+  if (T.vec)
+    VDVC_LAUNCH(vdvc::launch_restrict_vec(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
+                                          ctx->P.as<float>(), ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_restrict(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
+                                      ctx->P.as<float>(), ctx->stream));
   return timed_event(ctx);
 }
 
@@ -274,6 +299,8 @@ int prep_reg(vdvc_ctx* ctx, int reg_kind, const double* hK1, const double* hK2, 
 extern "C" {
 
 int vdvc_abi_version(void) { return VDVC_ABI_VERSION; }
+
+int vdvc_voxel_pitch(int nvx) { return nvx < 1 ? -1 : vdvc_pitch_of(nvx); }
 
 const char* vdvc_last_error(void) { return g_err.c_str(); }
 
@@ -375,7 +402,7 @@ int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, 
   if (gnx < 1 || gny < 1 || gnz < 1) return fail(-1, "invalid g dims");
   if (!d_U || !d_g || !d_gvals || !d_mask_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles_generic(M);
   if ((rc = ctx->stats.ensure((size_t)T.nwaves * 3 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   SampleIn S;
@@ -405,7 +432,7 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   const bool fold = d_fold_x != nullptr;
   if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   VoxIn in = vox_in(d_gx, d_gy, d_gz, d_bmask);
@@ -433,7 +460,7 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
                          VDVC_F32, false, d_v)))
     return rc;
@@ -451,7 +478,7 @@ int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
                          false, nullptr)))
     return rc;
@@ -556,7 +583,7 @@ int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   int bd1 = 0, bd2 = 0;
   if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
                          false, nullptr)))
     return rc;
@@ -592,7 +619,7 @@ int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
   int bd1 = 0, bd2 = 0;
   if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
                          VDVC_F32, false, d_v)))
     return rc;
@@ -615,7 +642,7 @@ int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, con
   int bd1 = 0, bd2 = 0;
   if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M);
+  TilesDev T = make_tiles(M, gtype);
   const long long ndof = 3 * M.Nnodes;
   const size_t vb = (size_t)ndof * sizeof(float);
   if ((rc = ctx->pr.ensure(vb)) || (rc = ctx->pz.ensure(vb)) || (rc = ctx->pp.ensure(vb)) ||

@@ -25,11 +25,19 @@
 #define VDVC_BLOCK 256
 #define VDVC_WAVES_PER_BLOCK (VDVC_BLOCK / VDVC_WAVE)
 
+// Voxel volumes of a mesh level are stored x fastest with a padded row
+// pitch (a multiple of 16 voxels, so rows start 16-byte aligned for every
+// element type and a lane can load 4-8 consecutive voxels at once):
+// index x + pitch * (y + Nvy * z).  Padding voxels are never read as data.
+__host__ __device__ inline int vdvc_pitch_of(int nvx) { return (nvx + 15) & ~15; }
+
 struct MeshDev {
   int Nx, Ny, Nz, h;
   int Nvx, Nvy, Nvz;
+  int pitch;              // row pitch of voxel volumes (elements)
   int nnx, nny, nnz;      // nodes per axis
-  long long Nvox;
+  long long Nvox;         // unique voxels
+  long long Nvox_p;       // stored voxels (pitch * Nvy * Nvz)
   long long Nnodes;
   float inv_h;
 };
@@ -41,7 +49,9 @@ struct MeshDev {
 // node gather kernel in fixed order, so results are deterministic and no
 // global atomics are used.
 struct TilesDev {
-  int nsx;      // 64-wide x strips
+  int vec;      // voxels per lane of the vectorised kernel (0: generic kernel)
+  int sw;       // strip width in voxels (64 * vec, or 64 for the generic kernel)
+  int nsx;      // x strips
   int ez;       // elements per z tile
   int nzt;      // z tiles
   int px;       // node slots per strip (>= nodes touched by 64 voxels)

@@ -1,0 +1,609 @@
+// Vectorised voxel-streaming kernel for the correlation Hessian matvec, its
+// diagonal and the fused residual + right-hand side (the GN hot path).
+//
+// Reference semantics: engine/kernels_dvc.py:452-543 (matvec_H,
+// build_diagonal_H), engine/correlate_gpu.py:341-378 and 1091-1109
+// (compute_b_gpu and the residual), in the unique-voxel form described in
+// vdvc_common.h.
+//
+// Decomposition (powers of two h <= 64).  A lane owns V consecutive voxels in
+// x (V | h, V * element size <= 16 bytes, so one aligned 16-byte load per
+// gradient plane and row); a wavefront owns a strip of 64 * V voxels -- whole
+// elements -- in x, one element row in y and ez elements in z, and walks
+// rows then planes.  Per row, the lane forms s = W m (g . u) for its V voxels
+// (u interpolated along x between the element's left and right node
+// columns), and accumulates the x-restriction S = sum s g, T = sum (t/h) s g;
+// the y-restriction is a running weighted sum per plane and the
+// z-restriction carries two node planes in registers.  At the end of every
+// element layer the h/V lanes of one element are reduced with a butterfly and
+// the strip's node plane is written to the wave's slot block; a node gather
+// kernel sums the slots of the (at most 8) waves touching a node in fixed
+// order.  The last voxel plane x = Nx*h is handled by the lane that owns the
+// last block of the last element, as its local coordinate t = h.
+#include <type_traits>
+
+#include "vdvc_kernels.h"
+
+namespace {
+
+// ---- aligned vector loads of V consecutive elements -------------------------
+template <int V>
+__device__ __forceinline__ void ld_bf16(const __hip_bfloat16* base, long long i, float (&o)[V]) {
+  const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + i;
+  if constexpr (V == 8) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      o[2 * j] = __uint_as_float(w[j] << 16);
+      o[2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
+    }
+  } else if constexpr (V == 4) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    o[0] = __uint_as_float(q.x << 16); o[1] = __uint_as_float(q.x & 0xffff0000u);
+    o[2] = __uint_as_float(q.y << 16); o[3] = __uint_as_float(q.y & 0xffff0000u);
+  } else if constexpr (V == 2) {
+    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+    o[0] = __uint_as_float(q << 16); o[1] = __uint_as_float(q & 0xffff0000u);
+  } else {
+    o[0] = __uint_as_float(((uint32_t)p[0]) << 16);
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void ld_f32(const float* base, long long i, float (&o)[V]) {
+  const float* p = base + i;
+  if constexpr (V == 8) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    const float4 b = *reinterpret_cast<const float4*>(p + 4);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  } else if constexpr (V == 4) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+  } else if constexpr (V == 2) {
+    const float2 a = *reinterpret_cast<const float2*>(p);
+    o[0] = a.x; o[1] = a.y;
+  } else {
+    o[0] = p[0];
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void ld_f16(const __half* base, long long i, float (&o)[V]) {
+  const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + i;
+  uint16_t r[V];
+  if constexpr (V == 8) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { r[2 * j] = (uint16_t)(w[j] & 0xffffu); r[2 * j + 1] = (uint16_t)(w[j] >> 16); }
+  } else if constexpr (V == 4) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    r[0] = (uint16_t)(q.x & 0xffffu); r[1] = (uint16_t)(q.x >> 16);
+    r[2] = (uint16_t)(q.y & 0xffffu); r[3] = (uint16_t)(q.y >> 16);
+  } else if constexpr (V == 2) {
+    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+    r[0] = (uint16_t)(q & 0xffffu); r[1] = (uint16_t)(q >> 16);
+  } else {
+    r[0] = p[0];
+  }
+#pragma unroll
+  for (int j = 0; j < V; ++j) o[j] = __half2float(__ushort_as_half(r[j]));
+}
+
+template <int V>
+__device__ __forceinline__ void ld_u8(const uint8_t* base, long long i, uint32_t (&o)[V]) {
+  const uint8_t* p = base + i;
+  if constexpr (V == 8) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { o[j] = (q.x >> (8 * j)) & 0xffu; o[4 + j] = (q.y >> (8 * j)) & 0xffu; }
+  } else if constexpr (V == 4) {
+    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (q >> (8 * j)) & 0xffu;
+  } else if constexpr (V == 2) {
+    const uint16_t q = *reinterpret_cast<const uint16_t*>(p);
+    o[0] = q & 0xffu; o[1] = (q >> 8) & 0xffu;
+  } else {
+    o[0] = p[0];
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void ld_any(const T* base, long long i, float (&o)[V]) {
+  if constexpr (sizeof(T) == 4) ld_f32<V>(reinterpret_cast<const float*>(base), i, o);
+  else if constexpr (std::is_same<T, __half>::value) ld_f16<V>(base, i, o);
+  else ld_bf16<V>(base, i, o);
+}
+
+template <int V>
+__device__ __forceinline__ void st_f32(float* base, long long i, const float (&v)[V]) {
+  float* p = base + i;
+  if constexpr (V == 8) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+  } else if constexpr (V == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (V == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+  } else {
+    p[0] = v[0];
+  }
+}
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  return (uint32_t)__bfloat16_as_ushort(__float2bfloat16(f));
+}
+
+template <int V>
+__device__ __forceinline__ void st_bf16(uint16_t* base, long long i, const float (&v)[V]) {
+  uint16_t* p = base + i;
+  if constexpr (V == 8) {
+    uint4 q;
+    q.x = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+    q.y = bf16_bits(v[2]) | (bf16_bits(v[3]) << 16);
+    q.z = bf16_bits(v[4]) | (bf16_bits(v[5]) << 16);
+    q.w = bf16_bits(v[6]) | (bf16_bits(v[7]) << 16);
+    *reinterpret_cast<uint4*>(p) = q;
+  } else if constexpr (V == 4) {
+    uint2 q;
+    q.x = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+    q.y = bf16_bits(v[2]) | (bf16_bits(v[3]) << 16);
+    *reinterpret_cast<uint2*>(p) = q;
+  } else if constexpr (V == 2) {
+    *reinterpret_cast<uint32_t*>(p) = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+  } else {
+    p[0] = (uint16_t)bf16_bits(v[0]);
+  }
+}
+
+// raw bits of V consecutive elements of T (a 16-, 8-, 4-, 2- or 1-byte
+// aligned load), unpacked separately so a row can be fetched one iteration
+// ahead of its use
+template <typename T, int V>
+struct Raw {
+  static constexpr int NB = V * (int)sizeof(T);
+  static constexpr int NW = NB >= 4 ? NB / 4 : 1;
+  uint32_t w[NW];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ void ld_raw(const T* base, long long i, Raw<T, V>& r) {
+  const T* p = base + i;
+  constexpr int NB = Raw<T, V>::NB;
+  if constexpr (NB == 16) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    r.w[0] = q.x; r.w[1] = q.y; r.w[2] = q.z; r.w[3] = q.w;
+  } else if constexpr (NB == 8) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    r.w[0] = q.x; r.w[1] = q.y;
+  } else if constexpr (NB == 4) {
+    r.w[0] = *reinterpret_cast<const uint32_t*>(p);
+  } else if constexpr (NB == 2) {
+    r.w[0] = *reinterpret_cast<const uint16_t*>(p);
+  } else {
+    r.w[0] = *reinterpret_cast<const uint8_t*>(p);
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void unpack(const Raw<T, V>& r, float (&o)[V]) {
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) o[j] = __uint_as_float(r.w[j]);
+  } else {  // bfloat16
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const uint32_t wd = r.w[j / 2];
+      o[j] = __uint_as_float((j & 1) ? (wd & 0xffff0000u) : (wd << 16));
+    }
+  }
+}
+
+template <int V>
+__device__ __forceinline__ uint32_t mask_bit(const Raw<uint8_t, V>& r, int j) {
+  return (r.w[j / 4] >> (8 * (j & 3))) & 0xffu;
+}
+
+// one voxel's contribution to the lane's x-restriction sums: for the linear
+// modes S += q, T += (t/h) q with q = s g; for the diagonal S += X0^2 q,
+// T += X1^2 q with q = W m g^2
+template <int MODE>
+__device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm, float X1,
+                                          float s_or_dot, float (&S)[3], float (&T)[3]) {
+  if (MODE == MODE_DIAG) {
+    const float X0 = 1.0f - X1;
+    const float a0 = X0 * X0 * wm, a1 = X1 * X1 * wm;
+    const float qx = gx * gx, qy = gy * gy, qz = gz * gz;
+    S[0] += a0 * qx; S[1] += a0 * qy; S[2] += a0 * qz;
+    T[0] += a1 * qx; T[1] += a1 * qy; T[2] += a1 * qz;
+  } else {
+    const float s = wm * s_or_dot;
+    const float qx = s * gx, qy = s * gy, qz = s * gz;
+    S[0] += qx; S[1] += qy; S[2] += qz;
+    T[0] += X1 * qx; T[1] += X1 * qy; T[2] += X1 * qz;
+  }
+}
+
+}  // namespace
+
+template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V>
+__global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
+                                                            const float* __restrict__ v,
+                                                            float* __restrict__ P) {
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const long long w = (long long)blk * VDVC_WAVES_PER_BLOCK + wib;
+  if (w >= T.nwaves) return;
+  if (in.done && *in.done) return;
+
+  const int h = M.h;
+  const float inv_h = M.inv_h;
+  const int sx = (int)(w % T.nsx);
+  const long long r = w / T.nsx;
+  const int ey = (int)(r % M.Ny);
+  const int zt = (int)(r / M.Ny);
+
+  const int SW = 64 * V;
+  const int x0 = sx * SW;
+  const int xend = M.Nx * h;              // owned voxels [0, xend); x = xend is the tail
+  const int xb = x0 + lane * V;
+  const bool act = xb < xend;
+  const int xbc = act ? xb : x0;          // in-range address for inactive lanes
+  const int ex = xbc / h;                 // element of this lane's block
+  const int tb = xbc - ex * h;            // local coordinate of the block's first voxel
+  const bool tail = act && (xb + V == xend);
+  const float X1b = tb * inv_h;
+  const float wx0 = (xb == 0) ? 0.5f : 1.0f;  // x = 0 boundary weight (voxel 0 only)
+  const int G = h / V;                    // lanes per element
+  const int e_lo = x0 / h;
+  const int e_hi = min(M.Nx, (x0 + SW) / h);
+  const int nn = e_hi - e_lo + 1;
+
+  const int ez0 = zt * T.ez;
+  const int ez1 = min(M.Nz, ez0 + T.ez);
+  const int y0 = ey * h;
+  const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
+  const long long pl = (long long)M.pitch * M.Nvy;  // plane stride
+
+  const GT* g0 = reinterpret_cast<const GT*>(in.g[0]);
+  const GT* g1 = reinterpret_cast<const GT*>(in.g[1]);
+  const GT* g2 = reinterpret_cast<const GT*>(in.g[2]);
+  const FT* fim = reinterpret_cast<const FT*>(in.fimg);
+  float* Pw = P + (size_t)w * T.wave_stride;
+
+  // node-plane accumulators C[cx][cy][cz][k]
+  float C[2][2][2][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) C[a][b][c][k] = 0.0f;
+
+  double st_c = 0.0, st_1 = 0.0, st_2 = 0.0, st_n = 0.0;
+
+  auto emit = [&](int kzl) {
+    // lanes of one element hold partial sums of that element's two x nodes
+    float a[2][2][3];
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a[cx][cy][k] = C[cx][cy][0][k];
+    for (int o = 1; o < G; o <<= 1) {
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) a[cx][cy][k] += __shfl_xor(a[cx][cy][k], o, 64);
+    }
+    float prev[2][3];
+    const int src = lane > 0 ? lane - 1 : 0;
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) prev[cy][k] = __shfl(a[1][cy][k], src, 64);
+    float* slot = Pw + (size_t)kzl * 2 * T.px * 3;
+    if ((lane % G) == 0) {
+      const int ln = lane / G;
+      if (ln < nn) {
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            slot[((size_t)cy * T.px + ln) * 3 + k] = a[0][cy][k] + (ln > 0 ? prev[cy][k] : 0.0f);
+      }
+    }
+    if (lane == 63) {
+      const int ln = 64 / G;
+      if (ln < nn) {
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) slot[((size_t)cy * T.px + ln) * 3 + k] = a[1][cy][k];
+      }
+    }
+  };
+
+  for (int ez = ez0; ez < ez1; ++ez) {
+    const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+    for (int tz = 0; tz < nzp; ++tz) {
+      const int z = ez * h + tz;
+      const float Z1 = tz * inv_h;
+      const float Z0 = 1.0f - Z1;
+      const float Wz = axis_weight(z, M.Nvz - 1);
+      const int mz = axis_mult(z, h, M.Nvz - 1);
+      // u at the element's left / right node columns, linear in y
+      float uL0[3] = {0, 0, 0}, duL[3] = {0, 0, 0}, uR0[3] = {0, 0, 0}, duR[3] = {0, 0, 0};
+      if (MODE == MODE_MATVEC) {
+        const long long n00 = node_index(M, ex, ey, ez), n01 = node_index(M, ex, ey, ez + 1);
+        const long long n10 = node_index(M, ex, ey + 1, ez), n11 = node_index(M, ex, ey + 1, ez + 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float l0 = Z0 * v[3 * n00 + k] + Z1 * v[3 * n01 + k];
+          const float l1 = Z0 * v[3 * n10 + k] + Z1 * v[3 * n11 + k];
+          const float r0 = Z0 * v[3 * (n00 + 1) + k] + Z1 * v[3 * (n01 + 1) + k];
+          const float r1 = Z0 * v[3 * (n10 + 1) + k] + Z1 * v[3 * (n11 + 1) + k];
+          uL0[k] = l0; duL[k] = (l1 - l0) * inv_h;
+          uR0[k] = r0; duR[k] = (r1 - r0) * inv_h;
+        }
+      }
+      float zw0 = Z0, zw1 = Z1;
+      if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
+      const long long rowbase = (long long)z * pl + (long long)y0 * M.pitch + xbc;
+      const long long tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
+      // software pipeline: the raw gradient (and mask) words of row ty + 1
+      // are requested before row ty is processed
+      Raw<GT, V> nx0, nx1, nx2;
+      Raw<uint8_t, V> nmk;
+      ld_raw<GT, V>(g0, rowbase, nx0);
+      ld_raw<GT, V>(g1, rowbase, nx1);
+      ld_raw<GT, V>(g2, rowbase, nx2);
+      if (HASMASK) ld_raw<uint8_t, V>(in.mask, rowbase, nmk);
+#pragma unroll 1
+      for (int ty = 0; ty < nrows; ++ty) {
+        const long long vi = rowbase + (long long)ty * M.pitch;
+        const Raw<GT, V> c0 = nx0, c1 = nx1, c2 = nx2;
+        Raw<uint8_t, V> cmk;
+        if (HASMASK) cmk = nmk;
+        if (ty + 1 < nrows) {
+          const long long vn = vi + M.pitch;
+          ld_raw<GT, V>(g0, vn, nx0);
+          ld_raw<GT, V>(g1, vn, nx1);
+          ld_raw<GT, V>(g2, vn, nx2);
+          if (HASMASK) ld_raw<uint8_t, V>(in.mask, vn, nmk);
+        }
+        const int y = y0 + ty;
+        const float fty = (float)ty;
+        const float Wr = Wz * axis_weight(y, M.Nvy - 1);
+        float gx[V], gy[V], gz[V];
+        unpack<GT, V>(c0, gx);
+        unpack<GT, V>(c1, gy);
+        unpack<GT, V>(c2, gz);
+        uint32_t mk[V];
+#pragma unroll
+        for (int j2 = 0; j2 < V; ++j2) mk[j2] = HASMASK ? mask_bit<V>(cmk, j2) : 1u;
+        float S[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
+        float ux0 = 0, uy0 = 0, uz0 = 0, dxx = 0, dxy = 0, dxz = 0;
+        if (MODE == MODE_MATVEC) {
+          ux0 = uL0[0] + fty * duL[0];
+          uy0 = uL0[1] + fty * duL[1];
+          uz0 = uL0[2] + fty * duL[2];
+          dxx = (uR0[0] + fty * duR[0]) - ux0;
+          dxy = (uR0[1] + fty * duR[1]) - uy0;
+          dxz = (uR0[2] + fty * duR[2]) - uz0;
+        }
+        float resv[V];
+        if (MODE == MODE_RHS) {
+          float fv[V], gv[V];
+          uint32_t lv[V];
+          ld_any<FT, V>(fim, vi, fv);
+          ld_f32<V>(in.gvals, vi, gv);
+          ld_u8<V>(in.live, vi, lv);
+          const int mxy = axis_mult(y, h, M.Nvy - 1) * mz;
+#pragma unroll
+          for (int j = 0; j < V; ++j) {
+            // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
+            // the reference's order, no contraction; zero outside b_mask
+            float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], in.gmean), in.kscale),
+                                 __fsub_rn(fv[j], in.mean0));
+            rv = mk[j] ? rv : 0.0f;
+            resv[j] = rv;
+            if (act && lv[j]) {
+              const int mxv = (tb + j == 0 && xb + j != 0) ? 2 : 1;
+              const double mlt = (double)(mxv * mxy);
+              st_c += mlt;
+              st_1 += mlt * (double)rv;
+              st_2 += mlt * (double)rv * (double)rv;
+              st_n += 1.0;
+            }
+          }
+          if (act) {
+            st_f32<V>(in.res, vi, resv);
+            if (FOLD) {
+              float fx[V], fy[V], fz[V];
+#pragma unroll
+              for (int j = 0; j < V; ++j) {
+                fx[j] = mk[j] ? gx[j] : 0.0f;
+                fy[j] = mk[j] ? gy[j] : 0.0f;
+                fz[j] = mk[j] ? gz[j] : 0.0f;
+              }
+              st_bf16<V>(in.fold_out[0], vi, fx);
+              st_bf16<V>(in.fold_out[1], vi, fy);
+              st_bf16<V>(in.fold_out[2], vi, fz);
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          const float X1 = X1b + (float)j * inv_h;
+          float wm = (j == 0) ? wx0 : 1.0f;
+          if (HASMASK) wm = mk[j] ? wm : 0.0f;
+          float sd = 0.0f;
+          if (MODE == MODE_MATVEC) {
+            const float ux = ux0 + X1 * dxx, uy = uy0 + X1 * dxy, uz = uz0 + X1 * dxz;
+            sd = gx[j] * ux + gy[j] * uy + gz[j] * uz;
+          } else if (MODE == MODE_RHS) {
+            sd = resv[j];
+          }
+          acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S, Tt);
+        }
+        if (tail) {
+          // last voxel plane x = Nx*h: t = h of this element, X1 = 1, W_x = 1/2
+          const long long ti = tailbase + (long long)ty * M.pitch;
+          const float tgx = ld_val(g0, ti), tgy = ld_val(g1, ti), tgz = ld_val(g2, ti);
+          float wm = 0.5f;
+          uint32_t tm = 1;
+          if (HASMASK) { tm = in.mask[ti]; wm = tm ? wm : 0.0f; }
+          float sd = 0.0f;
+          if (MODE == MODE_MATVEC) {
+            sd = tgx * (ux0 + dxx) + tgy * (uy0 + dxy) + tgz * (uz0 + dxz);
+          } else if (MODE == MODE_RHS) {
+            const float fv = ld_val(fim, ti);
+            const float gv = in.gvals[ti];
+            float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
+                                 __fsub_rn(fv, in.mean0));
+            rv = tm ? rv : 0.0f;
+            in.res[ti] = rv;
+            if (in.live[ti]) {
+              const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
+              st_c += mlt;
+              st_1 += mlt * (double)rv;
+              st_2 += mlt * (double)rv * (double)rv;
+              st_n += 1.0;
+            }
+            if (FOLD) {
+              in.fold_out[0][ti] = (uint16_t)bf16_bits(tm ? tgx : 0.0f);
+              in.fold_out[1][ti] = (uint16_t)bf16_bits(tm ? tgy : 0.0f);
+              in.fold_out[2][ti] = (uint16_t)bf16_bits(tm ? tgz : 0.0f);
+            }
+            sd = rv;
+          }
+          acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S, Tt);
+        }
+        if (!act) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { S[k] = 0.0f; Tt[k] = 0.0f; }
+        }
+        // x-restricted row values R[cx][k], weighted into the node planes
+        float R[2][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (MODE == MODE_DIAG) { R[0][k] = S[k]; R[1][k] = Tt[k]; }
+          else { R[0][k] = S[k] - Tt[k]; R[1][k] = Tt[k]; }
+        }
+        const float Y1 = fty * inv_h;
+        float wy0 = Wr * (1.0f - Y1), wy1 = Wr * Y1;
+        if (MODE == MODE_DIAG) { wy0 = Wr * (1.0f - Y1) * (1.0f - Y1); wy1 = Wr * Y1 * Y1; }
+        const float w00 = wy0 * zw0, w01 = wy0 * zw1, w10 = wy1 * zw0, w11 = wy1 * zw1;
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            C[cx][0][0][k] += w00 * R[cx][k];
+            C[cx][0][1][k] += w01 * R[cx][k];
+            C[cx][1][0][k] += w10 * R[cx][k];
+            C[cx][1][1][k] += w11 * R[cx][k];
+          }
+      }
+    }
+    emit(ez - ez0);
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { C[cx][cy][0][k] = C[cx][cy][1][k]; C[cx][cy][1][k] = 0.0f; }
+  }
+  emit(ez1 - ez0);
+  if (MODE == MODE_RHS) {
+    st_c = wave_sum(st_c);
+    st_1 = wave_sum(st_1);
+    st_2 = wave_sum(st_2);
+    st_n = wave_sum(st_n);
+    if (lane == 0) {
+      in.stats[4 * w + 0] = st_c;
+      in.stats[4 * w + 1] = st_1;
+      in.stats[4 * w + 2] = st_2;
+      in.stats[4 * w + 3] = st_n;
+    }
+  }
+}
+
+namespace vdvc {
+
+// voxels per lane for the vectorised kernel: the largest power of two that
+// divides h and fits 16 bytes of the gradient type; 0 when h is not a power
+// of two <= 64 (the generic kernel is used)
+int vec_width(int h, int gtype) {
+  if (h < 1 || h > 64 || (h & (h - 1)) != 0) return 0;
+  const int vmax = gtype == GT_F32 ? 4 : 8;
+  return h < vmax ? h : vmax;
+}
+
+int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
+                        const TilesDev& T, const VoxIn& in, const float* v, float* P,
+                        hipStream_t st) {
+  const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
+  dim3 g(nblk), b(VDVC_BLOCK);
+#define VV_L(MODE, GT, HM, FT, FO, VV) \
+  hipLaunchKernelGGL((vox_vec_kernel<MODE, GT, HM, FT, FO, VV>), g, b, 0, st, M, T, in, v, P)
+#define VV_V(MODE, GT, HM, FT, FO)                 \
+  do {                                             \
+    switch (T.vec) {                               \
+      case 8: VV_L(MODE, GT, HM, FT, FO, 8); break; \
+      case 4: VV_L(MODE, GT, HM, FT, FO, 4); break; \
+      case 2: VV_L(MODE, GT, HM, FT, FO, 2); break; \
+      default: VV_L(MODE, GT, HM, FT, FO, 1); break; \
+    }                                              \
+  } while (0)
+  if (gtype == GT_F32 && T.vec == 8) return (int)hipErrorInvalidValue;
+  if (mode == MODE_MATVEC) {
+    if (gtype == GT_F32) {
+      if (hasmask) VV_V(MODE_MATVEC, float, true, float, false);
+      else VV_V(MODE_MATVEC, float, false, float, false);
+    } else {
+      if (hasmask) VV_V(MODE_MATVEC, __hip_bfloat16, true, float, false);
+      else VV_V(MODE_MATVEC, __hip_bfloat16, false, float, false);
+    }
+  } else if (mode == MODE_DIAG) {
+    if (gtype == GT_F32) {
+      if (hasmask) VV_V(MODE_DIAG, float, true, float, false);
+      else VV_V(MODE_DIAG, float, false, float, false);
+    } else {
+      if (hasmask) VV_V(MODE_DIAG, __hip_bfloat16, true, float, false);
+      else VV_V(MODE_DIAG, __hip_bfloat16, false, float, false);
+    }
+  } else {
+    if (gtype == GT_F32) {
+      if (ftype == GT_F32) {
+        if (fold) VV_V(MODE_RHS, float, true, float, true);
+        else VV_V(MODE_RHS, float, true, float, false);
+      } else {
+        if (fold) VV_V(MODE_RHS, float, true, __half, true);
+        else VV_V(MODE_RHS, float, true, __half, false);
+      }
+    } else {
+      if (ftype == GT_F32) {
+        if (fold) VV_V(MODE_RHS, __hip_bfloat16, true, float, true);
+        else VV_V(MODE_RHS, __hip_bfloat16, true, float, false);
+      } else {
+        if (fold) VV_V(MODE_RHS, __hip_bfloat16, true, __half, true);
+        else VV_V(MODE_RHS, __hip_bfloat16, true, __half, false);
+      }
+    }
+  }
+#undef VV_V
+#undef VV_L
+  return (int)hipGetLastError();
+}
+
+}  // namespace vdvc

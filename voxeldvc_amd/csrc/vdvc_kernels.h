@@ -79,6 +79,10 @@ struct AffineIn {
 };
 
 namespace vdvc {
+int vec_width(int h, int gtype);
+int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
+                        const TilesDev& T, const VoxIn& in, const float* v, float* P,
+                        hipStream_t st);
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                     const TilesDev& T, const VoxIn& in, const float* v, float* P, hipStream_t st);
 int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,

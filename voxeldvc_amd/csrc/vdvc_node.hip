@@ -63,15 +63,23 @@ __device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp& op, int i,
   }
 }
 
-// sum of the node partials written by the voxel kernels for node (i,j,k)
+// sum of the node partials written by the voxel kernels for node (i,j,k),
+// in fixed order (z tiles, then y rows, then x strips)
 __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev& T,
                                                 const float* __restrict__ P, int i, int j, int k,
                                                 float (&s)[3]) {
   s[0] = s[1] = s[2] = 0.0f;
   const int h = M.h;
   const int za = max(0, k / T.ez - 1), zb = min(T.nzt - 1, k / T.ez);
-  const int xa = max(0, (i - 1) * h) / 64;
-  const int xb = min(M.Nvx - 1, (i + 1) * h) / 64;
+  int xa, xb;
+  if (T.vec) {
+    // strips of sw voxels over [0, Nx*h) holding element i-1 or element i
+    xa = (max(i - 1, 0) * h) / T.sw;
+    xb = (min(i, M.Nx - 1) * h) / T.sw;
+  } else {
+    xa = max(0, (i - 1) * h) / 64;
+    xb = min(M.Nvx - 1, (i + 1) * h) / 64;
+  }
   for (int zt = za; zt <= zb; ++zt) {
     const int kb = zt * T.ez;
     if (k < kb || k > min(M.Nz, kb + T.ez)) continue;
@@ -80,10 +88,16 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
       const int ey = j - cy;
       if (ey < 0 || ey >= M.Ny) continue;
       for (int sx = xa; sx <= xb; ++sx) {
-        const int x0 = sx * 64;
-        const int nact = min(64, M.Nvx - x0);
-        const int i_lo = x0 / h;
-        const int nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
+        const int x0 = sx * T.sw;
+        int i_lo, nn;
+        if (T.vec) {
+          i_lo = x0 / h;
+          nn = min(M.Nx, (x0 + T.sw) / h) - i_lo + 1;
+        } else {
+          const int nact = min(64, M.Nvx - x0);
+          i_lo = x0 / h;
+          nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
+        }
         const int ln = i - i_lo;
         if (ln < 0 || ln >= nn) continue;
         const long long w = (long long)sx + (long long)T.nsx * (ey + (long long)M.Ny * zt);
@@ -93,7 +107,6 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
     }
   }
 }
-
 
 __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp op) {
   __shared__ double lds[4];

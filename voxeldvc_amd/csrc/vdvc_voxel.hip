@@ -198,10 +198,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
         }
       }
       float A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
-      const long long base = ((long long)z * M.Nvy + y0) * M.Nvx + xe;
+      const long long base = ((long long)z * M.Nvy + y0) * M.pitch + xe;
 #pragma unroll 4
       for (int ty = 0; ty < nrows; ++ty) {
-        const long long vi = base + (long long)ty * M.Nvx;
+        const long long vi = base + (long long)ty * M.pitch;
         const int y = y0 + ty;
         const float W = Wxz * axis_weight(y, M.Nvy - 1);
         const float gx = ld_val(g0, vi);
@@ -395,9 +395,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev 
         const float u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
         du[k] = (u1 - u0[k]) * inv_h;
       }
-      const long long base = ((long long)z * M.Nvy + y0) * M.Nvx + xe;
+      const long long base = ((long long)z * M.Nvy + y0) * M.pitch + xe;
       for (int ty = 0; ty < nrows; ++ty) {
-        const long long vi = base + (long long)ty * M.Nvx;
+        const long long vi = base + (long long)ty * M.pitch;
         const int y = y0 + ty;
         const float fty = (float)ty;
         const float ux = u0[0] + fty * du[0];
@@ -452,14 +452,25 @@ __global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* 
                                                            GO* __restrict__ gyo,
                                                            GO* __restrict__ gzo,
                                                            uint8_t* __restrict__ mask0) {
-  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  const int nxb = (M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long row = blockIdx.x / nxb;
   const int x = (blockIdx.x % nxb) * VDVC_BLOCK + threadIdx.x;
-  if (x >= M.Nvx || row >= (long long)M.Nvy * M.Nvz) return;
+  if (x >= M.pitch || row >= (long long)M.Nvy * M.Nvz) return;
+  const long long vi = row * M.pitch + x;
+  if (x >= M.Nvx) {  // row padding: zero gradients and mask
+    mask0[vi] = 0;
+    if constexpr (sizeof(GO) == 4) {
+      gxo[vi] = 0.0f; gyo[vi] = 0.0f; gzo[vi] = 0.0f;
+    } else {
+      gxo[vi] = __float2bfloat16(0.0f);
+      gyo[vi] = __float2bfloat16(0.0f);
+      gzo[vi] = __float2bfloat16(0.0f);
+    }
+    return;
+  }
   const int y = (int)(row % M.Nvy);
   const int z = (int)(row / M.Nvy);
-  const long long vi = row * M.Nvx + x;
-  const long long sy = M.Nvx, sz = (long long)M.Nvx * M.Nvy;
+  const long long sy = M.pitch, sz = (long long)M.pitch * M.Nvy;
   const float fc = ld_val(f, vi);
   const int xm = x > 0 ? x - 1 : 0, xp = x < M.Nvx - 1 ? x + 1 : x;
   const int ym = y > 0 ? y - 1 : 0, yp = y < M.Nvy - 1 ? y + 1 : y;
@@ -508,7 +519,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
     if (x < M.Nvx) {
       const int y = (int)(row % M.Nvy);
       const int z = (int)(row / M.Nvy);
-      const long long vi = row * M.Nvx + x;
+      const long long vi = row * M.pitch + x;
       const float val = ld_val(fld, vi);
       const bool m = use_thresh ? (val > glt) : (mask ? mask[vi] != 0 : true);
       if (m) {
@@ -657,7 +668,7 @@ int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleI
 
 int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, float glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st) {
-  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  const int nxb = (M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long nb = (long long)nxb * M.Nvy * M.Nvz;
   if (nb > 0x7fffffffLL) return (int)hipErrorInvalidValue;
   dim3 g((unsigned)nb), b(VDVC_BLOCK);

@@ -48,16 +48,16 @@ def image_grad_and_values(pix, xp=np, dtype=np.float32):
     if np.dtype(dtype) != np.float32:
         raise NotImplementedError('device path computes float32 gradients')
     t = L.require_gpu()
-    f, shp = L.to_dev_volume(pix)
-    Nvox = f.numel()
-    g = [L.empty(Nvox) for _ in range(3)]
-    m = L.empty(Nvox, 'uint8')
-    # image_setup works on any (nx,ny,nz) volume through a 1-element mesh per
-    # axis of size n-1 with h = 1
+    fd, shp = L.to_dev_volume(pix)
+    f = L.pitch_dense(fd, shp)
+    g = [L.zeros(f.numel()) for _ in range(3)]
+    m = L.zeros(f.numel(), 'uint8')
+    # image_setup works on any (nx,ny,nz) volume through a mesh of n-1
+    # elements per axis with h = 1
     L.call('vdvc_image_setup', L.ctx(), shp[0] - 1, shp[1] - 1, shp[2] - 1, 1, L.ptr(f), L.F32,
            float('-inf'), L.ptr(g[0]), L.ptr(g[1]), L.ptr(g[2]), L.F32, L.ptr(m))
-    grad = t.stack(g, dim=1)
-    return _as_out(xp, f), _as_out(xp, grad)
+    grad = t.stack([L.unpitch(q, shp) for q in g], dim=1)
+    return _as_out(xp, fd), _as_out(xp, grad)
 
 
 def erode_mask0_vox(mask0_vox, Nvx, Nvy, Nvz, xp=np):
@@ -71,8 +71,9 @@ def erode_mask0_vox(mask0_vox, Nvx, Nvy, Nvz, xp=np):
 def masked_quadpoint_mean_std(field_vox, mask_vox, h, Nx, Ny, Nz, xp=np, chunk_size=None):
     """Multiplicity-weighted masked mean / population std, FP64
     (correlate_gpu.py:113-157)."""
-    f = L.to_dev_flat(field_vox)
-    m = L.to_dev_flat(mask_vox, 'uint8')
+    shp = K.level_shape(h, Nx, Ny, Nz)
+    f = L.to_dev_level(field_vox, shp)
+    m = L.to_dev_level(mask_vox, shp, 'uint8')
     s = (L._d * 3)()
     L.call('vdvc_masked_stats', L.ctx(), Nx, Ny, Nz, h, L.ptr(f), L.F32, L.ptr(m), 0, 0.0, s)
     return _stats_mean_std(s)
@@ -82,16 +83,19 @@ def sample_deformed_field(U, g_pix_x, N_stencil, h, Nx, Ny, Nz, xp=np, g_origin=
                           mask0_vox=None, chunk_size=None, glt=0.0):
     """(g_vals, mask_vox): g(x+u(x)) and in_bounds & mask0 & (g > glt) at every
     voxel (correlate_gpu.py:201-281)."""
-    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    shp = K.level_shape(h, Nx, Ny, Nz)
+    Np = L.pitch_of(shp[0]) * shp[1] * shp[2]
     dU = L.to_dev_flat(U)
     g, gshp = L.to_dev_volume(g_pix_x)
-    m0 = None if mask0_vox is None else L.to_dev_flat(mask0_vox, 'uint8')
-    gv = L.empty(Nvox)
-    mk = L.empty(Nvox, 'uint8')
+    m0 = None if mask0_vox is None else L.to_dev_level(mask0_vox, shp, 'uint8')
+    gv = L.zeros(Np)
+    mk = L.zeros(Np, 'uint8')
     s = (L._d * 3)()
     L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), L.F32, gshp[0], gshp[1],
            gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]), float(glt),
            L.ptr(m0), None, L.ptr(gv), L.ptr(mk), s)
+    gv = L.unpitch(gv, shp)
+    mk = L.unpitch(mk, shp)
     dev = L.is_tensor(U) or L.is_tensor(g_pix_x)
     return (gv, mk.bool()) if dev else (L.to_host(gv), L.to_host(mk).astype(bool))
 
@@ -101,9 +105,10 @@ def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny
     """b = sum_q w_q m_q res_q v_eq (correlate_gpu.py:341-378)."""
     t = L.require_gpu()
     dev = L.is_tensor(res_vox) or L.is_tensor(grad_f_vox)
-    gx, gy, gz = K.grad_planes(grad_f_vox)
-    r = L.to_dev_flat(res_vox)
-    m = L.to_dev_flat(mask_vox, 'uint8')
+    shp = K.level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = K.grad_planes(grad_f_vox, shp)
+    r = L.to_dev_level(res_vox, shp)
+    m = L.to_dev_level(mask_vox, shp, 'uint8')
     zero = t.zeros_like(r)
     rout = t.empty_like(r)
     b = L.empty(Ndof)
@@ -316,14 +321,20 @@ class _Level:
     """Device state of one pyramid level: images, gradients, masks."""
 
     def __init__(self, f, g, gshp, Nx, Ny, Nz, h, storage='fp32'):
+        """f: dense x-fastest reference volume of the level; g: dense deformed
+        volume of dims gshp."""
         self.M = (Nx, Ny, Nz, h)
-        self.Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+        self.shape = K.level_shape(h, Nx, Ny, Nz)
+        self.Nvox = self.shape[0] * self.shape[1] * self.shape[2]
+        self.Nvox_p = L.pitch_of(self.shape[0]) * self.shape[1] * self.shape[2]
         self.Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
         self.storage = storage
         self.gtype, gname, self.itype, iname = _grad_store(storage)
         t = L.require_gpu()
-        # images in the storage type (float16 is exact for 8-bit data)
-        self.f = f if iname == 'float32' else f.to(t.float16)
+        # images in the storage type (float16 is exact for 8-bit data); the
+        # reference volume in the pitched level layout
+        fp = L.pitch_dense(f, self.shape)
+        self.f = fp if iname == 'float32' else fp.to(t.float16)
         self.g = g if iname == 'float32' else g.to(t.float16)
         self.gshp = tuple(gshp)
         self.gname = gname
@@ -334,9 +345,10 @@ class _Level:
         t = L.require_gpu()
         c = L.ctx()
         Nx, Ny, Nz, h = self.M
-        Nvox, Ndof = self.Nvox, self.Ndof
+        Nvox, Ndof = self.Nvox_p, self.Ndof
         gt = self.gtype
-        # reference-image set-up: gradients (+ erosion zeroing) and mask0
+        # reference-image set-up: gradients (+ erosion zeroing) and mask0, in
+        # the pitched level layout (padding zeroed by the kernel)
         G = [L.empty(Nvox, self.gname) for _ in range(3)]
         mask0 = L.empty(Nvox, 'uint8')
         L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, float(glt),
@@ -353,10 +365,10 @@ class _Level:
         folded = gt == L.BF16
         # folded storage: H and b read g * mask; the working planes are
         # re-folded whenever the active mask changes
-        Gw = [L.empty(Nvox, 'bfloat16') for _ in range(3)] if folded else G
+        Gw = [L.zeros(Nvox, 'bfloat16') for _ in range(3)] if folded else G
 
         def fold(mask):
-            L.call('vdvc_fold_grad', c, Nvox, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
+            L.call('vdvc_fold_grad', c, self.Nvox_p, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
                    L.ptr(mask), L.ptr(Gw[0]), L.ptr(Gw[1]), L.ptr(Gw[2]))
 
         def hmask(mask):
@@ -412,14 +424,14 @@ class _Level:
         rk = reg.args() if l0 is not None else (L.REG_NONE, None, None, reg.E, reg.eps)
 
         U = t.zeros(Ndof, dtype=t.float32, device='cuda') if U0 is None else U0.clone()
-        res = t.zeros(Nvox, dtype=t.float32, device='cuda')
+        res = t.zeros(Nvox, dtype=t.float32, device='cuda')   # pitched
         dU = t.zeros(Ndof, dtype=t.float32, device='cuda')
         diag = L.empty(Ndof)
         Minv = L.empty(Ndof)
         b = L.empty(Ndof)
         RU = L.empty(Ndof)
-        gvals = L.empty(Nvox)
-        mask = L.empty(Nvox, 'uint8')
+        gvals = L.zeros(Nvox)
+        mask = L.zeros(Nvox, 'uint8')
 
         def jacobi(bm):
             Gh, mh = hmask(bm)
@@ -479,10 +491,10 @@ class _Level:
                                     pcg_iters=int(it_[0]), pcg_relres=float(rr_[0]), ll=ll))
                 if disp:
                     print("Iter # %2d | std(res)=%2.2f gl | dU/U=%1.2e | active pts=%d/%d (%.1f%%)"
-                          % (ik + 1, stdr, err, n_active, Nvox, 100 * n_active / Nvox))
+                          % (ik + 1, stdr, err, n_active, self.Nvox, 100 * n_active / self.Nvox))
             if err < eps:
                 break
-        return U, res, dU
+        return U, L.unpitch(res, self.shape), dU
 
 
 # ---------------------------------------------------------------------------

@@ -60,15 +60,19 @@ class _Out:
         return t if self.device else L.to_host(t)
 
 
-def grad_planes(grad_f_vox):
-    """(Nvox, 3) gradient (reference layout) -> three contiguous float32
-    device planes."""
+def level_shape(h, Nx, Ny, Nz):
+    return (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+
+
+def grad_planes(grad_f_vox, shape):
+    """(Nvox, 3) gradient (reference layout) -> three pitched float32 device
+    planes."""
     g = L.to_dev_flat(grad_f_vox).view(-1, 3)
-    return [g[:, c].contiguous() for c in range(3)]
+    return [L.pitch_dense(g[:, c].contiguous(), shape) for c in range(3)]
 
 
-def _mask_u8(mask_vox):
-    return None if mask_vox is None else L.to_dev_flat(mask_vox, 'uint8')
+def _mask_u8(mask_vox, shape):
+    return None if mask_vox is None else L.to_dev_level(mask_vox, shape, 'uint8')
 
 
 def _check_stencil(N_stencil, h):
@@ -105,10 +109,11 @@ def matvec_H(v, grad_f_vox, mask0_vox, N_stencil, w_stencil, h, Nx, Ny, Nz, xp=n
     """f = H @ v (kernels_dvc.py:452-506)."""
     _check_stencil(N_stencil, h)
     out = _Out(v, grad_f_vox)
-    gx, gy, gz = grad_planes(grad_f_vox)
+    shp = level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = grad_planes(grad_f_vox, shp)
     dv = L.to_dev_flat(v)
     res = L.empty(dv.numel())
-    m = _mask_u8(mask0_vox)
+    m = _mask_u8(mask0_vox, shp)
     L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
            L.ptr(m), L.ptr(dv), L.ptr(res))
     return out(res)
@@ -119,9 +124,10 @@ def build_diagonal_H(grad_f_vox, mask0_vox, N_stencil, w_stencil, h, Nx, Ny, Nz,
     """diag(H) (kernels_dvc.py:509-543)."""
     _check_stencil(N_stencil, h)
     out = _Out(grad_f_vox)
-    gx, gy, gz = grad_planes(grad_f_vox)
+    shp = level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = grad_planes(grad_f_vox, shp)
     res = L.empty(Ndof)
-    m = _mask_u8(mask0_vox)
+    m = _mask_u8(mask0_vox, shp)
     L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
            L.ptr(m), L.ptr(res))
     return out(res)
