@@ -21,7 +21,7 @@ def aw(t, nl):
     return np.where((t == 0) | (t == nl), 0.5, 1.0)
 
 
-def plan(Nx, Ny, Nz, h, V, target):
+def plan(Nx, Ny, Nz, h, V, target, nys=1):
     if V:
         sw = 64 * V
         nsx = -(-(Nx * h) // sw)
@@ -30,11 +30,12 @@ def plan(Nx, Ny, Nz, h, V, target):
         sw = 64
         nsx = -(-(Nx * h + 1) // 64)
         px = 63 // h + 3
-    base = nsx * Ny
+        nys = 1
+    base = nsx * Ny * nys
     nzt_t = max(1, -(-target // base))
     ez = max(1, -(-Nz // nzt_t))
     nzt = -(-Nz // ez)
-    return dict(V=V, sw=sw, nsx=nsx, px=px, ez=ez, nzt=nzt, nwaves=base * nzt,
+    return dict(V=V, sw=sw, nsx=nsx, nys=nys, px=px, ez=ez, nzt=nzt, nwaves=base * nzt,
                 stride=(ez + 1) * 2 * px * 3)
 
 
@@ -54,9 +55,13 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
     lane = np.arange(64)
     xend = Nx * h
     G = h // V
+    nys = T['nys']
+    rpp = h // nys                                    # rows per part
     for w in range(T['nwaves']):
         sx = w % T['nsx']
         r = w // T['nsx']
+        ys = r % nys
+        r //= nys
         ey, zt = r % Ny, r // Ny
         x0 = sx * T['sw']
         xb = x0 + lane * V
@@ -72,8 +77,9 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
         nn = e_hi - e_lo + 1
         ez0 = zt * T['ez']
         ez1 = min(Nz, ez0 + T['ez'])
-        y0 = ey * h
-        nrows = h + (1 if ey == Ny - 1 else 0)
+        ty0 = ys * rpp                                # first local row of this part
+        y0 = ey * h + ty0
+        nrows = rpp + (1 if (ey == Ny - 1 and ys == nys - 1) else 0)
         C = np.zeros((2, 2, 2, 3, 64))
 
         def emit(kzl):
@@ -116,8 +122,8 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
                     S = np.zeros((3, 64))
                     Tt = np.zeros((3, 64))
                     if mode == 'mv':
-                        uL = l0 + ty * duL
-                        d = (r0 + ty * duR) - uL
+                        uL = l0 + (ty0 + ty) * duL
+                        d = (r0 + (ty0 + ty) * duR) - uL
                     for j in range(V):
                         x = xbc + j
                         vi = x + Nvx * (y + Nvy * z)
@@ -148,7 +154,7 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
                     S[:, ~act] = 0
                     Tt[:, ~act] = 0
                     R = np.stack([S - Tt, Tt]) if mode == 'mv' else np.stack([S, Tt])
-                    Y1 = ty / h
+                    Y1 = (ty0 + ty) / h
                     wy = (Wr * (1 - Y1), Wr * Y1) if mode == 'mv' else (Wr * (1 - Y1) ** 2,
                                                                         Wr * Y1 ** 2)
                     A[:, 0] += wy[0] * R
@@ -187,7 +193,8 @@ def gather(P, T, Nx, Ny, Nz, h):
                         ey = j - cy
                         if ey < 0 or ey >= Ny:
                             continue
-                        for sx in range(xa, xb + 1):
+                        for sx, ys in ((a, b) for a in range(xa, xb + 1)
+                                       for b in range(T.get('nys', 1))):
                             x0 = sx * T['sw']
                             i_lo = x0 // h
                             if T['V']:
@@ -198,7 +205,7 @@ def gather(P, T, Nx, Ny, Nz, h):
                             ln = i - i_lo
                             if ln < 0 or ln >= nn:
                                 continue
-                            w = sx + T['nsx'] * (ey + Ny * zt)
+                            w = sx + T['nsx'] * (ys + T.get('nys', 1) * (ey + Ny * zt))
                             o = ((k - kb) * 2 + cy) * T['px'] * 3 + ln * 3
                             blk = P[w, o:o + 3]
                             assert not np.isnan(blk).any(), ('unwritten slot', i, j, k)
@@ -215,12 +222,15 @@ CASES = [  # (Nx, Ny, Nz, h, itemsize, target)
 ]
 
 
+@pytest.mark.parametrize('nys', [1, 2])
 @pytest.mark.parametrize('case', CASES)
-def test_vectorised_decomposition(case):
+def test_vectorised_decomposition(case, nys):
     Nx, Ny, Nz, h, isz, tg = case
     V = vec_width(h, isz)
     assert V and h % V == 0
-    T = plan(Nx, Ny, Nz, h, V, tg)
+    if h % nys:
+        pytest.skip('row split needs nys | h')
+    T = plan(Nx, Ny, Nz, h, V, tg, nys)
     rng = np.random.default_rng(sum(case))
     Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
     g = rng.normal(size=(Nvox, 3))

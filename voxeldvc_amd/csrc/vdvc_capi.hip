@@ -117,7 +117,7 @@ MeshDev make_mesh(int Nx, int Ny, int Nz, int h) {
 
 // z tiling shared by both decompositions: enough wavefronts to fill 256 CUs
 void plan_z(const MeshDev& M, TilesDev& T, long long target) {
-  const long long base = (long long)T.nsx * M.Ny;
+  const long long base = (long long)T.nsx * M.Ny * T.nys;
   long long nzt_t = (target + base - 1) / base;
   if (nzt_t < 1) nzt_t = 1;
   int ez = (int)((M.Nz + nzt_t - 1) / nzt_t);
@@ -133,6 +133,7 @@ TilesDev make_tiles_generic(const MeshDev& M) {
   TilesDev T;
   T.vec = 0;
   T.sw = 64;
+  T.nys = 1;
   T.nsx = (M.Nvx + 63) / 64;
   T.px = 63 / M.h + 3;
   plan_z(M, T, 16384);
@@ -149,7 +150,14 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   T.sw = 64 * V;
   T.nsx = (M.Nx * M.h + T.sw - 1) / T.sw;
   T.px = 64 * V / M.h + 2;
+  T.nys = 1;
   plan_z(M, T, 8192);
+  // too few wavefronts for the chip even with one element layer per wave:
+  // split each element row's rows between two wavefronts
+  if (T.nwaves < 8192 && M.h >= 4 && (M.h % 2) == 0) {
+    T.nys = 2;
+    plan_z(M, T, 8192);
+  }
   return T;
 }
 

@@ -52,6 +52,7 @@ struct TilesDev {
   int vec;      // voxels per lane of the vectorised kernel (0: generic kernel)
   int sw;       // strip width in voxels (64 * vec, or 64 for the generic kernel)
   int nsx;      // x strips
+  int nys;      // parts an element row's h rows are split into (vectorised kernel)
   int ez;       // elements per z tile
   int nzt;      // z tiles
   int px;       // node slots per strip (>= nodes touched by 64 voxels)

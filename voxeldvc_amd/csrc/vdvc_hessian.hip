@@ -20,6 +20,7 @@
 // kernel sums the slots of the (at most 8) waves touching a node in fixed
 // order.  The last voxel plane x = Nx*h is handled by the lane that owns the
 // last block of the last element, as its local coordinate t = h.
+#include <cstdlib>
 #include <type_traits>
 
 #include "vdvc_kernels.h"
@@ -202,6 +203,20 @@ __device__ __forceinline__ void unpack(const Raw<T, V>& r, float (&o)[V]) {
   }
 }
 
+// raw scalar of type T (the last voxel plane's values are fetched as scalars)
+template <typename T>
+__device__ __forceinline__ uint32_t ld_raw1(const T* base, long long i) {
+  if constexpr (sizeof(T) == 4) return *reinterpret_cast<const uint32_t*>(base + i);
+  else if constexpr (sizeof(T) == 2) return *reinterpret_cast<const uint16_t*>(base + i);
+  else return *reinterpret_cast<const uint8_t*>(base + i);
+}
+
+template <typename T>
+__device__ __forceinline__ float unpack1(uint32_t r) {
+  if constexpr (sizeof(T) == 4) return __uint_as_float(r);
+  else return __uint_as_float(r << 16);  // bfloat16
+}
+
 template <int V>
 __device__ __forceinline__ uint32_t mask_bit(const Raw<uint8_t, V>& r, int j) {
   return (r.w[j / 4] >> (8 * (j & 3))) & 0xffu;
@@ -221,9 +236,9 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
     T[0] += a1 * qx; T[1] += a1 * qy; T[2] += a1 * qz;
   } else {
     const float s = wm * s_or_dot;
-    const float qx = s * gx, qy = s * gy, qz = s * gz;
-    S[0] += qx; S[1] += qy; S[2] += qz;
-    T[0] += X1 * qx; T[1] += X1 * qy; T[2] += X1 * qz;
+    const float t = X1 * s;
+    S[0] += s * gx; S[1] += s * gy; S[2] += s * gz;
+    T[0] += t * gx; T[1] += t * gy; T[2] += t * gz;
   }
 }
 
@@ -233,6 +248,7 @@ template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V>
 __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
                                                             const float* __restrict__ v,
                                                             float* __restrict__ P) {
+  __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 24 : 1];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -243,7 +259,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
   const int h = M.h;
   const float inv_h = M.inv_h;
   const int sx = (int)(w % T.nsx);
-  const long long r = w / T.nsx;
+  long long r = w / T.nsx;
+  const int ys = (int)(r % T.nys);
+  r /= T.nys;
   const int ey = (int)(r % M.Ny);
   const int zt = (int)(r / M.Ny);
 
@@ -265,8 +283,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
 
   const int ez0 = zt * T.ez;
   const int ez1 = min(M.Nz, ez0 + T.ez);
-  const int y0 = ey * h;
-  const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
+  // this wave's rows of the element row: [ty0, ty0 + rows per part), plus
+  // the outer boundary row for the last part of the last element row
+  const int rpp = h / T.nys;
+  const int ty0 = ys * rpp;
+  const int y0 = ey * h + ty0;
+  const int nrows = rpp + ((ey == M.Ny - 1 && ys == T.nys - 1) ? 1 : 0);
   const long long pl = (long long)M.pitch * M.Nvy;  // plane stride
 
   const GT* g0 = reinterpret_cast<const GT*>(in.g[0]);
@@ -333,195 +355,280 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
     }
   };
 
-  for (int ez = ez0; ez < ez1; ++ez) {
-    const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
-    for (int tz = 0; tz < nzp; ++tz) {
-      const int z = ez * h + tz;
-      const float Z1 = tz * inv_h;
-      const float Z0 = 1.0f - Z1;
-      const float Wz = axis_weight(z, M.Nvz - 1);
-      const int mz = axis_mult(z, h, M.Nvz - 1);
-      // u at the element's left / right node columns, linear in y
-      float uL0[3] = {0, 0, 0}, duL[3] = {0, 0, 0}, uR0[3] = {0, 0, 0}, duR[3] = {0, 0, 0};
-      if (MODE == MODE_MATVEC) {
-        const long long n00 = node_index(M, ex, ey, ez), n01 = node_index(M, ex, ey, ez + 1);
-        const long long n10 = node_index(M, ex, ey + 1, ez), n11 = node_index(M, ex, ey + 1, ez + 1);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float l0 = Z0 * v[3 * n00 + k] + Z1 * v[3 * n01 + k];
-          const float l1 = Z0 * v[3 * n10 + k] + Z1 * v[3 * n11 + k];
-          const float r0 = Z0 * v[3 * (n00 + 1) + k] + Z1 * v[3 * (n01 + 1) + k];
-          const float r1 = Z0 * v[3 * (n10 + 1) + k] + Z1 * v[3 * (n11 + 1) + k];
-          uL0[k] = l0; duL[k] = (l1 - l0) * inv_h;
-          uR0[k] = r0; duR[k] = (r1 - r0) * inv_h;
-        }
-      }
-      float zw0 = Z0, zw1 = Z1;
-      if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
-      const long long rowbase = (long long)z * pl + (long long)y0 * M.pitch + xbc;
-      const long long tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
-      // software pipeline: the raw gradient (and mask) words of row ty + 1
-      // are requested before row ty is processed
-      Raw<GT, V> nx0, nx1, nx2;
-      Raw<uint8_t, V> nmk;
-      ld_raw<GT, V>(g0, rowbase, nx0);
-      ld_raw<GT, V>(g1, rowbase, nx1);
-      ld_raw<GT, V>(g2, rowbase, nx2);
-      if (HASMASK) ld_raw<uint8_t, V>(in.mask, rowbase, nmk);
-#pragma unroll 1
-      for (int ty = 0; ty < nrows; ++ty) {
-        const long long vi = rowbase + (long long)ty * M.pitch;
-        const Raw<GT, V> c0 = nx0, c1 = nx1, c2 = nx2;
-        Raw<uint8_t, V> cmk;
-        if (HASMASK) cmk = nmk;
-        if (ty + 1 < nrows) {
-          const long long vn = vi + M.pitch;
-          ld_raw<GT, V>(g0, vn, nx0);
-          ld_raw<GT, V>(g1, vn, nx1);
-          ld_raw<GT, V>(g2, vn, nx2);
-          if (HASMASK) ld_raw<uint8_t, V>(in.mask, vn, nmk);
-        }
-        const int y = y0 + ty;
-        const float fty = (float)ty;
-        const float Wr = Wz * axis_weight(y, M.Nvy - 1);
-        float gx[V], gy[V], gz[V];
-        unpack<GT, V>(c0, gx);
-        unpack<GT, V>(c1, gy);
-        unpack<GT, V>(c2, gz);
-        uint32_t mk[V];
-#pragma unroll
-        for (int j2 = 0; j2 < V; ++j2) mk[j2] = HASMASK ? mask_bit<V>(cmk, j2) : 1u;
-        float S[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
-        float ux0 = 0, uy0 = 0, uz0 = 0, dxx = 0, dxy = 0, dxz = 0;
-        if (MODE == MODE_MATVEC) {
-          ux0 = uL0[0] + fty * duL[0];
-          uy0 = uL0[1] + fty * duL[1];
-          uz0 = uL0[2] + fty * duL[2];
-          dxx = (uR0[0] + fty * duR[0]) - ux0;
-          dxy = (uR0[1] + fty * duR[1]) - uy0;
-          dxz = (uR0[2] + fty * duR[2]) - uz0;
-        }
-        float resv[V];
-        if (MODE == MODE_RHS) {
-          float fv[V], gv[V];
-          uint32_t lv[V];
-          ld_any<FT, V>(fim, vi, fv);
-          ld_f32<V>(in.gvals, vi, gv);
-          ld_u8<V>(in.live, vi, lv);
-          const int mxy = axis_mult(y, h, M.Nvy - 1) * mz;
-#pragma unroll
-          for (int j = 0; j < V; ++j) {
-            // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
-            // the reference's order, no contraction; zero outside b_mask
-            float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], in.gmean), in.kscale),
-                                 __fsub_rn(fv[j], in.mean0));
-            rv = mk[j] ? rv : 0.0f;
-            resv[j] = rv;
-            if (act && lv[j]) {
-              const int mxv = (tb + j == 0 && xb + j != 0) ? 2 : 1;
-              const double mlt = (double)(mxv * mxy);
-              st_c += mlt;
-              st_1 += mlt * (double)rv;
-              st_2 += mlt * (double)rv * (double)rv;
-              st_n += 1.0;
-            }
-          }
-          if (act) {
-            st_f32<V>(in.res, vi, resv);
-            if (FOLD) {
-              float fx[V], fy[V], fz[V];
-#pragma unroll
-              for (int j = 0; j < V; ++j) {
-                fx[j] = mk[j] ? gx[j] : 0.0f;
-                fy[j] = mk[j] ? gy[j] : 0.0f;
-                fz[j] = mk[j] ? gz[j] : 0.0f;
-              }
-              st_bf16<V>(in.fold_out[0], vi, fx);
-              st_bf16<V>(in.fold_out[1], vi, fy);
-              st_bf16<V>(in.fold_out[2], vi, fz);
-            }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < V; ++j) {
-          const float X1 = X1b + (float)j * inv_h;
-          float wm = (j == 0) ? wx0 : 1.0f;
-          if (HASMASK) wm = mk[j] ? wm : 0.0f;
-          float sd = 0.0f;
-          if (MODE == MODE_MATVEC) {
-            const float ux = ux0 + X1 * dxx, uy = uy0 + X1 * dxy, uz = uz0 + X1 * dxz;
-            sd = gx[j] * ux + gy[j] * uy + gz[j] * uz;
-          } else if (MODE == MODE_RHS) {
-            sd = resv[j];
-          }
-          acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S, Tt);
-        }
-        if (tail) {
-          // last voxel plane x = Nx*h: t = h of this element, X1 = 1, W_x = 1/2
-          const long long ti = tailbase + (long long)ty * M.pitch;
-          const float tgx = ld_val(g0, ti), tgy = ld_val(g1, ti), tgz = ld_val(g2, ti);
-          float wm = 0.5f;
-          uint32_t tm = 1;
-          if (HASMASK) { tm = in.mask[ti]; wm = tm ? wm : 0.0f; }
-          float sd = 0.0f;
-          if (MODE == MODE_MATVEC) {
-            sd = tgx * (ux0 + dxx) + tgy * (uy0 + dxy) + tgz * (uz0 + dxz);
-          } else if (MODE == MODE_RHS) {
-            const float fv = ld_val(fim, ti);
-            const float gv = in.gvals[ti];
-            float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
-                                 __fsub_rn(fv, in.mean0));
-            rv = tm ? rv : 0.0f;
-            in.res[ti] = rv;
-            if (in.live[ti]) {
-              const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
-              st_c += mlt;
-              st_1 += mlt * (double)rv;
-              st_2 += mlt * (double)rv * (double)rv;
-              st_n += 1.0;
-            }
-            if (FOLD) {
-              in.fold_out[0][ti] = (uint16_t)bf16_bits(tm ? tgx : 0.0f);
-              in.fold_out[1][ti] = (uint16_t)bf16_bits(tm ? tgy : 0.0f);
-              in.fold_out[2][ti] = (uint16_t)bf16_bits(tm ? tgz : 0.0f);
-            }
-            sd = rv;
-          }
-          acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S, Tt);
-        }
-        if (!act) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) { S[k] = 0.0f; Tt[k] = 0.0f; }
-        }
-        // x-restricted row values R[cx][k], weighted into the node planes
-        float R[2][3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (MODE == MODE_DIAG) { R[0][k] = S[k]; R[1][k] = Tt[k]; }
-          else { R[0][k] = S[k] - Tt[k]; R[1][k] = Tt[k]; }
-        }
-        const float Y1 = fty * inv_h;
-        float wy0 = Wr * (1.0f - Y1), wy1 = Wr * Y1;
-        if (MODE == MODE_DIAG) { wy0 = Wr * (1.0f - Y1) * (1.0f - Y1); wy1 = Wr * Y1 * Y1; }
-        const float w00 = wy0 * zw0, w01 = wy0 * zw1, w10 = wy1 * zw0, w11 = wy1 * zw1;
-#pragma unroll
-        for (int cx = 0; cx < 2; ++cx)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            C[cx][0][0][k] += w00 * R[cx][k];
-            C[cx][0][1][k] += w01 * R[cx][k];
-            C[cx][1][0][k] += w10 * R[cx][k];
-            C[cx][1][1][k] += w11 * R[cx][k];
-          }
-      }
+  // Row pipeline.  The wave's rows are streamed in the order they are used
+  // (z plane by z plane, rows within a plane); the raw gradient / mask words
+  // and the last-plane scalars of the next DEPTH rows are requested before a
+  // row is processed.  Every lane issues every load (inactive or non-tail
+  // lanes read valid in-tile addresses), so no branch separates a load from
+  // its use and the wait counts stay precise.
+  // Two buffer sets (A for even rows, B for odd rows of the tile): a row's
+  // data is copied out of its set, the set is immediately re-filled with the
+  // row two ahead, and the other set's loads stay in flight meanwhile.
+  struct RowBuf {
+    Raw<GT, V> g[3];
+    Raw<uint8_t, V> m;
+    uint32_t t[4];
+  };
+  const int NP = (ez1 - ez0) * h + (ez1 == M.Nz ? 1 : 0);  // planes of this tile
+  RowBuf bufA, bufB;
+  int cp = 0, ct = 0;  // prefetch cursor (plane, row), clamped to the tile's last row
+  auto issue = [&](RowBuf& b) {
+    const long long rb = (long long)(ez0 * h + cp) * pl + (long long)(y0 + ct) * M.pitch;
+    ld_raw<GT, V>(g0, rb + xbc, b.g[0]);
+    ld_raw<GT, V>(g1, rb + xbc, b.g[1]);
+    ld_raw<GT, V>(g2, rb + xbc, b.g[2]);
+    if (HASMASK) ld_raw<uint8_t, V>(in.mask, rb + xbc, b.m);
+    if (MODE != MODE_RHS) {
+      b.t[0] = ld_raw1<GT>(g0, rb + xend);
+      b.t[1] = ld_raw1<GT>(g1, rb + xend);
+      b.t[2] = ld_raw1<GT>(g2, rb + xend);
+      b.t[3] = HASMASK ? ld_raw1<uint8_t>(in.mask, rb + xend) : 1u;
     }
-    emit(ez - ez0);
+    if (++ct == nrows) {
+      ct = 0;
+      if (++cp == NP) { cp = NP - 1; ct = nrows - 1; }
+    }
+  };
+  issue(bufA);
+  issue(bufB);
+
+  // plane state (updated when the flattened row walk enters a new z plane)
+  int ez = ez0, tz = 0, ty = 0;
+  int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+  float Z0 = 1.0f, Z1 = 0.0f, Wz = 1.0f, zw0 = 1.0f, zw1 = 0.0f;
+  int mz = 1;
+  long long rowbase = 0, tailbase = 0;
+  float uL0[3] = {0, 0, 0}, duL[3] = {0, 0, 0}, uR0[3] = {0, 0, 0}, duR[3] = {0, 0, 0};
+  // the 24 nodal values of the lane's element (corner (cx,cy,cz), component
+  // k) live in LDS, slot q = ((cx*2 + cy)*2 + cz)*3 + k at [q][lane], loaded
+  // once per element layer; plane set-up then reads LDS only, whose wait
+  // counter is separate from the row pipeline's
+  float* nlds = lds_nodes + wib * 64 * 24;
+  auto element_setup = [&]() {
+    if (MODE != MODE_MATVEC) return;
 #pragma unroll
     for (int cx = 0; cx < 2; ++cx)
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { C[cx][cy][0][k] = C[cx][cy][1][k]; C[cx][cy][1][k] = 0.0f; }
+        for (int cz = 0; cz < 2; ++cz) {
+          const long long n = node_index(M, ex + cx, ey + cy, ez + cz);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) nlds[((((cx * 2 + cy) * 2 + cz) * 3) + k) * 64 + lane] = v[3 * n + k];
+        }
+  };
+  auto plane_setup = [&]() {
+    const int z = ez * h + tz;
+    Z1 = tz * inv_h;
+    Z0 = 1.0f - Z1;
+    Wz = axis_weight(z, M.Nvz - 1);
+    mz = axis_mult(z, h, M.Nvz - 1);
+    zw0 = Z0; zw1 = Z1;
+    if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
+    rowbase = (long long)z * pl + (long long)y0 * M.pitch + xbc;
+    tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
+    if (MODE == MODE_MATVEC) {
+      // u at the element's left / right node columns, linear in y
+      auto nv = [&](int cx, int cy, int cz, int k) {
+        return nlds[((((cx * 2 + cy) * 2 + cz) * 3) + k) * 64 + lane];
+      };
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float l0 = Z0 * nv(0, 0, 0, k) + Z1 * nv(0, 0, 1, k);
+        const float l1 = Z0 * nv(0, 1, 0, k) + Z1 * nv(0, 1, 1, k);
+        const float r0 = Z0 * nv(1, 0, 0, k) + Z1 * nv(1, 0, 1, k);
+        const float r1 = Z0 * nv(1, 1, 0, k) + Z1 * nv(1, 1, 1, k);
+        uL0[k] = l0; duL[k] = (l1 - l0) * inv_h;
+        uR0[k] = r0; duR[k] = (r1 - r0) * inv_h;
+      }
+    }
+  };
+  // advance to the next row; closes element layers (emit + shift) and sets
+  // up new planes
+  auto advance = [&]() {
+    if (++ty < nrows) return;
+    ty = 0;
+    if (++tz == nzp) {
+      emit(ez - ez0);
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { C[cx][cy][0][k] = C[cx][cy][1][k]; C[cx][cy][1][k] = 0.0f; }
+      ++ez;
+      tz = 0;
+      nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+      if (ez >= ez1) return;
+      element_setup();
+    }
+    plane_setup();
+  };
+  auto process = [&](const RowBuf& cur) {
+    const long long vi = rowbase + (long long)ty * M.pitch;
+    const Raw<GT, V> c0 = cur.g[0], c1 = cur.g[1], c2 = cur.g[2];
+    Raw<uint8_t, V> cmk;
+    if (HASMASK) cmk = cur.m;
+    const uint32_t ctl[4] = {cur.t[0], cur.t[1], cur.t[2], cur.t[3]};
+    const int y = y0 + ty;
+    const float fty = (float)(ty0 + ty);  // row within the element
+    const float Wr = Wz * axis_weight(y, M.Nvy - 1);
+    float gx[V], gy[V], gz[V];
+    unpack<GT, V>(c0, gx);
+    unpack<GT, V>(c1, gy);
+    unpack<GT, V>(c2, gz);
+    uint32_t mk[V];
+#pragma unroll
+    for (int j2 = 0; j2 < V; ++j2) mk[j2] = HASMASK ? mask_bit<V>(cmk, j2) : 1u;
+    float S[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
+    float ux0 = 0, uy0 = 0, uz0 = 0, dxx = 0, dxy = 0, dxz = 0;
+    if (MODE == MODE_MATVEC) {
+      ux0 = uL0[0] + fty * duL[0];
+      uy0 = uL0[1] + fty * duL[1];
+      uz0 = uL0[2] + fty * duL[2];
+      dxx = (uR0[0] + fty * duR[0]) - ux0;
+      dxy = (uR0[1] + fty * duR[1]) - uy0;
+      dxz = (uR0[2] + fty * duR[2]) - uz0;
+    }
+    float resv[V];
+    if (MODE == MODE_RHS) {
+      float fv[V], gv[V];
+      uint32_t lv[V];
+      ld_any<FT, V>(fim, vi, fv);
+      ld_f32<V>(in.gvals, vi, gv);
+      ld_u8<V>(in.live, vi, lv);
+      const int mxy = axis_mult(y, h, M.Nvy - 1) * mz;
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
+        // the reference's order, no contraction; zero outside b_mask
+        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], in.gmean), in.kscale),
+                             __fsub_rn(fv[j], in.mean0));
+        rv = mk[j] ? rv : 0.0f;
+        resv[j] = rv;
+        if (act && lv[j]) {
+          const int mxv = (tb + j == 0 && xb + j != 0) ? 2 : 1;
+          const double mlt = (double)(mxv * mxy);
+          st_c += mlt;
+          st_1 += mlt * (double)rv;
+          st_2 += mlt * (double)rv * (double)rv;
+          st_n += 1.0;
+        }
+      }
+      if (act) {
+        st_f32<V>(in.res, vi, resv);
+        if (FOLD) {
+          float fx[V], fy[V], fz[V];
+#pragma unroll
+          for (int j = 0; j < V; ++j) {
+            fx[j] = mk[j] ? gx[j] : 0.0f;
+            fy[j] = mk[j] ? gy[j] : 0.0f;
+            fz[j] = mk[j] ? gz[j] : 0.0f;
+          }
+          st_bf16<V>(in.fold_out[0], vi, fx);
+          st_bf16<V>(in.fold_out[1], vi, fy);
+          st_bf16<V>(in.fold_out[2], vi, fz);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      // when the host launched with h == V every block starts an element
+      // (tb == 0) and X1 = j / V is a constant
+      const float X1 = (h == V) ? (float)j * (1.0f / (float)V) : X1b + (float)j * inv_h;
+      float wm = (j == 0) ? wx0 : 1.0f;
+      if (HASMASK) wm = mk[j] ? wm : 0.0f;
+      float sd = 0.0f;
+      if (MODE == MODE_MATVEC) {
+        const float ux = ux0 + X1 * dxx, uy = uy0 + X1 * dxy, uz = uz0 + X1 * dxz;
+        sd = gx[j] * ux + gy[j] * uy + gz[j] * uz;
+      } else if (MODE == MODE_RHS) {
+        sd = resv[j];
+      }
+      acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S, Tt);
+    }
+    if (tail) {
+      // last voxel plane x = Nx*h: t = h of this element, X1 = 1, W_x = 1/2
+      const long long ti = tailbase + (long long)ty * M.pitch;
+      float tgx, tgy, tgz;
+      uint32_t tm = 1;
+      if (MODE == MODE_RHS) {
+        tgx = ld_val(g0, ti); tgy = ld_val(g1, ti); tgz = ld_val(g2, ti);
+        if (HASMASK) tm = in.mask[ti];
+      } else {
+        tgx = unpack1<GT>(ctl[0]); tgy = unpack1<GT>(ctl[1]); tgz = unpack1<GT>(ctl[2]);
+        tm = ctl[3];
+      }
+      float wm = tm ? 0.5f : 0.0f;
+      float sd = 0.0f;
+      if (MODE == MODE_MATVEC) {
+        sd = tgx * (ux0 + dxx) + tgy * (uy0 + dxy) + tgz * (uz0 + dxz);
+      } else if (MODE == MODE_RHS) {
+        const float fv = ld_val(fim, ti);
+        const float gv = in.gvals[ti];
+        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
+                             __fsub_rn(fv, in.mean0));
+        rv = tm ? rv : 0.0f;
+        in.res[ti] = rv;
+        if (in.live[ti]) {
+          const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
+          st_c += mlt;
+          st_1 += mlt * (double)rv;
+          st_2 += mlt * (double)rv * (double)rv;
+          st_n += 1.0;
+        }
+        if (FOLD) {
+          in.fold_out[0][ti] = (uint16_t)bf16_bits(tm ? tgx : 0.0f);
+          in.fold_out[1][ti] = (uint16_t)bf16_bits(tm ? tgy : 0.0f);
+          in.fold_out[2][ti] = (uint16_t)bf16_bits(tm ? tgz : 0.0f);
+        }
+        sd = rv;
+      }
+      acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S, Tt);
+    }
+    if (!act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { S[k] = 0.0f; Tt[k] = 0.0f; }
+    }
+    // x-restricted row values R[cx][k], weighted into the node planes
+    float R[2][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (MODE == MODE_DIAG) { R[0][k] = S[k]; R[1][k] = Tt[k]; }
+      else { R[0][k] = S[k] - Tt[k]; R[1][k] = Tt[k]; }
+    }
+    const float Y1 = fty * inv_h;
+    float wy0 = Wr * (1.0f - Y1), wy1 = Wr * Y1;
+    if (MODE == MODE_DIAG) { wy0 = Wr * (1.0f - Y1) * (1.0f - Y1); wy1 = Wr * Y1 * Y1; }
+    const float w00 = wy0 * zw0, w01 = wy0 * zw1, w10 = wy1 * zw0, w11 = wy1 * zw1;
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        C[cx][0][0][k] += w00 * R[cx][k];
+        C[cx][0][1][k] += w01 * R[cx][k];
+        C[cx][1][0][k] += w10 * R[cx][k];
+        C[cx][1][1][k] += w11 * R[cx][k];
+      }
+  };
+
+  element_setup();
+  plane_setup();
+  const int nrows_tile = NP * nrows;
+  // two rows per iteration with statically assigned buffer sets
+  for (int rr = 0; rr < nrows_tile; rr += 2) {
+    {
+      const RowBuf cur = bufA;
+      issue(bufA);
+      process(cur);
+      advance();
+    }
+    if (rr + 1 < nrows_tile) {
+      const RowBuf cur = bufB;
+      issue(bufB);
+      process(cur);
+      advance();
+    }
   }
   emit(ez1 - ez0);
   if (MODE == MODE_RHS) {
@@ -545,7 +652,17 @@ namespace vdvc {
 // of two <= 64 (the generic kernel is used)
 int vec_width(int h, int gtype) {
   if (h < 1 || h > 64 || (h & (h - 1)) != 0) return 0;
-  const int vmax = gtype == GT_F32 ? 4 : 8;
+  int vmax = gtype == GT_F32 ? 4 : 8;
+  // tuning override of the widest lane width (a power of two, at most the
+  // 16-byte width): VDVC_VEC_F32 / VDVC_VEC_BF16
+  static int ov[2] = {-1, -1};
+  const int slot = gtype == GT_F32 ? 0 : 1;
+  if (ov[slot] < 0) {
+    const char* e = getenv(slot == 0 ? "VDVC_VEC_F32" : "VDVC_VEC_BF16");
+    const int v = e ? atoi(e) : 0;
+    ov[slot] = (v == 1 || v == 2 || v == 4 || v == 8) && v <= vmax ? v : 0;
+  }
+  if (ov[slot] > 0) vmax = ov[slot];
   return h < vmax ? h : vmax;
 }
 

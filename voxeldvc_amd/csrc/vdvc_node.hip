@@ -87,7 +87,8 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
     for (int cy = 1; cy >= 0; --cy) {
       const int ey = j - cy;
       if (ey < 0 || ey >= M.Ny) continue;
-      for (int sx = xa; sx <= xb; ++sx) {
+      for (int sxy = xa * T.nys; sxy < (xb + 1) * T.nys; ++sxy) {
+        const int sx = sxy / T.nys, ys = sxy % T.nys;
         const int x0 = sx * T.sw;
         int i_lo, nn;
         if (T.vec) {
@@ -100,7 +101,8 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
         }
         const int ln = i - i_lo;
         if (ln < 0 || ln >= nn) continue;
-        const long long w = (long long)sx + (long long)T.nsx * (ey + (long long)M.Ny * zt);
+        const long long w =
+            (long long)sx + (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
         const float* p = P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln) * 3;
         s[0] += p[0]; s[1] += p[1]; s[2] += p[2];
       }
