@@ -31,7 +31,29 @@ def parse():
     p.add_argument('--size', type=int, default=513)
     p.add_argument('--h', type=int, default=8)
     p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--stream-probe', action='store_true',
+                   help='also run one axpby over 2^28 floats (2 GiB read, 1 GiB written): a '
+                        'streaming kernel of known size that scales HBM counter readings')
+    p.add_argument('--only', choices=('bf16', 'f32'), default=None,
+                   help='time one storage form only (for profiling runs)')
     return p.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hessian_matvec_traffic.json')
+
+
+def measured_traffic(Nv, h):
+    """HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement of the same workload (None if not measured)."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for row in t.get('rows', []):
+        if row.get('Nv') == Nv and row.get('h') == h and row.get('storage') == 'bf16-folded':
+            return row.get('hbm_bytes')
+    return None
 
 
 def cpu_baseline():
@@ -107,39 +129,49 @@ def main():
         L.call('vdvc_timing_end', c, tot, nl)
         return wall / steps * 1e3, tot[0] / max(1, nl[0])
 
-    print('timing bf16-folded matvec', file=sys.stderr, flush=True)
-    ms_b, kms_b = run(Gw, L.BF16, None, a.steps, a.warmup)
-    # float32 + uint8 mask form
-    del Gw
-    G32 = [L.empty(Np) for _ in range(3)]
-    L.call('vdvc_image_setup', c, n_el, n_el, n_el, h, L.ptr(f16), L.F16, 0.0,
-           *[L.ptr(q) for q in G32], L.F32, L.ptr(mask0))
-    print('timing fp32+mask matvec', file=sys.stderr, flush=True)
-    ms_f, kms_f = run(G32, L.F32, mask0, a.steps, a.warmup)
+    if a.stream_probe:
+        n = 1 << 28
+        xs, ys = L.empty(n), L.zeros(n)
+        L.call('vdvc_axpby', c, n, 0.5, L.ptr(xs), 1.0, L.ptr(ys))
+        torch.cuda.synchronize()
+        del xs, ys
 
     B_bf16 = Nvox * 6 + 8 * Ndof
     B_f32 = Nvox * 13 + 8 * Ndof
-    ach = B_bf16 / (kms_b * 1e-3) / 1e9
     res = {
         "metric": "Hessian matvec Gvox/s (% HBM peak)",
-        "value": round(Nvox / (ms_b * 1e-3) / 1e9, 3),
         "unit": "Gvox/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms_b, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "H matvec, %d^3 voxels, h=%d (%d^3 Hex8 elements), bf16 gradients "
                                "with mask folded (6 B/voxel), fp16 speckle 0..255" % (Nv, h, n_el),
                    "Nvox": Nvox, "Ndof": Ndof, "storage": "bf16-folded"},
-        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "vox_restrict_kernel<MATVEC,bf16>",
-                     "kernel_ms": round(kms_b, 5), "algorithmic_bytes": B_bf16},
-        "fp32_mask_form": {"value": round(Nvox / (ms_f * 1e-3) / 1e9, 3), "ms_per_step": round(ms_f, 4),
-                           "kernel_ms": round(kms_f, 5), "algorithmic_bytes": B_f32,
-                           "achieved_GBs": round(B_f32 / (kms_f * 1e-3) / 1e9, 1),
-                           "frac": round(B_f32 / (kms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
+    if a.only in (None, 'bf16'):
+        print('timing bf16-folded matvec', file=sys.stderr, flush=True)
+        ms_b, kms_b = run(Gw, L.BF16, None, a.steps, a.warmup)
+        ach = B_bf16 / (kms_b * 1e-3) / 1e9
+        res["value"] = round(Nvox / (ms_b * 1e-3) / 1e9, 3)
+        res["ms_per_step"] = round(ms_b, 4)
+        res["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                           "traffic": measured_traffic(Nv, h),
+                           "kernel": "vox_vec_kernel<MATVEC,bf16,V=8>",
+                           "kernel_ms": round(kms_b, 5), "algorithmic_bytes": B_bf16}
+    del Gw
+    if a.only in (None, 'f32'):
+        G32 = [L.empty(Np) for _ in range(3)]
+        L.call('vdvc_image_setup', c, n_el, n_el, n_el, h, L.ptr(f16), L.F16, 0.0,
+               *[L.ptr(q) for q in G32], L.F32, L.ptr(mask0))
+        print('timing fp32+mask matvec', file=sys.stderr, flush=True)
+        ms_f, kms_f = run(G32, L.F32, mask0, a.steps, a.warmup)
+        res["fp32_mask_form"] = {"value": round(Nvox / (ms_f * 1e-3) / 1e9, 3),
+                                 "ms_per_step": round(ms_f, 4), "kernel_ms": round(kms_f, 5),
+                                 "kernel": "vox_vec_kernel<MATVEC,f32,mask,V=4>",
+                                 "algorithmic_bytes": B_f32,
+                                 "achieved_GBs": round(B_f32 / (kms_f * 1e-3) / 1e9, 1),
+                                 "frac": round(B_f32 / (kms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not a.no_cpu:
         print('cpu baseline', file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

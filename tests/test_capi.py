@@ -37,6 +37,17 @@ def test_abi_version(lib):
     assert lib.vdvc_abi_version() == 1
 
 
+def test_voxel_pitch_matches_host_layout(lib):
+    """The library's row pitch and the host-side layout helper agree, and the
+    pitch keeps every vectorised lane load naturally aligned (a multiple of 16
+    voxels) with less than 16 voxels of padding."""
+    from voxeldvc_amd import _lib
+    for nvx in (1, 2, 7, 8, 9, 17, 33, 65, 129, 257, 513, 1025, 2049):
+        p = lib.vdvc_voxel_pitch(nvx)
+        assert p == _lib.pitch_of(nvx)
+        assert p % 16 == 0 and nvx <= p < nvx + 16
+
+
 def test_argument_errors_without_device(lib):
     from voxeldvc_amd import _lib as L
     rc = lib.vdvc_matvec_H(None, 2, 2, 2, 8, None, None, None, 0, None, None, None)

@@ -31,6 +31,34 @@ def test_matvec_H(gpu, shape):
     assert rel(got2, ref) < 2e-5
 
 
+@pytest.mark.parametrize('nys', ['1', '2'])
+@pytest.mark.parametrize('gt', ['f32', 'bf16'])
+@pytest.mark.parametrize('shape', [(9, 2, 3, 8), (8, 8, 8, 4), (3, 2, 2, 32), (65, 1, 1, 8)])
+def test_matvec_H_tilings(gpu, monkeypatch, shape, gt, nys):
+    """Both row splits of the vectorised kernel (VDVC_NYS tuning override),
+    float32 and bfloat16 gradient planes, with and without a separate mask."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = shape
+    print('matvec_H tiling', shape, gt, nys, flush=True)
+    monkeypatch.setenv('VDVC_NYS', nys)
+    g, m, v = rand_problem(11, Nx, Ny, Nz, h)
+    shp = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    tdt = torch.float32 if gt == 'f32' else torch.bfloat16
+    gq = torch.from_numpy(g).to(tdt)
+    g64 = gq.to(torch.float64).numpy()                 # the values the kernel sees
+    ref = O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz)
+    planes = [L.to_dev_level(gq[:, c].contiguous(), shp,
+                             'float32' if gt == 'f32' else 'bfloat16') for c in range(3)]
+    mk = L.to_dev_level(m, shp, 'uint8')
+    dv = L.to_dev_flat(v)
+    out = L.empty(dv.numel())
+    code = L.F32 if gt == 'f32' else L.BF16
+    L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in planes], code, L.ptr(mk),
+           L.ptr(dv), L.ptr(out))
+    assert rel(L.to_host(out), ref) < 2e-5
+
+
 @pytest.mark.parametrize('shape', SHAPES)
 def test_diag_H(gpu, shape):
     from voxeldvc_amd.engine import kernels_dvc as K

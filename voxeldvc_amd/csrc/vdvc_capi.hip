@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -152,9 +153,15 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   T.px = 64 * V / M.h + 2;
   T.nys = 1;
   plan_z(M, T, 8192);
-  // too few wavefronts for the chip even with one element layer per wave:
-  // split each element row's rows between two wavefronts
-  if (T.nwaves < 8192 && M.h >= 4 && (M.h % 2) == 0) {
+  // too few wavefronts for the chip even with one element layer per wave
+  // (fewer than one per SIMD slot at the kernel's occupancy of 3-4): split
+  // each element row's rows between two wavefronts.  The split doubles the
+  // node-partial traffic, so it is not used beyond that point.
+  // tuning override of the row split, read at every plan: VDVC_NYS = 1 or 2
+  const char* e = getenv("VDVC_NYS");
+  const int ov = e ? atoi(e) : 0;
+  const bool split = ov == 1 ? false : ov == 2 ? true : T.nwaves < 4096;
+  if (split && M.h >= 4 && (M.h % 2) == 0) {
     T.nys = 2;
     plan_z(M, T, 8192);
   }

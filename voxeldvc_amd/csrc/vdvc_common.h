@@ -26,9 +26,12 @@
 #define VDVC_WAVES_PER_BLOCK (VDVC_BLOCK / VDVC_WAVE)
 
 // Voxel volumes of a mesh level are stored x fastest with a padded row
-// pitch (a multiple of 16 voxels, so rows start 16-byte aligned for every
-// element type and a lane can load 4-8 consecutive voxels at once):
-// index x + pitch * (y + Nvy * z).  Padding voxels are never read as data.
+// pitch: index x + pitch * (y + Nvy * z).  The pitch is a multiple of 16
+// voxels, so every row starts 16-byte aligned for every element type and
+// each vectorised lane load (4-8 consecutive voxels) is naturally aligned.
+// (A multiple of 8 would also keep the loads aligned and pad less, but
+// measured on MI355X at 513^3 it streams 0.6-2% slower.)  Padding voxels
+// are never read as data.
 __host__ __device__ inline int vdvc_pitch_of(int nvx) { return (nvx + 15) & ~15; }
 
 struct MeshDev {

@@ -68,9 +68,9 @@ __device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp& op, int i,
 __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev& T,
                                                 const float* __restrict__ P, int i, int j, int k,
                                                 float (&s)[3]) {
-  s[0] = s[1] = s[2] = 0.0f;
   const int h = M.h;
-  const int za = max(0, k / T.ez - 1), zb = min(T.nzt - 1, k / T.ez);
+  const int kz = k / T.ez;
+  const int za = max(0, kz - 1), zb = min(T.nzt - 1, kz);
   int xa, xb;
   if (T.vec) {
     // strips of sw voxels over [0, Nx*h) holding element i-1 or element i
@@ -80,34 +80,72 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
     xa = max(0, (i - 1) * h) / 64;
     xb = min(M.Nvx - 1, (i + 1) * h) / 64;
   }
-  for (int zt = za; zt <= zb; ++zt) {
+  // the node's slot in each of the (at most two) strips sx = xa, xa + 1
+  int ln[2];
+  bool okx[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int sx = xa + q;
+    const int x0 = sx * T.sw;
+    const int i_lo = x0 / h;
+    int nn;
+    if (T.vec) {
+      nn = min(M.Nx, (x0 + T.sw) / h) - i_lo + 1;
+    } else {
+      const int nact = min(64, M.Nvx - x0);
+      nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
+    }
+    ln[q] = i - i_lo;
+    okx[q] = sx <= xb && ln[q] >= 0 && ln[q] < nn;
+  }
+  // At most 2 z tiles x 2 element rows x 2 strips x 2 row parts hold
+  // partials of a node.  All slot addresses are formed first and every load
+  // is issued before the first sum, so their latencies overlap; absent slots
+  // read P[0..2] and are dropped.  Summation order is fixed: z tiles, then
+  // element rows (upper first), then strips and row parts.
+  const float* ptr[16];
+  bool ok[16];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int zt = za + a;
     const int kb = zt * T.ez;
-    if (k < kb || k > min(M.Nz, kb + T.ez)) continue;
+    const bool okz = zt <= zb && k >= kb && k <= min(M.Nz, kb + T.ez);
     const int kzl = k - kb;
-    for (int cy = 1; cy >= 0; --cy) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int cy = 1 - b;
       const int ey = j - cy;
-      if (ey < 0 || ey >= M.Ny) continue;
-      for (int sxy = xa * T.nys; sxy < (xb + 1) * T.nys; ++sxy) {
-        const int sx = sxy / T.nys, ys = sxy % T.nys;
-        const int x0 = sx * T.sw;
-        int i_lo, nn;
-        if (T.vec) {
-          i_lo = x0 / h;
-          nn = min(M.Nx, (x0 + T.sw) / h) - i_lo + 1;
-        } else {
-          const int nact = min(64, M.Nvx - x0);
-          i_lo = x0 / h;
-          nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
-        }
-        const int ln = i - i_lo;
-        if (ln < 0 || ln >= nn) continue;
-        const long long w =
-            (long long)sx + (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
-        const float* p = P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln) * 3;
-        s[0] += p[0]; s[1] += p[1]; s[2] += p[2];
+      const bool oky = okz && ey >= 0 && ey < M.Ny;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = c >> 1, ys = c & 1;  // strip xa + q, row part ys
+        const bool okc = oky && okx[q] && ys < T.nys;
+        const long long w = (long long)(xa + q) +
+                            (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
+        const int e = (a * 2 + b) * 4 + c;
+        ok[e] = okc;
+        ptr[e] = okc ? P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln[q]) * 3
+                     : P;
       }
     }
   }
+  float val[16][3];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
+  }
+  s[0] = s[1] = s[2] = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
+}
+
+// node (i, j, k) of flat node index n (< 2^31: checked when a level is set up)
+__device__ __forceinline__ void node_ijk(const MeshDev& M, unsigned n, int& i, int& j, int& k) {
+  const unsigned r = n / (unsigned)M.nnx;
+  i = (int)(n - r * (unsigned)M.nnx);
+  k = (int)(r / (unsigned)M.nny);
+  j = (int)(r - (unsigned)k * (unsigned)M.nny);
 }
 
 __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp op) {
@@ -116,10 +154,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp o
   const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   double d = 0.0;
   if (n < M.Nnodes) {
-    const int i = (int)(n % M.nnx);
-    const long long r = n / M.nnx;
-    const int j = (int)(r % M.nny);
-    const int k = (int)(r / M.nny);
+    int i, j, k;
+    node_ijk(M, (unsigned)n, i, j, k);
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (op.P) {
       float s[3];
@@ -168,10 +204,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp o
   const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   float mn = INFINITY;
   if (n < M.Nnodes) {
-    const int i = (int)(n % M.nnx);
-    const long long r = n / M.nnx;
-    const int j = (int)(r % M.nny);
-    const int k = (int)(r / M.nny);
+    int i, j, k;
+    node_ijk(M, (unsigned)n, i, j, k);
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (op.P) gather_partials(M, op.T, op.P, i, j, k, acc);
     if (op.k1_on) {
