@@ -132,7 +132,7 @@ def main():
     if a.stream_probe:
         n = 1 << 28
         xs, ys = L.empty(n), L.zeros(n)
-        L.call('vdvc_axpby', c, n, 0.5, L.ptr(xs), 1.0, L.ptr(ys))
+        L.call('vdvc_axpby', c, n, L.F32, 0.5, L.ptr(xs), 1.0, L.ptr(ys))
         torch.cuda.synchronize()
         del xs, ys
 

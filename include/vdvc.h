@@ -18,7 +18,17 @@
  *   gradients:     three planes (gx, gy, gz), each a voxel volume;
  *   nodal vectors: DOF 3*(i + j*(Nx+1) + k*(Nx+1)*(Ny+1)) + c;
  *   deformed image g for sampling: x fastest, dims (gnx, gny, gnz).
- * Element types: VDVC_F32 float32, VDVC_BF16 bfloat16, VDVC_F16 float16.
+ * Element types: VDVC_F32 float32, VDVC_BF16 bfloat16, VDVC_F16 float16,
+ * VDVC_F64 float64.
+ *
+ * Working type.  Nodal vectors (v, U, b, x, diagonals, M_inv), sampled
+ * images, residuals and the node partials are float64 on the FP64 path and
+ * float32 otherwise.  The path is selected by the storage type: gradients
+ * (gtype) or deformed image (gitype) VDVC_F64 select float64 (the
+ * reference's dtype=float64 solve, engine/correlate_gpu.py:910-914), every
+ * other storage type float32; node-only functions take it as `rtype`
+ * (VDVC_F32 or VDVC_F64).  Scalars (E, eps_reg, ll, tol, means) are passed
+ * as double and rounded once to the working type.
  *
  * The reference has no FFI for this path; its boundary is the Python
  * function API of voxeldvc.engine with array-module injection.  Each entry
@@ -33,9 +43,9 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 1
+#define VDVC_ABI_VERSION 2
 
-enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2 };
+enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 enum { VDVC_REG_NONE = 0, VDVC_REG_K = 1, VDVC_REG_EQGAP = 2 };
 
 typedef struct vdvc_ctx vdvc_ctx;
@@ -66,9 +76,10 @@ int vdvc_timing_end(vdvc_ctx* ctx, double* h_total_ms, int* h_launches);
  * gradient zeroed where the 6-cross erosion of mask0 is false.
  * Replaces image_grad_and_values + erode_mask0_vox + the zeroing in
  * correlate_gpu (engine/correlate_gpu.py:49-94, 951-953).
- * d_f: ftype (F32/F16); gradient planes: gtype (F32/BF16); d_mask0: uint8. */
+ * d_f: ftype (F32/F16, or F64 with F64 gradients); gradient planes: gtype
+ * (F32/BF16/F64); d_mask0: uint8. */
 int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
-                     float glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
+                     double glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
                      uint8_t* d_mask0);
 
 /* Multiplicity-weighted masked (count, sum, sum of squares) of a voxel field
@@ -76,29 +87,30 @@ int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d
  * and !use_thresh means all voxels.  Replaces the reductions of
  * masked_quadpoint_mean_std (engine/correlate_gpu.py:113-157). */
 int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_field,
-                      int ftype, const uint8_t* d_mask, int use_thresh, float glt,
+                      int ftype, const uint8_t* d_mask, int use_thresh, double glt,
                       double* h_out3);
 
 /* g(x + u(x)) at every voxel, active mask = in_bounds & mask0 & (g > glt), and
  * the multiplicity-weighted (count, sum, sum sq) of g over d_bmask (or over
  * the new mask when d_bmask is NULL).  Replaces sample_deformed_field
  * (engine/correlate_gpu.py:201-281) fused with the b-mask statistics of
- * correlate_gpu (:1091).  d_U float32 nodal; d_g gitype (F32/F16). */
-int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, const void* d_g,
-                int gitype, int gnx, int gny, int gnz, float gox, float goy, float goz,
-                float glt, const uint8_t* d_mask0, const uint8_t* d_bmask, float* d_gvals,
+ * correlate_gpu (:1091).  d_g gitype (F32/F16/F64); d_U and d_gvals in the
+ * working type of gitype. */
+int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, const void* d_g,
+                int gitype, int gnx, int gny, int gnz, double gox, double goy, double goz,
+                double glt, const uint8_t* d_mask0, const uint8_t* d_bmask, void* d_gvals,
                 uint8_t* d_mask_out, double* h_stats3);
 
 /* Residual res = ((g - gmean) * kscale + (f - mean0)) zeroed outside d_bmask
- * (float32, the reference's operation order), RHS b = sum W bmask res v_eq,
+ * (working type, the reference's operation order), RHS b = sum W bmask res v_eq,
  * and (weighted count, sum, sum sq, plain voxel count) of res over d_live,
  * multiplicity-weighted except the last.  If d_fold_x is non-NULL
  * also writes bfloat16 planes g * bmask.  Replaces correlate_gpu.py:1092-1109
  * + compute_b_gpu (:341-378) + the display statistics (:1133). */
 int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
                       const void* d_gy, const void* d_gz, int gtype, const void* d_f, int ftype,
-                      float mean0, const float* d_gvals, float gmean, float kscale,
-                      const uint8_t* d_bmask, const uint8_t* d_live, float* d_res, float* d_b,
+                      double mean0, const void* d_gvals, double gmean, double kscale,
+                      const uint8_t* d_bmask, const uint8_t* d_live, void* d_res, void* d_b,
                       uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
                       double* h_stats4);
 
@@ -106,35 +118,35 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
  * folded into the gradients).  Replaces matvec_H (engine/kernels_dvc.py:452-506). */
 int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
                   const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
-                  const float* d_v, float* d_out);
+                  const void* d_v, void* d_out);
 
 /* out = diag(H).  Replaces build_diagonal_H (engine/kernels_dvc.py:509-543). */
 int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
-                const void* d_gz, int gtype, const uint8_t* d_mask, float* d_out);
+                const void* d_gz, int gtype, const uint8_t* d_mask, void* d_out);
 
 /* out = sum_e E_e P_e^T K P_e v with a 24x24 element matrix (h_K, float64
- * row-major, applied in float32) and E_e = E or d_E[e].  Replaces
- * matvec_K_ref (engine/kernels_dvc.py:349-374). */
-int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
-                  const float* d_E, const float* d_v, float* d_out);
+ * row-major, applied in the working type rtype) and E_e = E or d_E[e].
+ * Replaces matvec_K_ref (engine/kernels_dvc.py:349-374). */
+int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_K, double E,
+                  const void* d_E, const void* d_v, void* d_out);
 
 /* out = diagonal of the assembled K.  Replaces build_diagonal_K_ref
  * (engine/kernels_dvc.py:377-392). */
-int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
-                const float* d_E, float* d_out);
+int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_K, double E,
+                const void* d_E, void* d_out);
 
 /* out = K (P_i (K v)), P_i the interior-node DOF projector.  Replaces
  * matvec_equilibrium_gap (engine/kernels_dvc.py:402-417). */
-int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
-                      const float* d_E, const float* d_v, float* d_out);
+int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_Kel,
+                      double E, const void* d_E, const void* d_v, void* d_out);
 
 /* Hutchinson estimate diag(R_m) ~ mean_p z_p * (R_m z_p) over n_probes
  * Rademacher probes d_probes (int8, n_probes x Ndof), accumulated in FP64,
  * clipped below at 1e-12 * max|d|.  Replaces build_diagonal_equilibrium_gap
  * (engine/kernels_dvc.py:420-437); the caller supplies the reference's probe
  * stream. */
-int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
-                          const int8_t* d_probes, int n_probes, float* d_out);
+int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_Kel,
+                          double E, const int8_t* d_probes, int n_probes, void* d_out);
 
 /* Regularizer used by vdvc_jacobi / vdvc_pcg:
  *   VDVC_REG_NONE:  R = 0
@@ -148,8 +160,8 @@ int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h
  * 1113-1120. */
 int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
                 const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-                const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-                const float* d_extra, float* d_diag, float* d_minv);
+                const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                const void* d_extra, void* d_diag, void* d_minv);
 
 /* Jacobi-PCG for (H + ll R) x = b from x0 = 0, FP64 dots, stop when
  * sqrt(r.r/(b.b + 1e-30)) < tol, device-resident loop (the host polls the
@@ -158,27 +170,29 @@ int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
  * correlate_gpu.py:1122-1126. */
 int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
              const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-             const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-             const float* d_b, const float* d_minv, float* d_x, float tol, int max_iter,
+             const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+             const void* d_b, const void* d_minv, void* d_x, double tol, int max_iter,
              int check_every, int* h_iters, double* h_relres);
 
 /* out = H v + ll R v (the PCG operator).  Replaces matvec_full
  * (engine/correlate_gpu.py:1122-1126). */
 int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
                  const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-                 const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-                 const float* d_v, float* d_out);
+                 const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                 const void* d_v, void* d_out);
 
-/* FP64 dot product of two float32 vectors.  Replaces xp.dot / linalg.norm
- * (engine/kernels_dvc.py:558-578, engine/correlate_gpu.py:1131). */
-int vdvc_dot(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, double* h_out);
+/* FP64 dot product of two vectors of type rtype.  Replaces xp.dot /
+ * linalg.norm (engine/kernels_dvc.py:558-578, engine/correlate_gpu.py:1131). */
+int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b,
+             double* h_out);
 
-/* y = a*x + b*y (float32). */
-int vdvc_axpby(vdvc_ctx* ctx, int64_t n, float a, const float* d_x, float b, float* d_y);
+/* y = a*x + b*y in rtype. */
+int vdvc_axpby(vdvc_ctx* ctx, int64_t n, int rtype, double a, const void* d_x, double b,
+               void* d_y);
 
-/* out = a * b elementwise (float32); the Jacobi application z = M_inv r of
+/* out = a * b elementwise in rtype; the Jacobi application z = M_inv r of
  * pcg (engine/kernels_dvc.py:556, 577). */
-int vdvc_mul(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, float* d_out);
+int vdvc_mul(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b, void* d_out);
 
 /* Folded bfloat16 gradient planes o = g * mask (raw bf16 bits), the storage
  * the Hessian kernels read when d_mask is NULL.  No reference counterpart:

@@ -34,7 +34,8 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.vdvc_abi_version() == 1
+    from voxeldvc_amd import _lib
+    assert lib.vdvc_abi_version() == _lib.ABI_VERSION == 2
 
 
 def test_voxel_pitch_matches_host_layout(lib):

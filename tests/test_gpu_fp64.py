@@ -1,0 +1,180 @@
+"""GPU parity of the FP64 path (dtype=float64, the reference's oracle mode):
+every operator against the float64 oracle to near machine precision, and the
+end-to-end solve against the reference's own float64 output."""
+import numpy as np
+import pytest
+
+from conftest import golden, rand_problem
+from oracle import dvc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# pow2 and non-pow2 element sizes, single and multi-strip rows
+SHAPES = [(2, 2, 2, 8), (9, 2, 3, 8), (5, 3, 2, 3), (3, 2, 2, 32), (70, 1, 2, 1), (4, 3, 3, 4)]
+TOL = 1e-12
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(1e-300, np.abs(b).max()))
+
+
+def f64_problem(seed, Nx, Ny, Nz, h):
+    g, m, v = rand_problem(seed, Nx, Ny, Nz, h)
+    rng = np.random.default_rng(seed + 100)
+    # full float64 mantissas (the float32 problem would hide single rounding)
+    g = g.astype(np.float64) + rng.normal(scale=1e-3, size=g.shape)
+    v = v.astype(np.float64) + rng.normal(scale=1e-3, size=v.shape)
+    return g, m, v
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+def test_matvec_and_diag_H_f64(gpu, shape):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz, h = shape
+    print('f64 matvec/diag', shape, flush=True)
+    g, m, v = f64_problem(3, Nx, Ny, Nz, h)
+    got = K.matvec_H(v, g, m, None, None, h, Nx, Ny, Nz)
+    assert got.dtype == np.float64
+    assert rel(got, O.matvec_H(v, g, m, h, Nx, Ny, Nz)) < TOL
+    d = K.build_diagonal_H(g, m, None, None, h, Nx, Ny, Nz, v.size)
+    assert d.dtype == np.float64
+    assert rel(d, O.build_diagonal_H(g, m, h, Nx, Ny, Nz)) < TOL
+
+
+@pytest.mark.parametrize('shape', SHAPES[:4])
+def test_compute_b_f64(gpu, shape):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    Nx, Ny, Nz, h = shape
+    print('f64 compute_b', shape, flush=True)
+    g, m, _ = f64_problem(4, Nx, Ny, Nz, h)
+    res = np.random.default_rng(5).normal(size=m.size)
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    got = C.compute_b_gpu(res, m, g, None, None, h, Nx, Ny, Nz, Ndof)
+    assert got.dtype == np.float64
+    assert rel(got, O.compute_b(res, m, g, h, Nx, Ny, Nz)) < TOL
+
+
+def test_regularizers_f64(gpu):
+    from voxeldvc_amd.engine import kernels_dvc as K
+    Nx, Ny, Nz = 4, 3, 5
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    v = np.random.default_rng(6).normal(size=Ndof)
+    Klap, Kel = O.build_K_ref_laplacian(), O.build_K_ref(0.3)
+    im = O.build_interior_dof_mask(Nx, Ny, Nz)
+    assert rel(K.matvec_K_ref(v, Klap, 8.0, Nx, Ny, Nz),
+               O.matvec_K_ref(v, Klap, 8.0, Nx, Ny, Nz)) < TOL
+    assert rel(K.matvec_K_ref(v, Kel, 8.0, Nx, Ny, Nz),
+               O.matvec_K_ref(v, Kel, 8.0, Nx, Ny, Nz)) < TOL
+    assert rel(K.build_diagonal_K_ref(Kel, np.float64(8.0), Nx, Ny, Nz, Ndof),
+               O.build_diagonal_K_ref(Kel, 8.0, Nx, Ny, Nz)) < TOL
+    assert rel(K.matvec_equilibrium_gap(v, Kel, im, 8.0, Nx, Ny, Nz),
+               O.matvec_equilibrium_gap(v, Kel, im, 8.0, Nx, Ny, Nz)) < TOL
+    # the Hutchinson estimate over the reference's probe stream, FP64 path
+    got = K.build_diagonal_equilibrium_gap(Kel, im, np.float64(8.0), Nx, Ny, Nz, Ndof)
+    assert got.dtype == np.float64
+    assert rel(got, O.build_diagonal_equilibrium_gap(Kel, im, 8.0, Nx, Ny, Nz)) < TOL
+
+
+@pytest.mark.parametrize('go', [(0, 0, 0), (2, -1, 1)])
+def test_sample_f64(gpu, go):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    Nx, Ny, Nz, h = 3, 2, 2, 4
+    print('f64 sample', go, flush=True)
+    rng = np.random.default_rng(7)
+    gimg = rng.uniform(0, 100, size=(Nx * h + 3, Ny * h + 2, Nz * h + 4))
+    U = rng.normal(scale=1.5, size=3 * (Nx + 1) * (Ny + 1) * (Nz + 1))
+    m0 = rng.random((Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)) < 0.9
+    gv, mk = C.sample_deformed_field(U, gimg, None, h, Nx, Ny, Nz, g_origin=go, mask0_vox=m0,
+                                     glt=20.0)
+    rv, rm = O.sample_deformed_field(U, gimg, h, Nx, Ny, Nz, g_origin=go, mask0_vox=m0, glt=20.0)
+    assert gv.dtype == np.float64
+    assert np.array_equal(mk, rm)
+    assert np.abs(gv - rv).max() < 1e-11
+
+
+def test_image_setup_and_stats_f64(gpu):
+    from voxeldvc_amd.engine import correlate_gpu as C
+    rng = np.random.default_rng(8)
+    f = rng.normal(size=(17, 9, 13))
+    fv, gr = C.image_grad_and_values(f, dtype=np.float64)
+    rf, rg = O.image_grad_and_values(f, np.float64)
+    assert gr.dtype == np.float64
+    assert np.array_equal(fv, rf) and np.abs(gr - rg).max() == 0.0
+    Nx, Ny, Nz, h = 4, 2, 3, 4
+    fld = rng.normal(size=(Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1))
+    mk = rng.random(fld.size) < 0.7
+    mu, sd = C.masked_quadpoint_mean_std(fld, mk, h, Nx, Ny, Nz)
+    rmu, rsd = O.masked_quadpoint_mean_std(fld, mk, h, Nx, Ny, Nz)
+    assert abs(mu - rmu) < 1e-13 and abs(sd - rsd) < 1e-13
+
+
+def test_pcg_f64(gpu):
+    """Device PCG in FP64 against the oracle's PCG on the same operator."""
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    Nx, Ny, Nz, h = 3, 3, 2, 4
+    g, m, v = f64_problem(9, Nx, Ny, Nz, h)
+    Klap = O.build_K_ref_laplacian()
+    ll = 0.37
+    A = lambda x: O.matvec_H(x, g, m, h, Nx, Ny, Nz) + ll * O.matvec_K_ref(x, Klap, h, Nx, Ny, Nz)
+    d = O.build_diagonal_H(g, m, h, Nx, Ny, Nz) + ll * O.build_diagonal_K_ref(Klap, h, Nx, Ny, Nz)
+    Minv = 1.0 / (d + 1e-5 * d.min())
+    xr, itr = O.pcg(A, v, Minv, max_iter=500, tol=1e-10)
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = Kd.grad_planes(g, shp, 'float64')
+    md = Kd._mask_u8(m, shp)
+    Kp = np.ascontiguousarray(Klap)
+    diag = L.empty(v.size, 'float64')
+    minv = L.empty(v.size, 'float64')
+    L.call('vdvc_jacobi', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F64,
+           L.ptr(md), L.REG_K, Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, None,
+           L.ptr(diag), L.ptr(minv))
+    assert rel(L.to_host(minv), Minv) < TOL
+    bd = L.to_dev_flat(v, 'float64')
+    x = L.empty(v.size, 'float64')
+    it = (L._i * 1)()
+    rr = (L._d * 1)()
+    L.call('vdvc_pcg', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F64, L.ptr(md),
+           L.REG_K, Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, L.ptr(bd), L.ptr(minv),
+           L.ptr(x), 1e-10, 500, 4, it, rr)
+    print('f64 pcg iters gpu', it[0], 'oracle', itr, flush=True)
+    assert rr[0] < 1e-10
+    assert abs(it[0] - itr) <= 1
+    assert rel(L.to_host(x), xr) < 1e-9
+    # the host-driven pcg wrapper with an FP64 matvec closure
+    Ad = lambda p: Kd.matvec_H(p, g, m, None, None, h, Nx, Ny, Nz) + ll * Kd.matvec_K_ref(
+        p, Klap, float(h), Nx, Ny, Nz)
+    xw = Kd.pcg(lambda p: L.to_dev_flat(Ad(L.to_host(p)), 'float64'), v, Minv, max_iter=500,
+                tol=1e-10)
+    assert xw.dtype == np.float64
+    assert rel(xw, xr) < 1e-9
+
+
+def test_correlate_golden_17_f64(gpu):
+    """End-to-end single-level float64 solve against the reference's own
+    float64 output on its test problem."""
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('corr17_f64')
+    K = O.build_K_ref_laplacian()
+    U, res, dU = C.correlate_gpu(z['f'], z['g'], 2, 2, 2, 8, np, K_ref_laplacian=K, l0=36.0,
+                                 maxiter=20, eps=1e-3, disp=False, dtype=np.float64)
+    assert U.dtype == np.float64
+    print('f64 corr17 max|dU|', np.abs(U - z['U']).max(), flush=True)
+    assert np.abs(U - z['U']).max() < 1e-9
+    assert np.abs(res - z['res']).max() < 1e-8
+    assert np.abs(dU - z['dU']).max() < 1e-9
+
+
+def test_multiscale_f64_matches_oracle(gpu):
+    """The FP64 multiscale solve (FFT prealign, 3 levels) against the
+    oracle's float64 multiscale solve on the reference test problem."""
+    from voxeldvc_amd.engine import correlate_gpu as C
+    z = golden('multi17_fft')
+    K = O.build_K_ref_laplacian()
+    kw = dict(K_ref_laplacian=K, l0=36.0, maxiter=30, eps=1e-3)
+    U, res, dU = C.multiscale_correlate_gpu(z['f'], z['g'], 2, 2, 2, 8, np, scales=(2, 1, 0),
+                                            dtype=np.float64, disp=False, **kw)
+    Ur, _, _ = O.multiscale_correlate(z['f'], z['g'], 2, 2, 2, 8, scales=(2, 1, 0),
+                                      dtype=np.float64, **kw)
+    assert U.dtype == np.float64
+    assert np.abs(U - Ur).max() < 1e-9

@@ -25,6 +25,7 @@ def test_matvec_H(gpu, shape):
     g, m, v = rand_problem(1, Nx, Ny, Nz, h)
     ref = O.matvec_H(v.astype(np.float64), g.astype(np.float64), m, h, Nx, Ny, Nz)
     got = K.matvec_H(v, g, m, None, None, h, Nx, Ny, Nz)
+    assert got.dtype == np.float32          # float32 inputs take the float32 path
     assert rel(got, ref) < 2e-5
     # mask folded into the gradient: same operator
     got2 = K.matvec_H(v, g * m[:, None], None, None, None, h, Nx, Ny, Nz)
@@ -79,6 +80,7 @@ def test_compute_b(gpu, shape):
     res = np.random.default_rng(4).normal(size=m.size).astype(np.float32)
     ref = O.compute_b(res.astype(np.float64), m, g.astype(np.float64), h, Nx, Ny, Nz)
     got = C.compute_b_gpu(res, m, g, None, None, h, Nx, Ny, Nz, ref.size)
+    assert got.dtype == np.float32
     assert rel(got, ref) < 2e-5
 
 

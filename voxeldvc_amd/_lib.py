@@ -13,7 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
-F32, BF16, F16 = 0, 1, 2
+F32, BF16, F16, F64 = 0, 1, 2, 3
+ABI_VERSION = 2
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -34,27 +35,27 @@ SIGNATURES = {
     'vdvc_synchronize': [_vp],
     'vdvc_timing_begin': [_vp],
     'vdvc_timing_end': [_vp, _dp, _ip],
-    'vdvc_image_setup': [_vp, _i, _i, _i, _i, _vp, _i, _f, _vp, _vp, _vp, _i, _vp],
-    'vdvc_masked_stats': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _f, _dp],
-    'vdvc_sample': [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _vp, _vp,
+    'vdvc_image_setup': [_vp, _i, _i, _i, _i, _vp, _i, _d, _vp, _vp, _vp, _i, _vp],
+    'vdvc_masked_stats': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _d, _dp],
+    'vdvc_sample': [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _d, _d, _d, _d, _vp, _vp,
                     _vp, _vp, _dp],
-    'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _f, _vp, _f, _f, _vp,
+    'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d, _d, _vp,
                           _vp, _vp, _vp, _vp, _vp, _vp, _dp],
     'vdvc_matvec_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     'vdvc_diag_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
-    'vdvc_matvec_K': [_vp, _i, _i, _i, _dp, _f, _vp, _vp, _vp],
-    'vdvc_diag_K': [_vp, _i, _i, _i, _dp, _f, _vp, _vp],
-    'vdvc_matvec_eqgap': [_vp, _i, _i, _i, _dp, _f, _vp, _vp, _vp],
-    'vdvc_hutchinson_eqgap': [_vp, _i, _i, _i, _dp, _f, _vp, _i, _vp],
-    'vdvc_jacobi': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp,
+    'vdvc_matvec_K': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _vp, _vp],
+    'vdvc_diag_K': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _vp],
+    'vdvc_matvec_eqgap': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _vp, _vp],
+    'vdvc_hutchinson_eqgap': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _i, _vp],
+    'vdvc_jacobi': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _d, _d, _d, _vp,
                     _vp, _vp],
-    'vdvc_pcg': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp, _vp,
-                 _vp, _f, _i, _i, _ip, _dp],
-    'vdvc_apply_A': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _f, _f, _f, _vp,
+    'vdvc_pcg': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _d, _d, _d, _vp, _vp,
+                 _vp, _d, _i, _i, _ip, _dp],
+    'vdvc_apply_A': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _dp, _dp, _d, _d, _d, _vp,
                      _vp],
-    'vdvc_dot': [_vp, _i64, _vp, _vp, _dp],
-    'vdvc_axpby': [_vp, _i64, _f, _vp, _f, _vp],
-    'vdvc_mul': [_vp, _i64, _vp, _vp, _vp],
+    'vdvc_dot': [_vp, _i64, _i, _vp, _vp, _dp],
+    'vdvc_axpby': [_vp, _i64, _i, _d, _vp, _d, _vp],
+    'vdvc_mul': [_vp, _i64, _i, _vp, _vp, _vp],
     'vdvc_fold_grad': [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'vdvc_erode6': [_vp, _i, _i, _i, _vp, _vp],
     'vdvc_stride': [_vp, _vp, _i, _i, _i, _i, _i, _vp],
@@ -85,6 +86,9 @@ def load():
             fn.restype = ctypes.c_int
         lib.vdvc_last_error.argtypes = []
         lib.vdvc_last_error.restype = ctypes.c_char_p
+        if lib.vdvc_abi_version() != ABI_VERSION:
+            raise LibraryMissing('%s has ABI version %d, this binding expects %d: rebuild it'
+                                 % (LIB_PATH, lib.vdvc_abi_version(), ABI_VERSION))
         _lib = lib
         return lib
 
@@ -194,13 +198,16 @@ def to_dev_volume(a, dtype='float32'):
     if arr.ndim != 3:
         raise ValueError('volume must be 3-D')
     shp = arr.shape
-    if dtype == 'float32':
-        host = np.ravel(arr.astype(np.float32, copy=False), order='F')
-    elif dtype == 'float16':
-        host = np.ravel(arr.astype(np.float16, copy=False), order='F')
+    if dtype in ('float32', 'float16', 'float64'):
+        host = np.ravel(arr.astype(getattr(np, dtype), copy=False), order='F')
     else:
         host = np.ravel(arr, order='F')
-    return t.from_numpy(np.ascontiguousarray(host)).to('cuda'), shp
+    return t.from_numpy(np.ascontiguousarray(host)).to(device='cuda', dtype=tt), shp
+
+
+def rtype_of(x):
+    """Working-type code of a device vector: F64 for float64, else F32."""
+    return F64 if x.dtype == torch().float64 else F32
 
 
 def to_host(x):

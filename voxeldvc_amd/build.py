@@ -1,6 +1,7 @@
 """Build libvdvc.so (the gfx950 kernels + C ABI) in-tree with hipcc.
 
-    python -m voxeldvc_amd.build
+    python -m voxeldvc_amd.build            # the library
+    python -m voxeldvc_amd.build --probe    # also the HBM streaming probe
 """
 import os
 import subprocess
@@ -27,6 +28,20 @@ def build(verbose=True):
     return LIB
 
 
+def build_probe(verbose=True):
+    """The stand-alone HBM streaming-bandwidth probe (probe_hbm_stream)."""
+    out = os.path.join(HERE, 'probe_hbm_stream')
+    cmd = [HIPCC, '--offload-arch=gfx950', '-O3', '-std=c++17',
+           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+           '-o', out, os.path.join(CSRC, 'probe_hbm_stream.hip')]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 if __name__ == '__main__':
     build()
+    if '--probe' in sys.argv[1:]:
+        build_probe()
     sys.exit(0)

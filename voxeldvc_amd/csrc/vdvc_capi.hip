@@ -77,11 +77,11 @@ struct vdvc_ctx {
   Buf dparts, dparts2;      // FP64 dot partials
   Buf pstate;               // PcgState
   Buf fmin;                 // per-block minima
-  Buf K1, K2;               // element matrices (float32 24x24)
+  Buf K1[2], K2[2];         // element matrices (24x24) in float / double
   Buf hacc;                 // Hutchinson FP64 accumulator
   Buf dft;                  // DFT work volumes
   double* hpin = nullptr;   // pinned host staging (64 doubles)
-  std::vector<double> k1_cache, k2_cache;
+  std::vector<double> k1_cache[2], k2_cache[2];
   // optional per-launch timing of the voxel-streaming kernels (HIP events
   // recorded on the context stream around each launch)
   bool timing = false;
@@ -168,9 +168,15 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   return T;
 }
 
-int gtype_ok(int t) { return t == VDVC_F32 || t == VDVC_BF16; }
-int ftype_ok(int t) { return t == VDVC_F32 || t == VDVC_F16; }
+int gtype_ok(int t) { return t == VDVC_F32 || t == VDVC_BF16 || t == VDVC_F64; }
+int ftype_ok(int t) { return t == VDVC_F32 || t == VDVC_F16 || t == VDVC_F64; }
+int rtype_ok(int t) { return t == VDVC_F32 || t == VDVC_F64; }
 
+// index of the working type in per-type caches
+template <typename R> constexpr int rix() { return sizeof(R) == 8 ? 1 : 0; }
+
+// element matrix in the working type R, uploaded once per distinct matrix
+template <typename R>
 int upload_K(vdvc_ctx* ctx, Buf& dst, std::vector<double>& cache, const double* hK,
              int* blockdiag) {
   if (!hK) return fail(-1, "element matrix is NULL");
@@ -186,14 +192,23 @@ int upload_K(vdvc_ctx* ctx, Buf& dst, std::vector<double>& cache, const double* 
         }
   *blockdiag = bd ? 1 : 0;
   if (same && dst.p) return 0;
-  int rc = dst.ensure(576 * sizeof(float));
+  int rc = dst.ensure(576 * sizeof(R));
   if (rc) return rc;
-  float tmp[576];
-  for (int q = 0; q < 576; ++q) tmp[q] = (float)hK[q];
+  R tmp[576];
+  for (int q = 0; q < 576; ++q) tmp[q] = (R)hK[q];
   VDVC_HIP(hipMemcpyAsync(dst.p, tmp, sizeof(tmp), hipMemcpyHostToDevice, ctx->stream));
   VDVC_HIP(hipStreamSynchronize(ctx->stream));
   cache.assign(hK, hK + 576);
   return 0;
+}
+
+template <typename R>
+int upload_K1(vdvc_ctx* ctx, const double* hK, int* bd) {
+  return upload_K<R>(ctx, ctx->K1[rix<R>()], ctx->k1_cache[rix<R>()], hK, bd);
+}
+template <typename R>
+int upload_K2(vdvc_ctx* ctx, const double* hK, int* bd) {
+  return upload_K<R>(ctx, ctx->K2[rix<R>()], ctx->k2_cache[rix<R>()], hK, bd);
 }
 
 int fetch(vdvc_ctx* ctx, const void* d, size_t bytes, void* h) {
@@ -224,90 +239,300 @@ int timed_event(vdvc_ctx* ctx) {
 }
 
 int launch_restrict_timed(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, const TilesDev& T,
-                          const VoxIn& in, int ftype, bool fold, const float* v) {
+                          const VoxIn& in, int ftype, bool fold, const void* v) {
   int rc = timed_event(ctx);
   if (rc) return rc;
   if (T.vec)
-    VDVC_LAUNCH(vdvc::launch_restrict_vec(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
-                                          ctx->P.as<float>(), ctx->stream));
+    VDVC_LAUNCH(vdvc::launch_restrict_vec(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in,
+                                          static_cast<const float*>(v), ctx->P.as<float>(),
+                                          ctx->stream));
   else
     VDVC_LAUNCH(vdvc::launch_restrict(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
-                                      ctx->P.as<float>(), ctx->stream));
+                                      ctx->P.p, ctx->stream));
   return timed_event(ctx);
 }
 
+size_t real_size(int gtype) { return gtype == VDVC_F64 ? sizeof(double) : sizeof(float); }
+
 int run_restrict(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, const TilesDev& T,
-                 const VoxIn& in, int ftype, bool fold, const float* v) {
-  int rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * sizeof(float));
+                 const VoxIn& in, int ftype, bool fold, const void* v) {
+  int rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * real_size(gtype));
   if (rc) return rc;
   return launch_restrict_timed(ctx, mode, gtype, M, T, in, ftype, fold, v);
 }
 
-NodeOp node_op0() {
-  NodeOp op;
+template <typename R>
+NodeOp<R> node_op0() {
+  NodeOp<R> op;
   std::memset(&op, 0, sizeof(op));
-  op.cH = 1.0f;
+  op.cH = R(1);
   return op;
 }
 
-KOp kop(const Buf& b, int blockdiag, float E, const float* Earr) {
-  KOp k;
-  k.K = b.as<const float>();
+template <typename R>
+KOp<R> kop(const Buf& b, int blockdiag, double E, const R* Earr) {
+  KOp<R> k;
+  k.K = b.as<const R>();
   k.blockdiag = blockdiag;
-  k.E = E;
+  k.E = (R)E;
   k.Earr = Earr;
   return k;
 }
 
-// out = Hpartials (if P) + ll * R v, with R per reg_kind; eq-gap uses tmp
-// for the inner P_i K v; optional FP64 dot partials of (dotv . out)
+// out = Hpartials (if P) + ll * R v, with R per reg_kind; eq-gap uses a
+// temporary for the inner P_i K v; optional FP64 dot partials of (dotv . out)
+template <typename R>
 int apply_reg_sum(vdvc_ctx* ctx, const MeshDev& M, const TilesDev* T, bool useP, int reg_kind,
-                  int bd1, int bd2, float E, float eps_reg, float ll, const float* v, float* out,
-                  const float* dotv, double* dot_part, const int* done) {
-  NodeOp op = node_op0();
-  if (useP) { op.P = ctx->P.as<const float>(); op.T = *T; }
+                  int bd1, int bd2, double E, double eps_reg, double ll, const R* v, R* out,
+                  const R* dotv, double* dot_part, const int* done) {
+  NodeOp<R> op = node_op0<R>();
+  if (useP) { op.P = ctx->P.as<const R>(); op.T = *T; }
   op.done = done;
-  if (reg_kind == VDVC_REG_K && ll != 0.0f) {
-    op.k1 = kop(ctx->K1, bd1, E, nullptr); op.v1 = v; op.c1 = ll; op.k1_on = 1;
+  const Buf& K1 = ctx->K1[rix<R>()];
+  const Buf& K2 = ctx->K2[rix<R>()];
+  if (reg_kind == VDVC_REG_K && ll != 0.0) {
+    op.k1 = kop<R>(K1, bd1, E, nullptr); op.v1 = v; op.c1 = (R)ll; op.k1_on = 1;
   }
-  if (reg_kind != VDVC_REG_EQGAP || ll == 0.0f) {
+  if (reg_kind != VDVC_REG_EQGAP || ll == 0.0) {
     op.out = out; op.dotv = dotv; op.dot_part = dot_part;
     VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
     return 0;
   }
   // equilibrium gap: out = H v (partials); w = P_i (K1 v); out += ll (K1 w + eps K2 v)
-  int rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(float));
+  int rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(R));
   if (rc) return rc;
-  float* w = ctx->nodeC.as<float>();
+  R* w = ctx->nodeC.as<R>();
   op.out = out;
   VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
-  NodeOp o2 = node_op0();
+  NodeOp<R> o2 = node_op0<R>();
   o2.done = done;
-  o2.k1 = kop(ctx->K1, bd1, E, nullptr); o2.v1 = v; o2.c1 = 1.0f; o2.k1_on = 1;
+  o2.k1 = kop<R>(K1, bd1, E, nullptr); o2.v1 = v; o2.c1 = R(1); o2.k1_on = 1;
   o2.interior = 1;
   o2.out = w;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
-  NodeOp o3 = node_op0();
+  NodeOp<R> o3 = node_op0<R>();
   o3.done = done;
-  o3.addin = out; o3.cadd = 1.0f;
-  o3.k1 = kop(ctx->K1, bd1, E, nullptr); o3.v1 = w; o3.c1 = ll; o3.k1_on = 1;
-  o3.k2 = kop(ctx->K2, bd2, E, nullptr); o3.v2 = v; o3.c2 = ll * eps_reg; o3.k2_on = 1;
+  o3.addin = out; o3.cadd = R(1);
+  o3.k1 = kop<R>(K1, bd1, E, nullptr); o3.v1 = w; o3.c1 = (R)ll; o3.k1_on = 1;
+  o3.k2 = kop<R>(K2, bd2, E, nullptr); o3.v2 = v; o3.c2 = (R)(ll * eps_reg); o3.k2_on = 1;
   o3.out = out; o3.dotv = dotv; o3.dot_part = dot_part;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o3, ctx->stream));
   return 0;
 }
 
+template <typename R>
 int prep_reg(vdvc_ctx* ctx, int reg_kind, const double* hK1, const double* hK2, int* bd1,
              int* bd2) {
   *bd1 = *bd2 = 0;
   if (reg_kind == VDVC_REG_NONE) return 0;
   if (reg_kind != VDVC_REG_K && reg_kind != VDVC_REG_EQGAP)
     return fail(-1, "reg_kind must be VDVC_REG_NONE, VDVC_REG_K or VDVC_REG_EQGAP");
-  int rc = upload_K(ctx, ctx->K1, ctx->k1_cache, hK1, bd1);
+  int rc = upload_K1<R>(ctx, hK1, bd1);
   if (rc) return rc;
-  if (reg_kind == VDVC_REG_EQGAP) return upload_K(ctx, ctx->K2, ctx->k2_cache, hK2, bd2);
+  if (reg_kind == VDVC_REG_EQGAP) return upload_K2<R>(ctx, hK2, bd2);
   return 0;
 }
+
+// ---- working-type implementations of the entry points ----------------------
+
+template <typename R>
+int residual_rhs_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype,
+                      const VoxIn& in, int ftype, bool fold, R* d_b, double* h_stats4) {
+  int rc;
+  if ((rc = run_restrict(ctx, MODE_RHS, gtype, M, T, in, ftype, fold, nullptr))) return rc;
+  NodeOp<R> op = node_op0<R>();
+  op.P = ctx->P.as<const R>(); op.T = T; op.out = d_b;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 4, ctx->red.as<double>(),
+                                  ctx->stream));
+  if (h_stats4) return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
+  return 0;
+}
+
+template <typename R>
+int matvec_H_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
+                  const R* v, R* out) {
+  int rc;
+  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, v))) return rc;
+  NodeOp<R> op = node_op0<R>();
+  op.P = ctx->P.as<const R>(); op.T = T; op.out = out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int diag_H_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
+                R* out) {
+  int rc;
+  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, in, VDVC_F32, false, nullptr))) return rc;
+  DiagOp<R> op;
+  std::memset(&op, 0, sizeof(op));
+  op.P = ctx->P.as<const R>(); op.T = T; op.out = out;
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int matvec_K_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_K, double E, const R* d_E,
+                  const R* v, R* out) {
+  int bd = 0, rc;
+  if ((rc = upload_K1<R>(ctx, h_K, &bd))) return rc;
+  NodeOp<R> op = node_op0<R>();
+  op.k1 = kop<R>(ctx->K1[rix<R>()], bd, E, d_E); op.v1 = v; op.c1 = R(1); op.k1_on = 1;
+  op.out = out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int diag_K_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_K, double E, const R* d_E,
+                R* out) {
+  int bd = 0, rc;
+  if ((rc = upload_K1<R>(ctx, h_K, &bd))) return rc;
+  DiagOp<R> op;
+  std::memset(&op, 0, sizeof(op));
+  op.k1 = kop<R>(ctx->K1[rix<R>()], bd, E, d_E); op.c1 = R(1); op.k1_on = 1;
+  op.out = out;
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int eqgap_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_Kel, double E, const R* d_E,
+               const R* v, R* out) {
+  int bd = 0, rc;
+  if ((rc = upload_K1<R>(ctx, h_Kel, &bd))) return rc;
+  if ((rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(R)))) return rc;
+  R* w = ctx->nodeC.as<R>();
+  const Buf& K1 = ctx->K1[rix<R>()];
+  NodeOp<R> o1 = node_op0<R>();
+  o1.k1 = kop<R>(K1, bd, E, d_E); o1.v1 = v; o1.c1 = R(1); o1.k1_on = 1;
+  o1.interior = 1; o1.out = w;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o1, ctx->stream));
+  NodeOp<R> o2 = node_op0<R>();
+  o2.k1 = kop<R>(K1, bd, E, d_E); o2.v1 = w; o2.c1 = R(1); o2.k1_on = 1;
+  o2.out = out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int hutch_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_Kel, double E,
+               const int8_t* d_probes, int n_probes, R* out) {
+  const long long ndof = 3 * M.Nnodes;
+  int rc;
+  if ((rc = ctx->hacc.ensure((size_t)ndof * sizeof(double)))) return rc;
+  if ((rc = ctx->nodeA.ensure((size_t)ndof * sizeof(R)))) return rc;
+  if ((rc = ctx->nodeB.ensure((size_t)ndof * sizeof(R)))) return rc;
+  if ((rc = ctx->dparts.ensure(1024 * sizeof(double)))) return rc;
+  VDVC_HIP(hipMemsetAsync(ctx->hacc.p, 0, (size_t)ndof * sizeof(double), ctx->stream));
+  for (int p = 0; p < n_probes; ++p) {
+    const int8_t* z = d_probes + (size_t)p * ndof;
+    VDVC_LAUNCH(vdvc::launch_expand_i8(ndof, z, ctx->nodeA.as<R>(), ctx->stream));
+    if ((rc = eqgap_impl<R>(ctx, M, h_Kel, E, nullptr, ctx->nodeA.as<R>(), ctx->nodeB.as<R>())))
+      return rc;
+    VDVC_LAUNCH(vdvc::launch_hutch_acc(ndof, z, ctx->nodeB.as<const R>(), ctx->hacc.as<double>(),
+                                       ctx->stream));
+  }
+  VDVC_LAUNCH(vdvc::launch_hutch_final(ndof, ctx->hacc.as<double>(), n_probes, out,
+                                       ctx->dparts.as<double>(), ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int jacobi_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
+                int reg_kind, const double* h_K1, const double* h_K2, double E, double eps_reg,
+                double ll, const R* d_extra, R* d_diag, R* d_minv) {
+  int bd1 = 0, bd2 = 0, rc;
+  if ((rc = prep_reg<R>(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, in, VDVC_F32, false, nullptr))) return rc;
+  const int nb = vdvc::node_blocks(M);
+  if ((rc = ctx->fmin.ensure((size_t)nb * sizeof(R)))) return rc;
+  DiagOp<R> op;
+  std::memset(&op, 0, sizeof(op));
+  op.P = ctx->P.as<const R>(); op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<R>();
+  if (reg_kind == VDVC_REG_K) {
+    op.k1 = kop<R>(ctx->K1[rix<R>()], bd1, E, nullptr); op.c1 = (R)ll; op.k1_on = 1;
+  } else if (reg_kind == VDVC_REG_EQGAP) {
+    // diag R = hutchinson(K1 P_i K1) + eps * diag(K2): K2 part here, the
+    // Hutchinson part via d_extra
+    if (!d_extra) return fail(-1, "REG_EQGAP needs the Hutchinson diagonal in d_extra");
+    op.k1 = kop<R>(ctx->K2[rix<R>()], bd2, E, nullptr); op.c1 = (R)(ll * eps_reg);
+    op.k1_on = 1;
+    op.extra = d_extra; op.cextra = (R)ll;
+  }
+  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_jacobi(d_diag, ctx->fmin.as<const R>(), nb, 3 * M.Nnodes, d_minv,
+                                  ctx->stream));
+  return 0;
+}
+
+template <typename R>
+int apply_A_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
+                 int reg_kind, const double* h_K1, const double* h_K2, double E, double eps_reg,
+                 double ll, const R* v, R* out) {
+  int bd1 = 0, bd2 = 0, rc;
+  if ((rc = prep_reg<R>(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, v))) return rc;
+  return apply_reg_sum<R>(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, v, out, nullptr,
+                          nullptr, nullptr);
+}
+
+template <typename R>
+int pcg_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, VoxIn in,
+             int reg_kind, const double* h_K1, const double* h_K2, double E, double eps_reg,
+             double ll, const R* d_b, const R* d_minv, R* d_x, double tol, int max_iter,
+             int check_every, int* h_iters, double* h_relres) {
+  int bd1 = 0, bd2 = 0, rc;
+  if ((rc = prep_reg<R>(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
+  const long long ndof = 3 * M.Nnodes;
+  const size_t vb = (size_t)ndof * sizeof(R);
+  if ((rc = ctx->pr.ensure(vb)) || (rc = ctx->pz.ensure(vb)) || (rc = ctx->pp.ensure(vb)) ||
+      (rc = ctx->pAp.ensure(vb)))
+    return rc;
+  const int nvb = vdvc::vec_blocks(ndof);
+  const int nnb = vdvc::node_blocks(M);
+  if ((rc = ctx->dparts.ensure((size_t)2 * nvb * sizeof(double)))) return rc;
+  if ((rc = ctx->dparts2.ensure((size_t)nnb * sizeof(double)))) return rc;
+  if ((rc = ctx->pstate.ensure(sizeof(PcgState)))) return rc;
+  if ((rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * sizeof(R)))) return rc;
+  R* r = ctx->pr.as<R>();
+  R* z = ctx->pz.as<R>();
+  R* p = ctx->pp.as<R>();
+  R* Ap = ctx->pAp.as<R>();
+  PcgState* S = ctx->pstate.as<PcgState>();
+  const int* done = &S->done;
+  VDVC_LAUNCH(vdvc::launch_pcg_init(ndof, d_b, d_minv, d_x, r, z, p, ctx->dparts.as<double>(), S,
+                                    ctx->stream));
+  in.done = done;
+  int it = 0;
+  while (it < max_iter) {
+    const int nrun = (max_iter - it) < check_every ? (max_iter - it) : check_every;
+    for (int q = 0; q < nrun; ++q, ++it) {
+      if ((rc = launch_restrict_timed(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, p)))
+        return rc;
+      if ((rc = apply_reg_sum<R>(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, p, Ap, p,
+                                 ctx->dparts2.as<double>(), done)))
+        return rc;
+      VDVC_LAUNCH(vdvc::launch_pcg_update(ndof, it, ctx->dparts2.as<double>(), nnb, S, d_minv,
+                                          (const R*)p, (const R*)Ap, d_x, r, z,
+                                          ctx->dparts.as<double>(), ctx->stream));
+      VDVC_LAUNCH(vdvc::launch_pcg_pupdate(ndof, it, tol, ctx->dparts.as<double>(), S,
+                                           (const R*)z, p, ctx->stream));
+    }
+    PcgState hs;
+    if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
+    if (hs.done) break;
+  }
+  PcgState hs;
+  if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
+  if (h_iters) *h_iters = hs.iters;
+  if (h_relres) *h_relres = hs.relres;
+  return 0;
+}
+
+// call F<float> or F<double> by working type
+#define VDVC_BY_RTYPE(rt, call_f, call_d) ((rt) == VDVC_F64 ? (call_d) : (call_f))
 
 }  // namespace
 
@@ -377,12 +602,14 @@ int vdvc_synchronize(vdvc_ctx* ctx) {
 }
 
 int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
-                     float glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
+                     double glt, void* d_gx, void* d_gy, void* d_gz, int gtype,
                      uint8_t* d_mask0) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!ftype_ok(ftype) || !gtype_ok(gtype)) return fail(-1, "unsupported element type");
+  if ((ftype == VDVC_F64) != (gtype == VDVC_F64))
+    return fail(-1, "the FP64 path needs a float64 image and float64 gradients");
   if (!d_f || !d_gx || !d_gy || !d_gz || !d_mask0) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   VDVC_LAUNCH(vdvc::launch_setup(ftype, gtype, M, d_f, glt, d_gx, d_gy, d_gz, d_mask0, ctx->stream));
@@ -390,7 +617,7 @@ int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d
 }
 
 int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_field,
-                      int ftype, const uint8_t* d_mask, int use_thresh, float glt,
+                      int ftype, const uint8_t* d_mask, int use_thresh, double glt,
                       double* h_out3) {
   if (!ctx || !h_out3 || !d_field) return fail(-1, "NULL argument");
   int rc = check_mesh(Nx, Ny, Nz, h);
@@ -406,9 +633,9 @@ int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_out3);
 }
 
-int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, const void* d_g,
-                int gitype, int gnx, int gny, int gnz, float gox, float goy, float goz,
-                float glt, const uint8_t* d_mask0, const uint8_t* d_bmask, float* d_gvals,
+int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, const void* d_g,
+                int gitype, int gnx, int gny, int gnz, double gox, double goy, double goz,
+                double glt, const uint8_t* d_mask0, const uint8_t* d_bmask, void* d_gvals,
                 uint8_t* d_mask_out, double* h_stats3) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
@@ -434,18 +661,21 @@ int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const float* d_U, 
 
 int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
                       const void* d_gy, const void* d_gz, int gtype, const void* d_f, int ftype,
-                      float mean0, const float* d_gvals, float gmean, float kscale,
-                      const uint8_t* d_bmask, const uint8_t* d_live, float* d_res, float* d_b,
+                      double mean0, const void* d_gvals, double gmean, double kscale,
+                      const uint8_t* d_bmask, const uint8_t* d_live, void* d_res, void* d_b,
                       uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
                       double* h_stats4) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!gtype_ok(gtype) || !ftype_ok(ftype)) return fail(-1, "unsupported element type");
+  if ((ftype == VDVC_F64) != (gtype == VDVC_F64))
+    return fail(-1, "the FP64 path needs a float64 image and float64 gradients");
   if (!d_gx || !d_gy || !d_gz || !d_f || !d_gvals || !d_bmask || !d_live || !d_res || !d_b)
     return fail(-1, "NULL buffer");
   const bool fold = d_fold_x != nullptr;
   if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
+  if (fold && gtype == VDVC_F64) return fail(-1, "folded bfloat16 planes need float32 input");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
   if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
@@ -456,19 +686,16 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   in.res = d_res;
   in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
   in.stats = ctx->stats.as<double>();
-  if ((rc = run_restrict(ctx, MODE_RHS, gtype, M, T, in, ftype, fold, nullptr))) return rc;
-  NodeOp op = node_op0();
-  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_b;
-  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 4, ctx->red.as<double>(),
-                                  ctx->stream));
-  if (h_stats4) return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
-  return 0;
+  return VDVC_BY_RTYPE(gtype,
+                       residual_rhs_impl<float>(ctx, M, T, gtype, in, ftype, fold,
+                                                static_cast<float*>(d_b), h_stats4),
+                       residual_rhs_impl<double>(ctx, M, T, gtype, in, ftype, fold,
+                                                 static_cast<double*>(d_b), h_stats4));
 }
 
 int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
                   const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
-                  const float* d_v, float* d_out) {
+                  const void* d_v, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
@@ -476,17 +703,17 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
-  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
-                         VDVC_F32, false, d_v)))
-    return rc;
-  NodeOp op = node_op0();
-  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_out;
-  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
-  return 0;
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(gtype,
+                       matvec_H_impl<float>(ctx, M, T, gtype, in, static_cast<const float*>(d_v),
+                                            static_cast<float*>(d_out)),
+                       matvec_H_impl<double>(ctx, M, T, gtype, in,
+                                             static_cast<const double*>(d_v),
+                                             static_cast<double*>(d_out)));
 }
 
 int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
-                const void* d_gz, int gtype, const uint8_t* d_mask, float* d_out) {
+                const void* d_gz, int gtype, const uint8_t* d_mask, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
@@ -494,158 +721,121 @@ int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!d_gx || !d_gy || !d_gz || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
-  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
-                         false, nullptr)))
-    return rc;
-  DiagOp op;
-  std::memset(&op, 0, sizeof(op));
-  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_out;
-  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
-  return 0;
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(gtype, diag_H_impl<float>(ctx, M, T, gtype, in, static_cast<float*>(d_out)),
+                       diag_H_impl<double>(ctx, M, T, gtype, in, static_cast<double*>(d_out)));
 }
 
-int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
-                  const float* d_E, const float* d_v, float* d_out) {
+int vdvc_matvec_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_K, double E,
+                  const void* d_E, const void* d_v, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, 1);
   if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (!d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, 1);
-  int bd = 0;
-  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_K, &bd))) return rc;
-  NodeOp op = node_op0();
-  op.k1 = kop(ctx->K1, bd, E, d_E); op.v1 = d_v; op.c1 = 1.0f; op.k1_on = 1;
-  op.out = d_out;
-  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
-  return 0;
+  return VDVC_BY_RTYPE(rtype,
+                       matvec_K_impl<float>(ctx, M, h_K, E, static_cast<const float*>(d_E),
+                                            static_cast<const float*>(d_v),
+                                            static_cast<float*>(d_out)),
+                       matvec_K_impl<double>(ctx, M, h_K, E, static_cast<const double*>(d_E),
+                                             static_cast<const double*>(d_v),
+                                             static_cast<double*>(d_out)));
 }
 
-int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_K, float E,
-                const float* d_E, float* d_out) {
+int vdvc_diag_K(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_K, double E,
+                const void* d_E, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, 1);
   if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (!d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, 1);
-  int bd = 0;
-  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_K, &bd))) return rc;
-  DiagOp op;
-  std::memset(&op, 0, sizeof(op));
-  op.k1 = kop(ctx->K1, bd, E, d_E); op.c1 = 1.0f; op.k1_on = 1;
-  op.out = d_out;
-  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
-  return 0;
+  return VDVC_BY_RTYPE(rtype,
+                       diag_K_impl<float>(ctx, M, h_K, E, static_cast<const float*>(d_E),
+                                          static_cast<float*>(d_out)),
+                       diag_K_impl<double>(ctx, M, h_K, E, static_cast<const double*>(d_E),
+                                           static_cast<double*>(d_out)));
 }
 
-int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
-                      const float* d_E, const float* d_v, float* d_out) {
+int vdvc_matvec_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_Kel,
+                      double E, const void* d_E, const void* d_v, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, 1);
   if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (!d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, 1);
-  int bd = 0;
-  if ((rc = upload_K(ctx, ctx->K1, ctx->k1_cache, h_Kel, &bd))) return rc;
-  if ((rc = ctx->nodeC.ensure((size_t)M.Nnodes * 3 * sizeof(float)))) return rc;
-  float* w = ctx->nodeC.as<float>();
-  NodeOp o1 = node_op0();
-  o1.k1 = kop(ctx->K1, bd, E, d_E); o1.v1 = d_v; o1.c1 = 1.0f; o1.k1_on = 1;
-  o1.interior = 1; o1.out = w;
-  VDVC_LAUNCH(vdvc::launch_node_op(M, o1, ctx->stream));
-  NodeOp o2 = node_op0();
-  o2.k1 = kop(ctx->K1, bd, E, d_E); o2.v1 = w; o2.c1 = 1.0f; o2.k1_on = 1;
-  o2.out = d_out;
-  VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
-  return 0;
+  return VDVC_BY_RTYPE(rtype,
+                       eqgap_impl<float>(ctx, M, h_Kel, E, static_cast<const float*>(d_E),
+                                         static_cast<const float*>(d_v),
+                                         static_cast<float*>(d_out)),
+                       eqgap_impl<double>(ctx, M, h_Kel, E, static_cast<const double*>(d_E),
+                                          static_cast<const double*>(d_v),
+                                          static_cast<double*>(d_out)));
 }
 
-int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const double* h_Kel, float E,
-                          const int8_t* d_probes, int n_probes, float* d_out) {
+int vdvc_hutchinson_eqgap(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int rtype, const double* h_Kel,
+                          double E, const int8_t* d_probes, int n_probes, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, 1);
   if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (!d_probes || !d_out || n_probes < 1) return fail(-1, "invalid probes / output");
   MeshDev M = make_mesh(Nx, Ny, Nz, 1);
-  const long long ndof = 3 * M.Nnodes;
-  if ((rc = ctx->hacc.ensure((size_t)ndof * sizeof(double)))) return rc;
-  if ((rc = ctx->nodeA.ensure((size_t)ndof * sizeof(float)))) return rc;
-  if ((rc = ctx->nodeB.ensure((size_t)ndof * sizeof(float)))) return rc;
-  if ((rc = ctx->dparts.ensure(1024 * sizeof(double)))) return rc;
-  VDVC_HIP(hipMemsetAsync(ctx->hacc.p, 0, (size_t)ndof * sizeof(double), ctx->stream));
-  for (int p = 0; p < n_probes; ++p) {
-    const int8_t* z = d_probes + (size_t)p * ndof;
-    VDVC_LAUNCH(vdvc::launch_expand_i8(ndof, z, ctx->nodeA.as<float>(), ctx->stream));
-    if ((rc = vdvc_matvec_eqgap(ctx, Nx, Ny, Nz, h_Kel, E, nullptr, ctx->nodeA.as<float>(),
-                                ctx->nodeB.as<float>())))
-      return rc;
-    VDVC_LAUNCH(vdvc::launch_hutch_acc(ndof, z, ctx->nodeB.as<float>(), ctx->hacc.as<double>(),
-                                       ctx->stream));
-  }
-  VDVC_LAUNCH(vdvc::launch_hutch_final(ndof, ctx->hacc.as<double>(), n_probes, d_out,
-                                       ctx->dparts.as<double>(), ctx->stream));
-  return 0;
+  return VDVC_BY_RTYPE(rtype,
+                       hutch_impl<float>(ctx, M, h_Kel, E, d_probes, n_probes,
+                                         static_cast<float*>(d_out)),
+                       hutch_impl<double>(ctx, M, h_Kel, E, d_probes, n_probes,
+                                          static_cast<double*>(d_out)));
 }
 
 int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
                 const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-                const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-                const float* d_extra, float* d_diag, float* d_minv) {
+                const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                const void* d_extra, void* d_diag, void* d_minv) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_diag || !d_minv) return fail(-1, "NULL buffer");
-  int bd1 = 0, bd2 = 0;
-  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
-  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask), VDVC_F32,
-                         false, nullptr)))
-    return rc;
-  const int nb = vdvc::node_blocks(M);
-  if ((rc = ctx->fmin.ensure((size_t)nb * sizeof(float)))) return rc;
-  DiagOp op;
-  std::memset(&op, 0, sizeof(op));
-  op.P = ctx->P.as<const float>(); op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<float>();
-  if (reg_kind == VDVC_REG_K) {
-    op.k1 = kop(ctx->K1, bd1, E, nullptr); op.c1 = ll; op.k1_on = 1;
-  } else if (reg_kind == VDVC_REG_EQGAP) {
-    // diag R = hutchinson(K1 P_i K1) + eps * diag(K2): K2 part here, the
-    // Hutchinson part via d_extra
-    if (!d_extra) return fail(-1, "REG_EQGAP needs the Hutchinson diagonal in d_extra");
-    op.k1 = kop(ctx->K2, bd2, E, nullptr); op.c1 = ll * eps_reg; op.k1_on = 1;
-    op.extra = d_extra; op.cextra = ll;
-  }
-  VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_jacobi(d_diag, ctx->fmin.as<float>(), nb, 3 * M.Nnodes, d_minv,
-                                  ctx->stream));
-  return 0;
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(
+      gtype,
+      jacobi_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                         static_cast<const float*>(d_extra), static_cast<float*>(d_diag),
+                         static_cast<float*>(d_minv)),
+      jacobi_impl<double>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                          static_cast<const double*>(d_extra), static_cast<double*>(d_diag),
+                          static_cast<double*>(d_minv)));
 }
 
 int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
                  const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-                 const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-                 const float* d_v, float* d_out) {
+                 const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                 const void* d_v, void* d_out) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
-  int bd1 = 0, bd2 = 0;
-  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
-  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vox_in(d_gx, d_gy, d_gz, d_mask),
-                         VDVC_F32, false, d_v)))
-    return rc;
-  return apply_reg_sum(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, d_v, d_out, nullptr,
-                       nullptr, nullptr);
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(
+      gtype,
+      apply_A_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                          static_cast<const float*>(d_v), static_cast<float*>(d_out)),
+      apply_A_impl<double>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                           static_cast<const double*>(d_v), static_cast<double*>(d_out)));
 }
 
 int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
              const void* d_gz, int gtype, const uint8_t* d_mask, int reg_kind,
-             const double* h_K1, const double* h_K2, float E, float eps_reg, float ll,
-             const float* d_b, const float* d_minv, float* d_x, float tol, int max_iter,
+             const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+             const void* d_b, const void* d_minv, void* d_x, double tol, int max_iter,
              int check_every, int* h_iters, double* h_relres) {
   if (!ctx) return fail(-1, "ctx is NULL");
   int rc = check_mesh(Nx, Ny, Nz, h);
@@ -654,73 +844,53 @@ int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, con
   if (!d_gx || !d_gy || !d_gz || !d_b || !d_minv || !d_x) return fail(-1, "NULL buffer");
   if (max_iter < 0) return fail(-1, "max_iter < 0");
   if (check_every < 1) check_every = 1;
-  int bd1 = 0, bd2 = 0;
-  if ((rc = prep_reg(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles(M, gtype);
-  const long long ndof = 3 * M.Nnodes;
-  const size_t vb = (size_t)ndof * sizeof(float);
-  if ((rc = ctx->pr.ensure(vb)) || (rc = ctx->pz.ensure(vb)) || (rc = ctx->pp.ensure(vb)) ||
-      (rc = ctx->pAp.ensure(vb)))
-    return rc;
-  const int nvb = vdvc::vec_blocks(ndof);
-  const int nnb = vdvc::node_blocks(M);
-  if ((rc = ctx->dparts.ensure((size_t)2 * nvb * sizeof(double)))) return rc;
-  if ((rc = ctx->dparts2.ensure((size_t)nnb * sizeof(double)))) return rc;
-  if ((rc = ctx->pstate.ensure(sizeof(PcgState)))) return rc;
-  if ((rc = ctx->P.ensure((size_t)T.nwaves * T.wave_stride * sizeof(float)))) return rc;
-  float* r = ctx->pr.as<float>();
-  float* z = ctx->pz.as<float>();
-  float* p = ctx->pp.as<float>();
-  float* Ap = ctx->pAp.as<float>();
-  PcgState* S = ctx->pstate.as<PcgState>();
-  const int* done = &S->done;
-  VDVC_LAUNCH(vdvc::launch_pcg_init(ndof, d_b, d_minv, d_x, r, z, p, ctx->dparts.as<double>(), S,
-                                    ctx->stream));
-  VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  in.done = done;
-  int it = 0;
-  while (it < max_iter) {
-    const int nrun = (max_iter - it) < check_every ? (max_iter - it) : check_every;
-    for (int q = 0; q < nrun; ++q, ++it) {
-      if ((rc = launch_restrict_timed(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, p)))
-        return rc;
-      if ((rc = apply_reg_sum(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, p, Ap, p,
-                              ctx->dparts2.as<double>(), done)))
-        return rc;
-      VDVC_LAUNCH(vdvc::launch_pcg_update(ndof, it, ctx->dparts2.as<double>(), nnb, S, d_minv, p,
-                                          Ap, d_x, r, z, ctx->dparts.as<double>(), ctx->stream));
-      VDVC_LAUNCH(vdvc::launch_pcg_pupdate(ndof, it, tol, ctx->dparts.as<double>(), S, z, p,
-                                           ctx->stream));
-    }
-    PcgState hs;
-    if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
-    if (hs.done) break;
-  }
-  PcgState hs;
-  if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;
-  if (h_iters) *h_iters = hs.iters;
-  if (h_relres) *h_relres = hs.relres;
-  return 0;
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(
+      gtype,
+      pcg_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                      static_cast<const float*>(d_b), static_cast<const float*>(d_minv),
+                      static_cast<float*>(d_x), tol, max_iter, check_every, h_iters, h_relres),
+      pcg_impl<double>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                       static_cast<const double*>(d_b), static_cast<const double*>(d_minv),
+                       static_cast<double*>(d_x), tol, max_iter, check_every, h_iters,
+                       h_relres));
 }
 
-int vdvc_dot(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, double* h_out) {
+int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b,
+             double* h_out) {
   if (!ctx || !d_a || !d_b || !h_out) return fail(-1, "NULL argument");
   if (n < 0) return fail(-1, "n < 0");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   int rc;
   const int nb = 1024;
   if ((rc = ctx->stats.ensure((size_t)nb * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
-  VDVC_LAUNCH(vdvc::launch_dot(n, d_a, d_b, ctx->stats.as<double>(), nb, ctx->stream));
+  if (rtype == VDVC_F64)
+    VDVC_LAUNCH(vdvc::launch_dot(n, static_cast<const double*>(d_a),
+                                 static_cast<const double*>(d_b), ctx->stats.as<double>(), nb,
+                                 ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_dot(n, static_cast<const float*>(d_a),
+                                 static_cast<const float*>(d_b), ctx->stats.as<double>(), nb,
+                                 ctx->stream));
   VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 1, ctx->red.as<double>(), ctx->stream));
   return fetch(ctx, ctx->red.p, sizeof(double), h_out);
 }
 
-int vdvc_axpby(vdvc_ctx* ctx, int64_t n, float a, const float* d_x, float b, float* d_y) {
+int vdvc_axpby(vdvc_ctx* ctx, int64_t n, int rtype, double a, const void* d_x, double b,
+               void* d_y) {
   if (!ctx || !d_x || !d_y) return fail(-1, "NULL argument");
   if (n < 0) return fail(-1, "n < 0");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (n == 0) return 0;
-  VDVC_LAUNCH(vdvc::launch_axpby(n, a, d_x, b, d_y, ctx->stream));
+  if (rtype == VDVC_F64)
+    VDVC_LAUNCH(vdvc::launch_axpby(n, a, static_cast<const double*>(d_x), b,
+                                   static_cast<double*>(d_y), ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_axpby(n, a, static_cast<const float*>(d_x), b,
+                                   static_cast<float*>(d_y), ctx->stream));
   return 0;
 }
 
@@ -742,11 +912,19 @@ int vdvc_erode6(vdvc_ctx* ctx, int nx, int ny, int nz, const uint8_t* d_in, uint
   return 0;
 }
 
-int vdvc_mul(vdvc_ctx* ctx, int64_t n, const float* d_a, const float* d_b, float* d_out) {
+int vdvc_mul(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b, void* d_out) {
   if (!ctx || !d_a || !d_b || !d_out) return fail(-1, "NULL argument");
   if (n < 0) return fail(-1, "n < 0");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
   if (n == 0) return 0;
-  VDVC_LAUNCH(vdvc::launch_mul(n, d_a, d_b, d_out, ctx->stream));
+  if (rtype == VDVC_F64)
+    VDVC_LAUNCH(vdvc::launch_mul(n, static_cast<const double*>(d_a),
+                                 static_cast<const double*>(d_b), static_cast<double*>(d_out),
+                                 ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_mul(n, static_cast<const float*>(d_a),
+                                 static_cast<const float*>(d_b), static_cast<float*>(d_out),
+                                 ctx->stream));
   return 0;
 }
 

@@ -63,8 +63,27 @@ struct TilesDev {
   long long wave_stride;  // floats per wave in the partial buffer
 };
 
-enum { GT_F32 = 0, GT_BF16 = 1, GT_F16 = 2 };
+enum { GT_F32 = 0, GT_BF16 = 1, GT_F16 = 2, GT_F64 = 3 };
 
+// Arithmetic type of a storage type: double storage selects the FP64 path
+// (every voxel and node computation in double, the reference's
+// dtype=float64 solve); every other storage type computes in float.
+template <typename T> struct RealOf { using type = float; };
+template <> struct RealOf<double> { using type = double; };
+
+// correctly rounded single operations without contraction (the reference's
+// element-wise NumPy steps)
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+
+__device__ __forceinline__ double ld_val(const double* p, long long i) { return p[i]; }
 __device__ __forceinline__ float ld_val(const float* p, long long i) { return p[i]; }
 __device__ __forceinline__ float ld_val(const __hip_bfloat16* p, long long i) {
   // bf16 -> f32 is a 16-bit left shift of the raw bits

@@ -304,6 +304,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
   const GT* g1 = reinterpret_cast<const GT*>(in.g[1]);
   const GT* g2 = reinterpret_cast<const GT*>(in.g[2]);
   const FT* fim = reinterpret_cast<const FT*>(in.fimg);
+  // residual-mode inputs (this kernel computes in float)
+  const float* gvals_f = reinterpret_cast<const float*>(in.gvals);
+  float* res_f = reinterpret_cast<float*>(in.res);
+  const float mean0_f = (float)in.mean0, gmean_f = (float)in.gmean, kscale_f = (float)in.kscale;
   float* Pw = P + (size_t)w * T.wave_stride;
 
   // node-plane accumulators C[cx][cy][cz][k]
@@ -507,15 +511,15 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
       float fv[V], gv[V];
       uint32_t lv[V];
       ld_any<FT, V>(fim, vi, fv);
-      ld_f32<V>(in.gvals, vi, gv);
+      ld_f32<V>(gvals_f, vi, gv);
       ld_u8<V>(in.live, vi, lv);
       const int mxy = axis_mult(y, h, M.Nvy - 1) * mz;
 #pragma unroll
       for (int j = 0; j < V; ++j) {
         // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
         // the reference's order, no contraction; zero outside b_mask
-        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], in.gmean), in.kscale),
-                             __fsub_rn(fv[j], in.mean0));
+        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], gmean_f), kscale_f),
+                             __fsub_rn(fv[j], mean0_f));
         rv = mk[j] ? rv : 0.0f;
         resv[j] = rv;
         if (act && lv[j]) {
@@ -528,7 +532,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
         }
       }
       if (act) {
-        st_f32<V>(in.res, vi, resv);
+        st_f32<V>(res_f, vi, resv);
         if (FOLD) {
           float fx[V], fy[V], fz[V];
 #pragma unroll
@@ -577,11 +581,11 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
         sd = tgx * (ux0 + dxx) + tgy * (uy0 + dxy) + tgz * (uz0 + dxz);
       } else if (MODE == MODE_RHS) {
         const float fv = ld_val(fim, ti);
-        const float gv = in.gvals[ti];
-        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
-                             __fsub_rn(fv, in.mean0));
+        const float gv = gvals_f[ti];
+        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, gmean_f), kscale_f),
+                             __fsub_rn(fv, mean0_f));
         rv = tm ? rv : 0.0f;
-        in.res[ti] = rv;
+        res_f[ti] = rv;
         if (in.live[ti]) {
           const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
           st_c += mlt;
@@ -665,6 +669,7 @@ namespace vdvc {
 // of two <= 64 (the generic kernel is used)
 int vec_width(int h, int gtype) {
   if (h < 1 || h > 64 || (h & (h - 1)) != 0) return 0;
+  if (gtype == GT_F64) return 0;  // the FP64 path uses the generic kernel
   int vmax = gtype == GT_F32 ? 4 : 8;
   // tuning override of the widest lane width (a power of two, at most the
   // 16-byte width), read at every plan: VDVC_VEC_F32 / VDVC_VEC_BF16

@@ -9,10 +9,10 @@ struct VoxIn {
   const uint8_t* mask;       // optional voxel mask (HASMASK)
   // RHS mode
   const void* fimg;          // reference image (FT), raw values
-  const float* gvals;        // sampled deformed image
+  const void* gvals;         // sampled deformed image (R = RealOf<GT>)
   const uint8_t* live;       // live mask (display statistics)
-  float mean0, gmean, kscale;
-  float* res;                // residual out
+  double mean0, gmean, kscale;  // converted to R in the kernel
+  void* res;                 // residual out (R)
   uint16_t* fold_out[3];     // optional bf16 gradient * mask out
   double* stats;             // per-wave partial statistics [nwaves][4]
   const int* done;           // early-exit flag (device-resident PCG), may be null
@@ -21,11 +21,11 @@ struct VoxIn {
 struct SampleIn {
   const void* gimg;      // deformed image, x fastest, (gnx, gny, gnz)
   int gnx, gny, gnz;
-  float gox, goy, goz;   // g origin
-  float glt;
+  double gox, goy, goz;  // g origin (converted to R)
+  double glt;            // threshold (converted to R)
   const uint8_t* mask0;  // reference mask (may be null: all true)
   const uint8_t* bmask;  // frozen b_mask for the statistics (null: live mask)
-  float* gvals;
+  void* gvals;           // R
   uint8_t* mask_out;
   double* stats;         // per-wave partials [nwaves][3]
 };
@@ -33,23 +33,25 @@ struct SampleIn {
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over
 // the elements adjacent to node (i,j,k).  blockdiag=1 means K[3a+c][3b+d] =
 // L8[a][b] delta_cd with L8 = K[3a][3b] (vector Laplacian), cheaper path.
+template <typename R>
 struct KOp {
-  const float* K;      // 24*24 row-major (device)
+  const R* K;          // 24*24 row-major (device)
   int blockdiag;
-  float E;             // scalar element scale
-  const float* Earr;   // optional per-element scale (Nx*Ny*Nz)
+  R E;                 // scalar element scale
+  const R* Earr;       // optional per-element scale (Nx*Ny*Nz)
 };
 
+template <typename R>
 struct NodeOp {
-  const float* P;        // voxel partials (null: none)
+  const R* P;            // voxel partials (null: none)
   TilesDev T;
-  float cH;              // scale of the partial sum
-  KOp k1; const float* v1; float c1; int k1_on;
-  KOp k2; const float* v2; float c2; int k2_on;
-  const float* addin; float cadd;  // out += cadd * addin (may alias out)
+  R cH;                  // scale of the partial sum
+  KOp<R> k1; const R* v1; R c1; int k1_on;
+  KOp<R> k2; const R* v2; R c2; int k2_on;
+  const R* addin; R cadd;          // out += cadd * addin (may alias out)
   int interior;                    // if set: result *= interior-node mask (eq-gap P_i)
-  float* out;
-  const float* dotv;               // partial dot(dotv, out) in FP64 (null: none)
+  R* out;
+  const R* dotv;                   // partial dot(dotv, out) in FP64 (null: none)
   double* dot_part;
   const int* done;                 // early-exit flag (PCG), may be null
 };
@@ -57,12 +59,13 @@ struct NodeOp {
 // diag of the assembled H (from DIAG-mode partials) plus c1 * diag(K1)
 // plus c2 * extra (e.g. a precomputed regularizer diagonal), and per-block
 // minimum for the Jacobi shift
+template <typename R>
 struct DiagOp {
-  const float* P; TilesDev T;
-  KOp k1; float c1; int k1_on;
-  const float* extra; float cextra;
-  float* out;
-  float* min_part;
+  const R* P; TilesDev T;
+  KOp<R> k1; R c1; int k1_on;
+  const R* extra; R cextra;
+  R* out;
+  R* min_part;
 };
 
 struct PcgState {
@@ -83,38 +86,51 @@ int vec_width(int h, int gtype);
 int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                         const TilesDev& T, const VoxIn& in, const float* v, float* P,
                         hipStream_t st);
+// generic voxel kernel; v and P are R = double for gtype GT_F64, else float
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
-                    const TilesDev& T, const VoxIn& in, const float* v, float* P, hipStream_t st);
+                    const TilesDev& T, const VoxIn& in, const void* v, void* P, hipStream_t st);
+// U and gvals are double for an F64 image, else float
 int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,
-                  const float* U, hipStream_t st);
-int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, float glt, void* gx,
+                  const void* U, hipStream_t st);
+int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, double glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st);
 int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
-                 float glt, double* part, int nblocks, hipStream_t st);
+                 double glt, double* part, int nblocks, hipStream_t st);
 int launch_reduce(const double* part, long long n, int nv, double* out, hipStream_t st);
 int launch_fold(long long n, const uint16_t* gx, const uint16_t* gy, const uint16_t* gz,
                 const uint8_t* m, uint16_t* ox, uint16_t* oy, uint16_t* oz, hipStream_t st);
 int launch_erode6(int nx, int ny, int nz, const uint8_t* in, uint8_t* out, hipStream_t st);
-int launch_mul(long long n, const float* a, const float* b, float* out, hipStream_t st);
-int launch_node_op(const MeshDev& M, const NodeOp& op, hipStream_t st);
+template <typename R>
+int launch_mul(long long n, const R* a, const R* b, R* out, hipStream_t st);
+template <typename R>
+int launch_node_op(const MeshDev& M, const NodeOp<R>& op, hipStream_t st);
 int node_blocks(const MeshDev& M);
 int vec_blocks(long long n);
-int launch_diag_op(const MeshDev& M, const DiagOp& op, hipStream_t st);
-int launch_jacobi(const float* diag, const float* min_part, int nparts, long long ndof,
-                  float* minv, hipStream_t st);
-int launch_pcg_init(long long ndof, const float* b, const float* minv, float* x, float* r,
-                    float* z, float* p, double* part, PcgState* S, hipStream_t st);
+template <typename R>
+int launch_diag_op(const MeshDev& M, const DiagOp<R>& op, hipStream_t st);
+template <typename R>
+int launch_jacobi(const R* diag, const R* min_part, int nparts, long long ndof, R* minv,
+                  hipStream_t st);
+template <typename R>
+int launch_pcg_init(long long ndof, const R* b, const R* minv, R* x, R* r, R* z, R* p,
+                    double* part, PcgState* S, hipStream_t st);
+template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
-                      const float* minv, const float* p, const float* Ap, float* x, float* r,
-                      float* z, double* part, hipStream_t st);
-int launch_pcg_pupdate(long long ndof, int it, float tol, const double* part, PcgState* S,
-                       const float* z, float* p, hipStream_t st);
-int launch_axpby(long long n, float a, const float* x, float b, float* y, hipStream_t st);
-int launch_dot(long long n, const float* a, const float* b, double* part, int nparts,
-               hipStream_t st);
-int launch_hutch_acc(long long ndof, const int8_t* z, const float* Rz, double* acc, hipStream_t st);
-int launch_expand_i8(long long n, const int8_t* z, float* out, hipStream_t st);
-int launch_hutch_final(long long ndof, const double* acc, int nprobes, float* out, double* scratch,
+                      const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
+                      hipStream_t st);
+template <typename R>
+int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
+                       const R* z, R* p, hipStream_t st);
+template <typename R>
+int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t st);
+template <typename R>
+int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hipStream_t st);
+template <typename R>
+int launch_hutch_acc(long long ndof, const int8_t* z, const R* Rz, double* acc, hipStream_t st);
+template <typename R>
+int launch_expand_i8(long long n, const int8_t* z, R* out, hipStream_t st);
+template <typename R>
+int launch_hutch_final(long long ndof, const double* acc, int nprobes, R* out, double* scratch,
                        hipStream_t st);
 int launch_stride(int type, const void* src, int nx, int ny, int nz, int s, void* dst, int mx,
                   int my, int mz, hipStream_t st);

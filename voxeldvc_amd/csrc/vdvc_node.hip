@@ -19,26 +19,27 @@ __constant__ int c_cdy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __constant__ int c_cdz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 
-__device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp& op,
-                                          const float* __restrict__ v, int i, int j, int k,
-                                          float (&out)[3]) {
-  out[0] = out[1] = out[2] = 0.0f;
+template <typename R>
+__device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp<R>& op,
+                                          const R* __restrict__ v, int i, int j, int k,
+                                          R (&out)[3]) {
+  out[0] = out[1] = out[2] = R(0);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + (e >> 2);
     if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
     const int la = corner_of(i - ei, j - ej, k - ek);
-    const float Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
+    const R Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
                              : op.E;
-    float acc[3] = {0.0f, 0.0f, 0.0f};
+    R acc[3] = {R(0), R(0), R(0)};
     for (int lb = 0; lb < 8; ++lb) {
       const long long nb = node_index(M, ei + c_cdx[lb], ej + c_cdy[lb], ek + c_cdz[lb]);
-      const float vb0 = v[3 * nb], vb1 = v[3 * nb + 1], vb2 = v[3 * nb + 2];
+      const R vb0 = v[3 * nb], vb1 = v[3 * nb + 1], vb2 = v[3 * nb + 2];
       if (op.blockdiag) {
-        const float l = op.K[(3 * la) * 24 + 3 * lb];
+        const R l = op.K[(3 * la) * 24 + 3 * lb];
         acc[0] += l * vb0; acc[1] += l * vb1; acc[2] += l * vb2;
       } else {
-        const float* Kr = op.K + (3 * la) * 24 + 3 * lb;
+        const R* Kr = op.K + (3 * la) * 24 + 3 * lb;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           acc[c] += Kr[c * 24 + 0] * vb0 + Kr[c * 24 + 1] * vb1 + Kr[c * 24 + 2] * vb2;
@@ -48,15 +49,16 @@ __device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp& op,
   }
 }
 
-__device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp& op, int i, int j, int k,
-                                         float (&out)[3]) {
-  out[0] = out[1] = out[2] = 0.0f;
+template <typename R>
+__device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp<R>& op, int i, int j, int k,
+                                         R (&out)[3]) {
+  out[0] = out[1] = out[2] = R(0);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + (e >> 2);
     if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
     const int la = corner_of(i - ei, j - ej, k - ek);
-    const float Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
+    const R Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
                              : op.E;
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] += Ee * op.K[(3 * la + c) * 24 + 3 * la + c];
@@ -65,9 +67,10 @@ __device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp& op, int i,
 
 // sum of the node partials written by the voxel kernels for node (i,j,k),
 // in fixed order (z tiles, then y rows, then x strips)
+template <typename R>
 __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev& T,
-                                                const float* __restrict__ P, int i, int j, int k,
-                                                float (&s)[3]) {
+                                                const R* __restrict__ P, int i, int j, int k,
+                                                R (&s)[3]) {
   const int h = M.h;
   const int kz = k / T.ez;
   const int za = max(0, kz - 1), zb = min(T.nzt - 1, kz);
@@ -103,7 +106,7 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
   // is issued before the first sum, so their latencies overlap; absent slots
   // read P[0..2] and are dropped.  Summation order is fixed: z tiles, then
   // element rows (upper first), then strips and row parts.
-  const float* ptr[16];
+  const R* ptr[16];
   bool ok[16];
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
@@ -129,12 +132,12 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
       }
     }
   }
-  float val[16][3];
+  R val[16][3];
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
   }
-  s[0] = s[1] = s[2] = 0.0f;
+  s[0] = s[1] = s[2] = R(0);
 #pragma unroll
   for (int e = 0; e < 16; ++e)
     if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
@@ -148,7 +151,8 @@ __device__ __forceinline__ void node_ijk(const MeshDev& M, unsigned n, int& i, i
   j = (int)(r - (unsigned)k * (unsigned)M.nny);
 }
 
-__global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp op) {
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R> op) {
   __shared__ double lds[4];
   if (op.done && *op.done) return;
   const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
@@ -156,9 +160,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp o
   if (n < M.Nnodes) {
     int i, j, k;
     node_ijk(M, (unsigned)n, i, j, k);
-    float acc[3] = {0.0f, 0.0f, 0.0f};
+    R acc[3] = {R(0), R(0), R(0)};
     if (op.P) {
-      float s[3];
+      R s[3];
       gather_partials(M, op.T, op.P, i, j, k, s);
       acc[0] = op.cH * s[0]; acc[1] = op.cH * s[1]; acc[2] = op.cH * s[2];
     }
@@ -168,12 +172,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp o
       acc[2] += op.cadd * op.addin[3 * n + 2];
     }
     if (op.k1_on) {
-      float t[3];
+      R t[3];
       kop_apply(M, op.k1, op.v1, i, j, k, t);
       acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
     }
     if (op.k2_on) {
-      float t[3];
+      R t[3];
       kop_apply(M, op.k2, op.v2, i, j, k, t);
       acc[0] += op.c2 * t[0]; acc[1] += op.c2 * t[1]; acc[2] += op.c2 * t[2];
     }
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp o
       // interior nodes: strictly inside the mesh bounding box
       // (engine/geometry_dvc.py:201-217)
       const bool in = i > 0 && i < M.Nx && j > 0 && j < M.Ny && k > 0 && k < M.Nz;
-      if (!in) { acc[0] = 0.0f; acc[1] = 0.0f; acc[2] = 0.0f; }
+      if (!in) { acc[0] = R(0); acc[1] = R(0); acc[2] = R(0); }
     }
     op.out[3 * n] = acc[0];
     op.out[3 * n + 1] = acc[1];
@@ -199,17 +203,18 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp o
 }
 
 
-__global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp op) {
-  __shared__ float lds[VDVC_BLOCK];
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp<R> op) {
+  __shared__ R lds[VDVC_BLOCK];
   const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  float mn = INFINITY;
+  R mn = R(INFINITY);
   if (n < M.Nnodes) {
     int i, j, k;
     node_ijk(M, (unsigned)n, i, j, k);
-    float acc[3] = {0.0f, 0.0f, 0.0f};
+    R acc[3] = {R(0), R(0), R(0)};
     if (op.P) gather_partials(M, op.T, op.P, i, j, k, acc);
     if (op.k1_on) {
-      float t[3];
+      R t[3];
       kop_diag(M, op.k1, i, j, k, t);
       acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
     }
@@ -221,13 +226,13 @@ __global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp o
     op.out[3 * n] = acc[0];
     op.out[3 * n + 1] = acc[1];
     op.out[3 * n + 2] = acc[2];
-    mn = fminf(acc[0], fminf(acc[1], acc[2]));
+    mn = rmin(acc[0], rmin(acc[1], acc[2]));
   }
   if (op.min_part) {
     lds[threadIdx.x] = mn;
     __syncthreads();
     for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) lds[threadIdx.x] = fminf(lds[threadIdx.x], lds[threadIdx.x + o]);
+      if (threadIdx.x < o) lds[threadIdx.x] = rmin(lds[threadIdx.x], lds[threadIdx.x + o]);
       __syncthreads();
     }
     if (threadIdx.x == 0) op.min_part[blockIdx.x] = lds[0];
@@ -236,23 +241,24 @@ __global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp o
 
 // M_inv = 1 / (diag + 1e-5 * min(diag)); every block reduces the partial
 // minima itself (fixed order), so no host round trip is needed
-__global__ __launch_bounds__(VDVC_BLOCK) void jacobi_kernel(const float* __restrict__ diag,
-                                                            const float* __restrict__ min_part,
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void jacobi_kernel(const R* __restrict__ diag,
+                                                            const R* __restrict__ min_part,
                                                             int nparts, long long ndof,
-                                                            float* __restrict__ minv) {
-  __shared__ float lds[VDVC_BLOCK];
-  float mn = INFINITY;
-  for (int q = threadIdx.x; q < nparts; q += VDVC_BLOCK) mn = fminf(mn, min_part[q]);
+                                                            R* __restrict__ minv) {
+  __shared__ R lds[VDVC_BLOCK];
+  R mn = R(INFINITY);
+  for (int q = threadIdx.x; q < nparts; q += VDVC_BLOCK) mn = rmin(mn, min_part[q]);
   lds[threadIdx.x] = mn;
   __syncthreads();
   for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) lds[threadIdx.x] = fminf(lds[threadIdx.x], lds[threadIdx.x + o]);
+    if (threadIdx.x < o) lds[threadIdx.x] = rmin(lds[threadIdx.x], lds[threadIdx.x + o]);
     __syncthreads();
   }
-  const float eps0 = (float)(1e-5 * (double)lds[0]);
+  const R eps0 = (R)(1e-5 * (double)lds[0]);
   for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
        q += (long long)gridDim.x * VDVC_BLOCK)
-    minv[q] = 1.0f / (diag[q] + eps0);
+    minv[q] = R(1) / (diag[q] + eps0);
 }
 
 // ---------------------------------------------------------------------------
@@ -274,21 +280,22 @@ __device__ __forceinline__ double sum_parts(const double* __restrict__ part, int
   return r;
 }
 
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void pcg_init_kernel(long long ndof,
-                                                              const float* __restrict__ b,
-                                                              const float* __restrict__ minv,
-                                                              float* __restrict__ x,
-                                                              float* __restrict__ r,
-                                                              float* __restrict__ z,
-                                                              float* __restrict__ p,
+                                                              const R* __restrict__ b,
+                                                              const R* __restrict__ minv,
+                                                              R* __restrict__ x,
+                                                              R* __restrict__ r,
+                                                              R* __restrict__ z,
+                                                              R* __restrict__ p,
                                                               double* __restrict__ part) {
   __shared__ double lds[8];
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < ndof) {
-    const float bq = b[q];
-    const float zq = minv[q] * bq;
-    x[q] = 0.0f; r[q] = bq; z[q] = zq; p[q] = zq;
+    const R bq = b[q];
+    const R zq = minv[q] * bq;
+    x[q] = R(0); r[q] = bq; z[q] = zq; p[q] = zq;
     v[0] = (double)bq * zq;
     v[1] = (double)bq * bq;
   }
@@ -311,24 +318,25 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_init_state_kernel(const double
 
 // alpha = rz / (p.Ap + 1e-30); x += alpha p; r -= alpha Ap; z = M r;
 // partial dots (r.r, r.z)
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_kernel(
     long long ndof, int it, const double* __restrict__ pap_part, int npap,
-    PcgState* __restrict__ S, const float* __restrict__ minv, const float* __restrict__ p,
-    const float* __restrict__ Ap, float* __restrict__ x, float* __restrict__ r,
-    float* __restrict__ z, double* __restrict__ part) {
+    PcgState* __restrict__ S, const R* __restrict__ minv, const R* __restrict__ p,
+    const R* __restrict__ Ap, R* __restrict__ x, R* __restrict__ r,
+    R* __restrict__ z, double* __restrict__ part) {
   __shared__ double lds[VDVC_BLOCK];
   if (S->done) return;
   const double pAp = sum_parts(pap_part, npap, 1, 0, lds);
   const double alpha = S->rz[it & 1] / (pAp + 1e-30);
-  const float af = (float)alpha;
+  const R af = (R)alpha;
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < ndof) {
-    const float pq = p[q];
+    const R pq = p[q];
     x[q] = x[q] + af * pq;
-    const float rq = r[q] - af * Ap[q];
+    const R rq = r[q] - af * Ap[q];
     r[q] = rq;
-    const float zq = minv[q] * rq;
+    const R zq = minv[q] * rq;
     z[q] = zq;
     v[0] = (double)rq * rq;
     v[1] = (double)rq * zq;
@@ -339,15 +347,16 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_kernel(
 
 // convergence test sqrt(r.r / r0) < tol (then stop: x is final), else
 // beta = rz_new / (rz + 1e-30); p = z + beta p
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
-    long long ndof, int it, float tol, const double* __restrict__ part, int nparts,
-    PcgState* __restrict__ S, const float* __restrict__ z, float* __restrict__ p) {
+    long long ndof, int it, double tol, const double* __restrict__ part, int nparts,
+    PcgState* __restrict__ S, const R* __restrict__ z, R* __restrict__ p) {
   __shared__ double lds[VDVC_BLOCK];
   if (S->done) return;
   const double rr = sum_parts(part, nparts, 2, 0, lds);
   const double rzn = sum_parts(part, nparts, 2, 1, lds);
   const double res = sqrt(rr / S->r0);
-  const bool conv = res < (double)tol;
+  const bool conv = res < tol;
   if (conv) {
     // every block sees the same sums; block 0 records the stop
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
     return;
   }
   const double beta = rzn / (S->rz[it & 1] + 1e-30);
-  const float bf = (float)beta;
+  const R bf = (R)beta;
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   if (q < ndof) p[q] = z[q] + bf * p[q];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -372,17 +381,19 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
 // ---------------------------------------------------------------------------
 // small vector utilities
 // ---------------------------------------------------------------------------
-// y = a*x + b*y (float), optional partial FP64 dot(y, y)
-__global__ __launch_bounds__(VDVC_BLOCK) void axpby_kernel(long long n, float a,
-                                                           const float* __restrict__ x, float b,
-                                                           float* __restrict__ y) {
+// y = a*x + b*y
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void axpby_kernel(long long n, R a,
+                                                           const R* __restrict__ x, R b,
+                                                           R* __restrict__ y) {
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   if (q < n) y[q] = a * x[q] + b * y[q];
 }
 
 // partial FP64 dot products of two vectors
-__global__ __launch_bounds__(VDVC_BLOCK) void dot_kernel(long long n, const float* __restrict__ a,
-                                                         const float* __restrict__ b,
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void dot_kernel(long long n, const R* __restrict__ a,
+                                                         const R* __restrict__ b,
                                                          double* __restrict__ part) {
   __shared__ double lds[4];
   double v[1] = {0.0};
@@ -394,20 +405,22 @@ __global__ __launch_bounds__(VDVC_BLOCK) void dot_kernel(long long n, const floa
 }
 
 // Hutchinson accumulation acc += z * Rz (FP64), z a Rademacher probe
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void hutch_acc_kernel(long long n,
                                                                const int8_t* __restrict__ z,
-                                                               const float* __restrict__ Rz,
+                                                               const R* __restrict__ Rz,
                                                                double* __restrict__ acc) {
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   if (q < n) acc[q] += (double)z[q] * (double)Rz[q];
 }
 
-// float copy of an int8 probe vector
+// an int8 probe vector converted to the working type
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void expand_i8_kernel(long long n,
                                                                const int8_t* __restrict__ z,
-                                                               float* __restrict__ out) {
+                                                               R* __restrict__ out) {
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  if (q < n) out[q] = (float)z[q];
+  if (q < n) out[q] = (R)z[q];
 }
 
 // max |acc| / nprobes per block
@@ -430,12 +443,13 @@ __global__ __launch_bounds__(VDVC_BLOCK) void hutch_max_kernel(long long n,
 
 // d = acc / nprobes clipped below at 1e-12 * max|d| (every block reduces
 // the block maxima itself)
+template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void hutch_final_kernel(long long n,
                                                                  const double* __restrict__ acc,
                                                                  int nprobes,
                                                                  const double* __restrict__ part,
                                                                  int nparts,
-                                                                 float* __restrict__ out) {
+                                                                 R* __restrict__ out) {
   __shared__ double lds[VDVC_BLOCK];
   double m = 0.0;
   for (int q = threadIdx.x; q < nparts; q += VDVC_BLOCK) m = fmax(m, part[q]);
@@ -449,7 +463,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void hutch_final_kernel(long long n,
   for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
        q += (long long)gridDim.x * VDVC_BLOCK) {
     const double d = acc[q] / nprobes;
-    out[q] = (float)(d < lo ? lo : d);
+    out[q] = (R)(d < lo ? lo : d);
   }
 }
 
@@ -457,84 +471,113 @@ namespace vdvc {
 
 int vec_blocks(long long n) { return (int)((n + VDVC_BLOCK - 1) / VDVC_BLOCK); }
 
-int launch_hutch_acc(long long ndof, const int8_t* z, const float* Rz, double* acc,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(hutch_acc_kernel, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof, z,
-                     Rz, acc);
+int node_blocks(const MeshDev& M) { return (int)((M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK); }
+
+template <typename R>
+int launch_hutch_acc(long long ndof, const int8_t* z, const R* Rz, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(hutch_acc_kernel<R>, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
+                     z, Rz, acc);
   return (int)hipGetLastError();
 }
 
-int launch_expand_i8(long long n, const int8_t* z, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(expand_i8_kernel, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, z, out);
+template <typename R>
+int launch_expand_i8(long long n, const int8_t* z, R* out, hipStream_t st) {
+  hipLaunchKernelGGL(expand_i8_kernel<R>, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, z,
+                     out);
   return (int)hipGetLastError();
 }
 
-int launch_hutch_final(long long ndof, const double* acc, int nprobes, float* out,
-                       double* scratch, hipStream_t st) {
+template <typename R>
+int launch_hutch_final(long long ndof, const double* acc, int nprobes, R* out, double* scratch,
+                       hipStream_t st) {
   int nb = vec_blocks(ndof);
   if (nb > 1024) nb = 1024;
   hipLaunchKernelGGL(hutch_max_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, acc, scratch);
-  hipLaunchKernelGGL(hutch_final_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, acc, nprobes,
-                     scratch, nb, out);
+  hipLaunchKernelGGL(hutch_final_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, acc,
+                     nprobes, scratch, nb, out);
   return (int)hipGetLastError();
 }
 
-int launch_node_op(const MeshDev& M, const NodeOp& op, hipStream_t st) {
-  const long long nb = (M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK;
-  hipLaunchKernelGGL(node_op_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, M, op);
+template <typename R>
+int launch_node_op(const MeshDev& M, const NodeOp<R>& op, hipStream_t st) {
+  hipLaunchKernelGGL(node_op_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
   return (int)hipGetLastError();
 }
 
-int node_blocks(const MeshDev& M) { return (int)((M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK); }
-
-int launch_diag_op(const MeshDev& M, const DiagOp& op, hipStream_t st) {
-  hipLaunchKernelGGL(diag_op_kernel, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
+template <typename R>
+int launch_diag_op(const MeshDev& M, const DiagOp<R>& op, hipStream_t st) {
+  hipLaunchKernelGGL(diag_op_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
   return (int)hipGetLastError();
 }
 
-int launch_jacobi(const float* diag, const float* min_part, int nparts, long long ndof,
-                  float* minv, hipStream_t st) {
+template <typename R>
+int launch_jacobi(const R* diag, const R* min_part, int nparts, long long ndof, R* minv,
+                  hipStream_t st) {
   long long nb = (ndof + VDVC_BLOCK - 1) / VDVC_BLOCK;
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(jacobi_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, diag, min_part,
-                     nparts, ndof, minv);
+  hipLaunchKernelGGL(jacobi_kernel<R>, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, diag,
+                     min_part, nparts, ndof, minv);
   return (int)hipGetLastError();
 }
 
-int launch_pcg_init(long long ndof, const float* b, const float* minv, float* x, float* r,
-                    float* z, float* p, double* part, PcgState* S, hipStream_t st) {
+template <typename R>
+int launch_pcg_init(long long ndof, const R* b, const R* minv, R* x, R* r, R* z, R* p,
+                    double* part, PcgState* S, hipStream_t st) {
   const int nb = vec_blocks(ndof);
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, b, minv, x, r, z,
-                     p, part);
+  hipLaunchKernelGGL(pcg_init_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, b, minv, x, r,
+                     z, p, part);
   hipLaunchKernelGGL(pcg_init_state_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, part, nb, S);
   return (int)hipGetLastError();
 }
 
+template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
-                      const float* minv, const float* p, const float* Ap, float* x, float* r,
-                      float* z, double* part, hipStream_t st) {
-  hipLaunchKernelGGL(pcg_update_kernel, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
+                      const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(pcg_update_kernel<R>, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
                      it, pap_part, npap, S, minv, p, Ap, x, r, z, part);
   return (int)hipGetLastError();
 }
 
-int launch_pcg_pupdate(long long ndof, int it, float tol, const double* part, PcgState* S,
-                       const float* z, float* p, hipStream_t st) {
+template <typename R>
+int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
+                       const R* z, R* p, hipStream_t st) {
   const int nb = vec_blocks(ndof);
-  hipLaunchKernelGGL(pcg_pupdate_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol, part,
-                     nb, S, z, p);
+  hipLaunchKernelGGL(pcg_pupdate_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol,
+                     part, nb, S, z, p);
   return (int)hipGetLastError();
 }
 
-int launch_axpby(long long n, float a, const float* x, float b, float* y, hipStream_t st) {
-  hipLaunchKernelGGL(axpby_kernel, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, a, x, b, y);
+template <typename R>
+int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel<R>, dim3(vec_blocks(n)), dim3(VDVC_BLOCK), 0, st, n, (R)a, x,
+                     (R)b, y);
   return (int)hipGetLastError();
 }
 
-int launch_dot(long long n, const float* a, const float* b, double* part, int nparts,
-               hipStream_t st) {
-  hipLaunchKernelGGL(dot_kernel, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, a, b, part);
+template <typename R>
+int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(dot_kernel<R>, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, a, b, part);
   return (int)hipGetLastError();
 }
+
+#define VDVC_INST(R)                                                                         \
+  template int launch_hutch_acc<R>(long long, const int8_t*, const R*, double*, hipStream_t); \
+  template int launch_expand_i8<R>(long long, const int8_t*, R*, hipStream_t);               \
+  template int launch_hutch_final<R>(long long, const double*, int, R*, double*, hipStream_t); \
+  template int launch_node_op<R>(const MeshDev&, const NodeOp<R>&, hipStream_t);             \
+  template int launch_diag_op<R>(const MeshDev&, const DiagOp<R>&, hipStream_t);             \
+  template int launch_jacobi<R>(const R*, const R*, int, long long, R*, hipStream_t);        \
+  template int launch_pcg_init<R>(long long, const R*, const R*, R*, R*, R*, R*, double*,    \
+                                  PcgState*, hipStream_t);                                   \
+  template int launch_pcg_update<R>(long long, int, const double*, int, PcgState*, const R*, \
+                                    const R*, const R*, R*, R*, R*, double*, hipStream_t);   \
+  template int launch_pcg_pupdate<R>(long long, int, double, const double*, PcgState*,       \
+                                     const R*, R*, hipStream_t);                             \
+  template int launch_axpby<R>(long long, double, const R*, double, R*, hipStream_t);        \
+  template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);
+VDVC_INST(float)
+VDVC_INST(double)
+#undef VDVC_INST
 
 }  // namespace vdvc

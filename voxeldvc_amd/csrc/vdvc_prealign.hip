@@ -166,6 +166,9 @@ int launch_stride(int type, const void* src, int nx, int ny, int nz, int s, void
   if (type == GT_F32)
     hipLaunchKernelGGL(stride_kernel<float>, dim3(nb), dim3(VDVC_BLOCK), 0, st,
                        (const float*)src, nx, ny, nz, s, (float*)dst, mx, my, mz);
+  else if (type == GT_F64)
+    hipLaunchKernelGGL(stride_kernel<double>, dim3(nb), dim3(VDVC_BLOCK), 0, st,
+                       (const double*)src, nx, ny, nz, s, (double*)dst, mx, my, mz);
   else
     hipLaunchKernelGGL(stride_kernel<__half>, dim3(nb), dim3(VDVC_BLOCK), 0, st,
                        (const __half*)src, nx, ny, nz, s, (__half*)dst, mx, my, mz);

@@ -18,6 +18,8 @@
 // (running sums), along z by carrying two node planes in registers, and
 // along x by a segmented reduction across the lanes of one element at the
 // end of every element layer.  Node partials go to the wave's slot block.
+#include <type_traits>
+
 #include "vdvc_kernels.h"
 
 
@@ -25,13 +27,13 @@
 // Emit one node plane of a wave: c[cy*3+k] is the lane's z-complete
 // y-restricted sum for node row cy; a0/a1 are the lane's x weights for the
 // owning element's lower/upper node.  Writes nn node slots.
-template <bool POW2>
-__device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float a1,
+template <bool POW2, typename R>
+__device__ __forceinline__ void emit_plane(const R (&c)[6], R a0, R a1,
                                            int lane, int h, int x0, int nact, int i_lo,
-                                           int nn, float* __restrict__ slot, int px,
-                                           float* lds /* 64*13 floats */) {
+                                           int nn, R* __restrict__ slot, int px,
+                                           R* lds /* 64*13 values */) {
   if (POW2) {
-    float s0[6], s1[6];
+    R s0[6], s1[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) { s0[q] = a0 * c[q]; s1[q] = a1 * c[q]; }
     for (int o = 1; o < h; o <<= 1) {
@@ -41,25 +43,25 @@ __device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float 
         s1[q] += __shfl_xor(s1[q], o, 64);
       }
     }
-    float pv[6];
+    R pv[6];
     const int src = lane > 0 ? lane - 1 : 0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) pv[q] = __shfl(s1[q], src, 64);
     if ((lane % h) == 0) {
       const int ln = lane / h;
       if (ln < nn) {
-        float* d = slot + (size_t)ln * 3;
+        R* d = slot + (size_t)ln * 3;
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
           for (int k = 0; k < 3; ++k)
-            d[(size_t)cy * px * 3 + k] = s0[cy * 3 + k] + (ln > 0 ? pv[cy * 3 + k] : 0.0f);
+            d[(size_t)cy * px * 3 + k] = s0[cy * 3 + k] + (ln > 0 ? pv[cy * 3 + k] : R(0));
       }
     }
     if (lane == 63) {
       const int ln = 64 / h;
       if (ln < nn) {
-        float* d = slot + (size_t)ln * 3;
+        R* d = slot + (size_t)ln * 3;
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -68,14 +70,15 @@ __device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float 
     }
   } else {
     // generic element size: stage lane values in LDS, node lanes sum them
-    float* L = lds;
+    // (slot 12 holds the lane's element index, an exactly representable R)
+    R* L = lds;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       L[lane * 13 + q] = a0 * c[q];
       L[lane * 13 + 6 + q] = a1 * c[q];
     }
     const int x = x0 + lane;
-    L[lane * 13 + 12] = __int_as_float(x / h - i_lo);
+    L[lane * 13 + 12] = (R)(x / h - i_lo);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -85,9 +88,9 @@ __device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float 
       int hi = (i + 1) * h - x0;
       lo = lo < 0 ? 0 : lo;
       hi = hi > nact - 1 ? nact - 1 : hi;
-      float v[6] = {0, 0, 0, 0, 0, 0};
+      R v[6] = {0, 0, 0, 0, 0, 0};
       for (int l = lo; l <= hi; ++l) {
-        const int el = __float_as_int(L[l * 13 + 12]);
+        const int el = (int)L[l * 13 + 12];
         if (el == ln) {
 #pragma unroll
           for (int q = 0; q < 6; ++q) v[q] += L[l * 13 + q];
@@ -96,7 +99,7 @@ __device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float 
           for (int q = 0; q < 6; ++q) v[q] += L[l * 13 + 6 + q];
         }
       }
-      float* d = slot + (size_t)ln * 3;
+      R* d = slot + (size_t)ln * 3;
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -110,18 +113,20 @@ __device__ __forceinline__ void emit_plane(const float (&c)[6], float a0, float 
 
 template <int MODE, typename GT, bool HASMASK, typename FT, bool POW2, bool FOLD>
 __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
-    MeshDev M, TilesDev T, VoxIn in, const float* __restrict__ v, float* __restrict__ P) {
-  __shared__ float lds_all[POW2 ? 1 : VDVC_WAVES_PER_BLOCK * 64 * 13];
+    MeshDev M, TilesDev T, VoxIn in, const typename RealOf<GT>::type* __restrict__ v,
+    typename RealOf<GT>::type* __restrict__ P) {
+  using R = typename RealOf<GT>::type;
+  __shared__ R lds_all[POW2 ? 1 : VDVC_WAVES_PER_BLOCK * 64 * 13];
   const int lane = threadIdx.x & 63;
-  const int wib = threadIdx.x >> 6;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const long long w = (long long)blk * VDVC_WAVES_PER_BLOCK + wib;
   if (w >= T.nwaves) return;
   if (in.done && *in.done) return;
-  float* lds = POW2 ? nullptr : lds_all + wib * 64 * 13;
+  R* lds = POW2 ? nullptr : lds_all + wib * 64 * 13;
 
   const int h = M.h;
-  const float inv_h = M.inv_h;
+  const R inv_h = R(1) / R(h);
   const int sx = (int)(w % T.nsx);
   const long long r = w / T.nsx;
   const int ey = (int)(r % M.Ny);
@@ -134,9 +139,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
   const int xe = act ? x : x0 + nact - 1;
   const int ex = xe / h;               // owning element (== Nx for the last plane)
   const int tx = xe - ex * h;
-  const float X1 = tx * inv_h;
-  const float X0 = 1.0f - X1;
-  const float Wx = act ? axis_weight(xe, M.Nvx - 1) : 0.0f;
+  const R X1 = tx * inv_h;
+  const R X0 = R(1) - X1;
+  const R Wx = act ? (R)axis_weight(xe, M.Nvx - 1) : R(0);
   const int i0 = min(ex, M.Nx);
   const int i1 = min(ex + 1, M.Nx);
   const int i_lo = x0 / h;
@@ -149,27 +154,30 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
   const int y0 = ey * h;
   const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
 
-  float a0w = X0, a1w = X1;
+  R a0w = X0, a1w = X1;
   if (MODE == MODE_DIAG) { a0w = X0 * X0; a1w = X1 * X1; }
 
-  float C[2][2][3];
+  R C[2][2][3];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) C[a][b][k] = 0.0f;
+      for (int k = 0; k < 3; ++k) C[a][b][k] = R(0);
 
   double st_c = 0.0, st_1 = 0.0, st_2 = 0.0, st_n = 0.0;
 
   const GT* g0 = reinterpret_cast<const GT*>(in.g[0]);
   const GT* g1 = reinterpret_cast<const GT*>(in.g[1]);
   const GT* g2 = reinterpret_cast<const GT*>(in.g[2]);
-  float* Pw = P + (size_t)w * T.wave_stride;
+  const R* gvals = reinterpret_cast<const R*>(in.gvals);
+  R* resv = reinterpret_cast<R*>(in.res);
+  const R mean0 = (R)in.mean0, gmean = (R)in.gmean, kscale = (R)in.kscale;
+  R* Pw = P + (size_t)w * T.wave_stride;
 
   for (int ez = ez0; ez < ez1; ++ez) {
     const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
-    float Ux[2][2][3];
+    R Ux[2][2][3];
     if (MODE == MODE_MATVEC) {
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
@@ -184,52 +192,51 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
     }
     for (int tz = 0; tz < nzp; ++tz) {
       const int z = ez * h + tz;
-      const float Z1 = tz * inv_h;
-      const float Z0 = 1.0f - Z1;
-      const float Wxz = Wx * axis_weight(z, M.Nvz - 1);
+      const R Z1 = tz * inv_h;
+      const R Z0 = R(1) - Z1;
+      const R Wxz = Wx * (R)axis_weight(z, M.Nvz - 1);
       const int mxz = mx * axis_mult(z, h, M.Nvz - 1);
-      float u0[3] = {0, 0, 0}, du[3] = {0, 0, 0};
+      R u0[3] = {0, 0, 0}, du[3] = {0, 0, 0};
       if (MODE == MODE_MATVEC) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           u0[k] = Z0 * Ux[0][0][k] + Z1 * Ux[0][1][k];
-          const float u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
+          const R u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
           du[k] = (u1 - u0[k]) * inv_h;
         }
       }
-      float A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+      R A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
       const long long base = ((long long)z * M.Nvy + y0) * M.pitch + xe;
 #pragma unroll 4
       for (int ty = 0; ty < nrows; ++ty) {
         const long long vi = base + (long long)ty * M.pitch;
         const int y = y0 + ty;
-        const float W = Wxz * axis_weight(y, M.Nvy - 1);
-        const float gx = ld_val(g0, vi);
-        const float gy = ld_val(g1, vi);
-        const float gz = ld_val(g2, vi);
-        float m = 1.0f;
-        if (HASMASK) m = in.mask[vi] ? 1.0f : 0.0f;
-        const float fty = (float)ty;
+        const R W = Wxz * (R)axis_weight(y, M.Nvy - 1);
+        const R gx = ld_val(g0, vi);
+        const R gy = ld_val(g1, vi);
+        const R gz = ld_val(g2, vi);
+        R m = R(1);
+        if (HASMASK) m = in.mask[vi] ? R(1) : R(0);
+        const R fty = (R)ty;
         if (MODE == MODE_MATVEC) {
-          const float ux = u0[0] + fty * du[0];
-          const float uy = u0[1] + fty * du[1];
-          const float uz = u0[2] + fty * du[2];
-          const float s = W * m * (gx * ux + gy * uy + gz * uz);
-          const float st = s * fty;
+          const R ux = u0[0] + fty * du[0];
+          const R uy = u0[1] + fty * du[1];
+          const R uz = u0[2] + fty * du[2];
+          const R s = W * m * (gx * ux + gy * uy + gz * uz);
+          const R st = s * fty;
           A[0] += s * gx; A[1] += s * gy; A[2] += s * gz;
           B[0] += st * gx; B[1] += st * gy; B[2] += st * gz;
         } else if (MODE == MODE_RHS) {
           const uint8_t bm = in.mask[vi];
-          float rv = 0.0f;
+          R rv = R(0);
           if (act) {
-            const float fval = ld_val(reinterpret_cast<const FT*>(in.fimg), vi);
-            const float gv = in.gvals[vi];
-            // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
-            // the reference's order, no contraction
-            rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, in.gmean), in.kscale),
-                           __fsub_rn(fval, in.mean0));
-            if (!bm) rv = 0.0f;
-            in.res[vi] = rv;
+            const R fval = ld_val(reinterpret_cast<const FT*>(in.fimg), vi);
+            const R gv = gvals[vi];
+            // (g - g_mean) * (-(std0/std1)) + (f - mean0), single rounded
+            // steps in the reference's order, no contraction
+            rv = add_rn(mul_rn(sub_rn(gv, gmean), kscale), sub_rn(fval, mean0));
+            if (!bm) rv = R(0);
+            resv[vi] = rv;
             if (in.live[vi]) {
               const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
               const double rd = (double)rv;
@@ -238,71 +245,71 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
               st_2 += mlt * rd * rd;
               st_n += 1.0;
             }
-            if (FOLD) {
+            if constexpr (FOLD) {
               const float fm = bm ? 1.0f : 0.0f;
-              in.fold_out[0][vi] = __bfloat16_as_ushort(__float2bfloat16(gx * fm));
-              in.fold_out[1][vi] = __bfloat16_as_ushort(__float2bfloat16(gy * fm));
-              in.fold_out[2][vi] = __bfloat16_as_ushort(__float2bfloat16(gz * fm));
+              in.fold_out[0][vi] = __bfloat16_as_ushort(__float2bfloat16((float)gx * fm));
+              in.fold_out[1][vi] = __bfloat16_as_ushort(__float2bfloat16((float)gy * fm));
+              in.fold_out[2][vi] = __bfloat16_as_ushort(__float2bfloat16((float)gz * fm));
             }
           }
-          const float s = W * (bm ? 1.0f : 0.0f) * rv;
-          const float st = s * fty;
+          const R s = W * (bm ? R(1) : R(0)) * rv;
+          const R st = s * fty;
           A[0] += s * gx; A[1] += s * gy; A[2] += s * gz;
           B[0] += st * gx; B[1] += st * gy; B[2] += st * gz;
         } else {  // MODE_DIAG
-          const float Y1 = fty * inv_h;
-          const float Y0 = 1.0f - Y1;
-          const float wm = W * m;
-          const float q0 = wm * gx * gx, q1 = wm * gy * gy, q2 = wm * gz * gz;
-          const float y0w = Y0 * Y0, y1w = Y1 * Y1;
+          const R Y1 = fty * inv_h;
+          const R Y0 = R(1) - Y1;
+          const R wm = W * m;
+          const R q0 = wm * gx * gx, q1 = wm * gy * gy, q2 = wm * gz * gz;
+          const R y0w = Y0 * Y0, y1w = Y1 * Y1;
           A[0] += y0w * q0; A[1] += y0w * q1; A[2] += y0w * q2;
           B[0] += y1w * q0; B[1] += y1w * q1; B[2] += y1w * q2;
         }
       }
-      float R[2][3];
+      R Rr[2][3];
       if (MODE == MODE_DIAG) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { R[0][k] = A[k]; R[1][k] = B[k]; }
+        for (int k = 0; k < 3; ++k) { Rr[0][k] = A[k]; Rr[1][k] = B[k]; }
       } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          R[1][k] = B[k] * inv_h;
-          R[0][k] = A[k] - R[1][k];
+          Rr[1][k] = B[k] * inv_h;
+          Rr[0][k] = A[k] - Rr[1][k];
         }
       }
-      float zw0 = Z0, zw1 = Z1;
+      R zw0 = Z0, zw1 = Z1;
       if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          C[cy][0][k] += zw0 * R[cy][k];
-          C[cy][1][k] += zw1 * R[cy][k];
+          C[cy][0][k] += zw0 * Rr[cy][k];
+          C[cy][1][k] += zw1 * Rr[cy][k];
         }
     }
     // node plane kz = ez is complete for this wave: emit and shift
     {
-      float c[6];
+      R c[6];
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int k = 0; k < 3; ++k) c[cy * 3 + k] = C[cy][0][k];
-      emit_plane<POW2>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
-                       Pw + (size_t)(ez - ez0) * 2 * T.px * 3, T.px, lds);
+      emit_plane<POW2, R>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
+                          Pw + (size_t)(ez - ez0) * 2 * T.px * 3, T.px, lds);
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { C[cy][0][k] = C[cy][1][k]; C[cy][1][k] = 0.0f; }
+        for (int k = 0; k < 3; ++k) { C[cy][0][k] = C[cy][1][k]; C[cy][1][k] = R(0); }
     }
   }
   {
-    float c[6];
+    R c[6];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
       for (int k = 0; k < 3; ++k) c[cy * 3 + k] = C[cy][0][k];
-    emit_plane<POW2>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
-                     Pw + (size_t)(ez1 - ez0) * 2 * T.px * 3, T.px, lds);
+    emit_plane<POW2, R>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
+                        Pw + (size_t)(ez1 - ez0) * 2 * T.px * 3, T.px, lds);
   }
   if (MODE == MODE_RHS) {
     st_c = wave_sum(st_c);
@@ -322,12 +329,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
 // deformed-image sampling g(x + u(x)) + active mask + statistics over b_mask
 // ---------------------------------------------------------------------------
 
-template <typename GI>
+template <typename GI, typename C>
 __device__ __forceinline__ double sample_tri(const GI* __restrict__ img, int nx, int ny, int nz,
-                                             float cx, float cy, float cz) {
+                                             C cx, C cy, C cz) {
   // order-1 interpolation at coordinates inside [0, n-1], upper neighbour
   // clamped ('nearest'), weights in double
-  const int ix = (int)floorf(cx), iy = (int)floorf(cy), iz = (int)floorf(cz);
+  const int ix = (int)floor(cx), iy = (int)floor(cy), iz = (int)floor(cz);
   const int jx = min(ix + 1, nx - 1), jy = min(iy + 1, ny - 1), jz = min(iz + 1, nz - 1);
   const double fx = (double)cx - ix, fy = (double)cy - iy, fz = (double)cz - iz;
   const long long sy = nx, sz = (long long)nx * ny;
@@ -342,16 +349,24 @@ __device__ __forceinline__ double sample_tri(const GI* __restrict__ img, int nx,
          fz * (gy0 * (gx0 * v001 + fx * v101) + fy * (gx0 * v011 + fx * v111));
 }
 
+template <typename R>
+__device__ __forceinline__ R clamp_to(R c, R hi) {
+  return c < R(0) ? R(0) : (c > hi ? hi : c);
+}
+
+// The working type R is double for an FP64 image (the FP64 path) and float
+// otherwise; the coordinate composition and threshold test run in R.
 template <typename GI>
-__global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev T, SampleIn S,
-                                                            const float* __restrict__ U) {
+__global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
+    MeshDev M, TilesDev T, SampleIn S, const typename RealOf<GI>::type* __restrict__ U) {
+  using R = typename RealOf<GI>::type;
   const int lane = threadIdx.x & 63;
-  const int wib = threadIdx.x >> 6;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const long long w = (long long)blk * VDVC_WAVES_PER_BLOCK + wib;
   if (w >= T.nwaves) return;
   const int h = M.h;
-  const float inv_h = M.inv_h;
+  const R inv_h = R(1) / R(h);
   const int sx = (int)(w % T.nsx);
   const long long r = w / T.nsx;
   const int ey = (int)(r % M.Ny);
@@ -362,19 +377,21 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev 
   const int xe = act ? x0 + lane : x0 + nact - 1;
   const int ex = xe / h;
   const int tx = xe - ex * h;
-  const float X1 = tx * inv_h, X0 = 1.0f - X1;
+  const R X1 = tx * inv_h, X0 = R(1) - X1;
   const int i0 = min(ex, M.Nx), i1 = min(ex + 1, M.Nx);
   const int mx = axis_mult(xe, h, M.Nvx - 1);
   const int ez0 = zt * T.ez, ez1 = min(M.Nz, ez0 + T.ez);
   const int y0 = ey * h;
   const int nrows = h + (ey == M.Ny - 1 ? 1 : 0);
-  const float fgnx = (float)(S.gnx - 1), fgny = (float)(S.gny - 1), fgnz = (float)(S.gnz - 1);
+  const R fgnx = (R)(S.gnx - 1), fgny = (R)(S.gny - 1), fgnz = (R)(S.gnz - 1);
+  const R gox = (R)S.gox, goy = (R)S.goy, goz = (R)S.goz, glt = (R)S.glt;
   const GI* img = reinterpret_cast<const GI*>(S.gimg);
-  const float vxf = (float)xe;
+  R* gvals = reinterpret_cast<R*>(S.gvals);
+  const R vxf = (R)xe;
   double sc = 0.0, s1 = 0.0, s2 = 0.0;
   for (int ez = ez0; ez < ez1; ++ez) {
     const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
-    float Ux[2][2][3];
+    R Ux[2][2][3];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -386,39 +403,43 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev 
       }
     for (int tz = 0; tz < nzp; ++tz) {
       const int z = ez * h + tz;
-      const float Z1 = tz * inv_h, Z0 = 1.0f - Z1;
+      const R Z1 = tz * inv_h, Z0 = R(1) - Z1;
       const int mxz = mx * axis_mult(z, h, M.Nvz - 1);
-      float u0[3], du[3];
+      R u0[3], du[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         u0[k] = Z0 * Ux[0][0][k] + Z1 * Ux[0][1][k];
-        const float u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
+        const R u1 = Z0 * Ux[1][0][k] + Z1 * Ux[1][1][k];
         du[k] = (u1 - u0[k]) * inv_h;
       }
       const long long base = ((long long)z * M.Nvy + y0) * M.pitch + xe;
       for (int ty = 0; ty < nrows; ++ty) {
         const long long vi = base + (long long)ty * M.pitch;
         const int y = y0 + ty;
-        const float fty = (float)ty;
-        const float ux = u0[0] + fty * du[0];
-        const float uy = u0[1] + fty * du[1];
-        const float uz = u0[2] + fty * du[2];
-        // (u + x) - origin, float32, as the reference composes it
-        float cx = __fsub_rn(__fadd_rn(ux, vxf), S.gox);
-        float cy = __fsub_rn(__fadd_rn(uy, (float)y), S.goy);
-        float cz = __fsub_rn(__fadd_rn(uz, (float)z), S.goz);
-        const bool inb = (cx >= 0.0f) && (cx <= fgnx) && (cy >= 0.0f) && (cy <= fgny) &&
-                         (cz >= 0.0f) && (cz <= fgnz);
-        cx = fminf(fmaxf(cx, 0.0f), fgnx);
-        cy = fminf(fmaxf(cy, 0.0f), fgny);
-        cz = fminf(fmaxf(cz, 0.0f), fgnz);
-        // a NaN coordinate clamps to 0 here (NumPy would sample NaN); it is
-        // out of bounds either way, so the mask is false
-        const float gv = (float)sample_tri(img, S.gnx, S.gny, S.gnz, cx, cy, cz);
+        const R fty = (R)ty;
+        const R ux = u0[0] + fty * du[0];
+        const R uy = u0[1] + fty * du[1];
+        const R uz = u0[2] + fty * du[2];
+        // (u + x) - origin, in R, as the reference composes it
+        R cx = sub_rn(add_rn(ux, vxf), gox);
+        R cy = sub_rn(add_rn(uy, (R)y), goy);
+        R cz = sub_rn(add_rn(uz, (R)z), goz);
+        const bool inb = (cx >= R(0)) && (cx <= fgnx) && (cy >= R(0)) && (cy <= fgny) &&
+                         (cz >= R(0)) && (cz <= fgnz);
+        cx = clamp_to(cx, fgnx);
+        cy = clamp_to(cy, fgny);
+        cz = clamp_to(cz, fgnz);
+        // a NaN coordinate clamps to a NaN floor index in NumPy; here it
+        // samples a clamped voxel.  It is out of bounds either way, so the
+        // mask is false.
+        if (!(cx == cx)) cx = R(0);
+        if (!(cy == cy)) cy = R(0);
+        if (!(cz == cz)) cz = R(0);
+        const R gv = (R)sample_tri(img, S.gnx, S.gny, S.gnz, cx, cy, cz);
         const bool m0 = S.mask0 ? (S.mask0[vi] != 0) : true;
-        const bool mk = inb && m0 && (gv > S.glt);
+        const bool mk = inb && m0 && (gv > glt);
         if (act) {
-          S.gvals[vi] = gv;
+          gvals[vi] = gv;
           S.mask_out[vi] = mk ? 1 : 0;
           const bool bm = S.bmask ? (S.bmask[vi] != 0) : mk;
           if (bm) {
@@ -446,12 +467,20 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(MeshDev M, TilesDev 
 // reference-image setup: np.gradient (edge_order=1) per axis, mask0 = f > glt,
 // gradient zeroed where the 6-cross erosion of mask0 ('nearest') is false
 // ---------------------------------------------------------------------------
+template <typename GO>
+__device__ __forceinline__ void st_grad(GO* p, long long i, typename RealOf<GO>::type v) {
+  if constexpr (std::is_same<GO, __hip_bfloat16>::value) p[i] = __float2bfloat16(v);
+  else p[i] = v;
+}
+
+// R is double for the FP64 path (double image and gradients), else float
 template <typename FT, typename GO>
 __global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* __restrict__ f,
-                                                           float glt, GO* __restrict__ gxo,
+                                                           double glt_d, GO* __restrict__ gxo,
                                                            GO* __restrict__ gyo,
                                                            GO* __restrict__ gzo,
                                                            uint8_t* __restrict__ mask0) {
+  using R = typename RealOf<GO>::type;
   const int nxb = (M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long row = blockIdx.x / nxb;
   const int x = (blockIdx.x % nxb) * VDVC_BLOCK + threadIdx.x;
@@ -459,44 +488,33 @@ __global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* 
   const long long vi = row * M.pitch + x;
   if (x >= M.Nvx) {  // row padding: zero gradients and mask
     mask0[vi] = 0;
-    if constexpr (sizeof(GO) == 4) {
-      gxo[vi] = 0.0f; gyo[vi] = 0.0f; gzo[vi] = 0.0f;
-    } else {
-      gxo[vi] = __float2bfloat16(0.0f);
-      gyo[vi] = __float2bfloat16(0.0f);
-      gzo[vi] = __float2bfloat16(0.0f);
-    }
+    st_grad(gxo, vi, R(0)); st_grad(gyo, vi, R(0)); st_grad(gzo, vi, R(0));
     return;
   }
+  const R glt = (R)glt_d;
   const int y = (int)(row % M.Nvy);
   const int z = (int)(row / M.Nvy);
   const long long sy = M.pitch, sz = (long long)M.pitch * M.Nvy;
-  const float fc = ld_val(f, vi);
+  const R fc = ld_val(f, vi);
   const int xm = x > 0 ? x - 1 : 0, xp = x < M.Nvx - 1 ? x + 1 : x;
   const int ym = y > 0 ? y - 1 : 0, yp = y < M.Nvy - 1 ? y + 1 : y;
   const int zm = z > 0 ? z - 1 : 0, zp = z < M.Nvz - 1 ? z + 1 : z;
-  const float fxm = ld_val(f, vi + (xm - x)), fxp = ld_val(f, vi + (xp - x));
-  const float fym = ld_val(f, vi + (long long)(ym - y) * sy), fyp = ld_val(f, vi + (long long)(yp - y) * sy);
-  const float fzm = ld_val(f, vi + (long long)(zm - z) * sz), fzp = ld_val(f, vi + (long long)(zp - z) * sz);
-  auto grad1 = [](float fm, float fp, int t, int n) -> float {
-    if (t == 0 || t == n - 1) return __fsub_rn(fp, fm);  // one-sided, spacing 1
-    return __fsub_rn(fp, fm) * 0.5f;                    // central
+  const R fxm = ld_val(f, vi + (xm - x)), fxp = ld_val(f, vi + (xp - x));
+  const R fym = ld_val(f, vi + (long long)(ym - y) * sy), fyp = ld_val(f, vi + (long long)(yp - y) * sy);
+  const R fzm = ld_val(f, vi + (long long)(zm - z) * sz), fzp = ld_val(f, vi + (long long)(zp - z) * sz);
+  auto grad1 = [](R fm, R fp, int t, int n) -> R {
+    if (t == 0 || t == n - 1) return sub_rn(fp, fm);  // one-sided, spacing 1
+    return sub_rn(fp, fm) * R(0.5);                  // central
   };
-  float gx = grad1(fxm, fxp, x, M.Nvx);
-  float gy = grad1(fym, fyp, y, M.Nvy);
-  float gz = grad1(fzm, fzp, z, M.Nvz);
+  R gx = grad1(fxm, fxp, x, M.Nvx);
+  R gy = grad1(fym, fyp, y, M.Nvy);
+  R gz = grad1(fzm, fzp, z, M.Nvz);
   const bool m0 = fc > glt;
   const bool er = m0 && (fxm > glt) && (fxp > glt) && (fym > glt) && (fyp > glt) && (fzm > glt) &&
                   (fzp > glt);
-  if (!er) { gx = 0.0f; gy = 0.0f; gz = 0.0f; }
+  if (!er) { gx = R(0); gy = R(0); gz = R(0); }
   mask0[vi] = m0 ? 1 : 0;
-  if constexpr (sizeof(GO) == 4) {
-    gxo[vi] = gx; gyo[vi] = gy; gzo[vi] = gz;
-  } else {
-    gxo[vi] = __float2bfloat16(gx);
-    gyo[vi] = __float2bfloat16(gy);
-    gzo[vi] = __float2bfloat16(gz);
-  }
+  st_grad(gxo, vi, gx); st_grad(gyo, vi, gy); st_grad(gzo, vi, gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -506,9 +524,11 @@ __global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* 
 template <typename FT>
 __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* __restrict__ fld,
                                                            const uint8_t* __restrict__ mask,
-                                                           int use_thresh, float glt,
+                                                           int use_thresh, double glt_d,
                                                            double* __restrict__ part) {
+  using R = typename RealOf<FT>::type;
   __shared__ double lds[4 * 3];
+  const R glt = (R)glt_d;
   const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long nrows = (long long)M.Nvy * M.Nvz;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -520,7 +540,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
       const int y = (int)(row % M.Nvy);
       const int z = (int)(row / M.Nvy);
       const long long vi = row * M.pitch + x;
-      const float val = ld_val(fld, vi);
+      const R val = ld_val(fld, vi);
       const bool m = use_thresh ? (val > glt) : (mask ? mask[vi] != 0 : true);
       if (m) {
         const double mlt = (double)(axis_mult(x, M.h, M.Nvx - 1) * axis_mult(y, M.h, M.Nvy - 1) *
@@ -597,10 +617,11 @@ __global__ __launch_bounds__(VDVC_BLOCK) void erode6_kernel(int nx, int ny, int 
   out[q] = v ? 1 : 0;
 }
 
-// out = a * b elementwise (float32)
-__global__ __launch_bounds__(VDVC_BLOCK) void mul_kernel(long long n, const float* __restrict__ a,
-                                                         const float* __restrict__ b,
-                                                         float* __restrict__ out) {
+// out = a * b elementwise
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void mul_kernel(long long n, const R* __restrict__ a,
+                                                         const R* __restrict__ b,
+                                                         R* __restrict__ out) {
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   if (q < n) out[q] = a[q] * b[q];
 }
@@ -608,16 +629,26 @@ __global__ __launch_bounds__(VDVC_BLOCK) void mul_kernel(long long n, const floa
 // ---------------------------------------------------------------------------
 // host-side launchers (C++ linkage, used by the C-ABI layer)
 // ---------------------------------------------------------------------------
+// one instantiation of the generic voxel kernel; v and P are in its
+// working type
+template <int MODE, typename GT, bool HM, typename FT, bool P2, bool FO>
+void launch_restrict_one(dim3 g, dim3 b, hipStream_t st, const MeshDev& M, const TilesDev& T,
+                         const VoxIn& in, const void* v, void* P) {
+  using R = typename RealOf<GT>::type;
+  hipLaunchKernelGGL((vox_restrict_kernel<MODE, GT, HM, FT, P2, FO>), g, b, 0, st, M, T, in,
+                     static_cast<const R*>(v), static_cast<R*>(P));
+}
+
 namespace vdvc {
 
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
-                    const TilesDev& T, const VoxIn& in, const float* v, float* P,
+                    const TilesDev& T, const VoxIn& in, const void* vv, void* Pv,
                     hipStream_t st) {
   const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
   const bool pow2 = (M.h <= 64) && ((64 % M.h) == 0);
   dim3 g(nblk), b(VDVC_BLOCK);
 #define VDVC_L(MODE, GT, HM, FT, P2, FO) \
-  hipLaunchKernelGGL((vox_restrict_kernel<MODE, GT, HM, FT, P2, FO>), g, b, 0, st, M, T, in, v, P)
+  launch_restrict_one<MODE, GT, HM, FT, P2, FO>(g, b, st, M, T, in, vv, Pv)
 #define VDVC_P2(MODE, GT, HM, FT, FO) \
   do { if (pow2) VDVC_L(MODE, GT, HM, FT, true, FO); else VDVC_L(MODE, GT, HM, FT, false, FO); } while (0)
 #define VDVC_HM(MODE, GT, FT, FO) \
@@ -625,14 +656,19 @@ int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, con
   if (mode == MODE_MATVEC || mode == MODE_DIAG) {
     if (mode == MODE_MATVEC) {
       if (gtype == GT_F32) VDVC_HM(MODE_MATVEC, float, float, false);
+      else if (gtype == GT_F64) VDVC_HM(MODE_MATVEC, double, double, false);
       else VDVC_HM(MODE_MATVEC, __hip_bfloat16, float, false);
     } else {
       if (gtype == GT_F32) VDVC_HM(MODE_DIAG, float, float, false);
+      else if (gtype == GT_F64) VDVC_HM(MODE_DIAG, double, double, false);
       else VDVC_HM(MODE_DIAG, __hip_bfloat16, float, false);
     }
   } else {
     // RHS always reads the b_mask explicitly
-    if (gtype == GT_F32) {
+    if (gtype == GT_F64) {
+      if (ftype != GT_F64 || fold) return (int)hipErrorInvalidValue;
+      VDVC_P2(MODE_RHS, double, true, double, false);
+    } else if (gtype == GT_F32) {
       if (ftype == GT_F32) {
         if (fold) VDVC_P2(MODE_RHS, float, true, float, true);
         else VDVC_P2(MODE_RHS, float, true, float, false);
@@ -657,22 +693,32 @@ int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, con
 }
 
 int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,
-                  const float* U, hipStream_t st) {
+                  const void* U, hipStream_t st) {
   const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
   if (gitype == GT_F32)
-    hipLaunchKernelGGL(sample_kernel<float>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S, U);
+    hipLaunchKernelGGL(sample_kernel<float>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
+                       static_cast<const float*>(U));
+  else if (gitype == GT_F64)
+    hipLaunchKernelGGL(sample_kernel<double>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
+                       static_cast<const double*>(U));
   else
-    hipLaunchKernelGGL(sample_kernel<__half>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S, U);
+    hipLaunchKernelGGL(sample_kernel<__half>, dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
+                       static_cast<const float*>(U));
   return (int)hipGetLastError();
 }
 
-int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, float glt, void* gx,
+int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, double glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st) {
   const int nxb = (M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long nb = (long long)nxb * M.Nvy * M.Nvz;
   if (nb > 0x7fffffffLL) return (int)hipErrorInvalidValue;
   dim3 g((unsigned)nb), b(VDVC_BLOCK);
-  if (ftype == GT_F32) {
+  if (ftype == GT_F64 || gtype == GT_F64) {
+    // the FP64 path: double image, double gradients
+    if (ftype != GT_F64 || gtype != GT_F64) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL((setup_kernel<double, double>), g, b, 0, st, M, (const double*)f, glt,
+                       (double*)gx, (double*)gy, (double*)gz, mask0);
+  } else if (ftype == GT_F32) {
     if (gtype == GT_F32)
       hipLaunchKernelGGL((setup_kernel<float, float>), g, b, 0, st, M, (const float*)f, glt,
                          (float*)gx, (float*)gy, (float*)gz, mask0);
@@ -691,10 +737,13 @@ int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, float gl
 }
 
 int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
-                 float glt, double* part, int nblocks, hipStream_t st) {
+                 double glt, double* part, int nblocks, hipStream_t st) {
   if (ftype == GT_F32)
     hipLaunchKernelGGL(stats_kernel<float>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
                        (const float*)fld, mask, use_thresh, glt, part);
+  else if (ftype == GT_F64)
+    hipLaunchKernelGGL(stats_kernel<double>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
+                       (const double*)fld, mask, use_thresh, glt, part);
   else
     hipLaunchKernelGGL(stats_kernel<__half>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
                        (const __half*)fld, mask, use_thresh, glt, part);
@@ -715,11 +764,14 @@ int launch_erode6(int nx, int ny, int nz, const uint8_t* in, uint8_t* out, hipSt
   return (int)hipGetLastError();
 }
 
-int launch_mul(long long n, const float* a, const float* b, float* out, hipStream_t st) {
+template <typename R>
+int launch_mul(long long n, const R* a, const R* b, R* out, hipStream_t st) {
   const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
-  hipLaunchKernelGGL(mul_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, a, b, out);
+  hipLaunchKernelGGL(mul_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, a, b, out);
   return (int)hipGetLastError();
 }
+template int launch_mul<float>(long long, const float*, const float*, float*, hipStream_t);
+template int launch_mul<double>(long long, const double*, const double*, double*, hipStream_t);
 
 int launch_reduce(const double* part, long long n, int nv, double* out, hipStream_t st) {
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, part, n, nv, out);

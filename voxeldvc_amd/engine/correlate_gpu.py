@@ -11,7 +11,10 @@ Precision (the `storage` keyword, an extension):
           Hessian matvec), float32 matvec arithmetic, FP64 dots and norms;
   'bf16'  float16 images (exact for 8-bit data), bfloat16 gradients with the
           mask folded in (6 B/voxel), same arithmetic; exact for 8-bit
-          integer images, whose gradients are multiples of 1/2 below 256.
+          integer images, whose gradients are multiples of 1/2 below 256;
+  'fp64'  selected by dtype=np.float64 (the reference's oracle mode,
+          correlate_gpu.py:910-914): float64 images, gradients, nodal
+          vectors, residuals and PCG, through the generic voxel kernel.
 """
 import math
 
@@ -42,20 +45,29 @@ def _stats_mean_std(s):
 # per-voxel helpers (correlate_gpu.py:49-378)
 # ---------------------------------------------------------------------------
 
+def _dtype_code(dtype):
+    """('float32', F32) or ('float64', F64) for a reference `dtype` argument."""
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return 'float32', L.F32
+    if dt == np.float64:
+        return 'float64', L.F64
+    raise NotImplementedError('the device solve computes in float32 or float64, not %s' % dt)
+
+
 def image_grad_and_values(pix, xp=np, dtype=np.float32):
     """(f_vox (Nvox,), grad_f_vox (Nvox, 3)) in the reference layout
     (correlate_gpu.py:49-72).  No mask: glt = -inf keeps every gradient."""
-    if np.dtype(dtype) != np.float32:
-        raise NotImplementedError('device path computes float32 gradients')
+    wd, code = _dtype_code(dtype)
     t = L.require_gpu()
-    fd, shp = L.to_dev_volume(pix)
+    fd, shp = L.to_dev_volume(pix, wd)
     f = L.pitch_dense(fd, shp)
-    g = [L.zeros(f.numel()) for _ in range(3)]
+    g = [L.zeros(f.numel(), wd) for _ in range(3)]
     m = L.zeros(f.numel(), 'uint8')
     # image_setup works on any (nx,ny,nz) volume through a mesh of n-1
     # elements per axis with h = 1
-    L.call('vdvc_image_setup', L.ctx(), shp[0] - 1, shp[1] - 1, shp[2] - 1, 1, L.ptr(f), L.F32,
-           float('-inf'), L.ptr(g[0]), L.ptr(g[1]), L.ptr(g[2]), L.F32, L.ptr(m))
+    L.call('vdvc_image_setup', L.ctx(), shp[0] - 1, shp[1] - 1, shp[2] - 1, 1, L.ptr(f), code,
+           float('-inf'), L.ptr(g[0]), L.ptr(g[1]), L.ptr(g[2]), code, L.ptr(m))
     grad = t.stack([L.unpitch(q, shp) for q in g], dim=1)
     return _as_out(xp, fd), _as_out(xp, grad)
 
@@ -72,10 +84,11 @@ def masked_quadpoint_mean_std(field_vox, mask_vox, h, Nx, Ny, Nz, xp=np, chunk_s
     """Multiplicity-weighted masked mean / population std, FP64
     (correlate_gpu.py:113-157)."""
     shp = K.level_shape(h, Nx, Ny, Nz)
-    f = L.to_dev_level(field_vox, shp)
+    wd, code = K.work_dtype(field_vox)
+    f = L.to_dev_level(field_vox, shp, wd)
     m = L.to_dev_level(mask_vox, shp, 'uint8')
     s = (L._d * 3)()
-    L.call('vdvc_masked_stats', L.ctx(), Nx, Ny, Nz, h, L.ptr(f), L.F32, L.ptr(m), 0, 0.0, s)
+    L.call('vdvc_masked_stats', L.ctx(), Nx, Ny, Nz, h, L.ptr(f), code, L.ptr(m), 0, 0.0, s)
     return _stats_mean_std(s)
 
 
@@ -85,13 +98,14 @@ def sample_deformed_field(U, g_pix_x, N_stencil, h, Nx, Ny, Nz, xp=np, g_origin=
     voxel (correlate_gpu.py:201-281)."""
     shp = K.level_shape(h, Nx, Ny, Nz)
     Np = L.pitch_of(shp[0]) * shp[1] * shp[2]
-    dU = L.to_dev_flat(U)
-    g, gshp = L.to_dev_volume(g_pix_x)
+    wd, code = K.work_dtype(U)
+    dU = L.to_dev_flat(U, wd)
+    g, gshp = L.to_dev_volume(g_pix_x, wd)
     m0 = None if mask0_vox is None else L.to_dev_level(mask0_vox, shp, 'uint8')
-    gv = L.zeros(Np)
+    gv = L.zeros(Np, wd)
     mk = L.zeros(Np, 'uint8')
     s = (L._d * 3)()
-    L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), L.F32, gshp[0], gshp[1],
+    L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), code, gshp[0], gshp[1],
            gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]), float(glt),
            L.ptr(m0), None, L.ptr(gv), L.ptr(mk), s)
     gv = L.unpitch(gv, shp)
@@ -105,17 +119,18 @@ def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny
     """b = sum_q w_q m_q res_q v_eq (correlate_gpu.py:341-378)."""
     t = L.require_gpu()
     dev = L.is_tensor(res_vox) or L.is_tensor(grad_f_vox)
+    wd, code = K.work_dtype(res_vox, grad_f_vox)
     shp = K.level_shape(h, Nx, Ny, Nz)
-    gx, gy, gz = K.grad_planes(grad_f_vox, shp)
-    r = L.to_dev_level(res_vox, shp)
+    gx, gy, gz = K.grad_planes(grad_f_vox, shp, wd)
+    r = L.to_dev_level(res_vox, shp, wd)
     m = L.to_dev_level(mask_vox, shp, 'uint8')
     zero = t.zeros_like(r)
     rout = t.empty_like(r)
-    b = L.empty(Ndof)
+    b = L.empty(Ndof, wd)
     s = (L._d * 4)()
     # res = ((0 - 0) * 0 + (res - 0)) reproduces res exactly
-    L.call('vdvc_residual_rhs', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
-           L.ptr(r), L.F32, 0.0, L.ptr(zero), 0.0, 0.0, L.ptr(m), L.ptr(m), L.ptr(rout), L.ptr(b),
+    L.call('vdvc_residual_rhs', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), code,
+           L.ptr(r), code, 0.0, L.ptr(zero), 0.0, 0.0, L.ptr(m), L.ptr(m), L.ptr(rout), L.ptr(b),
            None, None, None, s)
     return b if dev else L.to_host(b)
 
@@ -139,8 +154,10 @@ def _crop_dev(vol, shp, crop):
 
 def _fft_shift_dev(f, fshp, g, gshp):
     shape = tuple(min(a, b) for a, b in zip(fshp, gshp))
-    fc = _crop_dev(f, fshp, shape)
-    gc = _crop_dev(g, gshp, shape)
+    # the phase correlation reads float32 volumes (an argmax of a float64
+    # spectrum either way)
+    fc = _crop_dev(f, fshp, shape).float()
+    gc = _crop_dev(g, gshp, shape).float()
     idx = (L._i64 * 1)()
     L.call('vdvc_phase_corr', L.ctx(), L.ptr(fc), L.ptr(gc), shape[0], shape[1], shape[2], idx)
     peak = _unravel_c(int(idx[0]), shape)
@@ -226,7 +243,8 @@ def _stride_dev(vol, shp, s, type_=L.F32):
     if s == 1:
         return vol, tuple(shp)
     out_shp = tuple((n - 1) // s + 1 for n in shp)
-    out = L.empty(out_shp[0] * out_shp[1] * out_shp[2], 'float32' if type_ == L.F32 else 'float16')
+    name = {L.F32: 'float32', L.F16: 'float16', L.F64: 'float64'}[type_]
+    out = L.empty(out_shp[0] * out_shp[1] * out_shp[2], name)
     L.call('vdvc_stride', L.ctx(), L.ptr(vol), type_, shp[0], shp[1], shp[2], s, L.ptr(out))
     return out, out_shp
 
@@ -260,27 +278,44 @@ class _Reg:
         return (self.kind, self.kp(self.K1), self.kp(self.K2), self.E, self.eps)
 
     def matvec(self, M, v, out):
-        """out = R v (without ll)."""
+        """out = R v (without ll), in the working type of v."""
         c = L.ctx()
+        rt = L.rtype_of(v)
         if self.kind == L.REG_K:
-            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], self.kp(self.K1), self.E, None,
+            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], rt, self.kp(self.K1), self.E, None,
                    L.ptr(v), L.ptr(out))
         else:
-            tmp = L.empty(out.numel())
-            L.call('vdvc_matvec_eqgap', c, M[0], M[1], M[2], self.kp(self.K1), self.E, None,
+            tmp = L.torch().empty_like(out)
+            L.call('vdvc_matvec_eqgap', c, M[0], M[1], M[2], rt, self.kp(self.K1), self.E, None,
                    L.ptr(v), L.ptr(out))
-            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], self.kp(self.K2), self.E, None,
+            L.call('vdvc_matvec_K', c, M[0], M[1], M[2], rt, self.kp(self.K2), self.E, None,
                    L.ptr(v), L.ptr(tmp))
             K.axpby(self.eps, tmp, 1.0, out)
         return out
 
 
 def _grad_store(storage):
+    """(gradient code, gradient dtype, image code, image dtype) of a storage."""
     if storage == 'fp32':
         return L.F32, 'float32', L.F32, 'float32'
     if storage == 'bf16':
         return L.BF16, 'bfloat16', L.F16, 'float16'
-    raise ValueError("storage must be 'fp32' or 'bf16'")
+    if storage == 'fp64':
+        return L.F64, 'float64', L.F64, 'float64'
+    raise ValueError("storage must be 'fp32', 'bf16' or 'fp64'")
+
+
+def _storage_for(dtype, storage):
+    """dtype=float64 is the FP64 path (the reference's oracle mode); float32
+    takes the requested storage form."""
+    wd, _ = _dtype_code(dtype)
+    if wd == 'float64':
+        if storage not in ('fp32', 'fp64'):
+            raise ValueError("dtype=float64 computes in FP64; storage=%r needs float32" % storage)
+        return 'fp64'
+    if storage == 'fp64':
+        raise ValueError("storage='fp64' needs dtype=float64")
+    return storage
 
 
 def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.0, l0=None,
@@ -298,11 +333,10 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
     if Nvox > np.iinfo(np.int32).max or Ndof > np.iinfo(np.int32).max:
         raise ValueError("Nvox=%d / Ndof=%d exceed int32 range; matrix-free index "
                          "arrays assume int32 indices" % (Nvox, Ndof))
-    if np.dtype(dtype) != np.float32:
-        raise NotImplementedError('the device solve computes in float32 with FP64 reductions; '
-                                  'use dtype=np.float32')
-    f, fshp = L.to_dev_volume(f_pix, 'float32')
-    g, gshp = L.to_dev_volume(g_pix, 'float32')
+    storage = _storage_for(dtype, storage)
+    wd = 'float64' if storage == 'fp64' else 'float32'
+    f, fshp = L.to_dev_volume(f_pix, wd)
+    g, gshp = L.to_dev_volume(g_pix, wd)
     if tuple(fshp) != (Nvx, Nvy, Nvz):
         raise ValueError('f_pix shape %r != (Nx*h+1, Ny*h+1, Nz*h+1) = %r'
                          % (tuple(fshp), (Nvx, Nvy, Nvz)))
@@ -311,7 +345,8 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
                           reg_exponent=reg_exponent, c_reg=c_reg, reg_type=reg_type,
                           K_ref_elastic=K_ref_elastic, eps_reg=eps_reg,
                           dynamic_mask=dynamic_mask, freeze_mask_after=freeze_mask_after,
-                          U0=None if U0 is None else L.to_dev_flat(U0), maxiter=maxiter, eps=eps,
+                          U0=None if U0 is None else L.to_dev_flat(U0, wd), maxiter=maxiter,
+                          eps=eps,
                           pcg_tol=pcg_tol, pcg_max_iter=pcg_max_iter, disp=disp, log=log,
                           pcg_check_every=pcg_check_every)
     return _as_out(xp, U), _as_out(xp, res), _as_out(xp, dU)
@@ -330,12 +365,14 @@ class _Level:
         self.Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
         self.storage = storage
         self.gtype, gname, self.itype, iname = _grad_store(storage)
+        # working type of nodal vectors, sampled values and residuals
+        self.wd = 'float64' if storage == 'fp64' else 'float32'
         t = L.require_gpu()
         # images in the storage type (float16 is exact for 8-bit data); the
         # reference volume in the pitched level layout
         fp = L.pitch_dense(f, self.shape)
-        self.f = fp if iname == 'float32' else fp.to(t.float16)
-        self.g = g if iname == 'float32' else g.to(t.float16)
+        self.f = fp.to(getattr(t, iname))
+        self.g = g.to(getattr(t, iname))
         self.gshp = tuple(gshp)
         self.gname = gname
 
@@ -357,7 +394,12 @@ class _Level:
         L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, None, 1,
                float(glt), s3)
         mean0, std0 = _stats_mean_std(s3)
-        mean0_f = float(np.float32(mean0))
+        wd = self.wd
+        tdt = getattr(t, wd)
+        ndt = np.float64 if wd == 'float64' else np.float32
+        # scalars enter the element-wise steps rounded to the working type
+        rnd = (lambda x: float(x)) if wd == 'float64' else (lambda x: float(np.float32(x)))
+        mean0_f = rnd(mean0)
 
         if reg_type not in ('laplacian', 'equilibrium_gap'):
             raise ValueError("reg_type must be 'laplacian' or 'equilibrium_gap'")
@@ -377,10 +419,10 @@ class _Level:
 
         hutch = None
         if reg.kind == L.REG_EQGAP:
-            hutch = L.empty(Ndof)
+            hutch = L.empty(Ndof, wd)
             pr = L.to_dev_flat(K.rademacher_probes(Ndof).ravel(), 'int8')
-            L.call('vdvc_hutchinson_eqgap', c, Nx, Ny, Nz, reg.kp(reg.K1), reg.E, L.ptr(pr), 32,
-                   L.ptr(hutch))
+            L.call('vdvc_hutchinson_eqgap', c, Nx, Ny, Nz, L.rtype_of(hutch), reg.kp(reg.K1),
+                   reg.E, L.ptr(pr), 32, L.ptr(hutch))
             del pr
 
         ll = 0.0
@@ -395,24 +437,25 @@ class _Level:
                 fold(mask0)
             Gh, mh = hmask(mask0)
             if reg_type == 'equilibrium_gap':
-                d = L.empty(Ndof)
+                d = L.empty(Ndof, wd)
                 L.call('vdvc_diag_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]),
                        gt, L.ptr(mh), L.ptr(d))
-                ones = t.ones(Ndof, dtype=t.float32, device='cuda')
+                ones = t.ones(Ndof, dtype=tdt, device='cuda')
                 H0 = K.dot(d, ones)
-                dl = L.empty(Ndof)
-                L.call('vdvc_diag_K', c, Nx, Ny, Nz, reg.kp(reg.K2), reg.E, None, L.ptr(dl))
+                dl = L.empty(Ndof, wd)
+                L.call('vdvc_diag_K', c, Nx, Ny, Nz, L.rtype_of(dl), reg.kp(reg.K2), reg.E, None,
+                       L.ptr(dl))
                 L0 = K.dot(hutch, ones) + reg.eps * K.dot(dl, ones)
             else:
                 nodes = np.arange((Nx + 1) * (Ny + 1) * (Nz + 1))
                 jj = (nodes // (Nx + 1)) % (Ny + 1)
-                V = np.zeros(Ndof, dtype=np.float32)
-                V[0::3] = np.cos(jj.astype(np.float32) * h / T * 2 * np.pi)
-                Vd = L.to_dev_flat(V)
-                Hv = L.empty(Ndof)
+                V = np.zeros(Ndof, dtype=ndt)
+                V[0::3] = np.cos(jj.astype(ndt) * h / T * 2 * np.pi)
+                Vd = L.to_dev_flat(V, wd)
+                Hv = L.empty(Ndof, wd)
                 L.call('vdvc_matvec_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]),
                        L.ptr(Gh[2]), gt, L.ptr(mh), L.ptr(Vd), L.ptr(Hv))
-                Lv = reg.matvec(self.M, Vd, L.empty(Ndof))
+                Lv = reg.matvec(self.M, Vd, L.empty(Ndof, wd))
                 H0 = K.dot(Vd, Hv)
                 L0 = K.dot(Vd, Lv)
             ll = c_reg * (l0 / T) ** reg_exponent * H0 / L0
@@ -423,14 +466,14 @@ class _Level:
         llf = float(ll)
         rk = reg.args() if l0 is not None else (L.REG_NONE, None, None, reg.E, reg.eps)
 
-        U = t.zeros(Ndof, dtype=t.float32, device='cuda') if U0 is None else U0.clone()
-        res = t.zeros(Nvox, dtype=t.float32, device='cuda')   # pitched
-        dU = t.zeros(Ndof, dtype=t.float32, device='cuda')
-        diag = L.empty(Ndof)
-        Minv = L.empty(Ndof)
-        b = L.empty(Ndof)
-        RU = L.empty(Ndof)
-        gvals = L.zeros(Nvox)
+        U = t.zeros(Ndof, dtype=tdt, device='cuda') if U0 is None else U0.to(tdt).clone()
+        res = t.zeros(Nvox, dtype=tdt, device='cuda')   # pitched
+        dU = t.zeros(Ndof, dtype=tdt, device='cuda')
+        diag = L.empty(Ndof, wd)
+        Minv = L.empty(Ndof, wd)
+        b = L.empty(Ndof, wd)
+        RU = L.empty(Ndof, wd)
+        gvals = L.zeros(Nvox, wd)
         mask = L.zeros(Nvox, 'uint8')
 
         def jacobi(bm):
@@ -462,7 +505,7 @@ class _Level:
             refold = folded and dynamic_mask
             L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
                    gt, L.ptr(self.f), self.itype, mean0_f, L.ptr(gvals),
-                   float(np.float32(g_mean)), float(np.float32(kscale)), L.ptr(bmask),
+                   rnd(g_mean), rnd(kscale), L.ptr(bmask),
                    L.ptr(mask), L.ptr(res), L.ptr(b),
                    L.ptr(Gw[0]) if refold else None, L.ptr(Gw[1]) if refold else None,
                    L.ptr(Gw[2]) if refold else None, s4)
@@ -501,16 +544,16 @@ class _Level:
 # multiscale driver (correlate_gpu.py:1143-1305)
 # ---------------------------------------------------------------------------
 
-def _affine_U(A, t, c, s, h_i, Nx, Ny, Nz):
+def _affine_U(A, t, c, s, h_i, Nx, Ny, Nz, dtype=np.float32):
     c_i = np.asarray(c, float) / s
     t_i = np.asarray(t, float) / s
     n = np.arange((Nx + 1) * (Ny + 1) * (Nz + 1))
     X = np.stack([n % (Nx + 1), (n // (Nx + 1)) % (Ny + 1), n // ((Nx + 1) * (Ny + 1))],
                  axis=1).astype(np.float64) * h_i
     u = (X - c_i) @ np.asarray(A, float).T + c_i + t_i - X
-    U = np.zeros(3 * n.size, dtype=np.float32)
+    U = np.zeros(3 * n.size, dtype=dtype)
     for k in range(3):
-        U[k::3] = u[:, k].astype(np.float32)
+        U[k::3] = u[:, k].astype(dtype)
     return U
 
 
@@ -521,10 +564,9 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
     (correlate_gpu.py:1143-1305).  Images are uploaded once and decimated on
     the device."""
     scales = sorted(set(scales), reverse=True)
-    storage = kwargs.pop('storage', 'fp32')
     dtype = kwargs.pop('dtype', np.float32)
-    if np.dtype(dtype) != np.float32:
-        raise NotImplementedError('the device solve computes in float32 with FP64 reductions')
+    storage = _storage_for(dtype, kwargs.pop('storage', 'fp32'))
+    wd = 'float64' if storage == 'fp64' else 'float32'
     disp = kwargs.get('disp', True)
     A_aff = t_aff = c_aff = None
     if ext_affine is not None:
@@ -552,8 +594,9 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
         if any(o % s != 0 for o in g_origin):
             raise ValueError("g_origin=%r not divisible by 2**%d=%d for all "
                              "requested scales" % (g_origin, isc, s))
-    f, fs = L.to_dev_volume(f_pix, 'float32')
-    g, gs = L.to_dev_volume(g_pix, 'float32')
+    f, fs = L.to_dev_volume(f_pix, wd)
+    g, gs = L.to_dev_volume(g_pix, wd)
+    vtype = L.F64 if wd == 'float64' else L.F32
     U = None
     prev = None
     res = dU = None
@@ -561,20 +604,21 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
     for isc in scales:
         s = 2 ** isc
         h_i = h // s
-        fi, fis = _stride_dev(f, fs, s)
-        gi, gis = _stride_dev(g, gs, s)
+        fi, fis = _stride_dev(f, fs, s, vtype)
+        gi, gis = _stride_dev(g, gs, s, vtype)
         go_i = tuple(o // s for o in g_origin)
         l0_i = None if l0 is None else l0 / s
         if U is not None:
             K.axpby(prev / s, U, 0.0, U)
         elif A_aff is not None:
-            U = L.to_dev_flat(_affine_U(A_aff, t_aff, c_aff, s, h_i, Nx, Ny, Nz))
+            U = L.to_dev_flat(_affine_U(A_aff, t_aff, c_aff, s, h_i, Nx, Ny, Nz,
+                                        getattr(np, wd)), wd)
         elif fft_prealign:
             d = _fft_shift_dev(fi, fis, gi, gis)
-            Un = np.zeros(3 * (Nx + 1) * (Ny + 1) * (Nz + 1), dtype=np.float32)
+            Un = np.zeros(3 * (Nx + 1) * (Ny + 1) * (Nz + 1), dtype=getattr(np, wd))
             for k in range(3):
                 Un[k::3] = d[k]
-            U = L.to_dev_flat(Un)
+            U = L.to_dev_flat(Un, wd)
             if disp:
                 print("FFT pre-alignment: rigid shift d = (%.1f, %.1f, %.1f) voxels"
                       % (d[0] * s, d[1] * s, d[2] * s))

@@ -50,6 +50,23 @@ def auto_chunk_size(nq, max_tensor_elems=20_000_000):
 # argument plumbing
 # ---------------------------------------------------------------------------
 
+def _is_f64(a):
+    if a is None or type(a) in (float, int):   # Python scalars are weak
+        return False
+    if L.is_tensor(a):
+        return a.dtype == L.torch().float64
+    return np.asarray(a).dtype == np.float64
+
+
+def work_dtype(*arrays):
+    """The working type of an operator call, NumPy-style: float64 if any
+    array (or NumPy scalar) input is float64 (the reference's dtype=float64
+    mode), else float32; Python scalars do not promote.  Element matrices are
+    applied in the working type of the vectors.  Returns (torch dtype name,
+    library type code)."""
+    return ('float64', L.F64) if any(_is_f64(a) for a in arrays) else ('float32', L.F32)
+
+
 class _Out:
     """Remembers whether the caller passed host or device arrays."""
 
@@ -64,10 +81,10 @@ def level_shape(h, Nx, Ny, Nz):
     return (Nx * h + 1, Ny * h + 1, Nz * h + 1)
 
 
-def grad_planes(grad_f_vox, shape):
-    """(Nvox, 3) gradient (reference layout) -> three pitched float32 device
-    planes."""
-    g = L.to_dev_flat(grad_f_vox).view(-1, 3)
+def grad_planes(grad_f_vox, shape, dtype='float32'):
+    """(Nvox, 3) gradient (reference layout) -> three pitched device planes of
+    `dtype`."""
+    g = L.to_dev_flat(grad_f_vox, dtype).view(-1, 3)
     return [L.pitch_dense(g[:, c].contiguous(), shape) for c in range(3)]
 
 
@@ -87,12 +104,12 @@ def _kptr(K):
     return a
 
 
-def _E(E_elem, Nx, Ny, Nz):
-    """Scalar E, or a device per-element array."""
+def _E(E_elem, Nx, Ny, Nz, dtype='float32'):
+    """Scalar E, or a device per-element array of `dtype`."""
     a = np.asarray(E_elem) if not L.is_tensor(E_elem) else None
     if a is not None and a.ndim == 0:
         return float(a), None
-    d = L.to_dev_flat(E_elem)
+    d = L.to_dev_flat(E_elem, dtype)
     if d.numel() == 1:
         return float(d.item()), None
     if d.numel() != Nx * Ny * Nz:
@@ -109,12 +126,13 @@ def matvec_H(v, grad_f_vox, mask0_vox, N_stencil, w_stencil, h, Nx, Ny, Nz, xp=n
     """f = H @ v (kernels_dvc.py:452-506)."""
     _check_stencil(N_stencil, h)
     out = _Out(v, grad_f_vox)
+    dt, code = work_dtype(v, grad_f_vox)
     shp = level_shape(h, Nx, Ny, Nz)
-    gx, gy, gz = grad_planes(grad_f_vox, shp)
-    dv = L.to_dev_flat(v)
-    res = L.empty(dv.numel())
+    gx, gy, gz = grad_planes(grad_f_vox, shp, dt)
+    dv = L.to_dev_flat(v, dt)
+    res = L.empty(dv.numel(), dt)
     m = _mask_u8(mask0_vox, shp)
-    L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+    L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), code,
            L.ptr(m), L.ptr(dv), L.ptr(res))
     return out(res)
 
@@ -124,11 +142,12 @@ def build_diagonal_H(grad_f_vox, mask0_vox, N_stencil, w_stencil, h, Nx, Ny, Nz,
     """diag(H) (kernels_dvc.py:509-543)."""
     _check_stencil(N_stencil, h)
     out = _Out(grad_f_vox)
+    dt, code = work_dtype(grad_f_vox)
     shp = level_shape(h, Nx, Ny, Nz)
-    gx, gy, gz = grad_planes(grad_f_vox, shp)
-    res = L.empty(Ndof)
+    gx, gy, gz = grad_planes(grad_f_vox, shp, dt)
+    res = L.empty(Ndof, dt)
     m = _mask_u8(mask0_vox, shp)
-    L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
+    L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), code,
            L.ptr(m), L.ptr(res))
     return out(res)
 
@@ -140,11 +159,12 @@ def build_diagonal_H(grad_f_vox, mask0_vox, N_stencil, w_stencil, h, Nx, Ny, Nz,
 def matvec_K_ref(v, K_ref, E_elem, Nx, Ny, Nz, xp=np, chunk_size=500_000):
     """f = sum_e E_e P_e^T K_ref P_e v (kernels_dvc.py:349-374)."""
     out = _Out(v, E_elem)
+    dt, code = work_dtype(v, E_elem)
     K = _kptr(K_ref)
-    E, dE = _E(E_elem, Nx, Ny, Nz)
-    dv = L.to_dev_flat(v)
-    res = L.empty(dv.numel())
-    L.call('vdvc_matvec_K', L.ctx(), Nx, Ny, Nz, K.ctypes.data_as(L._dp), E, L.ptr(dE),
+    E, dE = _E(E_elem, Nx, Ny, Nz, dt)
+    dv = L.to_dev_flat(v, dt)
+    res = L.empty(dv.numel(), dt)
+    L.call('vdvc_matvec_K', L.ctx(), Nx, Ny, Nz, code, K.ctypes.data_as(L._dp), E, L.ptr(dE),
            L.ptr(dv), L.ptr(res))
     return out(res)
 
@@ -152,10 +172,12 @@ def matvec_K_ref(v, K_ref, E_elem, Nx, Ny, Nz, xp=np, chunk_size=500_000):
 def build_diagonal_K_ref(K_ref, E_elem, Nx, Ny, Nz, Ndof, xp=np):
     """Diagonal of the assembled K (kernels_dvc.py:377-392)."""
     out = _Out(E_elem)
+    dt, code = work_dtype(E_elem)
     K = _kptr(K_ref)
-    E, dE = _E(E_elem, Nx, Ny, Nz)
-    res = L.empty(Ndof)
-    L.call('vdvc_diag_K', L.ctx(), Nx, Ny, Nz, K.ctypes.data_as(L._dp), E, L.ptr(dE), L.ptr(res))
+    E, dE = _E(E_elem, Nx, Ny, Nz, dt)
+    res = L.empty(Ndof, dt)
+    L.call('vdvc_diag_K', L.ctx(), Nx, Ny, Nz, code, K.ctypes.data_as(L._dp), E, L.ptr(dE),
+           L.ptr(res))
     return out(res)
 
 
@@ -165,13 +187,14 @@ def matvec_equilibrium_gap(v, K_ref_elastic, interior_mask, E_elem, Nx, Ny, Nz, 
     interior-node DOF mask of the mesh (geometry_dvc.build_interior_dof_mask);
     the kernel evaluates it from node indices."""
     out = _Out(v, E_elem)
+    dt, code = work_dtype(v, E_elem)
     _check_interior(interior_mask, Nx, Ny, Nz)
     K = _kptr(K_ref_elastic)
-    E, dE = _E(E_elem, Nx, Ny, Nz)
-    dv = L.to_dev_flat(v)
-    res = L.empty(dv.numel())
-    L.call('vdvc_matvec_eqgap', L.ctx(), Nx, Ny, Nz, K.ctypes.data_as(L._dp), E, L.ptr(dE),
-           L.ptr(dv), L.ptr(res))
+    E, dE = _E(E_elem, Nx, Ny, Nz, dt)
+    dv = L.to_dev_flat(v, dt)
+    res = L.empty(dv.numel(), dt)
+    L.call('vdvc_matvec_eqgap', L.ctx(), Nx, Ny, Nz, code, K.ctypes.data_as(L._dp), E,
+           L.ptr(dE), L.ptr(dv), L.ptr(res))
     return out(res)
 
 
@@ -197,14 +220,15 @@ def build_diagonal_equilibrium_gap(K_ref_elastic, interior_mask, E_elem, Nx, Ny,
     """Hutchinson estimate of diag(K P_i K), clipped at 1e-12 max
     (kernels_dvc.py:420-437); probes from the reference's RNG stream."""
     out = _Out(E_elem)
+    dt, code = work_dtype(E_elem)
     _check_interior(interior_mask, Nx, Ny, Nz)
     K = _kptr(K_ref_elastic)
-    E, dE = _E(E_elem, Nx, Ny, Nz)
+    E, dE = _E(E_elem, Nx, Ny, Nz, dt)
     if dE is not None:
         raise ValueError('per-element E_elem is not supported by the Hutchinson diagonal')
     pr = L.to_dev_flat(rademacher_probes(Ndof, n_probes, seed).ravel(), 'int8')
-    res = L.empty(Ndof)
-    L.call('vdvc_hutchinson_eqgap', L.ctx(), Nx, Ny, Nz, K.ctypes.data_as(L._dp), E,
+    res = L.empty(Ndof, dt)
+    L.call('vdvc_hutchinson_eqgap', L.ctx(), Nx, Ny, Nz, code, K.ctypes.data_as(L._dp), E,
            L.ptr(pr), n_probes, L.ptr(res))
     return out(res)
 
@@ -214,36 +238,42 @@ def build_diagonal_equilibrium_gap(K_ref_elastic, interior_mask, E_elem, Nx, Ny,
 # ---------------------------------------------------------------------------
 
 def dot(a, b):
-    """FP64 dot product of two float32 device vectors."""
+    """FP64 dot product of two device vectors of one working type."""
+    if a.dtype != b.dtype:
+        raise TypeError('dot of %s and %s vectors' % (a.dtype, b.dtype))
     r = (L._d * 1)()
-    L.call('vdvc_dot', L.ctx(), int(a.numel()), L.ptr(a), L.ptr(b), r)
+    L.call('vdvc_dot', L.ctx(), int(a.numel()), L.rtype_of(a), L.ptr(a), L.ptr(b), r)
     return float(r[0])
 
 
 def axpby(a, x, b, y):
-    """y = a x + b y in place (float32 device vectors)."""
-    L.call('vdvc_axpby', L.ctx(), int(y.numel()), float(a), L.ptr(x), float(b), L.ptr(y))
+    """y = a x + b y in place (device vectors of one working type)."""
+    if x.dtype != y.dtype:
+        raise TypeError('axpby of %s and %s vectors' % (x.dtype, y.dtype))
+    L.call('vdvc_axpby', L.ctx(), int(y.numel()), L.rtype_of(y), float(a), L.ptr(x), float(b),
+           L.ptr(y))
     return y
 
 
 def pcg(matvec, b, M_inv, xp=np, x0=None, max_iter=1000, tol=1e-8, verbose=False):
     """Jacobi PCG with a caller-supplied matvec (kernels_dvc.py:550-583).
-    Vector updates and FP64 dots run on the GPU; `matvec` receives and must
-    return float32 device tensors."""
+    Vector updates and FP64 dots run on the GPU in the working type of b and
+    M_inv; `matvec` receives and must return device tensors of that type."""
     out = _Out(b, M_inv)
     t = L.require_gpu()
-    bd = L.to_dev_flat(b)
-    Md = L.to_dev_flat(M_inv)
-    x = t.zeros_like(bd) if x0 is None else L.to_dev_flat(x0).clone()
+    dt, code = work_dtype(b, M_inv)
+    bd = L.to_dev_flat(b, dt)
+    Md = L.to_dev_flat(M_inv, dt)
+    x = t.zeros_like(bd) if x0 is None else L.to_dev_flat(x0, dt).clone()
     r = bd.clone()
-    axpby(-1.0, L.to_dev_flat(matvec(x)), 1.0, r)
+    axpby(-1.0, L.to_dev_flat(matvec(x), dt), 1.0, r)
     z = t.empty_like(r)
-    L.call('vdvc_mul', L.ctx(), int(r.numel()), L.ptr(Md), L.ptr(r), L.ptr(z))
+    L.call('vdvc_mul', L.ctx(), int(r.numel()), code, L.ptr(Md), L.ptr(r), L.ptr(z))
     p = z.clone()
     rz = dot(r, z)
     r0 = dot(bd, bd) + 1e-30
     for it in range(max_iter):
-        Ap = L.to_dev_flat(matvec(p))
+        Ap = L.to_dev_flat(matvec(p), dt)
         alpha = rz / (dot(p, Ap) + 1e-30)
         axpby(alpha, p, 1.0, x)
         axpby(-alpha, Ap, 1.0, r)
@@ -254,7 +284,7 @@ def pcg(matvec, b, M_inv, xp=np, x0=None, max_iter=1000, tol=1e-8, verbose=False
             if verbose:
                 print(f"  Converged at iter {it}, residual = {res:.3e}")
             break
-        L.call('vdvc_mul', L.ctx(), int(r.numel()), L.ptr(Md), L.ptr(r), L.ptr(z))
+        L.call('vdvc_mul', L.ctx(), int(r.numel()), code, L.ptr(Md), L.ptr(r), L.ptr(z))
         rz_new = dot(r, z)
         beta = rz_new / (rz + 1e-30)
         axpby(1.0, z, beta, p)
