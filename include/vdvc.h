@@ -226,6 +226,46 @@ int vdvc_affine_sums(vdvc_ctx* ctx, const float* d_f, int nx, int ny, int nz, co
                      int gnx, int gny, int gnz, const double* h_A9, const double* h_t3,
                      const double* h_c3, double* h_out6);
 
+/* ---- learned seed (no reference counterpart) ------------------------------
+ * A small 3-D convolutional network predicts coarse nodal displacements that
+ * seed the finest-scale GN solve in place of the coarse pyramid levels of
+ * multiscale_correlate_gpu (engine/correlate_gpu.py:1143-1305).  Tensors are
+ * float32, channels last: activations [n][z][y][x][c], weights [o][tap][c]
+ * with tap = (dz*3 + dy)*3 + dx; 3x3x3 kernels, zero padding 1, stride 1 or
+ * 2 (output extent (n - 1)/stride + 1).  Products run on the matrix cores
+ * with bfloat16 operands and float32 accumulation.  C, O <= 32. */
+
+/* y = conv(x, w) + b, optional ReLU. */
+int vdvc_conv3d_forward(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O, int stride,
+                        const float* d_x, const float* d_w, const float* d_b, int relu,
+                        float* d_y);
+
+/* dx = conv^T(dy, w), times (d_act > 0) when d_act (the layer input, a ReLU
+ * output) is given. */
+int vdvc_conv3d_backward_data(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O,
+                              int stride, const float* d_dy, const float* d_w,
+                              const float* d_act, float* d_dx);
+
+/* dw = sum_p dy (x) x-patch, db = sum_p dy (fixed-order reductions). */
+int vdvc_conv3d_backward_weight(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O,
+                                int stride, const float* d_x, const float* d_dy, float* d_dw,
+                                float* d_db);
+
+/* One bias-corrected Adam step (step >= 1) on n float32 parameters. */
+int vdvc_adam_step(vdvc_ctx* ctx, int64_t n, float* d_p, const float* d_g, float* d_m,
+                   float* d_v, double lr, double beta1, double beta2, double eps, int step);
+
+/* Mean squared error over n entries: *h_loss (FP64 sum / n) and
+ * d_grad = 2 (pred - label) / n. */
+int vdvc_mse(vdvc_ctx* ctx, int64_t n, const float* d_pred, const float* d_label, float* d_grad,
+             double* h_loss);
+
+/* Network input [z][y][x][2] from the stride-s decimation of f and g
+ * (itype F32 or F16, x-fastest dense volumes of equal dims), each channel
+ * standardised by its own mean and standard deviation. */
+int vdvc_seed_input(vdvc_ctx* ctx, const void* d_f, const void* d_g, int itype, int nx, int ny,
+                    int nz, int s, float* d_out);
+
 #ifdef __cplusplus
 }
 #endif

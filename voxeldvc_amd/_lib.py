@@ -61,6 +61,12 @@ SIGNATURES = {
     'vdvc_stride': [_vp, _vp, _i, _i, _i, _i, _i, _vp],
     'vdvc_phase_corr': [_vp, _vp, _vp, _i, _i, _i, _i64p],
     'vdvc_affine_sums': [_vp, _vp, _i, _i, _i, _vp, _i, _i, _i, _dp, _dp, _dp, _dp],
+    'vdvc_conv3d_forward': [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp],
+    'vdvc_conv3d_backward_data': [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    'vdvc_conv3d_backward_weight': [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    'vdvc_adam_step': [_vp, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _i],
+    'vdvc_mse': [_vp, _i64, _vp, _vp, _vp, _dp],
+    'vdvc_seed_input': [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp],
 }
 
 _lib = None
@@ -104,8 +110,13 @@ def check(rc):
 
 
 def call(name, *args):
+    """Call a library entry point.  Device tensors may be passed directly:
+    they are converted to pointers here and stay referenced until the work
+    is enqueued, so a temporary cannot be freed (and its memory handed to
+    another argument) before the kernel that reads it is on the stream."""
     lib = load()
-    check(getattr(lib, name)(*args))
+    cargs = [ptr(a) if is_tensor(a) else a for a in args]
+    check(getattr(lib, name)(*cargs))
 
 
 # ---------------------------------------------------------------------------

@@ -80,6 +80,7 @@ struct vdvc_ctx {
   Buf K1[2], K2[2];         // element matrices (24x24) in float / double
   Buf hacc;                 // Hutchinson FP64 accumulator
   Buf dft;                  // DFT work volumes
+  Buf wpart;                // conv weight-gradient partials
   double* hpin = nullptr;   // pinned host staging (64 doubles)
   std::vector<double> k1_cache[2], k2_cache[2];
   // optional per-launch timing of the voxel-streaming kernels (HIP events
@@ -971,6 +972,93 @@ int vdvc_affine_sums(vdvc_ctx* ctx, const float* d_f, int nx, int ny, int nz, co
                                        ctx->stats.as<double>(), nb, ctx->stream));
   VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 6, ctx->red.as<double>(), ctx->stream));
   return fetch(ctx, ctx->red.p, 6 * sizeof(double), h_out6);
+}
+
+static int conv_args(int N, int D, int H, int W, int C, int O, int stride) {
+  if (N < 1 || D < 1 || H < 1 || W < 1 || C < 1 || O < 1)
+    return fail(-1, "invalid conv shape N=%d D=%d H=%d W=%d C=%d O=%d", N, D, H, W, C, O);
+  if (C > 32 || O > 32) return fail(-1, "conv channels must be <= 32 (C=%d O=%d)", C, O);
+  if (stride != 1 && stride != 2) return fail(-1, "conv stride must be 1 or 2");
+  if ((long long)N * D * H * W * (C > O ? C : O) > 0x7fffffffLL)
+    return fail(-1, "conv tensor too large");
+  return 0;
+}
+
+int vdvc_conv3d_forward(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O, int stride,
+                        const float* d_x, const float* d_w, const float* d_b, int relu,
+                        float* d_y) {
+  if (!ctx || !d_x || !d_w || !d_y) return fail(-1, "NULL argument");
+  int rc = conv_args(N, D, H, W, C, O, stride);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_conv_fwd(N, D, H, W, C, O, stride, d_x, d_w, d_b, relu, d_y,
+                                    ctx->stream));
+  return 0;
+}
+
+int vdvc_conv3d_backward_data(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O,
+                              int stride, const float* d_dy, const float* d_w,
+                              const float* d_act, float* d_dx) {
+  if (!ctx || !d_dy || !d_w || !d_dx) return fail(-1, "NULL argument");
+  int rc = conv_args(N, D, H, W, C, O, stride);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_conv_bwd_data(N, D, H, W, C, O, stride, d_dy, d_w, d_act, d_dx,
+                                         ctx->stream));
+  return 0;
+}
+
+int vdvc_conv3d_backward_weight(vdvc_ctx* ctx, int N, int D, int H, int W, int C, int O,
+                                int stride, const float* d_x, const float* d_dy, float* d_dw,
+                                float* d_db) {
+  if (!ctx || !d_x || !d_dy || !d_dw || !d_db) return fail(-1, "NULL argument");
+  int rc = conv_args(N, D, H, W, C, O, stride);
+  if (rc) return rc;
+  const int np = vdvc::conv_wgrad_parts(N, D, H, W, C, O, stride);
+  if ((rc = ctx->wpart.ensure((size_t)np * O * 27 * C * sizeof(float)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_conv_bwd_weight(N, D, H, W, C, O, stride, d_x, d_dy,
+                                           ctx->wpart.as<float>(), d_dw, d_db, ctx->stream));
+  return 0;
+}
+
+int vdvc_adam_step(vdvc_ctx* ctx, int64_t n, float* d_p, const float* d_g, float* d_m,
+                   float* d_v, double lr, double beta1, double beta2, double eps, int step) {
+  if (!ctx || !d_p || !d_g || !d_m || !d_v) return fail(-1, "NULL argument");
+  if (n < 0 || step < 1) return fail(-1, "n < 0 or step < 1");
+  if (n == 0) return 0;
+  const double c1 = 1.0 - std::pow(beta1, step), c2 = 1.0 - std::pow(beta2, step);
+  VDVC_LAUNCH(vdvc::launch_adam(n, d_p, d_g, d_m, d_v, (float)lr, (float)beta1, (float)beta2,
+                                (float)eps, (float)c1, (float)c2, ctx->stream));
+  return 0;
+}
+
+int vdvc_mse(vdvc_ctx* ctx, int64_t n, const float* d_pred, const float* d_label, float* d_grad,
+             double* h_loss) {
+  if (!ctx || !d_pred || !d_label || !d_grad || !h_loss) return fail(-1, "NULL argument");
+  if (n < 1) return fail(-1, "n < 1");
+  const long long nb = (n + 255) / 256;
+  int rc;
+  if ((rc = ctx->stats.ensure((size_t)nb * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_mse(n, d_pred, d_label, d_grad, ctx->stats.as<double>(), ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 1, ctx->red.as<double>(),
+                                  ctx->stream));
+  double s = 0.0;
+  if ((rc = fetch(ctx, ctx->red.p, sizeof(double), &s))) return rc;
+  *h_loss = s / (double)n;
+  return 0;
+}
+
+int vdvc_seed_input(vdvc_ctx* ctx, const void* d_f, const void* d_g, int itype, int nx, int ny,
+                    int nz, int s, float* d_out) {
+  if (!ctx || !d_f || !d_g || !d_out) return fail(-1, "NULL argument");
+  if (itype != VDVC_F32 && itype != VDVC_F16) return fail(-1, "image type must be F32 or F16");
+  if (nx < 1 || ny < 1 || nz < 1 || s < 1) return fail(-1, "invalid dims / stride");
+  const int nparts = 512;
+  int rc;
+  if ((rc = ctx->stats.ensure((size_t)6 * nparts * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_seed_input(itype, d_f, d_g, nx, ny, nz, s, ctx->stats.as<double>(),
+                                      nparts, ctx->red.as<double>(), d_out, ctx->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -138,4 +138,18 @@ int launch_phase_corr(const float* f, const float* g, int nx, int ny, int nz, do
                       long long* out_idx, hipStream_t st);
 int launch_affine_cost(const float* f, int nx, int ny, int nz, const float* g, int gnx, int gny,
                        int gnz, const AffineIn& P, double* part, int nblocks, hipStream_t st);
+// learned-seed network (vdvc_seednet.hip)
+int launch_conv_fwd(int N, int D, int H, int W, int C, int O, int stride, const float* x,
+                    const float* w, const float* b, int relu, float* y, hipStream_t st);
+int launch_conv_bwd_data(int N, int D, int H, int W, int C, int O, int stride, const float* dy,
+                         const float* w, const float* act, float* dx, hipStream_t st);
+int conv_wgrad_parts(int N, int D, int H, int W, int C, int O, int stride);
+int launch_conv_bwd_weight(int N, int D, int H, int W, int C, int O, int stride, const float* x,
+                           const float* dy, float* part, float* dw, float* db, hipStream_t st);
+int launch_adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1,
+                float b2, float eps, float c1, float c2, hipStream_t st);
+int launch_mse(long long n, const float* pred, const float* label, float* grad, double* part,
+               hipStream_t st);
+int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
+                      double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

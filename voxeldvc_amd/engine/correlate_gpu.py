@@ -322,11 +322,12 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
                   K_ref_laplacian=None, reg_exponent=2, c_reg=1.0, reg_type='laplacian',
                   K_ref_elastic=None, eps_reg=1e-3, dynamic_mask=False, freeze_mask_after=None,
                   U0=None, maxiter=30, eps=1e-3, pcg_tol=1e-6, pcg_max_iter=2000, disp=True,
-                  dtype=np.float32, storage='fp32', log=None, pcg_check_every=8):
+                  dtype=np.float32, storage='fp32', log=None, pcg_check_every=8, callback=None):
     """Matrix-free GN DVC correlation on one level; returns (U, res, dU)
     (correlate_gpu.py:828-1140).  Extensions: `storage` (see module doc),
-    `log` (list receiving one dict per GN iteration) and `pcg_check_every`
-    (PCG iterations between host polls of the device stop flag)."""
+    `log` (list receiving one dict per GN iteration), `pcg_check_every`
+    (PCG iterations between host polls of the device stop flag) and
+    `callback(ik, U)` (called after every GN update with the device U)."""
     Nvx, Nvy, Nvz = Nx * h + 1, Ny * h + 1, Nz * h + 1
     Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
     Nvox = Nvx * Nvy * Nvz
@@ -348,7 +349,7 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
                           U0=None if U0 is None else L.to_dev_flat(U0, wd), maxiter=maxiter,
                           eps=eps,
                           pcg_tol=pcg_tol, pcg_max_iter=pcg_max_iter, disp=disp, log=log,
-                          pcg_check_every=pcg_check_every)
+                          pcg_check_every=pcg_check_every, callback=callback)
     return _as_out(xp, U), _as_out(xp, res), _as_out(xp, dU)
 
 
@@ -378,7 +379,7 @@ class _Level:
 
     def solve(self, g_origin, glt, l0, K_ref_laplacian, reg_exponent, c_reg, reg_type,
               K_ref_elastic, eps_reg, dynamic_mask, freeze_mask_after, U0, maxiter, eps,
-              pcg_tol, pcg_max_iter, disp, log, pcg_check_every):
+              pcg_tol, pcg_max_iter, disp, log, pcg_check_every, callback=None):
         t = L.require_gpu()
         c = L.ctx()
         Nx, Ny, Nz, h = self.M
@@ -523,6 +524,8 @@ class _Level:
                    rk[0], rk[1], rk[2], rk[3], rk[4], llf, L.ptr(b), L.ptr(Minv), L.ptr(dU),
                    float(pcg_tol), int(pcg_max_iter), int(pcg_check_every), it_, rr_)
             K.axpby(1.0, dU, 1.0, U)
+            if callback is not None:
+                callback(ik, U)
             nd = K.dot(dU, dU)
             nu = K.dot(U, U)
             err = math.sqrt(nd) / math.sqrt(nu) if nu > 0 else (math.nan if nd == 0 else math.inf)
@@ -562,7 +565,10 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
                              prealign_affine=False, ext_affine=None, **kwargs):
     """Coarse-to-fine stride-decimation pyramid warm start of correlate_gpu
     (correlate_gpu.py:1143-1305).  Images are uploaded once and decimated on
-    the device."""
+    the device.  Extensions: `callback(scale, ik, U)`, called after every GN
+    update of every level with the device U; `learned_seed` (a
+    voxeldvc_amd.seednet.SeedNet), whose nodal prediction starts the finest
+    requested level in place of the coarser levels and the pre-alignment."""
     scales = sorted(set(scales), reverse=True)
     dtype = kwargs.pop('dtype', np.float32)
     storage = _storage_for(dtype, kwargs.pop('storage', 'fp32'))
@@ -601,6 +607,24 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
     prev = None
     res = dU = None
     log = kwargs.pop('log', None)
+    cb = kwargs.pop('callback', None)
+    seed_net = kwargs.pop('learned_seed', None)
+    if seed_net is not None:
+        # the network's nodal prediction replaces the coarse levels and the
+        # pre-alignment: only the finest requested level is solved
+        if A_aff is not None or any(g_origin):
+            raise ValueError('learned_seed cannot be combined with an affine pre-alignment '
+                             'or a g_origin offset')
+        if tuple(fs) != tuple(gs):
+            raise ValueError('learned_seed needs f and g of equal dims')
+        s0 = 2 ** scales[-1]
+        fi, fis = _stride_dev(f, fs, s0, vtype)
+        gi, gis = _stride_dev(g, gs, s0, vtype)
+        U = seed_net.predict(fi.float(), gi.float(), fis, Nx, Ny, Nz, h // s0).to(
+            getattr(L.torch(), wd))
+        prev = s0
+        scales = scales[-1:]
+        fft_prealign = False
     for isc in scales:
         s = 2 ** isc
         h_i = h // s
@@ -629,7 +653,8 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
             raise ValueError('f_pix shape does not match the mesh at scale 2^%d' % isc)
         lv = _Level(fi, gi, gis, Nx, Ny, Nz, h_i, storage)
         sub = None if log is None else []
-        U, res, dU = lv.solve(g_origin=go_i, l0=l0_i, U0=U, log=sub,
+        lcb = None if cb is None else (lambda ik, Ud, _s=isc: cb(_s, ik, Ud))
+        U, res, dU = lv.solve(g_origin=go_i, l0=l0_i, U0=U, log=sub, callback=lcb,
                               **_solve_kwargs(kwargs))
         if log is not None:
             log.append(dict(scale=isc, iters=sub))

@@ -175,3 +175,110 @@ def speckle_torch(shape, n_blobs, seed, sigma=1.5, amp=(50.0, 255.0), background
                                                  + ny * idx[2][:, None, None, :]))
         f.index_add_(0, lin.reshape(-1), val.reshape(-1))
     return f
+
+
+def fourier_params(n_modes, wl_range, peak, seed, extent):
+    """The random plane-wave modes of fourier_field and the scale that makes
+    max |u| over a coarse grid of `extent` equal `peak` voxels."""
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=(n_modes, 3))
+    k /= np.linalg.norm(k, axis=1, keepdims=True)
+    wl = rng.uniform(wl_range[0], wl_range[1], size=n_modes)
+    kv = k * (2 * np.pi / wl)[:, None]
+    ph = rng.uniform(0, 2 * np.pi, size=n_modes)
+    av = rng.normal(size=(n_modes, 3))
+    g = [np.arange(0, n, max(1, n // 32), dtype=np.float64) for n in extent]
+    Xs, Ys, Zs = np.meshgrid(*g, indexing='ij')
+    r = [np.zeros_like(Xs) for _ in range(3)]
+    for m in range(n_modes):
+        s = np.sin(kv[m, 0] * Xs + kv[m, 1] * Ys + kv[m, 2] * Zs + ph[m])
+        for c in range(3):
+            r[c] += av[m, c] * s
+    scale = peak / max(1e-12, float(np.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2).max()))
+    return kv, ph, av * scale
+
+
+def random_affine(rng, max_rot_deg, max_strain):
+    """A = R (I + E): rotation about a random axis by <= max_rot_deg degrees,
+    symmetric strain E with entries <= max_strain."""
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    th = np.deg2rad(rng.uniform(-max_rot_deg, max_rot_deg))
+    Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(th) * Kx + (1 - np.cos(th)) * Kx @ Kx
+    E = rng.uniform(-max_strain, max_strain, size=(3, 3))
+    E = 0.5 * (E + E.T)
+    return R @ (np.eye(3) + E)
+
+
+def warp_backward_torch(f, shape, modes, noise_sigma, seed, quantize=True, slab=8,
+                        device='cuda', affine=None):
+    """GPU form of warp_backward for large volumes: g(y) = f(y - u(y)) with u
+    a sum of plane-wave modes (fourier_params), trilinear interpolation with
+    linear extrapolation at the border (float64 coordinates and weights),
+    plus N(0, noise_sigma) noise (torch generator `seed`), then optionally
+    rounded and clipped to [0, 255].  f: flat x-fastest float32 tensor of
+    `shape` (nx, ny, nz); returns the same layout.  `affine` = (A, c) adds
+    the displacement (A - I)(y - c) to u."""
+    import torch
+    nx, ny, nz = shape
+    kv, ph, av = (torch.as_tensor(a, dtype=torch.float64, device=device) for a in modes)
+    ff = f.to(device=device, dtype=torch.float64)
+    out = torch.empty(nx * ny * nz, dtype=torch.float32, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    x = torch.arange(nx, dtype=torch.float64, device=device)
+    y = torch.arange(ny, dtype=torch.float64, device=device)
+    for z0 in range(0, nz, slab):
+        z = torch.arange(z0, min(nz, z0 + slab), dtype=torch.float64, device=device)
+        Z, Y, X = torch.meshgrid(z, y, x, indexing='ij')      # x fastest
+        ph_ = (X[..., None] * kv[:, 0] + Y[..., None] * kv[:, 1] + Z[..., None] * kv[:, 2]
+               + ph)
+        s = torch.sin(ph_)
+        u = [(s * av[:, c]).sum(-1) for c in range(3)]
+        if affine is not None:
+            A, cen = affine
+            Am = np.asarray(A, np.float64) - np.eye(3)
+            d = [X - float(cen[0]), Y - float(cen[1]), Z - float(cen[2])]
+            for c in range(3):
+                u[c] = u[c] + Am[c, 0] * d[0] + Am[c, 1] * d[1] + Am[c, 2] * d[2]
+        p = [X - u[0], Y - u[1], Z - u[2]]
+        val = torch.zeros_like(X)
+        idx, fr = [], []
+        for q, n in zip(p, (nx, ny, nz)):
+            i0 = torch.clamp(torch.floor(q), 0, n - 2).to(torch.int64)
+            idx.append(i0)
+            fr.append(q - i0)
+        for dx in (0, 1):
+            wx = fr[0] if dx else 1 - fr[0]
+            for dy in (0, 1):
+                wy = fr[1] if dy else 1 - fr[1]
+                for dz in (0, 1):
+                    wz = fr[2] if dz else 1 - fr[2]
+                    lin = (idx[0] + dx) + nx * ((idx[1] + dy) + ny * (idx[2] + dz))
+                    val += wx * wy * wz * ff[lin]
+        if noise_sigma > 0:
+            val += noise_sigma * torch.randn(val.shape, generator=gen, dtype=torch.float64,
+                                             device=device)
+        if quantize:
+            val = val.round().clamp(0, 255)
+        out[z0 * nx * ny:(z0 + len(z)) * nx * ny] = val.reshape(-1).to(torch.float32)
+    return out
+
+
+def fourier_u_nodes(modes, Nx, Ny, Nz, h):
+    """The true displacement of a plane-wave field at the mesh nodes
+    (node i at voxel i*h), reference DOF layout (3 per node, x fastest).
+    The warp g(y) = f(y - u(y)) is matched by U(x) with x + U(x) = y, so the
+    nodal truth solves x + U = y with u evaluated at y; this returns the
+    first-order value u(x) (exact for translations)."""
+    kv, ph, av = modes
+    i = np.arange(Nx + 1) * h
+    j = np.arange(Ny + 1) * h
+    k = np.arange(Nz + 1) * h
+    Z, Y, X = np.meshgrid(k, j, i, indexing='ij')
+    s = np.sin(X[..., None] * kv[:, 0] + Y[..., None] * kv[:, 1] + Z[..., None] * kv[:, 2] + ph)
+    U = np.zeros(3 * X.size)
+    for c in range(3):
+        U[c::3] = (s * av[:, c]).sum(-1).ravel()
+    return U
