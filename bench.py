@@ -1,13 +1,17 @@
 """Benchmark of the global GN DVC hot path on one MI355X.
 
 Step = one correlation-Hessian matvec H v over a 513^3 voxel reference volume
-with a 64^3-element Hex8 mesh (h = 8): the configuration the tier metric
-("Hessian matvec Gvox/s (% HBM peak)") is quoted on.  Synthetic speckle
-(Gaussian blobs, quantised to 0..255, stored float16), gradients in bfloat16
-with the mask folded in (6 B/voxel); the float32 + uint8-mask form
-(13 B/voxel) is timed beside it.  Prints one JSON line.
+with a 64^3-element Hex8 mesh (h = 8), the configuration the metric
+("Hessian matvec Gvox/s (% HBM peak); GN time-to-solve at 0.01-vox RMS") is
+quoted on.  Synthetic speckle (Gaussian blobs, quantised to 0..255, stored
+float16), gradients in bfloat16 with the mask folded in (6 B/voxel); the
+float32 + uint8-mask form (13 B/voxel) is timed beside it.  The GN
+time-to-solve of the plain float32 multiscale solve and of the learned-seed
+mixed-precision solve on a 513^3 pair is measured after it (`time_to_solve`).
+Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 513] [--h 8]
+                    [--no-solve] [--no-cpu]
 """
 import argparse
 import json
@@ -36,7 +40,94 @@ def parse():
                         'streaming kernel of known size that scales HBM counter readings')
     p.add_argument('--only', choices=('bf16', 'f32'), default=None,
                    help='time one storage form only (for profiling runs)')
+    p.add_argument('--no-solve', action='store_true',
+                   help='skip the GN time-to-solve measurement')
+    p.add_argument('--solve-reps', type=int, default=3)
     return p.parse_args()
+
+
+def time_to_solve(n=513, h=8, reps=3, log=None):
+    """GN time-to-solve on one 8-bit speckle pair of n^3 voxels warped by 16
+    random Fourier modes (wavelengths 64-256 vox, peak 4 vox) plus N(0,1)
+    noise, Laplacian regularizer (l0 = 4.5 h), dynamic mask frozen after 4
+    iterations -- the production kwargs of the reference's run command.
+
+    * oracle: the same solver in float64 (dtype=np.float64), converged to
+      dU/U < 1e-6, defines the answer;
+    * plain: 3-level multiscale (scales 2,1,0) with FFT pre-alignment,
+      float32, the reference's default convergence (dU/U < 1e-3);
+    * learned + mixed: the seed network's prediction starts the finest level
+      (no coarse levels), float16 images / bfloat16 gradients, float32
+      matvec arithmetic, FP64 dots and norms; timed to the first GN
+      iterate within 0.01 voxel RMS of the oracle, and to its own
+      convergence.
+    Device-resident inputs; each timing is the median of `reps` solves."""
+    import torch
+    from voxeldvc_amd import seednet, synth
+    from voxeldvc_amd.engine import correlate_gpu as C
+    from voxeldvc_amd.engine.geometry_dvc import build_K_ref_laplacian
+    Ne = (n - 1) // h
+    f = synth.speckle_torch((n, n, n), synth.blobs_for((n, n, n)), seed=3).round().clamp_(0, 255)
+    modes = synth.fourier_params(16, (64, 256), 4.0, seed=7, extent=(n, n, n))
+    g = synth.warp_backward_torch(f, (n, n, n), modes, 1.0, seed=11, quantize=True)
+    fv = f.view(n, n, n).permute(2, 1, 0)
+    gv = g.view(n, n, n).permute(2, 1, 0)
+    kw = dict(K_ref_laplacian=build_K_ref_laplacian(), l0=4.5 * h, dynamic_mask=True,
+              freeze_mask_after=4, glt=0.0, disp=False, pcg_tol=1e-6, pcg_max_iter=2000)
+    net = seednet.SeedNet.load()
+
+    def solve(**extra):
+        lg = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        U, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, log=lg,
+                                             **{**kw, **extra})
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) * 1e3
+        gn = sum(len(lv['iters']) for lv in lg)
+        pcg = sum(q['pcg_iters'] for lv in lg for q in lv['iters'])
+        return U.double(), dt, gn, pcg
+
+    def timed(**extra):
+        runs = [solve(**extra) for _ in range(reps)]
+        i = int(np.argsort([r[1] for r in runs])[len(runs) // 2])
+        return runs[i]
+
+    Uo, t_o, gn_o, _ = solve(dtype=np.float64, maxiter=60, eps=1e-6)
+    rms = lambda U: float(((U - Uo) ** 2).mean().sqrt())
+    solve(maxiter=30, eps=1e-3)     # warm-up (level buffers, kernels)
+    Up, t_p, gn_p, pcg_p = timed(maxiter=30, eps=1e-3)
+    trace = []
+    solve(maxiter=30, eps=1e-3, storage='bf16', scales=(0,), learned_seed=net,
+          callback=lambda s, ik, U: trace.append(rms(U.double())))
+    k = next((i + 1 for i, r in enumerate(trace) if r <= 0.01), None)
+    if k is None:
+        raise RuntimeError('learned-seed solve did not reach 0.01 vox RMS: %r' % trace)
+    Ul, t_l, gn_l, pcg_l = timed(maxiter=k, eps=1e-3, storage='bf16', scales=(0,),
+                                 learned_seed=net)
+    Uc, t_c, gn_c, pcg_c = timed(maxiter=30, eps=1e-3, storage='bf16', scales=(0,),
+                                 learned_seed=net)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Un = net.predict(f, g, (n, n, n), Ne, Ne, Ne, h)
+    torch.cuda.synchronize()
+    t_net = (time.perf_counter() - t0) * 1e3
+    return {
+        "workload": "%d^3 8-bit speckle, 16 Fourier modes (wl 64-256, peak 4 vox) + N(0,1), "
+                    "h=%d (%d^3 elements), Laplacian l0=%.0f" % (n, h, Ne, 4.5 * h),
+        "criterion": "nodal RMS <= 0.01 vox vs the float64 solve (dU/U < 1e-6)",
+        "oracle_fp64": {"ms": round(t_o, 2), "gn_iters": gn_o},
+        "plain_fp32": {"ms": round(t_p, 2), "gn_iters": gn_p, "pcg_iters": pcg_p,
+                       "rms_vox": rms(Up), "levels": "2,1,0 + FFT prealign"},
+        "learned_mixed": {"ms_to_0.01": round(t_l, 2), "gn_iters": gn_l, "pcg_iters": pcg_l,
+                          "rms_vox": rms(Ul), "ms_to_convergence": round(t_c, 2),
+                          "gn_iters_to_convergence": gn_c, "rms_vox_converged": rms(Uc),
+                          "network_ms": round(t_net, 2), "network_rms_vox": rms(Un.double()),
+                          "per_iter_rms": [float('%.3g' % r) for r in trace]},
+        "ratio_to_0.01": round(t_l / t_p, 4),
+        "ratio_converged": round(t_c / t_p, 4),
+        "target_ratio": 1.0 / 3.0,
+    }
 
 
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hessian_matvec_traffic.json')
@@ -172,6 +263,12 @@ def main():
                                  "algorithmic_bytes": B_f32,
                                  "achieved_GBs": round(B_f32 / (kms_f * 1e-3) / 1e9, 1),
                                  "frac": round(B_f32 / (kms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del f16, mask0
+    if a.only in (None, 'f32'):
+        del G32
+    if not a.no_solve:
+        print('time to solve', file=sys.stderr, flush=True)
+        res["time_to_solve"] = time_to_solve(n=a.size, h=a.h, reps=a.solve_reps)
     if not a.no_cpu:
         print('cpu baseline', file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
