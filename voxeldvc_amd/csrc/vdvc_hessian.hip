@@ -251,8 +251,15 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
 
 }  // namespace
 
+// Blocks per CU the register budget is fitted to: 4 (occupancy 4 waves per
+// SIMD, <= 128 VGPRs) for 2-byte gradients, so a 513^3 h=8 grid's 4096
+// wavefronts run in one round; 3 for 4-byte gradients.
+#ifndef VDVC_VEC_MIN_BLOCKS
+#define VDVC_VEC_MIN_BLOCKS(GT) (sizeof(GT) == 2 ? 4 : 3)
+#endif
+
 template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V>
-__global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
+__global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
                                                             const float* __restrict__ v,
                                                             float* __restrict__ P) {
   __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 24 : 1];
@@ -384,6 +391,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
     uint32_t t[4];
   };
   const int NP = (ez1 - ez0) * h + (ez1 == M.Nz ? 1 : 0);  // planes of this tile
+  // pipeline depth: 2-byte gradients are unpacked into new registers before
+  // the reload, so two buffer sets keep two rows in flight; 4-byte values
+  // are used in place and need a third set
+  constexpr int PIPE = sizeof(GT) == 2 ? 2 : 3;
   RowBuf buf0, buf1, buf2;
   int cp = 0, ct = 0;  // prefetch cursor (plane, row), clamped to the tile's last row
   auto issue = [&](RowBuf& b) {
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
   };
   issue(buf0);
   issue(buf1);
-  issue(buf2);
+  if constexpr (PIPE == 3) issue(buf2);
 
   // plane state (updated when the flattened row walk enters a new z plane)
   int ez = ez0, tz = 0, ty = 0;
@@ -479,23 +490,32 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
     }
     plane_setup();
   };
-  auto process = [&](const RowBuf& cur) {
+  // a row's values unpacked from its raw words (for 2-byte types this
+  // frees the raw registers, which are then re-filled at once)
+  struct RowVals {
+    float gx[V], gy[V], gz[V];
+    uint32_t mk[V];
+    uint32_t t[4];
+  };
+  auto unpack_row = [&](const RowBuf& b, RowVals& o) {
+    unpack<GT, V>(b.g[0], o.gx);
+    unpack<GT, V>(b.g[1], o.gy);
+    unpack<GT, V>(b.g[2], o.gz);
+#pragma unroll
+    for (int j2 = 0; j2 < V; ++j2) o.mk[j2] = HASMASK ? mask_bit<V>(b.m, j2) : 1u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o.t[q] = b.t[q];
+  };
+  auto process = [&](const RowVals& cur) {
     const long long vi = rowbase + (long long)ty * M.pitch;
-    const Raw<GT, V>& c0 = cur.g[0];
-    const Raw<GT, V>& c1 = cur.g[1];
-    const Raw<GT, V>& c2 = cur.g[2];
-    const Raw<uint8_t, V>& cmk = cur.m;
     const uint32_t* ctl = cur.t;
     const int y = y0 + ty;
     const float fty = (float)(ty0 + ty);  // row within the element
     const float Wr = Wz * axis_weight(y, M.Nvy - 1);
-    float gx[V], gy[V], gz[V];
-    unpack<GT, V>(c0, gx);
-    unpack<GT, V>(c1, gy);
-    unpack<GT, V>(c2, gz);
-    uint32_t mk[V];
-#pragma unroll
-    for (int j2 = 0; j2 < V; ++j2) mk[j2] = HASMASK ? mask_bit<V>(cmk, j2) : 1u;
+    const float* gx = cur.gx;
+    const float* gy = cur.gy;
+    const float* gz = cur.gz;
+    const uint32_t* mk = cur.mk;
     float S[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
     float ux0 = 0, uy0 = 0, uz0 = 0, dxx = 0, dxy = 0, dxz = 0;
     if (MODE == MODE_MATVEC) {
@@ -631,20 +651,43 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_vec_kernel(MeshDev M, TilesDev
   element_setup();
   plane_setup();
   const int nrows_tile = NP * nrows;
-  // three rows per iteration with statically assigned buffer sets
-  for (int rr = 0; rr < nrows_tile; rr += 3) {
-    process(buf0);
-    issue(buf0);
-    advance();
-    if (rr + 1 < nrows_tile) {
-      process(buf1);
-      issue(buf1);
+  if constexpr (PIPE == 2) {
+    // two buffer sets: a row is unpacked (its raw registers are free after
+    // that), its set re-filled with the row two ahead, then processed
+    for (int rr = 0; rr < nrows_tile; rr += 2) {
+      RowVals cv;
+      unpack_row(buf0, cv);
+      issue(buf0);
+      process(cv);
       advance();
+      if (rr + 1 < nrows_tile) {
+        unpack_row(buf1, cv);
+        issue(buf1);
+        process(cv);
+        advance();
+      }
     }
-    if (rr + 2 < nrows_tile) {
-      process(buf2);
-      issue(buf2);
+  } else {
+    // three buffer sets: a row is processed straight from its set, which is
+    // then re-filled with the row three ahead
+    for (int rr = 0; rr < nrows_tile; rr += 3) {
+      RowVals cv;
+      unpack_row(buf0, cv);
+      process(cv);
+      issue(buf0);
       advance();
+      if (rr + 1 < nrows_tile) {
+        unpack_row(buf1, cv);
+        process(cv);
+        issue(buf1);
+        advance();
+      }
+      if (rr + 2 < nrows_tile) {
+        unpack_row(buf2, cv);
+        process(cv);
+        issue(buf2);
+        advance();
+      }
     }
   }
   emit(ez1 - ez0);
