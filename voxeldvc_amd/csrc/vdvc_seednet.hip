@@ -110,6 +110,157 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvShape S, const float*
   }
 }
 
+// Channel-specialised forward kernels (same GEMM, depth order tap*C + c):
+// with C a multiple of 16 a 16-deep chunk is 16 channels of one tap, so a
+// lane's 4 operand values are one aligned 16-byte load and the tap offsets
+// are compile-time; with C = 2 a lane's 4 values are two 8-byte loads of two
+// taps.  Both replace the generic kernel's per-element index division.
+__device__ __forceinline__ bf16x4_t to_bf16x4(float4 v) {
+  bf16x4_t r;
+  r[0] = bf16_rne(v.x); r[1] = bf16_rne(v.y); r[2] = bf16_rne(v.z); r[3] = bf16_rne(v.w);
+  return r;
+}
+
+template <int NT, int CC>
+__global__ __launch_bounds__(256) void conv_fwd_c16_kernel(ConvShape S,
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ w,
+                                                          const float* __restrict__ bias,
+                                                          int relu, float* __restrict__ y) {
+  constexpr int C = 16 * CC;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long P = (long long)S.N * S.Do * S.Ho * S.Wo;
+  const long long p0 = (long long)wave * 16;
+  if (p0 >= P) return;
+  const int r = lane & 15, q = lane >> 4;
+  const long long pa = p0 + r;
+  const bool va = pa < P;
+  int n = 0, z = 0, yy = 0, xx = 0;
+  if (va) {
+    long long t = pa;
+    xx = (int)(t % S.Wo); t /= S.Wo;
+    yy = (int)(t % S.Ho); t /= S.Ho;
+    z = (int)(t % S.Do); n = (int)(t / S.Do);
+  }
+  const int bz = z * S.stride - 1, by = yy * S.stride - 1, bx = xx * S.stride - 1;
+  f32x4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 3
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+    const int iz = bz + dz, iy = by + dy, ix = bx + dx;
+    const bool v = va && iz >= 0 && iz < S.D && iy >= 0 && iy < S.H && ix >= 0 && ix < S.W;
+    const float* xp =
+        x + ((((long long)n * S.D + (v ? iz : 0)) * S.H + (v ? iy : 0)) * S.W + (v ? ix : 0)) * C +
+        4 * q;
+#pragma unroll
+    for (int cc = 0; cc < CC; ++cc) {
+      const float4 av = v ? *reinterpret_cast<const float4*>(xp + cc * 16) : zero4;
+      const bf16x4_t a = to_bf16x4(av);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int o = t * 16 + r;
+        const float4 wv = o < S.O ? *reinterpret_cast<const float4*>(
+                                        w + ((long long)o * 27 + tap) * C + cc * 16 + 4 * q)
+                                  : zero4;
+        acc[t] = mfma16(a, to_bf16x4(wv), acc[t]);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int o = t * 16 + r;
+    if (o >= S.O) continue;
+    const float bo = bias ? bias[o] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long p = p0 + 4 * q + j;
+      if (p >= P) continue;
+      float v = acc[t][j] + bo;
+      if (relu) v = v > 0.0f ? v : 0.0f;
+      y[p * S.O + o] = v;
+    }
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void conv_fwd_c2_kernel(ConvShape S, const float* __restrict__ x,
+                                                         const float* __restrict__ w,
+                                                         const float* __restrict__ bias, int relu,
+                                                         float* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long P = (long long)S.N * S.Do * S.Ho * S.Wo;
+  const long long p0 = (long long)wave * 16;
+  if (p0 >= P) return;
+  const int r = lane & 15, q = lane >> 4;
+  const long long pa = p0 + r;
+  const bool va = pa < P;
+  int n = 0, z = 0, yy = 0, xx = 0;
+  if (va) {
+    long long t = pa;
+    xx = (int)(t % S.Wo); t /= S.Wo;
+    yy = (int)(t % S.Ho); t /= S.Ho;
+    z = (int)(t % S.Do); n = (int)(t / S.Do);
+  }
+  const int bz = z * S.stride - 1, by = yy * S.stride - 1, bx = xx * S.stride - 1;
+  f32x4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const float2 zero2 = make_float2(0.f, 0.f);
+  // depth k = 2*tap + c: chunk k0 holds taps k0/2 .. k0/2 + 7; lane q holds
+  // taps k0/2 + 2q and k0/2 + 2q + 1 (both channels each)
+#pragma unroll
+  for (int k0 = 0; k0 < 54; k0 += 16) {
+    float2 av[2], wv[NT][2];
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int tap = k0 / 2 + 2 * q + h2;
+      av[h2] = zero2;
+      if (tap < 27) {
+        const int iz = bz + tap / 9, iy = by + (tap / 3) % 3, ix = bx + tap % 3;
+        if (va && iz >= 0 && iz < S.D && iy >= 0 && iy < S.H && ix >= 0 && ix < S.W)
+          av[h2] = *reinterpret_cast<const float2*>(
+              x + ((((long long)n * S.D + iz) * S.H + iy) * S.W + ix) * 2);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int o = t * 16 + r;
+        wv[t][h2] = (tap < 27 && o < S.O)
+                        ? *reinterpret_cast<const float2*>(w + ((long long)o * 27 + tap) * 2)
+                        : zero2;
+      }
+    }
+    bf16x4_t a;
+    a[0] = bf16_rne(av[0].x); a[1] = bf16_rne(av[0].y);
+    a[2] = bf16_rne(av[1].x); a[3] = bf16_rne(av[1].y);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      bf16x4_t b;
+      b[0] = bf16_rne(wv[t][0].x); b[1] = bf16_rne(wv[t][0].y);
+      b[2] = bf16_rne(wv[t][1].x); b[3] = bf16_rne(wv[t][1].y);
+      acc[t] = mfma16(a, b, acc[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int o = t * 16 + r;
+    if (o >= S.O) continue;
+    const float bo = bias ? bias[o] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long p = p0 + 4 * q + j;
+      if (p >= P) continue;
+      float v = acc[t][j] + bo;
+      if (relu) v = v > 0.0f ? v : 0.0f;
+      y[p * S.O + o] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // backward data: dx[qpos][c] = sum_{tap,o} w[o][tap][c] dy[p][o] over the
 // outputs p with p*s + d - 1 = qpos; optionally times (act[qpos][c] > 0),
@@ -407,10 +558,18 @@ int launch_conv_fwd(int N, int D, int H, int W, int C, int O, int stride, const 
   const long long P = (long long)N * S.Do * S.Ho * S.Wo;
   const long long waves = (P + 15) / 16;
   const unsigned nb = (unsigned)((waves + 3) / 4);
-  if (O <= 16)
-    hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(nb), dim3(256), 0, st, S, x, w, b, relu, y);
-  else
-    hipLaunchKernelGGL(conv_fwd_kernel<2>, dim3(nb), dim3(256), 0, st, S, x, w, b, relu, y);
+#define VDVC_FWD(K) hipLaunchKernelGGL(K, dim3(nb), dim3(256), 0, st, S, x, w, b, relu, y)
+  const bool two = O > 16;
+  if (C == 16) {
+    if (two) VDVC_FWD((conv_fwd_c16_kernel<2, 1>)); else VDVC_FWD((conv_fwd_c16_kernel<1, 1>));
+  } else if (C == 32) {
+    if (two) VDVC_FWD((conv_fwd_c16_kernel<2, 2>)); else VDVC_FWD((conv_fwd_c16_kernel<1, 2>));
+  } else if (C == 2) {
+    if (two) VDVC_FWD(conv_fwd_c2_kernel<2>); else VDVC_FWD(conv_fwd_c2_kernel<1>);
+  } else {
+    if (two) VDVC_FWD(conv_fwd_kernel<2>); else VDVC_FWD(conv_fwd_kernel<1>);
+  }
+#undef VDVC_FWD
   return (int)hipGetLastError();
 }
 
