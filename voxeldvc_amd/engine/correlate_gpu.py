@@ -322,12 +322,15 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
                   K_ref_laplacian=None, reg_exponent=2, c_reg=1.0, reg_type='laplacian',
                   K_ref_elastic=None, eps_reg=1e-3, dynamic_mask=False, freeze_mask_after=None,
                   U0=None, maxiter=30, eps=1e-3, pcg_tol=1e-6, pcg_max_iter=2000, disp=True,
-                  dtype=np.float32, storage='fp32', log=None, pcg_check_every=8, callback=None):
+                  dtype=np.float32, storage='fp32', log=None, pcg_check_every=8, callback=None,
+                  refine_fp64=0):
     """Matrix-free GN DVC correlation on one level; returns (U, res, dU)
     (correlate_gpu.py:828-1140).  Extensions: `storage` (see module doc),
     `log` (list receiving one dict per GN iteration), `pcg_check_every`
     (PCG iterations between host polls of the device stop flag) and
-    `callback(ik, U)` (called after every GN update with the device U)."""
+    `callback(ik, U)` (called after every GN update with the device U) and
+    `refine_fp64` (number of FP32-against-FP64 iterative-refinement steps
+    after the GN loop; U is then returned as float64)."""
     Nvx, Nvy, Nvz = Nx * h + 1, Ny * h + 1, Nz * h + 1
     Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
     Nvox = Nvx * Nvy * Nvz
@@ -349,7 +352,8 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
                           U0=None if U0 is None else L.to_dev_flat(U0, wd), maxiter=maxiter,
                           eps=eps,
                           pcg_tol=pcg_tol, pcg_max_iter=pcg_max_iter, disp=disp, log=log,
-                          pcg_check_every=pcg_check_every, callback=callback)
+                          pcg_check_every=pcg_check_every, callback=callback,
+                          refine_fp64=refine_fp64)
     return _as_out(xp, U), _as_out(xp, res), _as_out(xp, dU)
 
 
@@ -379,7 +383,8 @@ class _Level:
 
     def solve(self, g_origin, glt, l0, K_ref_laplacian, reg_exponent, c_reg, reg_type,
               K_ref_elastic, eps_reg, dynamic_mask, freeze_mask_after, U0, maxiter, eps,
-              pcg_tol, pcg_max_iter, disp, log, pcg_check_every, callback=None):
+              pcg_tol, pcg_max_iter, disp, log, pcg_check_every, callback=None,
+              refine_fp64=0):
         t = L.require_gpu()
         c = L.ctx()
         Nx, Ny, Nz, h = self.M
@@ -540,7 +545,72 @@ class _Level:
                           % (ik + 1, stdr, err, n_active, self.Nvox, 100 * n_active / self.Nvox))
             if err < eps:
                 break
+        if refine_fp64 and self.wd != 'float64':
+            hm = (frozen if frozen is not None else mask) if dynamic_mask else mask0
+            U = self._refine_fp64(U, frozen, hm, mask0, g_origin, glt, reg, rk, llf,
+                                  l0 is not None, hmask, Minv, pcg_tol, pcg_max_iter,
+                                  pcg_check_every, int(refine_fp64), log, disp)
         return U, L.unpitch(res, self.shape), dU
+
+    def _refine_fp64(self, U, frozen, hm, mask0, g_origin, glt, reg, rk, llf, use_reg, hmask,
+                     Minv, pcg_tol, pcg_max_iter, pcg_check_every, nsteps, log, disp):
+        """FP32-against-FP64 iterative refinement of the converged GN state.
+        Each step evaluates the GN residual b(U) - ll R U entirely in FP64
+        (float64 images and gradients -- exact copies of the stored images --,
+        FP64 sampling, residual, right-hand side and regularizer; the b-mask
+        follows the solver's rule: the frozen mask if the solve froze one,
+        else the mask sampled at the current U), solves (H + ll R) dU = b with
+        the working-precision operator and PCG, and accumulates U in FP64.
+        The fixed point is therefore the FP64 solve's.  Returns U as float64."""
+        t = L.require_gpu()
+        c = L.ctx()
+        Nx, Ny, Nz, h = self.M
+        Nvox, Ndof = self.Nvox_p, self.Ndof
+        f64 = self.f.double()
+        g64 = self.g.double()
+        G64 = [L.empty(Nvox, 'float64') for _ in range(3)]
+        m64 = L.empty(Nvox, 'uint8')
+        L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, f64, L.F64, float(glt), G64[0], G64[1],
+               G64[2], L.F64, m64)
+        s3 = (L._d * 3)()
+        L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, f64, L.F64, None, 1, float(glt), s3)
+        mean0, std0 = _stats_mean_std(s3)
+        U64 = U.double()
+        gvals = L.zeros(Nvox, 'float64')
+        live = L.zeros(Nvox, 'uint8')
+        res = L.zeros(Nvox, 'float64')
+        b64 = L.empty(Ndof, 'float64')
+        RU = L.empty(Ndof, 'float64')
+        dU = L.zeros(Ndof)
+        s4 = (L._d * 4)()
+        it_ = (L._i * 1)()
+        rr_ = (L._d * 1)()
+        Gh, mh = hmask(hm)
+        for k in range(nsteps + 1):
+            L.call('vdvc_sample', c, Nx, Ny, Nz, h, U64, g64, L.F64, self.gshp[0], self.gshp[1],
+                   self.gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]),
+                   float(glt), mask0, frozen, gvals, live, s3)
+            bmask = frozen if frozen is not None else live
+            g_mean, std1 = _stats_mean_std(s3)
+            L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, G64[0], G64[1], G64[2], L.F64, f64,
+                   L.F64, float(mean0), gvals, float(g_mean), float(-(std0 / std1)), bmask,
+                   live, res, b64, None, None, None, s4)
+            if use_reg:
+                reg.matvec(self.M, U64, RU)
+                K.axpby(-llf, RU, 1.0, b64)
+            rn = math.sqrt(K.dot(b64, b64))
+            if log is not None:
+                log.append(dict(refine=k, residual_fp64=rn))
+            if disp:
+                print("FP64 refinement %d | ||b - ll R U||_2 = %.3e" % (k, rn))
+            if k == nsteps:
+                break
+            b32 = b64.float()
+            L.call('vdvc_pcg', c, Nx, Ny, Nz, h, Gh[0], Gh[1], Gh[2], self.gtype, mh,
+                   rk[0], rk[1], rk[2], rk[3], rk[4], llf, b32, Minv, dU, float(pcg_tol),
+                   int(pcg_max_iter), int(pcg_check_every), it_, rr_)
+            U64 += dU.double()
+        return U64
 
 
 # ---------------------------------------------------------------------------
@@ -609,6 +679,7 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
     log = kwargs.pop('log', None)
     cb = kwargs.pop('callback', None)
     seed_net = kwargs.pop('learned_seed', None)
+    refine = kwargs.pop('refine_fp64', 0)
     if seed_net is not None:
         # the network's nodal prediction replaces the coarse levels and the
         # pre-alignment: only the finest requested level is solved
@@ -655,6 +726,7 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
         sub = None if log is None else []
         lcb = None if cb is None else (lambda ik, Ud, _s=isc: cb(_s, ik, Ud))
         U, res, dU = lv.solve(g_origin=go_i, l0=l0_i, U0=U, log=sub, callback=lcb,
+                              refine_fp64=refine if isc == scales[-1] else 0,
                               **_solve_kwargs(kwargs))
         if log is not None:
             log.append(dict(scale=isc, iters=sub))
