@@ -19,7 +19,12 @@
  *   nodal vectors: DOF 3*(i + j*(Nx+1) + k*(Nx+1)*(Ny+1)) + c;
  *   deformed image g for sampling: x fastest, dims (gnx, gny, gnz).
  * Element types: VDVC_F32 float32, VDVC_BF16 bfloat16, VDVC_F16 float16,
- * VDVC_F64 float64.
+ * VDVC_F64 float64; the post-solve stages also read the images' integer
+ * dtypes VDVC_U8, VDVC_U16, VDVC_I16 and VDVC_BOOL.
+ *
+ * Post-solve stages (vdvc_interp_field ... vdvc_quality_cell) take dense
+ * voxel arrays, no row pitch: index x + Nvx*(y + Nvy*z), the reference's
+ * ravel(order='F').
  *
  * Working type.  Nodal vectors (v, U, b, x, diagonals, M_inv), sampled
  * images, residuals and the node partials are float64 on the FP64 path and
@@ -43,9 +48,12 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 2
+#define VDVC_ABI_VERSION 3
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
+/* image element types of the post-solve stages (the images' own dtypes;
+ * VDVC_BOOL is one byte, 0 or 1) */
+enum { VDVC_U8 = 4, VDVC_U16 = 5, VDVC_I16 = 6, VDVC_BOOL = 7 };
 enum { VDVC_REG_NONE = 0, VDVC_REG_K = 1, VDVC_REG_EQGAP = 2 };
 
 typedef struct vdvc_ctx vdvc_ctx;
@@ -265,6 +273,46 @@ int vdvc_mse(vdvc_ctx* ctx, int64_t n, const float* d_pred, const float* d_label
  * standardised by its own mean and standard deviation. */
 int vdvc_seed_input(vdvc_ctx* ctx, const void* d_f, const void* d_g, int itype, int nx, int ny,
                     int nz, int s, float* d_out);
+
+/* ---- post-solve voxel stages (dense voxel arrays) ---- */
+
+/* Nodal field U at every voxel with the voxel-centre Hex8 stencil; d_out is
+ * (Nvox, 3) or, with channels_first, (3, Nvox), in U's type (rtype F32 /
+ * F64).  Replaces interp_field_vox (engine/correlate_gpu.py:160-198). */
+int vdvc_interp_field(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                      void* d_out, int channels_first);
+
+/* Active mask mask0 & g.InBounds(x + u - origin) & g(x + u - origin) > glt,
+ * g sampled trilinearly in its own dtype (gtype).  d_mask0 may be NULL (all
+ * true); d_mask_out is one byte per voxel.  Replaces compute_active_mask
+ * (engine/correlate_gpu.py:284-338). */
+int vdvc_active_mask(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                     const void* d_g, int gtype, int gnx, int gny, int gnz, double gox, double goy,
+                     double goz, const uint8_t* d_mask0, double glt, uint8_t* d_mask_out);
+
+/* d_out(x) = field(x - u(x)) on the (Nvx, Nvy, Nvz) grid, in the field's type
+ * (ftype), with scipy.ndimage.map_coordinates semantics for order 0 / 1 and
+ * mode 0 'nearest' / 1 'constant' (cval).  Replaces push_field_forward /
+ * push_reference_forward (engine/write_output.py:571-622). */
+int vdvc_push_forward(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                      const void* d_field, int ftype, int order, int mode, double cval,
+                      void* d_out);
+
+/* Zero-normalised cross-correlation of f and g (or of f and f - g when
+ * g_is_residual) over d_mask (NULL: all n entries), FP64 two-pass; *h_out is
+ * NaN for an empty or constant selection.  Replaces compute_zncc
+ * (engine/write_output.py:658-690) and its caller's g_warped = f - res
+ * (:786-787). */
+int vdvc_zncc(vdvc_ctx* ctx, int64_t n, const void* d_f, int ftype, const void* d_g, int gtype,
+              int g_is_residual, const uint8_t* d_mask, double* h_out);
+
+/* Per-element population std of the residual and ZNCC of (ref, ref - res)
+ * over the element's (h+1)^3 voxels with res != 0; NaN below 8 active voxels
+ * (and ZNCC for zero variance).  d_res rtype (F32 / F64), d_ref ftype; d_std
+ * and d_zncc are (Nx*Ny*Nz,) float64 in element order i + Nx*(j + Ny*k).
+ * Replaces compute_correlation_quality_cell (engine/write_output.py:380-447). */
+int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_res, int rtype,
+                      const void* d_ref, int ftype, double* d_std, double* d_zncc);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
-ABI_VERSION = 2
+U8, U16, I16, BOOL = 4, 5, 6, 7
+ABI_VERSION = 3
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -67,6 +68,12 @@ SIGNATURES = {
     'vdvc_adam_step': [_vp, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _i],
     'vdvc_mse': [_vp, _i64, _vp, _vp, _vp, _dp],
     'vdvc_seed_input': [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp],
+    'vdvc_interp_field': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i],
+    'vdvc_active_mask': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _i, _i, _i, _d, _d, _d, _vp,
+                         _d, _vp],
+    'vdvc_push_forward': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp],
+    'vdvc_zncc': [_vp, _i64, _vp, _i, _vp, _i, _i, _vp, _dp],
+    'vdvc_quality_cell': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _vp, _vp],
 }
 
 _lib = None
@@ -214,6 +221,80 @@ def to_dev_volume(a, dtype='float32'):
     else:
         host = np.ravel(arr, order='F')
     return t.from_numpy(np.ascontiguousarray(host)).to(device='cuda', dtype=tt), shp
+
+
+# element types the post-solve stages read in the images' own dtype:
+# dtype name -> (library code, device dtype name)
+IMAGE_TYPES = {'float32': (F32, 'float32'), 'float64': (F64, 'float64'),
+               'uint8': (U8, 'uint8'), 'uint16': (U16, 'uint16'), 'int16': (I16, 'int16'),
+               'bool': (BOOL, 'uint8')}
+
+
+def image_type(a):
+    """(library code, device dtype name) of an image (numpy array or tensor)
+    for the post-solve stages; ValueError for other dtypes."""
+    name = str(a.dtype).replace('torch.', '')
+    if name not in IMAGE_TYPES:
+        raise ValueError('unsupported image dtype %s (float32, float64, uint8, uint16, int16 '
+                         'or bool)' % name)
+    return IMAGE_TYPES[name]
+
+
+def _image_bits(d):
+    """Device storage of an image tensor: bool as uint8, uint16 as int16 bits
+    (the library reads the raw element with the type code)."""
+    t = torch()
+    if d.dtype == t.bool:
+        return d.to(t.uint8)
+    if d.dtype == t.uint16:
+        return d.view(t.int16)
+    return d
+
+
+def _host_bits(arr):
+    if arr.dtype == np.bool_:
+        return arr.view(np.uint8)
+    if arr.dtype == np.uint16:
+        return arr.view(np.int16)
+    return arr
+
+
+def to_dev_image(a):
+    """(nx, ny, nz) image in its own dtype -> (device tensor laid out x
+    fastest, shape, type code)."""
+    code, _ = image_type(a)
+    t = require_gpu()
+    if is_tensor(a):
+        if a.dim() != 3:
+            raise ValueError('volume must be 3-D')
+        d = _image_bits(a.detach().to(device='cuda'))
+        return d.permute(2, 1, 0).contiguous().view(-1), tuple(int(s) for s in a.shape), code
+    arr = np.asarray(a)
+    if arr.ndim != 3:
+        raise ValueError('volume must be 3-D')
+    host = _host_bits(np.ravel(arr, order='F'))
+    return t.from_numpy(np.ascontiguousarray(host)).to(device='cuda'), arr.shape, code
+
+
+def to_dev_image_flat(a):
+    """1-D array in its own dtype -> (contiguous device tensor, type code)."""
+    code, _ = image_type(a)
+    t = require_gpu()
+    if is_tensor(a):
+        return _image_bits(a.detach().to(device='cuda')).contiguous().view(-1), code
+    host = _host_bits(np.ascontiguousarray(np.asarray(a).reshape(-1)))
+    return t.from_numpy(host).to(device='cuda'), code
+
+
+def image_to_host(d, dtype):
+    """Device image storage (see to_dev_image) -> NumPy array of `dtype`."""
+    h = to_host(d)
+    dtype = np.dtype(dtype)
+    if dtype == np.bool_:
+        return h.astype(bool)
+    if dtype == np.uint16:
+        return h.view(np.uint16)
+    return h
 
 
 def rtype_of(x):

@@ -3,6 +3,7 @@
 // host; every numeric step is one of the gfx950 kernels in this directory.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -250,6 +251,38 @@ int launch_restrict_timed(vdvc_ctx* ctx, int mode, int gtype, const MeshDev& M, 
   else
     VDVC_LAUNCH(vdvc::launch_restrict(mode, gtype, in.mask != nullptr, ftype, fold, M, T, in, v,
                                       ctx->P.p, ctx->stream));
+  return timed_event(ctx);
+}
+
+// element types of the post-solve stages' images (the images' own dtypes)
+int itype_ok(int t) {
+  return t == VDVC_F32 || t == VDVC_F64 || t == VDVC_U8 || t == VDVC_U16 || t == VDVC_I16 ||
+         t == VDVC_BOOL;
+}
+
+// dense voxel grid of a mesh for the post-solve kernels (grid y / z limits)
+int post_grid_ok(const MeshDev& M) {
+  if ((M.Nvy + 3) / 4 > 65535 || M.Nvz > 65535)
+    return fail(-1, "voxel grid %d x %d x %d exceeds the launch grid (Nvy <= 262140, Nvz <= 65535)",
+                M.Nvx, M.Nvy, M.Nvz);
+  return 0;
+}
+
+PostIn post_in(const MeshDev& M, const void* U) {
+  PostIn P;
+  std::memset(&P, 0, sizeof(P));
+  P.Nx = M.Nx; P.Ny = M.Ny; P.Nz = M.Nz; P.h = M.h;
+  P.Nvx = M.Nvx; P.Nvy = M.Nvy; P.Nvz = M.Nvz;
+  P.nnx = M.nnx; P.nny = M.nny;
+  P.Nvox = M.Nvox;
+  P.U = U;
+  return P;
+}
+
+int run_post_timed(vdvc_ctx* ctx, int op, int rtype, const PostIn& P) {
+  int rc = timed_event(ctx);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_post_voxel(op, rtype, P, ctx->stream));
   return timed_event(ctx);
 }
 
@@ -1059,6 +1092,112 @@ int vdvc_seed_input(vdvc_ctx* ctx, const void* d_f, const void* d_g, int itype, 
   VDVC_LAUNCH(vdvc::launch_seed_input(itype, d_f, d_g, nx, ny, nz, s, ctx->stats.as<double>(),
                                       nparts, ctx->red.as<double>(), d_out, ctx->stream));
   return 0;
+}
+
+int vdvc_interp_field(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                      void* d_out, int channels_first) {
+  if (!ctx || !d_U || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "U type must be F32 or F64");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  if ((rc = post_grid_ok(M))) return rc;
+  PostIn P = post_in(M, d_U);
+  P.out = d_out;
+  P.channels_first = channels_first ? 1 : 0;
+  return run_post_timed(ctx, POST_INTERP, rtype, P);
+}
+
+int vdvc_active_mask(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                     const void* d_g, int gtype, int gnx, int gny, int gnz, double gox, double goy,
+                     double goz, const uint8_t* d_mask0, double glt, uint8_t* d_mask_out) {
+  if (!ctx || !d_U || !d_g || !d_mask_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "U type must be F32 or F64");
+  if (!itype_ok(gtype)) return fail(-1, "unsupported image type %d", gtype);
+  if (gnx < 1 || gny < 1 || gnz < 1) return fail(-1, "invalid image dims");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  if ((rc = post_grid_ok(M))) return rc;
+  PostIn P = post_in(M, d_U);
+  P.img = d_g; P.itype = gtype; P.gnx = gnx; P.gny = gny; P.gnz = gnz;
+  P.ox = gox; P.oy = goy; P.oz = goz; P.glt = glt; P.mask0 = d_mask0;
+  P.out = d_mask_out;
+  return run_post_timed(ctx, POST_MASK, rtype, P);
+}
+
+int vdvc_push_forward(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int rtype,
+                      const void* d_field, int ftype, int order, int mode, double cval,
+                      void* d_out) {
+  if (!ctx || !d_U || !d_field || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "U type must be F32 or F64");
+  if (!itype_ok(ftype)) return fail(-1, "unsupported field type %d", ftype);
+  if (order != 0 && order != 1) return fail(-1, "order must be 0 or 1");
+  if (mode != 0 && mode != 1) return fail(-1, "mode must be 0 (nearest) or 1 (constant)");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  if ((rc = post_grid_ok(M))) return rc;
+  PostIn P = post_in(M, d_U);
+  P.img = d_field; P.itype = ftype; P.gnx = M.Nvx; P.gny = M.Nvy; P.gnz = M.Nvz;
+  P.order = order; P.mode = mode; P.cval = cval;
+  P.out = d_out;
+  return run_post_timed(ctx, POST_PUSH, rtype, P);
+}
+
+int vdvc_zncc(vdvc_ctx* ctx, int64_t n, const void* d_f, int ftype, const void* d_g, int gtype,
+              int g_is_residual, const uint8_t* d_mask, double* h_out) {
+  if (!ctx || !h_out) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (n > 0 && (!d_f || !d_g)) return fail(-1, "NULL argument");
+  if (!itype_ok(ftype) || !itype_ok(gtype)) return fail(-1, "unsupported element type");
+  const double nan = std::nan("");
+  if (n == 0) {
+    *h_out = nan;
+    return 0;
+  }
+  const int nb = (int)std::min<int64_t>(1024, (n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  int rc;
+  if ((rc = ctx->stats.ensure((size_t)nb * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  ZnccIn Z;
+  Z.n = n; Z.f = d_f; Z.ftype = ftype; Z.g = d_g; Z.gtype = gtype;
+  Z.g_is_residual = g_is_residual ? 1 : 0; Z.mask = d_mask;
+  Z.chunk = (n + nb - 1) / nb;
+  double* part = ctx->stats.as<double>();
+  double* red = ctx->red.as<double>();
+  VDVC_LAUNCH(vdvc::launch_zncc_pass(0, Z, nullptr, part, nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(part, nb, 3, red, ctx->stream));
+  double s0[3];
+  if ((rc = fetch(ctx, red, 3 * sizeof(double), s0))) return rc;
+  if (!(s0[0] > 0.0)) {  // empty selection: the mean is nan
+    *h_out = nan;
+    return 0;
+  }
+  VDVC_LAUNCH(vdvc::launch_zncc_pass(1, Z, red, part, nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(part, nb, 3, red + 4, ctx->stream));
+  double s1[3];
+  if ((rc = fetch(ctx, red + 4, 3 * sizeof(double), s1))) return rc;
+  const double den = std::sqrt(s1[1] * s1[2]);
+  *h_out = den > 0.0 ? s1[0] / den : nan;
+  return 0;
+}
+
+int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_res, int rtype,
+                      const void* d_ref, int ftype, double* d_std, double* d_zncc) {
+  if (!ctx || !d_res || !d_ref || !d_std || !d_zncc) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "residual type must be F32 or F64");
+  if (!itype_ok(ftype)) return fail(-1, "unsupported image type %d", ftype);
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  QualIn Q;
+  Q.Nx = Nx; Q.Ny = Ny; Q.Nz = Nz; Q.h = h; Q.Nvx = M.Nvx; Q.Nvy = M.Nvy;
+  Q.res = d_res; Q.rtype = rtype; Q.ref = d_ref; Q.ftype = ftype;
+  Q.std_out = d_std; Q.zncc_out = d_zncc; Q.min_active = 8;
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_quality_cell(Q, ctx->stream));
+  return timed_event(ctx);
 }
 
 }  // extern "C"

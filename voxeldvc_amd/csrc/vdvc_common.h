@@ -64,6 +64,8 @@ struct TilesDev {
 };
 
 enum { GT_F32 = 0, GT_BF16 = 1, GT_F16 = 2, GT_F64 = 3 };
+// image element types of the post-solve stages (the images' own dtypes)
+enum { GT_U8 = 4, GT_U16 = 5, GT_I16 = 6, GT_B8 = 7 };
 
 // Arithmetic type of a storage type: double storage selects the FP64 path
 // (every voxel and node computation in double, the reference's
@@ -71,14 +73,21 @@ enum { GT_F32 = 0, GT_BF16 = 1, GT_F16 = 2, GT_F64 = 3 };
 template <typename T> struct RealOf { using type = float; };
 template <> struct RealOf<double> { using type = double; };
 
-// correctly rounded single operations without contraction (the reference's
-// element-wise NumPy steps)
-__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
-__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+// Correctly rounded single operations that are never fused with a
+// neighbouring one (the reference's element-wise NumPy steps).  HIP's
+// __fadd_rn / __fmul_rn compile to plain operations, which FP contraction
+// fuses into FMAs; the pragma takes the contract flag off these.
+#define VDVC_RN_OP(T, name, op)                                    \
+  __device__ __forceinline__ T name(T a, T b) {                     \
+    _Pragma("clang fp contract(off)") return a op b;                \
+  }
+VDVC_RN_OP(float, add_rn, +)
+VDVC_RN_OP(float, sub_rn, -)
+VDVC_RN_OP(float, mul_rn, *)
+VDVC_RN_OP(double, add_rn, +)
+VDVC_RN_OP(double, sub_rn, -)
+VDVC_RN_OP(double, mul_rn, *)
+#undef VDVC_RN_OP
 
 __device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }

@@ -538,8 +538,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
       for (int j = 0; j < V; ++j) {
         // (g - g_mean) * (-(std0/std1)) + (f - mean0), float32 steps in
         // the reference's order, no contraction; zero outside b_mask
-        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv[j], gmean_f), kscale_f),
-                             __fsub_rn(fv[j], mean0_f));
+        float rv = add_rn(mul_rn(sub_rn(gv[j], gmean_f), kscale_f),
+                             sub_rn(fv[j], mean0_f));
         rv = mk[j] ? rv : 0.0f;
         resv[j] = rv;
         if (act && lv[j]) {
@@ -602,8 +602,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
       } else if (MODE == MODE_RHS) {
         const float fv = ld_val(fim, ti);
         const float gv = gvals_f[ti];
-        float rv = __fadd_rn(__fmul_rn(__fsub_rn(gv, gmean_f), kscale_f),
-                             __fsub_rn(fv, mean0_f));
+        float rv = add_rn(mul_rn(sub_rn(gv, gmean_f), kscale_f),
+                             sub_rn(fv, mean0_f));
         rv = tm ? rv : 0.0f;
         res_f[ti] = rv;
         if (in.live[ti]) {

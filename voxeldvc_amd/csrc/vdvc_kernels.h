@@ -30,6 +30,43 @@ struct SampleIn {
   double* stats;         // per-wave partials [nwaves][3]
 };
 
+// post-solve voxel stages on dense voxel arrays (flat index x + Nvx*(y + Nvy*z))
+enum { POST_INTERP = 0, POST_PUSH = 1, POST_MASK = 2 };
+struct PostIn {
+  int Nx, Ny, Nz, h;
+  int Nvx, Nvy, Nvz;
+  int nnx, nny;
+  long long Nvox;
+  const void* U;          // nodal vector, R
+  const void* img;        // sampled image, x fastest, (gnx, gny, gnz), type itype
+  int itype;
+  int gnx, gny, gnz;
+  double ox, oy, oz;      // mask: image origin in the voxel frame
+  int order, mode;        // push: spline order 0/1, mode 0 nearest / 1 constant
+  double cval, glt;
+  const uint8_t* mask0;   // mask: optional reference mask
+  void* out;              // interp: R; push: itype; mask: uint8
+  int channels_first;     // interp: (3, Nvox) instead of (Nvox, 3)
+};
+
+struct QualIn {
+  int Nx, Ny, Nz, h, Nvx, Nvy;
+  const void* res; int rtype;   // per-voxel residual (F32 / F64)
+  const void* ref; int ftype;   // reference image (x fastest)
+  double* std_out;              // (Ne,) element order x fastest
+  double* zncc_out;
+  int min_active;
+};
+
+struct ZnccIn {
+  long long n;
+  const void* f; int ftype;
+  const void* g; int gtype;
+  int g_is_residual;            // g := f - g
+  const uint8_t* mask;          // optional
+  long long chunk;              // voxels per block
+};
+
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over
 // the elements adjacent to node (i,j,k).  blockdiag=1 means K[3a+c][3b+d] =
 // L8[a][b] delta_cd with L8 = K[3a][3b] (vector Laplacian), cheaper path.
@@ -150,6 +187,12 @@ int launch_adam(long long n, float* p, const float* g, float* m, float* v, float
                 float b2, float eps, float c1, float c2, hipStream_t st);
 int launch_mse(long long n, const float* pred, const float* label, float* grad, double* part,
                hipStream_t st);
+int launch_post_voxel(int op, int rtype, const PostIn& P, hipStream_t st);
+int launch_quality_cell(const QualIn& Q, hipStream_t st);
+// pass 0: per-block (count, sum f, sum g); pass 1: per-block centred sums
+// (f g, f f, g g) around the means that `red` (count, sum f, sum g) defines
+int launch_zncc_pass(int pass, const ZnccIn& Z, const double* red, double* part, int nblocks,
+                     hipStream_t st);
 int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
                       double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

@@ -1,0 +1,345 @@
+// Post-solve voxel stages of a correlation run, on dense voxel arrays (flat
+// index x + Nvx*(y + Nvy*z), the reference's order, no row pitch):
+//   * post_voxel_kernel<R, POST_INTERP>: nodal field at every voxel
+//     (engine/correlate_gpu.py:160-198, interp_field_vox);
+//   * post_voxel_kernel<R, POST_MASK>: mask0 & g.InBounds(x+u) & g(x+u) > glt
+//     with the sample taken in g's own dtype (correlate_gpu.py:284-338);
+//   * post_voxel_kernel<R, POST_PUSH>: field(x - u(x)) in field's dtype
+//     (engine/write_output.py:571-622);
+//   * quality_cell_kernel: per-element residual std and ZNCC over the active
+//     voxels (write_output.py:380-447);
+//   * zncc_pass_kernel: masked ZNCC over all voxels in two deterministic FP64
+//     passes (write_output.py:658-690).
+//
+// Every stage is one HBM-bound sweep.  The voxel kernels are laid out as
+// 64 x-consecutive voxels per wavefront (coalesced dense rows), 4 rows per
+// block; nodal values are re-read from L1/L2 (8 nodes per voxel, shared by
+// every voxel of an element).
+//
+// Arithmetic follows the reference bit for bit where it is defined:
+//   * u(x) = sum over the 8 corners a = 0..7, in that order, of
+//     float(N_a) * U_a in U's dtype, N_a the voxel-centre Hex8 weight formed
+//     in double as geometry_dvc.build_N_stencil forms it; the owning element
+//     is the highest-numbered one containing the voxel (the reference's
+//     later-element-overwrites rule);
+//   * sampling is scipy.ndimage.map_coordinates for spline order 0 / 1 and
+//     modes 'nearest' / 'constant': float64 coordinates, order 0 at index
+//     floor(c + 0.5), order 1 summing ((v * wx) * wy) * wz over the corners
+//     with the z index fastest, corner indices clamped ('nearest'), cval for a
+//     point outside [0, n-1] ('constant'), then the output conversion of the
+//     image dtype (round half away from zero for integers, v != 0 for bool).
+// All products and sums use explicitly rounded operations (no contraction).
+#include "vdvc_common.h"
+#include "vdvc_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ double ld_img(const void* p, int t, long long i) {
+  switch (t) {
+    case GT_F32: return (double)reinterpret_cast<const float*>(p)[i];
+    case GT_F64: return reinterpret_cast<const double*>(p)[i];
+    case GT_U8: return (double)reinterpret_cast<const uint8_t*>(p)[i];
+    case GT_U16: return (double)reinterpret_cast<const uint16_t*>(p)[i];
+    case GT_I16: return (double)reinterpret_cast<const int16_t*>(p)[i];
+    default: return reinterpret_cast<const uint8_t*>(p)[i] != 0 ? 1.0 : 0.0;  // GT_B8
+  }
+}
+
+// integer output conversion of scipy.ndimage: round half away from zero,
+// then truncate; clamped to the type's range
+__device__ __forceinline__ double round_away(double v, double lo, double hi) {
+  const double r = trunc(v >= 0.0 ? add_rn(v, 0.5) : sub_rn(v, 0.5));
+  return fmin(fmax(r, lo), hi);
+}
+
+// value v as the image's dtype holds it, returned as double
+__device__ __forceinline__ double as_img_type(double v, int t) {
+  switch (t) {
+    case GT_F32: return (double)__double2float_rn(v);
+    case GT_F64: return v;
+    case GT_U8: return round_away(v, 0.0, 255.0);
+    case GT_U16: return round_away(v, 0.0, 65535.0);
+    case GT_I16: return round_away(v, -32768.0, 32767.0);
+    default: return v != 0.0 ? 1.0 : 0.0;  // GT_B8
+  }
+}
+
+__device__ __forceinline__ void st_img(void* p, int t, long long i, double v) {
+  const double w = as_img_type(v, t);
+  switch (t) {
+    case GT_F32: reinterpret_cast<float*>(p)[i] = (float)w; break;
+    case GT_F64: reinterpret_cast<double*>(p)[i] = w; break;
+    case GT_U8: reinterpret_cast<uint8_t*>(p)[i] = (uint8_t)w; break;
+    case GT_U16: reinterpret_cast<uint16_t*>(p)[i] = (uint16_t)w; break;
+    case GT_I16: reinterpret_cast<int16_t*>(p)[i] = (int16_t)w; break;
+    default: reinterpret_cast<uint8_t*>(p)[i] = (uint8_t)w; break;
+  }
+}
+
+__device__ __forceinline__ long long clamp_idx(double f, int n) {
+  return (long long)fmin(fmax(f, 0.0), (double)(n - 1));
+}
+
+// map_coordinates(order 0/1, mode 'nearest' (0) / 'constant' (1)) at
+// (cx, cy, cz), float64, before the output conversion
+__device__ double sample01(const void* img, int it, int gnx, int gny, int gnz, double cx,
+                           double cy, double cz, int order, int mode, double cval) {
+  if (mode == 1 && !(cx >= 0.0 && cx <= (double)(gnx - 1) && cy >= 0.0 &&
+                     cy <= (double)(gny - 1) && cz >= 0.0 && cz <= (double)(gnz - 1)))
+    return cval;
+  const long long sy = gnx, sz = (long long)gnx * gny;
+  if (order == 0) {
+    const long long ix = clamp_idx(floor(add_rn(cx, 0.5)), gnx);
+    const long long iy = clamp_idx(floor(add_rn(cy, 0.5)), gny);
+    const long long iz = clamp_idx(floor(add_rn(cz, 0.5)), gnz);
+    return ld_img(img, it, ix + sy * iy + sz * iz);
+  }
+  const double fx = floor(cx), fy = floor(cy), fz = floor(cz);
+  const double tx = sub_rn(cx, fx), ty = sub_rn(cy, fy), tz = sub_rn(cz, fz);
+  const double wx[2] = {sub_rn(1.0, tx), tx};
+  const double wy[2] = {sub_rn(1.0, ty), ty};
+  const double wz[2] = {sub_rn(1.0, tz), tz};
+  const long long ix[2] = {clamp_idx(fx, gnx), clamp_idx(add_rn(fx, 1.0), gnx)};
+  const long long iy[2] = {clamp_idx(fy, gny) * sy, clamp_idx(add_rn(fy, 1.0), gny) * sy};
+  const long long iz[2] = {clamp_idx(fz, gnz) * sz, clamp_idx(add_rn(fz, 1.0), gnz) * sz};
+  double v = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double val = ld_img(img, it, ix[a] + iy[b] + iz[c]);
+        v = add_rn(v, mul_rn(mul_rn(mul_rn(val, wx[a]), wy[b]), wz[c]));
+      }
+  return v;
+}
+
+// Hex8 corner offsets in the reference's local order
+__constant__ int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// nodal field at voxel (x, y, z)
+template <typename R>
+__device__ __forceinline__ void interp_u(const PostIn& P, int x, int y, int z, R (&u)[3]) {
+  const int h = P.h;
+  const int ex = min(x / h, P.Nx - 1), ey = min(y / h, P.Ny - 1), ez = min(z / h, P.Nz - 1);
+  // stencil coordinates 2k/h - 1 and the 1-D factors (1 - xi, 1 + xi)
+  const double xi = sub_rn(__ddiv_rn(2.0 * (x - ex * h), (double)h), 1.0);
+  const double et = sub_rn(__ddiv_rn(2.0 * (y - ey * h), (double)h), 1.0);
+  const double ze = sub_rn(__ddiv_rn(2.0 * (z - ez * h), (double)h), 1.0);
+  const double Fx[2] = {sub_rn(1.0, xi), add_rn(1.0, xi)};
+  const double Fy[2] = {sub_rn(1.0, et), add_rn(1.0, et)};
+  const double Fz[2] = {sub_rn(1.0, ze), add_rn(1.0, ze)};
+  const R* U = reinterpret_cast<const R*>(P.U);
+  u[0] = u[1] = u[2] = R(0);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int cx = kCorner[a][0], cy = kCorner[a][1], cz = kCorner[a][2];
+    const double Nd = mul_rn(mul_rn(mul_rn(0.125, Fx[cx]), Fy[cy]), Fz[cz]);
+    const R N = (R)Nd;
+    const long long node =
+        (long long)(ex + cx) + (long long)P.nnx * ((long long)(ey + cy) + (long long)P.nny * (ez + cz));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = add_rn(u[c], mul_rn(N, U[3 * node + c]));
+  }
+}
+
+__device__ __forceinline__ float clip_r(float c, float hi) { return fminf(fmaxf(c, 0.0f), hi); }
+__device__ __forceinline__ double clip_r(double c, double hi) { return fmin(fmax(c, 0.0), hi); }
+
+template <typename R, int OP>
+__global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int z = blockIdx.z;
+  if (x >= P.Nvx || y >= P.Nvy) return;
+  const long long v = (long long)x + (long long)P.Nvx * ((long long)y + (long long)P.Nvy * z);
+  R u[3];
+  interp_u<R>(P, x, y, z, u);
+  if (OP == POST_INTERP) {
+    R* o = reinterpret_cast<R*>(P.out);
+    if (P.channels_first) {
+      o[v] = u[0];
+      o[P.Nvox + v] = u[1];
+      o[2 * P.Nvox + v] = u[2];
+    } else {
+      o[3 * v] = u[0];
+      o[3 * v + 1] = u[1];
+      o[3 * v + 2] = u[2];
+    }
+  } else if (OP == POST_PUSH) {
+    // coordinates x - u(x) in U's dtype
+    const double cx = (double)sub_rn((R)x, u[0]);
+    const double cy = (double)sub_rn((R)y, u[1]);
+    const double cz = (double)sub_rn((R)z, u[2]);
+    const double val =
+        sample01(P.img, P.itype, P.gnx, P.gny, P.gnz, cx, cy, cz, P.order, P.mode, P.cval);
+    st_img(P.out, P.itype, v, val);
+  } else {
+    // (u + x) - origin in U's dtype; bounds on the closed box; clipped
+    R cx = sub_rn(add_rn(u[0], (R)x), (R)P.ox);
+    R cy = sub_rn(add_rn(u[1], (R)y), (R)P.oy);
+    R cz = sub_rn(add_rn(u[2], (R)z), (R)P.oz);
+    const R hx = (R)(P.gnx - 1), hy = (R)(P.gny - 1), hz = (R)(P.gnz - 1);
+    const bool inb = cx >= R(0) && cx <= hx && cy >= R(0) && cy <= hy && cz >= R(0) && cz <= hz;
+    cx = clip_r(cx, hx);
+    cy = clip_r(cy, hy);
+    cz = clip_r(cz, hz);
+    const double gv = as_img_type(
+        sample01(P.img, P.itype, P.gnx, P.gny, P.gnz, (double)cx, (double)cy, (double)cz, 1, 0,
+                 0.0),
+        P.itype);
+    // g > glt in g's dtype for a float32 image (the threshold is a weak
+    // scalar there), in double otherwise
+    const bool above = P.itype == GT_F32 ? (__double2float_rn(gv) > __double2float_rn(P.glt))
+                                         : (gv > P.glt);
+    bool m = inb && above;
+    if (P.mask0) m = m && (P.mask0[v] != 0);
+    reinterpret_cast<uint8_t*>(P.out)[v] = m ? 1 : 0;
+  }
+}
+
+// One wavefront per element: two sweeps over its (h+1)^3 voxels (the second
+// re-reads them from L2), FP64 sums reduced across the wave in fixed order.
+template <typename RT>
+__global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
+  const int lane = threadIdx.x & 63;
+  const long long e = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + (threadIdx.x >> 6);
+  const long long Ne = (long long)Q.Nx * Q.Ny * Q.Nz;
+  if (e >= Ne) return;
+  const int i = (int)(e % Q.Nx);
+  const int j = (int)((e / Q.Nx) % Q.Ny);
+  const int k = (int)(e / ((long long)Q.Nx * Q.Ny));
+  const int h1 = Q.h + 1;
+  const int nq = h1 * h1 * h1;
+  const long long base =
+      (long long)i * Q.h + (long long)Q.Nvx * ((long long)j * Q.h + (long long)Q.Nvy * k * Q.h);
+  const RT* res = reinterpret_cast<const RT*>(Q.res);
+  double cnt = 0.0, sr = 0.0, srr = 0.0, sf = 0.0, sg = 0.0;
+  for (int q = lane; q < nq; q += 64) {
+    const int qx = q % h1, qy = (q / h1) % h1, qz = q / (h1 * h1);
+    const long long vi = base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz);
+    const double r = (double)res[vi];
+    const double f = ld_img(Q.ref, Q.ftype, vi);
+    const double m = r != 0.0 ? 1.0 : 0.0;
+    const double g = sub_rn(f, r);
+    cnt += m;
+    sr = add_rn(sr, mul_rn(r, m));
+    srr = add_rn(srr, mul_rn(mul_rn(r, r), m));
+    sf = add_rn(sf, mul_rn(f, m));
+    sg = add_rn(sg, mul_rn(g, m));
+  }
+  cnt = wave_sum(cnt);
+  sr = wave_sum(sr);
+  srr = wave_sum(srr);
+  sf = wave_sum(sf);
+  sg = wave_sum(sg);
+  const double dc = fmax(cnt, 1.0);
+  const double mean_r = sr / dc;
+  const double var_r = sub_rn(srr / dc, mul_rn(mean_r, mean_r));
+  const double fm = sf / dc, gm = sg / dc;
+  double sfg = 0.0, sff = 0.0, sgg = 0.0;
+  for (int q = lane; q < nq; q += 64) {
+    const int qx = q % h1, qy = (q / h1) % h1, qz = q / (h1 * h1);
+    const long long vi = base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz);
+    const double r = (double)res[vi];
+    const double f = ld_img(Q.ref, Q.ftype, vi);
+    const double m = r != 0.0 ? 1.0 : 0.0;
+    const double fc = mul_rn(sub_rn(f, fm), m);
+    const double gc = mul_rn(sub_rn(sub_rn(f, r), gm), m);
+    sfg = add_rn(sfg, mul_rn(fc, gc));
+    sff = add_rn(sff, mul_rn(fc, fc));
+    sgg = add_rn(sgg, mul_rn(gc, gc));
+  }
+  sfg = wave_sum(sfg);
+  sff = wave_sum(sff);
+  sgg = wave_sum(sgg);
+  if (lane == 0) {
+    const double nan = __builtin_nan("");
+    const double den = sqrt(mul_rn(sff, sgg));
+    const bool few = cnt < (double)Q.min_active;
+    Q.std_out[e] = few ? nan : sqrt(fmax(var_r, 0.0));
+    Q.zncc_out[e] = (few || !(den > 0.0)) ? nan : sfg / den;
+  }
+}
+
+// Block b sums voxels [b*chunk, min(n, (b+1)*chunk)); fixed-order block
+// reduction; partials [nblocks][3].
+__global__ __launch_bounds__(VDVC_BLOCK) void zncc_pass_kernel(ZnccIn Z, int pass,
+                                                               const double* red, double* part) {
+  __shared__ double lds[4 * 3];
+  const long long b0 = (long long)blockIdx.x * Z.chunk;
+  const long long b1 = min(Z.n, b0 + Z.chunk);
+  double fm = 0.0, gm = 0.0;
+  if (pass == 1) {
+    fm = red[1] / red[0];
+    gm = red[2] / red[0];
+  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = b0 + threadIdx.x; i < b1; i += VDVC_BLOCK) {
+    if (Z.mask && Z.mask[i] == 0) continue;
+    const double f = ld_img(Z.f, Z.ftype, i);
+    double g = ld_img(Z.g, Z.gtype, i);
+    if (Z.g_is_residual) g = sub_rn(f, g);
+    if (pass == 0) {
+      acc[0] += 1.0;
+      acc[1] = add_rn(acc[1], f);
+      acc[2] = add_rn(acc[2], g);
+    } else {
+      const double fc = sub_rn(f, fm), gc = sub_rn(g, gm);
+      acc[0] = add_rn(acc[0], mul_rn(fc, gc));
+      acc[1] = add_rn(acc[1], mul_rn(fc, fc));
+      acc[2] = add_rn(acc[2], mul_rn(gc, gc));
+    }
+  }
+  block_sum_f64<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = acc[0];
+    part[3 * blockIdx.x + 1] = acc[1];
+    part[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
+}  // namespace
+
+namespace vdvc {
+
+int launch_post_voxel(int op, int rtype, const PostIn& P, hipStream_t st) {
+  dim3 g((unsigned)((P.Nvx + 63) / 64), (unsigned)((P.Nvy + 3) / 4), (unsigned)P.Nvz);
+  dim3 b(VDVC_BLOCK);
+#define VDVC_POST(R)                                                                        \
+  do {                                                                                      \
+    if (op == POST_INTERP)                                                                  \
+      hipLaunchKernelGGL((post_voxel_kernel<R, POST_INTERP>), g, b, 0, st, P);              \
+    else if (op == POST_PUSH)                                                               \
+      hipLaunchKernelGGL((post_voxel_kernel<R, POST_PUSH>), g, b, 0, st, P);                \
+    else                                                                                    \
+      hipLaunchKernelGGL((post_voxel_kernel<R, POST_MASK>), g, b, 0, st, P);                \
+  } while (0)
+  if (rtype == GT_F64)
+    VDVC_POST(double);
+  else
+    VDVC_POST(float);
+#undef VDVC_POST
+  return (int)hipGetLastError();
+}
+
+int launch_quality_cell(const QualIn& Q, hipStream_t st) {
+  const long long Ne = (long long)Q.Nx * Q.Ny * Q.Nz;
+  const unsigned nb = (unsigned)((Ne + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
+  if (Q.rtype == GT_F64)
+    hipLaunchKernelGGL(quality_cell_kernel<double>, dim3(nb), dim3(VDVC_BLOCK), 0, st, Q);
+  else
+    hipLaunchKernelGGL(quality_cell_kernel<float>, dim3(nb), dim3(VDVC_BLOCK), 0, st, Q);
+  return (int)hipGetLastError();
+}
+
+int launch_zncc_pass(int pass, const ZnccIn& Z, const double* red, double* part, int nblocks,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(zncc_pass_kernel, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, Z, pass, red,
+                     part);
+  return (int)hipGetLastError();
+}
+
+}  // namespace vdvc

@@ -114,6 +114,63 @@ def sample_deformed_field(U, g_pix_x, N_stencil, h, Nx, Ny, Nz, xp=np, g_origin=
     return (gv, mk.bool()) if dev else (L.to_host(gv), L.to_host(mk).astype(bool))
 
 
+def _check_voxel_stencil(N_stencil, h):
+    """The post-solve kernels evaluate the voxel-centre Hex8 stencil of h
+    themselves; any other N_stencil is rejected."""
+    if N_stencil is None:
+        return
+    a = np.asarray(N_stencil.cpu() if L.is_tensor(N_stencil) else N_stencil, dtype=np.float64)
+    ref = build_N_stencil(h)[0]
+    if a.shape != ref.shape or not np.allclose(a, ref, rtol=0, atol=1e-6):
+        raise ValueError('N_stencil must be the voxel-centre stencil build_N_stencil(%d)' % h)
+
+
+def _nodal(U, Nx, Ny, Nz):
+    wd, code = K.work_dtype(U)
+    dU = L.to_dev_flat(U, wd)
+    if dU.numel() != 3 * (Nx + 1) * (Ny + 1) * (Nz + 1):
+        raise ValueError('U has %d entries, expected 3*(Nx+1)*(Ny+1)*(Nz+1) = %d'
+                         % (dU.numel(), 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)))
+    return dU, wd, code
+
+
+def interp_field_vox(U, N_stencil, h, Nx, Ny, Nz, xp, chunk_size=200_000,
+                     channels_first=False):
+    """Nodal field evaluated at every voxel centre with the Hex8 voxel-centre
+    stencil (correlate_gpu.py:160-198): (Nvox, 3), or (3, Nvox) with
+    channels_first, in U's working type (float64 for float64 U, else
+    float32).  chunk_size is accepted for signature parity (the kernel
+    streams the voxels without chunking)."""
+    _check_voxel_stencil(N_stencil, h)
+    dU, wd, code = _nodal(U, Nx, Ny, Nz)
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    out = L.empty(3 * Nvox, wd)
+    L.call('vdvc_interp_field', L.ctx(), Nx, Ny, Nz, h, dU, code, out, int(bool(channels_first)))
+    out = out.view(3, Nvox) if channels_first else out.view(Nvox, 3)
+    return _as_out(xp, out)
+
+
+def compute_active_mask(U, g_pix, Nx, Ny, Nz, h, xp, g_origin=(0, 0, 0), mask0_vox=None,
+                        glt=0.0):
+    """Per-voxel mask0 & g.InBounds(x+u) & g(x+u) > glt for a given U
+    (correlate_gpu.py:284-338): g is sampled in its own dtype (an integer
+    image rounds half away from zero before the threshold, as
+    map_coordinates returns the input dtype).  Returns (Nvox,) bool."""
+    dU, wd, code = _nodal(U, Nx, Ny, Nz)
+    g, gshp, gcode = L.to_dev_image(g_pix)
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    m0 = None
+    if mask0_vox is not None:
+        m0 = L.to_dev_flat(mask0_vox, 'uint8')
+        if m0.numel() != Nvox:
+            raise ValueError('mask0_vox has %d entries, expected Nvox = %d' % (m0.numel(), Nvox))
+    out = L.empty(Nvox, 'uint8')
+    L.call('vdvc_active_mask', L.ctx(), Nx, Ny, Nz, h, dU, code, g, gcode, gshp[0], gshp[1],
+           gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]), m0, float(glt),
+           out)
+    return _as_out(xp, out.bool())
+
+
 def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny, Nz, Ndof,
                   xp=np, chunk_size=None):
     """b = sum_q w_q m_q res_q v_eq (correlate_gpu.py:341-378)."""
