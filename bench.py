@@ -43,6 +43,8 @@ def parse():
     p.add_argument('--no-solve', action='store_true',
                    help='skip the GN time-to-solve measurement')
     p.add_argument('--solve-reps', type=int, default=3)
+    p.add_argument('--no-post', action='store_true',
+                   help='skip the post-solve voxel stages')
     return p.parse_args()
 
 
@@ -131,6 +133,119 @@ def time_to_solve(n=513, h=8, reps=3, log=None):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hessian_matvec_traffic.json')
+
+
+def post_stages(n=513, h=8, reps=10):
+    """The voxel-scale post-solve stages of a correlation run (the GPU part of
+    the reference's write_outputs) at n^3 voxels, h: kernel time per launch
+    (HIP events on the library stream, mean of `reps` after one warm-up)
+    against the stage's algorithmic HBM bytes.  Inputs are device resident:
+    an 8-bit speckle reference f (uint8, as preprocess keeps the CT dtype),
+    a smooth Fourier displacement U (float32, up to ~4 vox), the per-voxel
+    float32 residual (zero outside the active mask) and mask0 = f > 0."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd import synth
+    n_el = (n - 1) // h
+    Nv = n_el * h + 1
+    Nvox = Nv ** 3
+    Nn = (n_el + 1) ** 3
+    Ne = n_el ** 3
+    c = L.ctx()
+    f = synth.speckle_torch((Nv, Nv, Nv), synth.blobs_for((Nv, Nv, Nv)), seed=5)
+    f8 = f.round().clamp_(0, 255).to(torch.uint8).view(-1)
+    del f
+    modes = synth.fourier_params(16, (64, 256), 4.0, 11, (Nv, Nv, Nv))
+    U = torch.from_numpy(synth.fourier_u_nodes(modes, n_el, n_el, n_el, h).astype(np.float32)).cuda()
+    mask0 = (f8 > 0).to(torch.uint8)
+    gen = torch.Generator(device='cuda').manual_seed(0)
+    res = torch.randn(Nvox, device='cuda', generator=gen) * 6
+    res = res * mask0
+    out = {}
+
+    def timed(name, call, nbytes):
+        call()
+        torch.cuda.synchronize()
+        L.call('vdvc_timing_begin', c)
+        for _ in range(reps):
+            call()
+        tot = (L._d * 1)()
+        nl = (L._i * 1)()
+        L.call('vdvc_timing_end', c, tot, nl)
+        ms = tot[0] / max(1, nl[0])
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"kernel_ms": round(ms, 4), "algorithmic_bytes": int(nbytes),
+                     "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    ui = L.empty(3 * Nvox)
+    timed("interp_field_vox (float32, channels_first)",
+          lambda: L.call('vdvc_interp_field', c, n_el, n_el, n_el, h, U, L.F32, ui, 1),
+          12 * Nvox + 12 * Nn)
+    del ui
+    act = L.empty(Nvox, 'uint8')
+    timed("compute_active_mask (uint8 g, mask0)",
+          lambda: L.call('vdvc_active_mask', c, n_el, n_el, n_el, h, U, L.F32, f8, L.U8, Nv, Nv,
+                         Nv, 0.0, 0.0, 0.0, mask0, 0.0, act),
+          3 * Nvox + 12 * Nn)
+    pushed = L.empty(Nvox, 'uint8')
+    timed("push_reference_forward (uint8, order 1, nearest)",
+          lambda: L.call('vdvc_push_forward', c, n_el, n_el, n_el, h, U, L.F32, f8, L.U8, 1, 0,
+                         0.0, pushed),
+          2 * Nvox + 12 * Nn)
+    timed("push_field_forward (bool, order 0, constant cval=1)",
+          lambda: L.call('vdvc_push_forward', c, n_el, n_el, n_el, h, U, L.F32, mask0, L.BOOL, 0,
+                         1, 1.0, pushed),
+          2 * Nvox + 12 * Nn)
+    std = torch.empty(Ne, dtype=torch.float64, device='cuda')
+    zn = torch.empty(Ne, dtype=torch.float64, device='cuda')
+    timed("compute_correlation_quality_cell (float32 res, uint8 ref)",
+          lambda: L.call('vdvc_quality_cell', c, n_el, n_el, n_el, h, res, L.F32, f8, L.U8, std,
+                         zn),
+          5 * Nvox + 16 * Ne)
+    z = (L._d * 1)()
+    L.call('vdvc_zncc', c, Nvox, f8, L.U8, res, L.F32, 1, act, z)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.call('vdvc_zncc', c, Nvox, f8, L.U8, res, L.F32, 1, act, z)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    gbs = 6 * Nvox / (ms * 1e-3) / 1e9
+    out["compute_zncc (uint8 f, float32 res, mask; two passes, wall incl. 2 syncs)"] = {
+        "ms": round(ms, 4), "algorithmic_bytes": 6 * Nvox, "achieved_GBs": round(gbs, 1),
+        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    total = sum(v.get("kernel_ms", v.get("ms", 0.0)) for v in out.values())
+    return {"workload": "post-solve voxel stages, %d^3 voxels, h=%d (%d^3 elements), uint8 "
+                        "speckle, Fourier U (16 modes, 4 vox), float32 residual" % (Nv, h, n_el),
+            "stages": out, "total_ms": round(total, 3)}
+
+
+def post_cpu_baseline(n=129, h=8):
+    """The oracle's NumPy post-solve stages (interp_field_vox, active mask,
+    push, quality cell, ZNCC) on one host thread at n^3: voxels per second
+    over the five stages together."""
+    from threadpoolctl import threadpool_limits
+    from oracle import post_oracle as P
+    from voxeldvc_amd import synth
+    n_el = (n - 1) // h
+    Nv = n_el * h + 1
+    f = synth.speckle_numpy((Nv, Nv, Nv), synth.blobs_for((Nv, Nv, Nv)), 5)
+    f = np.clip(np.round(f), 0, 255).astype(np.uint8)
+    modes = synth.fourier_params(16, (64, 256), 4.0, 11, (Nv, Nv, Nv))
+    U = synth.fourier_u_nodes(modes, n_el, n_el, n_el, h).astype(np.float32)
+    ff = f.reshape(-1, order='F')
+    m0 = ff > 0
+    res = (np.random.default_rng(0).standard_normal(Nv ** 3) * 6).astype(np.float32) * m0
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        P.interp_field_vox(U, h, n_el, n_el, n_el, channels_first=True)
+        act = P.compute_active_mask(U, f, h, n_el, n_el, n_el, mask0_vox=m0)
+        P.push_field_forward(f, U, n_el, n_el, n_el, h)
+        P.compute_correlation_quality_cell(res, f, h, n_el, n_el, n_el)
+        P.compute_zncc(ff, ff.astype(np.float64) - res, act)
+        t = time.perf_counter() - t0
+    return {"value": round(Nv ** 3 / t / 1e6, 3), "unit": "Mvox/s (five stages)", "cores": 1,
+            "kind": "port", "sample": "the five stages once at %d^3, h=%d: %.2f s; %d host cores "
+                                      "available" % (Nv, h, t, os.cpu_count())}
 
 
 def measured_traffic(Nv, h):
@@ -269,6 +384,11 @@ def main():
     if not a.no_solve:
         print('time to solve', file=sys.stderr, flush=True)
         res["time_to_solve"] = time_to_solve(n=a.size, h=a.h, reps=a.solve_reps)
+    if not a.no_post:
+        print('post-solve stages', file=sys.stderr, flush=True)
+        res["post_stages"] = post_stages(n=a.size, h=a.h)
+        if not a.no_cpu:
+            res["post_stages"]["cpu_baseline"] = post_cpu_baseline()
     if not a.no_cpu:
         print('cpu baseline', file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -262,8 +262,8 @@ int itype_ok(int t) {
 
 // dense voxel grid of a mesh for the post-solve kernels (grid y / z limits)
 int post_grid_ok(const MeshDev& M) {
-  if ((M.Nvy + 3) / 4 > 65535 || M.Nvz > 65535)
-    return fail(-1, "voxel grid %d x %d x %d exceeds the launch grid (Nvy <= 262140, Nvz <= 65535)",
+  if ((M.Nvy + 3) / 4 > 65535 || M.Nz > 65535)
+    return fail(-1, "voxel grid %d x %d x %d exceeds the launch grid (Nvy <= 262140, Nz <= 65535)",
                 M.Nvx, M.Nvy, M.Nvz);
   return 0;
 }
@@ -1163,7 +1163,9 @@ int vdvc_zncc(vdvc_ctx* ctx, int64_t n, const void* d_f, int ftype, const void* 
   ZnccIn Z;
   Z.n = n; Z.f = d_f; Z.ftype = ftype; Z.g = d_g; Z.gtype = gtype;
   Z.g_is_residual = g_is_residual ? 1 : 0; Z.mask = d_mask;
-  Z.chunk = (n + nb - 1) / nb;
+  auto al16 = [](const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
+  Z.vec = al16(d_f) && al16(d_g) && al16(d_mask) ? 1 : 0;
+  Z.chunk = ((n + nb - 1) / nb + 1023) / 1024 * 1024;
   double* part = ctx->stats.as<double>();
   double* red = ctx->red.as<double>();
   VDVC_LAUNCH(vdvc::launch_zncc_pass(0, Z, nullptr, part, nb, ctx->stream));

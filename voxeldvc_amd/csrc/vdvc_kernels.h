@@ -64,7 +64,8 @@ struct ZnccIn {
   const void* g; int gtype;
   int g_is_residual;            // g := f - g
   const uint8_t* mask;          // optional
-  long long chunk;              // voxels per block
+  long long chunk;              // voxels per block (a multiple of 1024 when vec)
+  int vec;                      // 4 voxels per thread step (16-byte aligned arrays)
 };
 
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over

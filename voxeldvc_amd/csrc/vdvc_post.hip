@@ -11,10 +11,11 @@
 //   * zncc_pass_kernel: masked ZNCC over all voxels in two deterministic FP64
 //     passes (write_output.py:658-690).
 //
-// Every stage is one HBM-bound sweep.  The voxel kernels are laid out as
-// 64 x-consecutive voxels per wavefront (coalesced dense rows), 4 rows per
-// block; nodal values are re-read from L1/L2 (8 nodes per voxel, shared by
-// every voxel of an element).
+// Every stage is one sweep over the voxels.  In the voxel kernels a thread
+// owns one voxel column (x, y) of one element layer: the element's 8 nodal
+// values and the x/y part of the Hex8 weights are formed once, then the
+// layer's z planes are walked; 64 x-consecutive columns per wavefront keep
+// every plane access a coalesced dense row.
 //
 // Arithmetic follows the reference bit for bit where it is defined:
 //   * u(x) = sum over the 8 corners a = 0..7, in that order, of
@@ -76,14 +77,18 @@ __device__ __forceinline__ void st_img(void* p, int t, long long i, double v) {
   }
 }
 
-__device__ __forceinline__ long long clamp_idx(double f, int n) {
-  return (long long)fmin(fmax(f, 0.0), (double)(n - 1));
+// index clamped to [0, n-1] (image dims are below 2^31)
+__device__ __forceinline__ int clamp_idx(double f, int n) {
+  return (int)fmin(fmax(f, 0.0), (double)(n - 1));
 }
 
 // map_coordinates(order 0/1, mode 'nearest' (0) / 'constant' (1)) at
-// (cx, cy, cz), float64, before the output conversion
-__device__ double sample01(const void* img, int it, int gnx, int gny, int gnz, double cx,
-                           double cy, double cz, int order, int mode, double cval) {
+// (cx, cy, cz), float64, before the output conversion; image type IT
+template <int IT>
+__device__ __forceinline__ double sample01(const void* img, int gnx, int gny, int gnz, double cx,
+                                           double cy, double cz, int order, int mode,
+                                           double cval) {
+  constexpr int it = IT;
   if (mode == 1 && !(cx >= 0.0 && cx <= (double)(gnx - 1) && cy >= 0.0 &&
                      cy <= (double)(gny - 1) && cz >= 0.0 && cz <= (double)(gnz - 1)))
     return cval;
@@ -115,68 +120,114 @@ __device__ double sample01(const void* img, int it, int gnx, int gny, int gnz, d
   return v;
 }
 
-// Hex8 corner offsets in the reference's local order
-__constant__ int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-
-// nodal field at voxel (x, y, z)
-template <typename R>
-__device__ __forceinline__ void interp_u(const PostIn& P, int x, int y, int z, R (&u)[3]) {
-  const int h = P.h;
-  const int ex = min(x / h, P.Nx - 1), ey = min(y / h, P.Ny - 1), ez = min(z / h, P.Nz - 1);
-  // stencil coordinates 2k/h - 1 and the 1-D factors (1 - xi, 1 + xi)
-  const double xi = sub_rn(__ddiv_rn(2.0 * (x - ex * h), (double)h), 1.0);
-  const double et = sub_rn(__ddiv_rn(2.0 * (y - ey * h), (double)h), 1.0);
-  const double ze = sub_rn(__ddiv_rn(2.0 * (z - ez * h), (double)h), 1.0);
-  const double Fx[2] = {sub_rn(1.0, xi), add_rn(1.0, xi)};
-  const double Fy[2] = {sub_rn(1.0, et), add_rn(1.0, et)};
-  const double Fz[2] = {sub_rn(1.0, ze), add_rn(1.0, ze)};
-  const R* U = reinterpret_cast<const R*>(P.U);
-  u[0] = u[1] = u[2] = R(0);
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int cx = kCorner[a][0], cy = kCorner[a][1], cz = kCorner[a][2];
-    const double Nd = mul_rn(mul_rn(mul_rn(0.125, Fx[cx]), Fy[cy]), Fz[cz]);
-    const R N = (R)Nd;
-    const long long node =
-        (long long)(ex + cx) + (long long)P.nnx * ((long long)(ey + cy) + (long long)P.nny * (ez + cz));
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u[c] = add_rn(u[c], mul_rn(N, U[3 * node + c]));
-  }
+// Hex8 corner offsets in the reference's local order 0=(0,0,0) 1=(1,0,0)
+// 2=(1,1,0) 3=(0,1,0) 4..7 = the same with z = 1 (engine/geometry_dvc.py:14-17);
+// compile-time functions so unrolled loops index registers statically
+__device__ __forceinline__ constexpr int corner_x(int a) {
+  return ((a & 3) == 1 || (a & 3) == 2) ? 1 : 0;
 }
+__device__ __forceinline__ constexpr int corner_y(int a) { return (a >> 1) & 1; }
+__device__ __forceinline__ constexpr int corner_z(int a) { return a >> 2; }
+
+// stencil coordinate 2k/h - 1 of local voxel k, formed as
+// geometry_dvc.build_N_stencil forms it (float64)
+__device__ __forceinline__ double stencil_xi(int k, int h) {
+  return sub_rn(__ddiv_rn(2.0 * k, (double)h), 1.0);
+}
+
+// Interpolation state of one voxel column (x, y) inside element layer ez:
+// the x/y part of the 8 Hex8 weights, 0.125 (1 +- xi) (1 +- eta), and the
+// element's 8 nodal values.  Per voxel only the z factor remains:
+// N_a = ((0.125 Fx) Fy) Fz, the reference's product order.
+template <typename R>
+struct Column {
+  double pxy[2][2];
+  R uc[8][3];
+  __device__ __forceinline__ void init(const PostIn& P, int x, int y, int ez) {
+    const int h = P.h;
+    const int ex = min(x / h, P.Nx - 1), ey = min(y / h, P.Ny - 1);
+    const double xi = stencil_xi(x - ex * h, h), et = stencil_xi(y - ey * h, h);
+    const double Fx[2] = {sub_rn(1.0, xi), add_rn(1.0, xi)};
+    const double Fy[2] = {sub_rn(1.0, et), add_rn(1.0, et)};
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy) pxy[cx][cy] = mul_rn(mul_rn(0.125, Fx[cx]), Fy[cy]);
+    const R* U = reinterpret_cast<const R*>(P.U);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const long long node = (long long)(ex + corner_x(a)) +
+                             (long long)P.nnx * ((long long)(ey + corner_y(a)) +
+                                                 (long long)P.nny * (ez + corner_z(a)));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) uc[a][c] = U[3 * node + c];
+    }
+  }
+  // u at local z voxel tz of the layer: sum over corners a = 0..7 in order
+  __device__ __forceinline__ void at(int tz, int h, R (&u)[3]) const {
+    const double ze = stencil_xi(tz, h);
+    const double Fz[2] = {sub_rn(1.0, ze), add_rn(1.0, ze)};
+    u[0] = u[1] = u[2] = R(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const R N = (R)mul_rn(pxy[corner_x(a)][corner_y(a)], Fz[corner_z(a)]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c] = add_rn(u[c], mul_rn(N, uc[a][c]));
+    }
+  }
+};
 
 __device__ __forceinline__ float clip_r(float c, float hi) { return fminf(fmaxf(c, 0.0f), hi); }
 __device__ __forceinline__ double clip_r(double c, double hi) { return fmin(fmax(c, 0.0), hi); }
 
-template <typename R, int OP>
+// One thread per voxel column (x, y) of one element layer (blockIdx.z): the
+// layer's h voxel planes (h + 1 for the last layer, which owns the last
+// plane), 64 x-consecutive columns per wavefront.  The sampling stages work
+// on two planes per step so both planes' gathers are in flight together.
+// IT is the sampled image's element type (POST_PUSH, POST_MASK).
+template <typename R, int OP, int IT>
 __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int z = blockIdx.z;
+  const int ez = blockIdx.z;
   if (x >= P.Nvx || y >= P.Nvy) return;
-  const long long v = (long long)x + (long long)P.Nvx * ((long long)y + (long long)P.Nvy * z);
-  R u[3];
-  interp_u<R>(P, x, y, z, u);
+  const int h = P.h;
+  Column<R> col;
+  col.init(P, x, y, ez);
+  const int nz = h + (ez == P.Nz - 1 ? 1 : 0);
+  const long long plane = (long long)P.Nvx * P.Nvy;
+  const long long v0 = (long long)x + (long long)P.Nvx * y + plane * ((long long)ez * h);
   if (OP == POST_INTERP) {
     R* o = reinterpret_cast<R*>(P.out);
-    if (P.channels_first) {
-      o[v] = u[0];
-      o[P.Nvox + v] = u[1];
-      o[2 * P.Nvox + v] = u[2];
-    } else {
-      o[3 * v] = u[0];
-      o[3 * v + 1] = u[1];
-      o[3 * v + 2] = u[2];
+    long long v = v0;
+    for (int tz = 0; tz < nz; ++tz, v += plane) {
+      R u[3];
+      col.at(tz, h, u);
+      if (P.channels_first) {
+        o[v] = u[0];
+        o[P.Nvox + v] = u[1];
+        o[2 * P.Nvox + v] = u[2];
+      } else {
+        o[3 * v] = u[0];
+        o[3 * v + 1] = u[1];
+        o[3 * v + 2] = u[2];
+      }
     }
-  } else if (OP == POST_PUSH) {
-    // coordinates x - u(x) in U's dtype
-    const double cx = (double)sub_rn((R)x, u[0]);
-    const double cy = (double)sub_rn((R)y, u[1]);
-    const double cz = (double)sub_rn((R)z, u[2]);
-    const double val =
-        sample01(P.img, P.itype, P.gnx, P.gny, P.gnz, cx, cy, cz, P.order, P.mode, P.cval);
-    st_img(P.out, P.itype, v, val);
-  } else {
+    return;
+  }
+  // the stage's value at local plane tz: the pushed field value (PUSH) or
+  // the active-mask bit (MASK)
+  auto value = [&](int tz) -> double {
+    const int z = ez * h + tz;
+    R u[3];
+    col.at(tz, h, u);
+    if (OP == POST_PUSH) {
+      // coordinates x - u(x) in U's dtype
+      const double cx = (double)sub_rn((R)x, u[0]);
+      const double cy = (double)sub_rn((R)y, u[1]);
+      const double cz = (double)sub_rn((R)z, u[2]);
+      return sample01<IT>(P.img, P.gnx, P.gny, P.gnz, cx, cy, cz, P.order, P.mode, P.cval);
+    }
     // (u + x) - origin in U's dtype; bounds on the closed box; clipped
     R cx = sub_rn(add_rn(u[0], (R)x), (R)P.ox);
     R cy = sub_rn(add_rn(u[1], (R)y), (R)P.oy);
@@ -186,17 +237,54 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
     cx = clip_r(cx, hx);
     cy = clip_r(cy, hy);
     cz = clip_r(cz, hz);
-    const double gv = as_img_type(
-        sample01(P.img, P.itype, P.gnx, P.gny, P.gnz, (double)cx, (double)cy, (double)cz, 1, 0,
-                 0.0),
-        P.itype);
+    const double gv = as_img_type(sample01<IT>(P.img, P.gnx, P.gny, P.gnz, (double)cx,
+                                               (double)cy, (double)cz, 1, 0, 0.0),
+                                  IT);
     // g > glt in g's dtype for a float32 image (the threshold is a weak
     // scalar there), in double otherwise
-    const bool above = P.itype == GT_F32 ? (__double2float_rn(gv) > __double2float_rn(P.glt))
-                                         : (gv > P.glt);
-    bool m = inb && above;
-    if (P.mask0) m = m && (P.mask0[v] != 0);
-    reinterpret_cast<uint8_t*>(P.out)[v] = m ? 1 : 0;
+    const bool above = IT == GT_F32 ? (__double2float_rn(gv) > __double2float_rn(P.glt))
+                                    : (gv > P.glt);
+    return (inb && above) ? 1.0 : 0.0;
+  };
+  auto store = [&](long long v, double val) {
+    if (OP == POST_PUSH) {
+      st_img(P.out, IT, v, val);
+    } else {
+      const bool m = val != 0.0 && (P.mask0 == nullptr || P.mask0[v] != 0);
+      reinterpret_cast<uint8_t*>(P.out)[v] = m ? 1 : 0;
+    }
+  };
+  long long v = v0;
+  for (int tz = 0; tz < nz; tz += 2, v += 2 * plane) {
+    const bool two = tz + 1 < nz;
+    const double va = value(tz);
+    const double vb = value(two ? tz + 1 : tz);
+    store(v, va);
+    if (two) store(v + plane, vb);
+  }
+}
+
+// Visit the (h+1)^3 voxels of an element with one wavefront: lane -> (row,
+// qx) once, rows (qy, qz) of h+1 voxels stepping by 64 / (h+1) rows; for
+// h + 1 > 64 each lane walks voxels directly.
+template <typename F>
+__device__ __forceinline__ void for_elem_voxels(const QualIn& Q, long long base, int lane, F f) {
+  const int h1 = Q.h + 1;
+  const int nrows = h1 * h1;
+  if (h1 <= 64) {
+    const int rpi = 64 / h1;
+    const int lr = lane / h1, qx = lane - lr * h1;
+    if (lr >= rpi) return;
+    for (int r = lr; r < nrows; r += rpi) {
+      const int qz = r / h1, qy = r - qz * h1;
+      f(base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz));
+    }
+  } else {
+    const int nq = nrows * h1;
+    for (int q = lane; q < nq; q += 64) {
+      const int qx = q % h1, qy = (q / h1) % h1, qz = q / nrows;
+      f(base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz));
+    }
   }
 }
 
@@ -211,15 +299,11 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
   const int i = (int)(e % Q.Nx);
   const int j = (int)((e / Q.Nx) % Q.Ny);
   const int k = (int)(e / ((long long)Q.Nx * Q.Ny));
-  const int h1 = Q.h + 1;
-  const int nq = h1 * h1 * h1;
   const long long base =
       (long long)i * Q.h + (long long)Q.Nvx * ((long long)j * Q.h + (long long)Q.Nvy * k * Q.h);
   const RT* res = reinterpret_cast<const RT*>(Q.res);
   double cnt = 0.0, sr = 0.0, srr = 0.0, sf = 0.0, sg = 0.0;
-  for (int q = lane; q < nq; q += 64) {
-    const int qx = q % h1, qy = (q / h1) % h1, qz = q / (h1 * h1);
-    const long long vi = base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz);
+  for_elem_voxels(Q, base, lane, [&](long long vi) {
     const double r = (double)res[vi];
     const double f = ld_img(Q.ref, Q.ftype, vi);
     const double m = r != 0.0 ? 1.0 : 0.0;
@@ -229,7 +313,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
     srr = add_rn(srr, mul_rn(mul_rn(r, r), m));
     sf = add_rn(sf, mul_rn(f, m));
     sg = add_rn(sg, mul_rn(g, m));
-  }
+  });
   cnt = wave_sum(cnt);
   sr = wave_sum(sr);
   srr = wave_sum(srr);
@@ -240,9 +324,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
   const double var_r = sub_rn(srr / dc, mul_rn(mean_r, mean_r));
   const double fm = sf / dc, gm = sg / dc;
   double sfg = 0.0, sff = 0.0, sgg = 0.0;
-  for (int q = lane; q < nq; q += 64) {
-    const int qx = q % h1, qy = (q / h1) % h1, qz = q / (h1 * h1);
-    const long long vi = base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz);
+  for_elem_voxels(Q, base, lane, [&](long long vi) {
     const double r = (double)res[vi];
     const double f = ld_img(Q.ref, Q.ftype, vi);
     const double m = r != 0.0 ? 1.0 : 0.0;
@@ -251,7 +333,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
     sfg = add_rn(sfg, mul_rn(fc, gc));
     sff = add_rn(sff, mul_rn(fc, fc));
     sgg = add_rn(sgg, mul_rn(gc, gc));
-  }
+  });
   sfg = wave_sum(sfg);
   sff = wave_sum(sff);
   sgg = wave_sum(sgg);
@@ -264,34 +346,91 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
   }
 }
 
+// four consecutive elements i..i+3 (i a multiple of 4, array 16-byte
+// aligned) with one vector load
+__device__ __forceinline__ void ld4_img(const void* p, int t, long long i, double (&v)[4]) {
+  switch (t) {
+    case GT_F32: {
+      const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(p) + i);
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+      break;
+    }
+    case GT_F64: {
+      const double2* d = reinterpret_cast<const double2*>(reinterpret_cast<const double*>(p) + i);
+      const double2 a = d[0], b = d[1];
+      v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+      break;
+    }
+    case GT_U16:
+    case GT_I16: {
+      const uint2 q = *reinterpret_cast<const uint2*>(reinterpret_cast<const uint16_t*>(p) + i);
+      const unsigned w[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v[k] = t == GT_I16 ? (double)(int16_t)(uint16_t)w[k] : (double)w[k];
+      break;
+    }
+    default: {  // GT_U8, GT_B8
+      const unsigned q = *reinterpret_cast<const unsigned*>(reinterpret_cast<const uint8_t*>(p) + i);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned b = (q >> (8 * k)) & 0xffu;
+        v[k] = t == GT_B8 ? (b != 0u ? 1.0 : 0.0) : (double)b;
+      }
+    }
+  }
+}
+
+// pass 0 accumulates (count, sum f, sum g), pass 1 the centred products, of
+// one selected element
+__device__ __forceinline__ void zncc_acc(int pass, double m, double f, double g, double fm,
+                                         double gm, double (&acc)[3]) {
+  if (pass == 0) {
+    acc[0] += m;
+    acc[1] = add_rn(acc[1], mul_rn(f, m));
+    acc[2] = add_rn(acc[2], mul_rn(g, m));
+  } else {
+    const double fc = mul_rn(sub_rn(f, fm), m), gc = mul_rn(sub_rn(g, gm), m);
+    acc[0] = add_rn(acc[0], mul_rn(fc, gc));
+    acc[1] = add_rn(acc[1], mul_rn(fc, fc));
+    acc[2] = add_rn(acc[2], mul_rn(gc, gc));
+  }
+}
+
 // Block b sums voxels [b*chunk, min(n, (b+1)*chunk)); fixed-order block
-// reduction; partials [nblocks][3].
+// reduction; partials [nblocks][3].  Unselected voxels contribute m = 0
+// (loads are unconditional, so each thread keeps several in flight).
 __global__ __launch_bounds__(VDVC_BLOCK) void zncc_pass_kernel(ZnccIn Z, int pass,
                                                                const double* red, double* part) {
   __shared__ double lds[4 * 3];
   const long long b0 = (long long)blockIdx.x * Z.chunk;
-  const long long b1 = min(Z.n, b0 + Z.chunk);
+  const long long b1 = max(b0, min(Z.n, b0 + Z.chunk));
   double fm = 0.0, gm = 0.0;
   if (pass == 1) {
     fm = red[1] / red[0];
     gm = red[2] / red[0];
   }
   double acc[3] = {0.0, 0.0, 0.0};
-  for (long long i = b0 + threadIdx.x; i < b1; i += VDVC_BLOCK) {
-    if (Z.mask && Z.mask[i] == 0) continue;
+  long long i0 = b0;
+  if (Z.vec) {
+    const long long e4 = b0 + (b1 - b0) / 4 * 4;
+    for (long long i = b0 + 4 * (long long)threadIdx.x; i < e4; i += 4 * VDVC_BLOCK) {
+      double f[4], g[4], m[4] = {1.0, 1.0, 1.0, 1.0};
+      ld4_img(Z.f, Z.ftype, i, f);
+      ld4_img(Z.g, Z.gtype, i, g);
+      if (Z.mask) ld4_img(Z.mask, GT_B8, i, m);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        zncc_acc(pass, m[k], f[k], Z.g_is_residual ? sub_rn(f[k], g[k]) : g[k], fm, gm, acc);
+    }
+    i0 = e4;
+  }
+  for (long long i = i0 + threadIdx.x; i < b1; i += VDVC_BLOCK) {
+    const double m = Z.mask ? (Z.mask[i] != 0 ? 1.0 : 0.0) : 1.0;
     const double f = ld_img(Z.f, Z.ftype, i);
     double g = ld_img(Z.g, Z.gtype, i);
     if (Z.g_is_residual) g = sub_rn(f, g);
-    if (pass == 0) {
-      acc[0] += 1.0;
-      acc[1] = add_rn(acc[1], f);
-      acc[2] = add_rn(acc[2], g);
-    } else {
-      const double fc = sub_rn(f, fm), gc = sub_rn(g, gm);
-      acc[0] = add_rn(acc[0], mul_rn(fc, gc));
-      acc[1] = add_rn(acc[1], mul_rn(fc, fc));
-      acc[2] = add_rn(acc[2], mul_rn(gc, gc));
-    }
+    zncc_acc(pass, m, f, g, fm, gm, acc);
   }
   block_sum_f64<3>(acc, lds);
   if (threadIdx.x == 0) {
@@ -305,23 +444,35 @@ __global__ __launch_bounds__(VDVC_BLOCK) void zncc_pass_kernel(ZnccIn Z, int pas
 
 namespace vdvc {
 
-int launch_post_voxel(int op, int rtype, const PostIn& P, hipStream_t st) {
-  dim3 g((unsigned)((P.Nvx + 63) / 64), (unsigned)((P.Nvy + 3) / 4), (unsigned)P.Nvz);
-  dim3 b(VDVC_BLOCK);
-#define VDVC_POST(R)                                                                        \
-  do {                                                                                      \
-    if (op == POST_INTERP)                                                                  \
-      hipLaunchKernelGGL((post_voxel_kernel<R, POST_INTERP>), g, b, 0, st, P);              \
-    else if (op == POST_PUSH)                                                               \
-      hipLaunchKernelGGL((post_voxel_kernel<R, POST_PUSH>), g, b, 0, st, P);                \
-    else                                                                                    \
-      hipLaunchKernelGGL((post_voxel_kernel<R, POST_MASK>), g, b, 0, st, P);                \
-  } while (0)
-  if (rtype == GT_F64)
-    VDVC_POST(double);
+template <typename R, int OP>
+void launch_post_typed(const PostIn& P, dim3 g, dim3 b, hipStream_t st) {
+  switch (P.itype) {
+    case GT_F32: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F32>), g, b, 0, st, P); break;
+    case GT_F64: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F64>), g, b, 0, st, P); break;
+    case GT_U8: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U8>), g, b, 0, st, P); break;
+    case GT_U16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U16>), g, b, 0, st, P); break;
+    case GT_I16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_I16>), g, b, 0, st, P); break;
+    default: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_B8>), g, b, 0, st, P); break;
+  }
+}
+
+template <typename R>
+void launch_post_r(int op, const PostIn& P, dim3 g, dim3 b, hipStream_t st) {
+  if (op == POST_INTERP)
+    hipLaunchKernelGGL((post_voxel_kernel<R, POST_INTERP, GT_F32>), g, b, 0, st, P);
+  else if (op == POST_PUSH)
+    launch_post_typed<R, POST_PUSH>(P, g, b, st);
   else
-    VDVC_POST(float);
-#undef VDVC_POST
+    launch_post_typed<R, POST_MASK>(P, g, b, st);
+}
+
+int launch_post_voxel(int op, int rtype, const PostIn& P, hipStream_t st) {
+  dim3 g((unsigned)((P.Nvx + 63) / 64), (unsigned)((P.Nvy + 3) / 4), (unsigned)P.Nz);
+  dim3 b(VDVC_BLOCK);
+  if (rtype == GT_F64)
+    launch_post_r<double>(op, P, g, b, st);
+  else
+    launch_post_r<float>(op, P, g, b, st);
   return (int)hipGetLastError();
 }
 

@@ -11,8 +11,13 @@ def_lo and the same GN log lines mirrored into run_log.txt.
 Outputs written (reference layouts, engine/write_output.py:738-752, 912-914):
 U_recovered.npy (Nx+1, Ny+1, Nz+1, 3), residual.npy (Nvx, Nvy, Nvz),
 U_local.npy (U minus the affine prior, flat), dU.npy (last GN update, flat),
-ref.npy and deformed.npy.  The derived post-solve fields of the reference's
-write_outputs (strains, masks, ZNCC, forward pushes, .vti) are not produced.
+ref.npy and deformed.npy; and the voxel-scale fields of the reference's
+write_outputs (engine/write_output.py:771-789, 838-849, 883-894), computed on
+the GPU: the active mask (compute_active_mask with g_origin = def_lo and
+mask0 = ref > glt) and the global ZNCC over it (printed),
+ref_forward_pushed.npy (ref pushed by U_local), residual_std_cell.npy and
+zncc_cell.npy.  The element-scale fields (strains, unsafe / overlap-lost
+masks, griddata remaps, .vti) are not produced.
 
     python -m voxeldvc_amd run WORK_DIR [--l0-factor F]
 """
@@ -56,6 +61,36 @@ def parse_args(argv=None):
     p.add_argument('--storage', choices=('fp32', 'bf16'), default='fp32',
                    help='voxel storage of the device solve (default fp32)')
     return p.parse_args(argv)
+
+
+def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt):
+    """The voxel-scale stages of the reference's write_outputs on the GPU
+    (engine/write_output.py:771-789, 838-849, 883-894): active mask and
+    global ZNCC, the forward-pushed reference, per-cell residual std and
+    ZNCC."""
+    from .engine.correlate_gpu import compute_active_mask
+    from .engine import write_output as W
+    ref_flat = ref.reshape(-1, order='F')
+    n_active = int(np.count_nonzero(res))
+    print('Nonzero (active) residual voxels: %d/%d (%.1f%%)'
+          % (n_active, res.size, 100 * n_active / res.size))
+    active = compute_active_mask(U, deformed, Nx, Ny, Nz, h, np,
+                                 g_origin=tuple(int(x) for x in lo_def),
+                                 mask0_vox=ref_flat > glt, glt=glt)
+    zncc = W.zncc_of_residual(ref_flat, res, active)
+    print('ZNCC (active voxels, no-ground-truth accuracy proxy): %.6f' % zncc)
+    pushed = W.push_reference_forward(ref, U_local, Nx, Ny, Nz, h)
+    np.save(os.path.join(wd, 'ref_forward_pushed.npy'), pushed)
+    std_c, zn_c = W.compute_correlation_quality_cell(res, ref, h, Nx, Ny, Nz)
+    np.save(os.path.join(wd, 'residual_std_cell.npy'), std_c)
+    np.save(os.path.join(wd, 'zncc_cell.npy'), zn_c)
+    if np.any(~np.isnan(std_c)):
+        print('  cell residual std: mean=%.4f  min=%.4f  max=%.4f'
+              % (np.nanmean(std_c), np.nanmin(std_c), np.nanmax(std_c)))
+    if np.any(~np.isnan(zn_c)):
+        print('  cell ZNCC: mean=%.5f  min=%.5f  max=%.5f'
+              % (np.nanmean(zn_c), np.nanmin(zn_c), np.nanmax(zn_c)))
+    return active, zncc
 
 
 def main(argv=None):
@@ -118,6 +153,7 @@ def main(argv=None):
         np.save(os.path.join(wd, 'ref.npy'), ref_c)
         np.save(os.path.join(wd, 'deformed.npy'),
                 deformed_cropped[:shp[0], :shp[1], :shp[2]])
+        post_outputs(wd, U, U_local, res, ref_c, deformed, Nx, Ny, Nz, h, lo_def, glt)
         print('\nDVC COMPLETE')
         t_end = time.perf_counter()
         print('Timing: setup %.2f s | multiscale correlation %.2f s | output %.2f s | total %.2f s'
