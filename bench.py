@@ -45,6 +45,9 @@ def parse():
     p.add_argument('--solve-reps', type=int, default=3)
     p.add_argument('--no-post', action='store_true',
                    help='skip the post-solve voxel stages')
+    p.add_argument('--configs', action='store_true',
+                   help='also measure the other benchmark configurations (C2, C3 at h=4, C5); '
+                        'several minutes')
     return p.parse_args()
 
 
@@ -130,6 +133,221 @@ def time_to_solve(n=513, h=8, reps=3, log=None):
         "ratio_converged": round(t_c / t_p, 4),
         "target_ratio": 1.0 / 3.0,
     }
+
+
+def _matvec_kernel_ms(n_el, h, G, gt, mask, reps=20):
+    """Mean vox_vec kernel time (HIP events on the library stream) of an H
+    matvec over device gradient planes G."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    c = L.ctx()
+    Ndof = 3 * (n_el[0] + 1) * (n_el[1] + 1) * (n_el[2] + 1)
+    v = torch.randn(Ndof, device='cuda')
+    out = L.empty(Ndof)
+    args = (c, n_el[0], n_el[1], n_el[2], h, G[0], G[1], G[2], gt, mask, v, out)
+    L.call('vdvc_matvec_H', *args)
+    torch.cuda.synchronize()
+    L.call('vdvc_timing_begin', c)
+    for _ in range(reps):
+        L.call('vdvc_matvec_H', *args)
+    tot = (L._d * 1)()
+    nl = (L._i * 1)()
+    L.call('vdvc_timing_end', c, tot, nl)
+    return tot[0] / max(1, nl[0])
+
+
+def _level_planes(f, shape, n_el, h, gtype):
+    """Pitched gradient planes (+ mask) of a dense x-fastest image, as the
+    solver builds them: (planes, mask0) for F32, folded planes for BF16."""
+    from voxeldvc_amd import _lib as L
+    c = L.ctx()
+    fp = L.pitch_dense(f, shape, 'float16')
+    Np = fp.numel()
+    dt = 'bfloat16' if gtype == L.BF16 else 'float32'
+    G = [L.empty(Np, dt) for _ in range(3)]
+    m = L.empty(Np, 'uint8')
+    L.call('vdvc_image_setup', c, n_el[0], n_el[1], n_el[2], h, fp, L.F16, 0.0, G[0], G[1], G[2],
+           gtype, m)
+    if gtype == L.BF16:
+        W = [L.empty(Np, 'bfloat16') for _ in range(3)]
+        L.call('vdvc_fold_grad', c, Np, G[0], G[1], G[2], m, W[0], W[1], W[2])
+        return W, None
+    return G, m
+
+
+def _wall_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def config_cases():
+    """Measurements the other benchmark configurations ask for (one GPU):
+      C2 257^3 necking, equilibrium gap, 4 levels: H and regularizer matvec
+         rates, FP32 solve time (and its distance to the FP64 solve);
+      C3 513^3 at h = 4: H matvec rates, FP32 / BF16 / FP64 solve times and
+         distances to the FP64 solve;
+      C5 1025 x 1025 x 769 (h = 16, equilibrium gap, necking + 8 Fourier
+         modes up to 8 vox + N(0,2)): BF16 H matvec % of peak, peak device
+         memory, and the learned-seed mixed-precision time to 0.01 vox RMS
+         against the FP64 solve of a 257^3 subvolume (interior nodes)."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd import seednet, synth
+    from voxeldvc_amd.engine import correlate_gpu as C
+    from voxeldvc_amd.engine.geometry_dvc import build_K_ref, build_K_ref_laplacian
+    out = {}
+    Kl, Kel = build_K_ref_laplacian(), build_K_ref(0.3)
+
+    def rms(a, b):
+        return float(((a.double() - b.double()) ** 2).mean().sqrt())
+
+    def solve(fv, gv, N, h, **kw):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lg = []
+        U, _, _ = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, log=lg, **kw)
+        torch.cuda.synchronize()
+        return U, (time.perf_counter() - t0) * 1e3, sum(len(lv['iters']) for lv in lg)
+
+    # ---- C2 ----
+    print('config C2', file=sys.stderr, flush=True)
+    n, h = 257, 8
+    N = ((n - 1) // h,) * 3
+    Nvox = n ** 3
+    Ndof = 3 * (N[0] + 1) ** 3
+    f, g, _, _ = synth.config_pair_torch('c2', (n, n, n), seed=2)
+    r = {}
+    for name, gt, bpv in (('bf16_folded', L.BF16, 6), ('fp32_mask', L.F32, 13)):
+        G, m = _level_planes(f, (n, n, n), N, h, gt)
+        ms = _matvec_kernel_ms(N, h, G, gt, m)
+        B = Nvox * bpv + 8 * Ndof
+        r['H_matvec_' + name] = {"kernel_ms": round(ms, 4), "Gvox_s": round(Nvox / ms / 1e6, 1),
+                                 "frac_8TBs": round(B / (ms * 1e-3) / 8e12, 4)}
+        del G, m
+    E = float(h)
+    v = torch.randn(Ndof, device='cuda')
+    o = torch.empty_like(v)
+    kp = np.ascontiguousarray(Kel, dtype=np.float64)
+    kl = np.ascontiguousarray(Kl, dtype=np.float64)
+    ms = _wall_ms(lambda: L.call('vdvc_matvec_eqgap', L.ctx(), N[0], N[1], N[2], L.F32,
+                                 kp.ctypes.data_as(L._dp), E, None, v, o), 50)
+    nn = Ndof // 3
+    r['eqgap_matvec'] = {"ms_wall": round(ms, 4), "Gnode_s": round(nn / ms / 1e6, 2),
+                         "algorithmic_bytes": 48 * nn, "GB_s": round(48 * nn / ms / 1e6, 1)}
+    ms = _wall_ms(lambda: L.call('vdvc_matvec_K', L.ctx(), N[0], N[1], N[2], L.F32,
+                                 kl.ctypes.data_as(L._dp), E, None, v, o), 50)
+    r['laplacian_matvec'] = {"ms_wall": round(ms, 4), "Gnode_s": round(nn / ms / 1e6, 2),
+                             "algorithmic_bytes": 24 * nn, "GB_s": round(24 * nn / ms / 1e6, 1)}
+    fv = f.view(n, n, n).permute(2, 1, 0)
+    gv = g.view(n, n, n).permute(2, 1, 0)
+    kw = dict(K_ref_laplacian=Kl, K_ref_elastic=Kel, reg_type='equilibrium_gap', l0=4.5 * h,
+              glt=0.0, dynamic_mask=True, freeze_mask_after=4, disp=False, scales=(3, 2, 1, 0))
+    U64, t64, gn64 = solve(fv, gv, N, h, dtype=np.float64, maxiter=60, eps=1e-6, **kw)
+    solve(fv, gv, N, h, **kw)
+    U32, t32, gn32 = solve(fv, gv, N, h, **kw)
+    r['solve_fp32'] = {"ms": round(t32, 2), "gn_iters": gn32, "rms_vs_fp64": rms(U32, U64)}
+    r['solve_fp64'] = {"ms": round(t64, 2), "gn_iters": gn64}
+    out['C2_257cube_necking_eqgap_4levels'] = r
+    del f, g, fv, gv, v, o
+
+    # ---- C3 at h = 4 ----
+    print('config C3 h=4', file=sys.stderr, flush=True)
+    n, h = 513, 4
+    N = ((n - 1) // h,) * 3
+    Nvox = n ** 3
+    Ndof = 3 * (N[0] + 1) ** 3
+    f, g, _, _ = synth.config_pair_torch('c3', (n, n, n), seed=3)
+    r = {}
+    for name, gt, bpv in (('bf16_folded', L.BF16, 6), ('fp32_mask', L.F32, 13)):
+        G, m = _level_planes(f, (n, n, n), N, h, gt)
+        ms = _matvec_kernel_ms(N, h, G, gt, m)
+        B = Nvox * bpv + 8 * Ndof
+        r['H_matvec_' + name] = {"kernel_ms": round(ms, 4), "Gvox_s": round(Nvox / ms / 1e6, 1),
+                                 "frac_8TBs": round(B / (ms * 1e-3) / 8e12, 4)}
+        del G, m
+    fv = f.view(n, n, n).permute(2, 1, 0)
+    gv = g.view(n, n, n).permute(2, 1, 0)
+    kw = dict(K_ref_laplacian=Kl, l0=4.5 * h, glt=0.0, dynamic_mask=True, freeze_mask_after=4,
+              disp=False)
+    U64, t64, gn64 = solve(fv, gv, N, h, dtype=np.float64, maxiter=60, eps=1e-6, **kw)
+    solve(fv, gv, N, h, **kw)
+    for st in ('fp32', 'bf16'):
+        U, t, gn = solve(fv, gv, N, h, storage=st, **kw)
+        r['solve_' + st] = {"ms": round(t, 2), "gn_iters": gn, "rms_vs_fp64": rms(U, U64)}
+    U, t, gn = solve(fv, gv, N, h, storage='bf16', refine_fp64=2, **kw)
+    r['solve_bf16_refined2'] = {"ms": round(t, 2), "rms_vs_fp64": rms(U, U64)}
+    r['solve_fp64'] = {"ms": round(t64, 2), "gn_iters": gn64}
+    out['C3_513cube_h4'] = r
+    del f, g, fv, gv
+
+    # ---- C5 ----
+    print('config C5', file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    # device memory reserved from here on (torch's cache keeps its peak
+    # reservation, the library's buffers only grow): the C5 footprint
+    free0 = torch.cuda.mem_get_info()[0]
+    shape, h = (1025, 1025, 769), 16
+    N = tuple((s - 1) // h for s in shape)
+    Nvox = shape[0] * shape[1] * shape[2]
+    Ndof = 3 * (N[0] + 1) * (N[1] + 1) * (N[2] + 1)
+    f, g, _, _ = synth.config_pair_torch('c5', shape, seed=5)
+    f = f.half()
+    g = g.half()
+    r = {}
+    G, m = _level_planes(f.float(), shape, N, h, L.BF16)
+    ms = _matvec_kernel_ms(N, h, G, L.BF16, m)
+    B = Nvox * 6 + 8 * Ndof
+    r['H_matvec_bf16_folded'] = {"kernel_ms": round(ms, 4), "Gvox_s": round(Nvox / ms / 1e6, 1),
+                                 "frac_8TBs": round(B / (ms * 1e-3) / 8e12, 4)}
+    del G, m
+    fv = f.view(shape[2], shape[1], shape[0]).permute(2, 1, 0)
+    gv = g.view(shape[2], shape[1], shape[0]).permute(2, 1, 0)
+    kw = dict(K_ref_laplacian=Kl, K_ref_elastic=Kel, reg_type='equilibrium_gap', l0=4.5 * h,
+              glt=0.0, dynamic_mask=True, freeze_mask_after=4, disp=False)
+    # FP64 answer on a 257^3 subvolume at the volume centre (g_origin = -offset)
+    o = tuple(((s - 1) // 2 - 128) // h * h for s in shape)
+    sub = fv[o[0]:o[0] + 257, o[1]:o[1] + 257, o[2]:o[2] + 257]
+    Ns = (16, 16, 16)
+    Us, _, _ = C.multiscale_correlate_gpu(sub.float().contiguous(), gv.float(), *Ns, h, torch,
+                                          dtype=np.float64, maxiter=60, eps=1e-6,
+                                          g_origin=tuple(-q for q in o), fft_prealign=False,
+                                          scales=(2, 1, 0), **kw)
+    # full-mesh nodes of the subvolume's interior nodes (2-element margin)
+    io = [q // h for q in o]
+    sl = (slice(2, Ns[2] - 1), slice(2, Ns[1] - 1), slice(2, Ns[0] - 1))
+    ref_nodes = Us.view(Ns[2] + 1, Ns[1] + 1, Ns[0] + 1, 3)[sl]
+
+    def sub_rms(U):
+        Uf = U.view(N[2] + 1, N[1] + 1, N[0] + 1, 3)[io[2]:io[2] + Ns[2] + 1,
+                                                     io[1]:io[1] + Ns[1] + 1,
+                                                     io[0]:io[0] + Ns[0] + 1][sl]
+        return float(((Uf.double() - ref_nodes) ** 2).mean().sqrt())
+
+    net = seednet.SeedNet.load()
+    trace = []
+    U, t, gn = solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net,
+                     callback=lambda s, ik, Ud: trace.append(sub_rms(Ud)), **kw)
+    k = next((i + 1 for i, e in enumerate(trace) if e <= 0.01), None)
+    r['learned_bf16_per_iter_rms_subvolume'] = [float('%.3g' % e) for e in trace]
+    if k is not None:
+        solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net, maxiter=k, **kw)
+        U, t, gn = solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net, maxiter=k,
+                         **kw)
+        r['learned_bf16_to_0.01'] = {"ms": round(t, 2), "gn_iters": gn,
+                                     "rms_subvolume": sub_rms(U)}
+    Up, tp, gnp = solve(fv, gv, N, h, scales=(3, 2, 1, 0), **kw)
+    r['plain_fp32_4levels'] = {"ms": round(tp, 2), "gn_iters": gnp, "rms_subvolume": sub_rms(Up)}
+    torch.cuda.synchronize()
+    r['peak_device_bytes'] = int(free0 - torch.cuda.mem_get_info()[0])
+    out['C5_1025x1025x769_h16'] = r
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hessian_matvec_traffic.json')
@@ -389,6 +607,8 @@ def main():
         res["post_stages"] = post_stages(n=a.size, h=a.h)
         if not a.no_cpu:
             res["post_stages"]["cpu_baseline"] = post_cpu_baseline()
+    if a.configs:
+        res["configs"] = config_cases()
     if not a.no_cpu:
         print('cpu baseline', file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

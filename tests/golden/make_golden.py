@@ -7,10 +7,14 @@ recorded here: the end-to-end single-level and multiscale Gauss-Newton solves
 equilibrium-gap solve and its Hutchinson diagonal (RNG stream and probe
 order), and the affine pre-alignment (SciPy Powell trajectory).
 
-    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [c1]
 
-Inputs are either stored in the fixture (small) or regenerated from the seeded
-generator in voxeldvc_amd/synth.py and checked by SHA-256.
+With no argument the small-case fixtures are written; `c1` writes the 65^3
+plain-method configuration (c1_*.npz: 4000-blob speckle, translation + 1 %
+z-strain + N(0,1) noise, h = 8, Laplacian, 3 levels, FFT and affine
+pre-alignment).  Inputs are either stored in the fixture (small) or
+regenerated from the seeded generator in voxeldvc_amd/synth.py and checked by
+SHA-256 (INPUT_SHA256.txt).
 """
 import hashlib
 import os
@@ -53,6 +57,38 @@ def speckle33(seed=7):
     g = synth.warp_backward(f, synth.translation_strain_field((1.3, -0.6, 0.4), 0.01, 16.0),
                             1.0, seed + 1000)
     return f, g
+
+
+def c1_inputs():
+    """The 65^3 plain-method pair as float32 (synth.c1_pair)."""
+    f, g = synth.c1_pair()
+    return f.astype(np.float32), g.astype(np.float32)
+
+
+def main_c1():
+    """Reference multiscale solves of the 65^3 configuration: FFT
+    pre-alignment (run_dvc-style dynamic mask frozen after 4 iterations) and
+    the affine pre-alignment.  Stores U, dU and residual statistics."""
+    K = rg.build_K_ref_laplacian()
+    f, g = c1_inputs()
+    kw = dict(K_ref_laplacian=K, l0=36.0, glt=0.0, dynamic_mask=True, freeze_mask_after=4,
+              disp=False)
+    out = {}
+    for name, pre in (('c1_fft', dict(fft_prealign=True)),
+                      ('c1_affine', dict(fft_prealign=False, prealign_affine=True))):
+        U, res, dU = rc.multiscale_correlate_gpu(f, g, 8, 8, 8, 8, np, scales=(2, 1, 0),
+                                                 **pre, **kw)
+        r = np.asarray(res, dtype=np.float64)
+        out[name] = dict(U=U, dU=dU, res_nonzero=np.int64(np.count_nonzero(r)),
+                         res_std=np.float64(r[r != 0].std()))
+    for name, arrs in out.items():
+        np.savez_compressed(os.path.join(HERE, name + '.npz'), **arrs)
+    path = os.path.join(HERE, 'INPUT_SHA256.txt')
+    lines = [ln for ln in open(path).read().splitlines() if not ln.startswith('c1_')]
+    lines += ['c1_f %s' % sha(f), 'c1_g %s' % sha(g)]
+    with open(path, 'w') as fh:
+        fh.write('\n'.join(lines) + '\n')
+    print('wrote', sorted(out))
 
 
 def main():
@@ -105,11 +141,19 @@ def main():
 
     for name, arrs in out.items():
         np.savez_compressed(os.path.join(HERE, name + '.npz'), **arrs)
-    with open(os.path.join(HERE, 'INPUT_SHA256.txt'), 'w') as fh:
+    path = os.path.join(HERE, 'INPUT_SHA256.txt')
+    keep = [ln for ln in open(path).read().splitlines() if ln.startswith('c1_')] \
+        if os.path.exists(path) else []
+    with open(path, 'w') as fh:
         fh.write('speckle33_f %s\nspeckle33_g %s\n' % (sha(f33), sha(g33)))
         fh.write('problem17_f %s\nproblem17_g %s\n' % (sha(f17), sha(g17)))
+        for ln in keep:
+            fh.write(ln + '\n')
     print('wrote', sorted(out))
 
 
 if __name__ == '__main__':
-    main()
+    if sys.argv[1:] == ['c1']:
+        main_c1()
+    else:
+        main()

@@ -142,15 +142,21 @@ class SeedNet:
     def predict(self, f, g, shape, Nx, Ny, Nz, h, itype=None):
         """Nodal displacement (device float32, reference DOF layout) of the
         h-mesh for device volumes f, g (dense x-fastest of dims `shape`).  The
-        mesh must have the element size the network was trained for."""
-        if h != self.h:
-            raise ValueError('the seed network predicts nodes of an h=%d mesh (got h=%d)'
-                             % (self.h, h))
-        x, dims = network_input(f, g, shape, h, itype)
+        network predicts the nodes of a mesh with its own element size
+        self.h; for h = r * self.h those include every node of the h-mesh
+        (every r-th node per axis), which is what is returned."""
+        if h % self.h or h < self.h:
+            raise ValueError('the seed network predicts nodes of an h=%d mesh; h=%d is not a '
+                             'multiple of it' % (self.h, h))
+        r = h // self.h
+        x, dims = network_input(f, g, shape, self.h, itype)
         out = self.forward(x, dims)
-        expect = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+        mx, my, mz = Nx * r, Ny * r, Nz * r
+        expect = 3 * (mx + 1) * (my + 1) * (mz + 1)
         if out.numel() != expect:
             raise ValueError('network output has %d values, the mesh %d' % (out.numel(), expect))
+        if r > 1:
+            out = out.view(mz + 1, my + 1, mx + 1, 3)[::r, ::r, ::r].contiguous().view(-1)
         return out
 
 

@@ -89,7 +89,9 @@ def necking_field(center, ez=0.02, nu_r=0.3, amp=0.5, width=40.0):
     def u(X, Y, Z):
         dx, dy = X - cx, Y - cy
         r2 = dx * dx + dy * dy
-        e = 1.0 + amp * np.exp(-r2 / (2 * width ** 2))
+        # NumPy arrays or torch tensors
+        e = 1.0 + amp * (np.exp(-r2 / (2 * width ** 2)) if isinstance(r2, np.ndarray)
+                         else (-r2 / (2 * width ** 2)).exp())
         return (-nu_r * ez * dx * e, -nu_r * ez * dy * e, ez * (Z - cz) * e)
     return u
 
@@ -212,16 +214,19 @@ def random_affine(rng, max_rot_deg, max_strain):
 
 
 def warp_backward_torch(f, shape, modes, noise_sigma, seed, quantize=True, slab=8,
-                        device='cuda', affine=None):
+                        device='cuda', affine=None, extra_u=None):
     """GPU form of warp_backward for large volumes: g(y) = f(y - u(y)) with u
-    a sum of plane-wave modes (fourier_params), trilinear interpolation with
-    linear extrapolation at the border (float64 coordinates and weights),
-    plus N(0, noise_sigma) noise (torch generator `seed`), then optionally
-    rounded and clipped to [0, 255].  f: flat x-fastest float32 tensor of
-    `shape` (nx, ny, nz); returns the same layout.  `affine` = (A, c) adds
-    the displacement (A - I)(y - c) to u."""
+    a sum of plane-wave modes (fourier_params; None for none), trilinear
+    interpolation with linear extrapolation at the border (float64
+    coordinates and weights), plus N(0, noise_sigma) noise (torch generator
+    `seed`), then optionally rounded and clipped to [0, 255].  f: flat
+    x-fastest float32 tensor of `shape` (nx, ny, nz); returns the same
+    layout.  `affine` = (A, c) adds the displacement (A - I)(y - c) to u;
+    `extra_u(X, Y, Z)` (e.g. necking_field) adds its three components."""
     import torch
     nx, ny, nz = shape
+    if modes is None:
+        modes = (np.zeros((0, 3)), np.zeros(0), np.zeros((0, 3)))
     kv, ph, av = (torch.as_tensor(a, dtype=torch.float64, device=device) for a in modes)
     ff = f.to(device=device, dtype=torch.float64)
     out = torch.empty(nx * ny * nz, dtype=torch.float32, device=device)
@@ -236,6 +241,9 @@ def warp_backward_torch(f, shape, modes, noise_sigma, seed, quantize=True, slab=
                + ph)
         s = torch.sin(ph_)
         u = [(s * av[:, c]).sum(-1) for c in range(3)]
+        if extra_u is not None:
+            eu = extra_u(X, Y, Z)
+            u = [u[c] + eu[c] for c in range(3)]
         if affine is not None:
             A, cen = affine
             Am = np.asarray(A, np.float64) - np.eye(3)
@@ -281,4 +289,44 @@ def fourier_u_nodes(modes, Nx, Ny, Nz, h):
     U = np.zeros(3 * X.size)
     for c in range(3):
         U[c::3] = (s * av[:, c]).sum(-1).ravel()
+    return U
+
+
+def config_pair_torch(kind, shape, seed=3):
+    """Device pairs of the benchmark configurations (flat x-fastest float32,
+    8-bit speckle quantised to 0..255 at the density of 250k blobs per 257^3):
+      'c2': tensile necking about the centre (necking_field) + N(0,1);
+      'c3': 16 Fourier modes, wavelengths 64-256, peak 4 vox + N(0,1);
+      'c5': necking + 8 Fourier modes, peak 8 vox + N(0,2).
+    Returns (f, g, modes, extra_u) so callers can evaluate the truth."""
+    f = speckle_torch(shape, blobs_for(shape), seed=seed).round().clamp_(0, 255)
+    c = tuple((n - 1) / 2.0 for n in shape)
+    if kind == 'c2':
+        modes, extra, noise = None, necking_field(c), 1.0
+    elif kind == 'c3':
+        modes, extra, noise = fourier_params(16, (64, 256), 4.0, seed + 4, shape), None, 1.0
+    elif kind == 'c5':
+        modes, extra, noise = fourier_params(8, (64, 256), 8.0, seed + 4, shape), \
+            necking_field(c), 2.0
+    else:
+        raise ValueError('unknown configuration %r' % (kind,))
+    g = warp_backward_torch(f, shape, modes, noise, seed=seed + 8, quantize=True,
+                            extra_u=extra)
+    return f, g, modes, extra
+
+
+def nodal_truth(modes, extra_u, Nx, Ny, Nz, h):
+    """First-order nodal truth u(x) of a config_pair_torch warp at the mesh
+    nodes (reference DOF layout, float64)."""
+    U = np.zeros(3 * (Nx + 1) * (Ny + 1) * (Nz + 1))
+    if modes is not None:
+        U += fourier_u_nodes(modes, Nx, Ny, Nz, h)
+    if extra_u is not None:
+        k = np.arange(Nz + 1) * h
+        j = np.arange(Ny + 1) * h
+        i = np.arange(Nx + 1) * h
+        Z, Y, X = np.meshgrid(k, j, i, indexing='ij')
+        eu = extra_u(X.astype(np.float64), Y.astype(np.float64), Z.astype(np.float64))
+        for c in range(3):
+            U[c::3] += np.asarray(eu[c]).ravel()
     return U
