@@ -215,6 +215,23 @@ def config_cases():
         torch.cuda.synchronize()
         return U, (time.perf_counter() - t0) * 1e3, sum(len(lv['iters']) for lv in lg)
 
+    def lap_eqgap(fv, gv, N, h, g_origin=(0, 0, 0), fft_prealign=True, scales=(3, 2, 1, 0),
+                  **extra):
+        """Laplacian pyramid, then the equilibrium gap on the finest level (the
+        equilibrium gap on the coarse levels stalls the reference's PCG)."""
+        lap = dict(K_ref_laplacian=Kl, l0=4.5 * h, glt=0.0, dynamic_mask=True,
+                   freeze_mask_after=4, disp=False, g_origin=g_origin)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        l1, l2 = [], []
+        U0, _, _ = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, log=l1,
+                                              scales=scales, fft_prealign=fft_prealign, **lap,
+                                              **extra)
+        U, _, _ = C.correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, U0=U0, log=l2,
+                                  K_ref_elastic=Kel, reg_type='equilibrium_gap', **lap, **extra)
+        torch.cuda.synchronize()
+        return U, (time.perf_counter() - t0) * 1e3, sum(len(lv['iters']) for lv in l1) + len(l2)
+
     # ---- C2 ----
     print('config C2', file=sys.stderr, flush=True)
     n, h = 257, 8
@@ -246,12 +263,11 @@ def config_cases():
                              "algorithmic_bytes": 24 * nn, "GB_s": round(24 * nn / ms / 1e6, 1)}
     fv = f.view(n, n, n).permute(2, 1, 0)
     gv = g.view(n, n, n).permute(2, 1, 0)
-    kw = dict(K_ref_laplacian=Kl, K_ref_elastic=Kel, reg_type='equilibrium_gap', l0=4.5 * h,
-              glt=0.0, dynamic_mask=True, freeze_mask_after=4, disp=False, scales=(3, 2, 1, 0))
-    U64, t64, gn64 = solve(fv, gv, N, h, dtype=np.float64, maxiter=60, eps=1e-6, **kw)
-    solve(fv, gv, N, h, **kw)
-    U32, t32, gn32 = solve(fv, gv, N, h, **kw)
-    r['solve_fp32'] = {"ms": round(t32, 2), "gn_iters": gn32, "rms_vs_fp64": rms(U32, U64)}
+    U64, t64, gn64 = lap_eqgap(fv, gv, N, h, dtype=np.float64, maxiter=60, eps=1e-6)
+    lap_eqgap(fv, gv, N, h)
+    U32, t32, gn32 = lap_eqgap(fv, gv, N, h)
+    r['solve_fp32'] = {"ms": round(t32, 2), "gn_iters": gn32, "rms_vs_fp64": rms(U32, U64),
+                       "levels": "Laplacian 3,2,1,0 then equilibrium gap at scale 0"}
     r['solve_fp64'] = {"ms": round(t64, 2), "gn_iters": gn64}
     out['C2_257cube_necking_eqgap_4levels'] = r
     del f, g, fv, gv, v, o
@@ -315,10 +331,9 @@ def config_cases():
     o = tuple(((s - 1) // 2 - 128) // h * h for s in shape)
     sub = fv[o[0]:o[0] + 257, o[1]:o[1] + 257, o[2]:o[2] + 257]
     Ns = (16, 16, 16)
-    Us, _, _ = C.multiscale_correlate_gpu(sub.float().contiguous(), gv.float(), *Ns, h, torch,
-                                          dtype=np.float64, maxiter=60, eps=1e-6,
-                                          g_origin=tuple(-q for q in o), fft_prealign=False,
-                                          scales=(2, 1, 0), **kw)
+    Us, _, _ = lap_eqgap(sub.float().contiguous(), gv.float(), Ns, h, dtype=np.float64,
+                         maxiter=60, eps=1e-6, g_origin=tuple(-q for q in o),
+                         fft_prealign=False, scales=(2, 1, 0))
     # full-mesh nodes of the subvolume's interior nodes (2-element margin)
     io = [q // h for q in o]
     sl = (slice(2, Ns[2] - 1), slice(2, Ns[1] - 1), slice(2, Ns[0] - 1))
@@ -342,8 +357,9 @@ def config_cases():
                          **kw)
         r['learned_bf16_to_0.01'] = {"ms": round(t, 2), "gn_iters": gn,
                                      "rms_subvolume": sub_rms(U)}
-    Up, tp, gnp = solve(fv, gv, N, h, scales=(3, 2, 1, 0), **kw)
-    r['plain_fp32_4levels'] = {"ms": round(tp, 2), "gn_iters": gnp, "rms_subvolume": sub_rms(Up)}
+    Up, tp, gnp = lap_eqgap(fv, gv, N, h)
+    r['plain_fp32_4levels'] = {"ms": round(tp, 2), "gn_iters": gnp, "rms_subvolume": sub_rms(Up),
+                               "levels": "Laplacian 3,2,1,0 then equilibrium gap at scale 0"}
     torch.cuda.synchronize()
     r['peak_device_bytes'] = int(free0 - torch.cuda.mem_get_info()[0])
     out['C5_1025x1025x769_h16'] = r

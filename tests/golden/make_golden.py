@@ -12,7 +12,8 @@ order), and the affine pre-alignment (SciPy Powell trajectory).
 With no argument the small-case fixtures are written; `c1` writes the 65^3
 plain-method configuration (c1_*.npz: 4000-blob speckle, translation + 1 %
 z-strain + N(0,1) noise, h = 8, Laplacian, 3 levels, FFT and affine
-pre-alignment).  Inputs are either stored in the fixture (small) or
+pre-alignment); `affine` writes affine_refine49.npz (the preprocess
+affine_refine_gn on a 49^3 pair pulled by a known affine).  Inputs are either stored in the fixture (small) or
 regenerated from the seeded generator in voxeldvc_amd/synth.py and checked by
 SHA-256 (INPUT_SHA256.txt).
 """
@@ -91,6 +92,37 @@ def main_c1():
     print('wrote', sorted(out))
 
 
+def affine49_inputs():
+    """49^3 speckle f and g(x) = f(x - M (x - c) - t_r) + N(0, 0.5): a known
+    affine in the generative convention of affine_refine_gn, float32."""
+    n = 49
+    f = synth.speckle_numpy((n, n, n), 1500, 31)
+    c = np.full(3, (n - 1) / 2.0)
+    M = np.array([[0.012, -0.004, 0.003], [0.002, -0.015, 0.005], [-0.003, 0.004, 0.02]])
+    tr = np.array([0.6, -0.4, 0.3])
+
+    def u(X, Y, Z):
+        d = [X - c[0], Y - c[1], Z - c[2]]
+        return tuple(M[i, 0] * d[0] + M[i, 1] * d[1] + M[i, 2] * d[2] + tr[i] for i in range(3))
+    g = synth.warp_backward(f, u, 0.5, 32)
+    return f.astype(np.float32), g.astype(np.float32), M, tr, c
+
+
+def main_affine():
+    f, g, M, tr, c = affine49_inputs()
+    A, t, info = rc.affine_refine_gn(f, g, np, np.eye(3), np.zeros(3), c)
+    np.savez_compressed(os.path.join(HERE, 'affine_refine49.npz'), A=A, t=t, c=c, M_true=M,
+                        tr_true=tr, rms_initial=np.float64(info['rms_initial']),
+                        rms_final=np.float64(info['rms_final']),
+                        iters=np.asarray(info['iters'], np.int64))
+    path = os.path.join(HERE, 'INPUT_SHA256.txt')
+    lines = [ln for ln in open(path).read().splitlines() if not ln.startswith('affine49_')]
+    lines += ['affine49_f %s' % sha(f), 'affine49_g %s' % sha(g)]
+    with open(path, 'w') as fh:
+        fh.write('\n'.join(lines) + '\n')
+    print('wrote affine_refine49', info)
+
+
 def main():
     out = {}
     K = rg.build_K_ref_laplacian()
@@ -126,6 +158,16 @@ def main():
                                   U0=None, disp=False)
     out['eqgap33'] = dict(U=U, res=res, dU=dU)
 
+    # equilibrium gap through a pyramid: the coarse level (h = 4) is where the
+    # Jacobi-preconditioned PCG stalls; 3 GN iterations, PCG capped at 300
+    U, res, dU = rc.multiscale_correlate_gpu(f33.astype(np.float32), g33.astype(np.float32),
+                                             4, 4, 4, 8, np, scales=(1, 0),
+                                             reg_type='equilibrium_gap', K_ref_elastic=Kel,
+                                             K_ref_laplacian=K, l0=36.0, eps_reg=1e-3,
+                                             maxiter=3, pcg_max_iter=300, fft_prealign=True,
+                                             dtype=np.float64, disp=False)
+    out['eqgap_multi33'] = dict(U=U, res=res, dU=dU)
+
     interior = rg.build_interior_dof_mask(3, 3, 3).astype(np.float32)
     d = rk.build_diagonal_equilibrium_gap(Kel, interior, 8, 3, 3, 3, 3 * 64, np)
     out['eqgap_diag3'] = dict(diag=d)
@@ -142,8 +184,8 @@ def main():
     for name, arrs in out.items():
         np.savez_compressed(os.path.join(HERE, name + '.npz'), **arrs)
     path = os.path.join(HERE, 'INPUT_SHA256.txt')
-    keep = [ln for ln in open(path).read().splitlines() if ln.startswith('c1_')] \
-        if os.path.exists(path) else []
+    keep = [ln for ln in open(path).read().splitlines()
+            if ln.startswith('c1_') or ln.startswith('affine49_')] if os.path.exists(path) else []
     with open(path, 'w') as fh:
         fh.write('speckle33_f %s\nspeckle33_g %s\n' % (sha(f33), sha(g33)))
         fh.write('problem17_f %s\nproblem17_g %s\n' % (sha(f17), sha(g17)))
@@ -155,5 +197,7 @@ def main():
 if __name__ == '__main__':
     if sys.argv[1:] == ['c1']:
         main_c1()
+    elif sys.argv[1:] == ['affine']:
+        main_affine()
     else:
         main()
