@@ -314,6 +314,60 @@ int vdvc_zncc(vdvc_ctx* ctx, int64_t n, const void* d_f, int ftype, const void* 
 int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_res, int rtype,
                       const void* d_ref, int ftype, double* d_std, double* d_zncc);
 
+/* ---- preprocess affine stage (dense float32 volumes, x fastest) ---- */
+
+/* scipy.ndimage.gaussian_filter of a float32 volume: one symmetric
+ * correlation per axis (0, 1, 2) in float64, SciPy's order, each pass stored
+ * float32; h_w = the r + 1 weights centre-first (the normalised Gaussian of
+ * radius r = int(4 sigma + 0.5)), mode 0 'nearest' / 1 'constant' (cval).
+ * Replaces the gaussian_filter calls of _smooth_plain / _smooth_masked
+ * (engine/correlate_gpu.py:570-598). */
+int vdvc_gaussian_filter(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_in, float* d_out,
+                         const double* h_w, int r, int mode, double cval);
+
+/* Normalised-convolution smoothing of g: smooth(g valid) / smooth(valid)
+ * ('constant' 0) where the denominator exceeds 1e-6, g elsewhere, with
+ * valid = (g != cval) also written to d_valid (optional); r < 0 writes only
+ * d_valid.  Replaces _smooth_masked (engine/correlate_gpu.py:586-598) and the
+ * fit mask valid_g (:697). */
+int vdvc_smooth_masked(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_g, double cval,
+                       const double* h_w, int r, float* d_out, uint8_t* d_valid);
+
+/* np.gradient of a float32 volume (unit spacing, edge order 1, dims >= 2).
+ * Replaces xp.gradient(f_s) of affine_refine_gn (engine/correlate_gpu.py:749). */
+int vdvc_gradient3(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, float* d_gx,
+                   float* d_gy, float* d_gz);
+
+/* One Gauss-Newton step's normal equations of the affine refinement:
+ * H = sum J J^T (12 x 12, row-major) and b = sum J e in float64 over all
+ * voxels, J and e in float32 at s = x - M (x - c) - t_r (M, t_r rounded to
+ * float32), e = (g - f(s)) mask, mask = in-bounds & valid.  Replaces the slab
+ * loop of affine_refine_gn (engine/correlate_gpu.py:755-793). */
+int vdvc_affine_normal(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, const float* d_g,
+                       const float* d_gx, const float* d_gy, const float* d_gz,
+                       const uint8_t* d_valid, const double* h_M, const double* h_tr,
+                       const double* h_c, double cval, double* h_H, double* h_b);
+
+/* (sum e^2, count) of the masked residual e = g - f(s) over in-bounds voxels
+ * with valid g.  Replaces _rms of affine_refine_gn
+ * (engine/correlate_gpu.py:725-743). */
+int vdvc_affine_rms(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, const float* d_g,
+                    const uint8_t* d_valid, const double* h_M, const double* h_tr,
+                    const double* h_c, double cval, double* h_out2);
+
+/* d_out (nx, ny, nz) = g (gnx, gny, gnz) sampled at A (x - c) + c + t
+ * (float32; A, t, c rounded to float32), trilinear, 0 outside.  Replaces
+ * warp_onto_ref_grid (preprocess.py:233-285). */
+int vdvc_warp_affine(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_g, int gnx, int gny,
+                     int gnz, const double* h_A, const double* h_t, const double* h_c,
+                     float* d_out);
+
+/* Bounding box (first and last index per axis, -1 when empty) of
+ * (f > glt) & (g_warped > glt), f in its own dtype (ftype).  Replaces the
+ * axis projections of compute_crop (preprocess.py:315-346). */
+int vdvc_overlap_bounds(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_f, int ftype,
+                        const float* d_gw, double glt, int* h_lo3, int* h_hi3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,14 @@ SIGNATURES = {
     'vdvc_push_forward': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp],
     'vdvc_zncc': [_vp, _i64, _vp, _i, _vp, _i, _i, _vp, _dp],
     'vdvc_quality_cell': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _vp, _vp],
+    'vdvc_gaussian_filter': [_vp, _i, _i, _i, _vp, _vp, _dp, _i, _i, _d],
+    'vdvc_smooth_masked': [_vp, _i, _i, _i, _vp, _d, _dp, _i, _vp, _vp],
+    'vdvc_gradient3': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp],
+    'vdvc_affine_normal': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, _d,
+                           _dp, _dp],
+    'vdvc_affine_rms': [_vp, _i, _i, _i, _vp, _vp, _vp, _dp, _dp, _dp, _d, _dp],
+    'vdvc_warp_affine': [_vp, _i, _i, _i, _vp, _i, _i, _i, _dp, _dp, _dp, _vp],
+    'vdvc_overlap_bounds': [_vp, _i, _i, _i, _vp, _i, _vp, _d, _ip, _ip],
 }
 
 _lib = None

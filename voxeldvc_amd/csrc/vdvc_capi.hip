@@ -40,6 +40,10 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != 0) return fail(-2, "%s: %s", #call, hipGetErrorString((hipError_t)e_));     \
   } while (0)
 
+// doubles of pinned host staging per context (largest host read-back: the
+// 90 normal-equation sums of the affine refinement)
+#define VDVC_HPIN 256
+
 // grow-only device scratch buffer
 struct Buf {
   void* p = nullptr;
@@ -82,7 +86,8 @@ struct vdvc_ctx {
   Buf hacc;                 // Hutchinson FP64 accumulator
   Buf dft;                  // DFT work volumes
   Buf wpart;                // conv weight-gradient partials
-  double* hpin = nullptr;   // pinned host staging (64 doubles)
+  Buf vtmp[4];              // dense float32 volume temporaries (affine stage)
+  double* hpin = nullptr;   // pinned host staging (VDVC_HPIN doubles)
   std::vector<double> k1_cache[2], k2_cache[2];
   // optional per-launch timing of the voxel-streaming kernels (HIP events
   // recorded on the context stream around each launch)
@@ -214,7 +219,7 @@ int upload_K2(vdvc_ctx* ctx, const double* hK, int* bd) {
 }
 
 int fetch(vdvc_ctx* ctx, const void* d, size_t bytes, void* h) {
-  if (bytes > 64 * sizeof(double)) return fail(-1, "fetch too large");
+  if (bytes > VDVC_HPIN * sizeof(double)) return fail(-1, "fetch too large");
   VDVC_HIP(hipMemcpyAsync(ctx->hpin, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
   VDVC_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(h, ctx->hpin, bytes);
@@ -587,7 +592,7 @@ int vdvc_ctx_create(int device, uintptr_t stream, vdvc_ctx** out) {
   vdvc_ctx* c = new vdvc_ctx();
   c->device = device;
   c->stream = reinterpret_cast<hipStream_t>(stream);
-  hipError_t e = hipHostMalloc((void**)&c->hpin, 64 * sizeof(double), hipHostMallocDefault);
+  hipError_t e = hipHostMalloc((void**)&c->hpin, VDVC_HPIN * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     delete c;
     return fail(-3, "hipHostMalloc: %s", hipGetErrorString(e));
@@ -1200,6 +1205,197 @@ int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   if ((rc = timed_event(ctx))) return rc;
   VDVC_LAUNCH(vdvc::launch_quality_cell(Q, ctx->stream));
   return timed_event(ctx);
+}
+
+}  // extern "C"
+
+// ---- preprocess affine stage ----
+
+static int dims_ok(int nx, int ny, int nz) {
+  if (nx < 2 || ny < 2 || nz < 2) return fail(-1, "volume dims must be >= 2 (got %d %d %d)", nx, ny, nz);
+  if (ny > 65535 || nz > 65535) return fail(-1, "ny, nz must be <= 65535");
+  if ((long long)nx * ny * nz > 0x7fffffffLL) return fail(-1, "volume exceeds 2^31 voxels");
+  return 0;
+}
+
+static int gauss_in(GaussIn& G, int nx, int ny, int nz, const double* h_w, int r, int mode,
+                    double cval) {
+  if (!h_w || r < 0 || r > VDVC_GAUSS_MAX_R) return fail(-1, "radius must be 0..%d", VDVC_GAUSS_MAX_R);
+  if (mode != 0 && mode != 1) return fail(-1, "mode must be 0 (nearest) or 1 (constant)");
+  std::memset(&G, 0, sizeof(G));
+  G.nx = nx; G.ny = ny; G.nz = nz; G.mode = mode; G.r = r; G.cval = cval;
+  for (int k = 0; k <= r; ++k) G.w[k] = h_w[k];
+  return 0;
+}
+
+// three axis passes in -> out -> tmp -> out
+static int gauss3(vdvc_ctx* ctx, GaussIn G, const float* in, float* out, float* tmp) {
+  G.axis = 0;
+  VDVC_LAUNCH(vdvc::launch_gauss_axis(G, in, out, ctx->stream));
+  G.axis = 1;
+  VDVC_LAUNCH(vdvc::launch_gauss_axis(G, out, tmp, ctx->stream));
+  G.axis = 2;
+  VDVC_LAUNCH(vdvc::launch_gauss_axis(G, tmp, out, ctx->stream));
+  return 0;
+}
+
+extern "C" {
+
+int vdvc_gaussian_filter(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_in, float* d_out,
+                         const double* h_w, int r, int mode, double cval) {
+  if (!ctx || !d_in || !d_out) return fail(-1, "NULL argument");
+  if (d_in == d_out) return fail(-1, "in-place filtering is not supported");
+  int rc = dims_ok(nx, ny, nz);
+  if (rc) return rc;
+  GaussIn G;
+  if ((rc = gauss_in(G, nx, ny, nz, h_w, r, mode, cval))) return rc;
+  const size_t n = (size_t)nx * ny * nz;
+  if ((rc = ctx->vtmp[0].ensure(n * sizeof(float)))) return rc;
+  return gauss3(ctx, G, d_in, d_out, ctx->vtmp[0].as<float>());
+}
+
+int vdvc_smooth_masked(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_g, double cval,
+                       const double* h_w, int r, float* d_out, uint8_t* d_valid) {
+  if (!ctx || !d_g || !d_out) return fail(-1, "NULL argument");
+  int rc = dims_ok(nx, ny, nz);
+  if (rc) return rc;
+  const long long n = (long long)nx * ny * nz;
+  if (r < 0) {  // no smoothing: the validity mask only
+    if (!d_valid) return fail(-1, "NULL valid mask");
+    VDVC_LAUNCH(vdvc::launch_mask_prep(n, d_g, (float)cval, nullptr, nullptr, d_valid,
+                                       ctx->stream));
+    return 0;
+  }
+  GaussIn G;
+  if ((rc = gauss_in(G, nx, ny, nz, h_w, r, 1, 0.0))) return rc;
+  for (int k = 0; k < 4; ++k)
+    if ((rc = ctx->vtmp[k].ensure((size_t)n * sizeof(float)))) return rc;
+  float* gv = ctx->vtmp[0].as<float>();
+  float* vf = ctx->vtmp[1].as<float>();
+  float* num = ctx->vtmp[2].as<float>();
+  float* tmp = ctx->vtmp[3].as<float>();
+  VDVC_LAUNCH(vdvc::launch_mask_prep(n, d_g, (float)cval, gv, vf, d_valid, ctx->stream));
+  if ((rc = gauss3(ctx, G, gv, num, tmp))) return rc;
+  if ((rc = gauss3(ctx, G, vf, gv, tmp))) return rc;   // den into gv
+  VDVC_LAUNCH(vdvc::launch_masked_ratio(n, num, gv, d_g, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_gradient3(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, float* d_gx,
+                   float* d_gy, float* d_gz) {
+  if (!ctx || !d_f || !d_gx || !d_gy || !d_gz) return fail(-1, "NULL argument");
+  int rc = dims_ok(nx, ny, nz);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_gradient(nx, ny, nz, d_f, d_gx, d_gy, d_gz, ctx->stream));
+  return 0;
+}
+
+static void aff_in(AffIn& A, int nx, int ny, int nz, const double* M, const double* tr,
+                   const double* c, double cval) {
+  std::memset(&A, 0, sizeof(A));
+  A.nx = nx; A.ny = ny; A.nz = nz; A.cval = cval;
+  for (int k = 0; k < 9; ++k) A.M[k] = (float)M[k];
+  for (int k = 0; k < 3; ++k) {
+    A.tr[k] = (float)tr[k];
+    A.c[k] = (float)c[k];
+  }
+}
+
+int vdvc_affine_normal(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, const float* d_g,
+                       const float* d_gx, const float* d_gy, const float* d_gz,
+                       const uint8_t* d_valid, const double* h_M, const double* h_tr,
+                       const double* h_c, double cval, double* h_H, double* h_b) {
+  if (!ctx || !d_f || !d_g || !d_gx || !d_gy || !d_gz || !d_valid || !h_M || !h_tr || !h_c ||
+      !h_H || !h_b)
+    return fail(-1, "NULL argument");
+  int rc = dims_ok(nx, ny, nz);
+  if (rc) return rc;
+  AffIn A;
+  aff_in(A, nx, ny, nz, h_M, h_tr, h_c, cval);
+  A.f = d_f; A.g = d_g; A.gx = d_gx; A.gy = d_gy; A.gz = d_gz; A.valid = d_valid;
+  const long long ntiles = ((long long)nx * ny * nz + 63) / 64;
+  const int nb = (int)std::min<long long>(1024, (ntiles + 3) / 4);
+  const int nw = nb * 4;
+  if ((rc = ctx->stats.ensure((size_t)nw * 90 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(96 * sizeof(double)))) return rc;
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_affine_normal(A, ctx->stats.as<double>(), nb, ctx->stream));
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nw, 90, ctx->red.as<double>(),
+                                  ctx->stream));
+  double s[90];
+  if ((rc = fetch(ctx, ctx->red.p, 90 * sizeof(double), s))) return rc;
+  int e = 0;
+  for (int a = 0; a < 12; ++a)
+    for (int b = a; b < 12; ++b, ++e) h_H[12 * a + b] = h_H[12 * b + a] = s[e];
+  for (int a = 0; a < 12; ++a) h_b[a] = s[78 + a];
+  return 0;
+}
+
+int vdvc_affine_rms(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, const float* d_g,
+                    const uint8_t* d_valid, const double* h_M, const double* h_tr,
+                    const double* h_c, double cval, double* h_out2) {
+  if (!ctx || !d_f || !d_g || !d_valid || !h_M || !h_tr || !h_c || !h_out2)
+    return fail(-1, "NULL argument");
+  int rc = dims_ok(nx, ny, nz);
+  if (rc) return rc;
+  AffIn A;
+  aff_in(A, nx, ny, nz, h_M, h_tr, h_c, cval);
+  A.f = d_f; A.g = d_g; A.valid = d_valid;
+  const int nb = 2048;
+  if ((rc = ctx->stats.ensure((size_t)nb * 2 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(96 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_affine_rms(A, ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 2, ctx->red.as<double>(),
+                                  ctx->stream));
+  return fetch(ctx, ctx->red.p, 2 * sizeof(double), h_out2);
+}
+
+int vdvc_warp_affine(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_g, int gnx, int gny,
+                     int gnz, const double* h_A, const double* h_t, const double* h_c,
+                     float* d_out) {
+  if (!ctx || !d_g || !d_out || !h_A || !h_t || !h_c) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1 || ny > 65535 || nz > 65535)
+    return fail(-1, "invalid output grid %d %d %d", nx, ny, nz);
+  if (gnx < 1 || gny < 1 || gnz < 1) return fail(-1, "invalid image dims");
+  WarpIn W;
+  std::memset(&W, 0, sizeof(W));
+  W.nx = nx; W.ny = ny; W.nz = nz; W.gnx = gnx; W.gny = gny; W.gnz = gnz;
+  for (int k = 0; k < 9; ++k) W.A[k] = (float)h_A[k];
+  for (int k = 0; k < 3; ++k) {
+    W.t[k] = (float)h_t[k];
+    W.c[k] = (float)h_c[k];
+  }
+  VDVC_LAUNCH(vdvc::launch_warp_affine(W, d_g, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_overlap_bounds(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_f, int ftype,
+                        const float* d_gw, double glt, int* h_lo3, int* h_hi3) {
+  if (!ctx || !d_f || !d_gw || !h_lo3 || !h_hi3) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1 || ny > 65535 || nz > 65535) return fail(-1, "invalid dims");
+  if (!itype_ok(ftype)) return fail(-1, "unsupported image type %d", ftype);
+  int rc;
+  const size_t nf = (size_t)nx + ny + nz;
+  if ((rc = ctx->vtmp[0].ensure(nf))) return rc;
+  uint8_t* flags = ctx->vtmp[0].as<uint8_t>();
+  VDVC_HIP(hipMemsetAsync(flags, 0, nf, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_overlap_proj(nx, ny, nz, d_f, ftype, d_gw, glt, flags, ctx->stream));
+  std::vector<uint8_t> h(nf);
+  VDVC_HIP(hipMemcpyAsync(h.data(), flags, nf, hipMemcpyDeviceToHost, ctx->stream));
+  VDVC_HIP(hipStreamSynchronize(ctx->stream));
+  const int n3[3] = {nx, ny, nz};
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    h_lo3[a] = h_hi3[a] = -1;
+    for (int i = 0; i < n3[a]; ++i)
+      if (h[off + i]) {
+        if (h_lo3[a] < 0) h_lo3[a] = i;
+        h_hi3[a] = i;
+      }
+    off += n3[a];
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -68,6 +68,35 @@ struct ZnccIn {
   int vec;                      // 4 voxels per thread step (16-byte aligned arrays)
 };
 
+// preprocess affine stage on dense float32 volumes (x fastest)
+#define VDVC_GAUSS_MAX_R 64
+struct GaussIn {
+  int nx, ny, nz;
+  int axis;                  // 0, 1, 2
+  int mode;                  // 0 'nearest', 1 'constant'
+  int r;                     // radius (<= VDVC_GAUSS_MAX_R)
+  double cval;
+  double w[VDVC_GAUSS_MAX_R + 1];   // centre tap, then taps 1..r
+};
+
+struct AffIn {
+  int nx, ny, nz;
+  float M[9], tr[3], c[3];   // generative affine (float32 as the reference rounds it)
+  double cval;
+  const float* f;            // (smoothed) reference
+  const float* g;            // (smoothed) deformed
+  const float* gx;           // gradient planes of f (normal equations)
+  const float* gy;
+  const float* gz;
+  const uint8_t* valid;      // g != cval
+};
+
+struct WarpIn {
+  int nx, ny, nz;            // output (reference) grid
+  int gnx, gny, gnz;         // sampled image
+  float A[9], t[3], c[3];
+};
+
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over
 // the elements adjacent to node (i,j,k).  blockdiag=1 means K[3a+c][3b+d] =
 // L8[a][b] delta_cd with L8 = K[3a][3b] (vector Laplacian), cheaper path.
@@ -194,6 +223,18 @@ int launch_quality_cell(const QualIn& Q, hipStream_t st);
 // (f g, f f, g g) around the means that `red` (count, sum f, sum g) defines
 int launch_zncc_pass(int pass, const ZnccIn& Z, const double* red, double* part, int nblocks,
                      hipStream_t st);
+int launch_gauss_axis(const GaussIn& G, const float* in, float* out, hipStream_t st);
+int launch_mask_prep(long long n, const float* g, float cval, float* gv, float* vf,
+                     uint8_t* valid, hipStream_t st);
+int launch_masked_ratio(long long n, const float* num, const float* den, const float* g,
+                        float* out, hipStream_t st);
+int launch_gradient(int nx, int ny, int nz, const float* f, float* gx, float* gy, float* gz,
+                    hipStream_t st);
+int launch_affine_normal(const AffIn& A, double* part, int nblocks, hipStream_t st);
+int launch_affine_rms(const AffIn& A, double* part, int nblocks, hipStream_t st);
+int launch_warp_affine(const WarpIn& W, const float* g, float* out, hipStream_t st);
+int launch_overlap_proj(int nx, int ny, int nz, const void* f, int ftype, const float* g,
+                        double glt, uint8_t* flags, hipStream_t st);
 int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
                       double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

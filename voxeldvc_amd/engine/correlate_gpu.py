@@ -295,6 +295,122 @@ def affine_prealign(f_pix, g_pix, xp=np, decimate=4, t0=None, angle_bound=0.3,
     return A, res.x[6:9] * decimate, c * decimate, info
 
 
+def gaussian_weights(sigma, truncate=4.0):
+    """(centre-first weights, radius) of scipy.ndimage.gaussian_filter's
+    normalised kernel: radius int(truncate * sigma + 0.5), exp(-x^2 / (2
+    sigma^2)) / sum, formed in float64 as SciPy forms it."""
+    sd = float(sigma)
+    r = int(truncate * sd + 0.5)
+    x = np.arange(-r, r + 1)
+    phi = np.exp(-0.5 / (sd * sd) * x ** 2)
+    phi = phi / phi.sum()
+    return np.ascontiguousarray(phi[r:]), r
+
+
+def _gauss_dev(d, shp, sigma, mode='nearest'):
+    """gaussian_filter of a dense x-fastest float32 device volume."""
+    w, r = gaussian_weights(sigma)
+    out = L.torch().empty_like(d)
+    L.call('vdvc_gaussian_filter', L.ctx(), shp[0], shp[1], shp[2], d, out,
+           w.ctypes.data_as(L._dp), r, 0 if mode == 'nearest' else 1, 0.0)
+    return out
+
+
+def _smooth_masked_dev(d, shp, sigma, cval, valid=None):
+    """smooth(g valid) / smooth(valid) where the denominator exceeds 1e-6
+    (correlate_gpu.py:586-598); also fills `valid` = (g != cval)."""
+    if sigma <= 0:
+        if valid is not None:
+            w = np.zeros(1)
+            L.call('vdvc_smooth_masked', L.ctx(), shp[0], shp[1], shp[2], d, float(cval),
+                   w.ctypes.data_as(L._dp), -1, d, valid)
+        return d
+    out = L.torch().empty_like(d)
+    w, r = gaussian_weights(sigma)
+    L.call('vdvc_smooth_masked', L.ctx(), shp[0], shp[1], shp[2], d, float(cval),
+           w.ctypes.data_as(L._dp), r, out, valid)
+    return out
+
+
+def affine_refine_gn(f_pix, g_pix, xp, A0, t0, c, sigmas=(4.0, 2.0, 1.0, 0.0), max_iter=50,
+                     tol=1e-6, patience=3, cval=0.0, chunk_voxels=8_000_000, disp=False):
+    """Full-resolution forward-additive Gauss-Newton refinement of the affine
+    x' = A (x - c) + c + t over a Gaussian scale space (sigmas, coarse
+    first), solved in the generative convention g(x) ~ f(x - M (x - c) - t_r)
+    and returned in the prealign convention A = (I - M)^-1, t = A t_r
+    (correlate_gpu.py:600-825).  Smoothing, gradients, the 12 x 12 normal
+    equations (float64 sums of float32 Jacobians) and the masked RMS run in
+    libvdvc.so; the 12 x 12 solve, the Levenberg damping and the per-level
+    stopping rules (||dp|| < tol, `patience` steps without a new minimum) on
+    the host.  chunk_voxels is accepted for signature parity (the device
+    kernels stream the whole volume).  Returns (A, t, info)."""
+    f, shp = L.to_dev_volume(f_pix, 'float32')
+    g, gshp = L.to_dev_volume(g_pix, 'float32')
+    if tuple(shp) != tuple(gshp):
+        raise ValueError('f_pix and g_pix must have the same shape (%r vs %r)'
+                         % (tuple(shp), tuple(gshp)))
+    nx, ny, nz = shp
+    c_ = L.ctx()
+    I3 = np.eye(3)
+    A0inv = np.linalg.inv(np.asarray(A0, dtype=np.float64))
+    M = I3 - A0inv
+    tr = A0inv @ np.asarray(t0, dtype=np.float64)
+    c = np.ascontiguousarray(np.asarray(c, dtype=np.float64))
+    valid = L.empty(nx * ny * nz, 'uint8')
+    _smooth_masked_dev(g, shp, 0.0, cval, valid)
+
+    def dptr(a):
+        return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(L._dp)
+
+    def rms(M_, tr_):
+        out = (L._d * 2)()
+        Mc = np.ascontiguousarray(M_, dtype=np.float64)
+        trc = np.ascontiguousarray(tr_, dtype=np.float64)
+        L.call('vdvc_affine_rms', c_, nx, ny, nz, f, g, valid, dptr(Mc), dptr(trc), dptr(c),
+               float(cval), out)
+        return float(np.sqrt(out[0] / max(out[1], 1.0)))
+
+    rms_initial = rms(M, tr)
+    iters = []
+    gx, gy, gz = (L.empty(nx * ny * nz) for _ in range(3))
+    H = np.zeros((12, 12))
+    b = np.zeros(12)
+    for sigma in sigmas:
+        f_s = _gauss_dev(f, shp, sigma) if sigma > 0 else f
+        g_s = _smooth_masked_dev(g, shp, sigma, cval)
+        L.call('vdvc_gradient3', c_, nx, ny, nz, f_s, gx, gy, gz)
+        n_it = 0
+        best = np.inf
+        stall = 0
+        for it in range(max_iter):
+            Mc = np.ascontiguousarray(M)
+            trc = np.ascontiguousarray(tr)
+            L.call('vdvc_affine_normal', c_, nx, ny, nz, f_s, g_s, gx, gy, gz, valid, dptr(Mc),
+                   dptr(trc), dptr(c), float(cval), H.ctypes.data_as(L._dp),
+                   b.ctypes.data_as(L._dp))
+            Hd = H + np.eye(12) * (1e-9 * float(np.trace(H)) / 12.0 + 1e-30)
+            dp = np.linalg.solve(Hd, b)
+            M = M + dp[:9].reshape(3, 3)
+            tr = tr + dp[9:]
+            step = float(np.linalg.norm(dp))
+            n_it = it + 1
+            if disp:
+                print("    sigma=%g iter %2d: ||dp||=%.3e" % (sigma, it, step), flush=True)
+            if step < tol:
+                break
+            if step < best - 1e-12:
+                best = step
+                stall = 0
+            else:
+                stall += 1
+                if stall >= patience:
+                    break
+        iters.append(n_it)
+    rms_final = rms(M, tr)
+    A = np.linalg.inv(I3 - M)
+    return A, A @ tr, {'rms_initial': rms_initial, 'rms_final': rms_final, 'iters': iters}
+
+
 def _stride_dev(vol, shp, s, type_=L.F32):
     """Stride-s decimation on the device; returns (tensor, shape)."""
     if s == 1:
