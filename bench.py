@@ -45,6 +45,8 @@ def parse():
     p.add_argument('--solve-reps', type=int, default=3)
     p.add_argument('--no-post', action='store_true',
                    help='skip the post-solve voxel stages')
+    p.add_argument('--no-affine', action='store_true',
+                   help='skip the preprocess affine stage')
     p.add_argument('--configs', action='store_true',
                    help='also measure the other benchmark configurations (C2, C3 at h=4, C5); '
                         'several minutes')
@@ -482,6 +484,104 @@ def post_cpu_baseline(n=129, h=8):
                                       "available" % (Nv, h, t, os.cpu_count())}
 
 
+def affine_stage(n=513, reps=5):
+    """The preprocess affine stage at n^3 (device-resident float32 volumes):
+    f = 8-bit-range speckle, g = f pulled by a known affine (rotation-free
+    strain/shear up to 3 %, translation ~1 vox) + N(0, 0.5).  Reports the
+    whole affine_refine_gn (4 scales) wall time and its recovery error, the
+    normal-equation kernel's time per GN step (HIP events) against its
+    algorithmic bytes (f_s, g_s, 3 gradient planes, valid: 21 B/voxel), one
+    Gaussian smoothing (sigma 4, 3 passes), warp_onto_ref_grid and
+    compute_crop."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd import preprocess as P
+    from voxeldvc_amd import synth
+    from voxeldvc_amd.engine import correlate_gpu as C
+    c_ = L.ctx()
+    N = n ** 3
+    f = synth.speckle_torch((n, n, n), synth.blobs_for((n, n, n)), seed=7)
+    c = np.full(3, (n - 1) / 2.0)
+    M = np.array([[0.02, -0.004, 0.003], [0.002, -0.012, 0.005], [-0.003, 0.004, 0.03]])
+    tr = np.array([1.2, -0.8, 0.6])
+    g = synth.warp_backward_torch(f, (n, n, n), None, 0.5, seed=8, quantize=False,
+                                  affine=(np.eye(3) + M, c),
+                                  extra_u=lambda X, Y, Z: (tr[0] + 0 * X, tr[1] + 0 * Y,
+                                                           tr[2] + 0 * Z))
+    fv = f.view(n, n, n).permute(2, 1, 0)
+    gv = g.view(n, n, n).permute(2, 1, 0)
+    out = {}
+    C.affine_refine_gn(fv, gv, torch, np.eye(3), np.zeros(3), c, sigmas=(1.0,), max_iter=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A_g, t_g, info = C.affine_refine_gn(fv, gv, torch, np.eye(3), np.zeros(3), c)
+    torch.cuda.synchronize()
+    A_true = np.linalg.inv(np.eye(3) - M)
+    out['affine_refine_gn'] = {
+        "ms": round((time.perf_counter() - t0) * 1e3, 2), "gn_iters": info['iters'],
+        "rms": [round(info['rms_initial'], 3), round(info['rms_final'], 4)],
+        "max_err_A": float('%.3g' % np.abs(A_g - A_true).max()),
+        "max_err_t_vox": float('%.3g' % np.abs(t_g - A_true @ tr).max())}
+    # one normal-equation step at sigma 0 (device planes)
+    valid = L.empty(N, 'uint8')
+    C._smooth_masked_dev(g, (n, n, n), 0.0, 0.0, valid)
+    gx, gy, gz = (L.empty(N) for _ in range(3))
+    L.call('vdvc_gradient3', c_, n, n, n, f, gx, gy, gz)
+    H = np.zeros((12, 12))
+    b = np.zeros(12)
+    dp = lambda a: np.ascontiguousarray(a, np.float64).ctypes.data_as(L._dp)
+    Mz, tz, cc = np.ascontiguousarray(M), np.ascontiguousarray(tr), np.ascontiguousarray(c)
+
+    def normal():
+        L.call('vdvc_affine_normal', c_, n, n, n, f, g, gx, gy, gz, valid, dp(Mz), dp(tz),
+               dp(cc), 0.0, H.ctypes.data_as(L._dp), b.ctypes.data_as(L._dp))
+    normal()
+    L.call('vdvc_timing_begin', c_)
+    for _ in range(reps):
+        normal()
+    tot = (L._d * 1)()
+    nl = (L._i * 1)()
+    L.call('vdvc_timing_end', c_, tot, nl)
+    ms = tot[0] / max(1, nl[0])
+    B = 21 * N
+    out['normal_equations_kernel'] = {"kernel_ms": round(ms, 4), "algorithmic_bytes": B,
+                                      "achieved_GBs": round(B / ms / 1e6, 1),
+                                      "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                      "Gvox_s": round(N / ms / 1e6, 2)}
+    out['gaussian_filter_sigma4_ms'] = round(_wall_ms(lambda: C._gauss_dev(f, (n, n, n), 4.0),
+                                                      reps), 3)
+    out['warp_onto_ref_grid_ms'] = round(_wall_ms(
+        lambda: P.warp_onto_ref_grid(np.eye(3) + M, tr, c, (n, n, n), gv, torch), reps), 3)
+    gw = P.warp_onto_ref_grid(np.eye(3) + M, tr, c, (n, n, n), gv, torch)
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):
+        out['compute_crop_ms'] = round(_wall_ms(lambda: P.compute_crop(fv, gw, 8), reps), 3)
+    return {"workload": "%d^3 speckle pulled by a known affine (strain/shear <= 3 %%, t ~1 "
+                        "vox) + N(0,0.5), float32, sigmas (4,2,1,0)" % n, "stages": out}
+
+
+def affine_cpu_baseline(n=97):
+    """The oracle's NumPy normal equations of one GN step on one host thread
+    at n^3: voxels per second."""
+    from threadpoolctl import threadpool_limits
+    from oracle import affine_oracle as AO
+    from voxeldvc_amd import synth
+    f = synth.speckle_numpy((n, n, n), synth.blobs_for((n, n, n)), 7).astype(np.float32)
+    c = np.full(3, (n - 1) / 2.0)
+    g = f.copy()
+    gg = AO.gradient(f)
+    with threadpool_limits(limits=1):
+        AO.normal_equations(f, g, gg, g != 0, np.zeros((3, 3)), np.zeros(3), c)
+        t0 = time.perf_counter()
+        AO.normal_equations(f, g, gg, g != 0, np.zeros((3, 3)), np.zeros(3), c)
+        t = time.perf_counter() - t0
+    return {"value": round(n ** 3 / t / 1e6, 3), "unit": "Mvox/s (one normal-equation step)",
+            "cores": 1, "kind": "port",
+            "sample": "oracle normal equations at %d^3: %.2f s; %d host cores available"
+                      % (n, t, os.cpu_count())}
+
+
 def measured_traffic(Nv, h):
     """HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement of the same workload (None if not measured)."""
@@ -623,6 +723,11 @@ def main():
         res["post_stages"] = post_stages(n=a.size, h=a.h)
         if not a.no_cpu:
             res["post_stages"]["cpu_baseline"] = post_cpu_baseline()
+    if not a.no_affine:
+        print('affine stage', file=sys.stderr, flush=True)
+        res["affine_stage"] = affine_stage(n=a.size)
+        if not a.no_cpu:
+            res["affine_stage"]["cpu_baseline"] = affine_cpu_baseline()
     if a.configs:
         res["configs"] = config_cases()
     if not a.no_cpu:

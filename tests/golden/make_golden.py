@@ -13,7 +13,10 @@ With no argument the small-case fixtures are written; `c1` writes the 65^3
 plain-method configuration (c1_*.npz: 4000-blob speckle, translation + 1 %
 z-strain + N(0,1) noise, h = 8, Laplacian, 3 levels, FFT and affine
 pre-alignment); `affine` writes affine_refine49.npz (the preprocess
-affine_refine_gn on a 49^3 pair pulled by a known affine).  Inputs are either stored in the fixture (small) or
+affine_refine_gn on a 49^3 pair pulled by a known affine); `gt` copies the
+reference's bundled ground-truth dataset GT_z+0.01 (assets/GT_z+0.01: the
+65^3 uint16 reference and its displacement field at the 65^3 image voxels,
+U[:65, :65, :65], float32) into gt_zp001.npz.  Inputs are either stored in the fixture (small) or
 regenerated from the seeded generator in voxeldvc_amd/synth.py and checked by
 SHA-256 (INPUT_SHA256.txt).
 """
@@ -123,6 +126,16 @@ def main_affine():
     print('wrote affine_refine49', info)
 
 
+def main_gt():
+    d = '/root/reference/assets/GT_z+0.01'
+    ref = np.load(os.path.join(d, 'ref.npy'), allow_pickle=False)
+    U = np.load(os.path.join(d, 'U.npy'), allow_pickle=False)
+    n = ref.shape[0]
+    np.savez_compressed(os.path.join(HERE, 'gt_zp001.npz'), ref=ref,
+                        u_gt_vox=np.ascontiguousarray(U[:n, :n, :n]))
+    print('wrote gt_zp001', ref.shape, ref.dtype, U.shape)
+
+
 def main():
     out = {}
     K = rg.build_K_ref_laplacian()
@@ -199,5 +212,7 @@ if __name__ == '__main__':
         main_c1()
     elif sys.argv[1:] == ['affine']:
         main_affine()
+    elif sys.argv[1:] == ['gt']:
+        main_gt()
     else:
         main()

@@ -1,5 +1,7 @@
-"""Command-line entry: `python -m voxeldvc_amd run WORK_DIR [--l0-factor F]`,
-the `run` subcommand of the reference's CLI (src/voxeldvc/cli.py:13-61)."""
+"""Command-line entry: `python -m voxeldvc_amd preprocess REF.npy DEF.npy
+--output-dir DIR [...]` and `python -m voxeldvc_amd run WORK_DIR
+[--l0-factor F]`, the `preprocess` and `run` subcommands of the reference's
+CLI (src/voxeldvc/cli.py:13-61)."""
 import sys
 
 
@@ -13,7 +15,12 @@ def main(argv=None):
         from .run_dvc import main as run_main
         run_main(rest)
         return 0
-    print('unknown subcommand %r (this package provides: run)' % cmd, file=sys.stderr)
+    if cmd == 'preprocess':
+        from .preprocess import main as pre_main
+        pre_main(rest)
+        return 0
+    print('unknown subcommand %r (this package provides: preprocess, run)' % cmd,
+          file=sys.stderr)
     return 2
 
 
