@@ -10,8 +10,9 @@
 //     affine_refine_gn (correlate_gpu.py:704-793): per voxel the sample
 //     location s = x - M (x - c) - t_r in float32 (the reference's
 //     element-wise order), f and the three gradient planes sampled there
-//     (map_coordinates order 1, 'constant' / 'nearest'), the float32 Jacobian
-//     J_k and residual e, and sum J J^T, sum J e in float64;
+//     (map_coordinates order 1, 'constant' / 'nearest', one set of corners
+//     and weights for the four images), the float32 Jacobian J_k and
+//     residual e, and sum J J^T, sum J e in float64 registers;
 //   * affine_rms_kernel: the masked residual sum of squares (:725-743);
 //   * warp_affine_kernel: g at A (x - c) + c + t, trilinear, 0 outside
 //     (preprocess.py:233-285);
@@ -28,32 +29,85 @@ using vdvc_img::as_img_type;
 using vdvc_img::ld_img;
 using vdvc_img::sample01;
 
+// Value at offset d (|d| <= r) along the filtered line from the voxel at
+// p (line position pos of len, element stride `stride`); with EDGE the
+// position is clamped ('nearest') or replaced by cval ('constant').
+template <bool EDGE>
+__device__ __forceinline__ double gauss_tap(const GaussIn& G, const float* p, int pos, int len,
+                                            int stride, int d) {
+  if (!EDGE) return (double)p[d * stride];
+  const int q = pos + d;
+  const int qc = min(max(q, 0), len - 1);
+  const double v = (double)p[(qc - pos) * stride];
+  return (G.mode == 1 && qc != q) ? G.cval : v;
+}
+
+// taps k, k-1, ..., k-C+1 of both sides loaded together, then accumulated
+// in that order
+template <int C, bool EDGE>
+__device__ __forceinline__ void gauss_chunk(const GaussIn& G, const float* p, int pos, int len,
+                                            int stride, int k, double& acc) {
+  double lo[C], hi[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    lo[j] = gauss_tap<EDGE>(G, p, pos, len, stride, -(k - j));
+    hi[j] = gauss_tap<EDGE>(G, p, pos, len, stride, k - j);
+  }
+#pragma unroll
+  for (int j = 0; j < C; ++j) acc = add_rn(acc, mul_rn(add_rn(lo[j], hi[j]), G.w[k - j]));
+}
+
+// SciPy's symmetric correlation at one voxel: the centre tap, then
+// (x[i-k] + x[i+k]) * w_k for k = r..1, in float64
+template <bool EDGE>
+__device__ __forceinline__ double gauss_line(const GaussIn& G, const float* p, int pos, int len,
+                                             int stride) {
+  double acc = mul_rn((double)p[0], G.w[0]);
+  int k = G.r;
+  for (; k >= 8; k -= 8) gauss_chunk<8, EDGE>(G, p, pos, len, stride, k, acc);
+  if (k >= 4) {
+    gauss_chunk<4, EDGE>(G, p, pos, len, stride, k, acc);
+    k -= 4;
+  }
+  for (; k >= 1; --k) gauss_chunk<1, EDGE>(G, p, pos, len, stride, k, acc);
+  return acc;
+}
+
+// One output voxel per thread.  Blocks cover 256 x-consecutive voxels of one
+// (y, z) row; the 1-D grid is ordered with the filtered axis fastest and
+// spread so that each XCD takes a contiguous run of it (the 2r + 1 lines a
+// block reads are the ones its XCD's L2 already holds for the blocks
+// before).  Interior voxels take the path without the extension tests.
 __global__ __launch_bounds__(VDVC_BLOCK) void gauss_axis_kernel(GaussIn G,
                                                                 const float* __restrict__ in,
                                                                 float* __restrict__ out) {
-  const int x = blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
+  const unsigned nxc = (unsigned)(G.nx + VDVC_BLOCK - 1) / VDVC_BLOCK;
+  unsigned L = xcd_remap(blockIdx.x, gridDim.x);
+  unsigned xc, y, z;
+  if (G.axis == 2) {
+    z = L % (unsigned)G.nz; L /= (unsigned)G.nz;
+    y = L % (unsigned)G.ny; xc = L / (unsigned)G.ny;
+  } else if (G.axis == 1) {
+    y = L % (unsigned)G.ny; L /= (unsigned)G.ny;
+    xc = L % nxc; z = L / nxc;
+  } else {
+    xc = L % nxc; L /= nxc;
+    y = L % (unsigned)G.ny; z = L / (unsigned)G.ny;
+  }
+  const int x = (int)xc * VDVC_BLOCK + threadIdx.x;
   if (x >= G.nx) return;
   const long long i = (long long)x + (long long)G.nx * ((long long)y + (long long)G.ny * z);
-  int len, pos;
-  long long stride;
+  int len, pos, stride;
   if (G.axis == 0) {
     len = G.nx; pos = x; stride = 1;
   } else if (G.axis == 1) {
-    len = G.ny; pos = y; stride = G.nx;
+    len = G.ny; pos = (int)y; stride = G.nx;
   } else {
-    len = G.nz; pos = z; stride = (long long)G.nx * G.ny;
+    len = G.nz; pos = (int)z; stride = G.nx * G.ny;
   }
-  const long long base = i - (long long)pos * stride;
-  auto at = [&](int q) -> double {
-    if (q < 0 || q >= len) {
-      if (G.mode == 1) return G.cval;
-      q = q < 0 ? 0 : len - 1;
-    }
-    return (double)in[base + (long long)q * stride];
-  };
-  double acc = mul_rn(at(pos), G.w[0]);
-  for (int k = G.r; k >= 1; --k) acc = add_rn(acc, mul_rn(add_rn(at(pos - k), at(pos + k)), G.w[k]));
+  const float* p = in + i;
+  const double acc = (pos >= G.r && pos + G.r < len) ? gauss_line<false>(G, p, pos, len, stride)
+                                                     : gauss_line<true>(G, p, pos, len, stride);
   out[i] = __double2float_rn(acc);
 }
 
@@ -133,90 +187,114 @@ __device__ __forceinline__ bool in_box(const float (&s)[3], int nx, int ny, int 
          s[2] >= 0.0f && s[2] <= (float)(nz - 1);
 }
 
-__device__ __forceinline__ float sample_f32(const float* img, int nx, int ny, int nz,
-                                            const float (&s)[3], int mode, double cval) {
-  return (float)as_img_type(
-      sample01<GT_F32>(img, nx, ny, nz, (double)s[0], (double)s[1], (double)s[2], 1, mode, cval),
-      GT_F32);
+// Trilinear corners of an in-box sample location, shared by every image
+// sampled there: base offset, the x / y / z steps to the upper corners (0 at
+// the last index, where the index is clamped) and SciPy's float64 weights.
+struct Corners {
+  long long o;
+  int ox, oy, oz;
+  double wx[2], wy[2], wz[2];
+};
+
+__device__ __forceinline__ Corners in_box_corners(const float (&s)[3], int nx, int ny, int nz) {
+  Corners C;
+  const double fx = floor((double)s[0]), fy = floor((double)s[1]), fz = floor((double)s[2]);
+  const double tx = sub_rn((double)s[0], fx), ty = sub_rn((double)s[1], fy),
+               tz = sub_rn((double)s[2], fz);
+  C.wx[0] = sub_rn(1.0, tx); C.wx[1] = tx;
+  C.wy[0] = sub_rn(1.0, ty); C.wy[1] = ty;
+  C.wz[0] = sub_rn(1.0, tz); C.wz[1] = tz;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const long long sy = nx, sz = (long long)nx * ny;
+  C.o = ix + sy * iy + sz * iz;
+  C.ox = ix + 1 < nx ? 1 : 0;
+  C.oy = iy + 1 < ny ? nx : 0;
+  C.oz = iz + 1 < nz ? (int)sz : 0;
+  return C;
 }
 
-// (a, b) of normal-equation entry e: the 78 upper-triangle entries of
-// H = sum J J^T row by row, then the 12 entries of b = sum J e (b = 12)
-__device__ __forceinline__ void entry_ab(int e, int& a, int& b) {
-  if (e >= 78) {
-    a = e - 78;
-    b = 12;
-    return;
-  }
-  a = 0;
-  int rem = e;
-  while (rem >= 12 - a) {
-    rem -= 12 - a;
-    ++a;
-  }
-  b = a + rem;
-}
-
-// Per wavefront: tiles of 64 consecutive voxels (one per lane); each lane
-// writes its voxel's 12 Jacobian channels and e to the wavefront's own LDS
-// tile (row pitch 65 floats: no bank conflicts), then lane l accumulates
-// entries l and l + 64 of the 90 normal-equation sums over the tile's voxels
-// in order (float32 products are exact in float64).  Wavefronts of a block
-// run different numbers of tiles, so only wavefront-level ordering is used
-// (no workgroup barrier).  Partials per wavefront [wave][90].
-constexpr int kRow = 65;
-__global__ __launch_bounds__(VDVC_BLOCK) void affine_normal_kernel(AffIn A, double* part) {
-  __shared__ float lds[VDVC_WAVES_PER_BLOCK][13 * kRow];
-  const int lane = threadIdx.x & 63;
-  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  float* T = lds[wib];
-  const long long n = (long long)A.nx * A.ny * A.nz;
-  const long long ntiles = (n + 63) / 64;
-  const long long wave = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + wib;
-  const long long nwaves = (long long)gridDim.x * VDVC_WAVES_PER_BLOCK;
-  int a1, b1, a2 = 0, b2 = 0;
-  entry_ab(lane, a1, b1);
-  const bool two = lane + 64 < 90;
-  if (two) entry_ab(lane + 64, a2, b2);
-  double acc1 = 0.0, acc2 = 0.0;
-  for (long long t = wave; t < ntiles; t += nwaves) {
-    const long long v = t * 64 + lane;
-    float J[13];
+// map_coordinates order 1 of a float32 image at an in-box point: the sum of
+// ((v * wx) * wy) * wz over the corners, z fastest, in float64, to float32
+// (identical for 'nearest' and 'constant' inside the box)
+__device__ __forceinline__ float tri_f32(const float* __restrict__ img, const Corners& C) {
+  const float* p = img + C.o;
+  double v = 0.0;
 #pragma unroll
-    for (int k = 0; k < 13; ++k) J[k] = 0.0f;
-    if (v < n) {
-      int x, y, z;
-      split_index(v, A.nx, A.ny, x, y, z);
-      float Xc[3], s[3];
-      affine_coords(A, x, y, z, Xc, s);
-      const float mask = (in_box(s, A.nx, A.ny, A.nz) && A.valid[v] != 0) ? 1.0f : 0.0f;
-      const float warped = sample_f32(A.f, A.nx, A.ny, A.nz, s, 1, A.cval);
-      const float* G[3] = {A.gx, A.gy, A.gz};
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double val = (double)p[a * C.ox + b * C.oy + c * C.oz];
+        v = add_rn(v, mul_rn(mul_rn(mul_rn(val, C.wx[a]), C.wy[b]), C.wz[c]));
+      }
+  return __double2float_rn(v);
+}
+
+// Normal equations: each thread strides over voxels v, v + S, ... (S the
+// grid's thread count; (x, y, z) stepped incrementally) and keeps the 90
+// float64 sums in registers -- the 78 upper-triangle entries of
+// H = sum J J^T row by row, then b = sum J e -- exact float32 products
+// accumulated with fma.  A voxel whose sample location leaves the box or
+// whose g is invalid has J = 0 and adds nothing (finite inputs), so it is
+// skipped.  The wavefront's sums are combined by a fixed butterfly; lane 0
+// writes the partials [wave][90].
+__global__ __launch_bounds__(VDVC_BLOCK) void affine_normal_kernel(AffIn A, double* part) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * VDVC_BLOCK + threadIdx.x) >> 6;
+  const long long n = (long long)A.nx * A.ny * A.nz;
+  const long long S = (long long)gridDim.x * VDVC_BLOCK;
+  long long v = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  int x, y, z, dx, dy, dz;
+  split_index(v, A.nx, A.ny, x, y, z);
+  split_index(S, A.nx, A.ny, dx, dy, dz);
+  double acc[90];
+#pragma unroll
+  for (int e = 0; e < 90; ++e) acc[e] = 0.0;
+  // valid and g of the next voxel are loaded one iteration ahead, so the
+  // gathers of a voxel do not wait for its mask
+  uint8_t vnext = v < n ? A.valid[v] : (uint8_t)0;
+  float gnext = v < n ? A.g[v] : 0.0f;
+  for (; v < n; v += S) {
+    const uint8_t vcur = vnext;
+    const float gcur = gnext;
+    if (v + S < n) {
+      vnext = A.valid[v + S];
+      gnext = A.g[v + S];
+    }
+    float Xc[3], s[3];
+    affine_coords(A, x, y, z, Xc, s);
+    if (vcur != 0 && in_box(s, A.nx, A.ny, A.nz)) {
+      const Corners C = in_box_corners(s, A.nx, A.ny, A.nz);
+      const float warped = tri_f32(A.f, C);
+      const float ng[3] = {-tri_f32(A.gx, C), -tri_f32(A.gy, C), -tri_f32(A.gz, C)};
+      double J[13];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const float ng = -sample_f32(G[i], A.nx, A.ny, A.nz, s, 0, 0.0);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) J[3 * i + j] = mul_rn(mul_rn(ng, Xc[j]), mask);
-        J[9 + i] = mul_rn(ng, mask);
+        for (int j = 0; j < 3; ++j) J[3 * i + j] = (double)mul_rn(ng[i], Xc[j]);
+        J[9 + i] = (double)ng[i];
       }
-      J[12] = mul_rn(sub_rn(A.g[v], warped), mask);
-    }
+      J[12] = (double)sub_rn(gcur, warped);
+      int e = 0;
 #pragma unroll
-    for (int k = 0; k < 13; ++k) T[k * kRow + lane] = J[k];
-    // the tile is read by the other lanes of this wavefront only
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int q = 0; q < 64; ++q) {
-      acc1 += (double)T[a1 * kRow + q] * (double)T[b1 * kRow + q];
-      if (two) acc2 += (double)T[a2 * kRow + q] * (double)T[b2 * kRow + q];
+      for (int a = 0; a < 12; ++a)
+#pragma unroll
+        for (int b = a; b < 12; ++b, ++e) acc[e] = fma(J[a], J[b], acc[e]);
+#pragma unroll
+      for (int a = 0; a < 12; ++a) acc[78 + a] = fma(J[a], J[12], acc[78 + a]);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    x += dx; y += dy; z += dz;
+    if (x >= A.nx) { x -= A.nx; ++y; }
+    if (y >= A.ny) { y -= A.ny; ++z; }
   }
-  part[wave * 90 + lane] = acc1;
-  if (two) part[wave * 90 + lane + 64] = acc2;
+#pragma unroll
+  for (int e = 0; e < 90; ++e) {
+    double t = acc[e];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) part[wave * 90 + e] = t;
+  }
 }
 
 // masked residual sum of squares: per block (sum e^2 in float64 of float32
@@ -232,7 +310,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void affine_rms_kernel(AffIn A, double*
     float Xc[3], s[3];
     affine_coords(A, x, y, z, Xc, s);
     if (in_box(s, A.nx, A.ny, A.nz) && A.valid[v] != 0) {
-      const float e = sub_rn(A.g[v], sample_f32(A.f, A.nx, A.ny, A.nz, s, 1, A.cval));
+      const float e = sub_rn(A.g[v], tri_f32(A.f, in_box_corners(s, A.nx, A.ny, A.nz)));
       acc[0] += (double)mul_rn(e, e);
       acc[1] += 1.0;
     }
@@ -296,8 +374,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void overlap_proj_kernel(int nx, int ny
 namespace vdvc {
 
 int launch_gauss_axis(const GaussIn& G, const float* in, float* out, hipStream_t st) {
-  dim3 g((unsigned)((G.nx + VDVC_BLOCK - 1) / VDVC_BLOCK), (unsigned)G.ny, (unsigned)G.nz);
-  hipLaunchKernelGGL(gauss_axis_kernel, g, dim3(VDVC_BLOCK), 0, st, G, in, out);
+  const unsigned long long nb =
+      (unsigned long long)((G.nx + VDVC_BLOCK - 1) / VDVC_BLOCK) * G.ny * G.nz;
+  // 32-bit tap offsets k * stride (k <= r) and a 32-bit 1-D grid
+  if (nb > 0x7fffffffull || (long long)G.nx * G.ny * (G.r > 0 ? G.r : 1) > 0x7fffffffll)
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(gauss_axis_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), 0, st, G, in, out);
   return (int)hipGetLastError();
 }
 

@@ -1313,9 +1313,10 @@ int vdvc_affine_normal(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_f, 
   AffIn A;
   aff_in(A, nx, ny, nz, h_M, h_tr, h_c, cval);
   A.f = d_f; A.g = d_g; A.gx = d_gx; A.gy = d_gy; A.gz = d_gz; A.valid = d_valid;
-  const long long ntiles = ((long long)nx * ny * nz + 63) / 64;
-  const int nb = (int)std::min<long long>(1024, (ntiles + 3) / 4);
-  const int nw = nb * 4;
+  // two 4-wavefront blocks per CU at the kernel's occupancy of 2 per SIMD
+  const long long nvox = (long long)nx * ny * nz;
+  const int nb = (int)std::min<long long>(512, (nvox + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  const int nw = nb * VDVC_WAVES_PER_BLOCK;
   if ((rc = ctx->stats.ensure((size_t)nw * 90 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(96 * sizeof(double)))) return rc;
   if ((rc = timed_event(ctx))) return rc;

@@ -560,23 +560,23 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
   }
 }
 
-// fixed-order reduction of n partial records of width NV (deterministic)
+// fixed-order reduction of n partial records of width nv (deterministic):
+// block q sums column q -- thread t over records t, t + 256, ..., then a
+// tree over the block
 __global__ __launch_bounds__(VDVC_BLOCK) void reduce_partials_kernel(const double* __restrict__ part,
                                                                      long long n, int nv,
                                                                      double* __restrict__ out) {
   __shared__ double lds[VDVC_BLOCK];
-  for (int q = 0; q < nv; ++q) {
-    double s = 0.0;
-    for (long long i = threadIdx.x; i < n; i += VDVC_BLOCK) s += part[i * nv + q];
-    lds[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[q] = lds[0];
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += VDVC_BLOCK) s += part[i * nv + q];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[q] = lds[0];
 }
 
 // bfloat16 gradient planes multiplied by a 0/1 voxel mask (the folded
@@ -774,7 +774,8 @@ template int launch_mul<float>(long long, const float*, const float*, float*, hi
 template int launch_mul<double>(long long, const double*, const double*, double*, hipStream_t);
 
 int launch_reduce(const double* part, long long n, int nv, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, part, n, nv, out);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nv), dim3(VDVC_BLOCK), 0, st, part, n, nv,
+                     out);
   return (int)hipGetLastError();
 }
 
