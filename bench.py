@@ -438,6 +438,34 @@ def post_stages(n=513, h=8, reps=10):
           lambda: L.call('vdvc_quality_cell', c, n_el, n_el, n_el, h, res, L.F32, f8, L.U8, std,
                          zn),
           5 * Nvox + 16 * Ne)
+    # element-scale output fields (unsafe / overlap-lost masks, safe cells,
+    # strains, GN convergence, the deformed-configuration mask remap)
+    ue = torch.empty(Ne, dtype=torch.uint8, device='cuda')
+    un = torch.empty(Nn, dtype=torch.uint8, device='cuda')
+    uv = L.empty(Nvox, 'uint8')
+    ol = torch.empty(Ne, dtype=torch.uint8, device='cuda')
+    timed("cell masks (unsafe element / node / voxel + overlap-lost)",
+          lambda: L.call('vdvc_cell_masks', c, n_el, n_el, n_el, h, act, mask0, 0.0, 0.5, ue, un,
+                         uv, ol),
+          3 * Nvox + 14 * Ne + Nn)
+    sc = torch.empty(Ne, dtype=torch.uint8, device='cuda')
+    timed("compute_safe_cell_mask", lambda: L.call('vdvc_safe_cells', c, n_el, n_el, n_el, h,
+                                                   uv, sc), Nvox + Ne)
+    ps = torch.empty(3 * Ne, dtype=torch.float64, device='cuda')
+    eq = torch.empty(Ne, dtype=torch.float64, device='cuda')
+    timed("principal + equivalent strains (float32 U)",
+          lambda: L.call('vdvc_principal_strains', c, n_el, n_el, n_el, h, U, L.F32, ps, eq),
+          12 * Nn + 32 * Ne)
+    conv = torch.empty(Ne, dtype=torch.float64, device='cuda')
+    timed("compute_gn_convergence_cell (float32 dU, U)",
+          lambda: L.call('vdvc_gn_convergence', c, n_el, n_el, n_el, U, L.F32, U, L.F32, conv),
+          24 * Nn + 8 * Ne)
+    crop = (L._d * 3)(0.0, 0.0, 0.0)
+    old_ = torch.empty(Ne, dtype=torch.uint8, device='cuda')
+    timed("remap_element_mask_to_deformed (exhaustive nearest, %d^2 pairs)" % Ne,
+          lambda: L.call('vdvc_remap_mask_deformed', c, n_el, n_el, n_el, h, U, L.F32, crop, ol,
+                         old_),
+          12 * Nn + 50 * Ne)
     z = (L._d * 1)()
     L.call('vdvc_zncc', c, Nvox, f8, L.U8, res, L.F32, 1, act, z)
     torch.cuda.synchronize()
@@ -450,8 +478,9 @@ def post_stages(n=513, h=8, reps=10):
         "ms": round(ms, 4), "algorithmic_bytes": 6 * Nvox, "achieved_GBs": round(gbs, 1),
         "frac": round(gbs / HBM_PEAK_GBS, 4)}
     total = sum(v.get("kernel_ms", v.get("ms", 0.0)) for v in out.values())
-    return {"workload": "post-solve voxel stages, %d^3 voxels, h=%d (%d^3 elements), uint8 "
-                        "speckle, Fourier U (16 modes, 4 vox), float32 residual" % (Nv, h, n_el),
+    return {"workload": "post-solve voxel and element stages, %d^3 voxels, h=%d (%d^3 "
+                        "elements), uint8 speckle, Fourier U (16 modes, 4 vox), float32 "
+                        "residual" % (Nv, h, n_el),
             "stages": out, "total_ms": round(total, 3)}
 
 

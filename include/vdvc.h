@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 3
+#define VDVC_ABI_VERSION 4
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -313,6 +313,63 @@ int vdvc_zncc(vdvc_ctx* ctx, int64_t n, const void* d_f, int ftype, const void* 
  * Replaces compute_correlation_quality_cell (engine/write_output.py:380-447). */
 int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_res, int rtype,
                       const void* d_ref, int ftype, double* d_std, double* d_zncc);
+
+/* ---- element-scale write-output fields (dense voxel arrays, elements
+ * i + Nx*(j + Ny*k), nodes i + (Nx+1)*(j + (Ny+1)*k); uint8 masks 0 / 1) ----
+ *
+ * Unsafe and overlap-lost masks from the per-voxel active mask d_active and
+ * material mask d_mask0 (ref > glt).  Outputs are optional (NULL: skipped):
+ *   d_unsafe_elem (Ne,): shares a node with an element holding an inactive
+ *     voxel, or lies on the outer element shell -- compute_unsafe_element_mask
+ *     (engine/kernels_dvc.py:196-213, via _unsafe_elements :118-163);
+ *   d_unsafe_node (Nn,): belongs to an element holding an inactive voxel or
+ *     to a shell element -- compute_unsafe_node_mask (:216-233);
+ *   d_unsafe_vox (Nvox,): voxel of an unsafe element --
+ *     compute_unsafe_voxel_mask (:166-193);
+ *   d_overlap_lost (Ne,): shares a node with an element whose material lost
+ *     overlap (lost / material > frac_threshold), or has material fraction
+ *     below min_material_frac, or lies on the shell -- needs d_mask0;
+ *     compute_overlap_lost_element_mask (:236-340). */
+int vdvc_cell_masks(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const uint8_t* d_active,
+                    const uint8_t* d_mask0, double frac_threshold, double min_material_frac,
+                    uint8_t* d_unsafe_elem, uint8_t* d_unsafe_node, uint8_t* d_unsafe_vox,
+                    uint8_t* d_overlap_lost);
+
+/* d_safe (Ne,) = 1 where none of the element's (h+1)^3 voxels is set in
+ * d_unsafe_vox.  Replaces compute_safe_cell_mask
+ * (engine/write_output.py:352-377). */
+int vdvc_safe_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const uint8_t* d_unsafe_vox,
+                    uint8_t* d_safe);
+
+/* Principal small strains e1 >= e2 >= e3 at each element centre from the
+ * nodal displacement d_U (utype F32 / F64): Hex8 gradients at the centre,
+ * eps = (grad u + grad u^T) / 2, eigenvalues in float64; d_principal
+ * (Ne, 3) float64, d_equiv (Ne,) float64 von Mises equivalent strain
+ * (optional).  Replaces compute_principal_strains_cell and
+ * compute_equivalent_strain (engine/write_output.py:256-306). */
+int vdvc_principal_strains(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                           int utype, double* d_principal, double* d_equiv);
+
+/* Equivalent strain of n principal-strain triples (float64).  Replaces
+ * compute_equivalent_strain (engine/write_output.py:284-306). */
+int vdvc_equivalent_strain(vdvc_ctx* ctx, long long n, const double* d_principal,
+                           double* d_out);
+
+/* d_out (Ne,) float64 = ||dU|| / ||U|| over each element's 8 corner nodes'
+ * 24 DOFs, NaN where ||U|| = 0.  Replaces compute_gn_convergence_cell
+ * (engine/write_output.py:450-491). */
+int vdvc_gn_convergence(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const void* d_dU, int dtype,
+                        const void* d_U, int utype, double* d_out);
+
+/* Element mask d_mask (Ne,) carried to the deformed configuration: element
+ * centres (i + 0.5) h moved by the mean of their 8 corner displacements;
+ * d_out at each regular centre + h_crop[3] takes the value of the nearest
+ * moved centre (exhaustive search, float64 squared distances; ties: lowest
+ * element index).  Replaces remap_element_mask_to_deformed
+ * (engine/write_output.py:494-530, scipy griddata 'nearest'). */
+int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                             int utype, const double* h_crop, const uint8_t* d_mask,
+                             uint8_t* d_out);
 
 /* ---- preprocess affine stage (dense float32 volumes, x fastest) ---- */
 

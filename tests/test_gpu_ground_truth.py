@@ -1,10 +1,12 @@
 """The reference's known-ground-truth accuracy check, end to end on the GPU:
 a bundled ground-truth displacement -> deformed image (prepare) -> preprocess
 (affine pre-alignment, full-resolution affine refinement, warp, overlap crop,
-work-dir files) -> run (multiscale GN, h = 8, l0 = 4.5 h) -> recoverable-node
-RMS against the ground truth.  The reference reports 0.0175 voxels for the
-bundled GT_z+0.01 dataset at these settings and about 0.015 voxels for a
-synthetic +1 % uniaxial tension (docs/TUTORIAL.md:883-924)."""
+work-dir files) -> run (multiscale GN, h = 8, l0 = 4.5 h, with the output
+masks) -> recoverable- and valid-node RMS against the ground truth.  The
+reference reports 0.0175 (recoverable) and 0.0060 (valid) voxels for the
+bundled GT_z+0.01 dataset at these settings and about 0.015 voxels
+(recoverable) for a synthetic +1 % uniaxial tension
+(docs/TUTORIAL.md:883-924)."""
 import os
 
 import numpy as np
@@ -24,12 +26,12 @@ def test_ground_truth_pipeline(gpu, tmp_path, case):
     ref = d['ref']
     if case == 'GT_z+0.01':
         u = d['u_gt_vox'].astype(np.float64)
-        published = 0.0175
+        published, published_valid = 0.0175, 0.0060
     else:
         n = ref.shape[0]
         u = np.zeros(ref.shape + (3,))
         u[..., 0] = 0.01 * np.arange(n, dtype=np.float64)[:, None, None]
-        published = 0.015
+        published, published_valid = 0.015, None
     print('ground truth', case, flush=True)
     deformed = H.prepare(ref, u)
     gin = tmp_path / 'gt_input'
@@ -40,9 +42,47 @@ def test_ground_truth_pipeline(gpu, tmp_path, case):
     meta = preprocess.main([str(gin / 'ref.npy'), str(gin / 'deformed.npy'), '--h', '8',
                             '--output-dir', str(wd)])
     run_main([str(wd), '--l0-factor', '4.5'])
-    n_rec, rms, mx = H.recoverable_node_rms(str(wd), u, deformed.shape)
+    reg = H.node_error_regions(str(wd), u, deformed.shape)
+    n_rec, rms, mx = reg['recoverable']
+    n_val, rms_v, mx_v = reg['valid']
     print('  %s: crop %s, refine rms %s, recoverable nodes %d, RMS %.4f vox (reference: %.4f), '
-          'max %.4f' % (case, meta['crop_shape'], meta['affine_refine'], n_rec, rms, published,
-                        mx), flush=True)
+          'max %.4f; valid nodes %d, RMS %.4f vox (reference: %s), max %.4f'
+          % (case, meta['crop_shape'], meta['affine_refine'], n_rec, rms, published, mx, n_val,
+             rms_v, published_valid, mx_v), flush=True)
     assert n_rec > 50
     assert rms < 1.5 * published
+    assert 0 < n_val < n_rec
+    assert rms_v <= rms
+    if published_valid is not None:
+        assert rms_v < 1.5 * published_valid
+
+
+# published sweep points on GT_z+0.01 (docs/TUTORIAL.md:905-924): (h,
+# l0_factor) -> (recoverable RMS, valid RMS), None where not published
+PUBLISHED_SWEEP = {(4, 9.0): (0.0159, None), (8, 4.5): (0.0175, 0.0060),
+                   (8, 9.0): (None, 0.0102), (16, 2.0): (0.0163, None)}
+
+
+def test_ground_truth_sweep(gpu, tmp_path):
+    """The h / l0 sweep at the reference's published points: every point
+    within 1.5x of its published RMS, and the valid RMS at h = 8 rises from
+    l0_factor 4.5 to 9 as published (the U-shaped curve's right branch)."""
+    import gt_harness as H
+    d = golden('gt_zp001')
+    u = d['u_gt_vox'].astype(np.float64)
+    print('ground-truth sweep', flush=True)
+    rows = H.sweep(str(tmp_path), d['ref'], u, list(PUBLISHED_SWEEP))
+    got = {}
+    for r in rows:
+        key = (r['h'], r['l0_factor'])
+        (nr, rr, mr), (nv, rv, mv) = r['recoverable'], r['valid']
+        got[key] = (rr, rv)
+        print('  h=%2d l0f=%-4g rec %4d nodes RMS %.4f max %.4f | valid %4d nodes RMS %.4f '
+              'max %.4f | published %s' % (key[0], key[1], nr, rr, mr, nv, rv, mv,
+                                           PUBLISHED_SWEEP[key]), flush=True)
+        pr, pv = PUBLISHED_SWEEP[key]
+        if pr is not None:
+            assert rr < 1.5 * pr
+        if pv is not None:
+            assert rv < 1.5 * pv
+    assert got[(8, 9.0)][1] > got[(8, 4.5)][1]

@@ -1,14 +1,16 @@
 """The `run` stage end to end on a synthetic work directory (the reference's
 preprocess outputs: affine_prealign.json, ref_preprocessed.npy,
 def_preprocessed.npy, def_ref_aligned.npy; run_dvc.py:65-192): the solve
-recovers the known displacement, and the voxel-scale outputs equal the
-oracle's evaluation of the same stages on the saved U and residual."""
+recovers the known displacement, and the voxel- and element-scale outputs
+equal the oracle's evaluation of the same stages on the saved U, dU and
+residual."""
 import json
 import os
 
 import numpy as np
 import pytest
 
+from oracle import cells_oracle as CO
 from oracle import post_oracle as P
 
 pytestmark = pytest.mark.gpu
@@ -73,3 +75,30 @@ def test_run_stage_outputs(gpu, tmp_path):
     assert float(line.split(':')[-1]) == pytest.approx(want_zncc, abs=2e-6)
     assert want_zncc > 0.95
     assert 'DVC COMPLETE' in log
+    # element-scale outputs against the oracle on the same active mask
+    uv = np.load(os.path.join(wd, 'unsafe_voxel_mask.npy'))
+    assert uv.dtype == np.bool_ and uv.shape == (n, n, n)
+    np.testing.assert_array_equal(uv.reshape(-1, order='F'),
+                                  CO.unsafe_voxel_mask(act, h, N, N, N))
+    ol = np.load(os.path.join(wd, 'overlap_lost_element_mask.npy'))
+    assert ol.shape == (N, N, N)
+    np.testing.assert_array_equal(ol.reshape(-1, order='F'),
+                                  CO.overlap_lost_element_mask(act, ff > 0.0, h, N, N, N))
+    ps = np.load(os.path.join(wd, 'principal_strains_cell.npy'))
+    want_ps = CO.principal_strains_cell(U_grid, h)
+    assert ps.shape == (N, N, N, 3)
+    assert np.abs(ps - want_ps).max() <= 1e-13 * np.abs(want_ps).max()
+    # the imposed strain: e_zz = 0.004 in the interior
+    assert abs(np.median(ps[2:-2, 2:-2, 2:-2, 0]) - 0.004) < 5e-4
+    eq = np.load(os.path.join(wd, 'equivalent_strain_cell.npy'))
+    np.testing.assert_allclose(eq, CO.equivalent_strain(ps), rtol=2.3e-16, atol=0)
+    dU = np.load(os.path.join(wd, 'dU.npy'))
+    conv = np.load(os.path.join(wd, 'gn_convergence_cell.npy'))
+    np.testing.assert_allclose(conv, CO.gn_convergence_cell(dU, U, N, N, N), rtol=2.3e-16,
+                               atol=0)
+    od = np.load(os.path.join(wd, 'overlap_lost_element_mask_deformed.npy'))
+    np.testing.assert_array_equal(od, CO.remap_mask_nearest(ol, U_grid, h, (0, 0, 0)))
+    up = np.load(os.path.join(wd, 'unsafe_voxel_mask_forward_pushed.npy'))
+    assert up.dtype == np.bool_
+    np.testing.assert_array_equal(
+        up, P.push_field_forward(uv, U_local, N, N, N, h, order=0, mode='constant', cval=1))

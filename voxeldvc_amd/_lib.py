@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 3
+ABI_VERSION = 4
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -82,6 +82,12 @@ SIGNATURES = {
     'vdvc_affine_rms': [_vp, _i, _i, _i, _vp, _vp, _vp, _dp, _dp, _dp, _d, _dp],
     'vdvc_warp_affine': [_vp, _i, _i, _i, _vp, _i, _i, _i, _dp, _dp, _dp, _vp],
     'vdvc_overlap_bounds': [_vp, _i, _i, _i, _vp, _i, _vp, _d, _ip, _ip],
+    'vdvc_cell_masks': [_vp, _i, _i, _i, _i, _vp, _vp, _d, _d, _vp, _vp, _vp, _vp],
+    'vdvc_safe_cells': [_vp, _i, _i, _i, _i, _vp, _vp],
+    'vdvc_principal_strains': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
+    'vdvc_equivalent_strain': [_vp, _i64, _vp, _vp],
+    'vdvc_gn_convergence': [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp],
+    'vdvc_remap_mask_deformed': [_vp, _i, _i, _i, _i, _vp, _i, _dp, _vp, _vp],
 }
 
 _lib = None

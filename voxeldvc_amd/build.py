@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libvdvc.so')
 SOURCES = ['vdvc_voxel.hip', 'vdvc_hessian.hip', 'vdvc_node.hip', 'vdvc_prealign.hip',
-           'vdvc_seednet.hip', 'vdvc_post.hip', 'vdvc_affine.hip',
+           'vdvc_seednet.hip', 'vdvc_post.hip', 'vdvc_affine.hip', 'vdvc_cells.hip',
            'vdvc_capi.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 

@@ -1207,6 +1207,119 @@ int vdvc_quality_cell(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   return timed_event(ctx);
 }
 
+// ---- element-scale write-output fields ----
+
+static int cell_voxel_grid_ok(int Nx, int Ny, int Nz, int h) {
+  if (Ny * (long long)h + 1 > 65535 || Nz * (long long)h + 1 > 65535)
+    return fail(-1, "voxel grid y / z extent exceeds 65535");
+  return 0;
+}
+
+int vdvc_cell_masks(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const uint8_t* d_active,
+                    const uint8_t* d_mask0, double frac_threshold, double min_material_frac,
+                    uint8_t* d_unsafe_elem, uint8_t* d_unsafe_node, uint8_t* d_unsafe_vox,
+                    uint8_t* d_overlap_lost) {
+  if (!ctx || !d_active) return fail(-1, "NULL argument");
+  if (d_overlap_lost && !d_mask0) return fail(-1, "the overlap-lost mask needs mask0");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (d_unsafe_vox && (rc = cell_voxel_grid_ok(Nx, Ny, Nz, h))) return rc;
+  const MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  const size_t Ne = (size_t)Nx * Ny * Nz;
+  for (int k = 0; k < 3; ++k)
+    if ((rc = ctx->vtmp[k].ensure(Ne * sizeof(int)))) return rc;
+  uint8_t* ue = d_unsafe_elem;
+  if (!ue && d_unsafe_vox) {
+    if ((rc = ctx->vtmp[3].ensure(Ne))) return rc;
+    ue = ctx->vtmp[3].as<uint8_t>();
+  }
+  CellIn C;
+  C.Nx = Nx; C.Ny = Ny; C.Nz = Nz; C.h = h; C.Nvx = M.Nvx; C.Nvy = M.Nvy;
+  C.a = d_active; C.b = d_mask0;
+  C.c0 = ctx->vtmp[0].as<int>(); C.c1 = ctx->vtmp[1].as<int>(); C.c2 = ctx->vtmp[2].as<int>();
+  C.all_a0 = nullptr;
+  CellFlagsIn F;
+  F.Nx = Nx; F.Ny = Ny; F.Nz = Nz; F.h = h;
+  F.c0 = C.c0; F.c1 = C.c1; F.c2 = C.c2;
+  F.frac_threshold = frac_threshold; F.min_material_frac = min_material_frac;
+  F.unsafe_elem = ue; F.lost_elem = d_overlap_lost; F.unsafe_node = d_unsafe_node;
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_elem_count(C, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_elem_flags(F, ctx->stream));
+  if (d_unsafe_vox) VDVC_LAUNCH(vdvc::launch_unsafe_voxels(Nx, Ny, Nz, h, ue, d_unsafe_vox, ctx->stream));
+  return timed_event(ctx);
+}
+
+int vdvc_safe_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const uint8_t* d_unsafe_vox,
+                    uint8_t* d_safe) {
+  if (!ctx || !d_unsafe_vox || !d_safe) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  const MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  CellIn C;
+  C.Nx = Nx; C.Ny = Ny; C.Nz = Nz; C.h = h; C.Nvx = M.Nvx; C.Nvy = M.Nvy;
+  C.a = d_unsafe_vox; C.b = nullptr;
+  C.c0 = nullptr; C.c1 = nullptr; C.c2 = nullptr; C.all_a0 = d_safe;
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_elem_count(C, ctx->stream));
+  return timed_event(ctx);
+}
+
+static const double kEquivStrain = std::sqrt(2.0) / 3.0;
+
+int vdvc_principal_strains(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                           int utype, double* d_principal, double* d_equiv) {
+  if (!ctx || !d_U || !d_principal) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(utype)) return fail(-1, "U type must be VDVC_F32 or VDVC_F64");
+  StrainIn S;
+  S.Nx = Nx; S.Ny = Ny; S.Nz = Nz; S.U = d_U; S.ucode = utype;
+  S.g = 0.25 / (double)h; S.k_eq = kEquivStrain;
+  S.principal = d_principal; S.equiv = d_equiv;
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_strains(S, ctx->stream));
+  return timed_event(ctx);
+}
+
+int vdvc_equivalent_strain(vdvc_ctx* ctx, long long n, const double* d_principal,
+                           double* d_out) {
+  if (!ctx || !d_principal || !d_out) return fail(-1, "NULL argument");
+  if (n < 0 || n > 0x7fffffffLL) return fail(-1, "invalid count %lld", n);
+  if (n == 0) return 0;
+  VDVC_LAUNCH(vdvc::launch_equiv_strain(n, d_principal, kEquivStrain, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_gn_convergence(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const void* d_dU, int dtype,
+                        const void* d_U, int utype, double* d_out) {
+  if (!ctx || !d_dU || !d_U || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, 1);
+  if (rc) return rc;
+  if (!rtype_ok(dtype) || !rtype_ok(utype)) return fail(-1, "dU / U types must be VDVC_F32 or VDVC_F64");
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_gn_convergence(Nx, Ny, Nz, d_dU, dtype, d_U, utype, d_out, ctx->stream));
+  return timed_event(ctx);
+}
+
+int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                             int utype, const double* h_crop, const uint8_t* d_mask,
+                             uint8_t* d_out) {
+  if (!ctx || !d_U || !h_crop || !d_mask || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(utype)) return fail(-1, "U type must be VDVC_F32 or VDVC_F64");
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if ((rc = ctx->vtmp[0].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->vtmp[1].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
+  double* pts = ctx->vtmp[0].as<double>();
+  double* qry = ctx->vtmp[1].as<double>();
+  if ((rc = timed_event(ctx))) return rc;
+  VDVC_LAUNCH(vdvc::launch_elem_centres(Nx, Ny, Nz, h, d_U, utype, h_crop, pts, qry, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_nearest_remap(Ne, pts, qry, d_mask, d_out, ctx->stream));
+  return timed_event(ctx);
+}
+
 }  // extern "C"
 
 // ---- preprocess affine stage ----

@@ -97,6 +97,45 @@ struct WarpIn {
   float A[9], t[3], c[3];
 };
 
+// Element-scale write-output fields.  Per element (x fastest) over its
+// (h+1)^3 voxels: c0 = #(a == 0), c1 = #(b != 0), c2 = #(b != 0 && a == 0)
+// (b NULL counts as all non-zero); all_a0 (optional) = (c0 == (h+1)^3).
+struct CellIn {
+  int Nx, Ny, Nz, h, Nvx, Nvy;
+  const uint8_t* a;
+  const uint8_t* b;
+  int* c0;
+  int* c1;
+  int* c2;
+  uint8_t* all_a0;
+};
+
+// per-element flags from the counts: unsafe = boundary shell | an element
+// within +-1 (sharing a node) has c0 > 0; lost = an element within +-1 has
+// material (c1 > 0) and c2 / c1 > frac_threshold, | c1 / (h+1)^3 <
+// min_material_frac | boundary shell; node = an adjacent element has
+// c0 > 0 or is on the shell.  Outputs optional.
+struct CellFlagsIn {
+  int Nx, Ny, Nz, h;
+  const int* c0;
+  const int* c1;
+  const int* c2;
+  double frac_threshold, min_material_frac;
+  uint8_t* unsafe_elem;
+  uint8_t* lost_elem;
+  uint8_t* unsafe_node;
+};
+
+// principal strains (descending) and equivalent strain at element centres
+struct StrainIn {
+  int Nx, Ny, Nz;
+  const void* U; int ucode;  // flat nodal vector, node-major (F32 / F64)
+  double g;                  // 0.25 / h: |dN_a / dx_d| at the centre
+  double k_eq;               // sqrt(2) / 3
+  double* principal;         // (Ne, 3)
+  double* equiv;             // (Ne,) optional
+};
+
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over
 // the elements adjacent to node (i,j,k).  blockdiag=1 means K[3a+c][3b+d] =
 // L8[a][b] delta_cd with L8 = K[3a][3b] (vector Laplacian), cheaper path.
@@ -235,6 +274,19 @@ int launch_affine_rms(const AffIn& A, double* part, int nblocks, hipStream_t st)
 int launch_warp_affine(const WarpIn& W, const float* g, float* out, hipStream_t st);
 int launch_overlap_proj(int nx, int ny, int nz, const void* f, int ftype, const float* g,
                         double glt, uint8_t* flags, hipStream_t st);
+int launch_elem_count(const CellIn& C, hipStream_t st);
+int launch_elem_flags(const CellFlagsIn& F, hipStream_t st);
+int launch_unsafe_voxels(int Nx, int Ny, int Nz, int h, const uint8_t* unsafe_elem,
+                         uint8_t* out, hipStream_t st);
+int launch_strains(const StrainIn& S, hipStream_t st);
+int launch_equiv_strain(long long n, const double* principal, double k_eq, double* out,
+                        hipStream_t st);
+int launch_gn_convergence(int Nx, int Ny, int Nz, const void* dU, int dcode, const void* U,
+                          int ucode, double* out, hipStream_t st);
+int launch_elem_centres(int Nx, int Ny, int Nz, int h, const void* U, int ucode,
+                        const double* crop, double* pts, double* qry, hipStream_t st);
+int launch_nearest_remap(long long n, const double* pts, const double* qry, const uint8_t* vals,
+                         uint8_t* out, hipStream_t st);
 int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
                       double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

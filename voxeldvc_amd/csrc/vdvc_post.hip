@@ -33,6 +33,7 @@
 #include "vdvc_common.h"
 #include "vdvc_kernels.h"
 #include "vdvc_sample01.h"
+#include "vdvc_elem.h"
 
 namespace {
 
@@ -185,28 +186,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
   }
 }
 
-// Visit the (h+1)^3 voxels of an element with one wavefront: lane -> (row,
-// qx) once, rows (qy, qz) of h+1 voxels stepping by 64 / (h+1) rows; for
-// h + 1 > 64 each lane walks voxels directly.
 template <typename F>
 __device__ __forceinline__ void for_elem_voxels(const QualIn& Q, long long base, int lane, F f) {
-  const int h1 = Q.h + 1;
-  const int nrows = h1 * h1;
-  if (h1 <= 64) {
-    const int rpi = 64 / h1;
-    const int lr = lane / h1, qx = lane - lr * h1;
-    if (lr >= rpi) return;
-    for (int r = lr; r < nrows; r += rpi) {
-      const int qz = r / h1, qy = r - qz * h1;
-      f(base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz));
-    }
-  } else {
-    const int nq = nrows * h1;
-    for (int q = lane; q < nq; q += 64) {
-      const int qx = q % h1, qy = (q / h1) % h1, qz = q / nrows;
-      f(base + qx + (long long)Q.Nvx * (qy + (long long)Q.Nvy * qz));
-    }
-  }
+  vdvc_elem::walk(Q.h, Q.Nvx, Q.Nvy, base, lane, f);
 }
 
 // One wavefront per element: two sweeps over its (h+1)^3 voxels (the second

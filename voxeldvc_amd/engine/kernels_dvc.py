@@ -290,3 +290,75 @@ def pcg(matvec, b, M_inv, xp=np, x0=None, max_iter=1000, tol=1e-8, verbose=False
         axpby(1.0, z, beta, p)
         rz = rz_new
     return out(x)
+
+
+# ---------------------------------------------------------------------------
+# unsafe / overlap-lost masks of the output stage (kernels_dvc.py:92-340)
+# ---------------------------------------------------------------------------
+
+def boundary_element_mask(Nx, Ny, Nz):
+    """(Nx*Ny*Nz,) bool, elements x fastest: True on the outer one-element
+    shell (kernels_dvc.py:92-115)."""
+    m = np.zeros((Nz, Ny, Nx), dtype=bool)
+    m[[0, -1], :, :] = True
+    m[:, [0, -1], :] = True
+    m[:, :, [0, -1]] = True
+    return m.ravel()
+
+
+def _cell_masks(active_vox, h, Nx, Ny, Nz, mask0_vox=None, frac_threshold=0.0,
+                min_material_frac=0.5, elem=False, node=False, vox=False, lost=False):
+    """One vdvc_cell_masks call; returns the requested outputs (device
+    uint8 tensors) in the order element, node, voxel, overlap-lost."""
+    t = L.require_gpu()
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    a = L.to_dev_flat(active_vox, 'uint8')
+    if a.numel() != Nvox:
+        raise ValueError('active_vox has %d entries, expected Nvox = %d' % (a.numel(), Nvox))
+    m0 = None
+    if lost:
+        m0 = L.to_dev_flat(mask0_vox, 'uint8')
+        if m0.numel() != Nvox:
+            raise ValueError('mask0_vox has %d entries, expected Nvox = %d' % (m0.numel(), Nvox))
+    Ne, Nn = Nx * Ny * Nz, (Nx + 1) * (Ny + 1) * (Nz + 1)
+    mk = lambda on, n: t.empty(n, dtype=t.uint8, device='cuda') if on else None
+    oe, on_, ov, ol = mk(elem, Ne), mk(node, Nn), mk(vox, Nvox), mk(lost, Ne)
+    L.call('vdvc_cell_masks', L.ctx(), Nx, Ny, Nz, h, a, m0, float(frac_threshold),
+           float(min_material_frac), oe, on_, ov, ol)
+    return [x for x in (oe, on_, ov, ol) if x is not None]
+
+
+def _bool_out(d, like):
+    return d.bool() if L.is_tensor(like) else L.to_host(d).astype(bool)
+
+
+def compute_unsafe_voxel_mask(active_vox, h, Nx, Ny, Nz):
+    """(Nvox,) bool: every voxel of an element that shares a node with an
+    element holding an inactive voxel, plus the outer element shell's voxels
+    (kernels_dvc.py:166-193)."""
+    (v,) = _cell_masks(active_vox, h, Nx, Ny, Nz, vox=True)
+    return _bool_out(v, active_vox)
+
+
+def compute_unsafe_element_mask(active_vox, h, Nx, Ny, Nz):
+    """(Nx*Ny*Nz,) bool, elements x fastest (kernels_dvc.py:196-213)."""
+    (e,) = _cell_masks(active_vox, h, Nx, Ny, Nz, elem=True)
+    return _bool_out(e, active_vox)
+
+
+def compute_unsafe_node_mask(active_vox, h, Nx, Ny, Nz):
+    """((Nx+1)*(Ny+1)*(Nz+1),) bool: nodes of elements holding an inactive
+    voxel and of the shell elements (kernels_dvc.py:216-233)."""
+    (n,) = _cell_masks(active_vox, h, Nx, Ny, Nz, node=True)
+    return _bool_out(n, active_vox)
+
+
+def compute_overlap_lost_element_mask(active_vox, mask0_vox, h, Nx, Ny, Nz,
+                                      frac_threshold=0.0, min_material_frac=0.5):
+    """(Nx*Ny*Nz,) bool, elements x fastest: shares a node with an element
+    whose material lost overlap (lost / material > frac_threshold), or has
+    a material count fraction below min_material_frac, or lies on the outer
+    shell (kernels_dvc.py:236-340)."""
+    (o,) = _cell_masks(active_vox, h, Nx, Ny, Nz, mask0_vox, frac_threshold, min_material_frac,
+                       lost=True)
+    return _bool_out(o, active_vox)

@@ -6,10 +6,16 @@ names (arguments, return values and layouts), computed by libvdvc.so.
   compute_zncc                                  write_output.py:658-690
   compute_correlation_quality_cell              write_output.py:380-447
   mesh_dims                                     write_output.py:228-230
+  compute_principal_strains_cell                write_output.py:256-281
+  compute_equivalent_strain                     write_output.py:284-306
+  compute_safe_cell_mask                        write_output.py:352-377
+  compute_gn_convergence_cell                   write_output.py:450-491
+  remap_element_mask_to_deformed                write_output.py:494-530
 
-The element-scale stages of write_outputs (principal strains, griddata
-remaps, unsafe / overlap-lost element masks, .vti export) are not part of
-this module.
+(the unsafe / overlap-lost masks are in engine/kernels_dvc.py, as in the
+reference).  Not provided: remap_cell_field_to_deformed (scipy griddata
+'linear' over a Qhull triangulation, write_output.py:533-568), the
+material-cell masks used only by the viewers, and the .vti export.
 
 Images are read in their own dtype (float32, float64, uint8, uint16, int16,
 bool), as scipy.ndimage.map_coordinates reads them; pushes return the
@@ -136,3 +142,138 @@ def compute_correlation_quality_cell(res, ref, h, Nx, Ny, Nz):
         return _grid_view(std, (Nx, Ny, Nz)), _grid_view(zn, (Nx, Ny, Nz))
     return (L.to_host(std).reshape(Nx, Ny, Nz, order='F'),
             L.to_host(zn).reshape(Nx, Ny, Nz, order='F'))
+
+
+# ---------------------------------------------------------------------------
+# element-scale fields (write_output.py:252-306, 352-377, 450-568)
+# ---------------------------------------------------------------------------
+
+def _nodal_from_grid(U_grid):
+    """(Nx+1, Ny+1, Nz+1, 3) nodal grid -> flat node-major device vector
+    (DOF 3*(i + (Nx+1)*(j + (Ny+1)*k)) + c) and its type code."""
+    t = L.require_gpu()
+    if L.is_tensor(U_grid):
+        g = U_grid.detach().to('cuda')
+        if g.dtype not in (t.float32, t.float64):
+            g = g.to(t.float64)
+        flat = g.permute(2, 1, 0, 3).contiguous().view(-1)
+    else:
+        g = np.asarray(U_grid)
+        if g.dtype not in (np.float32, np.float64):
+            g = g.astype(np.float64)
+        flat = t.from_numpy(np.ascontiguousarray(g.transpose(2, 1, 0, 3)).reshape(-1)).to('cuda')
+    return flat, L.rtype_of(flat)
+
+
+def _grid_dims(U_grid):
+    s = tuple(int(x) for x in U_grid.shape)
+    if len(s) != 4 or s[3] != 3 or min(s[:3]) < 2:
+        raise ValueError('U_grid must have shape (Nx+1, Ny+1, Nz+1, 3), got %r' % (s,))
+    return s[0] - 1, s[1] - 1, s[2] - 1
+
+
+def _cell_out(d, like, Nx, Ny, Nz, tail=()):
+    """Element-ordered (x fastest) device array -> (Nx, Ny, Nz, *tail)."""
+    if L.is_tensor(like):
+        return d.view(Nz, Ny, Nx, *tail).permute(2, 1, 0, *range(3, 3 + len(tail)))
+    a = L.to_host(d).reshape((Nz, Ny, Nx) + tuple(tail))
+    return np.ascontiguousarray(a.transpose((2, 1, 0) + tuple(range(3, 3 + len(tail)))))
+
+
+def compute_principal_strains_cell(U_grid, h):
+    """(Nx, Ny, Nz, 3) float64 principal small strains e1 >= e2 >= e3 at the
+    element centres: Hex8 gradients at the centre, eps = (grad u + grad
+    u^T) / 2, eigenvalues in float64 (write_output.py:256-281)."""
+    t = L.require_gpu()
+    Nx, Ny, Nz = _grid_dims(U_grid)
+    u, code = _nodal_from_grid(U_grid)
+    p = t.empty(Nx * Ny * Nz * 3, dtype=t.float64, device='cuda')
+    L.call('vdvc_principal_strains', L.ctx(), Nx, Ny, Nz, int(h), u, code, p, None)
+    return _cell_out(p, U_grid, Nx, Ny, Nz, (3,))
+
+
+def compute_equivalent_strain(principal_strains):
+    """sqrt(2)/3 * sqrt((e1-e2)^2 + (e2-e3)^2 + (e3-e1)^2) over the last
+    axis (write_output.py:284-306)."""
+    t = L.require_gpu()
+    shp = tuple(principal_strains.shape)
+    if not shp or shp[-1] != 3:
+        raise ValueError('principal_strains must have a last axis of 3')
+    if L.is_tensor(principal_strains):
+        p = principal_strains.detach().to(device='cuda', dtype=t.float64).contiguous().view(-1)
+    else:
+        p = t.from_numpy(np.ascontiguousarray(principal_strains, dtype=np.float64).reshape(-1)).to('cuda')
+    n = p.numel() // 3
+    out = t.empty(n, dtype=t.float64, device='cuda')
+    L.call('vdvc_equivalent_strain', L.ctx(), n, p, out)
+    out = out.view(shp[:-1])
+    return out if L.is_tensor(principal_strains) else L.to_host(out)
+
+
+def compute_safe_cell_mask(unsafe_vox, h, Nx, Ny, Nz):
+    """(Nx, Ny, Nz) bool: True where none of the element's (h+1)^3 voxels
+    is unsafe (write_output.py:352-377)."""
+    t = L.require_gpu()
+    u = L.to_dev_flat(_flat_f(unsafe_vox), 'uint8')
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    if u.numel() != Nvox:
+        raise ValueError('unsafe_vox has %d entries, expected Nvox = %d' % (u.numel(), Nvox))
+    s = t.empty(Nx * Ny * Nz, dtype=t.uint8, device='cuda')
+    L.call('vdvc_safe_cells', L.ctx(), Nx, Ny, Nz, int(h), u, s)
+    return _cell_out(s, unsafe_vox, Nx, Ny, Nz).astype(bool) if not L.is_tensor(unsafe_vox) \
+        else _cell_out(s, unsafe_vox, Nx, Ny, Nz).bool()
+
+
+def compute_gn_convergence_cell(dU, U, Nx, Ny, Nz):
+    """(Nx, Ny, Nz) float64 ||dU|| / ||U|| over each element's 24 corner
+    DOFs, NaN where ||U|| = 0 (write_output.py:450-491)."""
+    t = L.require_gpu()
+    Ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+
+    def dev(a):
+        if L.is_tensor(a):
+            d = a.detach().to('cuda').reshape(-1)
+            d = d if d.dtype in (t.float32, t.float64) else d.to(t.float64)
+        else:
+            arr = np.asarray(a).reshape(-1)
+            arr = arr if arr.dtype in (np.float32, np.float64) else arr.astype(np.float64)
+            d = t.from_numpy(np.ascontiguousarray(arr)).to('cuda')
+        if d.numel() != Ndof:
+            raise ValueError('expected %d DOFs, got %d' % (Ndof, d.numel()))
+        return d.contiguous()
+    dd, uu = dev(dU), dev(U)
+    out = t.empty(Nx * Ny * Nz, dtype=t.float64, device='cuda')
+    L.call('vdvc_gn_convergence', L.ctx(), Nx, Ny, Nz, dd, L.rtype_of(dd), uu, L.rtype_of(uu), out)
+    return _cell_out(out, U, Nx, Ny, Nz)
+
+
+def remap_element_mask_to_deformed(elem_mask, U_grid, h, crop_lo=(0, 0, 0)):
+    """(Nx, Ny, Nz) bool element mask carried to the deformed configuration:
+    centres (i + 0.5) h moved by the mean of their 8 corner displacements,
+    and each regular centre + crop_lo takes the value of the nearest moved
+    centre (exhaustive float64 search; ties: lowest element index, x
+    fastest) (write_output.py:494-530, scipy griddata 'nearest')."""
+    t = L.require_gpu()
+    Nx, Ny, Nz = _grid_dims(U_grid)
+    if tuple(int(x) for x in elem_mask.shape) != (Nx, Ny, Nz):
+        raise ValueError('elem_mask shape %r does not match U_grid' % (tuple(elem_mask.shape),))
+    u, code = _nodal_from_grid(U_grid)
+    if L.is_tensor(elem_mask):
+        m = elem_mask.detach().to('cuda').permute(2, 1, 0).reshape(-1).to(t.uint8).contiguous()
+    else:
+        m = t.from_numpy(np.ascontiguousarray(np.asarray(elem_mask, dtype=bool).transpose(2, 1, 0))
+                         .reshape(-1).astype(np.uint8)).to('cuda')
+    out = t.empty(Nx * Ny * Nz, dtype=t.uint8, device='cuda')
+    crop = (L._d * 3)(*[float(c) for c in crop_lo])
+    L.call('vdvc_remap_mask_deformed', L.ctx(), Nx, Ny, Nz, int(h), u, code, crop, m, out)
+    r = _cell_out(out, elem_mask, Nx, Ny, Nz)
+    return r.bool() if L.is_tensor(elem_mask) else r.astype(bool)
+
+
+def remap_cell_field_to_deformed(field, U_grid, h, crop_lo=(0, 0, 0)):
+    """Not provided: the reference interpolates with scipy griddata
+    'linear' (write_output.py:533-568), a Qhull Delaunay triangulation
+    whose simplices are not unique for the near-lattice element centres, so
+    its values cannot be reproduced by another implementation."""
+    raise NotImplementedError('remap_cell_field_to_deformed (griddata linear over a Qhull '
+                              'triangulation) is not provided')

@@ -16,8 +16,13 @@ write_outputs (engine/write_output.py:771-789, 838-849, 883-894), computed on
 the GPU: the active mask (compute_active_mask with g_origin = def_lo and
 mask0 = ref > glt) and the global ZNCC over it (printed),
 ref_forward_pushed.npy (ref pushed by U_local), residual_std_cell.npy and
-zncc_cell.npy.  The element-scale fields (strains, unsafe / overlap-lost
-masks, griddata remaps, .vti) are not produced.
+zncc_cell.npy; and the element-scale fields (engine/write_output.py:
+790-823, 846-906): unsafe_voxel_mask.npy, overlap_lost_element_mask.npy and
+its deformed-configuration remap, principal_strains_cell.npy,
+equivalent_strain_cell.npy, gn_convergence_cell.npy,
+unsafe_voxel_mask_forward_pushed.npy.  Not produced: the two
+deformed-configuration strain remaps (scipy griddata 'linear' over a Qhull
+triangulation) and the .vti export.
 
     python -m voxeldvc_amd run WORK_DIR [--l0-factor F]
 """
@@ -63,33 +68,98 @@ def parse_args(argv=None):
     return p.parse_args(argv)
 
 
-def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt):
-    """The voxel-scale stages of the reference's write_outputs on the GPU
-    (engine/write_output.py:771-789, 838-849, 883-894): active mask and
-    global ZNCC, the forward-pushed reference, per-cell residual std and
-    ZNCC."""
+def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt, dU=None,
+                 crop_lo=(0, 0, 0), deformed_cropped=None):
+    """The derived fields of the reference's write_outputs on the GPU
+    (engine/write_output.py:693-929): active mask and global ZNCC, unsafe
+    voxel mask and safe-cell count, overlap-lost element mask, principal and
+    equivalent strains, per-cell residual std / ZNCC / GN convergence, the
+    overlap-lost mask carried to the deformed configuration, the forward
+    pushes of ref and of the unsafe mask.  Not produced: the
+    deformed-configuration strain remaps (griddata 'linear') and the .vti."""
     from .engine.correlate_gpu import compute_active_mask
+    from .engine import kernels_dvc as K
     from .engine import write_output as W
+    Nvx, Nvy, Nvz = ref.shape
+    nbx, nby, nbz = Nx + 1, Ny + 1, Nz + 1
+    grid = np.stack([U[c::3].reshape(nbx, nby, nbz, order='F') for c in range(3)], axis=-1)
+    for c, name in enumerate(('ux', 'uy', 'uz')):
+        comp = grid[..., c]
+        print('  %s: mean=%.4f  std=%.4f  min=%.4f  max=%.4f'
+              % (name, comp.mean(), comp.std(), comp.min(), comp.max()))
     ref_flat = ref.reshape(-1, order='F')
     n_active = int(np.count_nonzero(res))
     print('Nonzero (active) residual voxels: %d/%d (%.1f%%)'
           % (n_active, res.size, 100 * n_active / res.size))
+    if n_active:
+        print('residual std (active voxels): %.4f' % res[res != 0].std())
+    mask0 = ref_flat > glt
     active = compute_active_mask(U, deformed, Nx, Ny, Nz, h, np,
                                  g_origin=tuple(int(x) for x in lo_def),
-                                 mask0_vox=ref_flat > glt, glt=glt)
+                                 mask0_vox=mask0, glt=glt)
     zncc = W.zncc_of_residual(ref_flat, res, active)
     print('ZNCC (active voxels, no-ground-truth accuracy proxy): %.6f' % zncc)
-    pushed = W.push_reference_forward(ref, U_local, Nx, Ny, Nz, h)
-    np.save(os.path.join(wd, 'ref_forward_pushed.npy'), pushed)
+    unsafe = K.compute_unsafe_voxel_mask(active, h, Nx, Ny, Nz)
+    unsafe_grid = unsafe.reshape(Nvx, Nvy, Nvz, order='F')
+    n_unsafe = int(np.count_nonzero(unsafe))
+    print('Unsafe voxels (affected by a node connected to an inactive voxel): %d/%d (%.1f%%)'
+          % (n_unsafe, unsafe.size, 100 * n_unsafe / unsafe.size))
+    safe = W.compute_safe_cell_mask(unsafe, h, Nx, Ny, Nz)
+    print('Safe elements (no unsafe voxel): %d/%d (%.1f%%)'
+          % (int(safe.sum()), safe.size, 100 * safe.sum() / safe.size))
+    lost = K.compute_overlap_lost_element_mask(active, mask0, h, Nx, Ny, Nz)
+    lost_grid = lost.reshape(Nx, Ny, Nz, order='F')
+    print('Overlap-lost elements (lost material overlap or <50%% material): %d/%d (%.1f%%)'
+          % (int(lost.sum()), lost.size, 100 * lost.sum() / lost.size))
+    if deformed_cropped is not None:
+        print('\ndeformed image: mean=%.2f  min=%.2f  max=%.2f'
+              % (deformed_cropped.mean(), deformed_cropped.min(), deformed_cropped.max()))
+    ps = W.compute_principal_strains_cell(grid, h)
+    for c, name in enumerate(('e1 (max)', 'e2 (mid)', 'e3 (min)')):
+        comp = ps[..., c]
+        print('  cell %s: mean=%.5f  std=%.5f  min=%.5f  max=%.5f'
+              % (name, comp.mean(), comp.std(), comp.min(), comp.max()))
+    eq = W.compute_equivalent_strain(ps)
+    print('  cell eq. strain: mean=%.5f  std=%.5f  min=%.5f  max=%.5f'
+          % (eq.mean(), eq.std(), eq.min(), eq.max()))
     std_c, zn_c = W.compute_correlation_quality_cell(res, ref, h, Nx, Ny, Nz)
-    np.save(os.path.join(wd, 'residual_std_cell.npy'), std_c)
-    np.save(os.path.join(wd, 'zncc_cell.npy'), zn_c)
     if np.any(~np.isnan(std_c)):
         print('  cell residual std: mean=%.4f  min=%.4f  max=%.4f'
               % (np.nanmean(std_c), np.nanmin(std_c), np.nanmax(std_c)))
     if np.any(~np.isnan(zn_c)):
         print('  cell ZNCC: mean=%.5f  min=%.5f  max=%.5f'
               % (np.nanmean(zn_c), np.nanmin(zn_c), np.nanmax(zn_c)))
+    conv = None
+    if dU is not None:
+        conv = W.compute_gn_convergence_cell(dU, U, Nx, Ny, Nz)
+        if np.any(~np.isnan(conv)):
+            print('  cell GN convergence dU/U: mean=%.2e  min=%.2e  max=%.2e'
+                  % (np.nanmean(conv), np.nanmin(conv), np.nanmax(conv)))
+    lost_def = W.remap_element_mask_to_deformed(lost_grid, grid, h, crop_lo=crop_lo)
+    print('Overlap-lost elements (deformed config): %d/%d (%.1f%%)'
+          % (int(lost_def.sum()), lost_def.size, 100 * lost_def.sum() / lost_def.size))
+    pushed = W.push_reference_forward(ref, U_local, Nx, Ny, Nz, h)
+    if deformed_cropped is not None:
+        diff = pushed.astype(np.float64) - deformed_cropped.astype(np.float64)
+        print('\nref_forward_pushed - deformed_cropped: mean=%.4f  std=%.4f  min=%.4f  max=%.4f'
+              % (diff.mean(), diff.std(), diff.min(), diff.max()))
+    unsafe_pushed = W.push_field_forward(unsafe_grid, U_local, Nx, Ny, Nz, h, order=0,
+                                         mode='constant', cval=1).astype(bool)
+    print('Unsafe voxels (forward-pushed to deformed grid): %d/%d (%.1f%%)'
+          % (int(unsafe_pushed.sum()), unsafe_pushed.size,
+             100 * unsafe_pushed.sum() / unsafe_pushed.size))
+    save = lambda name, a: np.save(os.path.join(wd, name + '.npy'), a)
+    save('principal_strains_cell', ps)
+    save('equivalent_strain_cell', eq)
+    save('residual_std_cell', std_c)
+    save('zncc_cell', zn_c)
+    if conv is not None:
+        save('gn_convergence_cell', conv)
+    save('unsafe_voxel_mask', unsafe_grid)
+    save('overlap_lost_element_mask', lost_grid)
+    save('overlap_lost_element_mask_deformed', lost_def)
+    save('ref_forward_pushed', pushed)
+    save('unsafe_voxel_mask_forward_pushed', unsafe_pushed)
     return active, zncc
 
 
@@ -153,7 +223,9 @@ def main(argv=None):
         np.save(os.path.join(wd, 'ref.npy'), ref_c)
         np.save(os.path.join(wd, 'deformed.npy'),
                 deformed_cropped[:shp[0], :shp[1], :shp[2]])
-        post_outputs(wd, U, U_local, res, ref_c, deformed, Nx, Ny, Nz, h, lo_def, glt)
+        post_outputs(wd, U, U_local, res, ref_c, deformed, Nx, Ny, Nz, h, lo_def, glt, dU=dU,
+                     crop_lo=tuple(int(x) for x in crop_lo),
+                     deformed_cropped=deformed_cropped[:shp[0], :shp[1], :shp[2]])
         print('\nDVC COMPLETE')
         t_end = time.perf_counter()
         print('Timing: setup %.2f s | multiscale correlation %.2f s | output %.2f s | total %.2f s'
