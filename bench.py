@@ -462,7 +462,7 @@ def post_stages(n=513, h=8, reps=10):
           24 * Nn + 8 * Ne)
     crop = (L._d * 3)(0.0, 0.0, 0.0)
     old_ = torch.empty(Ne, dtype=torch.uint8, device='cuda')
-    timed("remap_element_mask_to_deformed (exhaustive nearest, %d^2 pairs)" % Ne,
+    timed("remap_element_mask_to_deformed (%d elements, grid nearest, incl. one host sync)" % Ne,
           lambda: L.call('vdvc_remap_mask_deformed', c, n_el, n_el, n_el, h, U, L.F32, crop, ol,
                          old_),
           12 * Nn + 50 * Ne)

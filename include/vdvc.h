@@ -364,8 +364,8 @@ int vdvc_gn_convergence(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const void* d_dU,
 /* Element mask d_mask (Ne,) carried to the deformed configuration: element
  * centres (i + 0.5) h moved by the mean of their 8 corner displacements;
  * d_out at each regular centre + h_crop[3] takes the value of the nearest
- * moved centre (exhaustive search, float64 squared distances; ties: lowest
- * element index).  Replaces remap_element_mask_to_deformed
+ * moved centre (exact search over a uniform grid of cells, float64 squared
+ * distances; ties: lowest element index).  Replaces remap_element_mask_to_deformed
  * (engine/write_output.py:494-530, scipy griddata 'nearest'). */
 int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
                              int utype, const double* h_crop, const uint8_t* d_mask,

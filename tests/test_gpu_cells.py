@@ -181,3 +181,36 @@ def test_cells_full_size_properties(gpu):
     safe = W.compute_safe_cell_mask(uv.permute(2, 1, 0), h, Nx, Ny, Nz)   # (Nx, Ny, Nz)
     u = torch.nn.functional.max_pool3d(ue.float()[None, None], 3, stride=1, padding=1)[0, 0]
     assert torch.equal(safe.permute(2, 1, 0), u == 0)
+
+
+def test_remap_mask_spread_and_errors(gpu):
+    """A far outlier (the cell grid coarsens until it has at most ~4 cells
+    per point), queries well outside the moved cloud, and a non-finite
+    displacement (rejected)."""
+    from voxeldvc_amd.engine import write_output as W
+    Nx, Ny, Nz, h = 9, 7, 8, 4
+    rng = np.random.default_rng(43)
+    m = rng.random((Nx, Ny, Nz)) < 0.5
+    U = rng.normal(0, 0.5 * h, (Nx + 1, Ny + 1, Nz + 1, 3))
+    U[3, 2, 4] = (4000.0, -2500.0, 800.0)
+    for crop in ((0, 0, 0), (60, -45, 30)):
+        got = W.remap_element_mask_to_deformed(m, U, h, crop)
+        np.testing.assert_array_equal(got, CO.remap_mask_nearest_brute(m, U, h, crop))
+    U[1, 1, 1, 2] = np.nan
+    with pytest.raises(ValueError):
+        W.remap_element_mask_to_deformed(m, U, h)
+
+
+def test_remap_mask_lattice_ties(gpu):
+    """Whole-element shifts put queries exactly half way between moved
+    centres: ties go to the lowest element index (x fastest), as in the
+    exhaustive search."""
+    from voxeldvc_amd.engine import write_output as W
+    Nx, Ny, Nz, h = 6, 5, 4, 4
+    rng = np.random.default_rng(44)
+    m = rng.random((Nx, Ny, Nz)) < 0.5
+    U = np.zeros((Nx + 1, Ny + 1, Nz + 1, 3))
+    U[..., 0] = 0.5 * h          # every query midway between two x neighbours
+    U[..., 2] = -1.5 * h
+    got = W.remap_element_mask_to_deformed(m, U, h)
+    np.testing.assert_array_equal(got, CO.remap_mask_nearest_brute(m, U, h))

@@ -1309,14 +1309,54 @@ int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!rtype_ok(utype)) return fail(-1, "U type must be VDVC_F32 or VDVC_F64");
+  for (int c = 0; c < 3; ++c)
+    if (!std::isfinite(h_crop[c])) return fail(-1, "crop offset must be finite");
   const long long Ne = (long long)Nx * Ny * Nz;
+  const int nparts = 256;
   if ((rc = ctx->vtmp[0].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
   if ((rc = ctx->vtmp[1].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->stats.ensure((size_t)nparts * 6 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   double* pts = ctx->vtmp[0].as<double>();
   double* qry = ctx->vtmp[1].as<double>();
   if ((rc = timed_event(ctx))) return rc;
   VDVC_LAUNCH(vdvc::launch_elem_centres(Nx, Ny, Nz, h, d_U, utype, h_crop, pts, qry, ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_nearest_remap(Ne, pts, qry, d_mask, d_out, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_bbox(Ne, pts, ctx->stats.as<double>(), nparts, ctx->red.as<double>(),
+                                ctx->stream));
+  double bb[6];
+  if ((rc = fetch(ctx, ctx->red.p, 6 * sizeof(double), bb))) return rc;
+  for (int c = 0; c < 6; ++c)
+    if (!std::isfinite(bb[c])) return fail(-1, "displacement field is not finite");
+  // cells of edge h over the moved centres' box, coarsened while there are
+  // more than 4 cells per point
+  NNGrid g;
+  double cs = (double)h;
+  long long ncell = 0;
+  for (int it = 0; it < 64; ++it) {
+    ncell = 1;
+    for (int c = 0; c < 3; ++c) {
+      const double span = std::floor((bb[3 + c] - bb[c]) / cs) + 1.0;
+      g.G[c] = span > 1e9 ? 1000000000 : (int)span;
+      ncell *= g.G[c];
+      if (ncell > 0x3fffffffLL) break;
+    }
+    if (ncell <= 4 * Ne + 64) break;
+    cs *= 2.0;
+  }
+  if (ncell > 4 * Ne + 64) return fail(-1, "point spread too large for the remap grid");
+  for (int c = 0; c < 3; ++c) g.o[c] = bb[c];
+  g.cs = cs;
+  g.inv_cs = 1.0 / cs;
+  const long long nchunk = vdvc::nn_scan_chunks(ncell);
+  if ((rc = ctx->vtmp[3].ensure((size_t)(ncell + nchunk) * sizeof(int)))) return rc;
+  if ((rc = ctx->vtmp[2].ensure((size_t)(ncell + 1 + Ne) * sizeof(int)))) return rc;
+  int* cnt = ctx->vtmp[3].as<int>();
+  int* csum = cnt + ncell;
+  int* off = ctx->vtmp[2].as<int>();
+  int* list = off + ncell + 1;
+  VDVC_HIP(hipMemsetAsync(cnt, 0, (size_t)ncell * sizeof(int), ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_nn_build(Ne, pts, g, cnt, off, csum, list, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_nn_query(Ne, pts, qry, g, off, list, d_mask, d_out, ctx->stream));
   return timed_event(ctx);
 }
 

@@ -17,12 +17,12 @@
 //     strain (write_output.py:256-306);
 //   * gn_convergence_kernel: ||dU|| / ||U|| over the element's 24 corner
 //     DOFs in NumPy's summation order (write_output.py:450-491);
-//   * elem_centres_kernel + nearest_remap_kernel: the nearest-neighbour
-//     remap of an element mask to the deformed configuration
-//     (write_output.py:494-530): element centres moved by the mean corner
-//     displacement, and for every regular centre (+ crop offset) the value of
-//     the nearest moved centre, by exhaustive search over LDS tiles of
-//     points (ties: the lowest element index).
+//   * elem_centres_kernel + the nn_* kernels: the nearest-neighbour remap of
+//     an element mask to the deformed configuration (write_output.py:
+//     494-530): element centres moved by the mean corner displacement, and
+//     for every regular centre (+ crop offset) the value of the nearest
+//     moved centre, found exactly through a uniform grid of cells (ties:
+//     the lowest element index).
 // Elements x fastest (e = i + Nx*(j + Ny*k)), nodes n = i + (Nx+1)*(j +
 // (Ny+1)*k), voxels x fastest on the (Nx*h+1) x (Ny*h+1) x (Nz*h+1) grid.
 #include "vdvc_common.h"
@@ -293,41 +293,249 @@ __global__ __launch_bounds__(VDVC_BLOCK) void elem_centres_kernel(int Nx, int Ny
   }
 }
 
-// exhaustive nearest point per query: squared distance ((d0^2 + d1^2) +
-// d2^2), strict < so the lowest index wins ties
-__global__ __launch_bounds__(VDVC_BLOCK) void nearest_remap_kernel(long long n,
-                                                                   const double* __restrict__ pts,
-                                                                   const double* __restrict__ qry,
-                                                                   const uint8_t* __restrict__ vals,
-                                                                   uint8_t* __restrict__ out) {
-  __shared__ double tile[VDVC_BLOCK * 3];
-  const long long qi = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  const bool live = qi < n;
-  const double q0 = live ? qry[3 * qi] : 0.0, q1 = live ? qry[3 * qi + 1] : 0.0,
-               q2 = live ? qry[3 * qi + 2] : 0.0;
-  double best = __longlong_as_double(0x7ff0000000000000LL);  // +inf
-  long long bi = 0;
-  for (long long t0 = 0; t0 < n; t0 += VDVC_BLOCK) {
-    const long long pi = t0 + threadIdx.x;
-    if (pi < n) {
-      tile[3 * threadIdx.x] = pts[3 * pi];
-      tile[3 * threadIdx.x + 1] = pts[3 * pi + 1];
-      tile[3 * threadIdx.x + 2] = pts[3 * pi + 2];
+// Nearest moved centre per query on a uniform grid of cells (edge cs) over
+// the points' bounding box: counting sort of the point indices by cell
+// (integer atomics; the order inside a cell does not matter), then each
+// query walks Chebyshev rings of cells around its own cell.  After rings
+// 0..r every unvisited point lies at least r * cs away, so the walk stops
+// once the best squared distance is below (r * cs)^2 (with a 1e-9 relative
+// margin for the cell assignment's rounding).  Points and queries outside
+// the box are clamped into the border cells, which keeps that bound.  The
+// squared distance is ((d0^2 + d1^2) + d2^2) and ties go to the lowest
+// index, so the result equals an exhaustive search.
+__device__ __forceinline__ int nn_cell1(double p, double o, double inv_cs, int G) {
+  const double f = floor((p - o) * inv_cs);
+  return !(f >= 0.0) ? 0 : (f > (double)(G - 1) ? G - 1 : (int)f);
+}
+
+// min / max that propagate NaN (a non-finite point makes the box non-finite,
+// which the host rejects)
+__device__ __forceinline__ double nmin(double a, double b) { return (a != a || b != b) ? a + b : fmin(a, b); }
+__device__ __forceinline__ double nmax(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
+__global__ __launch_bounds__(VDVC_BLOCK) void bbox_part_kernel(long long n,
+                                                               const double* __restrict__ pts,
+                                                               double* __restrict__ part) {
+  __shared__ double lds[6][VDVC_BLOCK];
+  double v[6] = {__longlong_as_double(0x7ff0000000000000LL), __longlong_as_double(0x7ff0000000000000LL),
+                 __longlong_as_double(0x7ff0000000000000LL), -__longlong_as_double(0x7ff0000000000000LL),
+                 -__longlong_as_double(0x7ff0000000000000LL), -__longlong_as_double(0x7ff0000000000000LL)};
+  for (long long i = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; i < n;
+       i += (long long)gridDim.x * VDVC_BLOCK) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double x = pts[3 * i + c];
+      v[c] = nmin(v[c], x);
+      v[3 + c] = nmax(v[3 + c], x);
     }
-    __syncthreads();
-    const int m = (int)(n - t0 < VDVC_BLOCK ? n - t0 : VDVC_BLOCK);
-    for (int s = 0; s < m; ++s) {
-      const double d0 = sub_rn(q0, tile[3 * s]), d1 = sub_rn(q1, tile[3 * s + 1]),
-                   d2 = sub_rn(q2, tile[3 * s + 2]);
-      const double dd = add_rn(add_rn(mul_rn(d0, d0), mul_rn(d1, d1)), mul_rn(d2, d2));
-      if (dd < best) {
-        best = dd;
-        bi = t0 + s;
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) lds[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        lds[c][threadIdx.x] = nmin(lds[c][threadIdx.x], lds[c][threadIdx.x + o]);
+        lds[3 + c][threadIdx.x] = nmax(lds[3 + c][threadIdx.x], lds[3 + c][threadIdx.x + o]);
       }
     }
     __syncthreads();
   }
-  if (live) out[qi] = vals[bi];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) part[6 * blockIdx.x + c] = lds[c][0];
+  }
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void bbox_final_kernel(int nparts,
+                                                                const double* __restrict__ part,
+                                                                double* __restrict__ out) {
+  __shared__ double lds[6][VDVC_BLOCK];
+  const double inf = __longlong_as_double(0x7ff0000000000000LL);
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    lds[c][threadIdx.x] = threadIdx.x < nparts ? part[6 * threadIdx.x + c] : (c < 3 ? inf : -inf);
+  __syncthreads();
+  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        lds[c][threadIdx.x] = nmin(lds[c][threadIdx.x], lds[c][threadIdx.x + o]);
+        lds[3 + c][threadIdx.x] = nmax(lds[3 + c][threadIdx.x], lds[3 + c][threadIdx.x + o]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = lds[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void nn_count_kernel(long long n,
+                                                              const double* __restrict__ pts,
+                                                              NNGrid g, int* __restrict__ cnt) {
+  const long long i = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int cx = nn_cell1(pts[3 * i], g.o[0], g.inv_cs, g.G[0]);
+  const int cy = nn_cell1(pts[3 * i + 1], g.o[1], g.inv_cs, g.G[1]);
+  const int cz = nn_cell1(pts[3 * i + 2], g.o[2], g.inv_cs, g.G[2]);
+  atomicAdd(&cnt[cx + (long long)g.G[0] * (cy + (long long)g.G[1] * cz)], 1);
+}
+
+// Exclusive prefix sum of the cell counts in three passes: per-chunk sums
+// (chunks of kScanChunk cells, 8 consecutive cells per thread), a scan of the
+// chunk sums in one block (also writing off[ncell] = n), and per chunk the
+// local scan plus the chunk's offset.  off[c] = first slot of cell c; the
+// counts are replaced by the same offsets (the fill cursors).
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = VDVC_BLOCK * kScanItems;
+
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+  __shared__ int wsum[VDVC_WAVES_PER_BLOCK];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < VDVC_WAVES_PER_BLOCK; ++k) {
+    if (k < w) base += wsum[k];
+    total += wsum[k];
+  }
+  __syncthreads();
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void nn_chunk_sum_kernel(long long ncell,
+                                                                  const int* __restrict__ cnt,
+                                                                  int* __restrict__ csum) {
+  const long long b0 = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (b0 + k < ncell) s += cnt[b0 + k];
+  int total;
+  block_excl_scan(s, total);
+  if (threadIdx.x == 0) csum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void nn_chunk_scan_kernel(int nchunk, int* __restrict__ csum,
+                                                             long long ncell,
+                                                             int* __restrict__ off) {
+  __shared__ int lds[1024];
+  const int t = threadIdx.x;
+  const int per = (nchunk + 1023) / 1024;
+  const int b = t * per, e = b + per < nchunk ? b + per : nchunk;
+  int s = 0;
+  for (int i = b; i < e; ++i) s += csum[i];
+  lds[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int y = t >= o ? lds[t - o] : 0;
+    __syncthreads();
+    lds[t] += y;
+    __syncthreads();
+  }
+  int acc = lds[t] - s;
+  for (int i = b; i < e; ++i) {
+    const int c = csum[i];
+    csum[i] = acc;
+    acc += c;
+  }
+  if (t == 1023) off[ncell] = lds[1023];
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void nn_chunk_apply_kernel(long long ncell,
+                                                                    int* __restrict__ cnt,
+                                                                    const int* __restrict__ csum,
+                                                                    int* __restrict__ off) {
+  const long long b0 = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = b0 + k < ncell ? cnt[b0 + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int acc = csum[blockIdx.x] + block_excl_scan(s, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (b0 + k < ncell) {
+      off[b0 + k] = acc;
+      cnt[b0 + k] = acc;
+    }
+    acc += v[k];
+  }
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void nn_fill_kernel(long long n,
+                                                             const double* __restrict__ pts,
+                                                             NNGrid g, int* __restrict__ cur,
+                                                             int* __restrict__ list) {
+  const long long i = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int cx = nn_cell1(pts[3 * i], g.o[0], g.inv_cs, g.G[0]);
+  const int cy = nn_cell1(pts[3 * i + 1], g.o[1], g.inv_cs, g.G[1]);
+  const int cz = nn_cell1(pts[3 * i + 2], g.o[2], g.inv_cs, g.G[2]);
+  const int slot = atomicAdd(&cur[cx + (long long)g.G[0] * (cy + (long long)g.G[1] * cz)], 1);
+  list[slot] = (int)i;
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void nn_query_kernel(long long n,
+                                                              const double* __restrict__ pts,
+                                                              const double* __restrict__ qry,
+                                                              NNGrid g, const int* __restrict__ off,
+                                                              const int* __restrict__ list,
+                                                              const uint8_t* __restrict__ vals,
+                                                              uint8_t* __restrict__ out) {
+  const long long qi = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (qi >= n) return;
+  const double q0 = qry[3 * qi], q1 = qry[3 * qi + 1], q2 = qry[3 * qi + 2];
+  const int c0 = nn_cell1(q0, g.o[0], g.inv_cs, g.G[0]);
+  const int c1 = nn_cell1(q1, g.o[1], g.inv_cs, g.G[1]);
+  const int c2 = nn_cell1(q2, g.o[2], g.inv_cs, g.G[2]);
+  int rmax = c0 > g.G[0] - 1 - c0 ? c0 : g.G[0] - 1 - c0;
+  rmax = max(rmax, max(c1, g.G[1] - 1 - c1));
+  rmax = max(rmax, max(c2, g.G[2] - 1 - c2));
+  double best = __longlong_as_double(0x7ff0000000000000LL);
+  int bi = -1;
+  for (int r = 0; r <= rmax; ++r) {
+    for (int dz = -r; dz <= r; ++dz) {
+      const int z = c2 + dz;
+      if (z < 0 || z >= g.G[2]) continue;
+      for (int dy = -r; dy <= r; ++dy) {
+        const int y = c1 + dy;
+        if (y < 0 || y >= g.G[1]) continue;
+        const bool face = dz == -r || dz == r || dy == -r || dy == r;
+        const int step = (face || r == 0) ? 1 : 2 * r;
+        for (int dx = -r; dx <= r; dx += step) {
+          const int x = c0 + dx;
+          if (x < 0 || x >= g.G[0]) continue;
+          const long long c = x + (long long)g.G[0] * (y + (long long)g.G[1] * z);
+          for (int k = off[c]; k < off[c + 1]; ++k) {
+            const int i = list[k];
+            const double d0 = sub_rn(q0, pts[3 * (long long)i]);
+            const double d1 = sub_rn(q1, pts[3 * (long long)i + 1]);
+            const double d2 = sub_rn(q2, pts[3 * (long long)i + 2]);
+            const double dd = add_rn(add_rn(mul_rn(d0, d0), mul_rn(d1, d1)), mul_rn(d2, d2));
+            if (dd < best || (dd == best && i < bi)) {
+              best = dd;
+              bi = i;
+            }
+          }
+        }
+      }
+    }
+    const double lim = (double)r * g.cs * (1.0 - 1e-9);
+    if (bi >= 0 && best < lim * lim) break;
+  }
+  out[qi] = bi >= 0 ? vals[bi] : (uint8_t)0;
 }
 
 }  // namespace
@@ -399,11 +607,37 @@ int launch_elem_centres(int Nx, int Ny, int Nz, int h, const void* U, int ucode,
   return (int)hipGetLastError();
 }
 
-int launch_nearest_remap(long long n, const double* pts, const double* qry, const uint8_t* vals,
-                         uint8_t* out, hipStream_t st) {
+int launch_bbox(long long n, const double* pts, double* part, int nparts, double* out,
+                hipStream_t st) {
+  hipLaunchKernelGGL(bbox_part_kernel, dim3((unsigned)nparts), dim3(VDVC_BLOCK), 0, st, n, pts,
+                     part);
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(VDVC_BLOCK), 0, st, nparts, part, out);
+  return (int)hipGetLastError();
+}
+
+int nn_scan_chunks(long long ncell) { return (int)((ncell + kScanChunk - 1) / kScanChunk); }
+
+int launch_nn_build(long long n, const double* pts, const NNGrid& g, int* cnt, int* off,
+                    int* csum, int* list, hipStream_t st) {
+  const long long ncell = (long long)g.G[0] * g.G[1] * g.G[2];
   const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
-  hipLaunchKernelGGL(nearest_remap_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, pts, qry, vals,
-                     out);
+  const int nchunk = nn_scan_chunks(ncell);
+  hipLaunchKernelGGL(nn_count_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, pts, g, cnt);
+  hipLaunchKernelGGL(nn_chunk_sum_kernel, dim3((unsigned)nchunk), dim3(VDVC_BLOCK), 0, st, ncell,
+                     cnt, csum);
+  hipLaunchKernelGGL(nn_chunk_scan_kernel, dim3(1), dim3(1024), 0, st, nchunk, csum, ncell, off);
+  hipLaunchKernelGGL(nn_chunk_apply_kernel, dim3((unsigned)nchunk), dim3(VDVC_BLOCK), 0, st, ncell,
+                     cnt, csum, off);
+  hipLaunchKernelGGL(nn_fill_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, pts, g, cnt, list);
+  return (int)hipGetLastError();
+}
+
+int launch_nn_query(long long n, const double* pts, const double* qry, const NNGrid& g,
+                    const int* off, const int* list, const uint8_t* vals, uint8_t* out,
+                    hipStream_t st) {
+  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(nn_query_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, pts, qry, g, off,
+                     list, vals, out);
   return (int)hipGetLastError();
 }
 

@@ -136,6 +136,14 @@ struct StrainIn {
   double* equiv;             // (Ne,) optional
 };
 
+// uniform grid of cells for the nearest-neighbour remap: origin o, cell
+// edge cs (inv_cs = 1 / cs), G cells per axis
+struct NNGrid {
+  double o[3];
+  double cs, inv_cs;
+  int G[3];
+};
+
 // element stiffness operator: E_e * K (24x24, node-major DOFs) summed over
 // the elements adjacent to node (i,j,k).  blockdiag=1 means K[3a+c][3b+d] =
 // L8[a][b] delta_cd with L8 = K[3a][3b] (vector Laplacian), cheaper path.
@@ -285,8 +293,19 @@ int launch_gn_convergence(int Nx, int Ny, int Nz, const void* dU, int dcode, con
                           int ucode, double* out, hipStream_t st);
 int launch_elem_centres(int Nx, int Ny, int Nz, int h, const void* U, int ucode,
                         const double* crop, double* pts, double* qry, hipStream_t st);
-int launch_nearest_remap(long long n, const double* pts, const double* qry, const uint8_t* vals,
-                         uint8_t* out, hipStream_t st);
+// bounding box of n points (x, y, z): out = (min x, min y, min z, max x,
+// max y, max z); part holds 6 * nparts doubles
+int launch_bbox(long long n, const double* pts, double* part, int nparts, double* out,
+                hipStream_t st);
+// cnt (zeroed, ncell ints) -> cell offsets off (ncell + 1) and the point
+// list (n ints) sorted by cell; csum holds nn_scan_chunks(ncell) ints;
+// nparts of launch_bbox <= 256
+int nn_scan_chunks(long long ncell);
+int launch_nn_build(long long n, const double* pts, const NNGrid& g, int* cnt, int* off,
+                    int* csum, int* list, hipStream_t st);
+int launch_nn_query(long long n, const double* pts, const double* qry, const NNGrid& g,
+                    const int* off, const int* list, const uint8_t* vals, uint8_t* out,
+                    hipStream_t st);
 int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
                       double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

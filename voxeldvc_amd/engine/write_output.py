@@ -251,7 +251,8 @@ def remap_element_mask_to_deformed(elem_mask, U_grid, h, crop_lo=(0, 0, 0)):
     """(Nx, Ny, Nz) bool element mask carried to the deformed configuration:
     centres (i + 0.5) h moved by the mean of their 8 corner displacements,
     and each regular centre + crop_lo takes the value of the nearest moved
-    centre (exhaustive float64 search; ties: lowest element index, x
+    centre (exact float64 search through a uniform grid of cells; ties:
+    lowest element index, x
     fastest) (write_output.py:494-530, scipy griddata 'nearest')."""
     t = L.require_gpu()
     Nx, Ny, Nz = _grid_dims(U_grid)
