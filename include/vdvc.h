@@ -221,8 +221,9 @@ int vdvc_stride(vdvc_ctx* ctx, const void* d_src, int type, int nx, int ny, int 
 
 /* Phase correlation of two float32 volumes of equal dims (x fastest):
  * returns in *h_cidx the argmax of ifftn(F conj(G)/(|.|+1e-12)).real as a
- * C-order index ((x*ny + y)*nz + z).  Replaces the FFT part of
- * fft_rigid_shift (engine/correlate_gpu.py:393-405). */
+ * C-order index ((x*ny + y)*nz + z); each axis <= 4096, at most 2^26
+ * voxels.  Replaces the FFT part of fft_rigid_shift
+ * (engine/correlate_gpu.py:393-405). */
 int vdvc_phase_corr(vdvc_ctx* ctx, const float* d_f, const float* d_g, int nx, int ny, int nz,
                     int64_t* h_cidx);
 

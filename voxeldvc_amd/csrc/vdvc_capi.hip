@@ -983,11 +983,14 @@ int vdvc_phase_corr(vdvc_ctx* ctx, const float* d_f, const float* d_g, int nx, i
   if (nx < 1 || ny < 1 || nz < 1) return fail(-1, "invalid dims");
   const long long n = (long long)nx * ny * nz;
   if (n > (1LL << 26)) return fail(-1, "phase correlation volume too large (%lld voxels)", n);
+  if (nx > 4096 || ny > 4096 || nz > 4096)
+    return fail(-1, "phase correlation axes must be <= 4096 (got %d %d %d)", nx, ny, nz);
   int rc;
   if ((rc = ctx->dft.ensure((size_t)3 * n * sizeof(double2)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  if ((rc = ctx->stats.ensure((size_t)VDVC_ARG_PARTS * 16))) return rc;
   VDVC_LAUNCH(vdvc::launch_phase_corr(d_f, d_g, nx, ny, nz, ctx->dft.as<double2>(),
-                                      ctx->red.as<long long>(), ctx->stream));
+                                      ctx->stats.p, ctx->red.as<long long>(), ctx->stream));
   long long idx = 0;
   if ((rc = fetch(ctx, ctx->red.p, sizeof(long long), &idx))) return rc;
   *h_cidx = idx;

@@ -248,8 +248,10 @@ int launch_hutch_final(long long ndof, const double* acc, int nprobes, R* out, d
                        hipStream_t st);
 int launch_stride(int type, const void* src, int nx, int ny, int nz, int s, void* dst, int mx,
                   int my, int mz, hipStream_t st);
+// (value, index) partials of the phase-correlation argmax: 16 bytes each
+#define VDVC_ARG_PARTS 512
 int launch_phase_corr(const float* f, const float* g, int nx, int ny, int nz, double2* work,
-                      long long* out_idx, hipStream_t st);
+                      void* part, long long* out_idx, hipStream_t st);
 int launch_affine_cost(const float* f, int nx, int ny, int nz, const float* g, int gnx, int gny,
                        int gnz, const AffineIn& P, double* part, int nblocks, hipStream_t st);
 // learned-seed network (vdvc_seednet.hip)

@@ -8,33 +8,55 @@
 //                 equilibrium gap)
 //   Jacobi        engine/correlate_gpu.py:1042-1047, 1113-1120
 //   PCG           engine/kernels_dvc.py:550-583
+#include <algorithm>
+
 #include "vdvc_kernels.h"
 
 // corner index of local node offset (dx,dy,dz) in the reference Hex8 order
 __device__ __forceinline__ int corner_of(int dx, int dy, int dz) {
   return dz * 4 + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
 }
-__constant__ int c_cdx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-__constant__ int c_cdy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
-__constant__ int c_cdz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 
+// Hex8 corner offsets of the reference order as compile-time functions
+__device__ __forceinline__ constexpr int cdx(int a) { return (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0; }
+__device__ __forceinline__ constexpr int cdy(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0; }
+__device__ __forceinline__ constexpr int cdz(int a) { return a >= 4 ? 1 : 0; }
+
+// Element-stiffness operator at node (i,j,k): sum over the (up to 8)
+// adjacent elements e, in order, of E_e * sum over the element's nodes lb,
+// in order, of K[la][lb] v_lb (la the node's corner in e).  The 27
+// neighbouring node values are loaded once (zero outside the mesh; they are
+// only used by elements inside it) and both loops are unrolled, so every
+// neighbour index and stiffness offset is a compile-time constant.
 template <typename R>
 __device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp<R>& op,
                                           const R* __restrict__ v, int i, int j, int k,
                                           R (&out)[3]) {
+  R V[27][3];
+#pragma unroll
+  for (int m = 0; m < 27; ++m) {
+    const int ni = i + (m % 3) - 1, nj = j + (m / 3) % 3 - 1, nk = k + m / 9 - 1;
+    const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < M.nnx && nj < M.nny && nk < M.nnz;
+    const long long nb = in ? node_index(M, ni, nj, nk) : 0;
+    V[m][0] = in ? v[3 * nb] : R(0);
+    V[m][1] = in ? v[3 * nb + 1] : R(0);
+    V[m][2] = in ? v[3 * nb + 2] : R(0);
+  }
   out[0] = out[1] = out[2] = R(0);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + (e >> 2);
+    const int ox = (e & 1) - 1, oy = ((e >> 1) & 1) - 1, oz = (e >> 2) - 1;  // element offset
+    const int ei = i + ox, ej = j + oy, ek = k + oz;
     if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
-    const int la = corner_of(i - ei, j - ej, k - ek);
+    const int la = corner_of(-ox, -oy, -oz);
     const R Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
                              : op.E;
     R acc[3] = {R(0), R(0), R(0)};
+#pragma unroll
     for (int lb = 0; lb < 8; ++lb) {
-      const long long nb = node_index(M, ei + c_cdx[lb], ej + c_cdy[lb], ek + c_cdz[lb]);
-      const R vb0 = v[3 * nb], vb1 = v[3 * nb + 1], vb2 = v[3 * nb + 2];
+      const int m = (ox + cdx(lb) + 1) + 3 * (oy + cdy(lb) + 1) + 9 * (oz + cdz(lb) + 1);
+      const R vb0 = V[m][0], vb1 = V[m][1], vb2 = V[m][2];
       if (op.blockdiag) {
         const R l = op.K[(3 * la) * 24 + 3 * lb];
         acc[0] += l * vb0; acc[1] += l * vb1; acc[2] += l * vb2;
@@ -329,17 +351,17 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_kernel(
   const double pAp = sum_parts(pap_part, npap, 1, 0, lds);
   const double alpha = S->rz[it & 1] / (pAp + 1e-30);
   const R af = (R)alpha;
-  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (q < ndof) {
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
     const R pq = p[q];
     x[q] = x[q] + af * pq;
     const R rq = r[q] - af * Ap[q];
     r[q] = rq;
     const R zq = minv[q] * rq;
     z[q] = zq;
-    v[0] = (double)rq * rq;
-    v[1] = (double)rq * zq;
+    v[0] += (double)rq * rq;
+    v[1] += (double)rq * zq;
   }
   block_sum_f64<2>(v, lds);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
@@ -369,8 +391,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
   }
   const double beta = rzn / (S->rz[it & 1] + 1e-30);
   const R bf = (R)beta;
-  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  if (q < ndof) p[q] = z[q] + bf * p[q];
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
+       q += (long long)gridDim.x * VDVC_BLOCK)
+    p[q] = z[q] + bf * p[q];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S->rz[(it + 1) & 1] = rzn;
     S->relres = res;
@@ -471,6 +494,10 @@ namespace vdvc {
 
 int vec_blocks(long long n) { return (int)((n + VDVC_BLOCK - 1) / VDVC_BLOCK); }
 
+// grid of the PCG vector updates: grid-stride over at most 512 blocks, so
+// the next kernel's per-block sum over the partial dots stays short
+int pcg_blocks(long long n) { return std::min(vec_blocks(n), 512); }
+
 int node_blocks(const MeshDev& M) { return (int)((M.Nnodes + VDVC_BLOCK - 1) / VDVC_BLOCK); }
 
 template <typename R>
@@ -534,7 +561,7 @@ template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
                       const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
                       hipStream_t st) {
-  hipLaunchKernelGGL(pcg_update_kernel<R>, dim3(vec_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
+  hipLaunchKernelGGL(pcg_update_kernel<R>, dim3(pcg_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
                      it, pap_part, npap, S, minv, p, Ap, x, r, z, part);
   return (int)hipGetLastError();
 }
@@ -542,7 +569,7 @@ int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, 
 template <typename R>
 int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
                        const R* z, R* p, hipStream_t st) {
-  const int nb = vec_blocks(ndof);
+  const int nb = pcg_blocks(ndof);
   hipLaunchKernelGGL(pcg_pupdate_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol,
                      part, nb, S, z, p);
   return (int)hipGetLastError();

@@ -8,6 +8,8 @@
 // few hundred voxels per axis), so it uses separable direct DFTs in FP64
 // (exact twiddles from sincospi) instead of an FFT library: per axis each
 // output sample is one dot product over the line.
+#include <algorithm>
+
 #include "vdvc_kernels.h"
 
 template <typename T>
@@ -26,12 +28,22 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stride_kernel(const T* __restrict_
 }
 
 // one DFT pass along `axis` of a complex (interleaved double2) volume with
-// dims (nx,ny,nz) x fastest; real input when `src_real` is set
+// dims (nx,ny,nz) x fastest; real input when `src_real` is set.  The block
+// first fills an LDS table of the n twiddles (cos, sign * sin)(2 pi m / n)
+// from sincospi; output kk then walks m = kk * t mod n incrementally.
 __global__ __launch_bounds__(VDVC_BLOCK) void dft_pass_kernel(const double2* __restrict__ src,
                                                               const float* __restrict__ src_real,
                                                               double2* __restrict__ dst, int nx,
                                                               int ny, int nz, int axis,
                                                               double sign) {
+  extern __shared__ double2 tw[];
+  const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  for (int m = threadIdx.x; m < n; m += VDVC_BLOCK) {
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+    tw[m] = make_double2(cs, sn * sign);
+  }
+  __syncthreads();
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   const long long ntot = (long long)nx * ny * nz;
   if (q >= ntot) return;
@@ -39,17 +51,15 @@ __global__ __launch_bounds__(VDVC_BLOCK) void dft_pass_kernel(const double2* __r
   const long long r = q / nx;
   const int y = (int)(r % ny);
   const int z = (int)(r / ny);
-  int n, kk;
+  int kk;
   long long stride, base;
-  if (axis == 0) { n = nx; kk = x; stride = 1; base = q - x; }
-  else if (axis == 1) { n = ny; kk = y; stride = nx; base = q - (long long)y * nx; }
-  else { n = nz; kk = z; stride = (long long)nx * ny; base = q - (long long)z * nx * ny; }
+  if (axis == 0) { kk = x; stride = 1; base = q - x; }
+  else if (axis == 1) { kk = y; stride = nx; base = q - (long long)y * nx; }
+  else { kk = z; stride = (long long)nx * ny; base = q - (long long)z * nx * ny; }
   double re = 0.0, im = 0.0;
+  int m = 0;
   for (int t = 0; t < n; ++t) {
-    const long long m = ((long long)kk * t) % n;
-    double sn, cs;
-    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
-    sn *= sign;
+    const double cs = tw[m].x, sn = tw[m].y;
     if (src_real) {
       const double a = (double)src_real[base + t * stride];
       re += a * cs;
@@ -59,6 +69,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void dft_pass_kernel(const double2* __r
       re += a.x * cs - a.y * sn;
       im += a.x * sn + a.y * cs;
     }
+    m += kk;
+    if (m >= n) m -= n;
   }
   dst[q] = make_double2(re, im);
 }
@@ -77,26 +89,23 @@ __global__ __launch_bounds__(VDVC_BLOCK) void cross_power_kernel(double2* __rest
 }
 
 // argmax of the real part; ties resolve to the smallest C-order index
-// ((x*ny + y)*nz + z), as NumPy's argmax over a C-ordered (x,y,z) array
-__global__ __launch_bounds__(VDVC_BLOCK) void argmax_kernel(const double2* __restrict__ R, int nx,
-                                                            int ny, int nz,
-                                                            long long* __restrict__ out) {
+// ((x*ny + y)*nz + z), as NumPy's argmax over a C-ordered (x,y,z) array.
+// Per-block (value, index) partials, then one block over the partials; the
+// lexicographic rule gives the same winner in any reduction order.
+struct ArgBest {
+  double v;
+  long long i;
+};
+
+__device__ __forceinline__ void arg_take(ArgBest& a, double v, long long i) {
+  if (v > a.v || (v == a.v && i < a.i)) { a.v = v; a.i = i; }
+}
+
+__device__ __forceinline__ ArgBest arg_block(ArgBest a) {
   __shared__ double lv[VDVC_BLOCK];
   __shared__ long long li[VDVC_BLOCK];
-  const long long n = (long long)nx * ny * nz;
-  double best = -INFINITY;
-  long long bi = 0x7fffffffffffffffLL;
-  for (long long q = threadIdx.x; q < n; q += VDVC_BLOCK) {
-    const double v = R[q].x;
-    const int x = (int)(q % nx);
-    const long long r = q / nx;
-    const int y = (int)(r % ny);
-    const int z = (int)(r / ny);
-    const long long ci = ((long long)x * ny + y) * nz + z;
-    if (v > best || (v == best && ci < bi)) { best = v; bi = ci; }
-  }
-  lv[threadIdx.x] = best;
-  li[threadIdx.x] = bi;
+  lv[threadIdx.x] = a.v;
+  li[threadIdx.x] = a.i;
   __syncthreads();
   for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
@@ -109,7 +118,33 @@ __global__ __launch_bounds__(VDVC_BLOCK) void argmax_kernel(const double2* __res
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = li[0];
+  return ArgBest{lv[0], li[0]};
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void argmax_part_kernel(const double2* __restrict__ R,
+                                                                 int nx, int ny, int nz,
+                                                                 ArgBest* __restrict__ part) {
+  const long long n = (long long)nx * ny * nz;
+  ArgBest a{-INFINITY, 0x7fffffffffffffffLL};
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
+    const int x = (int)(q % nx);
+    const long long r = q / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    arg_take(a, R[q].x, ((long long)x * ny + y) * nz + z);
+  }
+  const ArgBest b = arg_block(a);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(VDVC_BLOCK) void argmax_final_kernel(const ArgBest* __restrict__ part,
+                                                                  int nparts,
+                                                                  long long* __restrict__ out) {
+  ArgBest a{-INFINITY, 0x7fffffffffffffffLL};
+  for (int k = threadIdx.x; k < nparts; k += VDVC_BLOCK) arg_take(a, part[k].v, part[k].i);
+  const ArgBest b = arg_block(a);
+  if (threadIdx.x == 0) out[0] = b.i;
 }
 
 // affine warp x' = A(x - c) + c + t of every voxel of f_d, order-1 sample of
@@ -175,30 +210,36 @@ int launch_stride(int type, const void* src, int nx, int ny, int nz, int s, void
   return (int)hipGetLastError();
 }
 
-// phase correlation: f, g float32 volumes (x fastest, same dims); work holds
-// 3 complex volumes; out_idx receives the C-order argmax index
+// phase correlation: f, g float32 volumes (x fastest, same dims, each axis
+// <= 4096); work holds 3 complex volumes, part VDVC_ARG_PARTS (value, index)
+// pairs; out_idx receives the C-order argmax index
 int launch_phase_corr(const float* f, const float* g, int nx, int ny, int nz, double2* work,
-                      long long* out_idx, hipStream_t st) {
+                      void* part, long long* out_idx, hipStream_t st) {
   const long long n = (long long)nx * ny * nz;
   const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
   double2* A = work;
   double2* B = work + n;
   double2* C = work + 2 * n;
   dim3 g3(nb), b3(VDVC_BLOCK);
+  const size_t sx = (size_t)nx * sizeof(double2), sy = (size_t)ny * sizeof(double2),
+               sz = (size_t)nz * sizeof(double2);
   // F = DFT(f): x (real) -> B, y -> C, z -> A
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, nullptr, f, B, nx, ny, nz, 0, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, A, nx, ny, nz, 2, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, nullptr, f, B, nx, ny, nz, 0, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, A, nx, ny, nz, 2, -1.0);
   // G = DFT(g) into C (via B)
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, nullptr, g, B, nx, ny, nz, 0, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, B, nx, ny, nz, 2, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, nullptr, g, B, nx, ny, nz, 0, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, B, nx, ny, nz, 2, -1.0);
   hipLaunchKernelGGL(cross_power_kernel, g3, b3, 0, st, A, B, n);
   // inverse DFT (positive sign; the 1/N scale does not move the argmax)
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, A, nullptr, B, nx, ny, nz, 0, 1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, B, nullptr, C, nx, ny, nz, 1, 1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, 0, st, C, nullptr, A, nx, ny, nz, 2, 1.0);
-  hipLaunchKernelGGL(argmax_kernel, dim3(1), b3, 0, st, A, nx, ny, nz, out_idx);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, A, nullptr, B, nx, ny, nz, 0, 1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, 1.0);
+  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, A, nx, ny, nz, 2, 1.0);
+  const int np = (int)std::min<long long>(VDVC_ARG_PARTS, (n + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(argmax_part_kernel, dim3((unsigned)np), b3, 0, st, A, nx, ny, nz,
+                     (ArgBest*)part);
+  hipLaunchKernelGGL(argmax_final_kernel, dim3(1), b3, 0, st, (const ArgBest*)part, np, out_idx);
   return (int)hipGetLastError();
 }
 
