@@ -529,22 +529,21 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
   using R = typename RealOf<FT>::type;
   __shared__ double lds[4 * 3];
   const R glt = (R)glt_d;
-  const int nxb = (M.Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK;
   const long long nrows = (long long)M.Nvy * M.Nvz;
   double acc[3] = {0.0, 0.0, 0.0};
-  // grid-stride over (row, x-block) pairs
-  for (long long it = blockIdx.x; it < nrows * nxb; it += gridDim.x) {
-    const long long row = it / nxb;
-    const int x = (int)(it % nxb) * VDVC_BLOCK + threadIdx.x;
-    if (x < M.Nvx) {
-      const int y = (int)(row % M.Nvy);
-      const int z = (int)(row / M.Nvy);
-      const long long vi = row * M.pitch + x;
+  // blocks stride over rows (the row, its y / z multiplicity and base are
+  // block-uniform); threads over the row's voxels
+  for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int y = (int)(row % M.Nvy);
+    const int z = (int)(row / M.Nvy);
+    const int myz = axis_mult(y, M.h, M.Nvy - 1) * axis_mult(z, M.h, M.Nvz - 1);
+    const long long rb = row * M.pitch;
+    for (int x = threadIdx.x; x < M.Nvx; x += VDVC_BLOCK) {
+      const long long vi = rb + x;
       const R val = ld_val(fld, vi);
       const bool m = use_thresh ? (val > glt) : (mask ? mask[vi] != 0 : true);
       if (m) {
-        const double mlt = (double)(axis_mult(x, M.h, M.Nvx - 1) * axis_mult(y, M.h, M.Nvy - 1) *
-                                    axis_mult(z, M.h, M.Nvz - 1));
+        const double mlt = (double)(axis_mult(x, M.h, M.Nvx - 1) * myz);
         const double d = (double)val;
         acc[0] += mlt;
         acc[1] += mlt * d;
@@ -580,8 +579,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void reduce_partials_kernel(const doubl
 }
 
 // bfloat16 gradient planes multiplied by a 0/1 voxel mask (the folded
-// storage read by the Hessian kernels when no separate mask is given)
-__global__ __launch_bounds__(VDVC_BLOCK) void fold_kernel(long long n,
+// storage read by the Hessian kernels when no separate mask is given); eight
+// voxels per thread (16-byte plane and 8-byte mask accesses) with `vec` (n a
+// multiple of 8, aligned pointers), one otherwise
+__global__ __launch_bounds__(VDVC_BLOCK) void fold_kernel(long long n, int vec,
                                                           const uint16_t* __restrict__ gx,
                                                           const uint16_t* __restrict__ gy,
                                                           const uint16_t* __restrict__ gz,
@@ -590,6 +591,27 @@ __global__ __launch_bounds__(VDVC_BLOCK) void fold_kernel(long long n,
                                                           uint16_t* __restrict__ oy,
                                                           uint16_t* __restrict__ oz) {
   const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (vec) {
+    if (8 * q >= n) return;
+    const uint2 mk = *reinterpret_cast<const uint2*>(m + 8 * q);
+    // 0xffff per voxel whose mask byte is set, 0 otherwise
+    uint32_t keep[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t w = j < 2 ? mk.x : mk.y;
+      const int s = 16 * (j & 1);
+      keep[j] = (((w >> s) & 0xffu) ? 0x0000ffffu : 0u) | (((w >> (s + 8)) & 0xffu) ? 0xffff0000u : 0u);
+    }
+    const uint16_t* src[3] = {gx, gy, gz};
+    uint16_t* dst[3] = {ox, oy, oz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uint4 v = *reinterpret_cast<const uint4*>(src[c] + 8 * q);
+      v.x &= keep[0]; v.y &= keep[1]; v.z &= keep[2]; v.w &= keep[3];
+      *reinterpret_cast<uint4*>(dst[c] + 8 * q) = v;
+    }
+    return;
+  }
   if (q >= n) return;
   const bool k = m[q] != 0;
   ox[q] = k ? gx[q] : (uint16_t)0;
@@ -752,8 +774,13 @@ int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* ma
 
 int launch_fold(long long n, const uint16_t* gx, const uint16_t* gy, const uint16_t* gz,
                 const uint8_t* m, uint16_t* ox, uint16_t* oy, uint16_t* oz, hipStream_t st) {
-  const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
-  hipLaunchKernelGGL(fold_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, gx, gy, gz, m, ox, oy, oz);
+  auto al = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; };
+  const int vec = n % 8 == 0 && al(gx, 16) && al(gy, 16) && al(gz, 16) && al(ox, 16) &&
+                  al(oy, 16) && al(oz, 16) && al(m, 8);
+  const long long work = vec ? n / 8 : n;
+  const unsigned nb = (unsigned)((work + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(fold_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, vec, gx, gy, gz, m, ox, oy,
+                     oz);
   return (int)hipGetLastError();
 }
 

@@ -548,8 +548,7 @@ class _Level:
         t = L.require_gpu()
         # images in the storage type (float16 is exact for 8-bit data); the
         # reference volume in the pitched level layout
-        fp = L.pitch_dense(f, self.shape)
-        self.f = fp.to(getattr(t, iname))
+        self.f = L.pitch_dense(f, self.shape, iname)
         self.g = g.to(getattr(t, iname))
         self.gshp = tuple(gshp)
         self.gname = gname
