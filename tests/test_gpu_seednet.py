@@ -40,6 +40,25 @@ def test_conv_forward(gpu, layer, shape):
         assert rel(L.to_host(y).reshape(ref.shape), r) < 1e-5
 
 
+@pytest.mark.parametrize('layer', [(16, 32, 2), (32, 32, 1), (32, 3, 1)])
+def test_conv_forward_many_blocks(gpu, layer):
+    """A 17^3 (and 33^3 for the stride-2 layer) volume: tens of blocks of
+    256 output positions, partial last block."""
+    from voxeldvc_amd import _lib as L
+    C, O, s = layer
+    n = 33 if s == 2 else 17
+    print('conv fwd many blocks', layer, n, flush=True)
+    rng = np.random.default_rng(7 + C + O)
+    x = rng.normal(size=(1, n, n, n, C)).astype(np.float32)
+    w = (rng.normal(size=(O, 27, C)) * 0.2).astype(np.float32)
+    b = rng.normal(size=O).astype(np.float32)
+    ref = CO.relu(CO.conv3d_forward(CO.bf16_round(x), CO.bf16_round(w), b.astype(np.float64), s))
+    y = L.empty(ref.size)
+    L.call('vdvc_conv3d_forward', L.ctx(), 1, n, n, n, C, O, s, dev(x), dev(w), dev(b), 1,
+           L.ptr(y))
+    assert rel(L.to_host(y).reshape(ref.shape), ref) < 1e-5
+
+
 @pytest.mark.parametrize('layer', [(2, 16, 1), (16, 32, 2), (32, 32, 1), (32, 3, 1), (3, 5, 2)])
 @pytest.mark.parametrize('shape', SHAPES)
 def test_conv_backward(gpu, layer, shape):

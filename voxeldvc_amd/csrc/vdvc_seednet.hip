@@ -112,78 +112,108 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvShape S, const float*
 
 // Channel-specialised forward kernels (same GEMM, depth order tap*C + c):
 // with C a multiple of 16 a 16-deep chunk is 16 channels of one tap, so a
-// lane's 4 operand values are one aligned 16-byte load and the tap offsets
-// are compile-time; with C = 2 a lane's 4 values are two 8-byte loads of two
-// taps.  Both replace the generic kernel's per-element index division.
+// lane's 4 operand values are one aligned 16-byte load (conv_fwd_lds_kernel);
+// with C = 2 a lane's 4 values are two 8-byte loads of two taps.  Both
+// replace the generic kernel's per-element index division.
 __device__ __forceinline__ bf16x4_t to_bf16x4(float4 v) {
   bf16x4_t r;
   r[0] = bf16_rne(v.x); r[1] = bf16_rne(v.y); r[2] = bf16_rne(v.z); r[3] = bf16_rne(v.w);
   return r;
 }
 
+// Forward for C a multiple of 16 with the layer's weights staged once per
+// block in LDS as bfloat16 B fragments: fragment (tap, cc, t) is 64 lanes x 4
+// values, 512 contiguous bytes (one conflict-free 8-byte read per lane).
+// kFwdWaves wavefronts per block, each computing kFwdTiles tiles of 16
+// positions x all NT*16 output channels, so an A fragment feeds NT products
+// and a B fragment kFwdTiles.  Per output the products accumulate over the
+// taps in order, then the 16-channel chunks, as in the other forward kernels.
+constexpr int kFwdWaves = 8;
+constexpr int kFwdTiles = 2;
+
 template <int NT, int CC>
-__global__ __launch_bounds__(256) void conv_fwd_c16_kernel(ConvShape S,
-                                                          const float* __restrict__ x,
-                                                          const float* __restrict__ w,
-                                                          const float* __restrict__ bias,
-                                                          int relu, float* __restrict__ y) {
+__global__ __launch_bounds__(64 * kFwdWaves) void conv_fwd_lds_kernel(
+    ConvShape S, const float* __restrict__ x, const float* __restrict__ w,
+    const float* __restrict__ bias, int relu, float* __restrict__ y) {
   constexpr int C = 16 * CC;
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const long long P = (long long)S.N * S.Do * S.Ho * S.Wo;
-  const long long p0 = (long long)wave * 16;
-  if (p0 >= P) return;
-  const int r = lane & 15, q = lane >> 4;
-  const long long pa = p0 + r;
-  const bool va = pa < P;
-  int n = 0, z = 0, yy = 0, xx = 0;
-  if (va) {
-    long long t = pa;
-    xx = (int)(t % S.Wo); t /= S.Wo;
-    yy = (int)(t % S.Ho); t /= S.Ho;
-    z = (int)(t % S.Do); n = (int)(t / S.Do);
-  }
-  const int bz = z * S.stride - 1, by = yy * S.stride - 1, bx = xx * S.stride - 1;
-  f32x4_t acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  constexpr int NF = 27 * CC * NT;
+  __shared__ bf16x4_t wl[NF * 64];
   const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int e = threadIdx.x; e < NF * 64; e += 64 * kFwdWaves) {
+    const int f = e >> 6, l = e & 63;
+    const int t = f % NT, cc = (f / NT) % CC, tap = f / (NT * CC);
+    const int o = t * 16 + (l & 15), c0 = cc * 16 + 4 * (l >> 4);
+    float4 v = zero4;
+    if (o < S.O) v = *reinterpret_cast<const float4*>(w + ((long long)o * 27 + tap) * C + c0);
+    wl[e] = to_bf16x4(v);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const long long P = (long long)S.N * S.Do * S.Ho * S.Wo;
+  const long long pbase = ((long long)blockIdx.x * kFwdWaves + wv) * (16 * kFwdTiles);
+  if (pbase >= P) return;
+  int bz[kFwdTiles], by[kFwdTiles], bx[kFwdTiles], nn[kFwdTiles];
+  bool va[kFwdTiles];
+#pragma unroll
+  for (int u = 0; u < kFwdTiles; ++u) {
+    const long long pa = pbase + 16 * u + r;
+    va[u] = pa < P;
+    long long t = va[u] ? pa : 0;
+    const int xx = (int)(t % S.Wo); t /= S.Wo;
+    const int yy = (int)(t % S.Ho); t /= S.Ho;
+    const int zz = (int)(t % S.Do);
+    nn[u] = (int)(t / S.Do);
+    bz[u] = zz * S.stride - 1; by[u] = yy * S.stride - 1; bx[u] = xx * S.stride - 1;
+  }
+  f32x4_t acc[kFwdTiles][NT];
+#pragma unroll
+  for (int u = 0; u < kFwdTiles; ++u)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[u][t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 3
   for (int tap = 0; tap < 27; ++tap) {
     const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-    const int iz = bz + dz, iy = by + dy, ix = bx + dx;
-    const bool v = va && iz >= 0 && iz < S.D && iy >= 0 && iy < S.H && ix >= 0 && ix < S.W;
-    const float* xp =
-        x + ((((long long)n * S.D + (v ? iz : 0)) * S.H + (v ? iy : 0)) * S.W + (v ? ix : 0)) * C +
-        4 * q;
+    bf16x4_t a[kFwdTiles][CC];
 #pragma unroll
-    for (int cc = 0; cc < CC; ++cc) {
-      const float4 av = v ? *reinterpret_cast<const float4*>(xp + cc * 16) : zero4;
-      const bf16x4_t a = to_bf16x4(av);
+    for (int u = 0; u < kFwdTiles; ++u) {
+      const int iz = bz[u] + dz, iy = by[u] + dy, ix = bx[u] + dx;
+      const bool v = va[u] && iz >= 0 && iz < S.D && iy >= 0 && iy < S.H && ix >= 0 && ix < S.W;
+      const float* xp = x +
+          ((((long long)nn[u] * S.D + (v ? iz : 0)) * S.H + (v ? iy : 0)) * S.W + (v ? ix : 0)) * C +
+          4 * q;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int o = t * 16 + r;
-        const float4 wv = o < S.O ? *reinterpret_cast<const float4*>(
-                                        w + ((long long)o * 27 + tap) * C + cc * 16 + 4 * q)
-                                  : zero4;
-        acc[t] = mfma16(a, to_bf16x4(wv), acc[t]);
+      for (int cc = 0; cc < CC; ++cc) {
+        float4 av = *reinterpret_cast<const float4*>(xp + cc * 16);
+        if (!v) av = zero4;
+        a[u][cc] = to_bf16x4(av);
       }
     }
+#pragma unroll
+    for (int cc = 0; cc < CC; ++cc)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x4_t b = wl[((tap * CC + cc) * NT + t) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < kFwdTiles; ++u) acc[u][t] = mfma16(a[u][cc], b, acc[u][t]);
+      }
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int o = t * 16 + r;
-    if (o >= S.O) continue;
-    const float bo = bias ? bias[o] : 0.0f;
+  for (int u = 0; u < kFwdTiles; ++u)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const long long p = p0 + 4 * q + j;
-      if (p >= P) continue;
-      float v = acc[t][j] + bo;
-      if (relu) v = v > 0.0f ? v : 0.0f;
-      y[p * S.O + o] = v;
+    for (int t = 0; t < NT; ++t) {
+      const int o = t * 16 + r;
+      if (o >= S.O) continue;
+      const float bo = bias ? bias[o] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long p = pbase + 16 * u + 4 * q + j;
+        if (p >= P) continue;
+        float v = acc[u][t][j] + bo;
+        if (relu) v = v > 0.0f ? v : 0.0f;
+        y[p * S.O + o] = v;
+      }
     }
-  }
 }
 
 template <int NT>
@@ -560,16 +590,19 @@ int launch_conv_fwd(int N, int D, int H, int W, int C, int O, int stride, const 
   const unsigned nb = (unsigned)((waves + 3) / 4);
 #define VDVC_FWD(K) hipLaunchKernelGGL(K, dim3(nb), dim3(256), 0, st, S, x, w, b, relu, y)
   const bool two = O > 16;
+  const unsigned nbl = (unsigned)((P + 16 * kFwdWaves * kFwdTiles - 1) / (16 * kFwdWaves * kFwdTiles));
+#define VDVC_FWDL(K) hipLaunchKernelGGL(K, dim3(nbl), dim3(64 * kFwdWaves), 0, st, S, x, w, b, relu, y)
   if (C == 16) {
-    if (two) VDVC_FWD((conv_fwd_c16_kernel<2, 1>)); else VDVC_FWD((conv_fwd_c16_kernel<1, 1>));
+    if (two) VDVC_FWDL((conv_fwd_lds_kernel<2, 1>)); else VDVC_FWDL((conv_fwd_lds_kernel<1, 1>));
   } else if (C == 32) {
-    if (two) VDVC_FWD((conv_fwd_c16_kernel<2, 2>)); else VDVC_FWD((conv_fwd_c16_kernel<1, 2>));
+    if (two) VDVC_FWDL((conv_fwd_lds_kernel<2, 2>)); else VDVC_FWDL((conv_fwd_lds_kernel<1, 2>));
   } else if (C == 2) {
     if (two) VDVC_FWD(conv_fwd_c2_kernel<2>); else VDVC_FWD(conv_fwd_c2_kernel<1>);
   } else {
     if (two) VDVC_FWD(conv_fwd_kernel<2>); else VDVC_FWD(conv_fwd_kernel<1>);
   }
 #undef VDVC_FWD
+#undef VDVC_FWDL
   return (int)hipGetLastError();
 }
 
