@@ -574,7 +574,6 @@ class _Level:
         mean0, std0 = _stats_mean_std(s3)
         wd = self.wd
         tdt = getattr(t, wd)
-        ndt = np.float64 if wd == 'float64' else np.float32
         # scalars enter the element-wise steps rounded to the working type
         rnd = (lambda x: float(x)) if wd == 'float64' else (lambda x: float(np.float32(x)))
         mean0_f = rnd(mean0)
@@ -583,17 +582,18 @@ class _Level:
             raise ValueError("reg_type must be 'laplacian' or 'equilibrium_gap'")
         reg = _Reg(reg_type, K_ref_laplacian, K_ref_elastic, h, eps_reg)
         folded = gt == L.BF16
-        # folded storage: H and b read g * mask; the working planes are
-        # re-folded whenever the active mask changes
-        Gw = [L.zeros(Nvox, 'bfloat16') for _ in range(3)] if folded else G
-
-        def fold(mask):
-            L.call('vdvc_fold_grad', c, self.Nvox_p, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
-                   L.ptr(mask), L.ptr(Gw[0]), L.ptr(Gw[1]), L.ptr(Gw[2]))
+        # folded storage: H and b read g * mask.  The set-up kernel zeroes the
+        # gradients wherever the eroded mask0 is false, and that erosion lies
+        # inside mask0, so g * mask0 = g: mask0 needs no folded copy.  With a
+        # dynamic mask the residual kernel writes g * b_mask into the working
+        # planes every iteration (every grid entry, before a product reads it).
+        Gw = [L.empty(Nvox, 'bfloat16') for _ in range(3)] if folded and dynamic_mask else G
 
         def hmask(mask):
             """(gradient planes, mask pointer) for H with `mask`."""
-            return (Gw, None) if folded else (G, mask)
+            if folded:
+                return (G, None) if mask is mask0 else (Gw, None)
+            return (G, mask)
 
         hutch = None
         if reg.kind == L.REG_EQGAP:
@@ -611,8 +611,6 @@ class _Level:
                 raise ValueError("K_ref_elastic is required when l0 is given and "
                                  "reg_type='equilibrium_gap'")
             T = 10 * h
-            if folded:
-                fold(mask0)
             Gh, mh = hmask(mask0)
             if reg_type == 'equilibrium_gap':
                 d = L.empty(Ndof, wd)
@@ -625,11 +623,12 @@ class _Level:
                        L.ptr(dl))
                 L0 = K.dot(hutch, ones) + reg.eps * K.dot(dl, ones)
             else:
-                nodes = np.arange((Nx + 1) * (Ny + 1) * (Nz + 1))
+                # the calibration mode cos(2 pi y / T) on the x DOFs, formed on
+                # the device in the working type (same operation order)
+                nodes = t.arange((Nx + 1) * (Ny + 1) * (Nz + 1), device='cuda')
                 jj = (nodes // (Nx + 1)) % (Ny + 1)
-                V = np.zeros(Ndof, dtype=ndt)
-                V[0::3] = np.cos(jj.astype(ndt) * h / T * 2 * np.pi)
-                Vd = L.to_dev_flat(V, wd)
+                Vd = t.zeros(Ndof, dtype=tdt, device='cuda')
+                Vd[0::3] = t.cos(jj.to(tdt) * h / T * 2 * np.pi)
                 Hv = L.empty(Ndof, wd)
                 L.call('vdvc_matvec_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]),
                        L.ptr(Gh[2]), gt, L.ptr(mh), L.ptr(Vd), L.ptr(Hv))
@@ -651,7 +650,9 @@ class _Level:
         Minv = L.empty(Ndof, wd)
         b = L.empty(Ndof, wd)
         RU = L.empty(Ndof, wd)
-        gvals = L.zeros(Nvox, wd)
+        # sampled values: written at every voxel of the grid (the row padding
+        # is never read into a result)
+        gvals = L.empty(Nvox, wd)
         mask = L.zeros(Nvox, 'uint8')
 
         def jacobi(bm):
@@ -661,8 +662,6 @@ class _Level:
                    L.ptr(diag), L.ptr(Minv))
 
         if not dynamic_mask:
-            if folded:
-                fold(mask0)
             jacobi(mask0)
         frozen = None
         s4 = (L._d * 4)()
