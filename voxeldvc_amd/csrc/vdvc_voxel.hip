@@ -521,23 +521,76 @@ __global__ __launch_bounds__(VDVC_BLOCK) void setup_kernel(MeshDev M, const FT* 
 // multiplicity-weighted masked statistics of a voxel field, per-block FP64
 // partials (count, sum, sum of squares)
 // ---------------------------------------------------------------------------
+// eight consecutive values (16-byte aligned) as the working type
+__device__ __forceinline__ void ld8(const __half* p, float (&v)[8]) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    v[j] = __half2float(__ushort_as_half((unsigned short)(w[j >> 1] >> (16 * (j & 1)))));
+}
+__device__ __forceinline__ void ld8(const float* p, float (&v)[8]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void ld8(const double* p, double (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2 a = reinterpret_cast<const double2*>(p)[k];
+    v[2 * k] = a.x; v[2 * k + 1] = a.y;
+  }
+}
+
+// With `vec` (16-byte aligned field, 8-byte aligned mask; the pitch is a
+// multiple of 16, so the loads stay inside the row) wavefronts stride over
+// rows and lanes take eight voxels each; otherwise blocks stride over rows
+// and threads take one voxel.  The row, its y / z multiplicity and base are
+// uniform over the wavefront or block.
 template <typename FT>
 __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* __restrict__ fld,
                                                            const uint8_t* __restrict__ mask,
-                                                           int use_thresh, double glt_d,
+                                                           int use_thresh, double glt_d, int vec,
                                                            double* __restrict__ part) {
   using R = typename RealOf<FT>::type;
   __shared__ double lds[4 * 3];
   const R glt = (R)glt_d;
   const long long nrows = (long long)M.Nvy * M.Nvz;
   double acc[3] = {0.0, 0.0, 0.0};
-  // blocks stride over rows (the row, its y / z multiplicity and base are
-  // block-uniform); threads over the row's voxels
-  for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 =
+      (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long r0 = vec ? wave0 : blockIdx.x;
+  const long long rstep = vec ? (long long)gridDim.x * VDVC_WAVES_PER_BLOCK : gridDim.x;
+  for (long long row = r0; row < nrows; row += rstep) {
     const int y = (int)(row % M.Nvy);
     const int z = (int)(row / M.Nvy);
     const int myz = axis_mult(y, M.h, M.Nvy - 1) * axis_mult(z, M.h, M.Nvz - 1);
     const long long rb = row * M.pitch;
+    if (vec) {
+      for (int x0 = 8 * lane; x0 < M.Nvx; x0 += 8 * 64) {
+        R v[8];
+        ld8(fld + rb + x0, v);
+        uint2 mk = make_uint2(0xffffffffu, 0xffffffffu);
+        if (!use_thresh && mask) mk = *reinterpret_cast<const uint2*>(mask + rb + x0);
+        int rx = x0 % M.h;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int x = x0 + j;
+          const bool mb = ((j < 4 ? mk.x : mk.y) >> (8 * (j & 3))) & 0xffu;
+          const bool m = x < M.Nvx && (use_thresh ? (v[j] > glt) : mb);
+          if (m) {
+            const int mx = (x > 0 && x < M.Nvx - 1 && rx == 0) ? 2 : 1;
+            const double mlt = (double)(mx * myz);
+            const double d = (double)v[j];
+            acc[0] += mlt;
+            acc[1] += mlt * d;
+            acc[2] += mlt * d * d;
+          }
+          rx = rx + 1 == M.h ? 0 : rx + 1;
+        }
+      }
+      continue;
+    }
     for (int x = threadIdx.x; x < M.Nvx; x += VDVC_BLOCK) {
       const long long vi = rb + x;
       const R val = ld_val(fld, vi);
@@ -557,6 +610,91 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
     part[3 * blockIdx.x + 1] = acc[1];
     part[3 * blockIdx.x + 2] = acc[2];
   }
+}
+
+// eight gradient values (16-byte aligned) stored in the plane's type
+__device__ __forceinline__ void st8(__hip_bfloat16* p, const float (&v)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    w[k] = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * k])) |
+           ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * k + 1])) << 16);
+  *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ void st8(float* p, const float (&v)[8]) {
+  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ void st8(double* p, const double (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) reinterpret_cast<double2*>(p)[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
+// setup_kernel for eight consecutive voxels of a row per thread (threads
+// numbered over (row, group of eight)): the row and
+// its y / z neighbour rows as 16-byte vector loads, the two x neighbours
+// outside the group as scalars, vector stores (16-byte aligned planes,
+// 8-byte aligned mask, pitch a multiple of 8).  Per voxel the same
+// operations as setup_kernel.
+template <typename FT, typename GO>
+__global__ __launch_bounds__(VDVC_BLOCK) void setup_vec_kernel(MeshDev M, const FT* __restrict__ f,
+                                                               double glt_d, GO* __restrict__ gxo,
+                                                               GO* __restrict__ gyo,
+                                                               GO* __restrict__ gzo,
+                                                               uint8_t* __restrict__ mask0) {
+  using R = typename RealOf<GO>::type;
+  const int groups = M.pitch / 8;
+  const long long gid = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;  // (row, group)
+  const long long row = gid / groups;
+  const int gi = (int)(gid - row * groups);
+  if (row >= (long long)M.Nvy * M.Nvz) return;
+  const int x0 = 8 * gi;
+  const long long vi = row * M.pitch + x0;
+  R gx[8], gy[8], gz[8];
+  uint32_t mw[2] = {0u, 0u};
+  if (x0 >= M.Nvx) {  // row padding: zero gradients and mask
+#pragma unroll
+    for (int j = 0; j < 8; ++j) gx[j] = gy[j] = gz[j] = R(0);
+  } else {
+    const R glt = (R)glt_d;
+    const int y = (int)(row % M.Nvy);
+    const int z = (int)(row / M.Nvy);
+    const long long sy = M.pitch, sz = (long long)M.pitch * M.Nvy;
+    const int ym = y > 0 ? y - 1 : 0, yp = y < M.Nvy - 1 ? y + 1 : y;
+    const int zm = z > 0 ? z - 1 : 0, zp = z < M.Nvz - 1 ? z + 1 : z;
+    R c[8], fym[8], fyp[8], fzm[8], fzp[8];
+    ld8(f + vi, c);
+    ld8(f + vi + (long long)(ym - y) * sy, fym);
+    ld8(f + vi + (long long)(yp - y) * sy, fyp);
+    ld8(f + vi + (long long)(zm - z) * sz, fzm);
+    ld8(f + vi + (long long)(zp - z) * sz, fzp);
+    const R left = x0 > 0 ? (R)ld_val(f, vi - 1) : c[0];
+    const R right = x0 + 8 < M.Nvx ? (R)ld_val(f, vi + 8) : c[7];
+    auto grad1 = [](R fm, R fp, int t, int n) -> R {
+      if (t == 0 || t == n - 1) return sub_rn(fp, fm);  // one-sided, spacing 1
+      return sub_rn(fp, fm) * R(0.5);                  // central
+    };
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int x = x0 + j;
+      if (x >= M.Nvx) { gx[j] = gy[j] = gz[j] = R(0); continue; }
+      const R fxm = x > 0 ? (j > 0 ? c[j - 1] : left) : c[j];
+      const R fxp = x < M.Nvx - 1 ? (j < 7 ? c[j + 1] : right) : c[j];
+      R a = grad1(fxm, fxp, x, M.Nvx);
+      R b = grad1(fym[j], fyp[j], y, M.Nvy);
+      R d = grad1(fzm[j], fzp[j], z, M.Nvz);
+      const bool m0 = c[j] > glt;
+      const bool er = m0 && (fxm > glt) && (fxp > glt) && (fym[j] > glt) && (fyp[j] > glt) &&
+                      (fzm[j] > glt) && (fzp[j] > glt);
+      if (!er) { a = R(0); b = R(0); d = R(0); }
+      gx[j] = a; gy[j] = b; gz[j] = d;
+      if (m0) mw[j >> 2] |= 1u << (8 * (j & 3));
+    }
+  }
+  *reinterpret_cast<uint2*>(mask0 + vi) = make_uint2(mw[0], mw[1]);
+  st8(gxo + vi, gx);
+  st8(gyo + vi, gy);
+  st8(gzo + vi, gz);
 }
 
 // fixed-order reduction of n partial records of width nv (deterministic):
@@ -731,44 +869,51 @@ int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleI
 
 int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, double glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st) {
-  const int nxb = (M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK;
-  const long long nb = (long long)nxb * M.Nvy * M.Nvz;
+  auto al = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; };
+  const bool vec = M.pitch % 8 == 0 && al(f, 16) && al(gx, 16) && al(gy, 16) && al(gz, 16) &&
+                   al(mask0, 8);
+  const long long nb = vec ? ((long long)(M.pitch / 8) * M.Nvy * M.Nvz + VDVC_BLOCK - 1) / VDVC_BLOCK
+                           : (long long)((M.pitch + VDVC_BLOCK - 1) / VDVC_BLOCK) * M.Nvy * M.Nvz;
   if (nb > 0x7fffffffLL) return (int)hipErrorInvalidValue;
   dim3 g((unsigned)nb), b(VDVC_BLOCK);
+#define VDVC_SETUP_K(FT, GO) (vec ? setup_vec_kernel<FT, GO> : setup_kernel<FT, GO>)
   if (ftype == GT_F64 || gtype == GT_F64) {
     // the FP64 path: double image, double gradients
     if (ftype != GT_F64 || gtype != GT_F64) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL((setup_kernel<double, double>), g, b, 0, st, M, (const double*)f, glt,
+    hipLaunchKernelGGL(VDVC_SETUP_K(double, double), g, b, 0, st, M, (const double*)f, glt,
                        (double*)gx, (double*)gy, (double*)gz, mask0);
   } else if (ftype == GT_F32) {
     if (gtype == GT_F32)
-      hipLaunchKernelGGL((setup_kernel<float, float>), g, b, 0, st, M, (const float*)f, glt,
+      hipLaunchKernelGGL(VDVC_SETUP_K(float, float), g, b, 0, st, M, (const float*)f, glt,
                          (float*)gx, (float*)gy, (float*)gz, mask0);
     else
-      hipLaunchKernelGGL((setup_kernel<float, __hip_bfloat16>), g, b, 0, st, M, (const float*)f,
+      hipLaunchKernelGGL(VDVC_SETUP_K(float, __hip_bfloat16), g, b, 0, st, M, (const float*)f,
                          glt, (__hip_bfloat16*)gx, (__hip_bfloat16*)gy, (__hip_bfloat16*)gz, mask0);
   } else {
     if (gtype == GT_F32)
-      hipLaunchKernelGGL((setup_kernel<__half, float>), g, b, 0, st, M, (const __half*)f, glt,
+      hipLaunchKernelGGL(VDVC_SETUP_K(__half, float), g, b, 0, st, M, (const __half*)f, glt,
                          (float*)gx, (float*)gy, (float*)gz, mask0);
     else
-      hipLaunchKernelGGL((setup_kernel<__half, __hip_bfloat16>), g, b, 0, st, M, (const __half*)f,
+      hipLaunchKernelGGL(VDVC_SETUP_K(__half, __hip_bfloat16), g, b, 0, st, M, (const __half*)f,
                          glt, (__hip_bfloat16*)gx, (__hip_bfloat16*)gy, (__hip_bfloat16*)gz, mask0);
   }
+#undef VDVC_SETUP_K
   return (int)hipGetLastError();
 }
 
 int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
                  double glt, double* part, int nblocks, hipStream_t st) {
+  const int vec = M.pitch % 16 == 0 && (reinterpret_cast<uintptr_t>(fld) & 15) == 0 &&
+                  (reinterpret_cast<uintptr_t>(mask) & 7) == 0;
   if (ftype == GT_F32)
     hipLaunchKernelGGL(stats_kernel<float>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
-                       (const float*)fld, mask, use_thresh, glt, part);
+                       (const float*)fld, mask, use_thresh, glt, vec, part);
   else if (ftype == GT_F64)
     hipLaunchKernelGGL(stats_kernel<double>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
-                       (const double*)fld, mask, use_thresh, glt, part);
+                       (const double*)fld, mask, use_thresh, glt, vec, part);
   else
     hipLaunchKernelGGL(stats_kernel<__half>, dim3(nblocks), dim3(VDVC_BLOCK), 0, st, M,
-                       (const __half*)fld, mask, use_thresh, glt, part);
+                       (const __half*)fld, mask, use_thresh, glt, vec, part);
   return (int)hipGetLastError();
 }
 
