@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 4
+#define VDVC_ABI_VERSION 5
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -193,6 +193,12 @@ int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
  * linalg.norm (engine/kernels_dvc.py:558-578, engine/correlate_gpu.py:1131). */
 int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b,
              double* h_out);
+
+/* The plane-wave calibration mode of the regularization weight, in rtype:
+ * out[3n] = cos(j*h/T*2*pi) at node n = i + (Nx+1)*(j + (Ny+1)*k), T = 10*h,
+ * evaluated one rounded operation at a time from the left; out[3n+1] =
+ * out[3n+2] = 0.  Replaces the V construction of correlate_gpu.py:1016-1020. */
+int vdvc_calibration_mode(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, int rtype, void* d_out);
 
 /* y = a*x + b*y in rtype. */
 int vdvc_axpby(vdvc_ctx* ctx, int64_t n, int rtype, double a, const void* d_x, double b,

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 4
+ABI_VERSION = 5
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -56,6 +56,7 @@ SIGNATURES = {
                      _vp],
     'vdvc_dot': [_vp, _i64, _i, _vp, _vp, _dp],
     'vdvc_axpby': [_vp, _i64, _i, _d, _vp, _d, _vp],
+    'vdvc_calibration_mode': [_vp, _i, _i, _i, _i, _i, _vp],
     'vdvc_mul': [_vp, _i64, _i, _vp, _vp, _vp],
     'vdvc_fold_grad': [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'vdvc_erode6': [_vp, _i, _i, _i, _vp, _vp],
@@ -331,9 +332,11 @@ def pitch_dense(d, shape, dtype=None):
     t = require_gpu()
     nx, ny, nz = shape
     p = pitch_of(nx)
-    out = t.zeros((nz, ny, p), dtype=d.dtype if dtype is None else getattr(t, dtype),
+    out = t.empty((nz, ny, p), dtype=d.dtype if dtype is None else getattr(t, dtype),
                   device=d.device)
     out[:, :, :nx] = d.view(nz, ny, nx)
+    if p > nx:
+        out[:, :, nx:] = 0
     return out.view(-1)
 
 

@@ -933,6 +933,18 @@ int vdvc_axpby(vdvc_ctx* ctx, int64_t n, int rtype, double a, const void* d_x, d
   return 0;
 }
 
+int vdvc_calibration_mode(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, int rtype, void* d_out) {
+  if (!ctx || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
+  if (rtype == VDVC_F64)
+    VDVC_LAUNCH(vdvc::launch_calib_mode(Nx, Ny, Nz, h, static_cast<double*>(d_out), ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_calib_mode(Nx, Ny, Nz, h, static_cast<float*>(d_out), ctx->stream));
+  return 0;
+}
+
 int vdvc_fold_grad(vdvc_ctx* ctx, int64_t nvox, const uint16_t* d_gx, const uint16_t* d_gy,
                    const uint16_t* d_gz, const uint8_t* d_mask, uint16_t* d_ox, uint16_t* d_oy,
                    uint16_t* d_oz) {

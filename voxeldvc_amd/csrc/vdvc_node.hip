@@ -582,6 +582,33 @@ int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t 
   return (int)hipGetLastError();
 }
 
+// the regularization weight's plane-wave calibration mode: x component
+// cos(j h / T * 2 pi) at node (i, j, k), T = 10 h, y and z components zero;
+// one correctly rounded operation at a time, left to right, in R
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void calib_mode_kernel(long long nn, int nnx, int nny,
+                                                                R hR, R TR, R* __restrict__ out) {
+  const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (n >= nn) return;
+  const long long j = (n / nnx) % nny;
+  R v = (R)j * hR;
+  v = v / TR;
+  v = v * R(2);
+  v = v * (R)3.141592653589793;
+  out[3 * n] = cos(v);
+  out[3 * n + 1] = R(0);
+  out[3 * n + 2] = R(0);
+}
+
+template <typename R>
+int launch_calib_mode(int Nx, int Ny, int Nz, int h, R* out, hipStream_t st) {
+  const long long nn = (long long)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  const unsigned nb = (unsigned)((nn + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(calib_mode_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, nn, Nx + 1, Ny + 1,
+                     (R)h, (R)(10 * h), out);
+  return (int)hipGetLastError();
+}
+
 template <typename R>
 int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hipStream_t st) {
   hipLaunchKernelGGL(dot_kernel<R>, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, a, b, part);
@@ -602,7 +629,8 @@ int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hi
   template int launch_pcg_pupdate<R>(long long, int, double, const double*, PcgState*,       \
                                      const R*, R*, hipStream_t);                             \
   template int launch_axpby<R>(long long, double, const R*, double, R*, hipStream_t);        \
-  template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);
+  template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);      \
+  template int launch_calib_mode<R>(int, int, int, int, R*, hipStream_t);
 VDVC_INST(float)
 VDVC_INST(double)
 #undef VDVC_INST

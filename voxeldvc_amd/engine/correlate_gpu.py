@@ -547,9 +547,14 @@ class _Level:
         self.wd = 'float64' if storage == 'fp64' else 'float32'
         t = L.require_gpu()
         # images in the storage type (float16 is exact for 8-bit data); the
-        # reference volume in the pitched level layout
+        # reference volume in the pitched level layout.  The deformed image is
+        # only gathered by the sampler, whose cost is per voxel and not per
+        # byte: a float32 volume is sampled as it is, with no float16 copy.
         self.f = L.pitch_dense(f, self.shape, iname)
-        self.g = g.to(getattr(t, iname))
+        if storage == 'bf16' and g.dtype == t.float32:
+            self.g, self.gitype = g, L.F32
+        else:
+            self.g, self.gitype = g.to(getattr(t, iname)), self.itype
         self.gshp = tuple(gshp)
         self.gname = gname
 
@@ -625,10 +630,8 @@ class _Level:
             else:
                 # the calibration mode cos(2 pi y / T) on the x DOFs, formed on
                 # the device in the working type (same operation order)
-                nodes = t.arange((Nx + 1) * (Ny + 1) * (Nz + 1), device='cuda')
-                jj = (nodes // (Nx + 1)) % (Ny + 1)
-                Vd = t.zeros(Ndof, dtype=tdt, device='cuda')
-                Vd[0::3] = t.cos(jj.to(tdt) * h / T * 2 * np.pi)
+                Vd = L.empty(Ndof, wd)
+                L.call('vdvc_calibration_mode', c, Nx, Ny, Nz, h, L.rtype_of(Vd), L.ptr(Vd))
                 Hv = L.empty(Ndof, wd)
                 L.call('vdvc_matvec_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]),
                        L.ptr(Gh[2]), gt, L.ptr(mh), L.ptr(Vd), L.ptr(Hv))
@@ -644,7 +647,9 @@ class _Level:
         rk = reg.args() if l0 is not None else (L.REG_NONE, None, None, reg.E, reg.eps)
 
         U = t.zeros(Ndof, dtype=tdt, device='cuda') if U0 is None else U0.to(tdt).clone()
-        res = t.zeros(Nvox, dtype=tdt, device='cuda')   # pitched
+        # pitched residual: every grid voxel is written by each residual pass
+        # (the row padding is never read into a result)
+        res = L.empty(Nvox, wd) if maxiter > 0 else L.zeros(Nvox, wd)
         dU = t.zeros(Ndof, dtype=tdt, device='cuda')
         diag = L.empty(Ndof, wd)
         Minv = L.empty(Ndof, wd)
@@ -653,7 +658,7 @@ class _Level:
         # sampled values: written at every voxel of the grid (the row padding
         # is never read into a result)
         gvals = L.empty(Nvox, wd)
-        mask = L.zeros(Nvox, 'uint8')
+        mask = L.empty(Nvox, 'uint8')
 
         def jacobi(bm):
             Gh, mh = hmask(bm)
@@ -668,7 +673,7 @@ class _Level:
         it_ = (L._i * 1)()
         rr_ = (L._d * 1)()
         for ik in range(maxiter):
-            L.call('vdvc_sample', c, Nx, Ny, Nz, h, L.ptr(U), L.ptr(self.g), self.itype,
+            L.call('vdvc_sample', c, Nx, Ny, Nz, h, L.ptr(U), L.ptr(self.g), self.gitype,
                    self.gshp[0], self.gshp[1], self.gshp[2], float(g_origin[0]),
                    float(g_origin[1]), float(g_origin[2]), float(glt), L.ptr(mask0),
                    L.ptr(frozen), L.ptr(gvals), L.ptr(mask), s3)
