@@ -181,7 +181,12 @@ template <typename T, int V>
 __device__ __forceinline__ void ld_raw(const T* base, long long i, Raw<T, V>& r) {
   const T* p = base + i;
   constexpr int NB = Raw<T, V>::NB;
-  if constexpr (NB == 16) {
+  if constexpr (NB == 32) {
+    const u32x4 q0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    const u32x4 q1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + 1);
+    r.w[0] = q0.x; r.w[1] = q0.y; r.w[2] = q0.z; r.w[3] = q0.w;
+    r.w[4] = q1.x; r.w[5] = q1.y; r.w[6] = q1.z; r.w[7] = q1.w;
+  } else if constexpr (NB == 16) {
     const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
     r.w[0] = q.x; r.w[1] = q.y; r.w[2] = q.z; r.w[3] = q.w;
   } else if constexpr (NB == 8) {
@@ -201,6 +206,12 @@ __device__ __forceinline__ void unpack(const Raw<T, V>& r, float (&o)[V]) {
   if constexpr (sizeof(T) == 4) {
 #pragma unroll
     for (int j = 0; j < V; ++j) o[j] = __uint_as_float(r.w[j]);
+  } else if constexpr (std::is_same<T, __half>::value) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const uint32_t wd = r.w[j / 2];
+      o[j] = __half2float(__ushort_as_half((uint16_t)((j & 1) ? (wd >> 16) : (wd & 0xffffu))));
+    }
   } else {  // bfloat16
 #pragma unroll
     for (int j = 0; j < V; ++j) {
@@ -252,14 +263,17 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
 }  // namespace
 
 // Blocks per CU the register budget is fitted to: 4 (occupancy 4 waves per
-// SIMD, <= 128 VGPRs) for 2-byte gradients, so a 513^3 h=8 grid's 4096
-// wavefronts run in one round; 3 for 4-byte gradients.
+// SIMD, <= 128 VGPRs) for the products with 2-byte gradients, so a 513^3 h=8
+// grid's 4096 wavefronts run in one round; 3 for 4-byte gradients.  The
+// residual pass streams three more voxel arrays (f, sampled g, live mask)
+// through the row pipeline and is fitted to 2 (<= 256 VGPRs): at 4 its
+// registers spilled.
 #ifndef VDVC_VEC_MIN_BLOCKS
-#define VDVC_VEC_MIN_BLOCKS(GT) (sizeof(GT) == 2 ? 4 : 3)
+#define VDVC_VEC_MIN_BLOCKS(MODE, GT) (MODE == MODE_RHS ? 2 : sizeof(GT) == 2 ? 4 : 3)
 #endif
 
 template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V>
-__global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
+__global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
                                                             const float* __restrict__ v,
                                                             float* __restrict__ P) {
   __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 24 : 1];
@@ -389,6 +403,10 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
     Raw<GT, V> g[3];
     Raw<uint8_t, V> m;
     uint32_t t[4];
+    // residual pass: reference image, sampled values, live mask
+    Raw<FT, V> f;
+    Raw<float, V> gv;
+    Raw<uint8_t, V> lv;
   };
   const int NP = (ez1 - ez0) * h + (ez1 == M.Nz ? 1 : 0);  // planes of this tile
   // pipeline depth: 2-byte gradients are unpacked into new registers before
@@ -403,6 +421,11 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
     ld_raw<GT, V>(g1, rb + xbc, b.g[1]);
     ld_raw<GT, V>(g2, rb + xbc, b.g[2]);
     if (HASMASK) ld_raw<uint8_t, V>(in.mask, rb + xbc, b.m);
+    if (MODE == MODE_RHS) {
+      ld_raw<FT, V>(fim, rb + xbc, b.f);
+      ld_raw<float, V>(gvals_f, rb + xbc, b.gv);
+      ld_raw<uint8_t, V>(in.live, rb + xbc, b.lv);
+    }
     if (MODE != MODE_RHS) {
       b.t[0] = ld_raw1<GT>(g0, rb + xend);
       b.t[1] = ld_raw1<GT>(g1, rb + xend);
@@ -496,6 +519,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
     float gx[V], gy[V], gz[V];
     uint32_t mk[V];
     uint32_t t[4];
+    float fv[V], gv[V];
+    uint32_t lv[V];
   };
   auto unpack_row = [&](const RowBuf& b, RowVals& o) {
     unpack<GT, V>(b.g[0], o.gx);
@@ -505,6 +530,12 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
     for (int j2 = 0; j2 < V; ++j2) o.mk[j2] = HASMASK ? mask_bit<V>(b.m, j2) : 1u;
 #pragma unroll
     for (int q = 0; q < 4; ++q) o.t[q] = b.t[q];
+    if (MODE == MODE_RHS) {
+      unpack<FT, V>(b.f, o.fv);
+      unpack<float, V>(b.gv, o.gv);
+#pragma unroll
+      for (int j2 = 0; j2 < V; ++j2) o.lv[j2] = mask_bit<V>(b.lv, j2);
+    }
   };
   auto process = [&](const RowVals& cur) {
     const long long vi = rowbase + (long long)ty * M.pitch;
@@ -528,11 +559,9 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(GT)) void vox_vec_k
     }
     float resv[V];
     if (MODE == MODE_RHS) {
-      float fv[V], gv[V];
-      uint32_t lv[V];
-      ld_any<FT, V>(fim, vi, fv);
-      ld_f32<V>(gvals_f, vi, gv);
-      ld_u8<V>(in.live, vi, lv);
+      const float* fv = cur.fv;
+      const float* gv = cur.gv;
+      const uint32_t* lv = cur.lv;
       const int mxy = axis_mult(y, h, M.Nvy - 1) * mz;
 #pragma unroll
       for (int j = 0; j < V; ++j) {
