@@ -34,6 +34,9 @@ namespace {
 
 using vdvc_img::ld_img;
 
+// HASB: a second mask is given (fixed at compile time, so that no branch
+// separates the walk's loads)
+template <bool HASB>
 __global__ __launch_bounds__(VDVC_BLOCK) void elem_count_kernel(CellIn C) {
   const int lane = threadIdx.x & 63;
   const long long e = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + (threadIdx.x >> 6);
@@ -43,7 +46,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void elem_count_kernel(CellIn C) {
   int n0 = 0, n1 = 0, n2 = 0;
   vdvc_elem::walk(C.h, C.Nvx, C.Nvy, base, lane, [&](long long vi) {
     const bool a0 = C.a[vi] == 0;
-    const bool b1 = C.b ? C.b[vi] != 0 : true;
+    const bool b1 = HASB ? C.b[vi] != 0 : true;
     n0 += a0 ? 1 : 0;
     n1 += b1 ? 1 : 0;
     n2 += (a0 && b1) ? 1 : 0;
@@ -128,22 +131,41 @@ __device__ __forceinline__ void axis_elems(int v, int h, int N, int& lo, int& hi
   lo = (v - q * h == 0 && q > 0) ? q - 1 : hi;
 }
 
+// One wavefront per voxel row (y, z): the flags of the (at most 2 x 2)
+// element rows holding the row are OR-ed per element column into LDS, then
+// each voxel ORs the columns of its (at most two) elements.  Dynamic LDS:
+// Nx bytes per wavefront.
 __global__ __launch_bounds__(VDVC_BLOCK) void unsafe_voxel_kernel(int Nx, int Ny, int Nz, int h,
                                                                   const uint8_t* __restrict__ ue,
                                                                   uint8_t* __restrict__ out) {
-  const int Nvx = Nx * h + 1, Nvy = Ny * h + 1;
-  const int x = blockIdx.x * VDVC_BLOCK + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= Nvx) return;
-  int x0, x1, y0, y1, z0, z1;
-  axis_elems(x, h, Nx, x0, x1);
-  axis_elems(y, h, Ny, y0, y1);
-  axis_elems(z, h, Nz, z0, z1);
-  bool u = false;
-  for (int k = z0; k <= z1; ++k)
-    for (int j = y0; j <= y1; ++j)
-      for (int i = x0; i <= x1; ++i) u = u || ue[i + (long long)Nx * (j + (long long)Ny * k)] != 0;
-  out[(long long)x + (long long)Nvx * ((long long)y + (long long)Nvy * z)] = u ? 1 : 0;
+  extern __shared__ uint8_t colf_all[];
+  const int Nvx = Nx * h + 1, Nvy = Ny * h + 1, Nvz = Nz * h + 1;
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  uint8_t* colf = colf_all + (size_t)wib * Nx;
+  const long long row = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + wib;
+  const bool live = row < (long long)Nvy * Nvz;  // uniform per wavefront
+  int y = 0, z = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  if (live) {
+    y = (int)(row % Nvy);
+    z = (int)(row / Nvy);
+    axis_elems(y, h, Ny, y0, y1);
+    axis_elems(z, h, Nz, z0, z1);
+    for (int i = lane; i < Nx; i += 64) {
+      const uint8_t a = ue[i + (long long)Nx * (y0 + (long long)Ny * z0)];
+      const uint8_t b = ue[i + (long long)Nx * (y1 + (long long)Ny * z0)];
+      const uint8_t c = ue[i + (long long)Nx * (y0 + (long long)Ny * z1)];
+      const uint8_t d = ue[i + (long long)Nx * (y1 + (long long)Ny * z1)];
+      colf[i] = (a | b | c | d) != 0 ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  uint8_t* orow = out + (long long)Nvx * ((long long)y + (long long)Nvy * z);
+  for (int x = lane; x < Nvx; x += 64) {
+    int x0, x1;
+    axis_elems(x, h, Nx, x0, x1);
+    orow[x] = colf[x0] | colf[x1];
+  }
 }
 
 // one Jacobi rotation zeroing A[p][q] of a symmetric 3x3 matrix (r the
@@ -545,7 +567,10 @@ namespace vdvc {
 int launch_elem_count(const CellIn& C, hipStream_t st) {
   const long long Ne = (long long)C.Nx * C.Ny * C.Nz;
   const unsigned nb = (unsigned)((Ne + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
-  hipLaunchKernelGGL(elem_count_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, C);
+  if (C.b)
+    hipLaunchKernelGGL(elem_count_kernel<true>, dim3(nb), dim3(VDVC_BLOCK), 0, st, C);
+  else
+    hipLaunchKernelGGL(elem_count_kernel<false>, dim3(nb), dim3(VDVC_BLOCK), 0, st, C);
   return (int)hipGetLastError();
 }
 
@@ -567,11 +592,12 @@ int launch_elem_flags(const CellFlagsIn& F, hipStream_t st) {
 
 int launch_unsafe_voxels(int Nx, int Ny, int Nz, int h, const uint8_t* unsafe_elem,
                          uint8_t* out, hipStream_t st) {
-  const int Nvx = Nx * h + 1;
-  dim3 g((unsigned)((Nvx + VDVC_BLOCK - 1) / VDVC_BLOCK), (unsigned)(Ny * h + 1),
-         (unsigned)(Nz * h + 1));
-  hipLaunchKernelGGL(unsafe_voxel_kernel, g, dim3(VDVC_BLOCK), 0, st, Nx, Ny, Nz, h, unsafe_elem,
-                     out);
+  const long long rows = (long long)(Ny * h + 1) * (Nz * h + 1);
+  const long long nb = (rows + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK;
+  const size_t lds = (size_t)VDVC_WAVES_PER_BLOCK * Nx;
+  if (nb > 0x7fffffffLL || lds > 65536) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(unsafe_voxel_kernel, dim3((unsigned)nb), dim3(VDVC_BLOCK), lds, st, Nx, Ny, Nz,
+                     h, unsafe_elem, out);
   return (int)hipGetLastError();
 }
 

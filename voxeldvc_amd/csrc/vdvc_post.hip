@@ -186,14 +186,11 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
   }
 }
 
-template <typename F>
-__device__ __forceinline__ void for_elem_voxels(const QualIn& Q, long long base, int lane, F f) {
-  vdvc_elem::walk(Q.h, Q.Nvx, Q.Nvy, base, lane, f);
-}
-
 // One wavefront per element: two sweeps over its (h+1)^3 voxels (the second
 // re-reads them from L2), FP64 sums reduced across the wave in fixed order.
-template <typename RT>
+// FTY: the reference image's type code, fixed at compile time so that the
+// walk's loads are not separated by a type switch
+template <typename RT, int FTY>
 __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
   const int lane = threadIdx.x & 63;
   const long long e = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + (threadIdx.x >> 6);
@@ -206,9 +203,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
       (long long)i * Q.h + (long long)Q.Nvx * ((long long)j * Q.h + (long long)Q.Nvy * k * Q.h);
   const RT* res = reinterpret_cast<const RT*>(Q.res);
   double cnt = 0.0, sr = 0.0, srr = 0.0, sf = 0.0, sg = 0.0;
-  for_elem_voxels(Q, base, lane, [&](long long vi) {
+  vdvc_elem::walk(Q.h, Q.Nvx, Q.Nvy, base, lane, [&](long long vi) {
     const double r = (double)res[vi];
-    const double f = ld_img(Q.ref, Q.ftype, vi);
+    const double f = ld_img(Q.ref, FTY, vi);
     const double m = r != 0.0 ? 1.0 : 0.0;
     const double g = sub_rn(f, r);
     cnt += m;
@@ -227,9 +224,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void quality_cell_kernel(QualIn Q) {
   const double var_r = sub_rn(srr / dc, mul_rn(mean_r, mean_r));
   const double fm = sf / dc, gm = sg / dc;
   double sfg = 0.0, sff = 0.0, sgg = 0.0;
-  for_elem_voxels(Q, base, lane, [&](long long vi) {
+  vdvc_elem::walk(Q.h, Q.Nvx, Q.Nvy, base, lane, [&](long long vi) {
     const double r = (double)res[vi];
-    const double f = ld_img(Q.ref, Q.ftype, vi);
+    const double f = ld_img(Q.ref, FTY, vi);
     const double m = r != 0.0 ? 1.0 : 0.0;
     const double fc = mul_rn(sub_rn(f, fm), m);
     const double gc = mul_rn(sub_rn(sub_rn(f, r), gm), m);
@@ -382,10 +379,21 @@ int launch_post_voxel(int op, int rtype, const PostIn& P, hipStream_t st) {
 int launch_quality_cell(const QualIn& Q, hipStream_t st) {
   const long long Ne = (long long)Q.Nx * Q.Ny * Q.Nz;
   const unsigned nb = (unsigned)((Ne + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
-  if (Q.rtype == GT_F64)
-    hipLaunchKernelGGL(quality_cell_kernel<double>, dim3(nb), dim3(VDVC_BLOCK), 0, st, Q);
-  else
-    hipLaunchKernelGGL(quality_cell_kernel<float>, dim3(nb), dim3(VDVC_BLOCK), 0, st, Q);
+#define VDVC_QC(RT)                                                                        \
+  switch (Q.ftype) {                                                                       \
+    case GT_F32: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_F32>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break; \
+    case GT_F64: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_F64>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break; \
+    case GT_U8: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_U8>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break;   \
+    case GT_U16: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_U16>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break; \
+    case GT_I16: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_I16>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break; \
+    default: hipLaunchKernelGGL((quality_cell_kernel<RT, GT_B8>), dim3(nb), dim3(VDVC_BLOCK), 0, st, Q); break;      \
+  }
+  if (Q.rtype == GT_F64) {
+    VDVC_QC(double)
+  } else {
+    VDVC_QC(float)
+  }
+#undef VDVC_QC
   return (int)hipGetLastError();
 }
 
