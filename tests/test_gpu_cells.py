@@ -14,8 +14,9 @@ from oracle import cells_oracle as CO
 
 pytestmark = pytest.mark.gpu
 
+# (Nx > 64: element columns beyond one wavefront's lanes in the row-wise count)
 MESHES = [(6, 5, 7, 2), (7, 7, 6, 3), (1, 1, 1, 4), (2, 3, 1, 5), (9, 4, 5, 8), (3, 3, 3, 1),
-          (12, 11, 10, 4)]
+          (12, 11, 10, 4), (70, 2, 3, 2), (130, 1, 2, 3)]
 
 
 def _fields(seed, Nx, Ny, Nz, h):

@@ -62,6 +62,52 @@ __global__ __launch_bounds__(VDVC_BLOCK) void elem_count_kernel(CellIn C) {
   }
 }
 
+// The same counts with one block per element row (ey, ez): the block's four
+// wavefronts take every fourth voxel row (qy, qz) of the row's (h+1)^2, a
+// lane takes element columns lane, lane + 64, ... and reads its element's
+// h+1 voxels of the row, so each voxel row is read by one block with
+// neighbouring lanes on neighbouring bytes (the per-element kernel above
+// fetches a cache line once per element it spans).  Integer totals are
+// combined with LDS atomics (exact, so the result is order-independent).
+// Dynamic LDS: 3 * Nx ints.
+template <bool HASB>
+__global__ __launch_bounds__(VDVC_BLOCK) void elem_row_count_kernel(CellIn C) {
+  extern __shared__ int rc_lds[];
+  const int Nx = C.Nx, h = C.h, h1 = C.h + 1;
+  const int ey = (int)(blockIdx.x % C.Ny), ez = (int)(blockIdx.x / C.Ny);
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int q = threadIdx.x; q < 3 * Nx; q += VDVC_BLOCK) rc_lds[q] = 0;
+  __syncthreads();
+  const int nrows = h1 * h1;
+  for (int ex = lane; ex < Nx; ex += 64) {
+    int n0 = 0, n1 = 0, n2 = 0;
+    for (int r = grp; r < nrows; r += VDVC_WAVES_PER_BLOCK) {
+      const int qz = r / h1, qy = r - qz * h1;
+      const long long vb = (long long)ex * h +
+                           (long long)C.Nvx * ((long long)(ey * h + qy) + (long long)C.Nvy * (ez * h + qz));
+      for (int x = 0; x < h1; ++x) {
+        const bool a0 = C.a[vb + x] == 0;
+        const bool b1 = HASB ? C.b[vb + x] != 0 : true;
+        n0 += a0 ? 1 : 0;
+        n1 += b1 ? 1 : 0;
+        n2 += (a0 && b1) ? 1 : 0;
+      }
+    }
+    atomicAdd(&rc_lds[ex], n0);
+    atomicAdd(&rc_lds[Nx + ex], n1);
+    atomicAdd(&rc_lds[2 * Nx + ex], n2);
+  }
+  __syncthreads();
+  const long long e0 = (long long)Nx * ((long long)ey + (long long)C.Ny * ez);
+  for (int ex = threadIdx.x; ex < Nx; ex += VDVC_BLOCK) {
+    const int n0 = rc_lds[ex], n1 = rc_lds[Nx + ex], n2 = rc_lds[2 * Nx + ex];
+    if (C.c0) C.c0[e0 + ex] = n0;
+    if (C.c1) C.c1[e0 + ex] = n1;
+    if (C.c2) C.c2[e0 + ex] = n2;
+    if (C.all_a0) C.all_a0[e0 + ex] = n0 == h1 * h1 * h1 ? 1 : 0;
+  }
+}
+
 __device__ __forceinline__ bool on_shell(int i, int j, int k, int Nx, int Ny, int Nz) {
   return i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1;
 }
@@ -566,6 +612,17 @@ namespace vdvc {
 
 int launch_elem_count(const CellIn& C, hipStream_t st) {
   const long long Ne = (long long)C.Nx * C.Ny * C.Nz;
+  const long long rows = (long long)C.Ny * C.Nz;
+  if (C.Nx <= 4096 && rows <= 0x7fffffffLL) {
+    const size_t lds = (size_t)3 * C.Nx * sizeof(int);
+    if (C.b)
+      hipLaunchKernelGGL(elem_row_count_kernel<true>, dim3((unsigned)rows), dim3(VDVC_BLOCK), lds,
+                         st, C);
+    else
+      hipLaunchKernelGGL(elem_row_count_kernel<false>, dim3((unsigned)rows), dim3(VDVC_BLOCK), lds,
+                         st, C);
+    return (int)hipGetLastError();
+  }
   const unsigned nb = (unsigned)((Ne + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
   if (C.b)
     hipLaunchKernelGGL(elem_count_kernel<true>, dim3(nb), dim3(VDVC_BLOCK), 0, st, C);
