@@ -136,7 +136,8 @@ def test_zncc(gpu):
 
 @pytest.mark.parametrize('fdtype', [np.float32, np.uint8])
 @pytest.mark.parametrize('rdtype', [np.float32, np.float64])
-@pytest.mark.parametrize('shape', [(3, 2, 2, 4), (2, 2, 3, 8), (2, 1, 1, 16), (4, 3, 2, 1)])
+@pytest.mark.parametrize('shape', [(3, 2, 2, 4), (2, 2, 3, 8), (2, 1, 1, 16), (4, 3, 2, 1),
+                                   (70, 2, 1, 2)])
 def test_quality_cell(gpu, shape, rdtype, fdtype):
     from voxeldvc_amd.engine import write_output as W
     Nx, Ny, Nz, h = shape
