@@ -9,9 +9,10 @@ Device arrays are torch tensors (memory ownership only).
 Precision (the `storage` keyword, an extension):
   'fp32'  float32 images and gradients plus a uint8 mask (13 B/voxel in the
           Hessian matvec), float32 matvec arithmetic, FP64 dots and norms;
-  'bf16'  float16 images (exact for 8-bit data), bfloat16 gradients with the
-          mask folded in (6 B/voxel), same arithmetic; exact for 8-bit
-          integer images, whose gradients are multiples of 1/2 below 256;
+  'bf16'  float16 reference image (exact for 8-bit data; a float32 deformed
+          image is sampled as it is, float16 otherwise), bfloat16 gradients
+          with the mask folded in (6 B/voxel), same arithmetic; exact for
+          8-bit integer images, whose gradients are multiples of 1/2 below 256;
   'fp64'  selected by dtype=np.float64 (the reference's oracle mode,
           correlate_gpu.py:910-914): float64 images, gradients, nodal
           vectors, residuals and PCG, through the generic voxel kernel.
