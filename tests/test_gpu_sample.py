@@ -9,8 +9,10 @@ from oracle import dvc_oracle as O
 pytestmark = pytest.mark.gpu
 
 # (Nx, Ny, Nz, h): several 64-voxel strips, odd and even rows per element,
-# one-voxel elements, the last element row's extra row
-SHAPES = [(10, 3, 2, 8), (5, 3, 2, 3), (70, 1, 2, 1), (4, 2, 3, 5), (2, 2, 2, 16)]
+# one-voxel elements, the last element row's extra row; Nx h = 64 k (65 and
+# 129 voxel columns), where the last column is folded into the last strip
+SHAPES = [(10, 3, 2, 8), (5, 3, 2, 3), (70, 1, 2, 1), (4, 2, 3, 5), (2, 2, 2, 16),
+          (8, 3, 2, 8), (32, 1, 2, 4)]
 
 
 def _run_sample(U, g, gcode, shp_g, Nx, Ny, Nz, h, go, m0, bm, glt):

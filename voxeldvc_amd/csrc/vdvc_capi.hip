@@ -684,9 +684,17 @@ int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, c
   if (!d_U || !d_g || !d_gvals || !d_mask_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   TilesDev T = make_tiles_generic(M);
+  // Nvx = 64 k + 1 (e.g. 513): the one-column last strip is folded into the
+  // strip before it rather than given wavefronts with one active lane
+  const int tail_fold = (M.Nvx > 64 && (M.Nvx - 1) % 64 == 0) ? 1 : 0;
+  if (tail_fold) {
+    T.nsx = (M.Nvx - 1) / 64;
+    plan_z(M, T, 16384);
+  }
   if ((rc = ctx->stats.ensure((size_t)T.nwaves * 3 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   SampleIn S;
+  S.tail_fold = tail_fold;
   S.gimg = d_g; S.gnx = gnx; S.gny = gny; S.gnz = gnz;
   S.gox = gox; S.goy = goy; S.goz = goz; S.glt = glt;
   S.mask0 = d_mask0; S.bmask = d_bmask; S.gvals = d_gvals; S.mask_out = d_mask_out;

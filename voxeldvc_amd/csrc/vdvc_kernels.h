@@ -28,6 +28,7 @@ struct SampleIn {
   void* gvals;           // R
   uint8_t* mask_out;
   double* stats;         // per-wave partials [nwaves][3]
+  int tail_fold;         // the last strip's wavefronts also take column Nvx - 1
 };
 
 // post-solve voxel stages on dense voxel arrays (flat index x + Nvx*(y + Nvy*z))

@@ -559,6 +559,70 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
       }
     }
   }
+  if (S.tail_fold && sx == T.nsx - 1) {
+    // The last voxel column x = Nvx - 1 (a strip of its own would hold one
+    // active lane) is taken by the last strip's wavefront, its lanes spread
+    // over the column's (row, plane) pairs of each element layer.  Per voxel
+    // the same operations as the main loop with x-weights X0 = 1, X1 = 0 on
+    // node column Nx.
+    const int xt = M.Nvx - 1;
+    const R vxt = (R)xt;
+    const int mxt = axis_mult(xt, h, M.Nvx - 1);
+    for (int ez = ez0; ez < ez1; ++ez) {
+      const int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
+      R Ut[2][2][3];
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+          const long long nt = node_index(M, M.Nx, ey + cy, ez + cz);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) Ut[cy][cz][k] = fma(R(1), U[3 * nt + k], R(0) * U[3 * nt + k]);
+        }
+      const int npairs = nzp * nrows;
+      for (int q = lane; q < npairs; q += 64) {
+        const int tz = q / nrows, ty = q - tz * nrows;
+        const int z = ez * h + tz, y = y0 + ty;
+        const R Z1 = tz * inv_h, Z0 = R(1) - Z1;
+        R ut[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const R a0 = fma(Z0, Ut[0][0][k], Z1 * Ut[0][1][k]);
+          const R a1 = fma(Z0, Ut[1][0][k], Z1 * Ut[1][1][k]);
+          const R d = (a1 - a0) * inv_h;
+          ut[k] = fma((R)ty, d, a0);
+        }
+        R cx = sub_rn(add_rn(ut[0], vxt), gox);
+        R cy = sub_rn(add_rn(ut[1], (R)y), goy);
+        R cz = sub_rn(add_rn(ut[2], (R)z), goz);
+        const bool inb = (cx >= R(0)) && (cx <= fgnx) && (cy >= R(0)) && (cy <= fgny) &&
+                         (cz >= R(0)) && (cz <= fgnz);
+        cx = clamp_to(cx, fgnx);
+        cy = clamp_to(cy, fgny);
+        cz = clamp_to(cz, fgnz);
+        if (!(cx == cx)) cx = R(0);
+        if (!(cy == cy)) cy = R(0);
+        if (!(cz == cz)) cz = R(0);
+        TriTaps<GI> tp;
+        tri_fetch<O32>(img, rs, S.gnx, S.gny, S.gnz, cx, cy, cz, tp);
+        const long long vi = ((long long)z * M.Nvy + y) * M.pitch + xt;
+        const R gv = (R)tri_combine(tp);
+        const bool m0 = mask0 ? (mask0[vi] != 0) : true;
+        const bool mk = inb && m0 && (gv > glt);
+        gvals[vi] = gv;
+        mask_out[vi] = mk ? 1 : 0;
+        const bool bmv = bmask ? (bmask[vi] != 0) : mk;
+        if (bmv) {
+          const double mlt =
+              (double)(mxt * axis_mult(z, h, M.Nvz - 1) * axis_mult(y, h, M.Nvy - 1));
+          const double gd = (double)gv;
+          sc += mlt;
+          s1 += mlt * gd;
+          s2 += mlt * gd * gd;
+        }
+      }
+    }
+  }
   sc = wave_sum(sc);
   s1 = wave_sum(s1);
   s2 = wave_sum(s2);
