@@ -47,10 +47,43 @@ def parse():
                    help='skip the post-solve voxel stages')
     p.add_argument('--no-affine', action='store_true',
                    help='skip the preprocess affine stage')
+    p.add_argument('--solve-only', choices=('plain', 'learned'), default=None,
+                   help='run one GN solve of the time-to-solve pair (after a warm-up solve) '
+                        'and exit: for kernel traces of a whole solve')
     p.add_argument('--configs', action='store_true',
                    help='also measure the other benchmark configurations (C2, C3 at h=4, C5); '
                         'several minutes')
     return p.parse_args()
+
+
+def _timed_trace(solve_fn, k_target, reps):
+    """Median over `reps` runs of (ms to the k_target-th GN update, ms to the
+    end of the solve, the solve's result): every GN update records a HIP
+    event on the library's stream (the callback runs right after U += dU),
+    so the time to an iterate carries no host synchronisation of its own."""
+    import torch
+    runs = []
+    for _ in range(reps):
+        evs = []
+        st = torch.cuda.current_stream()
+
+        def cb(s, ik, U):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(st)
+            evs.append(e)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(st)
+        out = solve_fn(cb)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3
+        t_k = e0.elapsed_time(evs[k_target - 1]) if k_target and k_target <= len(evs) else None
+        runs.append((t_k, wall, out))
+    order = np.argsort([r[1] for r in runs])
+    mid = runs[int(order[len(runs) // 2])]
+    tk = sorted(r[0] for r in runs if r[0] is not None)
+    return (tk[len(tk) // 2] if tk else None), mid[1], mid[2]
 
 
 def time_to_solve(n=513, h=8, reps=3, log=None):
@@ -62,13 +95,15 @@ def time_to_solve(n=513, h=8, reps=3, log=None):
     * oracle: the same solver in float64 (dtype=np.float64), converged to
       dU/U < 1e-6, defines the answer;
     * plain: 3-level multiscale (scales 2,1,0) with FFT pre-alignment,
-      float32, the reference's default convergence (dU/U < 1e-3);
+      float32;
     * learned + mixed: the seed network's prediction starts the finest level
       (no coarse levels), float16 images / bfloat16 gradients, float32
-      matvec arithmetic, FP64 dots and norms; timed to the first GN
-      iterate within 0.01 voxel RMS of the oracle, and to its own
-      convergence.
-    Device-resident inputs; each timing is the median of `reps` solves."""
+      matvec arithmetic, FP64 dots and norms.
+    Both solves are timed the same way: from device-resident f and g to the
+    first finest-level GN iterate within 0.01 voxel RMS of the oracle (the
+    iterate is found by a traced run; the timed runs record a HIP event at
+    every GN update), and to the solver's own convergence (dU/U < 1e-3).
+    Each timing is the median of `reps` solves."""
     import torch
     from voxeldvc_amd import seednet, synth
     from voxeldvc_amd.engine import correlate_gpu as C
@@ -82,38 +117,41 @@ def time_to_solve(n=513, h=8, reps=3, log=None):
     kw = dict(K_ref_laplacian=build_K_ref_laplacian(), l0=4.5 * h, dynamic_mask=True,
               freeze_mask_after=4, glt=0.0, disp=False, pcg_tol=1e-6, pcg_max_iter=2000)
     net = seednet.SeedNet.load()
+    plain = dict(maxiter=30, eps=1e-3)
+    learned = dict(maxiter=30, eps=1e-3, storage='bf16', scales=(0,), learned_seed=net)
 
-    def solve(**extra):
-        lg = []
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        U, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, log=lg,
-                                             **{**kw, **extra})
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) * 1e3
+    def run(extra, cb=None, lg=None):
+        return C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, log=lg, callback=cb,
+                                          **{**kw, **extra})[0]
+
+    t0 = time.perf_counter()
+    lg_o = []
+    Uo = run(dict(dtype=np.float64, maxiter=60, eps=1e-6), lg=lg_o).double()
+    torch.cuda.synchronize()
+    t_o = (time.perf_counter() - t0) * 1e3
+    rms = lambda U: float(((U.double() - Uo) ** 2).mean().sqrt())
+
+    def trace(extra):
+        """(per-update (scale, rms) list, GN and PCG counts) of one solve."""
+        tr, lg = [], []
+        U = run(extra, lambda s, ik, U: tr.append((s, rms(U))), lg)
         gn = sum(len(lv['iters']) for lv in lg)
         pcg = sum(q['pcg_iters'] for lv in lg for q in lv['iters'])
-        return U.double(), dt, gn, pcg
+        return tr, gn, pcg, rms(U)
 
-    def timed(**extra):
-        runs = [solve(**extra) for _ in range(reps)]
-        i = int(np.argsort([r[1] for r in runs])[len(runs) // 2])
-        return runs[i]
+    def first_within(tr, tol=0.01):
+        return next((i + 1 for i, (s, r) in enumerate(tr) if s == 0 and r <= tol), None)
 
-    Uo, t_o, gn_o, _ = solve(dtype=np.float64, maxiter=60, eps=1e-6)
-    rms = lambda U: float(((U - Uo) ** 2).mean().sqrt())
-    solve(maxiter=30, eps=1e-3)     # warm-up (level buffers, kernels)
-    Up, t_p, gn_p, pcg_p = timed(maxiter=30, eps=1e-3)
-    trace = []
-    solve(maxiter=30, eps=1e-3, storage='bf16', scales=(0,), learned_seed=net,
-          callback=lambda s, ik, U: trace.append(rms(U.double())))
-    k = next((i + 1 for i, r in enumerate(trace) if r <= 0.01), None)
-    if k is None:
-        raise RuntimeError('learned-seed solve did not reach 0.01 vox RMS: %r' % trace)
-    Ul, t_l, gn_l, pcg_l = timed(maxiter=k, eps=1e-3, storage='bf16', scales=(0,),
-                                 learned_seed=net)
-    Uc, t_c, gn_c, pcg_c = timed(maxiter=30, eps=1e-3, storage='bf16', scales=(0,),
-                                 learned_seed=net)
+    run(plain)                      # warm-up (level buffers, kernels)
+    tr_p, gn_p, pcg_p, rms_p = trace(plain)
+    k_p = first_within(tr_p)
+    tr_l, gn_l, pcg_l, rms_l = trace(learned)
+    k_l = first_within(tr_l)
+    if k_p is None or k_l is None:
+        raise RuntimeError('a solve did not reach 0.01 vox RMS: plain %r, learned %r'
+                           % (tr_p, tr_l))
+    tp01, tp, _ = _timed_trace(lambda cb: run(plain, cb), k_p, reps)
+    tl01, tl, _ = _timed_trace(lambda cb: run(learned, cb), k_l, reps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     Un = net.predict(f, g, (n, n, n), Ne, Ne, Ne, h)
@@ -122,19 +160,52 @@ def time_to_solve(n=513, h=8, reps=3, log=None):
     return {
         "workload": "%d^3 8-bit speckle, 16 Fourier modes (wl 64-256, peak 4 vox) + N(0,1), "
                     "h=%d (%d^3 elements), Laplacian l0=%.0f" % (n, h, Ne, 4.5 * h),
-        "criterion": "nodal RMS <= 0.01 vox vs the float64 solve (dU/U < 1e-6)",
-        "oracle_fp64": {"ms": round(t_o, 2), "gn_iters": gn_o},
-        "plain_fp32": {"ms": round(t_p, 2), "gn_iters": gn_p, "pcg_iters": pcg_p,
-                       "rms_vox": rms(Up), "levels": "2,1,0 + FFT prealign"},
-        "learned_mixed": {"ms_to_0.01": round(t_l, 2), "gn_iters": gn_l, "pcg_iters": pcg_l,
-                          "rms_vox": rms(Ul), "ms_to_convergence": round(t_c, 2),
-                          "gn_iters_to_convergence": gn_c, "rms_vox_converged": rms(Uc),
-                          "network_ms": round(t_net, 2), "network_rms_vox": rms(Un.double()),
-                          "per_iter_rms": [float('%.3g' % r) for r in trace]},
-        "ratio_to_0.01": round(t_l / t_p, 4),
-        "ratio_converged": round(t_c / t_p, 4),
+        "criterion": "first finest-level GN iterate with nodal RMS <= 0.01 vox vs the float64 "
+                     "solve (dU/U < 1e-6); both solves timed to it the same way (HIP event at "
+                     "every GN update, median of %d)" % reps,
+        "oracle_fp64": {"ms": round(t_o, 2), "gn_iters": sum(len(lv['iters']) for lv in lg_o)},
+        "plain_fp32": {"ms_to_0.01": round(tp01, 2), "gn_to_0.01": k_p,
+                       "ms_to_convergence": round(tp, 2), "gn_iters": gn_p,
+                       "pcg_iters": pcg_p, "rms_vox_converged": rms_p,
+                       "levels": "2,1,0 + FFT prealign",
+                       "per_iter_rms_finest": [float('%.3g' % r) for s, r in tr_p if s == 0]},
+        "learned_mixed": {"ms_to_0.01": round(tl01, 2), "gn_to_0.01": k_l,
+                          "ms_to_convergence": round(tl, 2), "gn_iters": gn_l,
+                          "pcg_iters": pcg_l, "rms_vox_converged": rms_l,
+                          "network_ms": round(t_net, 2), "network_rms_vox": rms(Un),
+                          "per_iter_rms": [float('%.3g' % r) for s, r in tr_l]},
+        "ratio_to_0.01": round(tl01 / tp01, 4),
+        "ratio_converged": round(tl / tp, 4),
         "target_ratio": 1.0 / 3.0,
     }
+
+
+def solve_only(kind, n=513, h=8):
+    """One plain FP32 multiscale or learned-seed BF16 solve of the
+    time-to-solve pair (after one untimed warm-up solve), for kernel traces
+    of a whole solve; prints its wall time."""
+    import torch
+    from voxeldvc_amd import seednet, synth
+    from voxeldvc_amd.engine import correlate_gpu as C
+    from voxeldvc_amd.engine.geometry_dvc import build_K_ref_laplacian
+    Ne = (n - 1) // h
+    f = synth.speckle_torch((n, n, n), synth.blobs_for((n, n, n)), seed=3).round().clamp_(0, 255)
+    modes = synth.fourier_params(16, (64, 256), 4.0, seed=7, extent=(n, n, n))
+    g = synth.warp_backward_torch(f, (n, n, n), modes, 1.0, seed=11, quantize=True)
+    fv = f.view(n, n, n).permute(2, 1, 0)
+    gv = g.view(n, n, n).permute(2, 1, 0)
+    kw = dict(K_ref_laplacian=build_K_ref_laplacian(), l0=4.5 * h, dynamic_mask=True,
+              freeze_mask_after=4, glt=0.0, disp=False, pcg_tol=1e-6, pcg_max_iter=2000,
+              maxiter=30, eps=1e-3)
+    if kind == 'learned':
+        kw.update(storage='bf16', scales=(0,), learned_seed=seednet.SeedNet.load())
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, **kw)
+        torch.cuda.synchronize()
+        print('%s solve %s: %.2f ms' % (kind, 'warm-up' if rep == 0 else 'timed',
+                                         (time.perf_counter() - t0) * 1e3), flush=True)
 
 
 def _matvec_kernel_ms(n_el, h, G, gt, mask, reps=20):
@@ -649,6 +720,47 @@ def cpu_baseline():
                       "warm-up, %.2f s each; %d host cores available" % (t, os.cpu_count())}
 
 
+def cpu_gn_solve():
+    """The plain method's whole solve at its CPU-runnable configuration (65^3
+    speckle, translation + 1 % z-strain, h = 8, Laplacian l0 = 36, 3 levels,
+    FFT pre-alignment, dynamic mask frozen after 4 iterations, float32): the
+    oracle's NumPy GN + PCG (the reference algorithm) on one host thread,
+    beside the device solve of the same inputs."""
+    import torch
+    from threadpoolctl import threadpool_limits
+    from oracle import dvc_oracle as O
+    from voxeldvc_amd import synth
+    from voxeldvc_amd.engine import correlate_gpu as C
+    f, g = synth.c1_pair()
+    f, g = f.astype(np.float32), g.astype(np.float32)
+    kw = dict(K_ref_laplacian=O.build_K_ref_laplacian(), l0=36.0, glt=0.0, dynamic_mask=True,
+              freeze_mask_after=4)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        U_cpu, _, _ = O.multiscale_correlate(f, g, 8, 8, 8, 8, scales=(2, 1, 0), **kw)
+        t_cpu = time.perf_counter() - t0
+    fd = torch.from_numpy(f).cuda()
+    gd = torch.from_numpy(g).cuda()
+    C.multiscale_correlate_gpu(fd, gd, 8, 8, 8, 8, torch, disp=False, **kw)
+    ts = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lg = []
+        U, _, _ = C.multiscale_correlate_gpu(fd, gd, 8, 8, 8, 8, torch, disp=False, log=lg, **kw)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gn = sum(len(lv['iters']) for lv in lg)
+    pcg = sum(q['pcg_iters'] for lv in lg for q in lv['iters'])
+    t_gpu = float(np.median(ts))
+    return {"value": round(t_cpu * 1e3, 1), "unit": "ms per GN solve", "cores": 1,
+            "kind": "port", "higher_is_better": False,
+            "sample": "the whole plain-method solve at 65^3 (3 levels, FFT pre-alignment), once; "
+                      "%d host cores available, 1 used" % os.cpu_count(),
+            "gpu_ms": round(t_gpu * 1e3, 2), "gn_iters": gn, "pcg_iters": pcg,
+            "max_abs_vs_cpu_vox": float(np.abs(U.double().cpu().numpy() - U_cpu).max())}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -659,6 +771,9 @@ def main():
     from voxeldvc_amd import _lib as L
     from voxeldvc_amd import synth
     assert torch.cuda.is_available(), 'bench needs a GPU'
+    if a.solve_only:
+        solve_only(a.solve_only, n=a.size, h=a.h)
+        return
     n_el = (a.size - 1) // a.h
     h = a.h
     Nv = n_el * h + 1
@@ -686,13 +801,18 @@ def main():
                 L.ptr(v), L.ptr(out))
         for _ in range(warm):
             L.call('vdvc_matvec_H', *args)
+        # the timed steps: the operator calls alone, nothing else on the stream
         torch.cuda.synchronize()
-        L.call('vdvc_timing_begin', c)
         t0 = time.perf_counter()
         for _ in range(steps):
             L.call('vdvc_matvec_H', *args)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        # the voxel kernel's own duration: the same calls again with HIP
+        # events around each launch of it
+        L.call('vdvc_timing_begin', c)
+        for _ in range(steps):
+            L.call('vdvc_matvec_H', *args)
         tot = (L._d * 1)()
         nl = (L._i * 1)()
         L.call('vdvc_timing_end', c, tot, nl)
@@ -762,6 +882,8 @@ def main():
     if not a.no_cpu:
         print('cpu baseline', file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
+        print('cpu gn solve', file=sys.stderr, flush=True)
+        res["cpu_baseline"]["gn_solve_c1"] = cpu_gn_solve()
     print(json.dumps(res), flush=True)
 
 

@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 5
+#define VDVC_ABI_VERSION 6
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -431,6 +431,51 @@ int vdvc_warp_affine(vdvc_ctx* ctx, int nx, int ny, int nz, const float* d_g, in
  * axis projections of compute_crop (preprocess.py:315-346). */
 int vdvc_overlap_bounds(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_f, int ftype,
                         const float* d_gw, double glt, int* h_lo3, int* h_hi3);
+
+/* ---- data movement at the solve's boundary --------------------------------
+ * Element-type conversions and copies the engine needs between the caller's
+ * arrays and the level layout.  They replace the xp.asarray(..., dtype)
+ * conversions of the reference's engine boundary (engine/correlate_gpu.py:
+ * 938-939), its per-level copies and the float64 promotion of its solve
+ * vectors (:1042-1047); no arithmetic beyond the conversions.  Types: any
+ * VDVC_* element type; conversions round to nearest even, float16 and
+ * bfloat16 from float64 without double rounding; integer targets truncate. */
+
+/* dst[i] = src[i] converted, i < n. */
+int vdvc_convert(vdvc_ctx* ctx, int64_t n, const void* d_src, int stype, void* d_dst,
+                 int dtype);
+
+/* Byte copy of nbytes (non-overlapping buffers). */
+int vdvc_copy(vdvc_ctx* ctx, int64_t nbytes, const void* d_src, void* d_dst);
+
+/* dst[i] = value converted to dtype, i < n. */
+int vdvc_fill(vdvc_ctx* ctx, int64_t n, int dtype, double value, void* d_dst);
+
+/* Strided volume copy with conversion: dst[x + dpitch*(y + ny*z)] =
+ * src[x*sx + y*sy + z*sz] (strides in elements) for x < nx, and 0 in the
+ * row padding nx <= x < dpitch.  With the source strides of a C-order
+ * [x][y][z] array (ny*nz, nz, 1) it is the reference's ravel(order='F')
+ * into the level layout (engine/correlate_gpu.py:50-52, 60); ny, nz <= 65535. */
+int vdvc_copy_volume(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_src, int stype,
+                     int64_t sx, int64_t sy, int64_t sz, void* d_dst, int dtype, int dpitch);
+
+/* Nodal field of an (Mx, My, Mz)-element mesh of element size hs resampled
+ * onto an (Nx, Ny, Nz)-element mesh of element size hd covering the same
+ * voxels (M*hs == N*hd per axis): each destination node takes the Hex8
+ * (trilinear) interpolation of the source field at its voxel position
+ * (float64 weights; node picks when hd is a multiple of hs).  Both vectors
+ * in rtype, the reference's DOF order.  The U <- U * (prev_s / s) carry-over
+ * between pyramid levels (engine/correlate_gpu.py:1263) keeps one mesh; this
+ * is its counterpart between element sizes, used to seed a mesh from the
+ * learned network's h = 8 prediction. */
+int vdvc_resample_nodes(vdvc_ctx* ctx, int Mx, int My, int Mz, int hs, const void* d_src,
+                        int Nx, int Ny, int Nz, int hd, void* d_dst, int rtype);
+
+/* y = a*x + b*y with x of xtype (F32/F16/BF16/F64) and y of ytype (F32/F64),
+ * evaluated in float64 and rounded once to ytype (a float32 correction
+ * accumulated into a float64 vector is exact before that rounding). */
+int vdvc_axpby_x(vdvc_ctx* ctx, int64_t n, double a, const void* d_x, int xtype, double b,
+                 void* d_y, int ytype);
 
 #ifdef __cplusplus
 }

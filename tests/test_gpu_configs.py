@@ -13,12 +13,14 @@ configuration is bench.py's; these are the others):
       does the same and is reproduced here), so the C2 workload is solved
       with the pyramid carried by the Laplacian and the equilibrium gap on
       the finest level; FP32 against FP64 and the analytic field;
-  C3  the precision form at h = 4 (257^3 8-bit speckle, 16 Fourier modes,
-      peak 4 vox): FP32 and FP16/BF16 solves against the FP64 solve, and the
-      FP32-against-FP64 refinement;
-  C5  the combined form (necking + 8 Fourier modes up to 8 vox + N(0,2),
-      h = 16, equilibrium gap on the finest level) at 257 x 257 x 193: the
-      learned seed with mixed precision reaches 0.01 vox of the FP64 solve.
+  C3  the precision form at full size (513^3 8-bit speckle, 16 Fourier
+      modes, peak 4 vox) at h = 8 and h = 4: FP32 and FP16/BF16 solves
+      against the FP64 solve, the learned seed with BF16 storage, and the
+      FP32-against-FP64 refinement (h = 8);
+  C5  the combined form at full size (1025 x 1025 x 769, necking + 8
+      Fourier modes up to 8 vox + N(0,2), h = 16, equilibrium gap on the
+      finest level): the learned seed with mixed precision reaches 0.01 vox
+      of the FP64 solve of a 257^3 subvolume.
 """
 import hashlib
 import os
@@ -88,7 +90,7 @@ def test_c1_plain_method_matches_reference(gpu, pre):
     print('  rms vs reference %.2e (max %.2e), vs analytic field (interior) %.4f vox, '
           'active %d vs %d' % (e_ref, d_max, e_true, np.count_nonzero(r),
                                int(want['res_nonzero'])), flush=True)
-    assert e_ref < 1e-3 and d_max < 5e-3
+    assert e_ref < 1e-3 and d_max < 1e-3
     assert abs(np.count_nonzero(r) - int(want['res_nonzero'])) <= 1e-3 * r.size
     assert r[r != 0].std() == pytest.approx(float(want['res_std']), rel=1e-3)
     assert e_true < 0.05 and _rms(want['U'], truth, inner) < 0.05
@@ -117,14 +119,14 @@ def test_eqgap_pyramid_matches_reference_golden(gpu):
     assert e < 1e-5
 
 
-def _lap_then_eqgap(C, torch, fv, gv, N, h, extra, scales=(3, 2, 1, 0)):
+def _lap_then_eqgap(C, torch, fv, gv, N, h, extra, scales=(3, 2, 1, 0), ms_extra=None):
     """The C2 / C5 workload: Laplacian pyramid, then the equilibrium gap on
     the finest level from its result."""
     lap = dict(K_ref_laplacian=O.build_K_ref_laplacian(), l0=4.5 * h, glt=0.0,
                dynamic_mask=True, freeze_mask_after=4, disp=False)
     eqg = dict(lap, K_ref_elastic=O.build_K_ref(0.3), reg_type='equilibrium_gap')
     U0, _, _ = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, scales=scales,
-                                          **lap, **extra)
+                                          **lap, **extra, **(ms_extra or {}))
     U, _, _ = C.correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, U0=U0, **eqg, **extra)
     return U
 
@@ -157,75 +159,157 @@ def test_c2_necking_equilibrium_gap(gpu):
     assert et < 0.15
 
 
-def test_c3_precision_form_h4(gpu):
-    import torch
+def _cube(kind, n, seed):
     from voxeldvc_amd import synth
+    f, g, modes, extra = synth.config_pair_torch(kind, (n, n, n), seed=seed)
+    return f, g, f.view(n, n, n).permute(2, 1, 0), g.view(n, n, n).permute(2, 1, 0), modes, extra
+
+
+def _trms(a, b):
+    return float(((a.double() - b.double()) ** 2).mean().sqrt())
+
+
+def _c3_common(h):
+    """C3 at full size: 513^3 8-bit speckle warped by 16 Fourier modes
+    (wavelengths 64-256, peak 4 vox) + N(0,1), quantised; Laplacian
+    l0 = 4.5 h, dynamic mask frozen after 4 iterations, 3 levels + FFT
+    pre-alignment.  FP32 and BF16 solves within 1e-3 vox RMS of the FP64
+    solve; the learned seed with BF16 storage reaches 0.01 vox in <= 3 GN."""
+    import torch
+    from voxeldvc_amd import seednet
     from voxeldvc_amd.engine import correlate_gpu as C
-    n, h = 257, 4
+    n = 513
     Ne = (n - 1) // h
-    f, g, modes, _ = synth.config_pair_torch('c3', (n, n, n), seed=3)
-    fv = f.view(n, n, n).permute(2, 1, 0)
-    gv = g.view(n, n, n).permute(2, 1, 0)
+    f, g, fv, gv, _, _ = _cube('c3', n, 3)
     kw = dict(K_ref_laplacian=O.build_K_ref_laplacian(), l0=4.5 * h, glt=0.0,
               dynamic_mask=True, freeze_mask_after=4, disp=False)
-    print('C3 257^3 h=4', flush=True)
+    print('C3 513^3 h=%d' % h, flush=True)
+    t0 = time.perf_counter()
     U64, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, dtype=np.float64,
                                            maxiter=60, eps=1e-6, **kw)
-    U64 = U64.cpu().numpy()
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
     out = {}
     for storage in ('fp32', 'bf16'):
-        U, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, storage=storage,
-                                             **kw)
-        out[storage] = _rms(U.cpu().numpy(), U64)
-    log = []
-    Ur, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, storage='bf16',
-                                          refine_fp64=2, log=log, **kw)
-    res = [q['residual_fp64'] for lv in log for q in lv['iters'] if 'refine' in q]
-    truth = synth.nodal_truth(modes, None, Ne, Ne, Ne, h)
-    et = _rms(U64, truth, _interior_dofs(Ne, Ne, Ne, margin=2))
-    print('  rms vs FP64 solve: fp32 %.2e, bf16 %.2e, bf16 + refinement %.2e vox; FP64 '
-          'residual %s; FP64 solve vs analytic field %.4f vox'
-          % (out['fp32'], out['bf16'], _rms(Ur.cpu().numpy(), U64),
-             ['%.2e' % r for r in res], et), flush=True)
+        U, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, storage=storage, **kw)
+        out[storage] = _trms(U, U64)
+    net = seednet.SeedNet.load()
+    trace = []
+    C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, storage='bf16', scales=(0,),
+                               learned_seed=net, maxiter=6,
+                               callback=lambda s, ik, U: trace.append(_trms(U, U64)), **kw)
+    k = next((i + 1 for i, r in enumerate(trace) if r <= 0.01), None)
+    print('  FP64 solve %.0f ms; rms vs FP64: fp32 %.2e, bf16 %.2e vox; learned + bf16 per GN '
+          'iteration %s (0.01 vox at GN %s)'
+          % (t64 * 1e3, out['fp32'], out['bf16'], ['%.2e' % r for r in trace], k), flush=True)
     assert out['fp32'] < 1e-3 and out['bf16'] < 1e-3
-    # refinement continues from the dU/U < 1e-3 stop towards the FP64 fixed
-    # point: never farther from the FP64 solve than the unrefined iterate
-    assert Ur.dtype == torch.float64 and _rms(Ur.cpu().numpy(), U64) <= 1.05 * out['bf16']
-    # the analytic field is first order in u (|u| up to 4 vox)
-    assert et < 0.1
+    assert k is not None and k <= 3
+    return f, g, fv, gv, kw, U64
 
 
-def test_c5_combined_h16_learned_mixed(gpu):
-    """Reduced C5: 257 x 257 x 193 voxels (16 x 16 x 12 elements at h = 16)."""
+def test_c3_513_h8(gpu):
+    _c3_common(8)
+
+
+def test_c3_513_h4(gpu):
+    _c3_common(4)
+
+
+def test_c3_513_h8_fp64_refinement(gpu):
+    """FP32-against-FP64 iterative refinement on the full C3 workload at
+    h = 8.  The plain FP32 multiscale solve (dynamic mask, not frozen)
+    converges to dU/U < 1e-3; the refinement then holds the b-mask at the
+    converged U and iterates with FP64 residuals.  Its target is the FP64 GN
+    solve of the same functional: correlate_gpu in float64 from the same U
+    with the mask frozen at its first iteration.  The FP64 residual
+    ||b - ll R U|| falls at every step and U ends much closer to that
+    solution than the unrefined iterate."""
+    import torch
+    from voxeldvc_amd.engine import correlate_gpu as C
+    n, h = 513, 8
+    Ne = (n - 1) // h
+    _, _, fv, gv, _, _ = _cube('c3', n, 3)
+    kw = dict(K_ref_laplacian=O.build_K_ref_laplacian(), l0=4.5 * h, glt=0.0,
+              dynamic_mask=True, freeze_mask_after=None, disp=False)
+    print('C3 513^3 h=8 FP64 refinement', flush=True)
+    last = {}
+
+    def keep(s, ik, U):
+        if s == 0:
+            last['U'] = U.clone()
+    log = []
+    Ur, _, _ = C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, refine_fp64=4, log=log,
+                                          callback=keep, **kw)
+    U32 = last['U']
+    res = [q['residual_fp64'] for lv in log for q in lv['iters'] if 'refine' in q]
+    U64, _, _ = C.correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, U0=U32.double(),
+                                dtype=np.float64, maxiter=60, eps=1e-9,
+                                **dict(kw, freeze_mask_after=0))
+    e0, e1 = _trms(U32, U64), _trms(Ur, U64)
+    print('  FP64 residual %s; rms vs the FP64 solve of the same functional: unrefined %.2e, '
+          'refined %.2e vox' % (['%.3e' % r for r in res], e0, e1), flush=True)
+    assert Ur.dtype == torch.float64 and len(res) == 5
+    assert all(b < a for a, b in zip(res, res[1:]))
+    assert res[-1] < 0.2 * res[0]
+    assert e1 < 0.2 * e0
+
+
+def test_c5_full_learned_mixed(gpu):
+    """C5 at full size: 1025 x 1025 x 769 FP16 speckle (806M voxels), necking
+    + 8 Fourier modes (peak 8 vox) + N(0,2), h = 16 (64 x 64 x 48
+    elements), equilibrium gap on the finest level.  The learned seed with
+    BF16 storage reaches 0.01 vox RMS of the FP64 solve of a 257^3
+    subvolume at the volume centre (its interior nodes, 2-element margin);
+    peak device memory is reported."""
     import torch
     from voxeldvc_amd import seednet, synth
     from voxeldvc_amd.engine import correlate_gpu as C
-    shape, h = (257, 257, 193), 16
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0 = torch.cuda.mem_get_info()[0]
+    shape, h = (1025, 1025, 769), 16
     N = tuple((s - 1) // h for s in shape)
-    f, g, modes, extra = synth.config_pair_torch('c5', shape, seed=5)
+    print('C5 1025x1025x769 h=16', flush=True)
+    f, g, _, _ = synth.config_pair_torch('c5', shape, seed=5)
+    f = f.half()
+    g = g.half()
     fv = f.view(shape[2], shape[1], shape[0]).permute(2, 1, 0)
     gv = g.view(shape[2], shape[1], shape[0]).permute(2, 1, 0)
-    print('C5 reduced 257x257x193 h=16', flush=True)
-    U64 = _lap_then_eqgap(C, torch, fv, gv, N, h, dict(dtype=np.float64, maxiter=60, eps=1e-6))
-    eqg = dict(K_ref_laplacian=O.build_K_ref_laplacian(), K_ref_elastic=O.build_K_ref(0.3),
-               reg_type='equilibrium_gap', l0=4.5 * h, glt=0.0, dynamic_mask=True,
-               freeze_mask_after=4, disp=False)
-    trace = []
+    kw = dict(K_ref_laplacian=O.build_K_ref_laplacian(), K_ref_elastic=O.build_K_ref(0.3),
+              reg_type='equilibrium_gap', l0=4.5 * h, glt=0.0, dynamic_mask=True,
+              freeze_mask_after=4, disp=False)
+    o = tuple(((s - 1) // 2 - 128) // h * h for s in shape)
+    sub = fv[o[0]:o[0] + 257, o[1]:o[1] + 257, o[2]:o[2] + 257]
+    Ns = (16, 16, 16)
+    Us = _lap_then_eqgap(C, torch, sub.float().contiguous(), gv.float(), Ns, h,
+                         dict(dtype=np.float64, maxiter=60, eps=1e-6,
+                              g_origin=tuple(-q for q in o)),
+                         scales=(2, 1, 0), ms_extra=dict(fft_prealign=False))
+    io = [q // h for q in o]
+    sl = (slice(2, Ns[2] - 1), slice(2, Ns[1] - 1), slice(2, Ns[0] - 1))
+    ref_nodes = Us.view(Ns[2] + 1, Ns[1] + 1, Ns[0] + 1, 3)[sl]
+
+    def sub_rms(U):
+        Uf = U.view(N[2] + 1, N[1] + 1, N[0] + 1, 3)[io[2]:io[2] + Ns[2] + 1,
+                                                     io[1]:io[1] + Ns[1] + 1,
+                                                     io[0]:io[0] + Ns[0] + 1][sl]
+        return float(((Uf.double() - ref_nodes) ** 2).mean().sqrt())
+
     net = seednet.SeedNet.load()
-    Ul, _, _ = C.multiscale_correlate_gpu(
-        fv, gv, N[0], N[1], N[2], h, torch, storage='bf16', scales=(0,), learned_seed=net,
-        callback=lambda s, ik, U: trace.append(float(((U.double() - U64) ** 2).mean().sqrt())),
-        **eqg)
-    Up = _lap_then_eqgap(C, torch, fv, gv, N, h, {})
-    seed = net.predict(f, g, shape, N[0], N[1], N[2], h)
-    truth = synth.nodal_truth(modes, extra, N[0], N[1], N[2], h)
-    inner = _interior_dofs(N[0], N[1], N[2], margin=2)
-    print('  seed rms vs FP64 %.3f vox; learned+BF16 per-iteration rms %s; plain FP32 rms '
-          '%.2e; FP64 vs analytic field (interior) %.3f vox'
-          % (float(((seed.double() - U64) ** 2).mean().sqrt()), ['%.2e' % r for r in trace],
-             float(((Up.double() - U64) ** 2).mean().sqrt()),
-             _rms(U64.cpu().numpy(), truth, inner)), flush=True)
-    assert seed.numel() == 3 * (N[0] + 1) * (N[1] + 1) * (N[2] + 1)
-    assert float(((Up.double() - U64) ** 2).mean().sqrt()) < 2e-3
-    assert min(trace) <= 0.01
-    assert float(((Ul.double() - U64) ** 2).mean().sqrt()) <= 0.01
+    trace = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Ul, _, _ = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, storage='bf16',
+                                          scales=(0,), learned_seed=net,
+                                          callback=lambda s, ik, U: trace.append(sub_rms(U)),
+                                          **kw)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    peak = free0 - torch.cuda.mem_get_info()[0]
+    k = next((i + 1 for i, e in enumerate(trace) if e <= 0.01), None)
+    print('  learned + BF16: %.0f ms incl. per-iteration checks, subvolume rms per GN %s '
+          '(0.01 vox at GN %s); device memory in use %.1f GB'
+          % (ms, ['%.2e' % e for e in trace], k, peak / 1e9), flush=True)
+    assert Ul.numel() == 3 * (N[0] + 1) * (N[1] + 1) * (N[2] + 1)
+    assert k is not None and k <= 4
+    assert sub_rms(Ul) <= 0.01

@@ -2,7 +2,8 @@
 
 The library is the only compute path: if it is missing or no GPU is visible,
 every entry point raises.  torch is used only to own device memory and to
-provide the stream handle the library enqueues on.
+provide the stream handle the library enqueues on; layout changes, element
+conversions, copies and fills of the solve's arrays are library kernels too.
 """
 import ctypes
 import os
@@ -15,7 +16,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 5
+ABI_VERSION = 6
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -89,6 +90,12 @@ SIGNATURES = {
     'vdvc_equivalent_strain': [_vp, _i64, _vp, _vp],
     'vdvc_gn_convergence': [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp],
     'vdvc_remap_mask_deformed': [_vp, _i, _i, _i, _i, _vp, _i, _dp, _vp, _vp],
+    'vdvc_convert': [_vp, _i64, _vp, _i, _vp, _i],
+    'vdvc_copy': [_vp, _i64, _vp, _vp],
+    'vdvc_fill': [_vp, _i64, _i, _d, _vp],
+    'vdvc_copy_volume': [_vp, _i, _i, _i, _vp, _i, _i64, _i64, _i64, _vp, _i, _i],
+    'vdvc_axpby_x': [_vp, _i64, _d, _vp, _i, _d, _vp, _i],
+    'vdvc_resample_nodes': [_vp, _i, _i, _i, _i, _vp, _i, _i, _i, _i, _vp, _i],
 }
 
 _lib = None
@@ -191,9 +198,55 @@ def empty(n, dtype='float32'):
     return t.empty(int(n), dtype=getattr(t, dtype), device='cuda')
 
 
+# torch dtype name -> library element-type code
+TYPE_CODES = {'float32': F32, 'bfloat16': BF16, 'float16': F16, 'float64': F64,
+              'uint8': U8, 'uint16': U16, 'int16': I16, 'bool': BOOL}
+
+
+def type_code(dtype):
+    """Library element-type code of a torch / NumPy dtype or dtype name
+    (None when the library has no such element type)."""
+    name = str(dtype).replace('torch.', '')
+    if name == 'bool_':
+        name = 'bool'
+    return TYPE_CODES.get(name)
+
+
 def zeros(n, dtype='float32'):
-    t = require_gpu()
-    return t.zeros(int(n), dtype=getattr(t, dtype), device='cuda')
+    """Zero-filled device vector (filled by the library)."""
+    d = empty(n, dtype)
+    fill(d, 0.0)
+    return d
+
+
+def fill(d, value):
+    """d[:] = value on the device (vdvc_fill)."""
+    call('vdvc_fill', ctx(), d.numel(), type_code(d.dtype), float(value), d)
+    return d
+
+
+def copy_(dst, src):
+    """dst[:] = src (same byte size) on the device (vdvc_copy)."""
+    nb = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nb:
+        raise ValueError('copy_: %d bytes into %d' % (nb, dst.numel() * dst.element_size()))
+    call('vdvc_copy', ctx(), nb, src, dst)
+    return dst
+
+
+def clone(d):
+    """A device copy of a contiguous tensor."""
+    return copy_(require_gpu().empty_like(d), d)
+
+
+def convert(d, dtype):
+    """Contiguous device tensor converted to `dtype` (the same tensor when
+    it already has it)."""
+    if str(d.dtype).replace('torch.', '') == dtype:
+        return d
+    out = empty(d.numel(), dtype).view(d.shape)
+    call('vdvc_convert', ctx(), d.numel(), d, type_code(d.dtype), out, type_code(dtype))
+    return out
 
 
 def is_tensor(a):
@@ -204,38 +257,68 @@ def is_tensor(a):
     return isinstance(a, t.Tensor)
 
 
+def _host_upload(arr, fallback):
+    """Host array -> device tensor of its own dtype (a plain copy); dtypes
+    the library does not read are converted on the host to `fallback`."""
+    t = require_gpu()
+    arr = np.asarray(arr)
+    if arr.dtype == np.bool_:
+        arr = arr.view(np.uint8)
+    elif type_code(arr.dtype) is None:
+        arr = arr.astype(fallback)
+    return t.from_numpy(np.ascontiguousarray(arr)).to(device='cuda')
+
+
+def _dev_tensor(a, fallback):
+    t = require_gpu()
+    d = a.detach()
+    if d.device.type != 'cuda':
+        d = d.to(device='cuda')
+    if d.dtype == t.bool:
+        d = d.view(t.uint8)
+    elif type_code(d.dtype) is None:
+        d = d.to(getattr(t, fallback))
+    return d
+
+
 def to_dev_flat(a, dtype='float32'):
-    """1-D array (numpy or tensor) -> contiguous device tensor of `dtype`."""
-    t = require_gpu()
-    tt = getattr(t, dtype)
+    """1-D array (numpy or tensor) -> contiguous device tensor of `dtype`
+    (conversion on the device)."""
     if is_tensor(a):
-        return a.detach().to(device='cuda', dtype=tt).contiguous().view(-1)
-    arr = np.ascontiguousarray(np.asarray(a).reshape(-1))
-    if dtype == 'uint8':
-        arr = arr.astype(np.uint8)
-    return t.from_numpy(arr).to(device='cuda', dtype=tt)
+        d = _dev_tensor(a, dtype)
+        if not d.is_contiguous():
+            d = d.contiguous()
+        return convert(d.view(-1), dtype)
+    return convert(_host_upload(np.asarray(a).reshape(-1), dtype).view(-1), dtype)
 
 
-def to_dev_volume(a, dtype='float32'):
-    """(nx, ny, nz) volume (numpy C-order or tensor) -> device tensor laid out x
-    fastest (flat index x + y*nx + z*nx*ny), plus its shape."""
-    t = require_gpu()
-    tt = getattr(t, dtype)
+def to_dev_volume(a, dtype='float32', pitched=False):
+    """(nx, ny, nz) volume (NumPy C-order array or tensor of any strides) ->
+    device tensor of `dtype` laid out x fastest, flat index x + p*(y + ny*z)
+    with p = nx (or the level pitch when `pitched`, padding zeroed), plus its
+    shape.  Host arrays are uploaded as they are and laid out on the device
+    (vdvc_copy_volume: the reference's ravel(order='F') with its dtype
+    conversion, engine/correlate_gpu.py:50-52, 938-939)."""
     if is_tensor(a):
         if a.dim() != 3:
             raise ValueError('volume must be 3-D')
-        shp = tuple(int(s) for s in a.shape)
-        d = a.detach().to(device='cuda', dtype=tt).permute(2, 1, 0).contiguous().view(-1)
-        return d, shp
-    arr = np.asarray(a)
-    if arr.ndim != 3:
-        raise ValueError('volume must be 3-D')
-    shp = arr.shape
-    if dtype in ('float32', 'float16', 'float64'):
-        host = np.ravel(arr.astype(getattr(np, dtype), copy=False), order='F')
+        src = _dev_tensor(a, dtype)
     else:
-        host = np.ravel(arr, order='F')
-    return t.from_numpy(np.ascontiguousarray(host)).to(device='cuda', dtype=tt), shp
+        if np.ndim(a) != 3:
+            raise ValueError('volume must be 3-D')
+        src = _host_upload(a, dtype)
+    shp = tuple(int(s) for s in src.shape)
+    sx, sy, sz = src.stride()
+    code = type_code(src.dtype)
+    if not pitched and (sx, sy, sz) == (1, shp[0], shp[0] * shp[1]) and \
+            str(src.dtype).replace('torch.', '') == dtype:
+        # already x fastest and dense in the requested type
+        return src.as_strided((src.numel(),), (1,)), shp
+    p = pitch_of(shp[0]) if pitched else shp[0]
+    out = empty(p * shp[1] * shp[2], dtype)
+    call('vdvc_copy_volume', ctx(), shp[0], shp[1], shp[2], src, code, sx, sy, sz, out,
+         type_code(dtype), p)
+    return out, shp
 
 
 # element types the post-solve stages read in the images' own dtype:
@@ -328,32 +411,34 @@ def pitch_of(nvx):
 
 def pitch_dense(d, shape, dtype=None):
     """Dense x-fastest device volume of `shape` (nx, ny, nz) -> zero-padded
-    pitched volume (index x + pitch*(y + ny*z))."""
-    t = require_gpu()
+    pitched volume (index x + pitch*(y + ny*z)) of `dtype` (default: d's)."""
     nx, ny, nz = shape
     p = pitch_of(nx)
-    out = t.empty((nz, ny, p), dtype=d.dtype if dtype is None else getattr(t, dtype),
-                  device=d.device)
-    out[:, :, :nx] = d.view(nz, ny, nx)
-    if p > nx:
-        out[:, :, nx:] = 0
-    return out.view(-1)
+    name = str(d.dtype).replace('torch.', '') if dtype is None else dtype
+    out = empty(p * ny * nz, name)
+    call('vdvc_copy_volume', ctx(), nx, ny, nz, d, type_code(d.dtype), 1, nx, nx * ny, out,
+         type_code(name), p)
+    return out
 
 
 def unpitch(d, shape):
     """Pitched device volume -> dense x-fastest (the reference's voxel order)."""
     nx, ny, nz = shape
-    return d.view(nz, ny, pitch_of(nx))[:, :, :nx].contiguous().view(-1)
+    p = pitch_of(nx)
+    out = empty(nx * ny * nz, str(d.dtype).replace('torch.', ''))
+    call('vdvc_copy_volume', ctx(), nx, ny, nz, d, type_code(d.dtype), 1, p, p * ny, out,
+         type_code(d.dtype), nx)
+    return out
 
 
 def to_dev_level(a, shape, dtype='float32'):
     """Reference-order voxel array (flat (Nvox,), or (Nvox, k) columns are not
     accepted here) or (nx, ny, nz) volume -> pitched device volume."""
     if (is_tensor(a) and a.dim() == 3) or (not is_tensor(a) and np.ndim(a) == 3):
-        d, shp = to_dev_volume(a, dtype)
+        d, shp = to_dev_volume(a, dtype, pitched=True)
         if tuple(shp) != tuple(shape):
             raise ValueError('volume shape %r != %r' % (tuple(shp), tuple(shape)))
-        return pitch_dense(d, shape)
+        return d
     d = to_dev_flat(a, dtype)
     if d.numel() != shape[0] * shape[1] * shape[2]:
         raise ValueError('voxel array has %d entries, expected %d'

@@ -1576,3 +1576,83 @@ int vdvc_overlap_bounds(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_f, 
 }
 
 }  // extern "C"
+
+// ---- data movement at the solve's boundary (no reference arithmetic) ------
+
+namespace {
+int any_type_ok(int t) { return t >= VDVC_F32 && t <= VDVC_BOOL; }
+}  // namespace
+
+extern "C" {
+
+int vdvc_convert(vdvc_ctx* ctx, int64_t n, const void* d_src, int stype, void* d_dst,
+                 int dtype) {
+  if (!ctx || (n > 0 && (!d_src || !d_dst))) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (!any_type_ok(stype) || !any_type_ok(dtype)) return fail(-1, "unsupported element type");
+  VDVC_LAUNCH(vdvc::launch_convert(n, d_src, stype, d_dst, dtype, ctx->stream));
+  return 0;
+}
+
+int vdvc_copy(vdvc_ctx* ctx, int64_t nbytes, const void* d_src, void* d_dst) {
+  if (!ctx || (nbytes > 0 && (!d_src || !d_dst))) return fail(-1, "NULL argument");
+  if (nbytes < 0) return fail(-1, "nbytes < 0");
+  VDVC_LAUNCH(vdvc::launch_copy(nbytes, d_src, d_dst, ctx->stream));
+  return 0;
+}
+
+int vdvc_fill(vdvc_ctx* ctx, int64_t n, int dtype, double value, void* d_dst) {
+  if (!ctx || (n > 0 && !d_dst)) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (!any_type_ok(dtype)) return fail(-1, "unsupported element type");
+  VDVC_LAUNCH(vdvc::launch_fill(n, d_dst, dtype, value, ctx->stream));
+  return 0;
+}
+
+int vdvc_copy_volume(vdvc_ctx* ctx, int nx, int ny, int nz, const void* d_src, int stype,
+                     int64_t sx, int64_t sy, int64_t sz, void* d_dst, int dtype, int dpitch) {
+  if (!ctx || !d_src || !d_dst) return fail(-1, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(-1, "invalid volume %d x %d x %d", nx, ny, nz);
+  if (dpitch < nx) return fail(-1, "dpitch %d < nx %d", dpitch, nx);
+  if (!any_type_ok(stype) || !any_type_ok(dtype)) return fail(-1, "unsupported element type");
+  if (ny > 65535 || nz > 65535) return fail(-1, "ny and nz must be <= 65535");
+  if (sx < 0 || sy < 0 || sz < 0) return fail(-1, "negative source stride");
+  VolCopy C;
+  C.nx = nx; C.ny = ny; C.nz = nz;
+  C.sx = sx; C.sy = sy; C.sz = sz;
+  C.src = d_src; C.st = stype;
+  C.dst = d_dst; C.dt = dtype;
+  C.dp = dpitch;
+  VDVC_LAUNCH(vdvc::launch_copy_volume(C, ctx->stream));
+  return 0;
+}
+
+int vdvc_resample_nodes(vdvc_ctx* ctx, int Mx, int My, int Mz, int hs, const void* d_src,
+                        int Nx, int Ny, int Nz, int hd, void* d_dst, int rtype) {
+  if (!ctx || !d_src || !d_dst) return fail(-1, "NULL argument");
+  if (Mx < 1 || My < 1 || Mz < 1 || hs < 1 || Nx < 1 || Ny < 1 || Nz < 1 || hd < 1)
+    return fail(-1, "invalid meshes");
+  if ((long long)Mx * hs != (long long)Nx * hd || (long long)My * hs != (long long)Ny * hd ||
+      (long long)Mz * hs != (long long)Nz * hd)
+    return fail(-1, "the meshes must cover the same voxels (M*hs == N*hd per axis)");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
+  if (3LL * (Nx + 1) * (Ny + 1) * (Nz + 1) > 0x7fffffffLL) return fail(-1, "mesh too large");
+  NodeResample R;
+  R.Mx = Mx; R.My = My; R.Mz = Mz; R.hs = hs;
+  R.Nx = Nx; R.Ny = Ny; R.Nz = Nz; R.hd = hd;
+  R.src = d_src; R.dst = d_dst; R.rt = rtype;
+  VDVC_LAUNCH(vdvc::launch_resample_nodes(R, ctx->stream));
+  return 0;
+}
+
+int vdvc_axpby_x(vdvc_ctx* ctx, int64_t n, double a, const void* d_x, int xtype, double b,
+                 void* d_y, int ytype) {
+  if (!ctx || (n > 0 && (!d_x || !d_y))) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (!ftype_ok(xtype) && xtype != VDVC_BF16) return fail(-1, "unsupported x type");
+  if (!rtype_ok(ytype)) return fail(-1, "y must be VDVC_F32 or VDVC_F64");
+  VDVC_LAUNCH(vdvc::launch_axpby_x(n, a, d_x, xtype, b, d_y, ytype, ctx->stream));
+  return 0;
+}
+
+}  // extern "C"

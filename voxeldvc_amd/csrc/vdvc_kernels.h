@@ -196,7 +196,35 @@ struct AffineIn {
   double A[9], t[3], c[3];
 };
 
+// strided volume copy with element-type conversion (vdvc_util.hip):
+// dst[x + dp (y + ny z)] = src[x sx + y sy + z sz] for x < nx, 0 for
+// nx <= x < dp
+struct VolCopy {
+  int nx, ny, nz;
+  long long sx, sy, sz;   // source strides (elements)
+  const void* src; int st;
+  void* dst; int dt;
+  int dp;                 // destination row pitch (>= nx)
+};
+
+// nodal field of one structured mesh interpolated onto another covering the
+// same voxels (vdvc_util.hip)
+struct NodeResample {
+  int Mx, My, Mz, hs;     // source mesh (elements) and element size
+  int Nx, Ny, Nz, hd;     // destination mesh and element size
+  const void* src;
+  void* dst;
+  int rt;                 // working type of both (GT_F32 / GT_F64)
+};
+
 namespace vdvc {
+int launch_resample_nodes(const NodeResample& R, hipStream_t s);
+int launch_convert(long long n, const void* src, int st, void* dst, int dt, hipStream_t s);
+int launch_copy(long long nbytes, const void* src, void* dst, hipStream_t s);
+int launch_fill(long long n, void* dst, int dt, double v, hipStream_t s);
+int launch_copy_volume(const VolCopy& C, hipStream_t s);
+int launch_axpby_x(long long n, double a, const void* x, int xt, double b, void* y, int yt,
+                   hipStream_t s);
 int vec_width(int h, int gtype);
 int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                         const TilesDev& T, const VoxIn& in, const float* v, float* P,

@@ -182,7 +182,7 @@ def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny
     gx, gy, gz = K.grad_planes(grad_f_vox, shp, wd)
     r = L.to_dev_level(res_vox, shp, wd)
     m = L.to_dev_level(mask_vox, shp, 'uint8')
-    zero = t.zeros_like(r)
+    zero = L.zeros(r.numel(), wd)
     rout = t.empty_like(r)
     b = L.empty(Ndof, wd)
     s = (L._d * 4)()
@@ -206,16 +206,19 @@ def _crop_dev(vol, shp, crop):
     """Crop an x-fastest device volume of dims shp to its first crop voxels."""
     if tuple(shp) == tuple(crop):
         return vol
-    v = vol.view(shp[2], shp[1], shp[0])[:crop[2], :crop[1], :crop[0]]
-    return v.contiguous().view(-1)
+    out = L.empty(crop[0] * crop[1] * crop[2], str(vol.dtype).replace('torch.', ''))
+    code = L.type_code(vol.dtype)
+    L.call('vdvc_copy_volume', L.ctx(), crop[0], crop[1], crop[2], vol, code, 1, shp[0],
+           shp[0] * shp[1], out, code, crop[0])
+    return out
 
 
 def _fft_shift_dev(f, fshp, g, gshp):
     shape = tuple(min(a, b) for a, b in zip(fshp, gshp))
     # the phase correlation reads float32 volumes (an argmax of a float64
     # spectrum either way)
-    fc = _crop_dev(f, fshp, shape).float()
-    gc = _crop_dev(g, gshp, shape).float()
+    fc = L.convert(_crop_dev(f, fshp, shape), 'float32')
+    gc = L.convert(_crop_dev(g, gshp, shape), 'float32')
     idx = (L._i64 * 1)()
     L.call('vdvc_phase_corr', L.ctx(), L.ptr(fc), L.ptr(gc), shape[0], shape[1], shape[2], idx)
     peak = _unravel_c(int(idx[0]), shape)
@@ -555,7 +558,7 @@ class _Level:
         if storage == 'bf16' and g.dtype == t.float32:
             self.g, self.gitype = g, L.F32
         else:
-            self.g, self.gitype = g.to(getattr(t, iname)), self.itype
+            self.g, self.gitype = L.convert(g, iname), self.itype
         self.gshp = tuple(gshp)
         self.gname = gname
 
@@ -571,6 +574,7 @@ class _Level:
         # reference-image set-up: gradients (+ erosion zeroing) and mask0, in
         # the pitched level layout (padding zeroed by the kernel)
         G = [L.empty(Nvox, self.gname) for _ in range(3)]
+        self.G = G
         mask0 = L.empty(Nvox, 'uint8')
         L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, float(glt),
                L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]), gt, L.ptr(mask0))
@@ -579,7 +583,6 @@ class _Level:
                float(glt), s3)
         mean0, std0 = _stats_mean_std(s3)
         wd = self.wd
-        tdt = getattr(t, wd)
         # scalars enter the element-wise steps rounded to the working type
         rnd = (lambda x: float(x)) if wd == 'float64' else (lambda x: float(np.float32(x)))
         mean0_f = rnd(mean0)
@@ -622,7 +625,7 @@ class _Level:
                 d = L.empty(Ndof, wd)
                 L.call('vdvc_diag_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]),
                        gt, L.ptr(mh), L.ptr(d))
-                ones = t.ones(Ndof, dtype=tdt, device='cuda')
+                ones = L.fill(L.empty(Ndof, wd), 1.0)
                 H0 = K.dot(d, ones)
                 dl = L.empty(Ndof, wd)
                 L.call('vdvc_diag_K', c, Nx, Ny, Nz, L.rtype_of(dl), reg.kp(reg.K2), reg.E, None,
@@ -647,11 +650,16 @@ class _Level:
         llf = float(ll)
         rk = reg.args() if l0 is not None else (L.REG_NONE, None, None, reg.E, reg.eps)
 
-        U = t.zeros(Ndof, dtype=tdt, device='cuda') if U0 is None else U0.to(tdt).clone()
+        if U0 is None:
+            U = L.zeros(Ndof, wd)
+        else:
+            U = L.convert(U0, wd)
+            if U is U0:
+                U = L.clone(U0)
         # pitched residual: every grid voxel is written by each residual pass
         # (the row padding is never read into a result)
         res = L.empty(Nvox, wd) if maxiter > 0 else L.zeros(Nvox, wd)
-        dU = t.zeros(Ndof, dtype=tdt, device='cuda')
+        dU = L.zeros(Ndof, wd)
         diag = L.empty(Ndof, wd)
         Minv = L.empty(Ndof, wd)
         b = L.empty(Ndof, wd)
@@ -679,7 +687,7 @@ class _Level:
                    float(g_origin[1]), float(g_origin[2]), float(glt), L.ptr(mask0),
                    L.ptr(frozen), L.ptr(gvals), L.ptr(mask), s3)
             if freeze_mask_after is not None and frozen is None and ik >= freeze_mask_after:
-                frozen = mask.clone()
+                frozen = L.clone(mask)
             bmask = frozen if frozen is not None else mask
             g_mean, std1 = _stats_mean_std(s3)
             kscale = -(std0 / std1)  # ZeroDivisionError on an empty mask, as the reference
@@ -723,28 +731,33 @@ class _Level:
             if err < eps:
                 break
         if refine_fp64 and self.wd != 'float64':
-            hm = (frozen if frozen is not None else mask) if dynamic_mask else mask0
-            U = self._refine_fp64(U, frozen, hm, mask0, g_origin, glt, reg, rk, llf,
-                                  l0 is not None, hmask, Minv, pcg_tol, pcg_max_iter,
+            U = self._refine_fp64(U, frozen, dynamic_mask, mask0, g_origin, glt, reg, rk, llf,
+                                  l0 is not None, Minv, pcg_tol, pcg_max_iter,
                                   pcg_check_every, int(refine_fp64), log, disp)
         return U, L.unpitch(res, self.shape), dU
 
-    def _refine_fp64(self, U, frozen, hm, mask0, g_origin, glt, reg, rk, llf, use_reg, hmask,
-                     Minv, pcg_tol, pcg_max_iter, pcg_check_every, nsteps, log, disp):
+    def _refine_fp64(self, U, frozen, dynamic_mask, mask0, g_origin, glt, reg, rk, llf,
+                     use_reg, Minv, pcg_tol, pcg_max_iter, pcg_check_every, nsteps, log, disp):
         """FP32-against-FP64 iterative refinement of the converged GN state.
+
         Each step evaluates the GN residual b(U) - ll R U entirely in FP64
         (float64 images and gradients -- exact copies of the stored images --,
-        FP64 sampling, residual, right-hand side and regularizer; the b-mask
-        follows the solver's rule: the frozen mask if the solve froze one,
-        else the mask sampled at the current U), solves (H + ll R) dU = b with
-        the working-precision operator and PCG, and accumulates U in FP64.
-        The fixed point is therefore the FP64 solve's.  Returns U as float64."""
-        t = L.require_gpu()
+        FP64 sampling, residual, right-hand side and regularizer), solves
+        (H + ll R) dU = b with the working-precision operator and PCG, and
+        accumulates U in FP64.  The b-mask is held fixed over the refinement:
+        the solve's frozen mask if it froze one, else the mask sampled (in
+        FP64) at the converged U; the operator's H uses the same mask when the
+        solve's H follows the b-mask (dynamic_mask), else mask0.  The fixed
+        point is therefore that of the FP64 GN iteration with this b-mask --
+        the FP64 solve from the same U with the mask frozen at its first
+        iteration.  Returns U as float64."""
         c = L.ctx()
         Nx, Ny, Nz, h = self.M
         Nvox, Ndof = self.Nvox_p, self.Ndof
-        f64 = self.f.double()
-        g64 = self.g.double()
+        f64 = L.empty(Nvox, 'float64')
+        L.call('vdvc_convert', c, Nvox, self.f, self.itype, f64, L.F64)
+        g64 = L.empty(self.g.numel(), 'float64')
+        L.call('vdvc_convert', c, self.g.numel(), self.g, self.gitype, g64, L.F64)
         G64 = [L.empty(Nvox, 'float64') for _ in range(3)]
         m64 = L.empty(Nvox, 'uint8')
         L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, f64, L.F64, float(glt), G64[0], G64[1],
@@ -752,25 +765,33 @@ class _Level:
         s3 = (L._d * 3)()
         L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, f64, L.F64, None, 1, float(glt), s3)
         mean0, std0 = _stats_mean_std(s3)
-        U64 = U.double()
-        gvals = L.zeros(Nvox, 'float64')
-        live = L.zeros(Nvox, 'uint8')
-        res = L.zeros(Nvox, 'float64')
+        U64 = L.empty(Ndof, 'float64')
+        L.call('vdvc_convert', c, Ndof, U, L.rtype_of(U), U64, L.F64)
+        gvals = L.empty(Nvox, 'float64')
+        live = L.empty(Nvox, 'uint8')
+        res = L.empty(Nvox, 'float64')
         b64 = L.empty(Ndof, 'float64')
+        b32 = L.empty(Ndof, 'float32')
         RU = L.empty(Ndof, 'float64')
-        dU = L.zeros(Ndof)
+        dU = L.empty(Ndof, 'float32')
         s4 = (L._d * 4)()
         it_ = (L._i * 1)()
         rr_ = (L._d * 1)()
-        Gh, mh = hmask(hm)
+        bfix = frozen
+        Gh = mh = None
         for k in range(nsteps + 1):
             L.call('vdvc_sample', c, Nx, Ny, Nz, h, U64, g64, L.F64, self.gshp[0], self.gshp[1],
                    self.gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]),
-                   float(glt), mask0, frozen, gvals, live, s3)
-            bmask = frozen if frozen is not None else live
+                   float(glt), mask0, bfix, gvals, live, s3)
+            if bfix is None:
+                # no frozen mask: hold the mask at the converged U fixed
+                bfix = L.empty(Nvox, 'uint8')
+                L.call('vdvc_copy', c, Nvox, live, bfix)
+            if Gh is None:
+                Gh, mh = self._operator_planes(bfix if dynamic_mask else mask0, mask0)
             g_mean, std1 = _stats_mean_std(s3)
             L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, G64[0], G64[1], G64[2], L.F64, f64,
-                   L.F64, float(mean0), gvals, float(g_mean), float(-(std0 / std1)), bmask,
+                   L.F64, float(mean0), gvals, float(g_mean), float(-(std0 / std1)), bfix,
                    live, res, b64, None, None, None, s4)
             if use_reg:
                 reg.matvec(self.M, U64, RU)
@@ -782,12 +803,27 @@ class _Level:
                 print("FP64 refinement %d | ||b - ll R U||_2 = %.3e" % (k, rn))
             if k == nsteps:
                 break
-            b32 = b64.float()
+            L.call('vdvc_convert', c, Ndof, b64, L.F64, b32, L.F32)
             L.call('vdvc_pcg', c, Nx, Ny, Nz, h, Gh[0], Gh[1], Gh[2], self.gtype, mh,
                    rk[0], rk[1], rk[2], rk[3], rk[4], llf, b32, Minv, dU, float(pcg_tol),
                    int(pcg_max_iter), int(pcg_check_every), it_, rr_)
-            U64 += dU.double()
+            # U64 += dU with dU widened to float64
+            L.call('vdvc_axpby_x', c, Ndof, 1.0, dU, L.F32, 1.0, U64, L.F64)
         return U64
+
+    def _operator_planes(self, mask, mask0):
+        """(gradient planes, mask pointer) of the working-precision H with
+        `mask`: folded bfloat16 planes g * mask for the bf16 storage (made
+        here unless mask is mask0, which the set-up planes already carry),
+        the stored planes and the mask otherwise."""
+        if self.gtype != L.BF16:
+            return self.G, mask
+        if mask is mask0:
+            return self.G, None
+        W = [L.empty(self.Nvox_p, 'bfloat16') for _ in range(3)]
+        L.call('vdvc_fold_grad', L.ctx(), self.Nvox_p, self.G[0], self.G[1], self.G[2], mask,
+               W[0], W[1], W[2])
+        return W, None
 
 
 # ---------------------------------------------------------------------------
@@ -868,8 +904,8 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
         s0 = 2 ** scales[-1]
         fi, fis = _stride_dev(f, fs, s0, vtype)
         gi, gis = _stride_dev(g, gs, s0, vtype)
-        U = seed_net.predict(fi.float(), gi.float(), fis, Nx, Ny, Nz, h // s0).to(
-            getattr(L.torch(), wd))
+        U = L.convert(seed_net.predict(L.convert(fi, 'float32'), L.convert(gi, 'float32'), fis,
+                                       Nx, Ny, Nz, h // s0), wd)
         prev = s0
         scales = scales[-1:]
         fft_prealign = False

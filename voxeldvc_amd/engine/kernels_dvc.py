@@ -85,7 +85,16 @@ def grad_planes(grad_f_vox, shape, dtype='float32'):
     """(Nvox, 3) gradient (reference layout) -> three pitched device planes of
     `dtype`."""
     g = L.to_dev_flat(grad_f_vox, dtype).view(-1, 3)
-    return [L.pitch_dense(g[:, c].contiguous(), shape) for c in range(3)]
+    nx, ny, nz = shape
+    p = L.pitch_of(nx)
+    out = []
+    for c in range(3):
+        o = L.empty(p * ny * nz, dtype)
+        # column c of the (Nvox, 3) array: element stride 3
+        L.call('vdvc_copy_volume', L.ctx(), nx, ny, nz, g[:, c], L.type_code(g.dtype), 3,
+               3 * nx, 3 * nx * ny, o, L.type_code(g.dtype), p)
+        out.append(o)
+    return out
 
 
 def _mask_u8(mask_vox, shape):
@@ -264,12 +273,12 @@ def pcg(matvec, b, M_inv, xp=np, x0=None, max_iter=1000, tol=1e-8, verbose=False
     dt, code = work_dtype(b, M_inv)
     bd = L.to_dev_flat(b, dt)
     Md = L.to_dev_flat(M_inv, dt)
-    x = t.zeros_like(bd) if x0 is None else L.to_dev_flat(x0, dt).clone()
-    r = bd.clone()
+    x = L.zeros(bd.numel(), dt) if x0 is None else L.clone(L.to_dev_flat(x0, dt))
+    r = L.clone(bd)
     axpby(-1.0, L.to_dev_flat(matvec(x), dt), 1.0, r)
     z = t.empty_like(r)
     L.call('vdvc_mul', L.ctx(), int(r.numel()), code, L.ptr(Md), L.ptr(r), L.ptr(z))
-    p = z.clone()
+    p = L.clone(z)
     rz = dot(r, z)
     r0 = dot(bd, bd) + 1e-30
     for it in range(max_iter):

@@ -143,21 +143,28 @@ class SeedNet:
         """Nodal displacement (device float32, reference DOF layout) of the
         h-mesh for device volumes f, g (dense x-fastest of dims `shape`).  The
         network predicts the nodes of a mesh with its own element size
-        self.h; for h = r * self.h those include every node of the h-mesh
-        (every r-th node per axis), which is what is returned."""
-        if h % self.h or h < self.h:
-            raise ValueError('the seed network predicts nodes of an h=%d mesh; h=%d is not a '
-                             'multiple of it' % (self.h, h))
-        r = h // self.h
+        self.h covering the same voxels; the h-mesh's nodes are picked from
+        it (h a multiple of self.h) or interpolated trilinearly from it (h a
+        divisor of self.h, the h-mesh then covering whole self.h elements)."""
+        if h % self.h and self.h % h:
+            raise ValueError('the seed network predicts nodes of an h=%d mesh; h=%d is neither '
+                             'a multiple nor a divisor of it' % (self.h, h))
+        ext = (Nx * h, Ny * h, Nz * h)
+        if any(e % self.h for e in ext):
+            raise ValueError('the mesh (%d x %d x %d elements of h=%d) does not cover whole '
+                             'h=%d elements' % (Nx, Ny, Nz, h, self.h))
+        mx, my, mz = (e // self.h for e in ext)
         x, dims = network_input(f, g, shape, self.h, itype)
         out = self.forward(x, dims)
-        mx, my, mz = Nx * r, Ny * r, Nz * r
         expect = 3 * (mx + 1) * (my + 1) * (mz + 1)
         if out.numel() != expect:
             raise ValueError('network output has %d values, the mesh %d' % (out.numel(), expect))
-        if r > 1:
-            out = out.view(mz + 1, my + 1, mx + 1, 3)[::r, ::r, ::r].contiguous().view(-1)
-        return out
+        if h == self.h:
+            return out
+        dst = L.empty(3 * (Nx + 1) * (Ny + 1) * (Nz + 1))
+        L.call('vdvc_resample_nodes', L.ctx(), mx, my, mz, self.h, out, Nx, Ny, Nz, h, dst,
+               L.F32)
+        return dst
 
 
 def network_input(f, g, shape, h, itype=None):
