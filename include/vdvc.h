@@ -128,6 +128,21 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
                   const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
                   const void* d_v, void* d_out);
 
+/* out = H v exactly as vdvc_matvec_H computes it (same kernels, same
+ * result bit for bit), without allocating: the node partials live in a
+ * library-static device array of 64 MiB per GPU (taken when the first
+ * context on that GPU is created and held until the process ends) instead
+ * of context scratch, and no timing events are recorded.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Returns -1 when the mesh needs more than 64 MiB of partials (every
+ * 513^3-voxel mesh at h >= 4 fits).  Nothing in the array has to start at
+ * zero; every call on a GPU shares that GPU's array, so calls that overlap
+ * on one GPU (any context or stream) race. */
+int vdvc_matvec_H_noalloc(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                          const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
+                          const void* d_v, void* d_out);
+
 /* out = diag(H).  Replaces build_diagonal_H (engine/kernels_dvc.py:509-543). */
 int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
                 const void* d_gz, int gtype, const uint8_t* d_mask, void* d_out);

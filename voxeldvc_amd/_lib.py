@@ -44,6 +44,7 @@ SIGNATURES = {
     'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d, _d, _vp,
                           _vp, _vp, _vp, _vp, _vp, _vp, _dp],
     'vdvc_matvec_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
+    'vdvc_matvec_H_noalloc': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     'vdvc_diag_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
     'vdvc_matvec_K': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _vp, _vp],
     'vdvc_diag_K': [_vp, _i, _i, _i, _i, _dp, _d, _vp, _vp],

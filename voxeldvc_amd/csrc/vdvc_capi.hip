@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,9 +72,38 @@ struct Buf {
 
 }  // namespace
 
+// The no-allocation entry points keep their scratch (the node partials of
+// the voxel kernels) in this file-scope device array, one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to
+// a kernel (in host code the name is a host-side stand-in); it passes the
+// device address hipGetSymbolAddress returned for the current device.
+#define VDVC_NOALLOC_SCRATCH_BYTES (64u << 20)
+static __device__ __align__(256) float scratch[VDVC_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+namespace {
+#define VDVC_MAX_DEVICES 64
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[VDVC_MAX_DEVICES] = {nullptr};
+
+// device address of `scratch` on `device` (current), taken once per device
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= VDVC_MAX_DEVICES) return fail(-1, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(-2, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+}  // namespace
+
 struct vdvc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this device's address of the static scratch array
   Buf P;          // voxel-kernel node partials
   Buf stats;      // per-wave / per-block FP64 partials
   Buf red;        // reduced FP64 results
@@ -399,6 +429,31 @@ int matvec_H_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype,
   return 0;
 }
 
+// H v with the node partials in the static scratch array: the same kernels
+// and decomposition as matvec_H_impl, no allocation, no timing events
+template <typename R>
+int matvec_H_noalloc_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype,
+                          const VoxIn& in, const R* v, R* out) {
+  const size_t need = (size_t)T.nwaves * T.wave_stride * sizeof(R);
+  if (need > VDVC_NOALLOC_SCRATCH_BYTES)
+    return fail(-1, "mesh %d x %d x %d (h=%d) needs %zu bytes of node partials; the "
+                    "no-allocation scratch holds %u", M.Nx, M.Ny, M.Nz, M.h, need,
+                VDVC_NOALLOC_SCRATCH_BYTES);
+  if (!ctx->scratch) return fail(-2, "no scratch address for device %d", ctx->device);
+  R* P = static_cast<R*>(ctx->scratch);
+  if (T.vec)
+    VDVC_LAUNCH(vdvc::launch_restrict_vec(MODE_MATVEC, gtype, in.mask != nullptr, VDVC_F32, false,
+                                          M, T, in, reinterpret_cast<const float*>(v),
+                                          reinterpret_cast<float*>(P), ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_restrict(MODE_MATVEC, gtype, in.mask != nullptr, VDVC_F32, false, M,
+                                      T, in, v, P, ctx->stream));
+  NodeOp<R> op = node_op0<R>();
+  op.P = P; op.T = T; op.out = out;
+  VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  return 0;
+}
+
 template <typename R>
 int diag_H_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
                 R* out) {
@@ -589,9 +644,13 @@ int vdvc_ctx_create(int device, uintptr_t stream, vdvc_ctx** out) {
   VDVC_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(-1, "device %d not present (%d devices)", device, ndev);
   VDVC_HIP(hipSetDevice(device));
+  void* sp = nullptr;
+  int rc = scratch_address(device, &sp);
+  if (rc) return rc;
   vdvc_ctx* c = new vdvc_ctx();
   c->device = device;
   c->stream = reinterpret_cast<hipStream_t>(stream);
+  c->scratch = sp;
   hipError_t e = hipHostMalloc((void**)&c->hpin, VDVC_HPIN * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     delete c;
@@ -757,6 +816,26 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
                        matvec_H_impl<double>(ctx, M, T, gtype, in,
                                              static_cast<const double*>(d_v),
                                              static_cast<double*>(d_out)));
+}
+
+int vdvc_matvec_H_noalloc(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                          const void* d_gy, const void* d_gz, int gtype, const uint8_t* d_mask,
+                          const void* d_v, void* d_out) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
+  if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  TilesDev T = make_tiles(M, gtype);
+  const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  return VDVC_BY_RTYPE(gtype,
+                       matvec_H_noalloc_impl<float>(ctx, M, T, gtype, in,
+                                                    static_cast<const float*>(d_v),
+                                                    static_cast<float*>(d_out)),
+                       matvec_H_noalloc_impl<double>(ctx, M, T, gtype, in,
+                                                     static_cast<const double*>(d_v),
+                                                     static_cast<double*>(d_out)));
 }
 
 int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,
