@@ -50,6 +50,9 @@ def parse():
     p.add_argument('--solve-only', choices=('plain', 'learned'), default=None,
                    help='run one GN solve of the time-to-solve pair (after a warm-up solve) '
                         'and exit: for kernel traces of a whole solve')
+    p.add_argument('--summarize-trace', default=None, metavar='CSV',
+                   help='print the per-kernel summary of the last --solve-only window of a '
+                        'rocprofv3 kernel-trace CSV and exit (no GPU needed)')
     p.add_argument('--configs', action='store_true',
                    help='also measure the other benchmark configurations (C2, C3 at h=4, C5); '
                         'several minutes')
@@ -201,11 +204,51 @@ def solve_only(kind, n=513, h=8):
         kw.update(storage='bf16', scales=(0,), learned_seed=seednet.SeedNet.load())
     for rep in range(2):
         torch.cuda.synchronize()
+        # a spin kernel on each side of the solve marks its window in a
+        # kernel trace (summarize_trace)
+        torch.cuda._sleep(1000)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         C.multiscale_correlate_gpu(fv, gv, Ne, Ne, Ne, h, torch, **kw)
         torch.cuda.synchronize()
-        print('%s solve %s: %.2f ms' % (kind, 'warm-up' if rep == 0 else 'timed',
-                                         (time.perf_counter() - t0) * 1e3), flush=True)
+        dt = (time.perf_counter() - t0) * 1e3
+        torch.cuda._sleep(1000)
+        torch.cuda.synchronize()
+        print('%s solve %s: %.2f ms' % (kind, 'warm-up' if rep == 0 else 'timed', dt), flush=True)
+
+
+def summarize_trace(path):
+    """Per-kernel summary of the last solve window of a rocprofv3 kernel
+    trace (CSV) of `--solve-only`: the kernels between the last two spin
+    markers, with the count of kernels that are not the library's."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r['Start_Timestamp']))
+    mk = [i for i, r in enumerate(rows) if 'sleep' in r['Kernel_Name'].lower()
+          or 'spin' in r['Kernel_Name'].lower()]
+    if len(mk) < 2:
+        raise SystemExit('no marker pair in %s' % path)
+    win = rows[mk[-2] + 1:mk[-1]]
+    t0 = int(win[0]['Start_Timestamp'])
+    t1 = int(win[-1]['End_Timestamp'])
+    agg = {}
+    for r in win:
+        d = int(r['End_Timestamp']) - int(r['Start_Timestamp'])
+        a = agg.setdefault(r['Kernel_Name'], [0, 0])
+        a[0] += 1
+        a[1] += d
+    busy = sum(v[1] for v in agg.values())
+    print('solve window: %d kernels, %.3f ms first start to last end, %.3f ms busy'
+          % (len(win), (t1 - t0) / 1e6, busy / 1e6))
+    print('%-8s %10s %10s  %s' % ('calls', 'total ms', 'avg us', 'kernel'))
+    for name, (n, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print('%-8d %10.3f %10.2f  %s' % (n, d / 1e6, d / n / 1e3, name[:110]))
+    # torch's own kernels carry its namespaces; the runtime's copy kernels
+    # (__amd_rocclr_*) are memory copies
+    other = [nm for nm in agg if 'at::' in nm or 'c10::' in nm or 'torch' in nm]
+    copies = [nm for nm in agg if nm.startswith('__amd_rocclr')]
+    print('torch kernels in the window: %s; runtime copy kernels: %s'
+          % (other or 'none', copies or 'none'))
 
 
 def _matvec_kernel_ms(n_el, h, G, gt, mask, reps=20):
@@ -763,6 +806,9 @@ def cpu_gn_solve():
 
 def main():
     a = parse()
+    if a.summarize_trace:
+        summarize_trace(a.summarize_trace)
+        return
     if a.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (--gpus %d '
               'requested)' % a.gpus, file=sys.stderr)

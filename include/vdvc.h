@@ -393,6 +393,35 @@ int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const
                              int utype, const double* h_crop, const uint8_t* d_mask,
                              uint8_t* d_out);
 
+/* Per-element field d_field (Ne, C) float64, element-major (e*C + c), carried
+ * to the deformed configuration: element centres moved as in
+ * vdvc_remap_mask_deformed; each regular centre + h_crop takes the linear
+ * interpolation of the field over the tetrahedron of the moved-centre
+ * lattice that contains it (each lattice cell split into the 6 tetrahedra
+ * around its (0,0,0)-(1,1,1) diagonal, a conforming piecewise-linear
+ * interpolant), or, outside the moved lattice, the value of the nearest
+ * moved centre.  Replaces remap_cell_field_to_deformed
+ * (engine/write_output.py:533-568: scipy griddata 'linear' with a
+ * 'nearest' fill), whose Qhull Delaunay tetrahedra can differ from these
+ * inside a cell: both reproduce linear fields exactly and agree to the
+ * field's second differences otherwise; 1 <= C <= 64. */
+int vdvc_remap_field_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                              int utype, const double* h_crop, const double* d_field, int C,
+                              double* d_out);
+
+/* d_out (Ne,) = 1 where the mean of the reference image (dense voxel array,
+ * its own dtype itype) over the element's (h+1)^3 voxels exceeds glt (float64
+ * sums: exact for integer images).  Replaces compute_material_cell_mask
+ * (engine/write_output.py:309-338). */
+int vdvc_material_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_ref,
+                        int itype, double glt, uint8_t* d_out);
+
+/* One-element erosion of an element mask (uint8, elements x fastest) over
+ * the 26-neighbourhood, the mesh border counting as set.  Replaces
+ * compute_material_cell_mask_eroded (engine/write_output.py:341-349, scipy
+ * binary_erosion with a 3x3x3 structure and border_value=1). */
+int vdvc_erode_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const uint8_t* d_in, uint8_t* d_out);
+
 /* ---- preprocess affine stage (dense float32 volumes, x fastest) ---- */
 
 /* scipy.ndimage.gaussian_filter of a float32 volume: one symmetric

@@ -19,6 +19,9 @@ counts and 27-neighbourhood tests, so the two formulations check each other:
   * equivalent_strain        engine/write_output.py:284-306
   * gn_convergence_cell      engine/write_output.py:450-491
   * remap_mask_nearest       engine/write_output.py:494-530
+  * remap_field_linear       engine/write_output.py:533-568
+  * material_cell_mask       engine/write_output.py:309-338
+  * material_cell_mask_eroded engine/write_output.py:341-349
 
 Element order everywhere is x fastest (e = i + Nx*(j + Ny*k)); node order
 n = i + (Nx+1)*(j + (Ny+1)*k); voxel order x fastest on the
@@ -211,3 +214,93 @@ def remap_mask_nearest_brute(elem_mask, U_grid, h, crop_lo=(0, 0, 0)):
         d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
         out[s:s + 512] = vals[np.argmin(d2, axis=1)]
     return out.reshape(Nz, Ny, Nx).transpose(2, 1, 0)
+
+
+def remap_field_linear(field, U_grid, h, crop_lo=(0, 0, 0)):
+    """(Nx, Ny, Nz[, C]) per-element field carried to the deformed
+    configuration the reference's way: scipy griddata 'linear' over the
+    moved element centres (a Qhull Delaunay triangulation), the 'nearest'
+    values where it gives NaN (outside the convex hull)."""
+    from scipy.interpolate import griddata
+    field = np.asarray(field, dtype=np.float64)
+    x_ref, x_def = element_centres_deformed(U_grid, h)
+    pts = x_def.reshape(-1, 3)
+    q = (x_ref + np.asarray(crop_lo, dtype=np.float64)).reshape(-1, 3)
+    vals = field.reshape(-1, field.shape[-1]) if field.ndim == 4 else field.reshape(-1, 1)
+    lin = griddata(pts, vals, q, method='linear')
+    near = griddata(pts, vals, q, method='nearest')
+    bad = np.isnan(lin)
+    lin[bad] = near[bad]
+    return lin.reshape(field.shape)
+
+
+def material_cell_mask(ref, h, Nx, Ny, Nz, glt=0.0):
+    """(Nx, Ny, Nz) True where the mean of ref over the element's (h+1)^3
+    voxel block exceeds glt (ref indexed [x][y][z])."""
+    flat = np.asarray(ref).reshape(-1, order='F')
+    mean = flat[elem_voxels(h, Nx, Ny, Nz)].mean(axis=1)
+    return (mean > glt).reshape(Nx, Ny, Nz, order='F')
+
+
+def material_cell_mask_eroded(mask):
+    """One-cell erosion over the 26-neighbourhood, the border counting as
+    True (scipy binary_erosion, 3x3x3 structure, border_value=1)."""
+    from scipy.ndimage import binary_erosion
+    return binary_erosion(np.asarray(mask, dtype=bool), structure=np.ones((3, 3, 3), bool),
+                          border_value=1)
+
+
+_KUHN = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+def remap_field_kuhn_brute(field, U_grid, h, crop_lo=(0, 0, 0)):
+    """The device path's interpolant by exhaustive search (small meshes):
+    every cell of the moved-centre lattice split into the 6 tetrahedra
+    around its (0,0,0)-(1,1,1) diagonal; a query takes the barycentric
+    interpolation over the first tetrahedron (cells x fastest, tetrahedra
+    in _KUHN order) whose coordinates are all >= -1e-10, else the value of
+    the nearest moved centre (lowest index on ties)."""
+    field = np.asarray(field, dtype=np.float64)
+    x_ref, x_def = element_centres_deformed(U_grid, h)
+    Nx, Ny, Nz = x_def.shape[:3]
+    C = field.shape[3] if field.ndim == 4 else 1
+    F = field.reshape(Nx, Ny, Nz, C)
+    Q = (x_ref + np.asarray(crop_lo, dtype=np.float64)).reshape(-1, 3)
+    P = x_def.transpose(2, 1, 0, 3).reshape(-1, 3)
+    Fx = F.transpose(2, 1, 0, 3).reshape(-1, C)
+    # tetrahedra: (4 vertex ids) over all cells
+    tets = []
+    for c in range(Nz - 1):
+        for b in range(Ny - 1):
+            for a in range(Nx - 1):
+                vid = lambda o: (a + o[0]) + Nx * ((b + o[1]) + Ny * (c + o[2]))
+                for p in _KUHN:
+                    e0 = [0, 0, 0]
+                    e1 = list(e0)
+                    e1[p[0]] = 1
+                    e2 = list(e1)
+                    e2[p[1]] = 1
+                    tets.append((vid(e0), vid(e1), vid(e2), vid((1, 1, 1))))
+    tets = np.array(tets, dtype=np.int64).reshape(-1, 4)
+    out = np.empty((Q.shape[0], C))
+    if len(tets):
+        V0 = P[tets[:, 0]]
+        M = np.stack([P[tets[:, 1]] - V0, P[tets[:, 2]] - V0, P[tets[:, 3]] - V0], axis=2)
+        Minv = np.linalg.inv(M)
+    for qi, q in enumerate(Q):
+        hit = None
+        if len(tets):
+            l123 = np.einsum('tij,tj->ti', Minv, q[None, :] - V0)
+            lam = np.concatenate([1.0 - l123.sum(1, keepdims=True), l123], axis=1)
+            ok = np.nonzero(lam.min(1) >= -1e-10)[0]
+            if ok.size:
+                hit = ok[0]
+        if hit is not None:
+            out[qi] = lam[hit] @ Fx[tets[hit]]
+        else:
+            d = q[None, :] - P
+            d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+            out[qi] = Fx[np.argmin(d2)]
+    # Q is in the reference centres' (i, j, k) order with k fastest
+    o = out.reshape(Nx, Ny, Nz, C)
+    return o if field.ndim == 4 else o[..., 0]

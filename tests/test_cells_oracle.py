@@ -158,3 +158,104 @@ def test_remap_identity_shift_and_brute_force():
     U = rng.normal(0, 1.3, (Nx + 1, Ny + 1, Nz + 1, 3))
     np.testing.assert_array_equal(CO.remap_mask_nearest(m, U, h, (1, -2, 3)),
                                   CO.remap_mask_nearest_brute(m, U, h, (1, -2, 3)))
+
+
+# ------------------------------------------ deformed-configuration field remap
+def _smooth_U(Nx, Ny, Nz, h, amp=0.6, k=17.0):
+    n = np.stack(np.meshgrid(np.arange(Nx + 1), np.arange(Ny + 1), np.arange(Nz + 1),
+                             indexing='ij'), -1).astype(float) * h
+    return amp * np.sin(n / k) + 0.3
+
+
+def _second_diff_max(F):
+    """Largest |second difference| of a lattice field along and across axes."""
+    m = 0.0
+    for a in range(3):
+        m = max(m, np.abs(np.diff(F, 2, axis=a)).max())
+        for b in range(a + 1, 3):
+            m = max(m, np.abs(np.diff(np.diff(F, axis=a), axis=b)).max())
+    return m
+
+
+@pytest.mark.parametrize('crop', [(0, 0, 0), (3, -2, 5)])
+def test_remap_field_linear_fields_are_exact(crop):
+    """Any piecewise-linear interpolant reproduces a field that is affine in
+    the moved positions: griddata (the reference) and the Kuhn split agree
+    with it, and with each other, inside the moved lattice."""
+    Nx, Ny, Nz, h = 6, 5, 4, 8
+    U = _smooth_U(Nx, Ny, Nz, h)
+    x_ref, x_def = CO.element_centres_deformed(U, h)
+    rng = np.random.default_rng(1)
+    A, b = rng.normal(size=(3, 3)), rng.normal(size=3)
+    F = x_def @ A.T + b
+    q = x_ref + np.asarray(crop, float)
+    want = q @ A.T + b
+    g = CO.remap_field_linear(F, U, h, crop)
+    k = CO.remap_field_kuhn_brute(F, U, h, crop)
+    # queries at least one cell inside the moved lattice
+    inner = np.zeros((Nx, Ny, Nz), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner &= np.all((q > x_def[0, 0, 0] + h) & (q < x_def[-1, -1, -1] - h), axis=-1)
+    assert inner.sum() > 0
+    assert np.abs(g - want)[inner].max() < 1e-11
+    assert np.abs(k - want)[inner].max() < 1e-11
+
+
+def test_remap_field_identity_and_nearest_fill():
+    Nx, Ny, Nz, h = 5, 4, 6, 4
+    rng = np.random.default_rng(2)
+    F = rng.normal(size=(Nx, Ny, Nz, 3))
+    Z = np.zeros((Nx + 1, Ny + 1, Nz + 1, 3))
+    # no displacement: every query is a lattice point
+    assert np.array_equal(CO.remap_field_kuhn_brute(F, Z, h), F)
+    assert np.allclose(CO.remap_field_linear(F, Z, h), F, atol=1e-12)
+    # queries shifted far outside the lattice take the nearest value
+    far = (100.0 * h, 0.0, 0.0)
+    k = CO.remap_field_kuhn_brute(F[..., 0], Z, h, far)
+    g = CO.remap_field_linear(F[..., 0], Z, h, far)
+    assert np.array_equal(k, g) and np.array_equal(k, np.broadcast_to(F[-1:, :, :, 0], k.shape))
+
+
+def test_remap_field_smooth_agrees_with_griddata():
+    """On a smooth field the two triangulations differ by at most a fraction
+    of the field's second differences; a wrong vertex or weight would be
+    off by first differences."""
+    Nx, Ny, Nz, h = 7, 6, 5, 8
+    U = _smooth_U(Nx, Ny, Nz, h)
+    _, x_def = CO.element_centres_deformed(U, h)
+    F = np.sin(x_def[..., 0] / 20) * np.cos(x_def[..., 1] / 15) + x_def[..., 2] / 50
+    g = CO.remap_field_linear(F, U, h)
+    k = CO.remap_field_kuhn_brute(F, U, h)
+    inner = np.zeros(F.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    d = np.abs(g - k)[inner].max()
+    assert d <= 0.5 * _second_diff_max(F)
+    assert d < 0.1 * np.abs(np.diff(F, axis=0)).max()
+
+
+@pytest.mark.parametrize('dtype', [np.uint8, np.uint16, np.float32])
+def test_material_cell_mask(dtype):
+    Nx, Ny, Nz, h = 4, 3, 5, 4
+    rng = np.random.default_rng(3)
+    shape = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    ref = rng.integers(0, 200, shape).astype(dtype)
+    ref[: 2 * h] = 0
+    glt = 90.0
+    m = CO.material_cell_mask(ref, h, Nx, Ny, Nz, glt)
+    for i in range(Nx):
+        for j in range(Ny):
+            for k in range(Nz):
+                blk = ref[i * h:i * h + h + 1, j * h:j * h + h + 1, k * h:k * h + h + 1]
+                assert m[i, j, k] == (blk.astype(np.float64).mean() > glt)
+    assert not m[:2].any() and m.any()
+
+
+def test_material_cell_mask_eroded():
+    rng = np.random.default_rng(4)
+    m = rng.random((6, 5, 7)) < 0.8
+    e = CO.material_cell_mask_eroded(m)
+    for i in range(6):
+        for j in range(5):
+            for k in range(7):
+                nb = m[max(i - 1, 0):i + 2, max(j - 1, 0):j + 2, max(k - 1, 0):k + 2]
+                assert e[i, j, k] == nb.all()

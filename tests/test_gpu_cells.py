@@ -215,3 +215,97 @@ def test_remap_mask_lattice_ties(gpu):
     U[..., 2] = -1.5 * h
     got = W.remap_element_mask_to_deformed(m, U, h)
     np.testing.assert_array_equal(got, CO.remap_mask_nearest_brute(m, U, h))
+
+
+def test_remap_mask_outlier_large_mesh(gpu):
+    """One diverged element far from the rest on a 40^3-element mesh: the
+    cell grid is sized around the queries (the outlier is filed in a
+    boundary cell), and the result still equals the nearest-point remap."""
+    from voxeldvc_amd.engine import write_output as W
+    Nx = Ny = Nz = 40
+    h = 8
+    rng = np.random.default_rng(45)
+    m = rng.random((Nx, Ny, Nz)) < 0.5
+    U = rng.normal(0, 0.3 * h, (Nx + 1, Ny + 1, Nz + 1, 3))
+    U[20, 20, 20] = (1e6, -3e5, 7e5)
+    got = W.remap_element_mask_to_deformed(m, U, h)
+    np.testing.assert_array_equal(got, CO.remap_mask_nearest(m, U, h))
+
+
+FIELD_MESHES = [(6, 5, 4, 8), (7, 6, 5, 4), (2, 2, 2, 2), (9, 3, 4, 2), (1, 3, 2, 4)]
+
+
+@pytest.mark.parametrize('mesh', FIELD_MESHES)
+def test_remap_field_matches_kuhn_oracle(gpu, mesh):
+    """vdvc_remap_field_deformed against the oracle's exhaustive-search
+    restatement of the same interpolant (every tetrahedron of every cell
+    tried): equal to rounding wherever the walk finds the tetrahedron, and
+    the nearest fill elsewhere; smooth and random displacements, crops,
+    1 and 3 channels."""
+    from voxeldvc_amd.engine import write_output as W
+    Nx, Ny, Nz, h = mesh
+    print('remap field', mesh, flush=True)
+    rng = np.random.default_rng(51)
+    n = np.stack(np.meshgrid(np.arange(Nx + 1), np.arange(Ny + 1), np.arange(Nz + 1),
+                             indexing='ij'), -1).astype(float) * h
+    cases = [(0.6 * np.sin(n / 17.0) + 0.3, (0, 0, 0)),
+             (rng.normal(0, 0.15 * h, n.shape), (1.5, -2.0, 0.5)),
+             (np.zeros(n.shape), (0, 0, 0)),
+             (0.6 * np.sin(n / 11.0) - 1.0, (3 * h, 0, -h))]
+    for U, crop in cases:
+        for F in (rng.normal(size=(Nx, Ny, Nz)), rng.normal(size=(Nx, Ny, Nz, 3))):
+            got = W.remap_cell_field_to_deformed(F, U, h, crop)
+            want = CO.remap_field_kuhn_brute(F, U, h, crop)
+            assert got.shape == F.shape
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * (1 + np.abs(F).max()))
+
+
+def test_remap_field_against_griddata(gpu):
+    """Against the reference's scipy griddata ('linear' + 'nearest' fill):
+    a field affine in the moved positions is reproduced exactly by both
+    inside the moved lattice; a smooth field agrees to a fraction of its
+    second differences."""
+    from voxeldvc_amd.engine import write_output as W
+    Nx, Ny, Nz, h = 12, 10, 9, 8
+    n = np.stack(np.meshgrid(np.arange(Nx + 1), np.arange(Ny + 1), np.arange(Nz + 1),
+                             indexing='ij'), -1).astype(float) * h
+    U = 0.8 * np.sin(n / 23.0) + 0.4
+    x_ref, x_def = CO.element_centres_deformed(U, h)
+    rng = np.random.default_rng(52)
+    A, b = rng.normal(size=(3, 3)), rng.normal(size=3)
+    F = x_def @ A.T + b
+    inner = np.zeros((Nx, Ny, Nz), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    got = W.remap_cell_field_to_deformed(F, U, h)
+    ref = CO.remap_field_linear(F, U, h)
+    want = x_ref @ A.T + b
+    assert np.abs(got - want)[inner].max() < 1e-11
+    assert np.abs(ref - want)[inner].max() < 1e-11
+    S = np.sin(x_def[..., 0] / 20) * np.cos(x_def[..., 1] / 15) + x_def[..., 2] / 50
+    got = W.remap_cell_field_to_deformed(S, U, h)
+    ref = CO.remap_field_linear(S, U, h)
+    d2 = max(np.abs(np.diff(S, 2, axis=a)).max() for a in range(3))
+    assert np.abs(got - ref)[inner].max() <= 0.5 * d2
+
+
+@pytest.mark.parametrize('dtype', ['uint8', 'uint16', 'float32', 'float64', 'bool'])
+@pytest.mark.parametrize('mesh', [(6, 5, 7, 2), (4, 3, 5, 4), (3, 2, 2, 8), (70, 2, 3, 2)])
+def test_material_cells(gpu, dtype, mesh):
+    from voxeldvc_amd.engine import write_output as W
+    Nx, Ny, Nz, h = mesh
+    rng = np.random.default_rng(61)
+    shape = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    if dtype == 'bool':
+        ref = rng.random(shape) < 0.6
+        glt = 0.5
+    else:
+        ref = rng.integers(0, 200, shape).astype(dtype)
+        ref[: h + 1] = 0
+        glt = 99.5
+    got = W.compute_material_cell_mask(ref, h, Nx, Ny, Nz, glt=glt)
+    want = CO.material_cell_mask(ref.astype(np.float64) if dtype == 'bool' else ref, h, Nx, Ny,
+                                 Nz, glt)
+    assert got.dtype == bool and got.shape == (Nx, Ny, Nz)
+    np.testing.assert_array_equal(got, want)
+    ero = W.compute_material_cell_mask_eroded(got)
+    np.testing.assert_array_equal(ero, CO.material_cell_mask_eroded(want))

@@ -102,3 +102,18 @@ def test_run_stage_outputs(gpu, tmp_path):
     assert up.dtype == np.bool_
     np.testing.assert_array_equal(
         up, P.push_field_forward(uv, U_local, N, N, N, h, order=0, mode='constant', cval=1))
+    # deformed-configuration strains: the device interpolant (the oracle's
+    # exhaustive-search restatement of it) and the reference's griddata
+    psd = np.load(os.path.join(wd, 'principal_strains_cell_deformed.npy'))
+    assert psd.shape == (N, N, N, 3)
+    want_psd = CO.remap_field_kuhn_brute(ps, U_grid, h)
+    np.testing.assert_allclose(psd, want_psd, rtol=0, atol=1e-12 * np.abs(ps).max())
+    grid_psd = CO.remap_field_linear(ps, U_grid, h)
+    d2 = max(np.abs(np.diff(ps, 2, axis=a)).max() for a in range(3))
+    assert np.abs(psd - grid_psd)[1:-1, 1:-1, 1:-1].max() <= 0.5 * d2 + 1e-12
+    eqd = np.load(os.path.join(wd, 'equivalent_strain_cell_deformed.npy'))
+    np.testing.assert_allclose(eqd, CO.equivalent_strain(psd), rtol=2.3e-16, atol=0)
+    mat = np.load(os.path.join(wd, 'material_cell_mask.npy'))
+    np.testing.assert_array_equal(mat, CO.material_cell_mask(f8, h, N, N, N, 0.0))
+    mero = np.load(os.path.join(wd, 'material_cell_mask_eroded.npy'))
+    np.testing.assert_array_equal(mero, CO.material_cell_mask_eroded(mat))

@@ -91,6 +91,9 @@ SIGNATURES = {
     'vdvc_equivalent_strain': [_vp, _i64, _vp, _vp],
     'vdvc_gn_convergence': [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp],
     'vdvc_remap_mask_deformed': [_vp, _i, _i, _i, _i, _vp, _i, _dp, _vp, _vp],
+    'vdvc_remap_field_deformed': [_vp, _i, _i, _i, _i, _vp, _i, _dp, _vp, _i, _vp],
+    'vdvc_material_cells': [_vp, _i, _i, _i, _i, _vp, _i, _d, _vp],
+    'vdvc_erode_cells': [_vp, _i, _i, _i, _vp, _vp],
     'vdvc_convert': [_vp, _i64, _vp, _i, _vp, _i],
     'vdvc_copy': [_vp, _i64, _vp, _vp],
     'vdvc_fill': [_vp, _i64, _i, _d, _vp],

@@ -1404,34 +1404,55 @@ int vdvc_gn_convergence(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const void* d_dU,
   return timed_event(ctx);
 }
 
-int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
-                             int utype, const double* h_crop, const uint8_t* d_mask,
-                             uint8_t* d_out) {
-  if (!ctx || !d_U || !h_crop || !d_mask || !d_out) return fail(-1, "NULL argument");
-  int rc = check_mesh(Nx, Ny, Nz, h);
-  if (rc) return rc;
-  if (!rtype_ok(utype)) return fail(-1, "U type must be VDVC_F32 or VDVC_F64");
-  for (int c = 0; c < 3; ++c)
-    if (!std::isfinite(h_crop[c])) return fail(-1, "crop offset must be finite");
+}  // extern "C"
+
+namespace {
+
+// Moved element centres (pts) and regular query centres + crop (qry) of a
+// mesh, and the uniform grid of cells over the moved centres that the exact
+// nearest-centre search walks (counting sort by cell: off / list).  The
+// grid box is the moved centres' bounding box clipped to the query box
+// +- 64 h, so a few far outliers (a diverged element) cannot coarsen the
+// cells for everyone else; points beyond the box are filed in its boundary
+// cells, which keeps the search exact (they are visited with those cells
+// and are farther than the ring bound the search stops on).
+struct NNPrep {
+  double* pts;
+  double* qry;
+  NNGrid g;
+  int* off;
+  int* list;
+};
+
+int nn_prepare(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int utype,
+               const double* h_crop, NNPrep* P) {
   const long long Ne = (long long)Nx * Ny * Nz;
   const int nparts = 256;
+  int rc;
   if ((rc = ctx->vtmp[0].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
   if ((rc = ctx->vtmp[1].ensure((size_t)Ne * 3 * sizeof(double)))) return rc;
   if ((rc = ctx->stats.ensure((size_t)nparts * 6 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
-  double* pts = ctx->vtmp[0].as<double>();
-  double* qry = ctx->vtmp[1].as<double>();
-  if ((rc = timed_event(ctx))) return rc;
-  VDVC_LAUNCH(vdvc::launch_elem_centres(Nx, Ny, Nz, h, d_U, utype, h_crop, pts, qry, ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_bbox(Ne, pts, ctx->stats.as<double>(), nparts, ctx->red.as<double>(),
-                                ctx->stream));
+  P->pts = ctx->vtmp[0].as<double>();
+  P->qry = ctx->vtmp[1].as<double>();
+  VDVC_LAUNCH(vdvc::launch_elem_centres(Nx, Ny, Nz, h, d_U, utype, h_crop, P->pts, P->qry,
+                                        ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_bbox(Ne, P->pts, ctx->stats.as<double>(), nparts,
+                                ctx->red.as<double>(), ctx->stream));
   double bb[6];
   if ((rc = fetch(ctx, ctx->red.p, 6 * sizeof(double), bb))) return rc;
   for (int c = 0; c < 6; ++c)
     if (!std::isfinite(bb[c])) return fail(-1, "displacement field is not finite");
-  // cells of edge h over the moved centres' box, coarsened while there are
-  // more than 4 cells per point
-  NNGrid g;
+  const int N[3] = {Nx, Ny, Nz};
+  for (int c = 0; c < 3; ++c) {
+    const double qlo = 0.5 * h + h_crop[c], qhi = (N[c] - 0.5) * h + h_crop[c];
+    const double lo = qlo - 64.0 * h, hi = qhi + 64.0 * h;
+    bb[c] = std::min(std::max(bb[c], lo), hi);
+    bb[3 + c] = std::min(std::max(bb[3 + c], lo), hi);
+  }
+  // cells of edge h over the box, coarsened while there are more than 4
+  // cells per point
+  NNGrid& g = P->g;
   double cs = (double)h;
   long long ncell = 0;
   for (int it = 0; it < 64; ++it) {
@@ -1454,12 +1475,80 @@ int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const
   if ((rc = ctx->vtmp[2].ensure((size_t)(ncell + 1 + Ne) * sizeof(int)))) return rc;
   int* cnt = ctx->vtmp[3].as<int>();
   int* csum = cnt + ncell;
-  int* off = ctx->vtmp[2].as<int>();
-  int* list = off + ncell + 1;
-  VDVC_HIP(hipMemsetAsync(cnt, 0, (size_t)ncell * sizeof(int), ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_nn_build(Ne, pts, g, cnt, off, csum, list, ctx->stream));
-  VDVC_LAUNCH(vdvc::launch_nn_query(Ne, pts, qry, g, off, list, d_mask, d_out, ctx->stream));
+  P->off = ctx->vtmp[2].as<int>();
+  P->list = P->off + ncell + 1;
+  // zero the int counters (float 0.0 has all-zero bits)
+  VDVC_LAUNCH(vdvc::launch_fill(ncell, cnt, VDVC_F32, 0.0, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_nn_build(Ne, P->pts, g, cnt, P->off, csum, P->list, ctx->stream));
+  return 0;
+}
+
+int remap_args_ok(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, int utype,
+                  const double* h_crop) {
+  if (!ctx || !d_U || !h_crop) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!rtype_ok(utype)) return fail(-1, "U type must be VDVC_F32 or VDVC_F64");
+  for (int c = 0; c < 3; ++c)
+    if (!std::isfinite(h_crop[c])) return fail(-1, "crop offset must be finite");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vdvc_remap_mask_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                             int utype, const double* h_crop, const uint8_t* d_mask,
+                             uint8_t* d_out) {
+  int rc = remap_args_ok(ctx, Nx, Ny, Nz, h, d_U, utype, h_crop);
+  if (rc) return rc;
+  if (!d_mask || !d_out) return fail(-1, "NULL argument");
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if ((rc = timed_event(ctx))) return rc;
+  NNPrep P;
+  if ((rc = nn_prepare(ctx, Nx, Ny, Nz, h, d_U, utype, h_crop, &P))) return rc;
+  VDVC_LAUNCH(vdvc::launch_nn_query(Ne, P.pts, P.qry, P.g, P.off, P.list, d_mask, d_out, nullptr,
+                                    ctx->stream));
   return timed_event(ctx);
+}
+
+int vdvc_remap_field_deformed(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U,
+                              int utype, const double* h_crop, const double* d_field, int C,
+                              double* d_out) {
+  int rc = remap_args_ok(ctx, Nx, Ny, Nz, h, d_U, utype, h_crop);
+  if (rc) return rc;
+  if (!d_field || !d_out) return fail(-1, "NULL argument");
+  if (C < 1 || C > 64) return fail(-1, "channels must be 1..64 (got %d)", C);
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if ((rc = timed_event(ctx))) return rc;
+  NNPrep P;
+  if ((rc = nn_prepare(ctx, Nx, Ny, Nz, h, d_U, utype, h_crop, &P))) return rc;
+  if ((rc = ctx->nodeA.ensure((size_t)Ne * sizeof(int)))) return rc;
+  int* idx = ctx->nodeA.as<int>();
+  VDVC_LAUNCH(vdvc::launch_nn_query(Ne, P.pts, P.qry, P.g, P.off, P.list, nullptr, nullptr, idx,
+                                    ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_remap_linear(Nx, Ny, Nz, h, P.pts, P.qry, d_field, C, idx, d_out,
+                                        ctx->stream));
+  return timed_event(ctx);
+}
+
+int vdvc_material_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_ref,
+                        int itype, double glt, uint8_t* d_out) {
+  if (!ctx || !d_ref || !d_out) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!itype_ok(itype)) return fail(-1, "unsupported image type");
+  VDVC_LAUNCH(vdvc::launch_material_cells(Nx, Ny, Nz, h, d_ref, itype, glt, d_out, ctx->stream));
+  return 0;
+}
+
+int vdvc_erode_cells(vdvc_ctx* ctx, int Nx, int Ny, int Nz, const uint8_t* d_in, uint8_t* d_out) {
+  if (!ctx || !d_in || !d_out) return fail(-1, "NULL argument");
+  if (Nx < 1 || Ny < 1 || Nz < 1) return fail(-1, "invalid mesh");
+  if ((long long)Nx * Ny * Nz > 0x7fffffffLL) return fail(-1, "mesh too large");
+  VDVC_LAUNCH(vdvc::launch_erode_cells(Nx, Ny, Nz, d_in, d_out, ctx->stream));
+  return 0;
 }
 
 }  // extern "C"

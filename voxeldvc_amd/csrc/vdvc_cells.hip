@@ -561,7 +561,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void nn_query_kernel(long long n,
                                                               NNGrid g, const int* __restrict__ off,
                                                               const int* __restrict__ list,
                                                               const uint8_t* __restrict__ vals,
-                                                              uint8_t* __restrict__ out) {
+                                                              uint8_t* __restrict__ out,
+                                                              int* __restrict__ idx_out) {
   const long long qi = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   if (qi >= n) return;
   const double q0 = qry[3 * qi], q1 = qry[3 * qi + 1], q2 = qry[3 * qi + 2];
@@ -603,12 +604,223 @@ __global__ __launch_bounds__(VDVC_BLOCK) void nn_query_kernel(long long n,
     const double lim = (double)r * g.cs * (1.0 - 1e-9);
     if (bi >= 0 && best < lim * lim) break;
   }
-  out[qi] = bi >= 0 ? vals[bi] : (uint8_t)0;
+  if (out) out[qi] = bi >= 0 ? vals[bi] : (uint8_t)0;
+  if (idx_out) idx_out[qi] = bi;
+}
+
+
+// Material cells (engine/write_output.py:309-338): the mean reference gray
+// level over an element's (h+1)^3 voxels (shared faces included) exceeds
+// glt.  One wavefront per element, lanes over the block's (h+1)^2 voxel
+// rows, float64 sums (exact for the integer image types, whose block sums
+// stay below 2^53); mean = sum / (h+1)^3 as NumPy's mean forms it.
+__global__ __launch_bounds__(VDVC_BLOCK) void material_cells_kernel(int Nx, int Ny, int Nz, int h,
+                                                                    const void* img, int itype,
+                                                                    double glt,
+                                                                    uint8_t* __restrict__ out) {
+  const long long e = ((long long)blockIdx.x * VDVC_BLOCK + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if (e >= Ne) return;  // uniform across the wavefront
+  const int i = (int)(e % Nx), j = (int)((e / Nx) % Ny), k = (int)(e / ((long long)Nx * Ny));
+  const long long Nvx = (long long)Nx * h + 1, Nvy = (long long)Ny * h + 1;
+  const int m = h + 1;
+  double s = 0.0;
+  for (int r = lane; r < m * m; r += 64) {
+    const int dy = r % m, dz = r / m;
+    const long long base = (long long)i * h + Nvx * ((long long)j * h + dy + Nvy * ((long long)k * h + dz));
+    for (int dx = 0; dx < m; ++dx) s += ld_img(img, itype, base + dx);
+  }
+  s = wave_sum(s);
+  if (lane == 0) out[e] = (s / (double)((long long)m * m * m) > glt) ? 1 : 0;
+}
+
+// One-element erosion of an element mask over the 26-neighbourhood with the
+// mesh border counted as set (scipy binary_erosion with a 3x3x3 structure
+// and border_value=1, engine/write_output.py:341-349).
+__global__ __launch_bounds__(VDVC_BLOCK) void erode_cells_kernel(int Nx, int Ny, int Nz,
+                                                                 const uint8_t* __restrict__ in,
+                                                                 uint8_t* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if (e >= Ne) return;
+  const int i = (int)(e % Nx), j = (int)((e / Nx) % Ny), k = (int)(e / ((long long)Nx * Ny));
+  uint8_t v = in[e] ? 1 : 0;
+  for (int dz = -1; dz <= 1 && v; ++dz)
+    for (int dy = -1; dy <= 1 && v; ++dy)
+      for (int dx = -1; dx <= 1 && v; ++dx) {
+        const int a = i + dx, b = j + dy, c = k + dz;
+        if (a < 0 || b < 0 || c < 0 || a >= Nx || b >= Ny || c >= Nz) continue;
+        if (!in[a + (long long)Nx * (b + (long long)Ny * c)]) v = 0;
+      }
+  out[e] = v;
+}
+
+// Barycentric coordinates of q in the tetrahedron (v0, v1, v2, v3) by
+// Cramer's rule; false for a degenerate tetrahedron.
+__device__ __forceinline__ bool bary4(const double (&q)[3], const double (&v0)[3],
+                                      const double (&v1)[3], const double (&v2)[3],
+                                      const double (&v3)[3], double (&l)[4]) {
+  double a[3], b[3], c[3], d[3];
+  for (int t = 0; t < 3; ++t) {
+    a[t] = v1[t] - v0[t]; b[t] = v2[t] - v0[t]; c[t] = v3[t] - v0[t]; d[t] = q[t] - v0[t];
+  }
+  const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - b[0] * (a[1] * c[2] - a[2] * c[1]) +
+                     c[0] * (a[1] * b[2] - a[2] * b[1]);
+  if (!(fabs(det) > 1e-300)) return false;
+  const double l1 = (d[0] * (b[1] * c[2] - b[2] * c[1]) - b[0] * (d[1] * c[2] - d[2] * c[1]) +
+                     c[0] * (d[1] * b[2] - d[2] * b[1])) / det;
+  const double l2 = (a[0] * (d[1] * c[2] - d[2] * c[1]) - d[0] * (a[1] * c[2] - a[2] * c[1]) +
+                     c[0] * (a[1] * d[2] - a[2] * d[1])) / det;
+  const double l3 = (a[0] * (b[1] * d[2] - b[2] * d[1]) - b[0] * (a[1] * d[2] - a[2] * d[1]) +
+                     d[0] * (a[1] * b[2] - a[2] * b[1])) / det;
+  l[0] = 1.0 - l1 - l2 - l3; l[1] = l1; l[2] = l2; l[3] = l3;
+  return true;
+}
+
+// Linear remap of a per-element field to the deformed configuration
+// (engine/write_output.py:533-568): the moved element centres pts form a
+// deformed lattice; each lattice cell (8 neighbouring centres) is split
+// into the 6 Kuhn tetrahedra along its (0,0,0)-(1,1,1) diagonal (a
+// conforming split of the whole lattice), and the query takes the
+// barycentric interpolation of the field over the tetrahedron containing
+// it.  The cell is found by a walk from the reference cell of the query,
+// moved by the displacement of its nearest reference centre: each step
+// maps the query into the cell's mean affine frame and moves by the whole
+// cells it lies beyond.  A query in no tetrahedron within 64 steps (outside
+// the moved lattice) takes the value of its nearest moved centre (nn_idx),
+// the reference's nearest fill.
+__global__ __launch_bounds__(VDVC_BLOCK) void remap_linear_kernel(int Nx, int Ny, int Nz, double h,
+                                                                  const double* __restrict__ pts,
+                                                                  const double* __restrict__ qry,
+                                                                  const double* __restrict__ field,
+                                                                  int C,
+                                                                  const int* __restrict__ nn_idx,
+                                                                  double* __restrict__ out) {
+  const long long qi = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  const long long Ne = (long long)Nx * Ny * Nz;
+  if (qi >= Ne) return;
+  const int N[3] = {Nx, Ny, Nz};
+  const double q[3] = {qry[3 * qi], qry[3 * qi + 1], qry[3 * qi + 2]};
+  auto pidx = [&](int a, int b, int c) { return (long long)a + (long long)Nx * (b + (long long)Ny * c); };
+  bool found = false;
+  long long vid[4] = {0, 0, 0, 0};
+  double lam[4] = {0.0, 0.0, 0.0, 0.0};
+  if (Nx >= 2 && Ny >= 2 && Nz >= 2) {
+    int nr[3], cell[3];
+    for (int d = 0; d < 3; ++d) {
+      const double f = rint(q[d] / h - 0.5);
+      nr[d] = (int)fmin(fmax(f, 0.0), (double)(N[d] - 1));
+    }
+    const long long n0 = pidx(nr[0], nr[1], nr[2]);
+    for (int d = 0; d < 3; ++d) {
+      const double u0 = pts[3 * n0 + d] - ((double)nr[d] + 0.5) * h;
+      const double sd = floor((q[d] - u0) / h - 0.5);
+      cell[d] = (int)fmin(fmax(sd, 0.0), (double)(N[d] - 2));
+    }
+    for (int it = 0; it < 64 && !found; ++it) {
+      double V[8][3];
+      for (int cc = 0; cc < 8; ++cc) {
+        const long long n = pidx(cell[0] + (cc & 1), cell[1] + ((cc >> 1) & 1), cell[2] + (cc >> 2));
+        for (int d = 0; d < 3; ++d) V[cc][d] = pts[3 * n + d];
+      }
+      // the 6 Kuhn tetrahedra: corner 000, + e_p0, + e_p1, then 111
+      const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+      for (int t = 0; t < 6 && !found; ++t) {
+        const int c1 = 1 << perm[t][0];
+        const int c2 = c1 | (1 << perm[t][1]);
+        double l[4];
+        if (!bary4(q, V[0], V[c1], V[c2], V[7], l)) continue;
+        if (fmin(fmin(l[0], l[1]), fmin(l[2], l[3])) >= -1e-10) {
+          found = true;
+          const int cs[4] = {0, c1, c2, 7};
+          for (int v = 0; v < 4; ++v) {
+            lam[v] = l[v];
+            vid[v] = pidx(cell[0] + (cs[v] & 1), cell[1] + ((cs[v] >> 1) & 1), cell[2] + (cs[v] >> 2));
+          }
+        }
+      }
+      if (found) break;
+      // local coordinates in the cell's mean affine frame (mean edge vector
+      // per axis, origin at the cell centroid)
+      double J[3][3], o[3];
+      for (int d = 0; d < 3; ++d) {
+        o[d] = 0.0;
+        for (int cc = 0; cc < 8; ++cc) o[d] += V[cc][d];
+        o[d] *= 0.125;
+      }
+      for (int ax = 0; ax < 3; ++ax)
+        for (int d = 0; d < 3; ++d) {
+          double e = 0.0;
+          for (int cc = 0; cc < 8; ++cc)
+            if (cc & (1 << ax)) e += V[cc][d] - V[cc ^ (1 << ax)][d];
+          J[d][ax] = 0.25 * e;
+        }
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      if (!(fabs(det) > 1e-300)) break;
+      const double r[3] = {q[0] - o[0], q[1] - o[1], q[2] - o[2]};
+      double tl[3];
+      tl[0] = (r[0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (r[1] * J[2][2] - J[1][2] * r[2]) +
+               J[0][2] * (r[1] * J[2][1] - J[1][1] * r[2])) / det;
+      tl[1] = (J[0][0] * (r[1] * J[2][2] - J[1][2] * r[2]) - r[0] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * r[2] - r[1] * J[2][0])) / det;
+      tl[2] = (J[0][0] * (J[1][1] * r[2] - r[1] * J[2][1]) - J[0][1] * (J[1][0] * r[2] - r[1] * J[2][0]) +
+               r[0] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / det;
+      bool moved = false;
+      for (int d = 0; d < 3; ++d) {
+        const double f = floor(tl[d] + 0.5);  // whole cells beyond this one
+        if (f != 0.0) {
+          const double nc = fmin(fmax((double)cell[d] + f, 0.0), (double)(N[d] - 2));
+          if ((int)nc != cell[d]) { cell[d] = (int)nc; moved = true; }
+        }
+      }
+      if (!moved) break;
+    }
+  }
+  for (int c = 0; c < C; ++c) {
+    double v;
+    if (found) {
+      v = 0.0;
+      for (int k = 0; k < 4; ++k) v += lam[k] * field[vid[k] * C + c];
+    } else {
+      const int b = nn_idx[qi];
+      v = b >= 0 ? field[(long long)b * C + c] : __longlong_as_double(0x7ff8000000000000LL);
+    }
+    out[qi * C + c] = v;
+  }
 }
 
 }  // namespace
 
 namespace vdvc {
+
+int launch_material_cells(int Nx, int Ny, int Nz, int h, const void* img, int itype, double glt,
+                          uint8_t* out, hipStream_t st) {
+  const long long Ne = (long long)Nx * Ny * Nz;
+  const unsigned nb = (unsigned)((Ne * 64 + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(material_cells_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, Nx, Ny, Nz, h, img,
+                     itype, glt, out);
+  return (int)hipGetLastError();
+}
+
+int launch_erode_cells(int Nx, int Ny, int Nz, const uint8_t* in, uint8_t* out, hipStream_t st) {
+  const long long Ne = (long long)Nx * Ny * Nz;
+  const unsigned nb = (unsigned)((Ne + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(erode_cells_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, Nx, Ny, Nz, in, out);
+  return (int)hipGetLastError();
+}
+
+int launch_remap_linear(int Nx, int Ny, int Nz, int h, const double* pts, const double* qry,
+                        const double* field, int C, const int* nn_idx, double* out,
+                        hipStream_t st) {
+  const long long Ne = (long long)Nx * Ny * Nz;
+  const unsigned nb = (unsigned)((Ne + VDVC_BLOCK - 1) / VDVC_BLOCK);
+  hipLaunchKernelGGL(remap_linear_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, Nx, Ny, Nz, (double)h,
+                     pts, qry, field, C, nn_idx, out);
+  return (int)hipGetLastError();
+}
 
 int launch_elem_count(const CellIn& C, hipStream_t st) {
   const long long Ne = (long long)C.Nx * C.Ny * C.Nz;
@@ -717,10 +929,10 @@ int launch_nn_build(long long n, const double* pts, const NNGrid& g, int* cnt, i
 
 int launch_nn_query(long long n, const double* pts, const double* qry, const NNGrid& g,
                     const int* off, const int* list, const uint8_t* vals, uint8_t* out,
-                    hipStream_t st) {
+                    int* idx_out, hipStream_t st) {
   const unsigned nb = (unsigned)((n + VDVC_BLOCK - 1) / VDVC_BLOCK);
   hipLaunchKernelGGL(nn_query_kernel, dim3(nb), dim3(VDVC_BLOCK), 0, st, n, pts, qry, g, off,
-                     list, vals, out);
+                     list, vals, out, idx_out);
   return (int)hipGetLastError();
 }
 

@@ -336,9 +336,16 @@ int launch_bbox(long long n, const double* pts, double* part, int nparts, double
 int nn_scan_chunks(long long ncell);
 int launch_nn_build(long long n, const double* pts, const NNGrid& g, int* cnt, int* off,
                     int* csum, int* list, hipStream_t st);
+// vals/out: the nearest point's value (optional); idx_out: its index (optional)
 int launch_nn_query(long long n, const double* pts, const double* qry, const NNGrid& g,
                     const int* off, const int* list, const uint8_t* vals, uint8_t* out,
-                    hipStream_t st);
+                    int* idx_out, hipStream_t st);
+int launch_material_cells(int Nx, int Ny, int Nz, int h, const void* img, int itype, double glt,
+                          uint8_t* out, hipStream_t st);
+int launch_erode_cells(int Nx, int Ny, int Nz, const uint8_t* in, uint8_t* out, hipStream_t st);
+int launch_remap_linear(int Nx, int Ny, int Nz, int h, const double* pts, const double* qry,
+                        const double* field, int C, const int* nn_idx, double* out,
+                        hipStream_t st);
 int launch_seed_input(int itype, const void* f, const void* g, int nx, int ny, int nz, int s,
                       double* part, int nparts, double* stats6, float* out, hipStream_t st);
 }  // namespace vdvc

@@ -11,11 +11,12 @@ names (arguments, return values and layouts), computed by libvdvc.so.
   compute_safe_cell_mask                        write_output.py:352-377
   compute_gn_convergence_cell                   write_output.py:450-491
   remap_element_mask_to_deformed                write_output.py:494-530
+  remap_cell_field_to_deformed                  write_output.py:533-568
+  compute_material_cell_mask                    write_output.py:309-338
+  compute_material_cell_mask_eroded             write_output.py:341-349
 
 (the unsafe / overlap-lost masks are in engine/kernels_dvc.py, as in the
-reference).  Not provided: remap_cell_field_to_deformed (scipy griddata
-'linear' over a Qhull triangulation, write_output.py:533-568), the
-material-cell masks used only by the viewers, and the .vti export.
+reference).  Not provided: the .vti export.
 
 Images are read in their own dtype (float32, float64, uint8, uint16, int16,
 bool), as scipy.ndimage.map_coordinates reads them; pushes return the
@@ -272,9 +273,69 @@ def remap_element_mask_to_deformed(elem_mask, U_grid, h, crop_lo=(0, 0, 0)):
 
 
 def remap_cell_field_to_deformed(field, U_grid, h, crop_lo=(0, 0, 0)):
-    """Not provided: the reference interpolates with scipy griddata
-    'linear' (write_output.py:533-568), a Qhull Delaunay triangulation
-    whose simplices are not unique for the near-lattice element centres, so
-    its values cannot be reproduced by another implementation."""
-    raise NotImplementedError('remap_cell_field_to_deformed (griddata linear over a Qhull '
-                              'triangulation) is not provided')
+    """(Nx, Ny, Nz[, C]) float64 per-element field carried to the deformed
+    configuration (write_output.py:533-568): element centres moved by the
+    mean of their 8 corner displacements; each regular centre + crop_lo
+    takes the linear interpolation of the field over the moved-centre
+    lattice, or the value of the nearest moved centre outside it.  The
+    reference interpolates over scipy's Delaunay tetrahedra of the moved
+    centres; here each lattice cell is split into the 6 tetrahedra around
+    its main diagonal (vdvc_remap_field_deformed).  Both interpolants are
+    piecewise linear through the same values at the same points: they agree
+    exactly on linear fields and to the field's second differences
+    otherwise."""
+    t = L.require_gpu()
+    Nx, Ny, Nz = _grid_dims(U_grid)
+    shp = tuple(int(x) for x in field.shape)
+    if shp[:3] != (Nx, Ny, Nz) or len(shp) not in (3, 4):
+        raise ValueError('field shape %r does not match U_grid (%d, %d, %d)' % (shp, Nx, Ny, Nz))
+    C = shp[3] if len(shp) == 4 else 1
+    u, code = _nodal_from_grid(U_grid)
+    if L.is_tensor(field):
+        f = field.detach().to(device='cuda', dtype=t.float64)
+        f = (f.permute(2, 1, 0, 3) if len(shp) == 4 else f.permute(2, 1, 0)).contiguous().view(-1)
+    else:
+        a = np.asarray(field, dtype=np.float64)
+        a = a.transpose(2, 1, 0, 3) if len(shp) == 4 else a.transpose(2, 1, 0)
+        f = t.from_numpy(np.ascontiguousarray(a).reshape(-1)).to('cuda')
+    out = t.empty(Nx * Ny * Nz * C, dtype=t.float64, device='cuda')
+    crop = (L._d * 3)(*[float(c) for c in crop_lo])
+    L.call('vdvc_remap_field_deformed', L.ctx(), Nx, Ny, Nz, int(h), u, code, crop, f, C, out)
+    o = out.view(Nz, Ny, Nx, C).permute(2, 1, 0, 3)
+    if len(shp) == 3:
+        o = o[..., 0]
+    return o if L.is_tensor(field) else L.to_host(o.contiguous())
+
+
+def compute_material_cell_mask(ref, h, Nx, Ny, Nz, glt=0.0):
+    """(Nx, Ny, Nz) bool: True where the mean reference gray level over the
+    element's (h+1)^3 voxels exceeds glt (write_output.py:309-338).  ref is
+    the (Nvx, Nvy, Nvz) reference volume in its own dtype; float64 block
+    sums (exact for integer images)."""
+    t = L.require_gpu()
+    d, shp, code = L.to_dev_image(ref)
+    want = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    if tuple(shp) != want:
+        raise ValueError('ref shape %r does not match the voxel grid %r of the mesh'
+                         % (tuple(shp), want))
+    out = t.empty(Nx * Ny * Nz, dtype=t.uint8, device='cuda')
+    L.call('vdvc_material_cells', L.ctx(), Nx, Ny, Nz, int(h), d, code, float(glt), out)
+    r = _cell_out(out, ref, Nx, Ny, Nz)
+    return r.bool() if L.is_tensor(ref) else r.astype(bool)
+
+
+def compute_material_cell_mask_eroded(material_cell_mask):
+    """One-cell erosion of an (Nx, Ny, Nz) element mask over the
+    26-neighbourhood, the mesh border counting as material
+    (write_output.py:341-349)."""
+    t = L.require_gpu()
+    Nx, Ny, Nz = (int(x) for x in material_cell_mask.shape)
+    if L.is_tensor(material_cell_mask):
+        m = material_cell_mask.detach().to('cuda').permute(2, 1, 0).reshape(-1).to(t.uint8).contiguous()
+    else:
+        m = t.from_numpy(np.ascontiguousarray(np.asarray(material_cell_mask, dtype=bool)
+                                              .transpose(2, 1, 0)).reshape(-1).astype(np.uint8)).to('cuda')
+    out = t.empty(Nx * Ny * Nz, dtype=t.uint8, device='cuda')
+    L.call('vdvc_erode_cells', L.ctx(), Nx, Ny, Nz, m, out)
+    r = _cell_out(out, material_cell_mask, Nx, Ny, Nz)
+    return r.bool() if L.is_tensor(material_cell_mask) else r.astype(bool)

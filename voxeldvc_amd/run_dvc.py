@@ -19,10 +19,11 @@ ref_forward_pushed.npy (ref pushed by U_local), residual_std_cell.npy and
 zncc_cell.npy; and the element-scale fields (engine/write_output.py:
 790-823, 846-906): unsafe_voxel_mask.npy, overlap_lost_element_mask.npy and
 its deformed-configuration remap, principal_strains_cell.npy,
-equivalent_strain_cell.npy, gn_convergence_cell.npy,
-unsafe_voxel_mask_forward_pushed.npy.  Not produced: the two
-deformed-configuration strain remaps (scipy griddata 'linear' over a Qhull
-triangulation) and the .vti export.
+equivalent_strain_cell.npy and their deformed-configuration remaps
+principal_strains_cell_deformed.npy / equivalent_strain_cell_deformed.npy,
+gn_convergence_cell.npy, unsafe_voxel_mask_forward_pushed.npy,
+material_cell_mask.npy and material_cell_mask_eroded.npy.  Not produced:
+the .vti export.
 
     python -m voxeldvc_amd run WORK_DIR [--l0-factor F]
 """
@@ -74,9 +75,9 @@ def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt,
     (engine/write_output.py:693-929): active mask and global ZNCC, unsafe
     voxel mask and safe-cell count, overlap-lost element mask, principal and
     equivalent strains, per-cell residual std / ZNCC / GN convergence, the
-    overlap-lost mask carried to the deformed configuration, the forward
-    pushes of ref and of the unsafe mask.  Not produced: the
-    deformed-configuration strain remaps (griddata 'linear') and the .vti."""
+    overlap-lost mask and the strains carried to the deformed configuration,
+    the forward pushes of ref and of the unsafe mask, the material-cell
+    masks.  Not produced: the .vti."""
     from .engine.correlate_gpu import compute_active_mask
     from .engine import kernels_dvc as K
     from .engine import write_output as W
@@ -119,7 +120,13 @@ def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt,
         comp = ps[..., c]
         print('  cell %s: mean=%.5f  std=%.5f  min=%.5f  max=%.5f'
               % (name, comp.mean(), comp.std(), comp.min(), comp.max()))
+    ps_def = W.remap_cell_field_to_deformed(ps, grid, h, crop_lo=crop_lo)
+    for c, name in enumerate(('e1 (max)', 'e2 (mid)', 'e3 (min)')):
+        comp = ps_def[..., c]
+        print('  deformed-config cell %s: mean=%.5f  std=%.5f  min=%.5f  max=%.5f'
+              % (name, comp.mean(), comp.std(), comp.min(), comp.max()))
     eq = W.compute_equivalent_strain(ps)
+    eq_def = W.compute_equivalent_strain(ps_def)
     print('  cell eq. strain: mean=%.5f  std=%.5f  min=%.5f  max=%.5f'
           % (eq.mean(), eq.std(), eq.min(), eq.max()))
     std_c, zn_c = W.compute_correlation_quality_cell(res, ref, h, Nx, Ny, Nz)
@@ -148,9 +155,19 @@ def post_outputs(wd, U, U_local, res, ref, deformed, Nx, Ny, Nz, h, lo_def, glt,
     print('Unsafe voxels (forward-pushed to deformed grid): %d/%d (%.1f%%)'
           % (int(unsafe_pushed.sum()), unsafe_pushed.size,
              100 * unsafe_pushed.sum() / unsafe_pushed.size))
+    mat = W.compute_material_cell_mask(ref, h, Nx, Ny, Nz, glt=glt)
+    print('Material cells (mean reference gray > glt): %d/%d (%.1f%%)'
+          % (int(mat.sum()), mat.size, 100 * mat.sum() / mat.size))
+    mat_ero = W.compute_material_cell_mask_eroded(mat)
+    print('Material cells (eroded, no non-material neighbour): %d/%d (%.1f%%)'
+          % (int(mat_ero.sum()), mat_ero.size, 100 * mat_ero.sum() / mat_ero.size))
     save = lambda name, a: np.save(os.path.join(wd, name + '.npy'), a)
     save('principal_strains_cell', ps)
+    save('principal_strains_cell_deformed', ps_def)
     save('equivalent_strain_cell', eq)
+    save('equivalent_strain_cell_deformed', eq_def)
+    save('material_cell_mask', mat)
+    save('material_cell_mask_eroded', mat_ero)
     save('residual_std_cell', std_c)
     save('zncc_cell', zn_c)
     if conv is not None:
