@@ -103,16 +103,21 @@ int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
  * the new mask when d_bmask is NULL).  Replaces sample_deformed_field
  * (engine/correlate_gpu.py:201-281) fused with the b-mask statistics of
  * correlate_gpu (:1091).  d_g gitype (F32/F16/F64); d_U and d_gvals in the
- * working type of gitype. */
+ * working type of gitype.  Interpolation weights: f32_weights = 0 forms them
+ * and the trilinear combination in float64 in scipy's map_coordinates order
+ * (the reference's values bit for bit); 1 in float32 (the mixed-precision
+ * form; float32 / float16 images only). */
 int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, const void* d_g,
                 int gitype, int gnx, int gny, int gnz, double gox, double goy, double goz,
                 double glt, const uint8_t* d_mask0, const uint8_t* d_bmask, void* d_gvals,
-                uint8_t* d_mask_out, double* h_stats3);
+                uint8_t* d_mask_out, double* h_stats3, int f32_weights);
 
 /* Residual res = ((g - gmean) * kscale + (f - mean0)) zeroed outside d_bmask
  * (working type, the reference's operation order), RHS b = sum W bmask res v_eq,
  * and (weighted count, sum, sum sq, plain voxel count) of res over d_live,
- * multiplicity-weighted except the last.  If d_fold_x is non-NULL
+ * multiplicity-weighted except the last, into h_stats4 -- only when
+ * h_stats4 is non-NULL (the reference forms them only to print them,
+ * engine/correlate_gpu.py:1132-1136; d_live is then not read).  If d_fold_x is non-NULL
  * also writes bfloat16 planes g * bmask.  Replaces correlate_gpu.py:1092-1109
  * + compute_b_gpu (:341-378) + the display statistics (:1133). */
 int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,

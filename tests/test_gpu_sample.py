@@ -15,7 +15,7 @@ SHAPES = [(10, 3, 2, 8), (5, 3, 2, 3), (70, 1, 2, 1), (4, 2, 3, 5), (2, 2, 2, 16
           (8, 3, 2, 8), (32, 1, 2, 4)]
 
 
-def _run_sample(U, g, gcode, shp_g, Nx, Ny, Nz, h, go, m0, bm, glt):
+def _run_sample(U, g, gcode, shp_g, Nx, Ny, Nz, h, go, m0, bm, glt, f32w=False):
     from voxeldvc_amd import _lib as L
     from voxeldvc_amd.engine import kernels_dvc as K
     shp = K.level_shape(h, Nx, Ny, Nz)
@@ -28,7 +28,7 @@ def _run_sample(U, g, gcode, shp_g, Nx, Ny, Nz, h, go, m0, bm, glt):
     s = (L._d * 3)()
     L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), gcode, shp_g[0], shp_g[1],
            shp_g[2], float(go[0]), float(go[1]), float(go[2]), float(glt), L.ptr(d0), L.ptr(db),
-           L.ptr(gv), L.ptr(mk), s)
+           L.ptr(gv), L.ptr(mk), s, int(f32w))
     return (L.to_host(L.unpitch(gv, shp)), L.to_host(L.unpitch(mk, shp)).astype(bool),
             np.array(list(s)))
 
@@ -96,6 +96,30 @@ def test_sample_matches_oracle(gpu, shape):
     _, mk2, st2 = _run_sample(U, g32, L.F32, gs, Nx, Ny, Nz, h, go, m0, None, glt=30.0)
     assert np.array_equal(mk2, mk)
     assert np.allclose(st2, _stats_np(gv, mk, Nx, Ny, Nz, h), rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+def test_sample_f32_weights(gpu, shape):
+    """The mixed-precision form's float32 interpolation weights against the
+    float64 (SciPy-order) weights on the same coordinates: values within
+    float32 rounding of the 0..255 range, the mask equal wherever the
+    value is not within that rounding of the threshold, statistics of the
+    kernel's own values; float16 and float32 images identical."""
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = shape
+    print('sample f32 weights', shape, flush=True)
+    gimg, U, m0, bm = _problem(24, Nx, Ny, Nz, h)
+    g32, gs = L.to_dev_volume(gimg, 'float32')
+    g16, _ = L.to_dev_volume(gimg, 'float16')
+    go = (1, 0, -1)
+    a = _run_sample(U, g32, L.F32, gs, Nx, Ny, Nz, h, go, m0, bm, 30.0)
+    b = _run_sample(U, g32, L.F32, gs, Nx, Ny, Nz, h, go, m0, bm, 30.0, f32w=True)
+    c = _run_sample(U, g16, L.F16, gs, Nx, Ny, Nz, h, go, m0, bm, 30.0, f32w=True)
+    assert np.abs(a[0] - b[0]).max() <= 256 * 2.0 ** -22
+    near = np.abs(a[0] - 30.0) <= 256 * 2.0 ** -22
+    assert np.array_equal(a[1][~near], b[1][~near])
+    assert np.allclose(b[2], _stats_np(b[0], bm, Nx, Ny, Nz, h), rtol=1e-12, atol=0)
+    assert all(np.array_equal(x, y) for x, y in zip(b, c))
 
 
 def test_sample_nan_displacement(gpu):

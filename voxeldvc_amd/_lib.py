@@ -40,7 +40,7 @@ SIGNATURES = {
     'vdvc_image_setup': [_vp, _i, _i, _i, _i, _vp, _i, _d, _vp, _vp, _vp, _i, _vp],
     'vdvc_masked_stats': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _d, _dp],
     'vdvc_sample': [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _d, _d, _d, _d, _vp, _vp,
-                    _vp, _vp, _dp],
+                    _vp, _vp, _dp, _i],
     'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d, _d, _vp,
                           _vp, _vp, _vp, _vp, _vp, _vp, _dp],
     'vdvc_matvec_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/vdvc.h"
@@ -181,8 +182,13 @@ TilesDev make_tiles_generic(const MeshDev& M) {
 // decomposition of the voxel-streaming Hessian kernels: vectorised strips of
 // 64*V voxels over [0, Nx*h) when h is a power of two, else generic
 TilesDev make_tiles(const MeshDev& M, int gtype) {
-  const int V = vdvc::vec_width(M.h, gtype);
+  int V = vdvc::vec_width(M.h, gtype);
   if (V == 0) return make_tiles_generic(M);
+  // a row narrower than a 64 V-voxel strip would leave lanes idle: narrow
+  // the lanes (V stays a power of two dividing h) until the strip fits
+  while (V > 1 && 64 * V > M.Nx * M.h && getenv("VDVC_VEC_BF16") == nullptr &&
+         getenv("VDVC_VEC_F32") == nullptr)
+    V >>= 1;
   TilesDev T;
   T.vec = V;
   T.sw = 64 * V;
@@ -201,6 +207,16 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   if (split && M.h >= 4 && (M.h % 2) == 0) {
     T.nys = 2;
     plan_z(M, T, 8192);
+  }
+  // tuning override of the element layers per wavefront, read at every
+  // plan: VDVC_EZ = layers (>= 1)
+  const char* ez = getenv("VDVC_EZ");
+  const int eo = ez ? atoi(ez) : 0;
+  if (eo >= 1) {
+    T.ez = std::min(eo, M.Nz);
+    T.nzt = (M.Nz + T.ez - 1) / T.ez;
+    T.wave_stride = (long long)(T.ez + 1) * 2 * T.px * 3;
+    T.nwaves = (long long)T.nsx * M.Ny * T.nys * T.nzt;
   }
   return T;
 }
@@ -412,10 +428,10 @@ int residual_rhs_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gt
   NodeOp<R> op = node_op0<R>();
   op.P = ctx->P.as<const R>(); op.T = T; op.out = d_b;
   VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
+  if (!h_stats4) return 0;
   VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 4, ctx->red.as<double>(),
                                   ctx->stream));
-  if (h_stats4) return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
-  return 0;
+  return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
 }
 
 template <typename R>
@@ -734,8 +750,10 @@ int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
 int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, const void* d_g,
                 int gitype, int gnx, int gny, int gnz, double gox, double goy, double goz,
                 double glt, const uint8_t* d_mask0, const uint8_t* d_bmask, void* d_gvals,
-                uint8_t* d_mask_out, double* h_stats3) {
+                uint8_t* d_mask_out, double* h_stats3, int f32_weights) {
   if (!ctx) return fail(-1, "ctx is NULL");
+  if (f32_weights && gitype == VDVC_F64)
+    return fail(-1, "float32 interpolation weights need a float32 or float16 image");
   int rc = check_mesh(Nx, Ny, Nz, h);
   if (rc) return rc;
   if (!ftype_ok(gitype)) return fail(-1, "unsupported image type");
@@ -758,7 +776,7 @@ int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, c
   S.gox = gox; S.goy = goy; S.goz = goz; S.glt = glt;
   S.mask0 = d_mask0; S.bmask = d_bmask; S.gvals = d_gvals; S.mask_out = d_mask_out;
   S.stats = ctx->stats.as<double>();
-  VDVC_LAUNCH(vdvc::launch_sample(gitype, M, T, S, d_U, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_sample(gitype, M, T, S, d_U, f32_weights != 0, ctx->stream));
   VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 3, ctx->red.as<double>(),
                                   ctx->stream));
   if (h_stats3) return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_stats3);
@@ -791,7 +809,8 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   in.mean0 = mean0; in.gmean = gmean; in.kscale = kscale;
   in.res = d_res;
   in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
-  in.stats = ctx->stats.as<double>();
+  // the display statistics only when the caller asks for them
+  in.stats = h_stats4 ? ctx->stats.as<double>() : nullptr;
   return VDVC_BY_RTYPE(gtype,
                        residual_rhs_impl<float>(ctx, M, T, gtype, in, ftype, fold,
                                                 static_cast<float*>(d_b), h_stats4),

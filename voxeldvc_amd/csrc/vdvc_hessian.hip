@@ -343,6 +343,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
         for (int k = 0; k < 3; ++k) C[a][b][c][k] = 0.0f;
 
   double st_c = 0.0, st_1 = 0.0, st_2 = 0.0, st_n = 0.0;
+  // residual pass: display statistics requested (uniform across the grid)
+  const bool stats = MODE == MODE_RHS && in.stats != nullptr;
 
   auto emit = [&](int kzl) {
     // lanes of one element hold partial sums of that element's two x nodes
@@ -424,7 +426,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     if (MODE == MODE_RHS) {
       ld_raw<FT, V>(fim, rb + xbc, b.f);
       ld_raw<float, V>(gvals_f, rb + xbc, b.gv);
-      ld_raw<uint8_t, V>(in.live, rb + xbc, b.lv);
+      // the live mask feeds only the display statistics
+      if (stats) ld_raw<uint8_t, V>(in.live, rb + xbc, b.lv);
     }
     if (MODE != MODE_RHS) {
       b.t[0] = ld_raw1<GT>(g0, rb + xend);
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
       unpack<FT, V>(b.f, o.fv);
       unpack<float, V>(b.gv, o.gv);
 #pragma unroll
-      for (int j2 = 0; j2 < V; ++j2) o.lv[j2] = mask_bit<V>(b.lv, j2);
+      for (int j2 = 0; j2 < V; ++j2) o.lv[j2] = stats ? mask_bit<V>(b.lv, j2) : 0u;
     }
   };
   auto process = [&](const RowVals& cur) {
@@ -635,7 +638,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
                              sub_rn(fv, mean0_f));
         rv = tm ? rv : 0.0f;
         res_f[ti] = rv;
-        if (in.live[ti]) {
+        if (stats && in.live[ti]) {
           const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
           st_c += mlt;
           st_1 += mlt * (double)rv;
@@ -720,7 +723,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     }
   }
   emit(ez1 - ez0);
-  if (MODE == MODE_RHS) {
+  if (stats) {
     st_c = wave_sum(st_c);
     st_1 = wave_sum(st_1);
     st_2 = wave_sum(st_2);

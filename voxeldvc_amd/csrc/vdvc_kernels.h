@@ -233,8 +233,9 @@ int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold,
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                     const TilesDev& T, const VoxIn& in, const void* v, void* P, hipStream_t st);
 // U and gvals are double for an F64 image, else float
+// f32w: float32 interpolation weights (float and half images)
 int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,
-                  const void* U, hipStream_t st);
+                  const void* U, bool f32w, hipStream_t st);
 int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, double glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st);
 int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
@@ -261,6 +262,7 @@ template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
                       const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
                       hipStream_t st);
+
 template <typename R>
 int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
                        const R* z, R* p, hipStream_t st);

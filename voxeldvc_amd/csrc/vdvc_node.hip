@@ -9,6 +9,7 @@
 //   Jacobi        engine/correlate_gpu.py:1042-1047, 1113-1120
 //   PCG           engine/kernels_dvc.py:550-583
 #include <algorithm>
+#include <cstdlib>
 
 #include "vdvc_kernels.h"
 
@@ -165,6 +166,77 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
     if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
 }
 
+// gather_partials with its loads issued in two batches (one z tile each, 8
+// slots): the same slots summed in the same order, half the values live
+template <typename R>
+__device__ __forceinline__ void gather_partials_lean(const MeshDev& M, const TilesDev& T,
+                                                     const R* __restrict__ P, int i, int j,
+                                                     int k, R (&s)[3]) {
+  const int h = M.h;
+  const int kz = k / T.ez;
+  const int za = max(0, kz - 1), zb = min(T.nzt - 1, kz);
+  int xa, xb;
+  if (T.vec) {
+    xa = (max(i - 1, 0) * h) / T.sw;
+    xb = (min(i, M.Nx - 1) * h) / T.sw;
+  } else {
+    xa = max(0, (i - 1) * h) / 64;
+    xb = min(M.Nvx - 1, (i + 1) * h) / 64;
+  }
+  int ln[2];
+  bool okx[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int sx = xa + q;
+    const int x0 = sx * T.sw;
+    const int i_lo = x0 / h;
+    int nn;
+    if (T.vec) {
+      nn = min(M.Nx, (x0 + T.sw) / h) - i_lo + 1;
+    } else {
+      const int nact = min(64, M.Nvx - x0);
+      nn = min(M.Nx, (x0 + nact - 1) / h + 1) - i_lo + 1;
+    }
+    ln[q] = i - i_lo;
+    okx[q] = sx <= xb && ln[q] >= 0 && ln[q] < nn;
+  }
+  s[0] = s[1] = s[2] = R(0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int zt = za + a;
+    const int kb = zt * T.ez;
+    const bool okz = zt <= zb && k >= kb && k <= min(M.Nz, kb + T.ez);
+    const int kzl = k - kb;
+    const R* ptr[8];
+    bool ok[8];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int cy = 1 - b;
+      const int ey = j - cy;
+      const bool oky = okz && ey >= 0 && ey < M.Ny;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = c >> 1, ys = c & 1;
+        const bool okc = oky && okx[q] && ys < T.nys;
+        const long long w = (long long)(xa + q) +
+                            (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
+        const int e = b * 4 + c;
+        ok[e] = okc;
+        ptr[e] = okc ? P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln[q]) * 3
+                     : P;
+      }
+    }
+    R val[8][3];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
+  }
+}
+
 // node (i, j, k) of flat node index n (< 2^31: checked when a level is set up)
 __device__ __forceinline__ void node_ijk(const MeshDev& M, unsigned n, int& i, int& j, int& k) {
   const unsigned r = n / (unsigned)M.nnx;
@@ -224,6 +296,35 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R
   }
 }
 
+
+// The node gather of the Hessian matvec alone (no regularizer): the
+// partials' loads in two batches (gather_partials_lean), so the kernel
+// needs few registers; the same sums in the same order as node_op_kernel.
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void node_gather_kernel(MeshDev M, NodeOp<R> op) {
+  __shared__ double lds[4];
+  if (op.done && *op.done) return;
+  const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  double d = 0.0;
+  if (n < M.Nnodes) {
+    int i, j, k;
+    node_ijk(M, (unsigned)n, i, j, k);
+    R sp[3];
+    gather_partials_lean(M, op.T, op.P, i, j, k, sp);
+    const R acc[3] = {op.cH * sp[0], op.cH * sp[1], op.cH * sp[2]};
+    op.out[3 * n] = acc[0];
+    op.out[3 * n + 1] = acc[1];
+    op.out[3 * n + 2] = acc[2];
+    if (op.dot_part)
+      d = (double)op.dotv[3 * n] * acc[0] + (double)op.dotv[3 * n + 1] * acc[1] +
+          (double)op.dotv[3 * n + 2] * acc[2];
+  }
+  if (op.dot_part) {
+    double vv[1] = {d};
+    block_sum_f64<1>(vv, lds);
+    if (threadIdx.x == 0) op.dot_part[blockIdx.x] = vv[0];
+  }
+}
 
 template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp<R> op) {
@@ -393,7 +494,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
   const R bf = (R)beta;
   for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
        q += (long long)gridDim.x * VDVC_BLOCK)
-    p[q] = z[q] + bf * p[q];
+    p[q] = fma(bf, p[q], z[q]);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S->rz[(it + 1) & 1] = rzn;
     S->relres = res;
@@ -527,7 +628,13 @@ int launch_hutch_final(long long ndof, const double* acc, int nprobes, R* out, d
 
 template <typename R>
 int launch_node_op(const MeshDev& M, const NodeOp<R>& op, hipStream_t st) {
-  hipLaunchKernelGGL(node_op_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
+  // the gather alone takes the low-register kernel
+  const bool gather_only = op.P && !op.addin && !op.k1_on && !op.k2_on && !op.interior;
+  if (gather_only)
+    hipLaunchKernelGGL(node_gather_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M,
+                       op);
+  else
+    hipLaunchKernelGGL(node_op_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
   return (int)hipGetLastError();
 }
 

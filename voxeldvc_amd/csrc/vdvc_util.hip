@@ -114,6 +114,58 @@ __global__ __launch_bounds__(VDVC_BLOCK) void copy_rows_kernel(VolCopy C) {
   }
 }
 
+// Element conversion of the unit-stride fast path: the same single
+// rounding as st_code for the pairs it is instantiated for.
+template <typename T> struct Elem;
+template <> struct Elem<float> {
+  static __device__ __forceinline__ float ld(const float* p, long long i) { return p[i]; }
+  static __device__ __forceinline__ void st(float* p, long long i, float v) { p[i] = v; }
+};
+template <> struct Elem<double> {
+  static __device__ __forceinline__ double ld(const double* p, long long i) { return p[i]; }
+  static __device__ __forceinline__ void st(double* p, long long i, double v) { p[i] = v; }
+};
+template <> struct Elem<__half> {
+  static __device__ __forceinline__ float ld(const __half* p, long long i) { return __half2float(p[i]); }
+  static __device__ __forceinline__ void st(__half* p, long long i, float v) { p[i] = __float2half_rn(v); }
+};
+template <> struct Elem<uint8_t> {
+  static __device__ __forceinline__ uint32_t ld(const uint8_t* p, long long i) { return p[i]; }
+  static __device__ __forceinline__ void st(uint8_t* p, long long i, uint32_t v) { p[i] = (uint8_t)v; }
+};
+
+// Unit-x-stride copy (sx == 1) specialised on the element types: each
+// wavefront takes whole rows (grid-stride); its lanes cover consecutive
+// voxels, four 64-voxel chunks of a row in flight at once, so every load and
+// store instruction moves one contiguous run; padding columns are written as
+// zero.
+template <typename ST, typename DT, typename W>
+__global__ __launch_bounds__(VDVC_BLOCK) void copy_rows_fast_kernel(VolCopy C) {
+  const ST* __restrict__ src = reinterpret_cast<const ST*>(C.src);
+  DT* __restrict__ dst = reinterpret_cast<DT*>(C.dst);
+  const int lane = threadIdx.x & 63;
+  const long long nrows = (long long)C.ny * C.nz;
+  for (long long row = (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + (threadIdx.x >> 6);
+       row < nrows; row += (long long)gridDim.x * VDVC_WAVES_PER_BLOCK) {
+    const long long y = row % C.ny, z = row / C.ny;
+    const long long sb = y * C.sy + z * C.sz;
+    const long long db = (long long)C.dp * row;
+    for (int x0 = 0; x0 < C.dp; x0 += 256) {
+      W v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = x0 + 64 * j + lane;
+        v[j] = x < C.nx ? (W)Elem<ST>::ld(src, sb + x) : (W)0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = x0 + 64 * j + lane;
+        if (x < C.dp) Elem<DT>::st(dst, db + x, v[j]);
+      }
+    }
+  }
+}
+
 // The same copy for a source whose z stride is 1 (a C-order [x][y][z]
 // array): a 64 x 64 (x, z) tile of one y row is read along z, transposed in
 // LDS, and written along x.
@@ -238,6 +290,26 @@ int launch_fill(long long n, void* dst, int dt, double v, hipStream_t s) {
 }
 
 int launch_copy_volume(const VolCopy& C, hipStream_t s) {
+  if (C.sx == 1) {
+    const long long nrows = (long long)C.ny * C.nz;
+    long long nb = (nrows + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK;
+    if (nb > 8192) nb = 8192;
+    const dim3 g((unsigned)nb), b(VDVC_BLOCK);
+#define VC_FAST(STC, DTC, ST, DT, W)                                              \
+    if (C.st == STC && C.dt == DTC) {                                             \
+      hipLaunchKernelGGL((copy_rows_fast_kernel<ST, DT, W>), g, b, 0, s, C);     \
+      return (int)hipGetLastError();                                            \
+    }
+    VC_FAST(GT_F32, GT_F32, float, float, float)
+    VC_FAST(GT_F32, GT_F16, float, __half, float)
+    VC_FAST(GT_F16, GT_F16, __half, __half, float)
+    VC_FAST(GT_F16, GT_F32, __half, float, float)
+    VC_FAST(GT_F32, GT_F64, float, double, double)
+    VC_FAST(GT_F64, GT_F64, double, double, double)
+    VC_FAST(GT_F64, GT_F32, double, float, double)
+    VC_FAST(GT_U8, GT_U8, uint8_t, uint8_t, uint32_t)
+#undef VC_FAST
+  }
   if (C.sz == 1 && C.sx != 1 && C.nz > 1) {
     const int ntx = (C.nx + 63) / 64, ntz = (C.nz + 63) / 64;
     hipLaunchKernelGGL(copy_transpose_kernel, dim3(ntx * ntz, C.ny), dim3(VDVC_BLOCK), 0, s, C);

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
             rv = add_rn(mul_rn(sub_rn(gv, gmean), kscale), sub_rn(fval, mean0));
             if (!bm) rv = R(0);
             resv[vi] = rv;
-            if (in.live[vi]) {
+            if (in.stats && in.live[vi]) {
               const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
               const double rd = (double)rv;
               st_c += mlt;
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
     emit_plane<POW2, R>(c, a0w, a1w, lane, h, x0, nact, i_lo, nn,
                         Pw + (size_t)(ez1 - ez0) * 2 * T.px * 3, T.px, lds);
   }
-  if (MODE == MODE_RHS) {
+  if (MODE == MODE_RHS && in.stats) {
     st_c = wave_sum(st_c);
     st_1 = wave_sum(st_1);
     st_2 = wave_sum(st_2);
@@ -400,6 +400,18 @@ __device__ __forceinline__ void tri_fetch(const GI* __restrict__ img, __amdgpu_b
   }
 }
 
+// the same combination in float32 (the mixed-precision form's weights):
+// same order of operations, each rounded to float
+template <typename GI>
+__device__ __forceinline__ float tri_combine_f32(const TriTaps<GI>& t) {
+  const float fx = (float)t.fx, fy = (float)t.fy, fz = (float)t.fz;
+  const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
+  const float v000 = t.v[0], v100 = t.v[1], v010 = t.v[2], v110 = t.v[3];
+  const float v001 = t.v[4], v101 = t.v[5], v011 = t.v[6], v111 = t.v[7];
+  return gz0 * (gy0 * (gx0 * v000 + fx * v100) + fy * (gx0 * v010 + fx * v110)) +
+         fz * (gy0 * (gx0 * v001 + fx * v101) + fy * (gx0 * v011 + fx * v111));
+}
+
 // sample_tri's combination, same operations in the same order
 template <typename GI>
 __device__ __forceinline__ double tri_combine(const TriTaps<GI>& t) {
@@ -427,8 +439,10 @@ __device__ __forceinline__ R clamp_to(R c, R hi) {
 constexpr int kSampleRows = 2;
 // The displacement at a voxel is interpolated with explicit fused
 // multiply-adds, so that its rounding does not depend on how the compiler
-// contracts the expression.
-template <typename GI, bool O32>
+// contracts the expression.  F32W: the eight interpolation weights and the
+// combination in float32 (the mixed-precision form) instead of SciPy's
+// float64 order.
+template <typename GI, bool O32, bool F32W>
 __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
     MeshDev M, TilesDev T, SampleIn S, const typename RealOf<GI>::type* __restrict__ U) {
   using R = typename RealOf<GI>::type;
@@ -519,7 +533,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
       auto finish = [&](int ty, const TriTaps<GI>& tp, bool inb, uint8_t m0b, uint8_t bmb) {
         const long long vi = base + (long long)ty * M.pitch;
         const int y = y0 + ty;
-        const R gv = (R)tri_combine(tp);
+        const R gv = F32W ? (R)tri_combine_f32(tp) : (R)tri_combine(tp);
         const bool m0 = mask0 ? (m0b != 0) : true;
         const bool mk = inb && m0 && (gv > glt);
         if (act) {
@@ -606,7 +620,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
         TriTaps<GI> tp;
         tri_fetch<O32>(img, rs, S.gnx, S.gny, S.gnz, cx, cy, cz, tp);
         const long long vi = ((long long)z * M.Nvy + y) * M.pitch + xt;
-        const R gv = (R)tri_combine(tp);
+        const R gv = F32W ? (R)tri_combine_f32(tp) : (R)tri_combine(tp);
         const bool m0 = mask0 ? (mask0[vi] != 0) : true;
         const bool mk = inb && m0 && (gv > glt);
         gvals[vi] = gv;
@@ -1023,21 +1037,30 @@ int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, con
 }
 
 int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleIn& S,
-                  const void* U, hipStream_t st) {
+                  const void* U, bool f32w, hipStream_t st) {
   const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
   // 32-bit buffer offsets: float and half images below 2^32 bytes
   const long long nimg = (long long)S.gnx * S.gny * S.gnz;
   const bool o32 = gitype == GT_F32 ? nimg < (1LL << 30) : gitype == GT_F64 ? false : nimg < (1LL << 31);
-#define VDVC_SAMPLE_K(GI) (o32 ? sample_kernel<GI, true> : sample_kernel<GI, false>)
-  if (gitype == GT_F32)
-    hipLaunchKernelGGL(VDVC_SAMPLE_K(float), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
-                       static_cast<const float*>(U));
-  else if (gitype == GT_F64)
-    hipLaunchKernelGGL(VDVC_SAMPLE_K(double), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
+#define VDVC_SAMPLE_K(GI, W) (o32 ? sample_kernel<GI, true, W> : sample_kernel<GI, false, W>)
+  if (gitype == GT_F32) {
+    if (f32w)
+      hipLaunchKernelGGL(VDVC_SAMPLE_K(float, true), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
+                         static_cast<const float*>(U));
+    else
+      hipLaunchKernelGGL(VDVC_SAMPLE_K(float, false), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T,
+                         S, static_cast<const float*>(U));
+  } else if (gitype == GT_F64) {
+    hipLaunchKernelGGL(VDVC_SAMPLE_K(double, false), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
                        static_cast<const double*>(U));
-  else
-    hipLaunchKernelGGL(VDVC_SAMPLE_K(__half), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
-                       static_cast<const float*>(U));
+  } else {
+    if (f32w)
+      hipLaunchKernelGGL(VDVC_SAMPLE_K(__half, true), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T,
+                         S, static_cast<const float*>(U));
+    else
+      hipLaunchKernelGGL(VDVC_SAMPLE_K(__half, false), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T,
+                         S, static_cast<const float*>(U));
+  }
 #undef VDVC_SAMPLE_K
   return (int)hipGetLastError();
 }

@@ -108,7 +108,7 @@ def sample_deformed_field(U, g_pix_x, N_stencil, h, Nx, Ny, Nz, xp=np, g_origin=
     s = (L._d * 3)()
     L.call('vdvc_sample', L.ctx(), Nx, Ny, Nz, h, L.ptr(dU), L.ptr(g), code, gshp[0], gshp[1],
            gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]), float(glt),
-           L.ptr(m0), None, L.ptr(gv), L.ptr(mk), s)
+           L.ptr(m0), None, L.ptr(gv), L.ptr(mk), s, 0)
     gv = L.unpitch(gv, shp)
     mk = L.unpitch(mk, shp)
     dev = L.is_tensor(U) or L.is_tensor(g_pix_x)
@@ -685,7 +685,7 @@ class _Level:
             L.call('vdvc_sample', c, Nx, Ny, Nz, h, L.ptr(U), L.ptr(self.g), self.gitype,
                    self.gshp[0], self.gshp[1], self.gshp[2], float(g_origin[0]),
                    float(g_origin[1]), float(g_origin[2]), float(glt), L.ptr(mask0),
-                   L.ptr(frozen), L.ptr(gvals), L.ptr(mask), s3)
+                   L.ptr(frozen), L.ptr(gvals), L.ptr(mask), s3, int(self.storage == 'bf16'))
             if freeze_mask_after is not None and frozen is None and ik >= freeze_mask_after:
                 frozen = L.clone(mask)
             bmask = frozen if frozen is not None else mask
@@ -699,7 +699,7 @@ class _Level:
                    rnd(g_mean), rnd(kscale), L.ptr(bmask),
                    L.ptr(mask), L.ptr(res), L.ptr(b),
                    L.ptr(Gw[0]) if refold else None, L.ptr(Gw[1]) if refold else None,
-                   L.ptr(Gw[2]) if refold else None, s4)
+                   L.ptr(Gw[2]) if refold else None, s4 if (disp or log is not None) else None)
             if l0 is not None:
                 reg.matvec(self.M, U, RU)
                 K.axpby(-llf, RU, 1.0, b)
@@ -774,7 +774,6 @@ class _Level:
         b32 = L.empty(Ndof, 'float32')
         RU = L.empty(Ndof, 'float64')
         dU = L.empty(Ndof, 'float32')
-        s4 = (L._d * 4)()
         it_ = (L._i * 1)()
         rr_ = (L._d * 1)()
         bfix = frozen
@@ -782,7 +781,7 @@ class _Level:
         for k in range(nsteps + 1):
             L.call('vdvc_sample', c, Nx, Ny, Nz, h, U64, g64, L.F64, self.gshp[0], self.gshp[1],
                    self.gshp[2], float(g_origin[0]), float(g_origin[1]), float(g_origin[2]),
-                   float(glt), mask0, bfix, gvals, live, s3)
+                   float(glt), mask0, bfix, gvals, live, s3, 0)
             if bfix is None:
                 # no frozen mask: hold the mask at the converged U fixed
                 bfix = L.empty(Nvox, 'uint8')
@@ -792,7 +791,7 @@ class _Level:
             g_mean, std1 = _stats_mean_std(s3)
             L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, G64[0], G64[1], G64[2], L.F64, f64,
                    L.F64, float(mean0), gvals, float(g_mean), float(-(std0 / std1)), bfix,
-                   live, res, b64, None, None, None, s4)
+                   live, res, b64, None, None, None, None)
             if use_reg:
                 reg.matvec(self.M, U64, RU)
                 K.axpby(-llf, RU, 1.0, b64)
