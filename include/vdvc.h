@@ -16,6 +16,11 @@
  *                  ravel(order='F') of an [x][y][z] array; padding voxels are
  *                  zero and never read as data;
  *   gradients:     three planes (gx, gy, gz), each a voxel volume;
+ *   alignment:     voxel volumes should start on a 16-byte boundary (any
+ *                  allocation does); the Hessian, diagonal, residual / RHS,
+ *                  Jacobi and PCG entry points accept any element-aligned
+ *                  base, but one that is not 16-byte aligned (an offset
+ *                  view) runs the slower one-voxel-per-lane kernels;
  *   nodal vectors: DOF 3*(i + j*(Nx+1) + k*(Nx+1)*(Ny+1)) + c;
  *   deformed image g for sampling: x fastest, dims (gnx, gny, gnz).
  * Element types: VDVC_F32 float32, VDVC_BF16 bfloat16, VDVC_F16 float16,

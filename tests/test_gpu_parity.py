@@ -36,8 +36,9 @@ def test_matvec_H(gpu, shape):
 @pytest.mark.parametrize('gt', ['f32', 'bf16'])
 @pytest.mark.parametrize('shape', [(9, 2, 3, 8), (8, 8, 8, 4), (3, 2, 2, 32), (65, 1, 1, 8)])
 def test_matvec_H_tilings(gpu, monkeypatch, shape, gt, nys):
-    """Both row splits of the vectorised kernel (VDVC_NYS tuning override),
-    float32 and bfloat16 gradient planes, with and without a separate mask."""
+    """The two row splits of the vectorised kernel (VDVC_NYS tuning
+    override), float32 and bfloat16 gradient planes, with and without a
+    separate mask."""
     import torch
     from voxeldvc_amd import _lib as L
     Nx, Ny, Nz, h = shape
@@ -58,6 +59,44 @@ def test_matvec_H_tilings(gpu, monkeypatch, shape, gt, nys):
     L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in planes], code, L.ptr(mk),
            L.ptr(dv), L.ptr(out))
     assert rel(L.to_host(out), ref) < 2e-5
+
+
+@pytest.mark.parametrize('gt', ['f32', 'bf16'])
+def test_matvec_H_offset_views(gpu, gt):
+    """Gradient and mask planes handed over as views whose base address is
+    not 16-byte aligned take the one-voxel-per-lane decomposition and give
+    the same operator and diagonal."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = 9, 2, 3, 8
+    print('matvec_H offset views', gt, flush=True)
+    g, m, v = rand_problem(12, Nx, Ny, Nz, h)
+    shp = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    tdt = torch.float32 if gt == 'f32' else torch.bfloat16
+    dts = 'float32' if gt == 'f32' else 'bfloat16'
+    gq = torch.from_numpy(g).to(tdt)
+    g64 = gq.to(torch.float64).numpy()
+    ref = O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz)
+    dref = O.build_diagonal_H(g64, m, h, Nx, Ny, Nz)
+    planes = [L.to_dev_level(gq[:, c].contiguous(), shp, dts) for c in range(3)]
+    mk = L.to_dev_level(m, shp, 'uint8')
+    views = []
+    for q in planes + [mk]:
+        buf = torch.zeros(q.numel() + 1, dtype=q.dtype, device=q.device)
+        buf[1:].copy_(q)
+        views.append(buf[1:])
+    assert all(t.data_ptr() % 16 for t in views)
+    dv = L.to_dev_flat(v)
+    code = L.F32 if gt == 'f32' else L.BF16
+    for src in (planes + [mk], views):
+        out = L.empty(dv.numel())
+        L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in src[:3]], code,
+               L.ptr(src[3]), L.ptr(dv), L.ptr(out))
+        assert rel(L.to_host(out), ref) < 2e-5
+        d = L.empty(dv.numel())
+        L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in src[:3]], code,
+               L.ptr(src[3]), L.ptr(d))
+        assert rel(L.to_host(d), dref) < 2e-5
 
 
 @pytest.mark.parametrize('shape', SHAPES)
@@ -328,3 +367,29 @@ def test_eqgap_golden_33(gpu):
         print('gpu', a, flush=True)
         print('ora', b, flush=True)
     assert np.abs(U - zr['U']).max() < 1e-3
+
+
+def test_diag_and_pcg_with_row_split(gpu, monkeypatch):
+    """The diagonal and the PCG operator read the node partials through the
+    same gather: with the row split they give the oracle's diag(H) and
+    H v + ll L v."""
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    monkeypatch.setenv('VDVC_NYS', '2')
+    Nx, Ny, Nz, h = 7, 3, 4, 8
+    g, m, v = rand_problem(13, Nx, Ny, Nz, h)
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    gx, gy, gz = Kd.grad_planes(g, shp)
+    md = Kd._mask_u8(m, shp)
+    g64 = g.astype(np.float64)
+    d = L.empty(v.size)
+    L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, gx, gy, gz, L.F32, md, d)
+    assert rel(L.to_host(d), O.build_diagonal_H(g64, m, h, Nx, Ny, Nz)) < 2e-5
+    Kp = np.ascontiguousarray(O.build_K_ref_laplacian())
+    out = L.empty(v.size)
+    ll = 0.4
+    L.call('vdvc_apply_A', L.ctx(), Nx, Ny, Nz, h, gx, gy, gz, L.F32, md, L.REG_K,
+           Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, L.to_dev_flat(v), out)
+    ref = (O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz)
+           + ll * O.matvec_K_ref(v.astype(np.float64), Kp, h, Nx, Ny, Nz))
+    assert rel(L.to_host(out), ref) < 2e-5

@@ -221,6 +221,23 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   return T;
 }
 
+// the vectorised voxel kernels load (and, for the residual and the folded
+// planes, store) 16-byte words at row starts, which sit on 16-element pitch
+// boundaries; a streamed array whose base is not 16-byte aligned (an offset
+// view) takes the generic one-voxel-per-lane decomposition instead
+bool streams_aligned(const VoxIn& in) {
+  auto ok = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+  return ok(in.g[0]) && ok(in.g[1]) && ok(in.g[2]) && ok(in.mask) && ok(in.fimg) &&
+         ok(in.gvals) && ok(in.live) && ok(in.res) && ok(in.fold_out[0]) &&
+         ok(in.fold_out[1]) && ok(in.fold_out[2]);
+}
+
+TilesDev tiles_for(const MeshDev& M, int gtype, const VoxIn& in) {
+  TilesDev T = make_tiles(M, gtype);
+  if (T.vec && !streams_aligned(in)) T = make_tiles_generic(M);
+  return T;
+}
+
 int gtype_ok(int t) { return t == VDVC_F32 || t == VDVC_BF16 || t == VDVC_F64; }
 int ftype_ok(int t) { return t == VDVC_F32 || t == VDVC_F16 || t == VDVC_F64; }
 int rtype_ok(int t) { return t == VDVC_F32 || t == VDVC_F64; }
@@ -801,14 +818,14 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
   if (fold && gtype == VDVC_F64) return fail(-1, "folded bfloat16 planes need float32 input");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
-  if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
-  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   VoxIn in = vox_in(d_gx, d_gy, d_gz, d_bmask);
   in.fimg = d_f; in.gvals = d_gvals; in.live = d_live;
   in.mean0 = mean0; in.gmean = gmean; in.kscale = kscale;
   in.res = d_res;
   in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
+  TilesDev T = tiles_for(M, gtype, in);
+  if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   // the display statistics only when the caller asks for them
   in.stats = h_stats4 ? ctx->stats.as<double>() : nullptr;
   return VDVC_BY_RTYPE(gtype,
@@ -827,8 +844,8 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(gtype,
                        matvec_H_impl<float>(ctx, M, T, gtype, in, static_cast<const float*>(d_v),
                                             static_cast<float*>(d_out)),
@@ -846,8 +863,8 @@ int vdvc_matvec_H_noalloc(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const vo
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(gtype,
                        matvec_H_noalloc_impl<float>(ctx, M, T, gtype, in,
                                                     static_cast<const float*>(d_v),
@@ -865,8 +882,8 @@ int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(gtype, diag_H_impl<float>(ctx, M, T, gtype, in, static_cast<float*>(d_out)),
                        diag_H_impl<double>(ctx, M, T, gtype, in, static_cast<double*>(d_out)));
 }
@@ -945,8 +962,8 @@ int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_diag || !d_minv) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(
       gtype,
       jacobi_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
@@ -967,8 +984,8 @@ int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
   if (!gtype_ok(gtype)) return fail(-1, "unsupported gradient type");
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(
       gtype,
       apply_A_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
@@ -990,8 +1007,8 @@ int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, con
   if (max_iter < 0) return fail(-1, "max_iter < 0");
   if (check_every < 1) check_every = 1;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
-  TilesDev T = make_tiles(M, gtype);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
+  const TilesDev T = tiles_for(M, gtype, in);
   return VDVC_BY_RTYPE(
       gtype,
       pcg_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,

@@ -13,7 +13,10 @@ the parameters are float32 masters updated by Adam.
 Training data: synthetic speckle pairs warped by random Fourier-mode fields
 plus a random affine, labelled by the plain multiscale GN solve itself
 (`make_training_set`).  `python -m voxeldvc_amd.seednet train` trains on the
-GPU and writes seednet_weights.npz next to this file.
+GPU and writes seednet_weights.npz next to this file (its defaults, 4096
+pairs and 8000 Adam steps, made the shipped weights);
+`python -m voxeldvc_amd.seednet curve` trains one network per training-set
+size and reports the validation RMS of each.
 """
 import os
 import time
@@ -264,18 +267,65 @@ def evaluate(net, X, Y, dims1):
     return np.array(out)
 
 
+def learning_curve(sizes, val=64, steps=5000, n=129, batch=16, lr=2e-3, out_dir=None,
+                   log=print):
+    """Validation RMS against the number of training pairs: one labelled
+    set of max(sizes) pairs (the smaller sets are its prefixes) and one
+    validation set of other seeds; a fresh network trained for `steps` Adam
+    steps on each size.  Returns a list of dicts; with out_dir, writes
+    learning_curve.json and the weights of each size there."""
+    import json
+    h = 8
+    t0 = time.time()
+    X, Y = make_training_set(max(sizes), n=n, h=h, seed=1, log=log)
+    Xv, Yv = make_training_set(val, n=n, h=h, seed=2, log=log)
+    log('data %.1fs' % (time.time() - t0))
+    m = out_extent(n, h // 2)
+    dims1 = (1, m, m, m)
+    zero = float(np.sqrt(float((Yv.double() ** 2).mean())))
+    rows = []
+    for k in sizes:
+        net = SeedNet(seed=0)
+        t1 = time.time()
+        hist = train(net, X[:k], Y[:k], dims1, steps, batch=batch, lr=lr, log=log,
+                     every=max(steps // 4, 1))
+        rv = evaluate(net, Xv, Yv, dims1)
+        rt = evaluate(net, X[:min(k, val)], Y[:min(k, val)], dims1)
+        row = dict(pairs=k, steps=steps, val_rms=float(rv.mean()), train_rms=float(rt.mean()),
+                   zero_rms=zero, final_loss=float(np.mean(hist[-100:])),
+                   train_s=round(time.time() - t1, 1))
+        rows.append(row)
+        log('pairs %6d: validation RMS %.4f vox, training-set RMS %.4f vox (zero %.3f)'
+            % (k, row['val_rms'], row['train_rms'], zero))
+        if out_dir:
+            net.save(os.path.join(out_dir, 'seednet_weights_%d.npz' % k), val_rms=row['val_rms'],
+                     zero_rms=zero, pairs=k, steps=steps, loss=np.array(hist, dtype=np.float32))
+    if out_dir:
+        with open(os.path.join(out_dir, 'learning_curve.json'), 'w') as fh:
+            json.dump(dict(n=n, h=h, val_pairs=val, batch=batch, lr=lr, rows=rows), fh, indent=1)
+    return rows
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog='python -m voxeldvc_amd.seednet')
-    ap.add_argument('cmd', choices=['train'])
-    ap.add_argument('--pairs', type=int, default=384)
-    ap.add_argument('--val', type=int, default=32)
-    ap.add_argument('--steps', type=int, default=3000)
+    ap.add_argument('cmd', choices=['train', 'curve'])
+    ap.add_argument('--sizes', default='256,1024,4096,8192',
+                    help='curve: training-set sizes (prefixes of the largest)')
+    ap.add_argument('--out-dir', default=None, help='curve: where the summary and weights go')
+    ap.add_argument('--pairs', type=int, default=4096)
+    ap.add_argument('--val', type=int, default=64)
+    ap.add_argument('--steps', type=int, default=8000)
     ap.add_argument('--batch', type=int, default=16)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--n', type=int, default=129)
     ap.add_argument('--out', default=WEIGHTS)
     a = ap.parse_args(argv)
+    if a.cmd == 'curve':
+        sizes = [int(x) for x in a.sizes.split(',')]
+        learning_curve(sizes, val=a.val, steps=a.steps, n=a.n, batch=a.batch, lr=a.lr,
+                       out_dir=a.out_dir, log=lambda m: print(m, flush=True))
+        return
     h = 8
     t0 = time.time()
     X, Y = make_training_set(a.pairs, n=a.n, h=h, seed=1)
