@@ -725,6 +725,40 @@ def affine_cpu_baseline(n=97):
                       % (n, t, os.cpu_count())}
 
 
+def stream_ceiling(reps=10):
+    """The measured HBM streaming ceiling on this GPU: the library's read
+    probe (vdvc_stream_read: 16-byte nontemporal loads, four in flight per
+    lane) over 2 GiB at four grid sizes, and its byte copy (vdvc_copy) of
+    2 GiB; kernel time from the library's HIP events, best of the grids."""
+    from voxeldvc_amd import _lib as L
+    c = L.ctx()
+    nb = 1 << 31
+    src = L.empty(nb // 4, 'float32')
+    L.fill(src, 1.5)
+    dst = L.empty(nb // 4, 'float32')
+    out = L.empty(8192, 'int32')
+    key = 0x12345677      # never the xor fold of equal 0x3fc00000 words
+
+    def timed(call):
+        call()
+        L.call('vdvc_timing_begin', c)
+        for _ in range(reps):
+            call()
+        tot, nl = (L._d * 1)(), (L._i * 1)()
+        L.call('vdvc_timing_end', c, tot, nl)
+        return tot[0] / max(1, nl[0])
+
+    read = {}
+    for grid in (1024, 2048, 4096, 8192):
+        ms = timed(lambda: L.call('vdvc_stream_read', c, nb, src, grid, key, out))
+        read[grid] = round(nb / (ms * 1e-3) / 1e9, 1)
+    ms = timed(lambda: L.call('vdvc_copy', c, nb, src, dst))
+    return {"read_GBs": max(read.values()), "read_GBs_by_grid": read,
+            "copy_GBs": round(2 * nb / (ms * 1e-3) / 1e9, 1),
+            "note": "2 GiB read (nontemporal 16-byte loads) / 2 GiB copied (read + write "
+                    "counted), library kernels, HIP-event kernel time"}
+
+
 def measured_traffic(Nv, h):
     """HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement of the same workload (None if not measured)."""
@@ -871,6 +905,8 @@ def main():
         torch.cuda.synchronize()
         del xs, ys
 
+    print('streaming ceiling', file=sys.stderr, flush=True)
+    ceiling = stream_ceiling()
     B_bf16 = Nvox * 6 + 8 * Ndof
     B_f32 = Nvox * 13 + 8 * Ndof
     res = {
@@ -893,7 +929,8 @@ def main():
                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": measured_traffic(Nv, h),
                            "kernel": "vox_vec_kernel<MATVEC,bf16,V=8>",
-                           "kernel_ms": round(kms_b, 5), "algorithmic_bytes": B_bf16}
+                           "kernel_ms": round(kms_b, 5), "algorithmic_bytes": B_bf16,
+                           "frac_of_measured_read_ceiling": round(ach / ceiling["read_GBs"], 4)}
     del Gw
     if a.only in (None, 'f32'):
         G32 = [L.empty(Np) for _ in range(3)]
@@ -907,6 +944,7 @@ def main():
                                  "algorithmic_bytes": B_f32,
                                  "achieved_GBs": round(B_f32 / (kms_f * 1e-3) / 1e9, 1),
                                  "frac": round(B_f32 / (kms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    res["stream_ceiling"] = ceiling
     del f16, mask0
     if a.only in (None, 'f32'):
         del G32

@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 6
+#define VDVC_ABI_VERSION 7
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -79,11 +79,20 @@ int vdvc_synchronize(vdvc_ctx* ctx);
 
 /* Measurement hook (no reference counterpart): between begin and end, every
  * launch of the voxel-streaming kernel (Hessian matvec / diagonal / RHS,
- * including those inside vdvc_pcg) is bracketed by HIP events on the context
- * stream; end synchronises and returns the summed kernel time and the number
- * of launches. */
+ * including those inside vdvc_pcg), of vdvc_copy and of vdvc_stream_read is
+ * bracketed by HIP events on the context stream; end synchronises and returns
+ * the summed kernel time and the number of launches. */
 int vdvc_timing_begin(vdvc_ctx* ctx);
 int vdvc_timing_end(vdvc_ctx* ctx, double* h_total_ms, int* h_launches);
+
+/* Streaming-read probe (no reference counterpart: it measures the HBM read
+ * ceiling the voxel kernels are compared against).  `grid` blocks of 256
+ * lanes read nbytes (a positive multiple of 16, d_src 16-byte aligned) with
+ * 16-byte nontemporal loads, four in flight per lane, and fold them with
+ * xor; block b writes its fold to d_out[b] only if it equals `key` (d_out
+ * holds `grid` words), so for a key that is not a fold nothing is written. */
+int vdvc_stream_read(vdvc_ctx* ctx, int64_t nbytes, const void* d_src, int grid, uint32_t key,
+                     uint32_t* d_out);
 
 /* Reference-image setup: np.gradient per axis (edge_order=1), mask0 = f > glt,
  * gradient zeroed where the 6-cross erosion of mask0 is false.

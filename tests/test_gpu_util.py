@@ -140,3 +140,34 @@ def test_axpby_widen(gpu):
     L.call('vdvc_axpby_x', L.ctx(), n, 0.5, _dev(x), 0, -1.0, y32, 0)
     want = (0.5 * x.astype(np.float64) - y.astype(np.float32).astype(np.float64)).astype(np.float32)
     assert np.array_equal(y32.cpu().numpy(), want)
+
+
+def test_stream_read_probe(gpu):
+    """The streaming-read probe stores nothing unless a lane's xor fold of
+    its 16-byte words equals the key; then it stores that fold in its block's
+    word (lane q of a grid of G blocks reads rows q, q + 256 G, ...)."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    grid = 4
+    stride = grid * 256
+    n16 = 5 * stride + 37                  # the unrolled loop and its tail
+    words = np.random.default_rng(9).integers(0, 2 ** 32, size=(n16, 4), dtype=np.uint32)
+    src = _dev(words.view(np.int32))
+    fold = np.zeros(stride, np.uint32)
+    for q in range(stride):
+        r = words[q::stride]
+        fold[q] = np.bitwise_xor.reduce(r.ravel())
+    tid = 2 * 256 + 77                     # a lane of block 2
+    assert np.sum(fold == fold[tid]) == 1
+    out = torch.full((grid,), -1, dtype=torch.int32, device='cuda')
+    L.call('vdvc_stream_read', L.ctx(), n16 * 16, src, grid, int(fold[tid]), out)
+    got = out.cpu().numpy().view(np.uint32)
+    assert got[2] == fold[tid]
+    assert np.all(got[[0, 1, 3]] == 0xFFFFFFFF)
+    # a key no lane folds to: nothing is written
+    miss = int(next(k for k in range(1, 1000) if k not in set(fold.tolist())))
+    out.fill_(-1)
+    L.call('vdvc_stream_read', L.ctx(), n16 * 16, src, grid, miss, out)
+    assert np.all(out.cpu().numpy() == -1)
+    with pytest.raises(ValueError):
+        L.call('vdvc_stream_read', L.ctx(), 24, src, grid, 0, out)

@@ -1801,8 +1801,23 @@ int vdvc_convert(vdvc_ctx* ctx, int64_t n, const void* d_src, int stype, void* d
 int vdvc_copy(vdvc_ctx* ctx, int64_t nbytes, const void* d_src, void* d_dst) {
   if (!ctx || (nbytes > 0 && (!d_src || !d_dst))) return fail(-1, "NULL argument");
   if (nbytes < 0) return fail(-1, "nbytes < 0");
+  if (nbytes == 0) return 0;
+  int rc = timed_event(ctx);
+  if (rc) return rc;
   VDVC_LAUNCH(vdvc::launch_copy(nbytes, d_src, d_dst, ctx->stream));
-  return 0;
+  return timed_event(ctx);
+}
+
+int vdvc_stream_read(vdvc_ctx* ctx, int64_t nbytes, const void* d_src, int grid, uint32_t key,
+                     uint32_t* d_out) {
+  if (!ctx || !d_src || !d_out) return fail(-1, "NULL argument");
+  if (nbytes < 16 || (nbytes & 15) || (reinterpret_cast<uintptr_t>(d_src) & 15))
+    return fail(-1, "the probe reads a positive multiple of 16 bytes from a 16-byte aligned buffer");
+  if (grid < 1) return fail(-1, "grid < 1");
+  int rc = timed_event(ctx);
+  if (rc) return rc;
+  VDVC_LAUNCH(vdvc::launch_stream_read(nbytes, d_src, grid, key, d_out, ctx->stream));
+  return timed_event(ctx);
 }
 
 int vdvc_fill(vdvc_ctx* ctx, int64_t n, int dtype, double value, void* d_dst) {

@@ -27,100 +27,10 @@
 
 namespace {
 
-// ---- aligned vector loads of V consecutive elements -------------------------
+// ---- aligned vector stores / loads of V consecutive elements --------------
+// (the index is a 32-bit element offset from a wave-uniform row pointer)
 template <int V>
-__device__ __forceinline__ void ld_bf16(const __hip_bfloat16* base, long long i, float (&o)[V]) {
-  const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + i;
-  if constexpr (V == 8) {
-    const uint4 q = *reinterpret_cast<const uint4*>(p);
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      o[2 * j] = __uint_as_float(w[j] << 16);
-      o[2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
-    }
-  } else if constexpr (V == 4) {
-    const uint2 q = *reinterpret_cast<const uint2*>(p);
-    o[0] = __uint_as_float(q.x << 16); o[1] = __uint_as_float(q.x & 0xffff0000u);
-    o[2] = __uint_as_float(q.y << 16); o[3] = __uint_as_float(q.y & 0xffff0000u);
-  } else if constexpr (V == 2) {
-    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
-    o[0] = __uint_as_float(q << 16); o[1] = __uint_as_float(q & 0xffff0000u);
-  } else {
-    o[0] = __uint_as_float(((uint32_t)p[0]) << 16);
-  }
-}
-
-template <int V>
-__device__ __forceinline__ void ld_f32(const float* base, long long i, float (&o)[V]) {
-  const float* p = base + i;
-  if constexpr (V == 8) {
-    const float4 a = *reinterpret_cast<const float4*>(p);
-    const float4 b = *reinterpret_cast<const float4*>(p + 4);
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
-  } else if constexpr (V == 4) {
-    const float4 a = *reinterpret_cast<const float4*>(p);
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-  } else if constexpr (V == 2) {
-    const float2 a = *reinterpret_cast<const float2*>(p);
-    o[0] = a.x; o[1] = a.y;
-  } else {
-    o[0] = p[0];
-  }
-}
-
-template <int V>
-__device__ __forceinline__ void ld_f16(const __half* base, long long i, float (&o)[V]) {
-  const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + i;
-  uint16_t r[V];
-  if constexpr (V == 8) {
-    const uint4 q = *reinterpret_cast<const uint4*>(p);
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) { r[2 * j] = (uint16_t)(w[j] & 0xffffu); r[2 * j + 1] = (uint16_t)(w[j] >> 16); }
-  } else if constexpr (V == 4) {
-    const uint2 q = *reinterpret_cast<const uint2*>(p);
-    r[0] = (uint16_t)(q.x & 0xffffu); r[1] = (uint16_t)(q.x >> 16);
-    r[2] = (uint16_t)(q.y & 0xffffu); r[3] = (uint16_t)(q.y >> 16);
-  } else if constexpr (V == 2) {
-    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
-    r[0] = (uint16_t)(q & 0xffffu); r[1] = (uint16_t)(q >> 16);
-  } else {
-    r[0] = p[0];
-  }
-#pragma unroll
-  for (int j = 0; j < V; ++j) o[j] = __half2float(__ushort_as_half(r[j]));
-}
-
-template <int V>
-__device__ __forceinline__ void ld_u8(const uint8_t* base, long long i, uint32_t (&o)[V]) {
-  const uint8_t* p = base + i;
-  if constexpr (V == 8) {
-    const uint2 q = *reinterpret_cast<const uint2*>(p);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) { o[j] = (q.x >> (8 * j)) & 0xffu; o[4 + j] = (q.y >> (8 * j)) & 0xffu; }
-  } else if constexpr (V == 4) {
-    const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) o[j] = (q >> (8 * j)) & 0xffu;
-  } else if constexpr (V == 2) {
-    const uint16_t q = *reinterpret_cast<const uint16_t*>(p);
-    o[0] = q & 0xffu; o[1] = (q >> 8) & 0xffu;
-  } else {
-    o[0] = p[0];
-  }
-}
-
-template <typename T, int V>
-__device__ __forceinline__ void ld_any(const T* base, long long i, float (&o)[V]) {
-  if constexpr (sizeof(T) == 4) ld_f32<V>(reinterpret_cast<const float*>(base), i, o);
-  else if constexpr (std::is_same<T, __half>::value) ld_f16<V>(base, i, o);
-  else ld_bf16<V>(base, i, o);
-}
-
-template <int V>
-__device__ __forceinline__ void st_f32(float* base, long long i, const float (&v)[V]) {
+__device__ __forceinline__ void st_f32(float* base, unsigned i, const float (&v)[V]) {
   float* p = base + i;
   if constexpr (V == 8) {
     *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
@@ -139,7 +49,7 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
 }
 
 template <int V>
-__device__ __forceinline__ void st_bf16(uint16_t* base, long long i, const float (&v)[V]) {
+__device__ __forceinline__ void st_bf16(uint16_t* base, unsigned i, const float (&v)[V]) {
   uint16_t* p = base + i;
   if constexpr (V == 8) {
     uint4 q;
@@ -178,7 +88,7 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // streaming read reaches about 6.0 TB/s this way against 5.5 TB/s with the
 // default policy.
 template <typename T, int V>
-__device__ __forceinline__ void ld_raw(const T* base, long long i, Raw<T, V>& r) {
+__device__ __forceinline__ void ld_raw(const T* base, unsigned i, Raw<T, V>& r) {
   const T* p = base + i;
   constexpr int NB = Raw<T, V>::NB;
   if constexpr (NB == 32) {
@@ -301,6 +211,12 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
   const int xb = x0 + lane * V;
   const bool act = xb < xend;
   const int xbc = act ? xb : x0;          // in-range address for inactive lanes
+  // the lane's offset within a row: a 32-bit offset from a wave-uniform row
+  // pointer, so every row load / store is a scalar base plus this register
+  const unsigned xbu = (unsigned)xbc;
+  // inactive lanes stream in-tile data like the others; their node weights
+  // are zero
+  const float actw = act ? 1.0f : 0.0f;
   const int ex = xbc / h;                 // element of this lane's block
   const int tb = xbc - ex * h;            // local coordinate of the block's first voxel
   const bool tail = act && (xb + V == xend);
@@ -419,15 +335,15 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
   int cp = 0, ct = 0;  // prefetch cursor (plane, row), clamped to the tile's last row
   auto issue = [&](RowBuf& b) {
     const long long rb = (long long)(ez0 * h + cp) * pl + (long long)(y0 + ct) * M.pitch;
-    ld_raw<GT, V>(g0, rb + xbc, b.g[0]);
-    ld_raw<GT, V>(g1, rb + xbc, b.g[1]);
-    ld_raw<GT, V>(g2, rb + xbc, b.g[2]);
-    if (HASMASK) ld_raw<uint8_t, V>(in.mask, rb + xbc, b.m);
+    ld_raw<GT, V>(g0 + rb, xbu, b.g[0]);
+    ld_raw<GT, V>(g1 + rb, xbu, b.g[1]);
+    ld_raw<GT, V>(g2 + rb, xbu, b.g[2]);
+    if (HASMASK) ld_raw<uint8_t, V>(in.mask + rb, xbu, b.m);
     if (MODE == MODE_RHS) {
-      ld_raw<FT, V>(fim, rb + xbc, b.f);
-      ld_raw<float, V>(gvals_f, rb + xbc, b.gv);
+      ld_raw<FT, V>(fim + rb, xbu, b.f);
+      ld_raw<float, V>(gvals_f + rb, xbu, b.gv);
       // the live mask feeds only the display statistics
-      if (stats) ld_raw<uint8_t, V>(in.live, rb + xbc, b.lv);
+      if (stats) ld_raw<uint8_t, V>(in.live + rb, xbu, b.lv);
     }
     if (MODE != MODE_RHS) {
       b.t[0] = ld_raw1<GT>(g0, rb + xend);
@@ -473,11 +389,11 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     const int z = ez * h + tz;
     Z1 = tz * inv_h;
     Z0 = 1.0f - Z1;
-    Wz = axis_weight(z, M.Nvz - 1);
+    Wz = axis_weight(z, M.Nvz - 1) * actw;
     mz = axis_mult(z, h, M.Nvz - 1);
     zw0 = Z0; zw1 = Z1;
     if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
-    rowbase = (long long)z * pl + (long long)y0 * M.pitch + xbc;
+    rowbase = (long long)z * pl + (long long)y0 * M.pitch;
     tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
     if (MODE == MODE_MATVEC) {
       // u at the element's left / right node columns, linear in y
@@ -584,7 +500,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
         }
       }
       if (act) {
-        st_f32<V>(res_f, vi, resv);
+        st_f32<V>(res_f + vi, xbu, resv);
         if (FOLD) {
           float fx[V], fy[V], fz[V];
 #pragma unroll
@@ -593,9 +509,9 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
             fy[j] = mk[j] ? gy[j] : 0.0f;
             fz[j] = mk[j] ? gz[j] : 0.0f;
           }
-          st_bf16<V>(in.fold_out[0], vi, fx);
-          st_bf16<V>(in.fold_out[1], vi, fy);
-          st_bf16<V>(in.fold_out[2], vi, fz);
+          st_bf16<V>(in.fold_out[0] + vi, xbu, fx);
+          st_bf16<V>(in.fold_out[1] + vi, xbu, fy);
+          st_bf16<V>(in.fold_out[2] + vi, xbu, fz);
         }
       }
     }
@@ -654,11 +570,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
       }
       acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S, Tt);
     }
-    if (!act) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) { S[k] = 0.0f; Tt[k] = 0.0f; }
-    }
-    // x-restricted row values R[cx][k], weighted into the node planes
+    // x-restricted row values R[cx][k] (the lanes past the row end have
+    // zero node weights), weighted into the node planes
     float R[2][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {

@@ -221,6 +221,9 @@ namespace vdvc {
 int launch_resample_nodes(const NodeResample& R, hipStream_t s);
 int launch_convert(long long n, const void* src, int st, void* dst, int dt, hipStream_t s);
 int launch_copy(long long nbytes, const void* src, void* dst, hipStream_t s);
+// read-only streaming probe over nbytes (a multiple of 16, 16-byte aligned)
+int launch_stream_read(long long nbytes, const void* src, int grid, uint32_t key, uint32_t* out,
+                       hipStream_t s);
 int launch_fill(long long n, void* dst, int dt, double v, hipStream_t s);
 int launch_copy_volume(const VolCopy& C, hipStream_t s);
 int launch_axpby_x(long long n, double a, const void* x, int xt, double b, void* y, int yt,

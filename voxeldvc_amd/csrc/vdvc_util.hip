@@ -94,6 +94,32 @@ __global__ __launch_bounds__(VDVC_BLOCK) void copy_kernel(long long nbytes,
   for (long long q = done + tid; q < nbytes; q += stride) dst[q] = src[q];
 }
 
+// Read-only streaming probe (a measurement of the HBM read ceiling the
+// voxel kernels are compared against): grid-stride 16-byte nontemporal loads,
+// four in flight per lane, folded with xor; a block stores its fold only if
+// it equals the caller's key, so the loads cannot be dropped and, for any
+// key that is not the fold, nothing is written
+__global__ __launch_bounds__(VDVC_BLOCK) void stream_read_kernel(long long n16,
+                                                                const uint4* __restrict__ src,
+                                                                uint32_t key,
+                                                                uint32_t* __restrict__ out) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4* s = reinterpret_cast<const u4*>(src);
+  const long long stride = (long long)gridDim.x * VDVC_BLOCK;
+  long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  u4 acc = {0u, 0u, 0u, 0u};
+  for (; q + 3 * stride < n16; q += 4 * stride) {
+    const u4 a = __builtin_nontemporal_load(s + q);
+    const u4 b = __builtin_nontemporal_load(s + q + stride);
+    const u4 c = __builtin_nontemporal_load(s + q + 2 * stride);
+    const u4 d = __builtin_nontemporal_load(s + q + 3 * stride);
+    acc ^= a ^ b ^ c ^ d;
+  }
+  for (; q < n16; q += stride) acc ^= __builtin_nontemporal_load(s + q);
+  const uint32_t f = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (f == key) out[blockIdx.x] = f;
+}
+
 __global__ __launch_bounds__(VDVC_BLOCK) void fill_kernel(long long n, void* dst, int dt,
                                                           double v) {
   for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
@@ -271,6 +297,13 @@ int launch_convert(long long n, const void* src, int st, void* dst, int dt, hipS
   if (n <= 0) return 0;
   hipLaunchKernelGGL(convert_kernel, dim3(grid_for(n)), dim3(VDVC_BLOCK), 0, s, n, src, st, dst,
                      dt);
+  return (int)hipGetLastError();
+}
+
+int launch_stream_read(long long nbytes, const void* src, int grid, uint32_t key, uint32_t* out,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(stream_read_kernel, dim3(grid), dim3(VDVC_BLOCK), 0, s, nbytes >> 4,
+                     reinterpret_cast<const uint4*>(src), key, out);
   return (int)hipGetLastError();
 }
 
