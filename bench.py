@@ -332,19 +332,22 @@ def config_cases():
         return U, (time.perf_counter() - t0) * 1e3, sum(len(lv['iters']) for lv in lg)
 
     def lap_eqgap(fv, gv, N, h, g_origin=(0, 0, 0), fft_prealign=True, scales=(3, 2, 1, 0),
-                  **extra):
+                  cb=None, **extra):
         """Laplacian pyramid, then the equilibrium gap on the finest level (the
-        equilibrium gap on the coarse levels stalls the reference's PCG)."""
+        equilibrium gap on the coarse levels stalls the reference's PCG);
+        cb(scale, ik, U) after every GN update of both solves."""
         lap = dict(K_ref_laplacian=Kl, l0=4.5 * h, glt=0.0, dynamic_mask=True,
                    freeze_mask_after=4, disp=False, g_origin=g_origin)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         l1, l2 = [], []
         U0, _, _ = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, log=l1,
-                                              scales=scales, fft_prealign=fft_prealign, **lap,
-                                              **extra)
+                                              scales=scales, fft_prealign=fft_prealign,
+                                              callback=cb, **lap, **extra)
         U, _, _ = C.correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, U0=U0, log=l2,
-                                  K_ref_elastic=Kel, reg_type='equilibrium_gap', **lap, **extra)
+                                  K_ref_elastic=Kel, reg_type='equilibrium_gap',
+                                  callback=None if cb is None else (lambda ik, Ud: cb(0, ik, Ud)),
+                                  **lap, **extra)
         torch.cuda.synchronize()
         return U, (time.perf_counter() - t0) * 1e3, sum(len(lv['iters']) for lv in l1) + len(l2)
 
@@ -462,20 +465,42 @@ def config_cases():
         return float(((Uf.double() - ref_nodes) ** 2).mean().sqrt())
 
     net = seednet.SeedNet.load()
-    trace = []
-    U, t, gn = solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net,
-                     callback=lambda s, ik, Ud: trace.append(sub_rms(Ud)), **kw)
-    k = next((i + 1 for i, e in enumerate(trace) if e <= 0.01), None)
-    r['learned_bf16_per_iter_rms_subvolume'] = [float('%.3g' % e) for e in trace]
-    if k is not None:
-        solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net, maxiter=k, **kw)
-        U, t, gn = solve(fv, gv, N, h, storage='bf16', scales=(0,), learned_seed=net, maxiter=k,
-                         **kw)
-        r['learned_bf16_to_0.01'] = {"ms": round(t, 2), "gn_iters": gn,
-                                     "rms_subvolume": sub_rms(U)}
-    Up, tp, gnp = lap_eqgap(fv, gv, N, h)
-    r['plain_fp32_4levels'] = {"ms": round(tp, 2), "gn_iters": gnp, "rms_subvolume": sub_rms(Up),
+
+    def learned_run(cb):
+        return C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, storage='bf16',
+                                          scales=(0,), learned_seed=net, callback=cb, **kw)[0]
+
+    def plain_run(cb):
+        return lap_eqgap(fv, gv, N, h, cb=cb)[0]
+
+    def first_within(tr, tol=0.01):
+        return next((i + 1 for i, (sc, e) in enumerate(tr) if sc == 0 and e <= tol), None)
+
+    # traced runs find the first finest-level iterate within 0.01 vox of the
+    # subvolume answer; the timed runs record a HIP event at every GN update
+    tr_l, tr_p = [], []
+    lg_l = []
+    U = C.multiscale_correlate_gpu(fv, gv, N[0], N[1], N[2], h, torch, storage='bf16',
+                                   scales=(0,), learned_seed=net, log=lg_l,
+                                   callback=lambda sc, ik, Ud: tr_l.append((sc, sub_rms(Ud))),
+                                   **kw)[0]
+    Up, _, gnp = lap_eqgap(fv, gv, N, h, cb=lambda sc, ik, Ud: tr_p.append((sc, sub_rms(Ud))))
+    k_l, k_p = first_within(tr_l), first_within(tr_p)
+    r['learned_bf16_per_iter_rms_subvolume'] = [float('%.3g' % e) for _, e in tr_l]
+    r['plain_per_iter_rms_subvolume_finest'] = [float('%.3g' % e) for sc, e in tr_p if sc == 0]
+    tl01, tl, _ = _timed_trace(learned_run, k_l, 1)
+    tp01, tp, _ = _timed_trace(plain_run, k_p, 1)
+    r['learned_bf16'] = {"ms_to_0.01": None if tl01 is None else round(tl01, 2), "gn_to_0.01": k_l,
+                         "ms_to_convergence": round(tl, 2),
+                         "gn_iters": sum(len(lv['iters']) for lv in lg_l),
+                         "rms_subvolume": sub_rms(U)}
+    r['plain_fp32_4levels'] = {"ms_to_0.01": None if tp01 is None else round(tp01, 2),
+                               "gn_to_0.01": k_p, "ms_to_convergence": round(tp, 2),
+                               "gn_iters": gnp, "rms_subvolume": sub_rms(Up),
                                "levels": "Laplacian 3,2,1,0 then equilibrium gap at scale 0"}
+    if tl01 is not None and tp01 is not None:
+        r['ratio_to_0.01'] = round(tl01 / tp01, 4)
+    r['ratio_converged'] = round(tl / tp, 4)
     torch.cuda.synchronize()
     r['peak_device_bytes'] = int(free0 - torch.cuda.mem_get_info()[0])
     out['C5_1025x1025x769_h16'] = r

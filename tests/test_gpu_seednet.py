@@ -20,7 +20,8 @@ def dev(a):
     return L.to_dev_flat(np.ascontiguousarray(a, dtype=np.float32))
 
 
-@pytest.mark.parametrize('layer', [(2, 16, 1), (16, 32, 2), (32, 32, 1), (32, 3, 1), (3, 5, 2)])
+@pytest.mark.parametrize('layer', [(2, 16, 1), (16, 32, 2), (32, 32, 1), (32, 3, 1), (3, 5, 2),
+                                   (16, 5, 1), (32, 20, 2)])
 @pytest.mark.parametrize('shape', SHAPES)
 def test_conv_forward(gpu, layer, shape):
     from voxeldvc_amd import _lib as L
@@ -43,7 +44,7 @@ def test_conv_forward(gpu, layer, shape):
 @pytest.mark.parametrize('layer', [(16, 32, 2), (32, 32, 1), (32, 3, 1)])
 def test_conv_forward_many_blocks(gpu, layer):
     """A 17^3 (and 33^3 for the stride-2 layer) volume: tens of blocks of
-    256 output positions, partial last block."""
+    4 x 4 x 16 output positions, partial blocks on every axis."""
     from voxeldvc_amd import _lib as L
     C, O, s = layer
     n = 33 if s == 2 else 17

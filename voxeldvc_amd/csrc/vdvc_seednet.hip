@@ -216,6 +216,149 @@ __global__ __launch_bounds__(64 * kFwdWaves) void conv_fwd_lds_kernel(
     }
 }
 
+// Forward with the input neighbourhood of each output box staged in LDS as
+// bfloat16.  An output box is kTileZ x kTileY x 16 (z, y, x) positions --
+// kFwdWaves waves x kFwdTiles rows of 16 x-consecutive positions -- by all
+// output channels.  Its input box, ((kTileZ-1)s+3) x ((kTileY-1)s+3) x
+// (15s+3) positions x C channels, is read from global memory once (16-byte
+// loads along the rows), rounded to bfloat16 and stored with a position
+// stride of C + 4 values, so the 16 rows of an A fragment spread over the
+// banks; each of the 27 taps then reads its A fragments from LDS instead of
+// re-gathering and re-rounding them from L2.  Blocks are persistent (one per
+// CU): a block stages the layer's weights once, as conv_fwd_lds_kernel does,
+// then walks the output boxes with a grid stride, with two input-box buffers:
+// the next box's loads are issued into registers before the current box is
+// computed and written to the other buffer after it.  Per output the
+// products accumulate over the taps in order, then the 16-channel chunks, as
+// in the other forward kernels.
+constexpr int kTileZ = 4, kTileY = 4;
+static_assert(kTileZ * kTileY == kFwdWaves * kFwdTiles, "a box's rows are its waves' tiles");
+
+template <int NT, int CC, int STR>
+__global__ __launch_bounds__(64 * kFwdWaves) void conv_fwd_tile_kernel(
+    ConvShape S, const float* __restrict__ x, const float* __restrict__ w,
+    const float* __restrict__ bias, int relu, float* __restrict__ y) {
+  constexpr int C = 16 * CC;
+  constexpr int NF = 27 * CC * NT;
+  constexpr int HZ = (kTileZ - 1) * STR + 3, HY = (kTileY - 1) * STR + 3, HX = 15 * STR + 3;
+  constexpr int CS4 = (C + 4) / 4;  // position stride in 4-value groups
+  constexpr int NPOS = HZ * HY * HX;
+  constexpr int NQ = NPOS * (C / 4);  // 16-byte loads per input box
+  constexpr int NT_ = 64 * kFwdWaves;
+  constexpr int NLD = (NQ + NT_ - 1) / NT_;  // per thread
+  __shared__ bf16x4_t wl[NF * 64];
+  __shared__ bf16x4_t xl[2][NPOS * CS4];
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int e = threadIdx.x; e < NF * 64; e += NT_) {
+    const int f = e >> 6, l = e & 63;
+    const int t = f % NT, cc = (f / NT) % CC, tap = f / (NT * CC);
+    const int o = t * 16 + (l & 15), c0 = cc * 16 + 4 * (l >> 4);
+    float4 v = zero4;
+    if (o < S.O) v = *reinterpret_cast<const float4*>(w + ((long long)o * 27 + tap) * C + c0);
+    wl[e] = to_bf16x4(v);
+  }
+  // output boxes, x fastest, then y, z, n
+  const int nbx = (S.Wo + 15) / 16, nby = (S.Ho + kTileY - 1) / kTileY,
+            nbz = (S.Do + kTileZ - 1) / kTileZ;
+  const int nboxes = S.N * nbz * nby * nbx;
+  float4 pre[NLD];
+  auto load_box = [&](int box) {
+    const int bx = box % nbx, by = (box / nbx) % nby, bz = (box / (nbx * nby)) % nbz;
+    const int n = box / (nbx * nby * nbz);
+    const int iz0 = bz * kTileZ * STR - 1, iy0 = by * kTileY * STR - 1, ix0 = bx * 16 * STR - 1;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int e = threadIdx.x + i * NT_;
+      const int pos = e / (C / 4), c4 = e - pos * (C / 4);
+      const int hx = pos % HX, hy = (pos / HX) % HY, hz = pos / (HX * HY);
+      const int iz = iz0 + hz, iy = iy0 + hy, ix = ix0 + hx;
+      pre[i] = zero4;
+      if (e < NQ && iz >= 0 && iz < S.D && iy >= 0 && iy < S.H && ix >= 0 && ix < S.W)
+        pre[i] = *reinterpret_cast<const float4*>(
+            x + ((((long long)n * S.D + iz) * S.H + iy) * S.W + ix) * C + 4 * c4);
+    }
+  };
+  auto store_box = [&](bf16x4_t* buf) {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int e = threadIdx.x + i * NT_;
+      const int pos = e / (C / 4), c4 = e - pos * (C / 4);
+      if (e < NQ) buf[pos * CS4 + c4] = to_bf16x4(pre[i]);
+    }
+  };
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  int tz[kFwdTiles], ty[kFwdTiles], base[kFwdTiles];
+#pragma unroll
+  for (int u = 0; u < kFwdTiles; ++u) {
+    const int rr = wv * kFwdTiles + u;
+    tz[u] = rr / kTileY;
+    ty[u] = rr % kTileY;
+    // input-box position of this lane's row at tap (0, 0, 0)
+    base[u] = ((tz[u] * STR) * HY + ty[u] * STR) * HX + r * STR;
+  }
+  int box = blockIdx.x;
+  if (box < nboxes) {
+    load_box(box);
+    store_box(xl[0]);
+  }
+  __syncthreads();
+  for (int k = 0; box < nboxes; box += gridDim.x, ++k) {
+    const int nxt = box + gridDim.x;
+    if (nxt < nboxes) load_box(nxt);  // in flight while this box is computed
+    const bf16x4_t* cur = xl[k & 1];
+    f32x4_t acc[kFwdTiles][NT];
+#pragma unroll
+    for (int u = 0; u < kFwdTiles; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[u][t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 3
+    for (int tap = 0; tap < 27; ++tap) {
+      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+      const int toff = (dz * HY + dy) * HX + dx;
+      bf16x4_t a[kFwdTiles][CC];
+#pragma unroll
+      for (int u = 0; u < kFwdTiles; ++u)
+#pragma unroll
+        for (int cc = 0; cc < CC; ++cc) a[u][cc] = cur[(base[u] + toff) * CS4 + cc * 4 + q];
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const bf16x4_t bw = wl[((tap * CC + cc) * NT + t) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < kFwdTiles; ++u) acc[u][t] = mfma16(a[u][cc], bw, acc[u][t]);
+        }
+    }
+    // D[4q + j][r]: output x = bx*16 + 4q + j of row u, channel t*16 + r
+    const int bx = box % nbx, by = (box / nbx) % nby, bz = (box / (nbx * nby)) % nbz;
+    const int n = box / (nbx * nby * nbz);
+#pragma unroll
+    for (int u = 0; u < kFwdTiles; ++u) {
+      const int oz = bz * kTileZ + tz[u], oy = by * kTileY + ty[u];
+      if (oz >= S.Do || oy >= S.Ho) continue;
+      const long long prow = (((long long)n * S.Do + oz) * S.Ho + oy) * S.Wo;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int o = t * 16 + r;
+        if (o >= S.O) continue;
+        const float bo = bias ? bias[o] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int ox = bx * 16 + 4 * q + j;
+          if (ox >= S.Wo) continue;
+          float v = acc[u][t][j] + bo;
+          if (relu) v = v > 0.0f ? v : 0.0f;
+          y[(prow + ox) * S.O + o] = v;
+        }
+      }
+    }
+    // the other buffer was last read before the previous barrier
+    if (nxt < nboxes) store_box(xl[(k + 1) & 1]);
+    __syncthreads();
+  }
+}
+
 template <int NT>
 __global__ __launch_bounds__(256) void conv_fwd_c2_kernel(ConvShape S, const float* __restrict__ x,
                                                          const float* __restrict__ w,
@@ -592,8 +735,24 @@ int launch_conv_fwd(int N, int D, int H, int W, int C, int O, int stride, const 
   const bool two = O > 16;
   const unsigned nbl = (unsigned)((P + 16 * kFwdWaves * kFwdTiles - 1) / (16 * kFwdWaves * kFwdTiles));
 #define VDVC_FWDL(K) hipLaunchKernelGGL(K, dim3(nbl), dim3(64 * kFwdWaves), 0, st, S, x, w, b, relu, y)
-  if (C == 16) {
+  // the LDS-staged kernel for stride-1 inputs of 16 or 32 channels (a
+  // stride-2 input box is eight times the output box: the per-tap gathers
+  // of conv_fwd_lds_kernel measured faster there)
+  const long long nboxes = (long long)N * ((S.Do + kTileZ - 1) / kTileZ) *
+                           ((S.Ho + kTileY - 1) / kTileY) * ((S.Wo + 15) / 16);
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      ncu < 1)
+    ncu = 256;
+  const unsigned nbt = (unsigned)(nboxes < ncu ? nboxes : ncu);
+#define VDVC_FWDT(K) hipLaunchKernelGGL(K, dim3(nbt), dim3(64 * kFwdWaves), 0, st, S, x, w, b, relu, y)
+  if (C == 16 && stride == 1) {
+    if (two) VDVC_FWDT((conv_fwd_tile_kernel<2, 1, 1>)); else VDVC_FWDT((conv_fwd_tile_kernel<1, 1, 1>));
+  } else if (C == 16) {
     if (two) VDVC_FWDL((conv_fwd_lds_kernel<2, 1>)); else VDVC_FWDL((conv_fwd_lds_kernel<1, 1>));
+  } else if (C == 32 && stride == 1) {
+    if (two) VDVC_FWDT((conv_fwd_tile_kernel<2, 2, 1>)); else VDVC_FWDT((conv_fwd_tile_kernel<1, 2, 1>));
   } else if (C == 32) {
     if (two) VDVC_FWDL((conv_fwd_lds_kernel<2, 2>)); else VDVC_FWDL((conv_fwd_lds_kernel<1, 2>));
   } else if (C == 2) {
@@ -603,6 +762,7 @@ int launch_conv_fwd(int N, int D, int H, int W, int C, int O, int stride, const 
   }
 #undef VDVC_FWD
 #undef VDVC_FWDL
+#undef VDVC_FWDT
   return (int)hipGetLastError();
 }
 
