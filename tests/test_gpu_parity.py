@@ -61,6 +61,48 @@ def test_matvec_H_tilings(gpu, monkeypatch, shape, gt, nys):
     assert rel(L.to_host(out), ref) < 2e-5
 
 
+@pytest.mark.parametrize('epl', ['2', 'auto'])
+@pytest.mark.parametrize('nys', ['1', '2'])
+@pytest.mark.parametrize('gt', ['f32', 'bf16'])
+@pytest.mark.parametrize('shape', [(8, 8, 8, 4), (2, 1, 3, 4), (130, 2, 2, 4), (6, 3, 2, 2),
+                                   (128, 2, 2, 2), (7, 2, 2, 4)])
+def test_matvec_H_paired(gpu, monkeypatch, shape, gt, nys, epl):
+    """Lanes of two elements (V = 2h: bfloat16 at h = 4 and 2, float32 at
+    h = 2), forced (VDVC_EPL=2) and as planned (pairs where a strip fits the
+    row; odd Nx keeps single elements): the Hessian product and its diagonal
+    against the oracle, with and without a separate mask."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = shape
+    print('matvec_H paired', shape, gt, nys, epl, flush=True)
+    if epl == '2':
+        monkeypatch.setenv('VDVC_EPL', '2')
+    monkeypatch.setenv('VDVC_NYS', nys)
+    g, m, v = rand_problem(14, Nx, Ny, Nz, h)
+    shp = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    tdt = torch.float32 if gt == 'f32' else torch.bfloat16
+    dts = 'float32' if gt == 'f32' else 'bfloat16'
+    gq = torch.from_numpy(g).to(tdt)
+    g64 = gq.to(torch.float64).numpy()
+    ref = O.matvec_H(v.astype(np.float64), g64, m, h, Nx, Ny, Nz)
+    dref = O.build_diagonal_H(g64, m, h, Nx, Ny, Nz)
+    planes = [L.to_dev_level(gq[:, c].contiguous(), shp, dts) for c in range(3)]
+    mk = L.to_dev_level(m, shp, 'uint8')
+    folded = [L.to_dev_level((gq[:, c].float() * torch.from_numpy(m)).to(tdt).contiguous(), shp,
+                             dts) for c in range(3)]
+    dv = L.to_dev_flat(v)
+    code = L.F32 if gt == 'f32' else L.BF16
+    for G, mask in ((planes, mk), (folded, None)):
+        out = L.empty(dv.numel())
+        L.call('vdvc_matvec_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in G], code,
+               None if mask is None else L.ptr(mask), L.ptr(dv), L.ptr(out))
+        assert rel(L.to_host(out), ref) < 2e-5
+        d = L.empty(dv.numel())
+        L.call('vdvc_diag_H', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in G], code,
+               None if mask is None else L.ptr(mask), L.ptr(d))
+        assert rel(L.to_host(d), dref) < 2e-5
+
+
 @pytest.mark.parametrize('gt', ['f32', 'bf16'])
 def test_matvec_H_offset_views(gpu, gt):
     """Gradient and mask planes handed over as views whose base address is
@@ -369,14 +411,18 @@ def test_eqgap_golden_33(gpu):
     assert np.abs(U - zr['U']).max() < 1e-3
 
 
-def test_diag_and_pcg_with_row_split(gpu, monkeypatch):
+@pytest.mark.parametrize('knobs', [{'VDVC_NYS': '2'}, {'VDVC_EPL': '2'},
+                                   {'VDVC_EPL': '2', 'VDVC_NYS': '2'}])
+def test_diag_and_pcg_with_row_split(gpu, monkeypatch, knobs):
     """The diagonal and the PCG operator read the node partials through the
-    same gather: with the row split they give the oracle's diag(H) and
-    H v + ll L v."""
+    same gather: with the row split and with paired-element lanes they give
+    the oracle's diag(H) and H v + ll L v."""
     from voxeldvc_amd import _lib as L
     from voxeldvc_amd.engine import kernels_dvc as Kd
-    monkeypatch.setenv('VDVC_NYS', '2')
-    Nx, Ny, Nz, h = 7, 3, 4, 8
+    for k, val in knobs.items():
+        monkeypatch.setenv(k, val)
+    Nx, Ny, Nz, h = (8, 3, 4, 4) if 'VDVC_EPL' in knobs else (7, 3, 4, 8)
+    print('diag / apply_A', knobs, (Nx, Ny, Nz, h), flush=True)
     g, m, v = rand_problem(13, Nx, Ny, Nz, h)
     shp = Kd.level_shape(h, Nx, Ny, Nz)
     gx, gy, gz = Kd.grad_planes(g, shp)

@@ -5,10 +5,11 @@ rows x z tiles), restrict per lane, reduce across the lanes of one element
 and hand node partials to a gather kernel through per-wave slot blocks.  This
 model replays that index arithmetic lane by lane (64 lanes per wave) for both
 decompositions -- the vectorised one (V voxels per lane, element-aligned
-strips, last voxel plane as a per-row tail) and the generic one (one voxel per
-lane, 64-voxel strips) -- and checks on CPU that every voxel is visited exactly
-once, every slot the gather reads was written, and the result equals the
-oracle's matvec_H / build_diagonal_H.  It mirrors vdvc_hessian.hip,
+strips, last voxel plane as a per-row tail; with E = 2 a lane owns two
+elements, V = 2h) and the generic one (one voxel per lane, 64-voxel strips)
+-- and checks on CPU that every voxel is visited exactly once, every slot the
+gather reads was written, and the result equals the oracle's matvec_H /
+build_diagonal_H.  It mirrors vdvc_hessian.hip,
 vdvc_voxel.hip (generic kernel) and gather_partials in vdvc_node.hip.
 """
 import numpy as np
@@ -21,7 +22,7 @@ def aw(t, nl):
     return np.where((t == 0) | (t == nl), 0.5, 1.0)
 
 
-def plan(Nx, Ny, Nz, h, V, target, nys=1):
+def plan(Nx, Ny, Nz, h, V, target, nys=1, E=1):
     if V:
         sw = 64 * V
         nsx = -(-(Nx * h) // sw)
@@ -35,7 +36,7 @@ def plan(Nx, Ny, Nz, h, V, target, nys=1):
     nzt_t = max(1, -(-target // base))
     ez = max(1, -(-Nz // nzt_t))
     nzt = -(-Nz // ez)
-    return dict(V=V, sw=sw, nsx=nsx, nys=nys, px=px, ez=ez, nzt=nzt, nwaves=base * nzt,
+    return dict(V=V, E=E, sw=sw, nsx=nsx, nys=nys, px=px, ez=ez, nzt=nzt, nwaves=base * nzt,
                 stride=(ez + 1) * 2 * px * 3)
 
 
@@ -54,7 +55,8 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
     P = np.full((T['nwaves'], T['stride']), np.nan)
     lane = np.arange(64)
     xend = Nx * h
-    G = h // V
+    E = T.get('E', 1)
+    G = h // V if E == 1 else 1                       # lanes per element
     nys = T['nys']
     rpp = h // nys                                    # rows per part
     for w in range(T['nwaves']):
@@ -80,20 +82,21 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
         ty0 = ys * rpp                                # first local row of this part
         y0 = ey * h + ty0
         nrows = rpp + (1 if (ey == Ny - 1 and ys == nys - 1) else 0)
-        C = np.zeros((2, 2, 2, 3, 64))
+        C = np.zeros((E + 1, 2, 2, 3, 64))
 
         def emit(kzl):
             a = C[:, :, 0].copy()                     # (cx, cy, k, lane)
-            grp = a.reshape(2, 2, 3, 64 // G, G).sum(-1)
+            grp = a.reshape(E + 1, 2, 3, 64 // G, G).sum(-1)
             base = kzl * 2 * T['px'] * 3
-            for ln in range(64 // G + 1):
+            for ln in range((64 // G) * E + 1):
                 if ln >= nn:
                     continue
+                gi, cx = divmod(ln, E)                # lane group, its node column
                 val = np.zeros((2, 3))
-                if ln < 64 // G:
-                    val += grp[0, :, :, ln]
-                if ln >= 1:
-                    val += grp[1, :, :, ln - 1]
+                if gi < 64 // G:
+                    val += grp[cx, :, :, gi]
+                if cx == 0 and ln >= 1:
+                    val += grp[E, :, :, gi - 1]
                 for cy in range(2):
                     o = base + (cy * T['px'] + ln) * 3
                     P[w, o:o + 3] = val[cy]
@@ -109,51 +112,63 @@ def run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, covered):
                     def nv(i, j, k):
                         n = nid(i, j, k)
                         return np.stack([v[3 * n + c] for c in range(3)])
-                    l0 = Z0 * nv(ex, ey, ez) + Z1 * nv(ex, ey, ez + 1)
-                    l1 = Z0 * nv(ex, ey + 1, ez) + Z1 * nv(ex, ey + 1, ez + 1)
-                    r0 = Z0 * nv(ex + 1, ey, ez) + Z1 * nv(ex + 1, ey, ez + 1)
-                    r1 = Z0 * nv(ex + 1, ey + 1, ez) + Z1 * nv(ex + 1, ey + 1, ez + 1)
-                    duL = (l1 - l0) / h
-                    duR = (r1 - r0) / h
-                A = np.zeros((2, 2, 3, 64))
+                    # u at the lane's E + 1 node columns, row 0, and y slope
+                    c0 = [Z0 * nv(ex + cx, ey, ez) + Z1 * nv(ex + cx, ey, ez + 1)
+                          for cx in range(E + 1)]
+                    c1 = [Z0 * nv(ex + cx, ey + 1, ez) + Z1 * nv(ex + cx, ey + 1, ez + 1)
+                          for cx in range(E + 1)]
+                    dc = [(c1[cx] - c0[cx]) / h for cx in range(E + 1)]
+                A = np.zeros((E + 1, 2, 3, 64))
                 for ty in range(nrows):
                     y = y0 + ty
                     Wr = Wz * float(aw(y, Nvy - 1))
-                    S = np.zeros((3, 64))
-                    Tt = np.zeros((3, 64))
+                    S = np.zeros((E, 3, 64))
+                    Tt = np.zeros((E, 3, 64))
                     if mode == 'mv':
-                        uL = l0 + (ty0 + ty) * duL
-                        d = (r0 + (ty0 + ty) * duR) - uL
+                        uc = [c0[cx] + (ty0 + ty) * dc[cx] for cx in range(E + 1)]
+                        d = [uc[e + 1] - uc[e] for e in range(E)]
                     for j in range(V):
                         x = xbc + j
                         vi = x + Nvx * (y + Nvy * z)
                         covered[vi[act]] += 1
-                        X1 = X1b + j / h
+                        e = (j + tb) // h                   # the lane's element of voxel j
+                        X1 = X1b + j / h - e
                         wm = (wx0 if j == 0 else 1.0) * m[vi]
                         gg = g[vi].T
                         if mode == 'mv':
-                            u = uL + X1 * d
+                            # u of each lane's own element at its X1
+                            u = np.stack([uc[q] + X1 * d[q] for q in range(E)])[e, :, lane].T
                             s = wm * (gg * u).sum(0)
-                            S += s * gg
-                            Tt += X1 * s * gg
+                            for q in range(E):
+                                sel = e == q
+                                S[q][:, sel] += (s * gg)[:, sel]
+                                Tt[q][:, sel] += (X1 * s * gg)[:, sel]
                         else:
-                            S += (1 - X1) ** 2 * wm * gg ** 2
-                            Tt += X1 ** 2 * wm * gg ** 2
+                            for q in range(E):
+                                sel = e == q
+                                S[q][:, sel] += ((1 - X1) ** 2 * wm * gg ** 2)[:, sel]
+                                Tt[q][:, sel] += (X1 ** 2 * wm * gg ** 2)[:, sel]
                     if tail.any():
                         vi = xend + Nvx * (y + Nvy * z)
                         covered[vi] += 1
                         gg = g[vi][:, None]
                         wm = 0.5 * m[vi]
                         if mode == 'mv':
-                            s = wm * (gg * (uL + d)).sum(0)          # t = h: X1 = 1
+                            s = wm * (gg * (uc[E - 1] + d[E - 1])).sum(0)   # t = h: X1 = 1
                             add = (s * gg)[:, tail]
-                            S[:, tail] += add
+                            S[E - 1][:, tail] += add
                         else:
                             add = wm * gg ** 2
-                        Tt[:, tail] += add
-                    S[:, ~act] = 0
-                    Tt[:, ~act] = 0
-                    R = np.stack([S - Tt, Tt]) if mode == 'mv' else np.stack([S, Tt])
+                        Tt[E - 1][:, tail] += add
+                    S[:, :, ~act] = 0
+                    Tt[:, :, ~act] = 0
+                    # node-column row values; a paired lane's middle column takes
+                    # the right part of element 0 and the left part of element 1
+                    R = np.zeros((E + 1, 3, 64))
+                    R[0] = S[0] - Tt[0] if mode == 'mv' else S[0]
+                    for c in range(1, E):
+                        R[c] = Tt[c - 1] + (S[c] - Tt[c] if mode == 'mv' else S[c])
+                    R[E] = Tt[E - 1]
                     Y1 = (ty0 + ty) / h
                     wy = (Wr * (1 - Y1), Wr * Y1) if mode == 'mv' else (Wr * (1 - Y1) ** 2,
                                                                         Wr * Y1 ** 2)
@@ -220,6 +235,39 @@ CASES = [  # (Nx, Ny, Nz, h, itemsize, target)
     (130, 1, 1, 4, 2, 2), (65, 2, 1, 2, 2, 3), (3, 1, 2, 1, 2, 1), (2, 1, 2, 32, 2, 1),
     (1, 1, 1, 64, 4, 1), (17, 2, 2, 4, 4, 7), (64, 1, 1, 8, 2, 1),
 ]
+
+
+PAIRED = [  # (Nx, Ny, Nz, h, itemsize, target): Nx even, V = 2h fits 16 bytes
+    (2, 2, 2, 4, 2, 8192), (4, 1, 3, 4, 2, 2), (130, 1, 1, 4, 2, 2), (6, 2, 2, 2, 4, 3),
+    (66, 1, 2, 2, 2, 1), (8, 3, 2, 4, 2, 5),
+]
+
+
+@pytest.mark.parametrize('nys', [1, 2])
+@pytest.mark.parametrize('case', PAIRED)
+def test_paired_decomposition(case, nys):
+    """Lanes of two elements (V = 2h): three node columns per lane, the
+    middle one shared by the pair, the lanes' columns written without a
+    butterfly."""
+    Nx, Ny, Nz, h, isz, tg = case
+    V = 2 * h
+    assert V * isz <= 16 and Nx % 2 == 0
+    if h % nys:
+        pytest.skip('row split needs nys | h')
+    T = plan(Nx, Ny, Nz, h, V, tg, nys, E=2)
+    assert T['px'] == 64 * 2 + 2
+    rng = np.random.default_rng(sum(case) + 1)
+    Nvox = (Nx * h + 1) * (Ny * h + 1) * (Nz * h + 1)
+    g = rng.normal(size=(Nvox, 3))
+    m = rng.random(Nvox) < 0.8
+    v = rng.normal(size=3 * (Nx + 1) * (Ny + 1) * (Nz + 1))
+    for mode, ref in (('mv', O.matvec_H(v, g, m, h, Nx, Ny, Nz)),
+                      ('dg', O.build_diagonal_H(g, m, h, Nx, Ny, Nz))):
+        cov = np.zeros(Nvox, int)
+        P = run_vec(mode, v, g, m, Nx, Ny, Nz, h, T, cov)
+        assert (cov == 1).all(), np.unique(cov)
+        got = gather(P, T, Nx, Ny, Nz, h)
+        assert np.abs(got - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
 
 
 @pytest.mark.parametrize('nys', [1, 2])

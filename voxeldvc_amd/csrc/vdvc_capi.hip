@@ -171,6 +171,7 @@ void plan_z(const MeshDev& M, TilesDev& T, long long target) {
 TilesDev make_tiles_generic(const MeshDev& M) {
   TilesDev T;
   T.vec = 0;
+  T.epl = 1;
   T.sw = 64;
   T.nys = 1;
   T.nsx = (M.Nvx + 63) / 64;
@@ -180,8 +181,10 @@ TilesDev make_tiles_generic(const MeshDev& M) {
 }
 
 // decomposition of the voxel-streaming Hessian kernels: vectorised strips of
-// 64*V voxels over [0, Nx*h) when h is a power of two, else generic
-TilesDev make_tiles(const MeshDev& M, int gtype) {
+// 64*V voxels over [0, Nx*h) when h is a power of two, else generic.
+// `pairs`: the caller's kernel mode takes paired-element lanes (the products
+// and the diagonal; not the residual pass).
+TilesDev make_tiles(const MeshDev& M, int gtype, bool pairs) {
   int V = vdvc::vec_width(M.h, gtype);
   if (V == 0) return make_tiles_generic(M);
   // a row narrower than a 64 V-voxel strip would leave lanes idle: narrow
@@ -189,8 +192,25 @@ TilesDev make_tiles(const MeshDev& M, int gtype) {
   while (V > 1 && 64 * V > M.Nx * M.h && getenv("VDVC_VEC_BF16") == nullptr &&
          getenv("VDVC_VEC_F32") == nullptr)
     V >>= 1;
+  // paired elements: where a lane of one element holds h = 2 or 4 voxels and
+  // a 16-byte lane holds two elements, a lane may take two (V = 2h) if every
+  // lane owns two whole elements (Nx even).  Planned for bfloat16 at h = 4
+  // when a strip fits the row (measured at 513^3: 191 against 203-210 us);
+  // for float32 at h = 2 the paired lanes' registers cost more than they
+  // save (257^3: 80 against 75 us).  Tuning override, read at every plan:
+  // VDVC_EPL = 1 (never) or 2 (whenever the element count allows).
+  const char* ep = getenv("VDVC_EPL");
+  const int po = ep ? atoi(ep) : 0;
+  int E = 1;
+  if (pairs && po != 1 && V == M.h && (M.h == 2 || M.h == 4) &&
+      vdvc::vec_width(2 * M.h, gtype) == 2 * M.h && M.Nx % 2 == 0 &&
+      (po == 2 || (gtype == VDVC_BF16 && M.h == 4 && 64 * 2 * M.h <= M.Nx * M.h))) {
+    V = 2 * M.h;
+    E = 2;
+  }
   TilesDev T;
   T.vec = V;
+  T.epl = E;
   T.sw = 64 * V;
   T.nsx = (M.Nx * M.h + T.sw - 1) / T.sw;
   T.px = 64 * V / M.h + 2;
@@ -232,8 +252,8 @@ bool streams_aligned(const VoxIn& in) {
          ok(in.fold_out[1]) && ok(in.fold_out[2]);
 }
 
-TilesDev tiles_for(const MeshDev& M, int gtype, const VoxIn& in) {
-  TilesDev T = make_tiles(M, gtype);
+TilesDev tiles_for(const MeshDev& M, int gtype, const VoxIn& in, bool pairs) {
+  TilesDev T = make_tiles(M, gtype, pairs);
   if (T.vec && !streams_aligned(in)) T = make_tiles_generic(M);
   return T;
 }
@@ -823,7 +843,7 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   in.mean0 = mean0; in.gmean = gmean; in.kscale = kscale;
   in.res = d_res;
   in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
-  TilesDev T = tiles_for(M, gtype, in);
+  TilesDev T = tiles_for(M, gtype, in, false);
   if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
   if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
   // the display statistics only when the caller asks for them
@@ -845,7 +865,7 @@ int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  const TilesDev T = tiles_for(M, gtype, in, true);
   return VDVC_BY_RTYPE(gtype,
                        matvec_H_impl<float>(ctx, M, T, gtype, in, static_cast<const float*>(d_v),
                                             static_cast<float*>(d_out)),
@@ -864,7 +884,12 @@ int vdvc_matvec_H_noalloc(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const vo
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  TilesDev T = tiles_for(M, gtype, in, true);
+  // paired-element strips write more node partials; single-element lanes
+  // when the pairs' would not fit the static scratch
+  const size_t rsz = gtype == VDVC_F64 ? sizeof(double) : sizeof(float);
+  if (T.epl > 1 && (size_t)T.nwaves * T.wave_stride * rsz > VDVC_NOALLOC_SCRATCH_BYTES)
+    T = tiles_for(M, gtype, in, false);
   return VDVC_BY_RTYPE(gtype,
                        matvec_H_noalloc_impl<float>(ctx, M, T, gtype, in,
                                                     static_cast<const float*>(d_v),
@@ -883,7 +908,7 @@ int vdvc_diag_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!d_gx || !d_gy || !d_gz || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  const TilesDev T = tiles_for(M, gtype, in, true);
   return VDVC_BY_RTYPE(gtype, diag_H_impl<float>(ctx, M, T, gtype, in, static_cast<float*>(d_out)),
                        diag_H_impl<double>(ctx, M, T, gtype, in, static_cast<double*>(d_out)));
 }
@@ -963,7 +988,7 @@ int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
   if (!d_gx || !d_gy || !d_gz || !d_diag || !d_minv) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  const TilesDev T = tiles_for(M, gtype, in, true);
   return VDVC_BY_RTYPE(
       gtype,
       jacobi_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
@@ -985,7 +1010,7 @@ int vdvc_apply_A(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
   if (!d_gx || !d_gy || !d_gz || !d_v || !d_out) return fail(-1, "NULL buffer");
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  const TilesDev T = tiles_for(M, gtype, in, true);
   return VDVC_BY_RTYPE(
       gtype,
       apply_A_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,
@@ -1008,7 +1033,7 @@ int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, con
   if (check_every < 1) check_every = 1;
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   const VoxIn in = vox_in(d_gx, d_gy, d_gz, d_mask);
-  const TilesDev T = tiles_for(M, gtype, in);
+  const TilesDev T = tiles_for(M, gtype, in, true);
   return VDVC_BY_RTYPE(
       gtype,
       pcg_impl<float>(ctx, M, T, gtype, in, reg_kind, h_K1, h_K2, E, eps_reg, ll,

@@ -59,6 +59,7 @@ struct TilesDev {
   int ez;       // elements per z tile
   int nzt;      // z tiles
   int px;       // node slots per strip (>= nodes touched by 64 voxels)
+  int epl;      // elements per lane of the vectorised kernel: 1, or 2 (paired, V = 2h)
   long long nwaves;
   long long wave_stride;  // floats per wave in the partial buffer
 };

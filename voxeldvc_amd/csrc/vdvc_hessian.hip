@@ -20,6 +20,15 @@
 // kernel sums the slots of the (at most 8) waves touching a node in fixed
 // order.  The last voxel plane x = Nx*h is handled by the lane that owns the
 // last block of the last element, as its local coordinate t = h.
+//
+// Paired elements (E = 2, for h = V / 2: bfloat16 at h = 4, float32 or
+// bfloat16 at h = 2).  A lane of 4-voxel rows spends as much on its per-row
+// work -- the node interpolation, the y / z weights, the accumulation into
+// the node planes -- as on its voxels.  With E = 2 a lane owns two adjacent
+// elements (V = 2h voxels, one 16-byte load per plane): the two share their
+// middle node column, so a row interpolates three node columns and
+// accumulates three, for twice the voxels, and the lanes write their node
+// columns without a butterfly.
 #include <cstdlib>
 #include <type_traits>
 
@@ -178,15 +187,21 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
 // residual pass streams three more voxel arrays (f, sampled g, live mask)
 // through the row pipeline and is fitted to 2 (<= 256 VGPRs): at 4 its
 // registers spilled.
+// Paired-element lanes carry a third node column and are fitted to 3.
 #ifndef VDVC_VEC_MIN_BLOCKS
-#define VDVC_VEC_MIN_BLOCKS(MODE, GT) (MODE == MODE_RHS ? 2 : sizeof(GT) == 2 ? 4 : 3)
+#define VDVC_VEC_MIN_BLOCKS(MODE, GT, E) \
+  (MODE == MODE_RHS ? 2 : (E) > 1 ? 3 : sizeof(GT) == 2 ? 4 : 3)
 #endif
 
-template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V>
-__global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
+template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V, int E>
+__global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
                                                             const float* __restrict__ v,
                                                             float* __restrict__ P) {
-  __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 24 : 1];
+  static_assert(E == 1 || (E == 2 && MODE != MODE_RHS && V % 2 == 0),
+                "paired elements: products and diagonal only");
+  // element width of a paired-element lane (h = V / E); unused for E = 1
+  constexpr int HE = V / E;
+  __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 12 * (E + 1) : 1];
   const int lane = threadIdx.x & 63;
   // the wave index is uniform across the wave; saying so keeps every tile,
   // row and address computation below on the scalar unit
@@ -222,7 +237,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
   const bool tail = act && (xb + V == xend);
   const float X1b = tb * inv_h;
   const float wx0 = (xb == 0) ? 0.5f : 1.0f;  // x = 0 boundary weight (voxel 0 only)
-  const int G = h / V;                    // lanes per element
+  const int G = E > 1 ? 1 : h / V;       // lanes per element
   const int e_lo = x0 / h;
   const int e_hi = min(M.Nx, (x0 + SW) / h);
   const int nn = e_hi - e_lo + 1;
@@ -247,10 +262,10 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
   const float mean0_f = (float)in.mean0, gmean_f = (float)in.gmean, kscale_f = (float)in.kscale;
   float* Pw = P + (size_t)w * T.wave_stride;
 
-  // node-plane accumulators C[cx][cy][cz][k]
-  float C[2][2][2][3];
+  // node-plane accumulators C[cx][cy][cz][k] of the lane's E + 1 node columns
+  float C[E + 1][2][2][3];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < E + 1; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -264,16 +279,17 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
 
   auto emit = [&](int kzl) {
     // lanes of one element hold partial sums of that element's two x nodes
-    float a[2][2][3];
+    // (a paired-element lane: of its three)
+    float a[E + 1][2][3];
 #pragma unroll
-    for (int cx = 0; cx < 2; ++cx)
+    for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int k = 0; k < 3; ++k) a[cx][cy][k] = C[cx][cy][0][k];
     for (int o = 1; o < G; o <<= 1) {
 #pragma unroll
-      for (int cx = 0; cx < 2; ++cx)
+      for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -284,25 +300,31 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) prev[cy][k] = __shfl(a[1][cy][k], src, 64);
+      for (int k = 0; k < 3; ++k) prev[cy][k] = __shfl(a[E][cy][k], src, 64);
     float* slot = Pw + (size_t)kzl * 2 * T.px * 3;
     if ((lane % G) == 0) {
-      const int ln = lane / G;
-      if (ln < nn) {
+      // the element group's first node column ln (the previous group's last
+      // column added in), then a paired lane's middle column
+      const int ln = (lane / G) * E;
 #pragma unroll
-        for (int cy = 0; cy < 2; ++cy)
+      for (int cx = 0; cx < E; ++cx) {
+        if (ln + cx < nn) {
 #pragma unroll
-          for (int k = 0; k < 3; ++k)
-            slot[((size_t)cy * T.px + ln) * 3 + k] = a[0][cy][k] + (ln > 0 ? prev[cy][k] : 0.0f);
+          for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+              slot[((size_t)cy * T.px + ln + cx) * 3 + k] =
+                  a[cx][cy][k] + (cx == 0 && ln > 0 ? prev[cy][k] : 0.0f);
+        }
       }
     }
     if (lane == 63) {
-      const int ln = 64 / G;
+      const int ln = (64 / G) * E;
       if (ln < nn) {
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) slot[((size_t)cy * T.px + ln) * 3 + k] = a[1][cy][k];
+          for (int k = 0; k < 3; ++k) slot[((size_t)cy * T.px + ln) * 3 + k] = a[E][cy][k];
       }
     }
   };
@@ -366,16 +388,23 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
   float Z0 = 1.0f, Z1 = 0.0f, Wz = 1.0f, zw0 = 1.0f, zw1 = 0.0f;
   int mz = 1;
   long long rowbase = 0, tailbase = 0;
-  float uL0[3] = {0, 0, 0}, duL[3] = {0, 0, 0}, uR0[3] = {0, 0, 0}, duR[3] = {0, 0, 0};
-  // the 24 nodal values of the lane's element (corner (cx,cy,cz), component
-  // k) live in LDS, slot q = ((cx*2 + cy)*2 + cz)*3 + k at [q][lane], loaded
-  // once per element layer; plane set-up then reads LDS only, whose wait
-  // counter is separate from the row pipeline's
-  float* nlds = lds_nodes + wib * 64 * 24;
+  // u of the plane at the lane's node columns (row 0 of the element row)
+  // and its y slope
+  float u0[E + 1][3], du[E + 1][3];
+#pragma unroll
+  for (int cx = 0; cx < E + 1; ++cx)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { u0[cx][k] = 0.0f; du[cx][k] = 0.0f; }
+  // the nodal values of the lane's element(s) (corner (cx,cy,cz), component
+  // k; cx over the E + 1 node columns) live in LDS, slot
+  // q = ((cx*2 + cy)*2 + cz)*3 + k at [q][lane], loaded once per element
+  // layer; plane set-up then reads LDS only, whose wait counter is separate
+  // from the row pipeline's
+  float* nlds = lds_nodes + wib * 64 * 12 * (E + 1);
   auto element_setup = [&]() {
     if (MODE != MODE_MATVEC) return;
 #pragma unroll
-    for (int cx = 0; cx < 2; ++cx)
+    for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -396,19 +425,19 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     rowbase = (long long)z * pl + (long long)y0 * M.pitch;
     tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
     if (MODE == MODE_MATVEC) {
-      // u at the element's left / right node columns, linear in y
+      // u at the element's node columns (left / right; a paired lane's
+      // left / middle / right), linear in y
       auto nv = [&](int cx, int cy, int cz, int k) {
         return nlds[((((cx * 2 + cy) * 2 + cz) * 3) + k) * 64 + lane];
       };
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float l0 = Z0 * nv(0, 0, 0, k) + Z1 * nv(0, 0, 1, k);
-        const float l1 = Z0 * nv(0, 1, 0, k) + Z1 * nv(0, 1, 1, k);
-        const float r0 = Z0 * nv(1, 0, 0, k) + Z1 * nv(1, 0, 1, k);
-        const float r1 = Z0 * nv(1, 1, 0, k) + Z1 * nv(1, 1, 1, k);
-        uL0[k] = l0; duL[k] = (l1 - l0) * inv_h;
-        uR0[k] = r0; duR[k] = (r1 - r0) * inv_h;
-      }
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int cx = 0; cx < E + 1; ++cx) {
+          const float c0 = Z0 * nv(cx, 0, 0, k) + Z1 * nv(cx, 0, 1, k);
+          const float c1 = Z0 * nv(cx, 1, 0, k) + Z1 * nv(cx, 1, 1, k);
+          u0[cx][k] = c0; du[cx][k] = (c1 - c0) * inv_h;
+        }
     }
   };
   // advance to the next row; closes element layers (emit + shift) and sets
@@ -419,7 +448,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     if (++tz == nzp) {
       emit(ez - ez0);
 #pragma unroll
-      for (int cx = 0; cx < 2; ++cx)
+      for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -466,15 +495,21 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
     const float* gy = cur.gy;
     const float* gz = cur.gz;
     const uint32_t* mk = cur.mk;
-    float S[3] = {0, 0, 0}, Tt[3] = {0, 0, 0};
-    float ux0 = 0, uy0 = 0, uz0 = 0, dxx = 0, dxy = 0, dxz = 0;
-    if (MODE == MODE_MATVEC) {
-      ux0 = uL0[0] + fty * duL[0];
-      uy0 = uL0[1] + fty * duL[1];
-      uz0 = uL0[2] + fty * duL[2];
-      dxx = (uR0[0] + fty * duR[0]) - ux0;
-      dxy = (uR0[1] + fty * duR[1]) - uy0;
-      dxz = (uR0[2] + fty * duR[2]) - uz0;
+    // x-restriction sums of the lane's element(s)
+    float S[E][3], Tt[E][3];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { S[e][k] = 0.0f; Tt[e][k] = 0.0f; }
+    // u of this row at the node columns, and its step across each element
+    float uc[E + 1][3], dxe[E][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int cx = 0; cx < E + 1; ++cx)
+        uc[cx][k] = MODE == MODE_MATVEC ? u0[cx][k] + fty * du[cx][k] : 0.0f;
+#pragma unroll
+      for (int e = 0; e < E; ++e) dxe[e][k] = uc[e + 1][k] - uc[e][k];
     }
     float resv[V];
     if (MODE == MODE_RHS) {
@@ -518,18 +553,22 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
 #pragma unroll
     for (int j = 0; j < V; ++j) {
       // when the host launched with h == V every block starts an element
-      // (tb == 0) and X1 = j / V is a constant
-      const float X1 = (h == V) ? (float)j * (1.0f / (float)V) : X1b + (float)j * inv_h;
+      // (tb == 0) and X1 = j / V is a constant; a paired lane's voxel j lies
+      // in its element j / HE at X1 = (j % HE) / HE
+      const int e = E > 1 ? j / HE : 0;
+      const float X1 = E > 1 ? (float)(j % HE) * (1.0f / (float)HE)
+                             : (h == V) ? (float)j * (1.0f / (float)V) : X1b + (float)j * inv_h;
       float wm = (j == 0) ? wx0 : 1.0f;
       if (HASMASK) wm = mk[j] ? wm : 0.0f;
       float sd = 0.0f;
       if (MODE == MODE_MATVEC) {
-        const float ux = ux0 + X1 * dxx, uy = uy0 + X1 * dxy, uz = uz0 + X1 * dxz;
+        const float ux = uc[e][0] + X1 * dxe[e][0], uy = uc[e][1] + X1 * dxe[e][1],
+                    uz = uc[e][2] + X1 * dxe[e][2];
         sd = gx[j] * ux + gy[j] * uy + gz[j] * uz;
       } else if (MODE == MODE_RHS) {
         sd = resv[j];
       }
-      acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S, Tt);
+      acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S[e], Tt[e]);
     }
     if (tail) {
       // last voxel plane x = Nx*h: t = h of this element, X1 = 1, W_x = 1/2
@@ -546,7 +585,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
       float wm = tm ? 0.5f : 0.0f;
       float sd = 0.0f;
       if (MODE == MODE_MATVEC) {
-        sd = tgx * (ux0 + dxx) + tgy * (uy0 + dxy) + tgz * (uz0 + dxz);
+        sd = tgx * (uc[E - 1][0] + dxe[E - 1][0]) + tgy * (uc[E - 1][1] + dxe[E - 1][1]) +
+             tgz * (uc[E - 1][2] + dxe[E - 1][2]);
       } else if (MODE == MODE_RHS) {
         const float fv = ld_val(fim, ti);
         const float gv = gvals_f[ti];
@@ -568,22 +608,27 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT)) void vox
         }
         sd = rv;
       }
-      acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S, Tt);
+      acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S[E - 1], Tt[E - 1]);
     }
     // x-restricted row values R[cx][k] (the lanes past the row end have
     // zero node weights), weighted into the node planes
-    float R[2][3];
+    // (a paired lane's middle column takes the right part of element 0 and
+    // the left part of element 1)
+    float R[E + 1][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      if (MODE == MODE_DIAG) { R[0][k] = S[k]; R[1][k] = Tt[k]; }
-      else { R[0][k] = S[k] - Tt[k]; R[1][k] = Tt[k]; }
+      R[0][k] = MODE == MODE_DIAG ? S[0][k] : S[0][k] - Tt[0][k];
+#pragma unroll
+      for (int c = 1; c < E; ++c)
+        R[c][k] = Tt[c - 1][k] + (MODE == MODE_DIAG ? S[c][k] : S[c][k] - Tt[c][k]);
+      R[E][k] = Tt[E - 1][k];
     }
     const float Y1 = fty * inv_h;
     float wy0 = Wr * (1.0f - Y1), wy1 = Wr * Y1;
     if (MODE == MODE_DIAG) { wy0 = Wr * (1.0f - Y1) * (1.0f - Y1); wy1 = Wr * Y1 * Y1; }
     const float w00 = wy0 * zw0, w01 = wy0 * zw1, w10 = wy1 * zw0, w11 = wy1 * zw1;
 #pragma unroll
-    for (int cx = 0; cx < 2; ++cx)
+    for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         C[cx][0][0][k] += w00 * R[cx][k];
@@ -672,18 +717,48 @@ int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold,
                         hipStream_t st) {
   const unsigned nblk = (unsigned)((T.nwaves + VDVC_WAVES_PER_BLOCK - 1) / VDVC_WAVES_PER_BLOCK);
   dim3 g(nblk), b(VDVC_BLOCK);
-#define VV_L(MODE, GT, HM, FT, FO, VV) \
-  hipLaunchKernelGGL((vox_vec_kernel<MODE, GT, HM, FT, FO, VV>), g, b, 0, st, M, T, in, v, P)
-#define VV_V(MODE, GT, HM, FT, FO)                 \
-  do {                                             \
-    switch (T.vec) {                               \
-      case 8: VV_L(MODE, GT, HM, FT, FO, 8); break; \
-      case 4: VV_L(MODE, GT, HM, FT, FO, 4); break; \
-      case 2: VV_L(MODE, GT, HM, FT, FO, 2); break; \
-      default: VV_L(MODE, GT, HM, FT, FO, 1); break; \
-    }                                              \
+#define VV_L(MODE, GT, HM, FT, FO, VV, EE) \
+  hipLaunchKernelGGL((vox_vec_kernel<MODE, GT, HM, FT, FO, VV, EE>), g, b, 0, st, M, T, in, v, P)
+#define VV_V(MODE, GT, HM, FT, FO)                    \
+  do {                                                \
+    switch (T.vec) {                                  \
+      case 8: VV_L(MODE, GT, HM, FT, FO, 8, 1); break; \
+      case 4: VV_L(MODE, GT, HM, FT, FO, 4, 1); break; \
+      case 2: VV_L(MODE, GT, HM, FT, FO, 2, 1); break; \
+      default: VV_L(MODE, GT, HM, FT, FO, 1, 1); break; \
+    }                                                 \
   } while (0)
   if (gtype == GT_F32 && T.vec == 8) return (int)hipErrorInvalidValue;
+  if (T.epl == 2) {
+    // paired elements: bfloat16 lanes of 8 (h = 4) or 4 (h = 2) voxels,
+    // float32 lanes of 4 (h = 2); products and diagonal only
+    if (mode == MODE_RHS || M.h * 2 != T.vec || (T.vec != 8 && T.vec != 4) ||
+        (gtype == GT_F32 && T.vec != 4))
+      return (int)hipErrorInvalidValue;
+#define VV_P(MODE, GT, HM)                                          \
+    do {                                                            \
+      if (T.vec == 8) VV_L(MODE, GT, HM, float, false, 8, 2);       \
+      else VV_L(MODE, GT, HM, float, false, 4, 2);                  \
+    } while (0)
+#define VV_PM(MODE, GT) \
+    do { if (hasmask) VV_P(MODE, GT, true); else VV_P(MODE, GT, false); } while (0)
+    if (gtype == GT_F32) {
+      if (mode == MODE_MATVEC) {
+        if (hasmask) VV_L(MODE_MATVEC, float, true, float, false, 4, 2);
+        else VV_L(MODE_MATVEC, float, false, float, false, 4, 2);
+      } else {
+        if (hasmask) VV_L(MODE_DIAG, float, true, float, false, 4, 2);
+        else VV_L(MODE_DIAG, float, false, float, false, 4, 2);
+      }
+    } else if (mode == MODE_MATVEC) {
+      VV_PM(MODE_MATVEC, __hip_bfloat16);
+    } else {
+      VV_PM(MODE_DIAG, __hip_bfloat16);
+    }
+#undef VV_PM
+#undef VV_P
+    return (int)hipGetLastError();
+  }
   if (mode == MODE_MATVEC) {
     if (gtype == GT_F32) {
       if (hasmask) VV_V(MODE_MATVEC, float, true, float, false);
