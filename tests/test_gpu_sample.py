@@ -74,6 +74,34 @@ def test_sample_half_and_float_images_agree(gpu, shape):
         assert np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize('gnx', [1, 2, 3])
+def test_sample_narrow_images(gpu, gnx):
+    """Images one, two and three voxels wide in x: a float32 image takes both
+    x-neighbours of a tap from one 8-byte load clamped to the row (a
+    one-voxel row reads its high word past the image), a float16 image loads
+    each tap alone.  Values, masks and statistics are identical, and match
+    the oracle."""
+    from voxeldvc_amd import _lib as L
+    Nx, Ny, Nz, h = 3, 2, 2, 4
+    print('sample narrow image', gnx, flush=True)
+    rng = np.random.default_rng(40 + gnx)
+    gimg = rng.integers(0, 256, size=(gnx, Ny * h + 2, Nz * h + 1)).astype(np.float32)
+    U = rng.normal(scale=1.5, size=3 * (Nx + 1) * (Ny + 1) * (Nz + 1)).astype(np.float32)
+    g32, gs = L.to_dev_volume(gimg, 'float32')
+    g16, _ = L.to_dev_volume(gimg, 'float16')
+    go = (-0.25, -1, 0)
+    for f32w in (False, True):
+        a = _run_sample(U, g32, L.F32, gs, Nx, Ny, Nz, h, go, None, None, -1.0, f32w)
+        b = _run_sample(U, g16, L.F16, gs, Nx, Ny, Nz, h, go, None, None, -1.0, f32w)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    # clamped samples carry values outside the box too; the mask is the
+    # in-bounds test there
+    a = _run_sample(U, g32, L.F32, gs, Nx, Ny, Nz, h, go, None, None, -1.0)
+    rv, rm = O.sample_deformed_field(U, gimg, h, Nx, Ny, Nz, go, None, glt=-1.0)
+    assert (a[1] != rm).mean() < 2e-3
+    assert np.abs(a[0] - rv).max() < 2e-3
+
+
 @pytest.mark.parametrize('shape', SHAPES)
 def test_sample_matches_oracle(gpu, shape):
     from voxeldvc_amd import _lib as L

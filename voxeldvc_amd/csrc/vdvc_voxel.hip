@@ -363,6 +363,22 @@ __device__ __forceinline__ float buf_tap(__amdgpu_buffer_rsrc_t r, unsigned i, c
 __device__ __forceinline__ double buf_tap(__amdgpu_buffer_rsrc_t, unsigned, const double*) {
   return 0.0;  // not used: the FP64 image takes 64-bit addresses
 }
+// the two x-neighbours (i, j) of a float image row from one 8-byte load at
+// p = clamp(i, 0, nx - 2): (p, p + 1) holds both for i <= nx - 2, and
+// i = j = nx - 1 reads element p + 1 twice.  A one-voxel-wide row (nx = 1,
+// p = 0, i = j = 0) takes the low word; its high word may lie past the
+// image, which the buffer's range check returns as 0.  The sampler's
+// gathers are bound by the texture addresser's per-lane work, so half the
+// load instructions carry the same taps.  Dword alignment is what a
+// multi-dword buffer load needs.
+__device__ __forceinline__ void buf_tap2(__amdgpu_buffer_rsrc_t r, unsigned b, int i, int j,
+                                         int nx, float& vi, float& vj) {
+  const int p = max(min(i, nx - 2), 0);
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, (b + (unsigned)p) * 4u, 0, 0);
+  const float lo = __uint_as_float(w[0]), hi = __uint_as_float(w[1]);
+  vi = i > p ? hi : lo;
+  vj = j > p ? hi : lo;
+}
 template <typename GI>
 struct TriTaps {
   using C = typename RealOf<GI>::type;
@@ -385,10 +401,17 @@ __device__ __forceinline__ void tri_fetch(const GI* __restrict__ img, __amdgpu_b
     const unsigned sy = (unsigned)nx, sz = (unsigned)nx * (unsigned)ny;
     const unsigned b00 = iy * sy + iz * sz, b10 = jy * sy + iz * sz;
     const unsigned b01 = iy * sy + jz * sz, b11 = jy * sy + jz * sz;
-    t.v[0] = buf_tap(rs, b00 + ix, img); t.v[1] = buf_tap(rs, b00 + jx, img);
-    t.v[2] = buf_tap(rs, b10 + ix, img); t.v[3] = buf_tap(rs, b10 + jx, img);
-    t.v[4] = buf_tap(rs, b01 + ix, img); t.v[5] = buf_tap(rs, b01 + jx, img);
-    t.v[6] = buf_tap(rs, b11 + ix, img); t.v[7] = buf_tap(rs, b11 + jx, img);
+    if constexpr (std::is_same<GI, float>::value) {
+      buf_tap2(rs, b00, ix, jx, nx, t.v[0], t.v[1]);
+      buf_tap2(rs, b10, ix, jx, nx, t.v[2], t.v[3]);
+      buf_tap2(rs, b01, ix, jx, nx, t.v[4], t.v[5]);
+      buf_tap2(rs, b11, ix, jx, nx, t.v[6], t.v[7]);
+    } else {
+      t.v[0] = buf_tap(rs, b00 + ix, img); t.v[1] = buf_tap(rs, b00 + jx, img);
+      t.v[2] = buf_tap(rs, b10 + ix, img); t.v[3] = buf_tap(rs, b10 + jx, img);
+      t.v[4] = buf_tap(rs, b01 + ix, img); t.v[5] = buf_tap(rs, b01 + jx, img);
+      t.v[6] = buf_tap(rs, b11 + ix, img); t.v[7] = buf_tap(rs, b11 + jx, img);
+    }
   } else {
     const long long sy = nx, sz = (long long)nx * ny;
     const long long b00 = iy * sy + iz * sz, b10 = jy * sy + iz * sz;
