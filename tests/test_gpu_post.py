@@ -11,8 +11,10 @@ from oracle import post_oracle as P
 
 pytestmark = pytest.mark.gpu
 
+# h = 64 is the largest spacing whose per-plane weight factors come from the
+# launch's table, h = 66 forms them in the kernel
 SHAPES = [(2, 2, 2, 8), (3, 2, 4, 4), (1, 1, 1, 1), (5, 3, 2, 3), (17, 2, 2, 4), (2, 1, 3, 16),
-          (70, 1, 2, 1)]
+          (70, 1, 2, 1), (1, 1, 1, 64), (1, 1, 1, 66)]
 
 
 def _U(seed, Nx, Ny, Nz, scale, dtype):
@@ -60,7 +62,7 @@ def test_interp_rejects_other_stencils(gpu):
 @pytest.mark.parametrize('udtype', [np.float32, np.float64])
 @pytest.mark.parametrize('gdtype', [np.float32, np.float64, np.uint8, np.uint16, np.int16,
                                     np.bool_])
-@pytest.mark.parametrize('shape', [(3, 2, 2, 4), (2, 3, 1, 8), (5, 3, 2, 3)])
+@pytest.mark.parametrize('shape', [(3, 2, 2, 4), (2, 3, 1, 8), (5, 3, 2, 3), (1, 1, 1, 66)])
 def test_compute_active_mask(gpu, shape, gdtype, udtype):
     from voxeldvc_amd.engine import correlate_gpu as C
     Nx, Ny, Nz, h = shape

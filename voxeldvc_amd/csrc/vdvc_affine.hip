@@ -340,7 +340,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void warp_affine_kernel(WarpIn W,
     p[i] = (double)add_rn(add_rn(m, W.c[i]), W.t[i]);
   }
   const long long i = (long long)x + (long long)W.nx * ((long long)y + (long long)W.ny * z);
-  out[i] = (float)as_img_type(sample01<GT_F32>(g, W.gnx, W.gny, W.gnz, p[0], p[1], p[2], 1, 1, 0.0),
+  out[i] = (float)as_img_type(sample01<GT_F32>(g, W.gnx, W.gny, W.gnz, p[0], p[1], p[2], 1, 1, 0.0,
+                                               vdvc_img::img_rsrc(g, 0u)),
                               GT_F32);
 }
 

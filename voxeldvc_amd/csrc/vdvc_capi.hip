@@ -364,7 +364,29 @@ PostIn post_in(const MeshDev& M, const void* U) {
   P.nnx = M.nnx; P.nny = M.nny;
   P.Nvox = M.Nvox;
   P.U = U;
+  // z factors of the Hex8 voxel weights per local plane k = 0..h: the same
+  // IEEE double operations as the kernels' (2k / h) - 1, 1 - xi, 1 + xi
+  if (M.h <= kPostFzMax) {
+    P.fz_tab = 1;
+    for (int k = 0; k <= M.h; ++k) {
+      const double q = (2.0 * k) / (double)M.h;
+      const double xi = q - 1.0;
+      P.fz[k][0] = 1.0 - xi;
+      P.fz[k][1] = 1.0 + xi;
+    }
+  }
   return P;
+}
+
+// the sampled image of a post-solve stage; images below 2^32 bytes are read
+// with 32-bit offsets
+void post_img(PostIn& P, const void* img, int itype, int gnx, int gny, int gnz) {
+  P.img = img; P.itype = itype; P.gnx = gnx; P.gny = gny; P.gnz = gnz;
+  const size_t es = itype == VDVC_F64 ? 8 : itype == VDVC_F32 ? 4
+                    : (itype == VDVC_U16 || itype == VDVC_I16) ? 2 : 1;
+  const unsigned long long bytes =
+      (unsigned long long)gnx * (unsigned long long)gny * (unsigned long long)gnz * es;
+  P.img_bytes = bytes < (1ull << 32) ? (unsigned)bytes : 0u;
 }
 
 int run_post_timed(vdvc_ctx* ctx, int op, int rtype, const PostIn& P) {
@@ -1288,7 +1310,7 @@ int vdvc_active_mask(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   if ((rc = post_grid_ok(M))) return rc;
   PostIn P = post_in(M, d_U);
-  P.img = d_g; P.itype = gtype; P.gnx = gnx; P.gny = gny; P.gnz = gnz;
+  post_img(P, d_g, gtype, gnx, gny, gnz);
   P.ox = gox; P.oy = goy; P.oz = goz; P.glt = glt; P.mask0 = d_mask0;
   P.out = d_mask_out;
   return run_post_timed(ctx, POST_MASK, rtype, P);
@@ -1307,7 +1329,7 @@ int vdvc_push_forward(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   MeshDev M = make_mesh(Nx, Ny, Nz, h);
   if ((rc = post_grid_ok(M))) return rc;
   PostIn P = post_in(M, d_U);
-  P.img = d_field; P.itype = ftype; P.gnx = M.Nvx; P.gny = M.Nvy; P.gnz = M.Nvz;
+  post_img(P, d_field, ftype, M.Nvx, M.Nvy, M.Nvz);
   P.order = order; P.mode = mode; P.cval = cval;
   P.out = d_out;
   return run_post_timed(ctx, POST_PUSH, rtype, P);

@@ -33,6 +33,8 @@ struct SampleIn {
 
 // post-solve voxel stages on dense voxel arrays (flat index x + Nvx*(y + Nvy*z))
 enum { POST_INTERP = 0, POST_PUSH = 1, POST_MASK = 2 };
+constexpr int kPostFzMax = 64;
+
 struct PostIn {
   int Nx, Ny, Nz, h;
   int Nvx, Nvy, Nvz;
@@ -48,6 +50,11 @@ struct PostIn {
   const uint8_t* mask0;   // mask: optional reference mask
   void* out;              // interp: R; push: itype; mask: uint8
   int channels_first;     // interp: (3, Nvox) instead of (Nvox, 3)
+  unsigned img_bytes;     // bytes of img when below 2^32 (32-bit offsets), else 0
+  // z factors (1 - xi, 1 + xi) of the Hex8 weights at local plane k, xi =
+  // 2k/h - 1 formed in double as the kernels form it; filled for h <= 64
+  int fz_tab;
+  double fz[kPostFzMax + 1][2];
 };
 
 struct QualIn {

@@ -85,10 +85,19 @@ struct Column {
       for (int c = 0; c < 3; ++c) uc[a][c] = U[3 * node + c];
     }
   }
-  // u at local z voxel tz of the layer: sum over corners a = 0..7 in order
-  __device__ __forceinline__ void at(int tz, int h, R (&u)[3]) const {
-    const double ze = stencil_xi(tz, h);
-    const double Fz[2] = {sub_rn(1.0, ze), add_rn(1.0, ze)};
+  // u at local z voxel tz of the layer: sum over corners a = 0..7 in order.
+  // The z factors are the same for every lane of a plane: they come from the
+  // launch's table (h <= 64) instead of a float64 division per voxel.
+  __device__ __forceinline__ void at(const PostIn& P, int tz, R (&u)[3]) const {
+    double Fz[2];
+    if (P.fz_tab) {
+      Fz[0] = P.fz[tz][0];
+      Fz[1] = P.fz[tz][1];
+    } else {
+      const double ze = stencil_xi(tz, P.h);
+      Fz[0] = sub_rn(1.0, ze);
+      Fz[1] = add_rn(1.0, ze);
+    }
     u[0] = u[1] = u[2] = R(0);
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -106,8 +115,9 @@ __device__ __forceinline__ double clip_r(double c, double hi) { return fmin(fmax
 // layer's h voxel planes (h + 1 for the last layer, which owns the last
 // plane), 64 x-consecutive columns per wavefront.  The sampling stages work
 // on two planes per step so both planes' gathers are in flight together.
-// IT is the sampled image's element type (POST_PUSH, POST_MASK).
-template <typename R, int OP, int IT>
+// IT is the sampled image's element type (POST_PUSH, POST_MASK); O32: the
+// image holds fewer than 2^32 bytes and is read with 32-bit offsets.
+template <typename R, int OP, int IT, bool O32>
 __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -124,7 +134,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
     long long v = v0;
     for (int tz = 0; tz < nz; ++tz, v += plane) {
       R u[3];
-      col.at(tz, h, u);
+      col.at(P, tz, u);
       if (P.channels_first) {
         o[v] = u[0];
         o[P.Nvox + v] = u[1];
@@ -137,18 +147,21 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
     }
     return;
   }
+  const __amdgpu_buffer_rsrc_t rs =
+      vdvc_img::img_rsrc(P.img, O32 ? P.img_bytes : 0u);
   // the stage's value at local plane tz: the pushed field value (PUSH) or
   // the active-mask bit (MASK)
   auto value = [&](int tz) -> double {
     const int z = ez * h + tz;
     R u[3];
-    col.at(tz, h, u);
+    col.at(P, tz, u);
     if (OP == POST_PUSH) {
       // coordinates x - u(x) in U's dtype
       const double cx = (double)sub_rn((R)x, u[0]);
       const double cy = (double)sub_rn((R)y, u[1]);
       const double cz = (double)sub_rn((R)z, u[2]);
-      return sample01<IT>(P.img, P.gnx, P.gny, P.gnz, cx, cy, cz, P.order, P.mode, P.cval);
+      return sample01<IT, O32>(P.img, P.gnx, P.gny, P.gnz, cx, cy, cz, P.order, P.mode, P.cval,
+                               rs);
     }
     // (u + x) - origin in U's dtype; bounds on the closed box; clipped
     R cx = sub_rn(add_rn(u[0], (R)x), (R)P.ox);
@@ -159,8 +172,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void post_voxel_kernel(PostIn P) {
     cx = clip_r(cx, hx);
     cy = clip_r(cy, hy);
     cz = clip_r(cz, hz);
-    const double gv = as_img_type(sample01<IT>(P.img, P.gnx, P.gny, P.gnz, (double)cx,
-                                               (double)cy, (double)cz, 1, 0, 0.0),
+    const double gv = as_img_type(sample01<IT, O32>(P.img, P.gnx, P.gny, P.gnz, (double)cx,
+                                                    (double)cy, (double)cz, 1, 0, 0.0, rs),
                                   IT);
     // g > glt in g's dtype for a float32 image (the threshold is a weak
     // scalar there), in double otherwise
@@ -344,22 +357,30 @@ __global__ __launch_bounds__(VDVC_BLOCK) void zncc_pass_kernel(ZnccIn Z, int pas
 
 namespace vdvc {
 
-template <typename R, int OP>
+template <typename R, int OP, bool O32>
 void launch_post_typed(const PostIn& P, dim3 g, dim3 b, hipStream_t st) {
   switch (P.itype) {
-    case GT_F32: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F32>), g, b, 0, st, P); break;
-    case GT_F64: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F64>), g, b, 0, st, P); break;
-    case GT_U8: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U8>), g, b, 0, st, P); break;
-    case GT_U16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U16>), g, b, 0, st, P); break;
-    case GT_I16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_I16>), g, b, 0, st, P); break;
-    default: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_B8>), g, b, 0, st, P); break;
+    case GT_F32: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F32, O32>), g, b, 0, st, P); break;
+    case GT_F64: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_F64, O32>), g, b, 0, st, P); break;
+    case GT_U8: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U8, O32>), g, b, 0, st, P); break;
+    case GT_U16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_U16, O32>), g, b, 0, st, P); break;
+    case GT_I16: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_I16, O32>), g, b, 0, st, P); break;
+    default: hipLaunchKernelGGL((post_voxel_kernel<R, OP, GT_B8, O32>), g, b, 0, st, P); break;
   }
+}
+
+template <typename R, int OP>
+void launch_post_typed(const PostIn& P, dim3 g, dim3 b, hipStream_t st) {
+  if (P.img_bytes)
+    launch_post_typed<R, OP, true>(P, g, b, st);
+  else
+    launch_post_typed<R, OP, false>(P, g, b, st);
 }
 
 template <typename R>
 void launch_post_r(int op, const PostIn& P, dim3 g, dim3 b, hipStream_t st) {
   if (op == POST_INTERP)
-    hipLaunchKernelGGL((post_voxel_kernel<R, POST_INTERP, GT_F32>), g, b, 0, st, P);
+    hipLaunchKernelGGL((post_voxel_kernel<R, POST_INTERP, GT_F32, false>), g, b, 0, st, P);
   else if (op == POST_PUSH)
     launch_post_typed<R, POST_PUSH>(P, g, b, st);
   else
