@@ -28,8 +28,11 @@ def _host(d, shp):
     return L.to_host(d).reshape(shp, order='F')
 
 
-@pytest.mark.parametrize('shape', [(40, 9, 7), (33, 20, 17)])
-@pytest.mark.parametrize('sigma', [4.0, 2.0, 1.0])
+# (300, 70, 130): several 256-voxel row tiles along x and 64-position tiles
+# along y and z, with partial last tiles; sigma 9 (radius 36) takes the
+# per-voxel pass instead of the LDS tiles
+@pytest.mark.parametrize('shape', [(40, 9, 7), (33, 20, 17), (300, 70, 130)])
+@pytest.mark.parametrize('sigma', [4.0, 2.0, 1.0, 9.0])
 @pytest.mark.parametrize('mode', ['nearest', 'constant'])
 def test_gaussian_filter(gpu, shape, sigma, mode):
     from voxeldvc_amd.engine import correlate_gpu as C
@@ -38,6 +41,25 @@ def test_gaussian_filter(gpu, shape, sigma, mode):
     d, shp = _dev(x)
     got = _host(C._gauss_dev(d, shp, sigma, mode), shp)
     np.testing.assert_array_equal(got, ndimage.gaussian_filter(x, sigma, mode=mode, cval=0.0))
+
+
+@pytest.mark.parametrize('cval', [2.5, 0.1])
+def test_gaussian_filter_constant_cval(gpu, cval):
+    """'constant' extension with a cval float32 holds exactly (2.5: the LDS
+    tiles store it as float32) and one it does not (0.1: SciPy extends the
+    float64 line with the float64 cval, so the per-voxel pass runs)."""
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import correlate_gpu as C
+    shape = (70, 66, 67)
+    print('gaussian_filter cval', cval, flush=True)
+    x = _vol(3, shape)
+    d, shp = _dev(x)
+    w, r = C.gaussian_weights(3.0)
+    out = L.torch().empty_like(d)
+    L.call('vdvc_gaussian_filter', L.ctx(), shp[0], shp[1], shp[2], d, out,
+           w.ctypes.data_as(L._dp), r, 1, cval)
+    np.testing.assert_array_equal(_host(out, shp),
+                                  ndimage.gaussian_filter(x, 3.0, mode='constant', cval=cval))
 
 
 @pytest.mark.parametrize('sigma', [0.0, 2.0, 4.0])

@@ -111,6 +111,100 @@ __global__ __launch_bounds__(VDVC_BLOCK) void gauss_axis_kernel(GaussIn G,
   out[i] = __double2float_rn(acc);
 }
 
+// The same pass with the block's input lines staged in LDS.  A block loads
+// its tile of lines once, with the r-voxel halo on both sides already
+// extended as the mode says ('nearest': the end value; 'constant': cval,
+// which the launcher only allows here when float32 holds it exactly), then
+// forms every output of the tile from LDS in SciPy's order -- the same
+// operations on the same values as gauss_axis_kernel, without its 2r + 1
+// global loads per output.
+//   AXIS 0: one row segment of kGaussRowTile outputs per block, thread =
+//           output;
+//   AXIS 1, 2: kGaussColX x-columns times kGaussColT positions along the
+//           axis per block; lane = x-column, the block's four wavefronts
+//           take every fourth position, so each LDS row read is 64
+//           consecutive floats.
+// The 1-D grid runs along the filtered axis's tiles first and is spread so
+// that each XCD takes a contiguous run of it (neighbouring tiles' halos come
+// from that XCD's L2).
+constexpr int kGaussRowTile = 256;
+constexpr int kGaussColX = 64;
+constexpr int kGaussColT = 64;
+
+// sum of the symmetric taps of the output at LDS element c, lines `ld`
+// floats apart: the centre tap, then (t[c-k] + t[c+k]) * w_k for k = r..1
+__device__ __forceinline__ double gauss_lds_line(const GaussIn& G, const float* t, int c,
+                                                 int ld) {
+  double acc = mul_rn((double)t[c * ld], G.w[0]);
+  int k = G.r;
+  for (; k >= 4; k -= 4) {
+    const float* lo = t + (c - k) * ld;
+    const float* hi = t + (c + k) * ld;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc = add_rn(acc, mul_rn(add_rn((double)lo[j * ld], (double)hi[-j * ld]), G.w[k - j]));
+  }
+  for (; k >= 1; --k)
+    acc = add_rn(acc, mul_rn(add_rn((double)t[(c - k) * ld], (double)t[(c + k) * ld]), G.w[k]));
+  return acc;
+}
+
+template <int AXIS>
+__global__ __launch_bounds__(VDVC_BLOCK) void gauss_tile_kernel(GaussIn G,
+                                                                const float* __restrict__ in,
+                                                                float* __restrict__ out) {
+  extern __shared__ float tile[];
+  const int r = G.r;
+  const int tid = threadIdx.x;
+  const float cv = (float)G.cval;
+  unsigned L = xcd_remap(blockIdx.x, gridDim.x);
+  if constexpr (AXIS == 0) {
+    const unsigned nxt = (unsigned)(G.nx + kGaussRowTile - 1) / kGaussRowTile;
+    const unsigned xt = L % nxt;
+    L /= nxt;
+    const unsigned y = L % (unsigned)G.ny, z = L / (unsigned)G.ny;
+    const int x0 = (int)xt * kGaussRowTile;
+    const long long rowo = ((long long)z * G.ny + y) * G.nx;
+    const float* row = in + rowo;
+    for (int j = tid; j < kGaussRowTile + 2 * r; j += VDVC_BLOCK) {
+      const int q = x0 - r + j;
+      const int qc = min(max(q, 0), G.nx - 1);
+      const float v = row[qc];
+      tile[j] = (G.mode == 1 && qc != q) ? cv : v;
+    }
+    __syncthreads();
+    const int x = x0 + tid;
+    if (x < G.nx) out[rowo + x] = __double2float_rn(gauss_lds_line(G, tile, tid + r, 1));
+  } else {
+    const int len = AXIS == 1 ? G.ny : G.nz;
+    const unsigned nt = (unsigned)(len + kGaussColT - 1) / kGaussColT;
+    const unsigned nxc = (unsigned)(G.nx + kGaussColX - 1) / kGaussColX;
+    const unsigned t = L % nt;
+    L /= nt;
+    const unsigned xc = L % nxc, o = L / nxc;  // o: z (axis 1) or y (axis 2)
+    const long long stride = AXIS == 1 ? (long long)G.nx : (long long)G.nx * G.ny;
+    const long long obase = AXIS == 1 ? (long long)o * G.ny * G.nx : (long long)o * G.nx;
+    const int p0 = (int)t * kGaussColT;
+    const int lx = tid & 63, wv = tid >> 6;
+    const int x = (int)xc * kGaussColX + lx;
+    const bool xin = x < G.nx;
+    const int xs = xin ? x : G.nx - 1;  // columns past the volume load a valid voxel, unused
+    for (int j = wv; j < kGaussColT + 2 * r; j += VDVC_WAVES_PER_BLOCK) {
+      const int q = p0 - r + j;
+      const int qc = min(max(q, 0), len - 1);
+      const float v = in[obase + (long long)qc * stride + xs];
+      tile[j * kGaussColX + lx] = (G.mode == 1 && qc != q) ? cv : v;
+    }
+    __syncthreads();
+    if (xin) {
+      const int pend = min(kGaussColT, len - p0);
+      for (int jj = wv; jj < pend; jj += VDVC_WAVES_PER_BLOCK)
+        out[obase + (long long)(p0 + jj) * stride + x] =
+            __double2float_rn(gauss_lds_line(G, tile + lx, jj + r, kGaussColX));
+    }
+  }
+}
+
 // gv = g * valid, vf = valid, valid = (g != cval) in float32
 __global__ __launch_bounds__(VDVC_BLOCK) void mask_prep_kernel(long long n, const float* g,
                                                                float cval, float* gv, float* vf,
@@ -375,6 +469,30 @@ __global__ __launch_bounds__(VDVC_BLOCK) void overlap_proj_kernel(int nx, int ny
 namespace vdvc {
 
 int launch_gauss_axis(const GaussIn& G, const float* in, float* out, hipStream_t st) {
+  // the LDS-tiled pass for radii up to 32 (tiles of at most 32 KB) and an
+  // extension value float32 holds exactly
+  if (G.r <= 32 && (G.mode == 0 || (double)(float)G.cval == G.cval)) {
+    const unsigned long long nxt = (unsigned long long)(G.nx + kGaussRowTile - 1) / kGaussRowTile;
+    const unsigned long long nxc = (unsigned long long)(G.nx + kGaussColX - 1) / kGaussColX;
+    const int len = G.axis == 1 ? G.ny : G.nz;
+    const unsigned long long nt = (unsigned long long)(len + kGaussColT - 1) / kGaussColT;
+    const unsigned long long nb = G.axis == 0   ? nxt * G.ny * G.nz
+                                  : G.axis == 1 ? nt * nxc * G.nz
+                                                : nt * nxc * G.ny;
+    if (nb > 0x7fffffffull) return (int)hipErrorInvalidValue;
+    const size_t lds = G.axis == 0 ? (size_t)(kGaussRowTile + 2 * G.r) * sizeof(float)
+                                   : (size_t)(kGaussColT + 2 * G.r) * kGaussColX * sizeof(float);
+    if (G.axis == 0)
+      hipLaunchKernelGGL(gauss_tile_kernel<0>, dim3((unsigned)nb), dim3(VDVC_BLOCK), lds, st, G,
+                         in, out);
+    else if (G.axis == 1)
+      hipLaunchKernelGGL(gauss_tile_kernel<1>, dim3((unsigned)nb), dim3(VDVC_BLOCK), lds, st, G,
+                         in, out);
+    else
+      hipLaunchKernelGGL(gauss_tile_kernel<2>, dim3((unsigned)nb), dim3(VDVC_BLOCK), lds, st, G,
+                         in, out);
+    return (int)hipGetLastError();
+  }
   const unsigned long long nb =
       (unsigned long long)((G.nx + VDVC_BLOCK - 1) / VDVC_BLOCK) * G.ny * G.nz;
   // 32-bit tap offsets k * stride (k <= r) and a 32-bit 1-D grid
