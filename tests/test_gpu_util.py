@@ -127,6 +127,36 @@ def test_copy_volume_layouts(gpu, shape, src_order):
     assert np.array_equal(d.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize('dtype', [np.uint8, np.uint16, np.int16, np.float64, np.bool_])
+def test_image_and_flat_uploads_of_strided_tensors(gpu, dtype):
+    """Device tensors of the post-solve images in any strides, and strided
+    1-D / 2-D tensors, are laid out by the library's volume copy: the
+    reference's ravel(order='F') of the image in its own dtype, and C-order
+    flattening of the flat inputs."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 200, (9, 7, 5))
+    a = (a > 100) if dtype == np.bool_ else a.astype(dtype)
+    host = a.view(np.uint8) if dtype == np.bool_ else a
+    for src in (torch.from_numpy(np.ascontiguousarray(a)).cuda(),
+                torch.from_numpy(np.ascontiguousarray(a.transpose(2, 0, 1))).cuda()
+                .permute(1, 2, 0)):
+        assert tuple(src.shape) == a.shape
+        d, shp, code = L.to_dev_image(src)
+        assert shp == a.shape
+        got = d.cpu().numpy()
+        want = np.ravel(host, order='F')
+        assert np.array_equal(got.view(want.dtype) if got.dtype != want.dtype else got, want)
+    # strided flat inputs, flattened in C order
+    m = torch.from_numpy(rng.random((6, 10)).astype(np.float32)).cuda()
+    np.testing.assert_array_equal(L.to_dev_flat(m[:, ::3].t()).cpu().numpy(),
+                                  m.cpu().numpy()[:, ::3].T.ravel())
+    v = torch.from_numpy(np.arange(40, dtype=np.int16)).cuda()
+    d, code = L.to_dev_image_flat(v[::3])
+    assert np.array_equal(d.cpu().numpy(), np.arange(40, dtype=np.int16)[::3])
+
+
 def test_axpby_widen(gpu):
     from voxeldvc_amd import _lib as L
     rng = np.random.default_rng(3)

@@ -286,14 +286,34 @@ def _dev_tensor(a, fallback):
     return d
 
 
+def _gather_dense(d, code):
+    """Dense copy of a strided device tensor of up to three axes, axis 0
+    fastest, in its own element type (vdvc_copy_volume)."""
+    shp = list(d.shape) + [1] * (3 - d.dim())
+    st = list(d.stride()) + [0] * (3 - d.dim())
+    out = torch().empty(shp[0] * shp[1] * shp[2], dtype=d.dtype, device=d.device)
+    call('vdvc_copy_volume', ctx(), shp[0], shp[1], shp[2], d, code, st[0], st[1], st[2], out,
+         code, shp[0])
+    return out
+
+
+def _flat(d, code):
+    """A device tensor flattened in C order (a view when it is contiguous,
+    else a library copy; tensors of more than three axes are copied by
+    torch)."""
+    if d.is_contiguous():
+        return d.view(-1)
+    if d.dim() > 3:
+        return d.contiguous().view(-1)
+    return _gather_dense(d.permute(*reversed(range(d.dim()))), code)
+
+
 def to_dev_flat(a, dtype='float32'):
     """1-D array (numpy or tensor) -> contiguous device tensor of `dtype`
     (conversion on the device)."""
     if is_tensor(a):
         d = _dev_tensor(a, dtype)
-        if not d.is_contiguous():
-            d = d.contiguous()
-        return convert(d.view(-1), dtype)
+        return convert(_flat(d, type_code(d.dtype)), dtype)
     return convert(_host_upload(np.asarray(a).reshape(-1), dtype).view(-1), dtype)
 
 
@@ -344,11 +364,11 @@ def image_type(a):
 
 
 def _image_bits(d):
-    """Device storage of an image tensor: bool as uint8, uint16 as int16 bits
-    (the library reads the raw element with the type code)."""
+    """Device storage of an image tensor: bool as its uint8 bytes, uint16 as
+    int16 bits (the library reads the raw element with the type code)."""
     t = torch()
     if d.dtype == t.bool:
-        return d.to(t.uint8)
+        return d.view(t.uint8)
     if d.dtype == t.uint16:
         return d.view(t.int16)
     return d
@@ -371,7 +391,8 @@ def to_dev_image(a):
         if a.dim() != 3:
             raise ValueError('volume must be 3-D')
         d = _image_bits(a.detach().to(device='cuda'))
-        return d.permute(2, 1, 0).contiguous().view(-1), tuple(int(s) for s in a.shape), code
+        # x fastest: element (x, y, z) at x sx + y sy + z sz of the source
+        return _gather_dense(d, code), tuple(int(s) for s in a.shape), code
     arr = np.asarray(a)
     if arr.ndim != 3:
         raise ValueError('volume must be 3-D')
@@ -384,7 +405,7 @@ def to_dev_image_flat(a):
     code, _ = image_type(a)
     t = require_gpu()
     if is_tensor(a):
-        return _image_bits(a.detach().to(device='cuda')).contiguous().view(-1), code
+        return _flat(_image_bits(a.detach().to(device='cuda')), code), code
     host = _host_bits(np.ascontiguousarray(np.asarray(a).reshape(-1)))
     return t.from_numpy(host).to(device='cuda'), code
 
