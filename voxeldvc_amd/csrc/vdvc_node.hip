@@ -89,8 +89,12 @@ __device__ __forceinline__ void kop_diag(const MeshDev& M, const KOp<R>& op, int
 }
 
 // sum of the node partials written by the voxel kernels for node (i,j,k),
-// in fixed order (z tiles, then y rows, then x strips)
-template <typename R>
+// in fixed order (z tiles, then y rows, then x strips).  SINGLE: the level
+// has one x strip and one row part per element row (T.nsx == T.nys == 1), so
+// only strip 0, row part 0 of each (z tile, element row) can hold a partial
+// and the other slots' loads are not issued (the same sum: those slots are
+// absent and add nothing).
+template <typename R, bool SINGLE = false>
 __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev& T,
                                                 const R* __restrict__ P, int i, int j, int k,
                                                 R (&s)[3]) {
@@ -129,8 +133,9 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
   // is issued before the first sum, so their latencies overlap; absent slots
   // read P[0..2] and are dropped.  Summation order is fixed: z tiles, then
   // element rows (upper first), then strips and row parts.
-  const R* ptr[16];
-  bool ok[16];
+  constexpr int NC = SINGLE ? 1 : 4;  // (strip, row part) slots per (z tile, element row)
+  const R* ptr[4 * NC];
+  bool ok[4 * NC];
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
     const int zt = za + a;
@@ -143,32 +148,32 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
       const int ey = j - cy;
       const bool oky = okz && ey >= 0 && ey < M.Ny;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const int q = c >> 1, ys = c & 1;  // strip xa + q, row part ys
         const bool okc = oky && okx[q] && ys < T.nys;
         const long long w = (long long)(xa + q) +
                             (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
-        const int e = (a * 2 + b) * 4 + c;
+        const int e = (a * 2 + b) * NC + c;
         ok[e] = okc;
         ptr[e] = okc ? P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln[q]) * 3
                      : P;
       }
     }
   }
-  R val[16][3];
+  R val[4 * NC][3];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) {
+  for (int e = 0; e < 4 * NC; ++e) {
     val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
   }
   s[0] = s[1] = s[2] = R(0);
 #pragma unroll
-  for (int e = 0; e < 16; ++e)
+  for (int e = 0; e < 4 * NC; ++e)
     if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
 }
 
 // gather_partials with its loads issued in two batches (one z tile each, 8
 // slots): the same slots summed in the same order, half the values live
-template <typename R>
+template <typename R, bool SINGLE = false>
 __device__ __forceinline__ void gather_partials_lean(const MeshDev& M, const TilesDev& T,
                                                      const R* __restrict__ P, int i, int j,
                                                      int k, R (&s)[3]) {
@@ -207,32 +212,33 @@ __device__ __forceinline__ void gather_partials_lean(const MeshDev& M, const Til
     const int kb = zt * T.ez;
     const bool okz = zt <= zb && k >= kb && k <= min(M.Nz, kb + T.ez);
     const int kzl = k - kb;
-    const R* ptr[8];
-    bool ok[8];
+    constexpr int NC = SINGLE ? 1 : 4;
+    const R* ptr[2 * NC];
+    bool ok[2 * NC];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int cy = 1 - b;
       const int ey = j - cy;
       const bool oky = okz && ey >= 0 && ey < M.Ny;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const int q = c >> 1, ys = c & 1;
         const bool okc = oky && okx[q] && ys < T.nys;
         const long long w = (long long)(xa + q) +
                             (long long)T.nsx * (ys + (long long)T.nys * (ey + (long long)M.Ny * zt));
-        const int e = b * 4 + c;
+        const int e = b * NC + c;
         ok[e] = okc;
         ptr[e] = okc ? P + (size_t)w * T.wave_stride + ((size_t)(kzl * 2 + cy) * T.px + ln[q]) * 3
                      : P;
       }
     }
-    R val[8][3];
+    R val[2 * NC][3];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < 2 * NC; ++e) {
       val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
     }
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
+    for (int e = 0; e < 2 * NC; ++e)
       if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
   }
 }
@@ -257,7 +263,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R
     R acc[3] = {R(0), R(0), R(0)};
     if (op.P) {
       R s[3];
-      gather_partials(M, op.T, op.P, i, j, k, s);
+      if (op.T.nsx == 1 && op.T.nys == 1)
+        gather_partials<R, true>(M, op.T, op.P, i, j, k, s);
+      else
+        gather_partials<R, false>(M, op.T, op.P, i, j, k, s);
       acc[0] = op.cH * s[0]; acc[1] = op.cH * s[1]; acc[2] = op.cH * s[2];
     }
     if (op.addin) {
@@ -310,7 +319,10 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_gather_kernel(MeshDev M, Node
     int i, j, k;
     node_ijk(M, (unsigned)n, i, j, k);
     R sp[3];
-    gather_partials_lean(M, op.T, op.P, i, j, k, sp);
+    if (op.T.nsx == 1 && op.T.nys == 1)
+      gather_partials_lean<R, true>(M, op.T, op.P, i, j, k, sp);
+    else
+      gather_partials_lean<R, false>(M, op.T, op.P, i, j, k, sp);
     const R acc[3] = {op.cH * sp[0], op.cH * sp[1], op.cH * sp[2]};
     op.out[3 * n] = acc[0];
     op.out[3 * n + 1] = acc[1];
