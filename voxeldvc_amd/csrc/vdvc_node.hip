@@ -400,19 +400,36 @@ __global__ __launch_bounds__(VDVC_BLOCK) void jacobi_kernel(const R* __restrict_
 // PCG (x0 = 0) with FP64 dot products; state lives in device memory
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int n, int stride,
-                                            int off, double* lds) {
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += VDVC_BLOCK) s += part[(size_t)q * stride + off];
-  lds[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = VDVC_BLOCK / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
-    __syncthreads();
+// Fixed-order sums of NV interleaved partials, out[t] = sum over q < n of
+// part[q * NV + t]: a strided sum per thread, a butterfly across the
+// wavefront (every lane ends with the same value), then the block's
+// wavefronts in order.  Every thread of every block computes the same sums
+// in the same order, so all blocks take the same step length.
+template <int NV>
+__device__ __forceinline__ void sum_parts(const double* __restrict__ part, int n,
+                                          double (&out)[NV], double* lds /* >= 4 * NV */) {
+  double s[NV];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) s[t] = 0.0;
+  for (int q = threadIdx.x; q < n; q += VDVC_BLOCK) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) s[t] += part[(size_t)q * NV + t];
   }
-  const double r = lds[0];
+#pragma unroll
+  for (int t = 0; t < NV; ++t) s[t] = wave_sum(s[t]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) lds[(threadIdx.x >> 6) * NV + t] = s[t];
+  }
   __syncthreads();
-  return r;
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < VDVC_WAVES_PER_BLOCK; ++w) r += lds[w * NV + t];
+    out[t] = r;
+  }
+  __syncthreads();  // lds is free again
 }
 
 template <typename R>
@@ -442,8 +459,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_init_state_kernel(const double
                                                                     int nparts,
                                                                     PcgState* __restrict__ S) {
   __shared__ double lds[VDVC_BLOCK];
-  const double rz = sum_parts(part, nparts, 2, 0, lds);
-  const double bb = sum_parts(part, nparts, 2, 1, lds);
+  double rzbb[2];
+  sum_parts<2>(part, nparts, rzbb, lds);
+  const double rz = rzbb[0], bb = rzbb[1];
   if (threadIdx.x == 0) {
     S->rz[0] = rz; S->rz[1] = 0.0;
     S->r0 = bb + 1e-30;
@@ -461,7 +479,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_kernel(
     R* __restrict__ z, double* __restrict__ part) {
   __shared__ double lds[VDVC_BLOCK];
   if (S->done) return;
-  const double pAp = sum_parts(pap_part, npap, 1, 0, lds);
+  double pAp1[1];
+  sum_parts<1>(pap_part, npap, pAp1, lds);
+  const double pAp = pAp1[0];
   const double alpha = S->rz[it & 1] / (pAp + 1e-30);
   const R af = (R)alpha;
   double v[2] = {0.0, 0.0};
@@ -488,8 +508,9 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
     PcgState* __restrict__ S, const R* __restrict__ z, R* __restrict__ p) {
   __shared__ double lds[VDVC_BLOCK];
   if (S->done) return;
-  const double rr = sum_parts(part, nparts, 2, 0, lds);
-  const double rzn = sum_parts(part, nparts, 2, 1, lds);
+  double rrz[2];
+  sum_parts<2>(part, nparts, rrz, lds);
+  const double rr = rrz[0], rzn = rrz[1];
   const double res = sqrt(rr / S->r0);
   const bool conv = res < tol;
   if (conv) {
