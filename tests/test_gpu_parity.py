@@ -265,10 +265,17 @@ def test_affine_cost(gpu):
     assert abs(C._zncc_cost(list(out)) - ref) < 1e-6
 
 
-def test_pcg_matches_oracle(gpu):
-    """Device-resident PCG on (H + ll L) against the oracle's PCG."""
+@pytest.mark.parametrize('shape', [(4, 3, 3, 4), (9, 7, 5, 2)])
+@pytest.mark.parametrize('aligned', [True, False])
+def test_pcg_matches_oracle(gpu, shape, aligned):
+    """Device-resident PCG on (H + ll L) against the oracle's PCG.  With the
+    solution and M_inv vectors 16-byte aligned the vector updates take the
+    vectorised kernels (16-byte chunks; (9, 7, 5, 2) has Ndof = 1440, a
+    partial last chunk per thread and several blocks); at an offset of one
+    float they take the grid-stride kernels."""
     from voxeldvc_amd import _lib as L
-    Nx, Ny, Nz, h = 4, 3, 3, 4
+    Nx, Ny, Nz, h = shape
+    print('pcg', shape, 'aligned' if aligned else 'offset', flush=True)
     g, m, b = rand_problem(10, Nx, Ny, Nz, h)
     Klap = O.build_K_ref_laplacian()
     ll = 0.7
@@ -283,14 +290,15 @@ def test_pcg_matches_oracle(gpu):
     gx, gy, gz = Kd.grad_planes(g, shp)
     md = Kd._mask_u8(m, shp)
     diag = L.empty(b.size)
-    minv = L.empty(b.size)
+    off = 0 if aligned else 1
+    minv = L.empty(b.size + off)[off:]
     Kp = np.ascontiguousarray(Klap)
     L.call('vdvc_jacobi', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32,
            L.ptr(md), L.REG_K, Kp.ctypes.data_as(L._dp), None, float(h), 0.0, ll, None,
            L.ptr(diag), L.ptr(minv))
     assert rel(L.to_host(minv), Minv) < 1e-5
     bd = L.to_dev_flat(b)
-    x = L.empty(b.size)
+    x = L.empty(b.size + off)[off:]
     it = (L._i * 1)()
     rr = (L._d * 1)()
     L.call('vdvc_pcg', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.F32, L.ptr(md),

@@ -413,14 +413,31 @@ NodeOp<R> node_op0() {
   return op;
 }
 
+// element-matrix operand of a node kernel: the device copy, and for a
+// block-diagonal matrix its 8 x 8 scalar form as launch arguments (from the
+// host copy the device copy was converted from, so the values are the same)
 template <typename R>
-KOp<R> kop(const Buf& b, int blockdiag, double E, const R* Earr) {
+KOp<R> kop(const Buf& b, const std::vector<double>& hk, int blockdiag, double E,
+           const R* Earr) {
   KOp<R> k;
+  std::memset(&k, 0, sizeof(k));
   k.K = b.as<const R>();
   k.blockdiag = blockdiag;
   k.E = (R)E;
   k.Earr = Earr;
+  if (blockdiag && hk.size() == 576)
+    for (int q = 0; q < 64; ++q) k.kc[q] = (R)hk[(3 * (q >> 3)) * 24 + 3 * (q & 7)];
   return k;
+}
+
+template <typename R>
+KOp<R> kop1(vdvc_ctx* ctx, int blockdiag, double E, const R* Earr) {
+  return kop<R>(ctx->K1[rix<R>()], ctx->k1_cache[rix<R>()], blockdiag, E, Earr);
+}
+
+template <typename R>
+KOp<R> kop2(vdvc_ctx* ctx, int blockdiag, double E, const R* Earr) {
+  return kop<R>(ctx->K2[rix<R>()], ctx->k2_cache[rix<R>()], blockdiag, E, Earr);
 }
 
 // out = Hpartials (if P) + ll * R v, with R per reg_kind; eq-gap uses a
@@ -435,7 +452,7 @@ int apply_reg_sum(vdvc_ctx* ctx, const MeshDev& M, const TilesDev* T, bool useP,
   const Buf& K1 = ctx->K1[rix<R>()];
   const Buf& K2 = ctx->K2[rix<R>()];
   if (reg_kind == VDVC_REG_K && ll != 0.0) {
-    op.k1 = kop<R>(K1, bd1, E, nullptr); op.v1 = v; op.c1 = (R)ll; op.k1_on = 1;
+    op.k1 = kop1<R>(ctx, bd1, E, nullptr); op.v1 = v; op.c1 = (R)ll; op.k1_on = 1;
   }
   if (reg_kind != VDVC_REG_EQGAP || ll == 0.0) {
     op.out = out; op.dotv = dotv; op.dot_part = dot_part;
@@ -450,15 +467,15 @@ int apply_reg_sum(vdvc_ctx* ctx, const MeshDev& M, const TilesDev* T, bool useP,
   VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
   NodeOp<R> o2 = node_op0<R>();
   o2.done = done;
-  o2.k1 = kop<R>(K1, bd1, E, nullptr); o2.v1 = v; o2.c1 = R(1); o2.k1_on = 1;
+  o2.k1 = kop1<R>(ctx, bd1, E, nullptr); o2.v1 = v; o2.c1 = R(1); o2.k1_on = 1;
   o2.interior = 1;
   o2.out = w;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
   NodeOp<R> o3 = node_op0<R>();
   o3.done = done;
   o3.addin = out; o3.cadd = R(1);
-  o3.k1 = kop<R>(K1, bd1, E, nullptr); o3.v1 = w; o3.c1 = (R)ll; o3.k1_on = 1;
-  o3.k2 = kop<R>(K2, bd2, E, nullptr); o3.v2 = v; o3.c2 = (R)(ll * eps_reg); o3.k2_on = 1;
+  o3.k1 = kop1<R>(ctx, bd1, E, nullptr); o3.v1 = w; o3.c1 = (R)ll; o3.k1_on = 1;
+  o3.k2 = kop2<R>(ctx, bd2, E, nullptr); o3.v2 = v; o3.c2 = (R)(ll * eps_reg); o3.k2_on = 1;
   o3.out = out; o3.dotv = dotv; o3.dot_part = dot_part;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o3, ctx->stream));
   return 0;
@@ -547,7 +564,7 @@ int matvec_K_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_K, double E, 
   int bd = 0, rc;
   if ((rc = upload_K1<R>(ctx, h_K, &bd))) return rc;
   NodeOp<R> op = node_op0<R>();
-  op.k1 = kop<R>(ctx->K1[rix<R>()], bd, E, d_E); op.v1 = v; op.c1 = R(1); op.k1_on = 1;
+  op.k1 = kop1<R>(ctx, bd, E, d_E); op.v1 = v; op.c1 = R(1); op.k1_on = 1;
   op.out = out;
   VDVC_LAUNCH(vdvc::launch_node_op(M, op, ctx->stream));
   return 0;
@@ -560,7 +577,7 @@ int diag_K_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_K, double E, co
   if ((rc = upload_K1<R>(ctx, h_K, &bd))) return rc;
   DiagOp<R> op;
   std::memset(&op, 0, sizeof(op));
-  op.k1 = kop<R>(ctx->K1[rix<R>()], bd, E, d_E); op.c1 = R(1); op.k1_on = 1;
+  op.k1 = kop1<R>(ctx, bd, E, d_E); op.c1 = R(1); op.k1_on = 1;
   op.out = out;
   VDVC_LAUNCH(vdvc::launch_diag_op(M, op, ctx->stream));
   return 0;
@@ -575,11 +592,11 @@ int eqgap_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_Kel, double E, c
   R* w = ctx->nodeC.as<R>();
   const Buf& K1 = ctx->K1[rix<R>()];
   NodeOp<R> o1 = node_op0<R>();
-  o1.k1 = kop<R>(K1, bd, E, d_E); o1.v1 = v; o1.c1 = R(1); o1.k1_on = 1;
+  o1.k1 = kop1<R>(ctx, bd, E, d_E); o1.v1 = v; o1.c1 = R(1); o1.k1_on = 1;
   o1.interior = 1; o1.out = w;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o1, ctx->stream));
   NodeOp<R> o2 = node_op0<R>();
-  o2.k1 = kop<R>(K1, bd, E, d_E); o2.v1 = w; o2.c1 = R(1); o2.k1_on = 1;
+  o2.k1 = kop1<R>(ctx, bd, E, d_E); o2.v1 = w; o2.c1 = R(1); o2.k1_on = 1;
   o2.out = out;
   VDVC_LAUNCH(vdvc::launch_node_op(M, o2, ctx->stream));
   return 0;
@@ -621,12 +638,12 @@ int jacobi_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, c
   std::memset(&op, 0, sizeof(op));
   op.P = ctx->P.as<const R>(); op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<R>();
   if (reg_kind == VDVC_REG_K) {
-    op.k1 = kop<R>(ctx->K1[rix<R>()], bd1, E, nullptr); op.c1 = (R)ll; op.k1_on = 1;
+    op.k1 = kop1<R>(ctx, bd1, E, nullptr); op.c1 = (R)ll; op.k1_on = 1;
   } else if (reg_kind == VDVC_REG_EQGAP) {
     // diag R = hutchinson(K1 P_i K1) + eps * diag(K2): K2 part here, the
     // Hutchinson part via d_extra
     if (!d_extra) return fail(-1, "REG_EQGAP needs the Hutchinson diagonal in d_extra");
-    op.k1 = kop<R>(ctx->K2[rix<R>()], bd2, E, nullptr); op.c1 = (R)(ll * eps_reg);
+    op.k1 = kop2<R>(ctx, bd2, E, nullptr); op.c1 = (R)(ll * eps_reg);
     op.k1_on = 1;
     op.extra = d_extra; op.cextra = (R)ll;
   }
@@ -674,6 +691,9 @@ int pcg_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, VoxI
   VDVC_LAUNCH(vdvc::launch_pcg_init(ndof, d_b, d_minv, d_x, r, z, p, ctx->dparts.as<double>(), S,
                                     ctx->stream));
   in.done = done;
+  // the library's own vectors are 256-byte aligned; the caller's x and M_inv
+  // decide whether the vectorised updates apply
+  const bool vec = vdvc::pcg_vec_ok(ndof) && (((uintptr_t)d_x | (uintptr_t)d_minv) & 15u) == 0;
   int it = 0;
   while (it < max_iter) {
     const int nrun = (max_iter - it) < check_every ? (max_iter - it) : check_every;
@@ -685,9 +705,9 @@ int pcg_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, VoxI
         return rc;
       VDVC_LAUNCH(vdvc::launch_pcg_update(ndof, it, ctx->dparts2.as<double>(), nnb, S, d_minv,
                                           (const R*)p, (const R*)Ap, d_x, r, z,
-                                          ctx->dparts.as<double>(), ctx->stream));
+                                          ctx->dparts.as<double>(), vec, ctx->stream));
       VDVC_LAUNCH(vdvc::launch_pcg_pupdate(ndof, it, tol, ctx->dparts.as<double>(), S,
-                                           (const R*)z, p, ctx->stream));
+                                           (const R*)z, p, vec, ctx->stream));
     }
     PcgState hs;
     if ((rc = fetch(ctx, S, sizeof(PcgState), &hs))) return rc;

@@ -161,6 +161,7 @@ struct KOp {
   int blockdiag;
   R E;                 // scalar element scale
   const R* Earr;       // optional per-element scale (Nx*Ny*Nz)
+  R kc[64];            // block-diagonal form: kc[a * 8 + b] = K[3a][3b] (launch argument)
 };
 
 template <typename R>
@@ -173,7 +174,7 @@ struct NodeOp {
   const R* addin; R cadd;          // out += cadd * addin (may alias out)
   int interior;                    // if set: result *= interior-node mask (eq-gap P_i)
   R* out;
-  const R* dotv;                   // partial dot(dotv, out) in FP64 (null: none)
+  const R* dotv;                   // partial dot(dotv, out) in FP64 (null: none; not out)
   double* dot_part;
   const int* done;                 // early-exit flag (PCG), may be null
 };
@@ -268,14 +269,17 @@ int launch_jacobi(const R* diag, const R* min_part, int nparts, long long ndof, 
 template <typename R>
 int launch_pcg_init(long long ndof, const R* b, const R* minv, R* x, R* r, R* z, R* p,
                     double* part, PcgState* S, hipStream_t st);
+// vec: the vectorised update kernels (every vector 16-byte aligned and
+// pcg_vec_ok(ndof)); both updates of one solve take the same form
+bool pcg_vec_ok(long long ndof);
 template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
                       const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
-                      hipStream_t st);
+                      bool vec, hipStream_t st);
 
 template <typename R>
 int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
-                       const R* z, R* p, hipStream_t st);
+                       const R* z, R* p, bool vec, hipStream_t st);
 template <typename R>
 int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t st);
 template <typename R>

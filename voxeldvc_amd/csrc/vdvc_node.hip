@@ -24,25 +24,62 @@ __device__ __forceinline__ constexpr int cdx(int a) { return (a == 1 || a == 2 |
 __device__ __forceinline__ constexpr int cdy(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0; }
 __device__ __forceinline__ constexpr int cdz(int a) { return a >= 4 ? 1 : 0; }
 
+// the three components of one node: one 12-byte (float) / 24-byte (double)
+// load, so a wavefront's load of 64 consecutive nodes is one instruction
+// instead of three strided ones
+template <typename R>
+struct Node3 {
+  R c[3];
+};
+template <typename R>
+__device__ __forceinline__ void ld3(const R* __restrict__ p, R& a, R& b, R& c) {
+  const Node3<R> t = *reinterpret_cast<const Node3<R>*>(p);
+  a = t.c[0]; b = t.c[1]; c = t.c[2];
+}
+template <typename R>
+__device__ __forceinline__ void st3(R* __restrict__ p, R a, R b, R c) {
+  Node3<R> t;
+  t.c[0] = a; t.c[1] = b; t.c[2] = c;
+  *reinterpret_cast<Node3<R>*>(p) = t;
+}
+
 // Element-stiffness operator at node (i,j,k): sum over the (up to 8)
 // adjacent elements e, in order, of E_e * sum over the element's nodes lb,
 // in order, of K[la][lb] v_lb (la the node's corner in e).  The 27
 // neighbouring node values are loaded once (zero outside the mesh; they are
 // only used by elements inside it) and both loops are unrolled, so every
-// neighbour index and stiffness offset is a compile-time constant.
-template <typename R>
-__device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp<R>& op,
+// neighbour index and stiffness offset is a compile-time constant.  Each
+// neighbour is one 3-component load at a 32-bit DOF offset (Ndof < 2^31 is
+// checked when a mesh is set up); a neighbour outside the mesh loads the
+// node itself and is replaced by zero, so no load is predicated.  The
+// element matrix has one of two compile-time forms KM: 1 = block-diagonal
+// with equal 3x3 blocks, read from the launch arguments (Ks = KOp::kc,
+// Ks[la * 8 + lb] = K[3 la][3 lb]); 2 = the full 24 x 24 matrix, read from
+// the block's LDS copy (stage_K).
+template <typename R, int KM>
+__device__ __forceinline__ void kop_apply(const MeshDev& M, const R* Ks, R E,
+                                          const R* __restrict__ Earr,
                                           const R* __restrict__ v, int i, int j, int k,
                                           R (&out)[3]) {
+  const unsigned sy = 3u * (unsigned)M.nnx, sz = sy * (unsigned)M.nny;
+  const unsigned o = 3u * (unsigned)i + sy * (unsigned)j + sz * (unsigned)k;
+  const bool xl = i > 0, xh = i + 1 < M.nnx, yl = j > 0, yh = j + 1 < M.nny, zl = k > 0,
+             zh = k + 1 < M.nnz;
   R V[27][3];
 #pragma unroll
   for (int m = 0; m < 27; ++m) {
-    const int ni = i + (m % 3) - 1, nj = j + (m / 3) % 3 - 1, nk = k + m / 9 - 1;
-    const bool in = ni >= 0 && nj >= 0 && nk >= 0 && ni < M.nnx && nj < M.nny && nk < M.nnz;
-    const long long nb = in ? node_index(M, ni, nj, nk) : 0;
-    V[m][0] = in ? v[3 * nb] : R(0);
-    V[m][1] = in ? v[3 * nb + 1] : R(0);
-    V[m][2] = in ? v[3 * nb + 2] : R(0);
+    const int dx = (m % 3) - 1, dy = (m / 3) % 3 - 1, dz = m / 9 - 1;
+    const bool in = (dx < 0 ? xl : dx > 0 ? xh : true) && (dy < 0 ? yl : dy > 0 ? yh : true) &&
+                    (dz < 0 ? zl : dz > 0 ? zh : true);
+    unsigned off = o;
+    if (dx) off = dx > 0 ? off + 3u : off - 3u;
+    if (dy) off = dy > 0 ? off + sy : off - sy;
+    if (dz) off = dz > 0 ? off + sz : off - sz;
+    R a, b, c;
+    ld3(v + (in ? off : o), a, b, c);
+    V[m][0] = in ? a : R(0);
+    V[m][1] = in ? b : R(0);
+    V[m][2] = in ? c : R(0);
   }
   out[0] = out[1] = out[2] = R(0);
 #pragma unroll
@@ -51,18 +88,17 @@ __device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp<R>& op,
     const int ei = i + ox, ej = j + oy, ek = k + oz;
     if (ei < 0 || ej < 0 || ek < 0 || ei >= M.Nx || ej >= M.Ny || ek >= M.Nz) continue;
     const int la = corner_of(-ox, -oy, -oz);
-    const R Ee = op.Earr ? op.Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)]
-                             : op.E;
+    const R Ee = Earr ? Earr[(long long)ei + (long long)M.Nx * (ej + (long long)M.Ny * ek)] : E;
     R acc[3] = {R(0), R(0), R(0)};
 #pragma unroll
     for (int lb = 0; lb < 8; ++lb) {
       const int m = (ox + cdx(lb) + 1) + 3 * (oy + cdy(lb) + 1) + 9 * (oz + cdz(lb) + 1);
       const R vb0 = V[m][0], vb1 = V[m][1], vb2 = V[m][2];
-      if (op.blockdiag) {
-        const R l = op.K[(3 * la) * 24 + 3 * lb];
+      if (KM == 1) {
+        const R l = Ks[la * 8 + lb];
         acc[0] += l * vb0; acc[1] += l * vb1; acc[2] += l * vb2;
       } else {
-        const R* Kr = op.K + (3 * la) * 24 + 3 * lb;
+        const R* Kr = Ks + (3 * la) * 24 + 3 * lb;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           acc[c] += Kr[c * 24 + 0] * vb0 + Kr[c * 24 + 1] * vb1 + Kr[c * 24 + 2] * vb2;
@@ -70,6 +106,20 @@ __device__ __forceinline__ void kop_apply(const MeshDev& M, const KOp<R>& op,
     }
     out[0] += Ee * acc[0]; out[1] += Ee * acc[1]; out[2] += Ee * acc[2];
   }
+}
+
+// size of the LDS copy of an element matrix in form KM (only the full form
+// KM = 2 is staged; 1 word otherwise)
+template <int KM>
+struct KsLen {
+  static constexpr int n = KM == 2 ? 576 : 1;
+};
+
+// the block's LDS copy of a full element matrix (form 2); the caller
+// synchronises the block before the first use
+template <typename R>
+__device__ __forceinline__ void stage_K(const R* __restrict__ K, R* Ks) {
+  for (int t = threadIdx.x; t < 576; t += VDVC_BLOCK) Ks[t] = K[t];
 }
 
 template <typename R>
@@ -162,9 +212,7 @@ __device__ __forceinline__ void gather_partials(const MeshDev& M, const TilesDev
   }
   R val[4 * NC][3];
 #pragma unroll
-  for (int e = 0; e < 4 * NC; ++e) {
-    val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
-  }
+  for (int e = 0; e < 4 * NC; ++e) ld3(ptr[e], val[e][0], val[e][1], val[e][2]);
   s[0] = s[1] = s[2] = R(0);
 #pragma unroll
   for (int e = 0; e < 4 * NC; ++e)
@@ -234,9 +282,7 @@ __device__ __forceinline__ void gather_partials_lean(const MeshDev& M, const Til
     }
     R val[2 * NC][3];
 #pragma unroll
-    for (int e = 0; e < 2 * NC; ++e) {
-      val[e][0] = ptr[e][0]; val[e][1] = ptr[e][1]; val[e][2] = ptr[e][2];
-    }
+    for (int e = 0; e < 2 * NC; ++e) ld3(ptr[e], val[e][0], val[e][1], val[e][2]);
 #pragma unroll
     for (int e = 0; e < 2 * NC; ++e)
       if (ok[e]) { s[0] += val[e][0]; s[1] += val[e][1]; s[2] += val[e][2]; }
@@ -251,15 +297,26 @@ __device__ __forceinline__ void node_ijk(const MeshDev& M, unsigned n, int& i, i
   j = (int)(r - (unsigned)k * (unsigned)M.nny);
 }
 
-template <typename R>
+// KM1 / KM2: compile-time forms of the two element matrices (0: the term
+// is off, 1: block-diagonal, 2: full; see kop_apply)
+template <typename R, int KM1, int KM2>
 __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R> op) {
   __shared__ double lds[4];
+  __shared__ R ks1[KsLen<KM1>::n];
+  __shared__ R ks2[KsLen<KM2>::n];
   if (op.done && *op.done) return;
+  if (KM1 == 2) stage_K(op.k1.K, ks1);
+  if (KM2 == 2) stage_K(op.k2.K, ks2);
+  if (KM1 == 2 || KM2 == 2) __syncthreads();
   const long long n = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
   double d = 0.0;
   if (n < M.Nnodes) {
     int i, j, k;
     node_ijk(M, (unsigned)n, i, j, k);
+    // the dot operand (never the output vector) is loaded first, so its
+    // latency overlaps the gather and the stencil
+    R d0 = R(0), d1 = R(0), d2 = R(0);
+    if (op.dot_part) ld3(op.dotv + 3 * n, d0, d1, d2);
     R acc[3] = {R(0), R(0), R(0)};
     if (op.P) {
       R s[3];
@@ -270,18 +327,20 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R
       acc[0] = op.cH * s[0]; acc[1] = op.cH * s[1]; acc[2] = op.cH * s[2];
     }
     if (op.addin) {
-      acc[0] += op.cadd * op.addin[3 * n];
-      acc[1] += op.cadd * op.addin[3 * n + 1];
-      acc[2] += op.cadd * op.addin[3 * n + 2];
+      R a0, a1, a2;
+      ld3(op.addin + 3 * n, a0, a1, a2);
+      acc[0] += op.cadd * a0;
+      acc[1] += op.cadd * a1;
+      acc[2] += op.cadd * a2;
     }
-    if (op.k1_on) {
+    if (KM1) {
       R t[3];
-      kop_apply(M, op.k1, op.v1, i, j, k, t);
+      kop_apply<R, KM1>(M, KM1 == 1 ? op.k1.kc : ks1, op.k1.E, op.k1.Earr, op.v1, i, j, k, t);
       acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
     }
-    if (op.k2_on) {
+    if (KM2) {
       R t[3];
-      kop_apply(M, op.k2, op.v2, i, j, k, t);
+      kop_apply<R, KM2>(M, KM2 == 1 ? op.k2.kc : ks2, op.k2.E, op.k2.Earr, op.v2, i, j, k, t);
       acc[0] += op.c2 * t[0]; acc[1] += op.c2 * t[1]; acc[2] += op.c2 * t[2];
     }
     if (op.interior) {
@@ -290,13 +349,8 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_op_kernel(MeshDev M, NodeOp<R
       const bool in = i > 0 && i < M.Nx && j > 0 && j < M.Ny && k > 0 && k < M.Nz;
       if (!in) { acc[0] = R(0); acc[1] = R(0); acc[2] = R(0); }
     }
-    op.out[3 * n] = acc[0];
-    op.out[3 * n + 1] = acc[1];
-    op.out[3 * n + 2] = acc[2];
-    if (op.dot_part) {
-      d = (double)op.dotv[3 * n] * acc[0] + (double)op.dotv[3 * n + 1] * acc[1] +
-          (double)op.dotv[3 * n + 2] * acc[2];
-    }
+    st3(op.out + 3 * n, acc[0], acc[1], acc[2]);
+    if (op.dot_part) d = (double)d0 * acc[0] + (double)d1 * acc[1] + (double)d2 * acc[2];
   }
   if (op.dot_part) {
     double v[1] = {d};
@@ -324,12 +378,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void node_gather_kernel(MeshDev M, Node
     else
       gather_partials_lean<R, false>(M, op.T, op.P, i, j, k, sp);
     const R acc[3] = {op.cH * sp[0], op.cH * sp[1], op.cH * sp[2]};
-    op.out[3 * n] = acc[0];
-    op.out[3 * n + 1] = acc[1];
-    op.out[3 * n + 2] = acc[2];
-    if (op.dot_part)
-      d = (double)op.dotv[3 * n] * acc[0] + (double)op.dotv[3 * n + 1] * acc[1] +
-          (double)op.dotv[3 * n + 2] * acc[2];
+    st3(op.out + 3 * n, acc[0], acc[1], acc[2]);
+    if (op.dot_part) {
+      R d0, d1, d2;
+      ld3(op.dotv + 3 * n, d0, d1, d2);
+      d = (double)d0 * acc[0] + (double)d1 * acc[1] + (double)d2 * acc[2];
+    }
   }
   if (op.dot_part) {
     double vv[1] = {d};
@@ -354,13 +408,13 @@ __global__ __launch_bounds__(VDVC_BLOCK) void diag_op_kernel(MeshDev M, DiagOp<R
       acc[0] += op.c1 * t[0]; acc[1] += op.c1 * t[1]; acc[2] += op.c1 * t[2];
     }
     if (op.extra) {
-      acc[0] += op.cextra * op.extra[3 * n];
-      acc[1] += op.cextra * op.extra[3 * n + 1];
-      acc[2] += op.cextra * op.extra[3 * n + 2];
+      R e0, e1, e2;
+      ld3(op.extra + 3 * n, e0, e1, e2);
+      acc[0] += op.cextra * e0;
+      acc[1] += op.cextra * e1;
+      acc[2] += op.cextra * e2;
     }
-    op.out[3 * n] = acc[0];
-    op.out[3 * n + 1] = acc[1];
-    op.out[3 * n + 2] = acc[2];
+    st3(op.out + 3 * n, acc[0], acc[1], acc[2]);
     mn = rmin(acc[0], rmin(acc[1], acc[2]));
   }
   if (op.min_part) {
@@ -411,9 +465,22 @@ __device__ __forceinline__ void sum_parts(const double* __restrict__ part, int n
   double s[NV];
 #pragma unroll
   for (int t = 0; t < NV; ++t) s[t] = 0.0;
-  for (int q = threadIdx.x; q < n; q += VDVC_BLOCK) {
+  // batches of U strided partials: every load of a batch is issued before
+  // the first add (absent ones read as 0.0, which leaves the sum unchanged),
+  // so a block of up to U * VDVC_BLOCK partials costs one memory round trip
+  constexpr int U = 8;
+  for (int q0 = threadIdx.x; q0 < n; q0 += U * VDVC_BLOCK) {
+    double tmp[U][NV];
 #pragma unroll
-    for (int t = 0; t < NV; ++t) s[t] += part[(size_t)q * NV + t];
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * VDVC_BLOCK;
+#pragma unroll
+      for (int t = 0; t < NV; ++t) tmp[u][t] = q < n ? part[(size_t)q * NV + t] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int t = 0; t < NV; ++t) s[t] += tmp[u][t];
   }
 #pragma unroll
   for (int t = 0; t < NV; ++t) s[t] = wave_sum(s[t]);
@@ -528,6 +595,150 @@ __global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_kernel(
   for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < ndof;
        q += (long long)gridDim.x * VDVC_BLOCK)
     p[q] = fma(bf, p[q], z[q]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->rz[(it + 1) & 1] = rzn;
+    S->relres = res;
+    S->iters = it + 1;
+  }
+}
+
+// Vectorised forms of the two PCG updates: a thread owns PCG_EPT entries as
+// 16-byte chunks (every pointer 16-byte aligned), chunk c of thread t at
+// entry b0 + (c * VDVC_BLOCK + t) * W (W entries per chunk, b0 the block's
+// first entry), so each load instruction of a wavefront reads 1 KiB
+// contiguously; it issues all of its loads before the block reduces the
+// previous kernel's partial dots, and the grid covers the vector once, so a
+// kernel costs one memory round trip instead of one per grid-stride step.
+// The same arithmetic per entry as the scalar kernels; the partial dots are
+// summed per thread in chunk and entry order, then per wavefront and block
+// in fixed order.
+constexpr int PCG_EPT = 8;
+
+template <typename R>
+struct PackW {
+  static constexpr int W = 16 / sizeof(R);     // entries per 16-byte chunk
+  static constexpr int NC = PCG_EPT / W;       // chunks per thread
+};
+
+template <typename R>
+struct alignas(16) Chunk {
+  R v[PackW<R>::W];
+};
+
+// first entry of chunk c of this thread
+template <typename R>
+__device__ __forceinline__ long long chunk_base(int c) {
+  return (long long)blockIdx.x * VDVC_BLOCK * PCG_EPT +
+         ((long long)c * VDVC_BLOCK + threadIdx.x) * PackW<R>::W;
+}
+
+template <typename R>
+__device__ __forceinline__ void ld_pack(const R* __restrict__ a, long long n, R (&o)[PCG_EPT]) {
+  constexpr int W = PackW<R>::W;
+#pragma unroll
+  for (int c = 0; c < PackW<R>::NC; ++c) {
+    const long long q0 = chunk_base<R>(c);
+    if (q0 + W <= n) {
+      const Chunk<R> t = *reinterpret_cast<const Chunk<R>*>(a + q0);
+#pragma unroll
+      for (int e = 0; e < W; ++e) o[c * W + e] = t.v[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < W; ++e) o[c * W + e] = q0 + e < n ? a[q0 + e] : R(0);
+    }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void st_pack(R* __restrict__ a, long long n, const R (&o)[PCG_EPT]) {
+  constexpr int W = PackW<R>::W;
+#pragma unroll
+  for (int c = 0; c < PackW<R>::NC; ++c) {
+    const long long q0 = chunk_base<R>(c);
+    if (q0 + W <= n) {
+      Chunk<R> t;
+#pragma unroll
+      for (int e = 0; e < W; ++e) t.v[e] = o[c * W + e];
+      *reinterpret_cast<Chunk<R>*>(a + q0) = t;
+    } else {
+#pragma unroll
+      for (int e = 0; e < W; ++e)
+        if (q0 + e < n) a[q0 + e] = o[c * W + e];
+    }
+  }
+}
+
+// is entry (chunk c, e) of this thread inside the vector
+template <typename R>
+__device__ __forceinline__ bool pack_in(long long n, int idx) {
+  return chunk_base<R>(idx / PackW<R>::W) + idx % PackW<R>::W < n;
+}
+
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_update_vec_kernel(
+    long long ndof, int it, const double* __restrict__ pap_part, int npap,
+    PcgState* __restrict__ S, const R* __restrict__ minv, const R* __restrict__ p,
+    const R* __restrict__ Ap, R* __restrict__ x, R* __restrict__ r,
+    R* __restrict__ z, double* __restrict__ part) {
+  __shared__ double lds[VDVC_BLOCK];
+  R pv[PCG_EPT], av[PCG_EPT], xv[PCG_EPT], rv[PCG_EPT], mv[PCG_EPT], zv[PCG_EPT];
+  ld_pack(p, ndof, pv);
+  ld_pack(Ap, ndof, av);
+  ld_pack(x, ndof, xv);
+  ld_pack(r, ndof, rv);
+  ld_pack(minv, ndof, mv);
+  if (S->done) return;
+  const double rz = S->rz[it & 1];
+  double pAp1[1];
+  sum_parts<1>(pap_part, npap, pAp1, lds);
+  const double alpha = rz / (pAp1[0] + 1e-30);
+  const R af = (R)alpha;
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int e = 0; e < PCG_EPT; ++e) {
+    xv[e] = xv[e] + af * pv[e];
+    rv[e] = rv[e] - af * av[e];
+    zv[e] = mv[e] * rv[e];
+    if (pack_in<R>(ndof, e)) {
+      v[0] += (double)rv[e] * rv[e];
+      v[1] += (double)rv[e] * zv[e];
+    }
+  }
+  st_pack(x, ndof, xv);
+  st_pack(r, ndof, rv);
+  st_pack(z, ndof, zv);
+  block_sum_f64<2>(v, lds);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void pcg_pupdate_vec_kernel(
+    long long ndof, int it, double tol, const double* __restrict__ part, int nparts,
+    PcgState* __restrict__ S, const R* __restrict__ z, R* __restrict__ p) {
+  __shared__ double lds[VDVC_BLOCK];
+  R zv[PCG_EPT], pv[PCG_EPT];
+  ld_pack(z, ndof, zv);
+  ld_pack(p, ndof, pv);
+  if (S->done) return;
+  const double r0 = S->r0, rzo = S->rz[it & 1];
+  double rrz[2];
+  sum_parts<2>(part, nparts, rrz, lds);
+  const double rr = rrz[0], rzn = rrz[1];
+  const double res = sqrt(rr / r0);
+  if (res < tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S->relres = res;
+      S->iters = it + 1;
+      __threadfence();
+      S->done = 1;
+    }
+    return;
+  }
+  const double beta = rzn / (rzo + 1e-30);
+  const R bf = (R)beta;
+#pragma unroll
+  for (int e = 0; e < PCG_EPT; ++e) pv[e] = fma(bf, pv[e], zv[e]);
+  st_pack(p, ndof, pv);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S->rz[(it + 1) & 1] = rzn;
     S->relres = res;
@@ -666,8 +877,26 @@ int launch_node_op(const MeshDev& M, const NodeOp<R>& op, hipStream_t st) {
   if (gather_only)
     hipLaunchKernelGGL(node_gather_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M,
                        op);
-  else
-    hipLaunchKernelGGL(node_op_kernel<R>, dim3(node_blocks(M)), dim3(VDVC_BLOCK), 0, st, M, op);
+  else {
+    const int km1 = op.k1_on ? (op.k1.blockdiag ? 1 : 2) : 0;
+    const int km2 = op.k2_on ? (op.k2.blockdiag ? 1 : 2) : 0;
+    const dim3 g(node_blocks(M)), b(VDVC_BLOCK);
+#define VDVC_NODE_OP(a, c)                                                     \
+  if (km1 == a && km2 == c) {                                                  \
+    hipLaunchKernelGGL((node_op_kernel<R, a, c>), g, b, 0, st, M, op);         \
+    return (int)hipGetLastError();                                             \
+  }
+    VDVC_NODE_OP(0, 0)
+    VDVC_NODE_OP(1, 0)
+    VDVC_NODE_OP(2, 0)
+    VDVC_NODE_OP(1, 1)
+    VDVC_NODE_OP(1, 2)
+    VDVC_NODE_OP(2, 1)
+    VDVC_NODE_OP(2, 2)
+#undef VDVC_NODE_OP
+    // a second matrix without a first is not formed by any entry point
+    return (int)hipErrorInvalidValue;
+  }
   return (int)hipGetLastError();
 }
 
@@ -697,21 +926,42 @@ int launch_pcg_init(long long ndof, const R* b, const R* minv, R* x, R* r, R* z,
   return (int)hipGetLastError();
 }
 
+// the vectorised PCG updates cover the vector with one grid; beyond 4096
+// blocks (ndof > 8.4M) the grid-stride kernels keep the partial count short
+bool pcg_vec_ok(long long ndof) {
+  return (ndof + (long long)VDVC_BLOCK * PCG_EPT - 1) / ((long long)VDVC_BLOCK * PCG_EPT) <= 4096;
+}
+
+int pcg_upd_blocks(long long ndof, bool vec) {
+  return vec ? (int)((ndof + (long long)VDVC_BLOCK * PCG_EPT - 1) /
+                     ((long long)VDVC_BLOCK * PCG_EPT))
+             : pcg_blocks(ndof);
+}
+
 template <typename R>
 int launch_pcg_update(long long ndof, int it, const double* pap_part, int npap, PcgState* S,
                       const R* minv, const R* p, const R* Ap, R* x, R* r, R* z, double* part,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(pcg_update_kernel<R>, dim3(pcg_blocks(ndof)), dim3(VDVC_BLOCK), 0, st, ndof,
-                     it, pap_part, npap, S, minv, p, Ap, x, r, z, part);
+                      bool vec, hipStream_t st) {
+  if (vec)
+    hipLaunchKernelGGL(pcg_update_vec_kernel<R>, dim3(pcg_upd_blocks(ndof, true)),
+                       dim3(VDVC_BLOCK), 0, st, ndof, it, pap_part, npap, S, minv, p, Ap, x, r,
+                       z, part);
+  else
+    hipLaunchKernelGGL(pcg_update_kernel<R>, dim3(pcg_blocks(ndof)), dim3(VDVC_BLOCK), 0, st,
+                       ndof, it, pap_part, npap, S, minv, p, Ap, x, r, z, part);
   return (int)hipGetLastError();
 }
 
 template <typename R>
 int launch_pcg_pupdate(long long ndof, int it, double tol, const double* part, PcgState* S,
-                       const R* z, R* p, hipStream_t st) {
-  const int nb = pcg_blocks(ndof);
-  hipLaunchKernelGGL(pcg_pupdate_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol,
-                     part, nb, S, z, p);
+                       const R* z, R* p, bool vec, hipStream_t st) {
+  const int nb = pcg_upd_blocks(ndof, vec);
+  if (vec)
+    hipLaunchKernelGGL(pcg_pupdate_vec_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it,
+                       tol, part, nb, S, z, p);
+  else
+    hipLaunchKernelGGL(pcg_pupdate_kernel<R>, dim3(nb), dim3(VDVC_BLOCK), 0, st, ndof, it, tol,
+                       part, nb, S, z, p);
   return (int)hipGetLastError();
 }
 
@@ -765,9 +1015,10 @@ int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hi
   template int launch_pcg_init<R>(long long, const R*, const R*, R*, R*, R*, R*, double*,    \
                                   PcgState*, hipStream_t);                                   \
   template int launch_pcg_update<R>(long long, int, const double*, int, PcgState*, const R*, \
-                                    const R*, const R*, R*, R*, R*, double*, hipStream_t);   \
+                                    const R*, const R*, R*, R*, R*, double*, bool,           \
+                                    hipStream_t);                                            \
   template int launch_pcg_pupdate<R>(long long, int, double, const double*, PcgState*,       \
-                                     const R*, R*, hipStream_t);                             \
+                                     const R*, R*, bool, hipStream_t);                       \
   template int launch_axpby<R>(long long, double, const R*, double, R*, hipStream_t);        \
   template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);      \
   template int launch_calib_mode<R>(int, int, int, int, R*, hipStream_t);
