@@ -34,11 +34,14 @@ def test_matvec_H(gpu, shape):
 
 @pytest.mark.parametrize('nys', ['1', '2'])
 @pytest.mark.parametrize('gt', ['f32', 'bf16'])
-@pytest.mark.parametrize('shape', [(9, 2, 3, 8), (8, 8, 8, 4), (3, 2, 2, 32), (65, 1, 1, 8)])
+@pytest.mark.parametrize('shape', [(9, 2, 3, 8), (8, 8, 8, 4), (3, 2, 2, 32), (65, 1, 1, 8),
+                                   (4, 64, 64, 2), (1, 64, 64, 8)])
 def test_matvec_H_tilings(gpu, monkeypatch, shape, gt, nys):
     """The two row splits of the vectorised kernel (VDVC_NYS tuning
     override), float32 and bfloat16 gradient planes, with and without a
-    separate mask."""
+    separate mask.  (4, 64, 64, 2) and (1, 64, 64, 8) give 4096 wavefronts
+    with one element layer each, so without the row split the plan takes
+    two layers per wavefront (bfloat16, or h <= 4)."""
     import torch
     from voxeldvc_amd import _lib as L
     Nx, Ny, Nz, h = shape
