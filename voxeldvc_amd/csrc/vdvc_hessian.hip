@@ -181,16 +181,17 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
 
 }  // namespace
 
-// Blocks per CU the register budget is fitted to: 4 (occupancy 4 waves per
-// SIMD, <= 128 VGPRs) for the products with 2-byte gradients, so a 513^3 h=8
-// grid's 4096 wavefronts run in one round; 3 for 4-byte gradients.  The
-// residual pass streams three more voxel arrays (f, sampled g, live mask)
-// through the row pipeline and is fitted to 2 (<= 256 VGPRs): at 4 its
-// registers spilled.
-// Paired-element lanes carry a third node column and are fitted to 3.
+// Blocks per CU the register budget is fitted to: 3 (occupancy 3 waves per
+// SIMD, <= 168 VGPRs) for the products and the diagonal, 2-byte or 4-byte
+// gradients, single or paired elements.  Fitted to 4 (<= 128 VGPRs) the
+// 2-byte build spilled 28 bytes per lane; with two element layers per
+// wavefront a 513^3 h = 8 grid is 2048 wavefronts, which occupancy 3 runs in
+// one round without the spill (product 155.7 -> 145 us, 1025^2 x 769 h = 16
+// 798 -> 787 us).  The residual pass streams three more voxel arrays (f,
+// sampled g, live mask) through the row pipeline and is fitted to 2 (<= 256
+// VGPRs): at 4 its registers spilled.
 #ifndef VDVC_VEC_MIN_BLOCKS
-#define VDVC_VEC_MIN_BLOCKS(MODE, GT, E) \
-  (MODE == MODE_RHS ? 2 : (E) > 1 ? 3 : sizeof(GT) == 2 ? 4 : 3)
+#define VDVC_VEC_MIN_BLOCKS(MODE, GT, E) (MODE == MODE_RHS ? 2 : 3)
 #endif
 
 template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V, int E>
@@ -352,7 +353,11 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
   // pipeline depth: 2-byte gradients are unpacked into new registers before
   // the reload, so two buffer sets keep two rows in flight; 4-byte values
   // are used in place and need a third set
-  constexpr int PIPE = sizeof(GT) == 2 ? 2 : 3;
+#ifndef VDVC_PIPE_2B
+#define VDVC_PIPE_2B 2
+#endif
+  constexpr int PIPE = sizeof(GT) == 2 ? VDVC_PIPE_2B : 3;
+  constexpr bool UNPACK_FIRST = sizeof(GT) == 2;
   RowBuf buf0, buf1, buf2;
   int cp = 0, ct = 0;  // prefetch cursor (plane, row), clamped to the tile's last row
   auto issue = [&](RowBuf& b) {
@@ -659,23 +664,27 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
     }
   } else {
     // three buffer sets: a row is processed straight from its set, which is
-    // then re-filled with the row three ahead
+    // then re-filled with the row three ahead (2-byte values: unpacked
+    // first, so the set is re-filled before the row is processed)
     for (int rr = 0; rr < nrows_tile; rr += 3) {
       RowVals cv;
       unpack_row(buf0, cv);
+      if (UNPACK_FIRST) issue(buf0);
       process(cv);
-      issue(buf0);
+      if (!UNPACK_FIRST) issue(buf0);
       advance();
       if (rr + 1 < nrows_tile) {
         unpack_row(buf1, cv);
+        if (UNPACK_FIRST) issue(buf1);
         process(cv);
-        issue(buf1);
+        if (!UNPACK_FIRST) issue(buf1);
         advance();
       }
       if (rr + 2 < nrows_tile) {
         unpack_row(buf2, cv);
+        if (UNPACK_FIRST) issue(buf2);
         process(cv);
-        issue(buf2);
+        if (!UNPACK_FIRST) issue(buf2);
         advance();
       }
     }
