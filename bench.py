@@ -53,6 +53,9 @@ def parse():
     p.add_argument('--summarize-trace', default=None, metavar='CSV',
                    help='print the per-kernel summary of the last --solve-only window of a '
                         'rocprofv3 kernel-trace CSV and exit (no GPU needed)')
+    p.add_argument('--pcg-levels', action='store_true',
+                   help='time device PCG iterations (fixed count) on the three pyramid levels '
+                        'of the 513^3, h=8 solve and exit: per-iteration cost of the PCG')
     p.add_argument('--configs', action='store_true',
                    help='also measure the other benchmark configurations (C2, C3 at h=4, C5); '
                         'several minutes')
@@ -750,6 +753,74 @@ def affine_cpu_baseline(n=97):
                       % (n, t, os.cpu_count())}
 
 
+def pcg_levels(iters=60, reps=5):
+    """Per-iteration time of the device PCG (vdvc_pcg with tol 0 and a fixed
+    iteration count, so every iteration runs) on the three pyramid levels of
+    the 513^3, h = 8 solve, all with 65^3 nodes: 129^3 voxels at h = 2 and
+    257^3 at h = 4 (float32 gradients + uint8 mask, the plain path's coarse
+    levels), 513^3 at h = 8 (bfloat16 folded: the learned path; float32 +
+    mask: the plain path's finest level).  Gradients and mask0 of 8-bit
+    speckle, Laplacian regularizer at ll = 0.5, Jacobi preconditioner.  HIP
+    events around whole vdvc_pcg calls, median of `reps`."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd import synth
+    from voxeldvc_amd.engine.geometry_dvc import build_K_ref_laplacian
+    K = np.ascontiguousarray(build_K_ref_laplacian(), dtype=np.float64)
+    c = L.ctx()
+    out = []
+    for n_el, h, store in ((64, 2, 'fp32'), (64, 4, 'fp32'), (64, 8, 'bf16'), (64, 8, 'fp32')):
+        Nv = n_el * h + 1
+        f = synth.speckle_torch((Nv, Nv, Nv), synth.blobs_for((Nv, Nv, Nv)), seed=5)
+        f = f.round().clamp_(0, 255)
+        f16 = L.pitch_dense(f, (Nv, Nv, Nv), 'float16')
+        del f
+        Np = f16.numel()
+        gt = L.BF16 if store == 'bf16' else L.F32
+        G = [L.empty(Np, 'bfloat16' if store == 'bf16' else 'float32') for _ in range(3)]
+        m = L.empty(Np, 'uint8')
+        L.call('vdvc_image_setup', c, n_el, n_el, n_el, h, L.ptr(f16), L.F16, 0.0,
+               *[L.ptr(q) for q in G], gt, L.ptr(m))
+        if store == 'bf16':
+            Gf = [L.empty(Np, 'bfloat16') for _ in range(3)]
+            L.call('vdvc_fold_grad', c, Np, *[L.ptr(q) for q in G], L.ptr(m),
+                   *[L.ptr(q) for q in Gf])
+            G, mk = Gf, None
+        else:
+            mk = m
+        del f16
+        ndof = 3 * (n_el + 1) ** 3
+        diag, minv, x = L.empty(ndof), L.empty(ndof), L.empty(ndof)
+        geo = (c, n_el, n_el, n_el, h, *[L.ptr(q) for q in G], gt, L.ptr(mk), L.REG_K,
+               K.ctypes.data_as(L._dp), None, float(h), 0.0, 0.5)
+        L.call('vdvc_jacobi', *geo, None, L.ptr(diag), L.ptr(minv))
+        b = torch.randn(ndof, device='cuda', dtype=torch.float32)
+        it_, rr_ = (L._i * 1)(), (L._d * 1)()
+
+        def once():
+            L.call('vdvc_pcg', *geo, L.ptr(b), L.ptr(minv), L.ptr(x), 0.0, iters, iters, it_, rr_)
+
+        once()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            once()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e3 / iters)
+        out.append({"voxels": "%d^3" % Nv, "h": h, "storage": store, "iters": int(it_[0]),
+                    "us_per_iteration": round(float(np.median(ts)), 2)})
+        print('pcg %d^3 h=%d %s: %.2f us/iteration' % (Nv, h, store, out[-1]["us_per_iteration"]),
+              file=sys.stderr, flush=True)
+        del G, m, diag, minv, x, b
+    return {"pcg_levels": out, "nodes": "65^3", "regularizer": "laplacian, ll=0.5",
+            "timing": "HIP events around vdvc_pcg (tol 0, %d iterations), median of %d"
+                      % (iters, reps)}
+
+
 def stream_ceiling(reps=10):
     """The measured HBM streaming ceiling on this GPU: the library's read
     probe (vdvc_stream_read: 16-byte nontemporal loads, four in flight per
@@ -878,6 +949,9 @@ def main():
     assert torch.cuda.is_available(), 'bench needs a GPU'
     if a.solve_only:
         solve_only(a.solve_only, n=a.size, h=a.h)
+        return
+    if a.pcg_levels:
+        print(json.dumps(pcg_levels()), flush=True)
         return
     n_el = (a.size - 1) // a.h
     h = a.h
