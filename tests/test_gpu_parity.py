@@ -312,6 +312,66 @@ def test_pcg_matches_oracle(gpu, shape, aligned):
     assert rel(L.to_host(x), xr) < 1e-4
 
 
+@pytest.mark.parametrize('fuse', ['voxel', 'node'])
+@pytest.mark.parametrize('case', [((9, 7, 5, 2), 'f32', '0'), ((64, 2, 3, 2), 'f32', '0'),
+                                  ((6, 5, 4, 8), 'bf16', '0'),
+                                  ((65, 2, 3, 8), 'bf16', '0'), ((3, 4, 3, 8), 'bf16', '2')])
+def test_apply_A_laplacian_term_in_voxel_kernel(gpu, monkeypatch, case, fuse):
+    """H + ll L with the Laplacian term added by the voxel kernel element by
+    element (float32 lanes of <= 2 voxels, 2-byte gradients) against the
+    oracle, and the same operator and PCG with the term in the node pass
+    (VDVC_NO_REG_FUSE=1).  (64, 2, 3, 2) takes float32 lanes of 2 voxels (the
+    coarse pyramid level's kernel); (65, 2, 3, 8) takes the 8-voxel lanes of two
+    strips; (3, 4, 3, 8) with VDVC_NYS=2 splits each element row's rows
+    between two wavefronts (the term is added by the first part only); the
+    bf16 grids of 6 and 3 elements narrow the lanes (several lanes per
+    element: the term is added by the first lane of each group)."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    (Nx, Ny, Nz, h), gt, nys = case
+    print('apply_A laplacian term', case, fuse, flush=True)
+    if nys != '0':
+        monkeypatch.setenv('VDVC_NYS', nys)
+    if fuse == 'node':
+        monkeypatch.setenv('VDVC_NO_REG_FUSE', '1')
+    g, m, v = rand_problem(21, Nx, Ny, Nz, h)
+    shp = (Nx * h + 1, Ny * h + 1, Nz * h + 1)
+    if gt == 'bf16':
+        gq = torch.from_numpy(g * m[:, None]).to(torch.bfloat16)    # mask folded in
+        g64 = gq.to(torch.float64).numpy()
+        m64 = np.ones_like(m)
+        planes = [L.to_dev_level(gq[:, c].contiguous(), shp, 'bfloat16') for c in range(3)]
+        md, code = None, L.BF16
+    else:
+        g64, m64 = g.astype(np.float64), m
+        planes = [L.to_dev_level(np.ascontiguousarray(g[:, c]), shp, 'float32') for c in range(3)]
+        md, code = L.to_dev_level(m, shp, 'uint8'), L.F32
+    Klap = O.build_K_ref_laplacian()
+    ll = 0.61
+    A = lambda x: (O.matvec_H(x, g64, m64, h, Nx, Ny, Nz)
+                   + ll * O.matvec_K_ref(x, Klap, h, Nx, Ny, Nz))
+    Kp = np.ascontiguousarray(Klap)
+    kp = Kp.ctypes.data_as(L._dp)
+    vd = L.to_dev_flat(v)
+    out = L.empty(v.size)
+    geo = (L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in planes], code, L.ptr(md), L.REG_K, kp,
+           None, float(h), 0.0, ll)
+    L.call('vdvc_apply_A', *geo, L.ptr(vd), L.ptr(out))
+    assert rel(L.to_host(out), A(v.astype(np.float64))) < 2e-5
+    d = (O.build_diagonal_H(g64, m64, h, Nx, Ny, Nz)
+         + ll * O.build_diagonal_K_ref(Klap, h, Nx, Ny, Nz))
+    Minv = 1.0 / (d + 1e-5 * d.min())
+    xr, itr = O.pcg(A, v.astype(np.float64), Minv, max_iter=500, tol=1e-6)
+    diag, minv, x = L.empty(v.size), L.empty(v.size), L.empty(v.size)
+    L.call('vdvc_jacobi', *geo, None, L.ptr(diag), L.ptr(minv))
+    it = (L._i * 1)()
+    rr = (L._d * 1)()
+    L.call('vdvc_pcg', *geo, L.ptr(vd), L.ptr(minv), L.ptr(x), 1e-6, 500, 5, it, rr)
+    assert rr[0] < 1e-6
+    assert abs(it[0] - itr) <= 2
+    assert rel(L.to_host(x), xr) < 1e-4
+
+
 def _eqgap_problem():
     Nx, Ny, Nz, h = 3, 3, 2, 4
     g, m, v = rand_problem(11, Nx, Ny, Nz, h)

@@ -120,6 +120,10 @@ struct vdvc_ctx {
   Buf vtmp[4];              // dense float32 volume temporaries (affine stage)
   double* hpin = nullptr;   // pinned host staging (VDVC_HPIN doubles)
   std::vector<double> k1_cache[2], k2_cache[2];
+  // 8 x 8 float form of a block-diagonal K1 for the voxel kernels, and the
+  // matrix it was formed from
+  Buf kc32;
+  std::vector<double> kc32_src;
   // optional per-launch timing of the voxel-streaming kernels (HIP events
   // recorded on the context stream around each launch)
   bool timing = false;
@@ -457,13 +461,12 @@ KOp<R> kop2(vdvc_ctx* ctx, int blockdiag, double E, const R* Earr) {
 template <typename R>
 int apply_reg_sum(vdvc_ctx* ctx, const MeshDev& M, const TilesDev* T, bool useP, int reg_kind,
                   int bd1, int bd2, double E, double eps_reg, double ll, const R* v, R* out,
-                  const R* dotv, double* dot_part, const int* done) {
+                  const R* dotv, double* dot_part, const int* done, bool reg_fused = false) {
   NodeOp<R> op = node_op0<R>();
   if (useP) { op.P = ctx->P.as<const R>(); op.T = *T; }
   op.done = done;
-  const Buf& K1 = ctx->K1[rix<R>()];
-  const Buf& K2 = ctx->K2[rix<R>()];
-  if (reg_kind == VDVC_REG_K && ll != 0.0) {
+  // reg_fused: the voxel kernel already added ll R v to the partials
+  if (reg_kind == VDVC_REG_K && ll != 0.0 && !reg_fused) {
     op.k1 = kop1<R>(ctx, bd1, E, nullptr); op.v1 = v; op.c1 = (R)ll; op.k1_on = 1;
   }
   if (reg_kind != VDVC_REG_EQGAP || ll == 0.0) {
@@ -665,15 +668,50 @@ int jacobi_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, c
   return 0;
 }
 
+// The Laplacian term of a float32 product is added by the vectorised voxel
+// kernel itself (single-element lanes), element by element, as c E K (the
+// block-diagonal matrix's 8 x 8 form, c = ll): the node pass is then the
+// partials' gather alone.  Returns whether `in` now carries the term.
+template <typename R>
+int fuse_reg(vdvc_ctx* ctx, const TilesDev& T, int reg_kind, int bd1, double E, double ll,
+             VoxIn& in, bool* fused) {
+  *fused = false;
+  in.kreg = nullptr;
+  in.kc = 0.0f;
+  const std::vector<double>& hk = ctx->k1_cache[rix<R>()];
+  // the voxel kernels that carry the term: the vectorised kernel with
+  // single-element lanes (paired-element lanes would spill registers)
+  const bool kernel_has = T.vec && T.epl == 1;
+  if (sizeof(R) != 4 || !kernel_has || reg_kind != VDVC_REG_K || ll == 0.0 || !bd1 ||
+      hk.size() != 576 || getenv("VDVC_NO_REG_FUSE"))
+    return 0;
+  if (ctx->kc32_src != hk || !ctx->kc32.p) {
+    int rc = ctx->kc32.ensure(64 * sizeof(float));
+    if (rc) return rc;
+    float tmp[64];
+    for (int q = 0; q < 64; ++q) tmp[q] = (float)hk[(3 * (q >> 3)) * 24 + 3 * (q & 7)];
+    VDVC_HIP(hipMemcpyAsync(ctx->kc32.p, tmp, sizeof(tmp), hipMemcpyHostToDevice, ctx->stream));
+    VDVC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->kc32_src = hk;
+  }
+  in.kreg = ctx->kc32.as<const float>();
+  in.kc = (float)(ll * E);
+  *fused = true;
+  return 0;
+}
+
 template <typename R>
 int apply_A_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
                  int reg_kind, const double* h_K1, const double* h_K2, double E, double eps_reg,
                  double ll, const R* v, R* out) {
   int bd1 = 0, bd2 = 0, rc;
   if ((rc = prep_reg<R>(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
-  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, v))) return rc;
+  VoxIn vin = in;
+  bool fused = false;
+  if ((rc = fuse_reg<R>(ctx, T, reg_kind, bd1, E, ll, vin, &fused))) return rc;
+  if ((rc = run_restrict(ctx, MODE_MATVEC, gtype, M, T, vin, VDVC_F32, false, v))) return rc;
   return apply_reg_sum<R>(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, v, out, nullptr,
-                          nullptr, nullptr);
+                          nullptr, nullptr, fused);
 }
 
 template <typename R>
@@ -703,6 +741,8 @@ int pcg_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, VoxI
   VDVC_LAUNCH(vdvc::launch_pcg_init(ndof, d_b, d_minv, d_x, r, z, p, ctx->dparts.as<double>(), S,
                                     ctx->stream));
   in.done = done;
+  bool fused = false;
+  if ((rc = fuse_reg<R>(ctx, T, reg_kind, bd1, E, ll, in, &fused))) return rc;
   // the library's own vectors are 256-byte aligned; the caller's x and M_inv
   // decide whether the vectorised updates apply
   const bool vec = vdvc::pcg_vec_ok(ndof) && (((uintptr_t)d_x | (uintptr_t)d_minv) & 15u) == 0;
@@ -713,7 +753,7 @@ int pcg_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, VoxI
       if ((rc = launch_restrict_timed(ctx, MODE_MATVEC, gtype, M, T, in, VDVC_F32, false, p)))
         return rc;
       if ((rc = apply_reg_sum<R>(ctx, M, &T, true, reg_kind, bd1, bd2, E, eps_reg, ll, p, Ap, p,
-                                 ctx->dparts2.as<double>(), done)))
+                                 ctx->dparts2.as<double>(), done, fused)))
         return rc;
       VDVC_LAUNCH(vdvc::launch_pcg_update(ndof, it, ctx->dparts2.as<double>(), nnb, S, d_minv,
                                           (const R*)p, (const R*)Ap, d_x, r, z,

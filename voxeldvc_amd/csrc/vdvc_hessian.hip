@@ -406,8 +406,14 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
   // layer; plane set-up then reads LDS only, whose wait counter is separate
   // from the row pipeline's
   float* nlds = lds_nodes + wib * 64 * 12 * (E + 1);
-  auto element_setup = [&]() {
+  // the element-stiffness term is added once per element: by the first lane
+  // of its lane group, in the first row part, by an active lane
+  // (single-element lanes; the host plans it for those only)
+  constexpr bool KREG = MODE == MODE_MATVEC && E == 1;
+  const bool kadd = KREG && in.kreg != nullptr && act && (lane % G) == 0 && ys == 0;
+  auto element_setup = [&]() __attribute__((always_inline)) {
     if (MODE != MODE_MATVEC) return;
+    float nvr[E + 1][2][2][3];
 #pragma unroll
     for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
@@ -416,8 +422,35 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         for (int cz = 0; cz < 2; ++cz) {
           const long long n = node_index(M, ex + cx, ey + cy, ez + cz);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) nlds[((((cx * 2 + cy) * 2 + cz) * 3) + k) * 64 + lane] = v[3 * n + k];
+          for (int k = 0; k < 3; ++k) {
+            nvr[cx][cy][cz][k] = v[3 * n + k];
+            nlds[((((cx * 2 + cy) * 2 + cz) * 3) + k) * 64 + lane] = nvr[cx][cy][cz][k];
+          }
         }
+    if constexpr (KREG) {
+      if (kadd) {
+        // corner a = (dx, dy, dz) of the element (reference Hex8 order)
+        // receives kc sum_b kreg[a][b] v_b, its share of c E K v
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const int ax = (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0;
+          const int ay = (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0;
+          const int az = a >= 4 ? 1 : 0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+              const int bx = (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0;
+              const int by = (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0;
+              const int bz = b >= 4 ? 1 : 0;
+              acc += in.kreg[a * 8 + b] * nvr[bx][by][bz][k];
+            }
+            C[ax][ay][az][k] += in.kc * acc;
+          }
+        }
+      }
+    }
   };
   auto plane_setup = [&]() {
     const int z = ez * h + tz;

@@ -16,6 +16,12 @@ struct VoxIn {
   uint16_t* fold_out[3];     // optional bf16 gradient * mask out
   double* stats;             // per-wave partial statistics [nwaves][4]
   const int* done;           // early-exit flag (device-resident PCG), may be null
+  // products of the vectorised kernel: an element-stiffness term added to
+  // the node partials, kc * sum_b kreg[a * 8 + b] v_b with kreg[a * 8 + b] =
+  // K[3a][3b] of a block-diagonal element matrix (the Laplacian; device
+  // array of 64) and kc = c E, each element once (kreg null: none)
+  const float* kreg;
+  float kc;
 };
 
 struct SampleIn {
