@@ -768,7 +768,64 @@ __global__ __launch_bounds__(VDVC_BLOCK) void stats_kernel(MeshDev M, const FT* 
       (long long)blockIdx.x * VDVC_WAVES_PER_BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long r0 = vec ? wave0 : blockIdx.x;
   const long long rstep = vec ? (long long)gridDim.x * VDVC_WAVES_PER_BLOCK : gridDim.x;
-  for (long long row = r0; row < nrows; row += rstep) {
+  // vectorised rows of at most two 512-voxel chunks: the loads of kRB rows
+  // are issued before any is summed, then the voxels are summed in the
+  // order of the one-row loop below (row, chunk, voxel), so the sums are
+  // bit-identical to it and a wavefront waits on one memory round trip per
+  // kRB rows instead of one per chunk
+  constexpr int kRB = 4;
+  if (vec && M.Nvx <= 1024) {
+    for (long long rg = r0; rg < nrows; rg += kRB * rstep) {
+      R v[kRB][2][8];
+      uint2 mk[kRB][2];
+      bool ok[kRB][2];
+#pragma unroll
+      for (int q = 0; q < kRB; ++q) {
+        const long long row = rg + q * rstep;
+        const long long rb = (row < nrows ? row : 0) * M.pitch;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int x0 = 8 * lane + 512 * c;
+          ok[q][c] = row < nrows && x0 < M.Nvx;
+          mk[q][c] = make_uint2(0xffffffffu, 0xffffffffu);
+          if (ok[q][c]) {
+            ld8(fld + rb + x0, v[q][c]);
+            if (!use_thresh && mask) mk[q][c] = *reinterpret_cast<const uint2*>(mask + rb + x0);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kRB; ++q) {
+        const long long row = rg + q * rstep;
+        if (row >= nrows) break;
+        const int y = (int)(row % M.Nvy);
+        const int z = (int)(row / M.Nvy);
+        const int myz = axis_mult(y, M.h, M.Nvy - 1) * axis_mult(z, M.h, M.Nvz - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          if (!ok[q][c]) continue;
+          const int x0 = 8 * lane + 512 * c;
+          int rx = x0 % M.h;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int x = x0 + j;
+            const bool mb = ((j < 4 ? mk[q][c].x : mk[q][c].y) >> (8 * (j & 3))) & 0xffu;
+            const bool m = x < M.Nvx && (use_thresh ? (v[q][c][j] > glt) : mb);
+            if (m) {
+              const int mx = (x > 0 && x < M.Nvx - 1 && rx == 0) ? 2 : 1;
+              const double mlt = (double)(mx * myz);
+              const double d = (double)v[q][c][j];
+              acc[0] += mlt;
+              acc[1] += mlt * d;
+              acc[2] += mlt * d * d;
+            }
+            rx = rx + 1 == M.h ? 0 : rx + 1;
+          }
+        }
+      }
+    }
+  }
+  for (long long row = (vec && M.Nvx <= 1024) ? nrows : r0; row < nrows; row += rstep) {
     const int y = (int)(row % M.Nvy);
     const int z = (int)(row / M.Nvy);
     const int myz = axis_mult(y, M.h, M.Nvy - 1) * axis_mult(z, M.h, M.Nvz - 1);
