@@ -248,6 +248,17 @@ def test_stride_and_phase_corr(gpu):
     g2 = np.roll(f2, (-4, 5, 2), axis=(0, 1, 2))
     g2 = np.pad(g2, ((0, 3), (0, 0), (0, 2)))
     assert np.allclose(C.fft_rigid_shift(f2, g2), O.fft_rigid_shift(f2, g2))
+    # longer axes: the LDS-staged passes with 9, 3 and 4 outputs per thread
+    # and line counts that are not multiples of the 16-line groups (129 x 37
+    # x 50), and an axis over the staged pass's 227-sample limit (240, the
+    # per-output pass) beside short ones
+    for shp, d in (((129, 37, 50), (7, -3, 11)), ((240, 9, 7), (-17, 2, 3))):
+        print('phase correlation', shp, d, flush=True)
+        f3 = rng.uniform(0, 1, size=shp)
+        g3 = np.roll(f3, d, axis=(0, 1, 2))
+        got = C.fft_rigid_shift(f3, g3)
+        assert np.allclose(got, O.fft_rigid_shift(f3, g3))
+        assert np.allclose(got, d)
 
 
 def test_affine_cost(gpu):

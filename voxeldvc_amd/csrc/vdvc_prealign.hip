@@ -75,6 +75,97 @@ __global__ __launch_bounds__(VDVC_BLOCK) void dft_pass_kernel(const double2* __r
   dst[q] = make_double2(re, im);
 }
 
+// The same DFT pass over groups of DFT_LB lines staged in LDS.  Block b
+// takes lines li = DFT_LB b + l (l < DFT_LB) of the pass's line enumeration,
+// in which the fastest memory axis other than the transform axis varies
+// fastest, so that a group's loads and stores are contiguous runs.  LDS
+// holds the n twiddles and the lines as buf[t * DFT_LP + l] (a padded row
+// per t); thread (l = tid % DFT_LB, g = tid / DFT_LB) computes outputs
+// k = g + 16 j (j < KJ = ceil(n / 16); a k >= n is computed as k = 0 and
+// not stored) of line l: the sum over t in order, one fused multiply-add
+// per real product, the twiddle index m_j = k t mod n kept incrementally.
+// The outputs go back through the LDS buffer so that the stores are as
+// contiguous as the loads.  Each line is read from global memory once
+// (dft_pass_kernel reads it once per output sample).  REAL: real input
+// (float32 volume), imaginary part zero.
+constexpr int DFT_LB = 16;
+constexpr int DFT_LP = DFT_LB + 1;
+
+template <int KJ, bool REAL>
+__global__ __launch_bounds__(VDVC_BLOCK) void dft_lines_kernel(const double2* __restrict__ src,
+                                                               const float* __restrict__ src_real,
+                                                               double2* __restrict__ dst, int nx,
+                                                               int ny, int nz, int axis,
+                                                               double sign) {
+  extern __shared__ double2 sm[];
+  const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  double2* tw = sm;
+  double2* buf = sm + n;
+  for (int m = threadIdx.x; m < n; m += VDVC_BLOCK) {
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+    tw[m] = make_double2(cs, sn * sign);
+  }
+  const long long nlines = (long long)nx * ny * nz / n;
+  const long long li0 = (long long)blockIdx.x * DFT_LB;
+  const long long stride = axis == 0 ? 1 : (axis == 1 ? (long long)nx : (long long)nx * ny);
+  auto line_base = [&](long long li) -> long long {
+    if (axis == 0) return li * nx;                       // (y, z) -> row start
+    const long long x = li % nx, o = li / nx;            // o = z (axis 1) or y (axis 2)
+    return axis == 1 ? x + (long long)nx * ny * o : x + (long long)nx * o;
+  };
+  // load: along the row for axis 0 (t fastest), across lines otherwise
+  const int nq = n * DFT_LB;
+  for (int q = threadIdx.x; q < nq; q += VDVC_BLOCK) {
+    const int t = axis == 0 ? q % n : q / DFT_LB;
+    const int l = axis == 0 ? q / n : q % DFT_LB;
+    const long long li = li0 + l;
+    double2 a = make_double2(0.0, 0.0);
+    if (li < nlines) {
+      const long long at = line_base(li) + (long long)t * stride;
+      if (REAL) a.x = (double)src_real[at];
+      else a = src[at];
+    }
+    buf[t * DFT_LP + l] = a;
+  }
+  __syncthreads();
+  const int l = threadIdx.x % DFT_LB, g = threadIdx.x / DFT_LB;
+  double re[KJ], im[KJ];
+  int kk[KJ], mm[KJ];
+#pragma unroll
+  for (int j = 0; j < KJ; ++j) {
+    re[j] = 0.0; im[j] = 0.0;
+    kk[j] = g + 16 * j < n ? g + 16 * j : 0;
+    mm[j] = 0;
+  }
+  for (int t = 0; t < n; ++t) {
+    const double2 a = buf[t * DFT_LP + l];
+#pragma unroll
+    for (int j = 0; j < KJ; ++j) {
+      const double2 w = tw[mm[j]];
+      re[j] = fma(a.x, w.x, re[j]);
+      im[j] = fma(a.x, w.y, im[j]);
+      if (!REAL) {
+        re[j] = fma(-a.y, w.y, re[j]);
+        im[j] = fma(a.y, w.x, im[j]);
+      }
+      const int m = mm[j] + kk[j];
+      mm[j] = m >= n ? m - n : m;
+    }
+  }
+  __syncthreads();  // every thread has read the lines: the buffer takes the outputs
+#pragma unroll
+  for (int j = 0; j < KJ; ++j)
+    if (g + 16 * j < n) buf[(g + 16 * j) * DFT_LP + l] = make_double2(re[j], im[j]);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nq; q += VDVC_BLOCK) {
+    const int k = axis == 0 ? q % n : q / DFT_LB;
+    const int ll = axis == 0 ? q / n : q % DFT_LB;
+    const long long li = li0 + ll;
+    if (li < nlines) dst[line_base(li) + (long long)k * stride] = buf[k * DFT_LP + ll];
+  }
+}
+
 // R = F conj(G) / (|F conj(G)| + 1e-12), in place into F
 __global__ __launch_bounds__(VDVC_BLOCK) void cross_power_kernel(double2* __restrict__ F,
                                                                  const double2* __restrict__ G,
@@ -221,21 +312,51 @@ int launch_phase_corr(const float* f, const float* g, int nx, int ny, int nz, do
   double2* B = work + n;
   double2* C = work + 2 * n;
   dim3 g3(nb), b3(VDVC_BLOCK);
-  const size_t sx = (size_t)nx * sizeof(double2), sy = (size_t)ny * sizeof(double2),
-               sz = (size_t)nz * sizeof(double2);
+  // one DFT pass along `axis`: lines staged in LDS in groups of DFT_LB for
+  // axes whose lines and twiddles fit 64 KiB of LDS (<= 227 samples), else
+  // the per-output kernel
+  auto pass = [&](const double2* in, const float* in_real, double2* out, int axis,
+                  double sign) {
+    const int na = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    if ((size_t)na * (1 + DFT_LP) * sizeof(double2) <= 65536) {
+      const long long nlines = n / na;
+      const dim3 gl((unsigned)((nlines + DFT_LB - 1) / DFT_LB));
+      const size_t lds = (size_t)na * (1 + DFT_LP) * sizeof(double2);
+      const int kj = (na + 15) / 16;  // 1..15
+#define VDVC_DFT_L(KJ)                                                                        \
+  case KJ:                                                                                    \
+    if (in_real)                                                                              \
+      hipLaunchKernelGGL((dft_lines_kernel<KJ, true>), gl, b3, lds, st, in, in_real, out, nx, \
+                         ny, nz, axis, sign);                                                 \
+    else                                                                                      \
+      hipLaunchKernelGGL((dft_lines_kernel<KJ, false>), gl, b3, lds, st, in, in_real, out, nx,\
+                         ny, nz, axis, sign);                                                 \
+    break;
+      switch (kj) {
+        VDVC_DFT_L(1) VDVC_DFT_L(2) VDVC_DFT_L(3) VDVC_DFT_L(4) VDVC_DFT_L(5)
+        VDVC_DFT_L(6) VDVC_DFT_L(7) VDVC_DFT_L(8) VDVC_DFT_L(9) VDVC_DFT_L(10)
+        VDVC_DFT_L(11) VDVC_DFT_L(12) VDVC_DFT_L(13) VDVC_DFT_L(14) VDVC_DFT_L(15)
+        default: break;
+      }
+#undef VDVC_DFT_L
+    } else {
+      hipLaunchKernelGGL(dft_pass_kernel, g3, b3, (size_t)na * sizeof(double2), st, in, in_real,
+                         out, nx, ny, nz, axis, sign);
+    }
+  };
   // F = DFT(f): x (real) -> B, y -> C, z -> A
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, nullptr, f, B, nx, ny, nz, 0, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, A, nx, ny, nz, 2, -1.0);
+  pass(nullptr, f, B, 0, -1.0);
+  pass(B, nullptr, C, 1, -1.0);
+  pass(C, nullptr, A, 2, -1.0);
   // G = DFT(g) into C (via B)
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, nullptr, g, B, nx, ny, nz, 0, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, -1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, B, nx, ny, nz, 2, -1.0);
+  pass(nullptr, g, B, 0, -1.0);
+  pass(B, nullptr, C, 1, -1.0);
+  pass(C, nullptr, B, 2, -1.0);
   hipLaunchKernelGGL(cross_power_kernel, g3, b3, 0, st, A, B, n);
   // inverse DFT (positive sign; the 1/N scale does not move the argmax)
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sx, st, A, nullptr, B, nx, ny, nz, 0, 1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sy, st, B, nullptr, C, nx, ny, nz, 1, 1.0);
-  hipLaunchKernelGGL(dft_pass_kernel, g3, b3, sz, st, C, nullptr, A, nx, ny, nz, 2, 1.0);
+  pass(A, nullptr, B, 0, 1.0);
+  pass(B, nullptr, C, 1, 1.0);
+  pass(C, nullptr, A, 2, 1.0);
   const int np = (int)std::min<long long>(VDVC_ARG_PARTS, (n + VDVC_BLOCK - 1) / VDVC_BLOCK);
   hipLaunchKernelGGL(argmax_part_kernel, dim3((unsigned)np), b3, 0, st, A, nx, ny, nz,
                      (ArgBest*)part);
