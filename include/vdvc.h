@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 7
+#define VDVC_ABI_VERSION 8
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -127,7 +127,8 @@ int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, c
                 uint8_t* d_mask_out, double* h_stats3, int f32_weights);
 
 /* Residual res = ((g - gmean) * kscale + (f - mean0)) zeroed outside d_bmask
- * (working type, the reference's operation order), RHS b = sum W bmask res v_eq,
+ * (working type, the reference's operation order; stored into d_res in the
+ * level layout unless d_res is NULL), RHS b = sum W bmask res v_eq,
  * and (weighted count, sum, sum sq, plain voxel count) of res over d_live,
  * multiplicity-weighted except the last, into h_stats4 -- only when
  * h_stats4 is non-NULL (the reference forms them only to print them,
@@ -140,6 +141,18 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
                       const uint8_t* d_bmask, const uint8_t* d_live, void* d_res, void* d_b,
                       uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
                       double* h_stats4);
+
+/* The residual of vdvc_residual_rhs, (g - g_mean) * kscale + (f - mean0) in
+ * the working type rtype (the type of d_gvals) and zero outside d_bmask,
+ * formed from the same level-layout inputs with the same single-rounded
+ * steps (bit for bit the d_res of that call) and written in the dense layout
+ * x + Nvx (y + Nvy z) (d_res of Nvox entries).  d_res of vdvc_residual_rhs
+ * may be NULL: a GN loop then stores no residual per iteration and forms the
+ * returned one (the last iteration's) once with this call.  The residual of
+ * correlate_gpu.py:1092-1109 as the solve returns it (:1138). */
+int vdvc_residual_out(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
+                      double mean0, const void* d_gvals, int rtype, double gmean, double kscale,
+                      const uint8_t* d_bmask, void* d_res);
 
 /* out = H v, H = sum_e sum_q w_q m_q v_eq v_eq^T.  d_mask may be NULL (mask
  * folded into the gradients).  Replaces matvec_H (engine/kernels_dvc.py:452-506). */

@@ -168,6 +168,53 @@ def test_compute_b(gpu, shape):
     assert rel(got, ref) < 2e-5
 
 
+@pytest.mark.parametrize('form', ['f32', 'bf16', 'f64'])
+@pytest.mark.parametrize('shape', [(3, 2, 2, 8), (70, 2, 1, 8), (5, 3, 4, 3)])
+def test_residual_out_matches_rhs_residual(gpu, shape, form):
+    """vdvc_residual_out forms the residual of vdvc_residual_rhs bit for bit
+    (dense layout) from the same inputs, and vdvc_residual_rhs gives the same
+    b, bit for bit, whether or not it stores the residual: the float32
+    image + gradients, the mixed form (float16 image, bfloat16 gradients:
+    the vectorised kernel at h = 8, the generic one at h = 3) and FP64."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    Nx, Ny, Nz, h = shape
+    print('residual_out', shape, form, flush=True)
+    rng = np.random.default_rng(12)
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    nv = shp[0] * shp[1] * shp[2]
+    g, m, _ = rand_problem(13, Nx, Ny, Nz, h)
+    f = rng.uniform(0, 255, nv).round()
+    gv = rng.uniform(0, 255, nv)
+    rd, fd, gd = {'f32': ('float32', 'float32', 'float32'),
+                  'bf16': ('float32', 'float16', 'bfloat16'),
+                  'f64': ('float64', 'float64', 'float64')}[form]
+    rt = L.F64 if rd == 'float64' else L.F32
+    gx, gy, gz = Kd.grad_planes(g, shp, gd)
+    fimg = L.to_dev_level(f, shp, fd)
+    gvals = L.to_dev_level(gv, shp, rd)
+    bm = L.to_dev_level(m, shp, 'uint8')
+    ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    mean0, gmean, kscale = 97.25, 101.5, -1.0625
+    args = (L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.type_code(gd), L.ptr(fimg),
+            L.type_code(fd), mean0, L.ptr(gvals), gmean, kscale, L.ptr(bm), L.ptr(bm))
+    res_p = L.empty(fimg.numel(), rd)
+    b1, b2 = L.empty(ndof, rd), L.empty(ndof, rd)
+    L.call('vdvc_residual_rhs', *args, L.ptr(res_p), L.ptr(b1), None, None, None, None)
+    L.call('vdvc_residual_rhs', *args, None, L.ptr(b2), None, None, None, None)
+    assert torch.equal(b1, b2)
+    res_d = L.empty(nv, rd)
+    L.call('vdvc_residual_out', L.ctx(), Nx, Ny, Nz, h, L.ptr(fimg), L.type_code(fd), mean0,
+           L.ptr(gvals), rt, gmean, kscale, L.ptr(bm), L.ptr(res_d))
+    assert torch.equal(res_d, L.unpitch(res_p, shp))
+    # the reference's formula on the host (same single-rounded steps)
+    dt = np.float64 if rd == 'float64' else np.float32
+    fh = f.astype(np.float16).astype(dt) if fd == 'float16' else f.astype(dt)
+    ref = ((gv.astype(dt) - dt(gmean)) * dt(kscale) + (fh - dt(mean0))) * m
+    assert np.array_equal(L.to_host(res_d), ref.astype(dt))
+
+
 @pytest.mark.parametrize('shape', [(2, 2, 2, 1), (3, 4, 2, 1), (5, 5, 5, 1)])
 def test_matvec_K_ref(gpu, shape):
     from voxeldvc_amd.engine import kernels_dvc as K

@@ -926,7 +926,7 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   if (!gtype_ok(gtype) || !ftype_ok(ftype)) return fail(-1, "unsupported element type");
   if ((ftype == VDVC_F64) != (gtype == VDVC_F64))
     return fail(-1, "the FP64 path needs a float64 image and float64 gradients");
-  if (!d_gx || !d_gy || !d_gz || !d_f || !d_gvals || !d_bmask || !d_live || !d_res || !d_b)
+  if (!d_gx || !d_gy || !d_gz || !d_f || !d_gvals || !d_bmask || !d_live || !d_b)
     return fail(-1, "NULL buffer");
   const bool fold = d_fold_x != nullptr;
   if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
@@ -947,6 +947,22 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
                                                 static_cast<float*>(d_b), h_stats4),
                        residual_rhs_impl<double>(ctx, M, T, gtype, in, ftype, fold,
                                                  static_cast<double*>(d_b), h_stats4));
+}
+
+int vdvc_residual_out(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,
+                      double mean0, const void* d_gvals, int rtype, double gmean, double kscale,
+                      const uint8_t* d_bmask, void* d_res) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!d_f || !d_gvals || !d_bmask || !d_res) return fail(-1, "NULL buffer");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
+  if (rtype == VDVC_F64 ? ftype != VDVC_F64 : (ftype != VDVC_F32 && ftype != VDVC_F16))
+    return fail(-1, "image type %d does not match working type %d", ftype, rtype);
+  const MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  VDVC_LAUNCH(vdvc::launch_residual_out(rtype, ftype, M, d_f, d_gvals, d_bmask, mean0, gmean,
+                                        kscale, d_res, ctx->stream));
+  return 0;
 }
 
 int vdvc_matvec_H(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,

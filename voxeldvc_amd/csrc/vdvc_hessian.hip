@@ -573,7 +573,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         }
       }
       if (act) {
-        st_f32<V>(res_f + vi, xbu, resv);
+        // the residual is stored only when the caller keeps it
+        if (res_f) st_f32<V>(res_f + vi, xbu, resv);
         if (FOLD) {
           float fx[V], fy[V], fz[V];
 #pragma unroll
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         float rv = add_rn(mul_rn(sub_rn(gv, gmean_f), kscale_f),
                              sub_rn(fv, mean0_f));
         rv = tm ? rv : 0.0f;
-        res_f[ti] = rv;
+        if (res_f) res_f[ti] = rv;
         if (stats && in.live[ti]) {
           const double mlt = (double)(axis_mult(y, h, M.Nvy - 1) * mz);
           st_c += mlt;

@@ -247,6 +247,11 @@ int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold,
                         const TilesDev& T, const VoxIn& in, const float* v, float* P,
                         hipStream_t st);
 // generic voxel kernel; v and P are R = double for gtype GT_F64, else float
+// the dense residual (g - g_mean) kscale + (f - mean0) over the b-mask from
+// a residual pass's inputs (working type rtype, image type ftype)
+int launch_residual_out(int rtype, int ftype, const MeshDev& M, const void* f, const void* gv,
+                        const uint8_t* bm, double mean0, double gmean, double kscale, void* out,
+                        hipStream_t st);
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                     const TilesDev& T, const VoxIn& in, const void* v, void* P, hipStream_t st);
 // U and gvals are double for an F64 image, else float

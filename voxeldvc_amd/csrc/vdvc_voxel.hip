@@ -236,7 +236,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void vox_restrict_kernel(
             // steps in the reference's order, no contraction
             rv = add_rn(mul_rn(sub_rn(gv, gmean), kscale), sub_rn(fval, mean0));
             if (!bm) rv = R(0);
-            resv[vi] = rv;
+            if (resv) resv[vi] = rv;
             if (in.stats && in.live[vi]) {
               const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
               const double rd = (double)rv;
@@ -1063,7 +1063,50 @@ void launch_restrict_one(dim3 g, dim3 b, hipStream_t st, const MeshDev& M, const
                      static_cast<const R*>(v), static_cast<R*>(P));
 }
 
+// The residual of a residual pass formed again from the same inputs with
+// the same single-rounded steps, (g - g_mean) * kscale + (f - mean0) in the
+// working type R and zero outside the b-mask, written in the dense layout
+// x + Nvx (y + Nvy z).  The GN loop forms the residual every iteration
+// (engine/correlate_gpu.py:341-378) but returns only the last one, so the
+// residual passes do not store it and this pass writes the returned array
+// once, from the last pass's sampled values.
+template <typename R, typename FT>
+__global__ __launch_bounds__(VDVC_BLOCK) void residual_out_kernel(
+    MeshDev M, const FT* __restrict__ f, const R* __restrict__ gv,
+    const uint8_t* __restrict__ bm, double mean0_d, double gmean_d, double kscale_d,
+    R* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x;
+  if (q >= M.Nvox) return;
+  const long long r = q / M.Nvx;
+  const long long vi = r * M.pitch + (q - r * M.Nvx);
+  const R mean0 = (R)mean0_d, gmean = (R)gmean_d, kscale = (R)kscale_d;
+  const R fval = ld_val(f, vi);
+  R rv = add_rn(mul_rn(sub_rn(gv[vi], gmean), kscale), sub_rn(fval, mean0));
+  if (!bm[vi]) rv = R(0);
+  out[q] = rv;
+}
+
 namespace vdvc {
+
+int launch_residual_out(int rtype, int ftype, const MeshDev& M, const void* f, const void* gv,
+                        const uint8_t* bm, double mean0, double gmean, double kscale, void* out,
+                        hipStream_t st) {
+  const dim3 g((unsigned)((M.Nvox + VDVC_BLOCK - 1) / VDVC_BLOCK)), b(VDVC_BLOCK);
+  if (rtype == GT_F64) {
+    if (ftype != GT_F64) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL((residual_out_kernel<double, double>), g, b, 0, st, M, (const double*)f,
+                       (const double*)gv, bm, mean0, gmean, kscale, (double*)out);
+  } else if (ftype == GT_F32) {
+    hipLaunchKernelGGL((residual_out_kernel<float, float>), g, b, 0, st, M, (const float*)f,
+                       (const float*)gv, bm, mean0, gmean, kscale, (float*)out);
+  } else if (ftype == GT_F16) {
+    hipLaunchKernelGGL((residual_out_kernel<float, __half>), g, b, 0, st, M, (const __half*)f,
+                       (const float*)gv, bm, mean0, gmean, kscale, (float*)out);
+  } else {
+    return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
 
 int launch_restrict(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                     const TilesDev& T, const VoxIn& in, const void* vv, void* Pv,

@@ -183,12 +183,11 @@ def compute_b_gpu(res_vox, mask_vox, grad_f_vox, N_stencil, w_stencil, h, Nx, Ny
     r = L.to_dev_level(res_vox, shp, wd)
     m = L.to_dev_level(mask_vox, shp, 'uint8')
     zero = L.zeros(r.numel(), wd)
-    rout = t.empty_like(r)
     b = L.empty(Ndof, wd)
     s = (L._d * 4)()
-    # res = ((0 - 0) * 0 + (res - 0)) reproduces res exactly
+    # res = ((0 - 0) * 0 + (res - 0)) reproduces res exactly (not stored)
     L.call('vdvc_residual_rhs', L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), code,
-           L.ptr(r), code, 0.0, L.ptr(zero), 0.0, 0.0, L.ptr(m), L.ptr(m), L.ptr(rout), L.ptr(b),
+           L.ptr(r), code, 0.0, L.ptr(zero), 0.0, 0.0, L.ptr(m), L.ptr(m), None, L.ptr(b),
            None, None, None, s)
     return b if dev else L.to_host(b)
 
@@ -565,7 +564,10 @@ class _Level:
     def solve(self, g_origin, glt, l0, K_ref_laplacian, reg_exponent, c_reg, reg_type,
               K_ref_elastic, eps_reg, dynamic_mask, freeze_mask_after, U0, maxiter, eps,
               pcg_tol, pcg_max_iter, disp, log, pcg_check_every, callback=None,
-              refine_fp64=0):
+              refine_fp64=0, want_res=True):
+        """One level's GN solve; returns (U, res, dU) with res the dense
+        residual of the last iteration (None when `want_res` is false: the
+        multiscale driver discards the residuals of its coarser levels)."""
         t = L.require_gpu()
         c = L.ctx()
         Nx, Ny, Nz, h = self.M
@@ -656,9 +658,10 @@ class _Level:
             U = L.convert(U0, wd)
             if U is U0:
                 U = L.clone(U0)
-        # pitched residual: every grid voxel is written by each residual pass
-        # (the row padding is never read into a result)
-        res = L.empty(Nvox, wd) if maxiter > 0 else L.zeros(Nvox, wd)
+        # the residual passes store no residual: the returned one (the last
+        # iteration's) is formed once after the loop from that iteration's
+        # sampled values, statistics and b-mask (vdvc_residual_out)
+        last = None
         dU = L.zeros(Ndof, wd)
         diag = L.empty(Ndof, wd)
         Minv = L.empty(Ndof, wd)
@@ -697,9 +700,10 @@ class _Level:
             L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
                    gt, L.ptr(self.f), self.itype, mean0_f, L.ptr(gvals),
                    rnd(g_mean), rnd(kscale), L.ptr(bmask),
-                   L.ptr(mask), L.ptr(res), L.ptr(b),
+                   L.ptr(mask), None, L.ptr(b),
                    L.ptr(Gw[0]) if refold else None, L.ptr(Gw[1]) if refold else None,
                    L.ptr(Gw[2]) if refold else None, s4 if (disp or log is not None) else None)
+            last = (rnd(g_mean), rnd(kscale), bmask)
             if l0 is not None:
                 reg.matvec(self.M, U, RU)
                 K.axpby(-llf, RU, 1.0, b)
@@ -730,11 +734,20 @@ class _Level:
                           % (ik + 1, stdr, err, n_active, self.Nvox, 100 * n_active / self.Nvox))
             if err < eps:
                 break
+        res = None
+        if want_res:
+            res = L.empty(self.shape[0] * self.shape[1] * self.shape[2], wd)
+            if last is None:
+                L.fill(res, 0.0)
+            else:
+                L.call('vdvc_residual_out', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, mean0_f,
+                       L.ptr(gvals), L.F64 if wd == 'float64' else L.F32, last[0], last[1],
+                       L.ptr(last[2]), L.ptr(res))
         if refine_fp64 and self.wd != 'float64':
             U = self._refine_fp64(U, frozen, dynamic_mask, mask0, g_origin, glt, reg, rk, llf,
                                   l0 is not None, Minv, pcg_tol, pcg_max_iter,
                                   pcg_check_every, int(refine_fp64), log, disp)
-        return U, L.unpitch(res, self.shape), dU
+        return U, res, dU
 
     def _refine_fp64(self, U, frozen, dynamic_mask, mask0, g_origin, glt, reg, rk, llf,
                      use_reg, Minv, pcg_tol, pcg_max_iter, pcg_check_every, nsteps, log, disp):
@@ -769,7 +782,6 @@ class _Level:
         L.call('vdvc_convert', c, Ndof, U, L.rtype_of(U), U64, L.F64)
         gvals = L.empty(Nvox, 'float64')
         live = L.empty(Nvox, 'uint8')
-        res = L.empty(Nvox, 'float64')
         b64 = L.empty(Ndof, 'float64')
         b32 = L.empty(Ndof, 'float32')
         RU = L.empty(Ndof, 'float64')
@@ -791,7 +803,7 @@ class _Level:
             g_mean, std1 = _stats_mean_std(s3)
             L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, G64[0], G64[1], G64[2], L.F64, f64,
                    L.F64, float(mean0), gvals, float(g_mean), float(-(std0 / std1)), bfix,
-                   live, res, b64, None, None, None, None)
+                   live, None, b64, None, None, None, None)
             if use_reg:
                 reg.matvec(self.M, U64, RU)
                 K.axpby(-llf, RU, 1.0, b64)
@@ -939,7 +951,7 @@ def multiscale_correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, scales=(2, 1, 0
         lcb = None if cb is None else (lambda ik, Ud, _s=isc: cb(_s, ik, Ud))
         U, res, dU = lv.solve(g_origin=go_i, l0=l0_i, U0=U, log=sub, callback=lcb,
                               refine_fp64=refine if isc == scales[-1] else 0,
-                              **_solve_kwargs(kwargs))
+                              want_res=isc == scales[-1], **_solve_kwargs(kwargs))
         if log is not None:
             log.append(dict(scale=isc, iters=sub))
         prev = s
