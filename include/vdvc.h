@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 8
+#define VDVC_ABI_VERSION 9
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -111,6 +111,19 @@ int vdvc_image_setup(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d
 int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_field,
                       int ftype, const uint8_t* d_mask, int use_thresh, double glt,
                       double* h_out3);
+
+/* The reference image's upload and mask0 statistics in one pass: d_src, a
+ * dense x-fastest volume of (Nx*h+1, Ny*h+1, Nz*h+1) voxels of stype, is
+ * rounded once to dtype and written to d_dst in the pitched level layout
+ * (padding columns zero), and h_out3 receives the multiplicity-weighted
+ * (count, sum, sum of squares) of the stored values above glt, as
+ * vdvc_masked_stats with use_thresh on d_dst would (the summation order
+ * differs).  Replaces the conversion of f_pix to the working dtype and the
+ * mean0 / std0 reduction of correlate_gpu (engine/correlate_gpu.py:938,
+ * 952, 961; masked_quadpoint_mean_std at :113-157).  Type pairs (stype,
+ * dtype): (F32, F16), (F32, F32), (F64, F64), (F64, F32). */
+int vdvc_level_image(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_src, int stype,
+                     void* d_dst, int dtype, double glt, double* h_out3);
 
 /* g(x + u(x)) at every voxel, active mask = in_bounds & mask0 & (g > glt), and
  * the multiplicity-weighted (count, sum, sum sq) of g over d_bmask (or over

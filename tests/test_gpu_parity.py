@@ -168,6 +168,54 @@ def test_compute_b(gpu, shape):
     assert rel(got, ref) < 2e-5
 
 
+@pytest.mark.parametrize('types', [('float32', 'float16'), ('float32', 'float32'),
+                                   ('float64', 'float64'), ('float64', 'float32')])
+@pytest.mark.parametrize('shape', [(64, 2, 1, 8), (32, 1, 2, 8), (3, 2, 2, 8), (5, 3, 4, 3),
+                                   (70, 1, 2, 1), (1, 1, 1, 1), (9, 2, 3, 8)])
+def test_level_image_matches_copy_and_stats(gpu, shape, types):
+    """vdvc_level_image writes the pitched copy of vdvc_copy_volume bit for
+    bit (padding zero) and the statistics of vdvc_masked_stats over the
+    stored values above glt (same count; sums to 1e-13), and the mean / std
+    match the oracle's masked_quadpoint_mean_std of the stored values.
+    Widths 513 and 257 (eight-chunk kernel), 25 / 71 / 2 (four-chunk)."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import correlate_gpu as C
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    Nx, Ny, Nz, h = shape
+    st, dt = types
+    print('level_image', shape, types, flush=True)
+    rng = np.random.default_rng(21)
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    nv = shp[0] * shp[1] * shp[2]
+    f = rng.uniform(0, 255, nv)
+    src = L.to_dev_flat(f, st)
+    ref = L.pitch_dense(src, shp, dt)
+    got = L.fill(L.empty(ref.numel(), dt), 7.0)
+    s3 = (L._d * 3)()
+    L.call('vdvc_level_image', L.ctx(), Nx, Ny, Nz, h, L.ptr(src), L.type_code(st), L.ptr(got),
+           L.type_code(dt), 30.0, s3)
+    assert torch.equal(got, ref)
+    if dt == 'float16':
+        # a destination 2 bytes off 4-byte alignment takes the one-voxel-per-
+        # lane kernel instead of the paired one: same values, same statistics
+        buf = L.fill(L.empty(ref.numel() + 1, dt), 7.0)
+        o3 = (L._d * 3)()
+        L.call('vdvc_level_image', L.ctx(), Nx, Ny, Nz, h, L.ptr(src), L.type_code(st),
+               L.ptr(buf[1:]), L.type_code(dt), 30.0, o3)
+        assert torch.equal(buf[1:], ref)
+        assert o3[0] == s3[0] and abs(o3[1] - s3[1]) <= 1e-13 * abs(s3[1])
+    r3 = (L._d * 3)()
+    L.call('vdvc_masked_stats', L.ctx(), Nx, Ny, Nz, h, L.ptr(ref), L.type_code(dt), None, 1,
+           30.0, r3)
+    assert s3[0] == r3[0]
+    assert abs(s3[1] - r3[1]) <= 1e-13 * abs(r3[1]) and abs(s3[2] - r3[2]) <= 1e-13 * r3[2]
+    stored = L.to_host(L.unpitch(ref, shp)).astype(np.float64)
+    rmean, rstd = O.masked_quadpoint_mean_std(stored, stored > 30.0, h, Nx, Ny, Nz)
+    mean, std = C._stats_mean_std(s3)
+    assert abs(mean - rmean) < 1e-9 * abs(rmean) and abs(std - rstd) < 1e-9 * rstd
+
+
 @pytest.mark.parametrize('form', ['f32', 'bf16', 'f64'])
 @pytest.mark.parametrize('shape', [(3, 2, 2, 8), (70, 2, 1, 8), (5, 3, 4, 3)])
 def test_residual_out_matches_rhs_residual(gpu, shape, form):

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 8
+ABI_VERSION = 9
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -40,6 +40,7 @@ SIGNATURES = {
     'vdvc_stream_read': [_vp, _i64, _vp, _i, ctypes.c_uint32, _vp],
     'vdvc_image_setup': [_vp, _i, _i, _i, _i, _vp, _i, _d, _vp, _vp, _vp, _i, _vp],
     'vdvc_masked_stats': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _d, _dp],
+    'vdvc_level_image': [_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _d, _dp],
     'vdvc_sample': [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _d, _d, _d, _d, _vp, _vp,
                     _vp, _vp, _dp, _i],
     'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d, _d, _vp,

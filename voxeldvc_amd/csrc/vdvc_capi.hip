@@ -878,6 +878,27 @@ int vdvc_masked_stats(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
   return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_out3);
 }
 
+int vdvc_level_image(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_src, int stype,
+                     void* d_dst, int dtype, double glt, double* h_out3) {
+  if (!ctx || !h_out3 || !d_src || !d_dst) return fail(-1, "NULL argument");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!((stype == VDVC_F32 && (dtype == VDVC_F16 || dtype == VDVC_F32)) ||
+        (stype == VDVC_F64 && (dtype == VDVC_F64 || dtype == VDVC_F32))))
+    return fail(-1, "unsupported (source, level) type pair");
+  auto esz = [](int t) -> uintptr_t { return t == VDVC_F64 ? 8 : (t == VDVC_F16 ? 2 : 4); };
+  if (((uintptr_t)d_src & (esz(stype) - 1)) || ((uintptr_t)d_dst & (esz(dtype) - 1)))
+    return fail(-1, "volume not aligned to its element size");
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  const int nb = 2048;
+  if ((rc = ctx->stats.ensure((size_t)nb * 3 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  VDVC_LAUNCH(vdvc::launch_level_image(stype, dtype, M, d_src, d_dst, glt,
+                                       ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 3, ctx->red.as<double>(), ctx->stream));
+  return fetch(ctx, ctx->red.p, 3 * sizeof(double), h_out3);
+}
+
 int vdvc_sample(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_U, const void* d_g,
                 int gitype, int gnx, int gny, int gnz, double gox, double goy, double goz,
                 double glt, const uint8_t* d_mask0, const uint8_t* d_bmask, void* d_gvals,

@@ -260,6 +260,8 @@ int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleI
                   const void* U, bool f32w, hipStream_t st);
 int launch_setup(int ftype, int gtype, const MeshDev& M, const void* f, double glt, void* gx,
                  void* gy, void* gz, uint8_t* mask0, hipStream_t st);
+int launch_level_image(int stype, int dtype, const MeshDev& M, const void* src, void* dst,
+                       double glt, double* part, int nblocks, hipStream_t st);
 int launch_stats(int ftype, const MeshDev& M, const void* fld, const uint8_t* mask, int use_thresh,
                  double glt, double* part, int nblocks, hipStream_t st);
 int launch_reduce(const double* part, long long n, int nv, double* out, hipStream_t st);

@@ -553,7 +553,10 @@ class _Level:
         # reference volume in the pitched level layout.  The deformed image is
         # only gathered by the sampler, whose cost is per voxel and not per
         # byte: a float32 volume is sampled as it is, with no float16 copy.
-        self.f = L.pitch_dense(f, self.shape, iname)
+        # (the pitched copy is written by the solve's first pass, which also
+        # takes the statistics of mask0 over it: vdvc_level_image)
+        self.f_src = f
+        self.f = L.empty(self.Nvox_p, iname)
         if storage == 'bf16' and g.dtype == t.float32:
             self.g, self.gitype = g, L.F32
         else:
@@ -578,11 +581,18 @@ class _Level:
         G = [L.empty(Nvox, self.gname) for _ in range(3)]
         self.G = G
         mask0 = L.empty(Nvox, 'uint8')
+        s3 = (L._d * 3)()
+        if self.f_src is not None:
+            # f in the storage type and level layout, and the statistics of
+            # the stored values over mask0 = f > glt, in one pass
+            L.call('vdvc_level_image', c, Nx, Ny, Nz, h, L.ptr(self.f_src),
+                   L.type_code(self.f_src.dtype), L.ptr(self.f), self.itype, float(glt), s3)
+            self.f_src = None
+        else:
+            L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, None, 1,
+                   float(glt), s3)
         L.call('vdvc_image_setup', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, float(glt),
                L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]), gt, L.ptr(mask0))
-        s3 = (L._d * 3)()
-        L.call('vdvc_masked_stats', c, Nx, Ny, Nz, h, L.ptr(self.f), self.itype, None, 1,
-               float(glt), s3)
         mean0, std0 = _stats_mean_std(s3)
         wd = self.wd
         # scalars enter the element-wise steps rounded to the working type
