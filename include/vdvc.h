@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 9
+#define VDVC_ABI_VERSION 10
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -259,6 +259,13 @@ int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d
  * evaluated one rounded operation at a time from the left; out[3n+1] =
  * out[3n+2] = 0.  Replaces the V construction of correlate_gpu.py:1016-1020. */
 int vdvc_calibration_mode(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, int rtype, void* d_out);
+
+/* The GN update and its convergence norms: U <- U + dU in rtype, then
+ * h_out2 = (dU.dU, U.U) in FP64, equal bit for bit to vdvc_axpby(1, dU, 1, U)
+ * followed by vdvc_dot(dU, dU) and vdvc_dot(U, U).  Replaces U = U + dU and
+ * the two norms of err = ||dU|| / ||U|| (engine/correlate_gpu.py:1129-1131). */
+int vdvc_gn_update(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_dU, void* d_U,
+                   double* h_out2);
 
 /* y = a*x + b*y in rtype. */
 int vdvc_axpby(vdvc_ctx* ctx, int64_t n, int rtype, double a, const void* d_x, double b,

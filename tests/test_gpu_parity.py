@@ -168,6 +168,28 @@ def test_compute_b(gpu, shape):
     assert rel(got, ref) < 2e-5
 
 
+@pytest.mark.parametrize('rd', ['float32', 'float64'])
+@pytest.mark.parametrize('n', [1, 1000, 824875])
+def test_gn_update_matches_axpby_and_dots(gpu, n, rd):
+    """vdvc_gn_update equals vdvc_axpby(1, dU, 1, U) followed by the two
+    vdvc_dot norms, bit for bit, and the norms match NumPy's float64 sums."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as K
+    print('gn_update', n, rd, flush=True)
+    rng = np.random.default_rng(31)
+    u0, du = rng.normal(size=n), rng.normal(scale=1e-2, size=n)
+    U1, dU = L.to_dev_flat(u0, rd), L.to_dev_flat(du, rd)
+    U2 = L.clone(U1)
+    s2 = (L._d * 2)()
+    L.call('vdvc_gn_update', L.ctx(), n, L.rtype_of(U1), L.ptr(dU), L.ptr(U1), s2)
+    K.axpby(1.0, dU, 1.0, U2)
+    assert torch.equal(U1, U2)
+    assert s2[0] == K.dot(dU, dU) and s2[1] == K.dot(U2, U2)
+    uh, dh = L.to_host(U2).astype(np.float64), L.to_host(dU).astype(np.float64)
+    assert abs(s2[0] - dh @ dh) <= 1e-12 * (dh @ dh) and abs(s2[1] - uh @ uh) <= 1e-12 * (uh @ uh)
+
+
 @pytest.mark.parametrize('types', [('float32', 'float16'), ('float32', 'float32'),
                                    ('float64', 'float64'), ('float64', 'float32')])
 @pytest.mark.parametrize('shape', [(64, 2, 1, 8), (32, 1, 2, 8), (3, 2, 2, 8), (5, 3, 4, 3),

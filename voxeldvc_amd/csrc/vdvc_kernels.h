@@ -298,6 +298,8 @@ int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t 
 template <typename R>
 int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hipStream_t st);
 template <typename R>
+int launch_gn_update(long long n, const R* dU, R* U, double* part, int nparts, hipStream_t st);
+template <typename R>
 int launch_calib_mode(int Nx, int Ny, int Nz, int h, R* out, hipStream_t st);
 template <typename R>
 int launch_hutch_acc(long long ndof, const int8_t* z, const R* Rz, double* acc, hipStream_t st);

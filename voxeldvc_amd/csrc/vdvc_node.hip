@@ -772,6 +772,32 @@ __global__ __launch_bounds__(VDVC_BLOCK) void dot_kernel(long long n, const R* _
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
+// The GN update U <- U + dU with the partial FP64 dot products dU.dU and
+// U.U of the updated U: the same element operations, grid and per-thread
+// summation order as axpby_kernel (a = b = 1) followed by dot_kernel on each
+// pair, so the two sums equal those of separate dot products bit for bit.
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void gn_update_kernel(long long n,
+                                                               const R* __restrict__ dU,
+                                                               R* __restrict__ U,
+                                                               double* __restrict__ part) {
+  __shared__ double lds[4 * 2];
+  double v[2] = {0.0, 0.0};
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
+    const R d = dU[q];
+    const R u = d + U[q];
+    U[q] = u;
+    v[0] += (double)d * d;
+    v[1] += (double)u * u;
+  }
+  block_sum_f64<2>(v, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
 // Hutchinson accumulation acc += z * Rz (FP64), z a Rademacher probe
 template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void hutch_acc_kernel(long long n,
@@ -1005,6 +1031,12 @@ int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hi
   return (int)hipGetLastError();
 }
 
+template <typename R>
+int launch_gn_update(long long n, const R* dU, R* U, double* part, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(gn_update_kernel<R>, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, dU, U, part);
+  return (int)hipGetLastError();
+}
+
 #define VDVC_INST(R)                                                                         \
   template int launch_hutch_acc<R>(long long, const int8_t*, const R*, double*, hipStream_t); \
   template int launch_expand_i8<R>(long long, const int8_t*, R*, hipStream_t);               \
@@ -1021,6 +1053,7 @@ int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hi
                                      const R*, R*, bool, hipStream_t);                       \
   template int launch_axpby<R>(long long, double, const R*, double, R*, hipStream_t);        \
   template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);      \
+  template int launch_gn_update<R>(long long, const R*, R*, double*, int, hipStream_t);      \
   template int launch_calib_mode<R>(int, int, int, int, R*, hipStream_t);
 VDVC_INST(float)
 VDVC_INST(double)

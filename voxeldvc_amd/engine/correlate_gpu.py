@@ -692,6 +692,7 @@ class _Level:
             jacobi(mask0)
         frozen = None
         s4 = (L._d * 4)()
+        s2 = (L._d * 2)()
         it_ = (L._i * 1)()
         rr_ = (L._d * 1)()
         for ik in range(maxiter):
@@ -727,11 +728,11 @@ class _Level:
                    L.ptr(mh),
                    rk[0], rk[1], rk[2], rk[3], rk[4], llf, L.ptr(b), L.ptr(Minv), L.ptr(dU),
                    float(pcg_tol), int(pcg_max_iter), int(pcg_check_every), it_, rr_)
-            K.axpby(1.0, dU, 1.0, U)
+            # U += dU and the two norms in one pass (one host round trip)
+            L.call('vdvc_gn_update', c, Ndof, L.rtype_of(U), L.ptr(dU), L.ptr(U), s2)
             if callback is not None:
                 callback(ik, U)
-            nd = K.dot(dU, dU)
-            nu = K.dot(U, U)
+            nd, nu = s2[0], s2[1]
             err = math.sqrt(nd) / math.sqrt(nu) if nu > 0 else (math.nan if nd == 0 else math.inf)
             if disp or log is not None:
                 _, stdr = _stats_mean_std(s4)
