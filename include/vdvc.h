@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 10
+#define VDVC_ABI_VERSION 11
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -154,6 +154,24 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
                       const uint8_t* d_bmask, const uint8_t* d_live, void* d_res, void* d_b,
                       uint16_t* d_fold_x, uint16_t* d_fold_y, uint16_t* d_fold_z,
                       double* h_stats4);
+
+/* vdvc_residual_rhs followed by vdvc_jacobi for the H of a dynamic mask (the
+ * refolded planes d_fold_* when given, else the gradients with d_bmask), in
+ * one voxel pass where the library has it (float32 image and gradients
+ * without refold, or float16 image and bfloat16 gradients with refold, on
+ * its vectorised plan): the diagonal's voxel sums are taken beside b's from
+ * the same gradient loads.  Outputs as the two calls; the diagonal equals
+ * vdvc_jacobi's to float32 rounding (its partial sums follow the residual
+ * pass's tiling).  Replaces one iteration's b, diag(H) and M_inv of
+ * correlate_gpu (engine/correlate_gpu.py:1092-1120). */
+int vdvc_residual_rhs_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                             const void* d_gy, const void* d_gz, int gtype, const void* d_f,
+                             int ftype, double mean0, const void* d_gvals, double gmean,
+                             double kscale, const uint8_t* d_bmask, const uint8_t* d_live,
+                             void* d_res, void* d_b, uint16_t* d_fold_x, uint16_t* d_fold_y,
+                             uint16_t* d_fold_z, double* h_stats4, int reg_kind,
+                             const double* h_K1, const double* h_K2, double E, double eps_reg,
+                             double ll, const void* d_extra, void* d_diag, void* d_minv);
 
 /* The residual of vdvc_residual_rhs, (g - g_mean) * kscale + (f - mean0) in
  * the working type rtype (the type of d_gvals) and zero outside d_bmask,

@@ -168,6 +168,78 @@ def test_compute_b(gpu, shape):
     assert rel(got, ref) < 2e-5
 
 
+@pytest.mark.parametrize('reg', ['laplacian', 'none'])
+@pytest.mark.parametrize('form', ['f32', 'bf16', 'f64'])
+@pytest.mark.parametrize('shape', [(3, 2, 2, 8), (64, 2, 1, 8), (5, 3, 4, 4), (9, 7, 5, 2),
+                                   (5, 3, 4, 3)])
+def test_residual_rhs_jacobi_matches_two_passes(gpu, shape, form, reg, monkeypatch):
+    """vdvc_residual_rhs_jacobi = vdvc_residual_rhs + vdvc_jacobi of the
+    dynamic mask's H: b and the refolded planes bit for bit, diag and M_inv
+    to float32 rounding where the diagonal rides on the residual pass (the
+    float32 and mixed forms at power-of-two h), bit for bit where the entry
+    runs the two passes (FP64, h = 3, or VDVC_NO_RHS_DIAG=1)."""
+    import torch
+    from voxeldvc_amd import _lib as L
+    from voxeldvc_amd.engine import kernels_dvc as Kd
+    Nx, Ny, Nz, h = shape
+    print('residual_rhs_jacobi', shape, form, reg, flush=True)
+    rng = np.random.default_rng(14)
+    shp = Kd.level_shape(h, Nx, Ny, Nz)
+    nv = shp[0] * shp[1] * shp[2]
+    g, m, _ = rand_problem(15, Nx, Ny, Nz, h)
+    f = rng.uniform(0, 255, nv).round()
+    gv = rng.uniform(0, 255, nv)
+    rd, fd, gd = {'f32': ('float32', 'float32', 'float32'),
+                  'bf16': ('float32', 'float16', 'bfloat16'),
+                  'f64': ('float64', 'float64', 'float64')}[form]
+    fold = form == 'bf16'
+    gx, gy, gz = Kd.grad_planes(g, shp, gd)
+    fimg = L.to_dev_level(f, shp, fd)
+    gvals = L.to_dev_level(gv, shp, rd)
+    bm = L.to_dev_level(m, shp, 'uint8')
+    ndof = 3 * (Nx + 1) * (Ny + 1) * (Nz + 1)
+    Kp = np.ascontiguousarray(O.build_K_ref_laplacian(), dtype=np.float64)
+    rk = ((L.REG_K, Kp.ctypes.data_as(L._dp)) if reg == 'laplacian' else (L.REG_NONE, None))
+    rk = rk + (None, float(h), 0.0, 0.5 if reg == 'laplacian' else 0.0, None)
+    npl = fimg.numel()
+
+    def run(fused):
+        b = L.empty(ndof, rd)
+        fo = [L.empty(npl, 'bfloat16') for _ in range(3)] if fold else [None] * 3
+        d, mi = L.empty(ndof, rd), L.empty(ndof, rd)
+        s4 = (L._d * 4)()
+        args = (L.ctx(), Nx, Ny, Nz, h, L.ptr(gx), L.ptr(gy), L.ptr(gz), L.type_code(gd),
+                L.ptr(fimg), L.type_code(fd), 97.25, L.ptr(gvals), 101.5, -1.0625, L.ptr(bm),
+                L.ptr(bm), None, L.ptr(b), *[L.ptr(q) for q in fo], s4)
+        if fused:
+            L.call('vdvc_residual_rhs_jacobi', *args, *rk, L.ptr(d), L.ptr(mi))
+        else:
+            L.call('vdvc_residual_rhs', *args)
+            hg = fo if fold else [gx, gy, gz]
+            L.call('vdvc_jacobi', L.ctx(), Nx, Ny, Nz, h, *[L.ptr(q) for q in hg],
+                   L.BF16 if fold else L.type_code(gd), None if fold else L.ptr(bm), *rk,
+                   L.ptr(d), L.ptr(mi))
+        return b, fo, d, mi, tuple(s4)
+
+    b0, f0, d0, m0, s0 = run(False)
+    b1, f1, d1, m1, s1 = run(True)
+    assert torch.equal(b1, b0) and s1 == s0
+    if fold:
+        # (the row padding of the refolded planes is never written)
+        assert all(torch.equal(L.unpitch(p, shp), L.unpitch(q, shp)) for p, q in zip(f1, f0))
+    exact = form == 'f64' or h == 3
+    if exact:
+        assert torch.equal(d1, d0) and torch.equal(m1, m0)
+    else:
+        # sums of non-negative float32 terms in another order: a few ulps
+        tol = 1e-5 * float(d0.abs().max())
+        assert float((d1.double() - d0.double()).abs().max()) <= tol
+        assert float(((m1.double() - m0.double()) / m0.double()).abs().max()) < 1e-4
+        monkeypatch.setenv('VDVC_NO_RHS_DIAG', '1')
+        b2, f2, d2, m2, s2 = run(True)
+        assert torch.equal(b2, b0) and torch.equal(d2, d0) and torch.equal(m2, m0)
+
+
 @pytest.mark.parametrize('rd', ['float32', 'float64'])
 @pytest.mark.parametrize('n', [1, 1000, 824875])
 def test_gn_update_matches_axpby_and_dots(gpu, n, rd):

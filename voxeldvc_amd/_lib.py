@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 10
+ABI_VERSION = 11
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -45,6 +45,9 @@ SIGNATURES = {
                     _vp, _vp, _dp, _i],
     'vdvc_residual_rhs': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d, _d, _vp,
                           _vp, _vp, _vp, _vp, _vp, _vp, _dp],
+    'vdvc_residual_rhs_jacobi': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _d, _vp, _d,
+                                 _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dp, _i, _dp, _dp, _d,
+                                 _d, _d, _vp, _vp, _vp],
     'vdvc_residual_out': [_vp, _i, _i, _i, _i, _vp, _i, _d, _vp, _i, _d, _d, _vp, _vp],
     'vdvc_matvec_H': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     'vdvc_matvec_H_noalloc': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp],

@@ -106,6 +106,7 @@ struct vdvc_ctx {
   hipStream_t stream = nullptr;
   void* scratch = nullptr;  // this device's address of the static scratch array
   Buf P;          // voxel-kernel node partials
+  Buf Pd;         // the residual pass's Hessian-diagonal partials
   Buf stats;      // per-wave / per-block FP64 partials
   Buf red;        // reduced FP64 results
   Buf nodeA, nodeB, nodeC;  // node-vector temporaries
@@ -641,17 +642,33 @@ int hutch_impl(vdvc_ctx* ctx, const MeshDev& M, const double* h_Kel, double E,
 }
 
 template <typename R>
+int jacobi_finish(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, const R* P, int reg_kind,
+                  const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                  const R* d_extra, R* d_diag, R* d_minv);
+
+template <typename R>
 int jacobi_impl(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, int gtype, const VoxIn& in,
                 int reg_kind, const double* h_K1, const double* h_K2, double E, double eps_reg,
                 double ll, const R* d_extra, R* d_diag, R* d_minv) {
+  int rc;
+  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, in, VDVC_F32, false, nullptr))) return rc;
+  return jacobi_finish<R>(ctx, M, T, ctx->P.as<const R>(), reg_kind, h_K1, h_K2, E, eps_reg, ll,
+                          d_extra, d_diag, d_minv);
+}
+
+// the node side of the Jacobi preconditioner from the diagonal's voxel
+// partials P (plan T): gather + regularizer diagonal, then the inverse
+template <typename R>
+int jacobi_finish(vdvc_ctx* ctx, const MeshDev& M, const TilesDev& T, const R* P, int reg_kind,
+                  const double* h_K1, const double* h_K2, double E, double eps_reg, double ll,
+                  const R* d_extra, R* d_diag, R* d_minv) {
   int bd1 = 0, bd2 = 0, rc;
   if ((rc = prep_reg<R>(ctx, reg_kind, h_K1, h_K2, &bd1, &bd2))) return rc;
-  if ((rc = run_restrict(ctx, MODE_DIAG, gtype, M, T, in, VDVC_F32, false, nullptr))) return rc;
   const int nb = vdvc::node_blocks(M);
   if ((rc = ctx->fmin.ensure((size_t)nb * sizeof(R)))) return rc;
   DiagOp<R> op;
   std::memset(&op, 0, sizeof(op));
-  op.P = ctx->P.as<const R>(); op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<R>();
+  op.P = P; op.T = T; op.out = d_diag; op.min_part = ctx->fmin.as<R>();
   if (reg_kind == VDVC_REG_K) {
     op.k1 = kop1<R>(ctx, bd1, E, nullptr); op.c1 = (R)ll; op.k1_on = 1;
   } else if (reg_kind == VDVC_REG_EQGAP) {
@@ -968,6 +985,65 @@ int vdvc_residual_rhs(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* 
                                                 static_cast<float*>(d_b), h_stats4),
                        residual_rhs_impl<double>(ctx, M, T, gtype, in, ftype, fold,
                                                  static_cast<double*>(d_b), h_stats4));
+}
+
+int vdvc_residual_rhs_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx,
+                             const void* d_gy, const void* d_gz, int gtype, const void* d_f,
+                             int ftype, double mean0, const void* d_gvals, double gmean,
+                             double kscale, const uint8_t* d_bmask, const uint8_t* d_live,
+                             void* d_res, void* d_b, uint16_t* d_fold_x, uint16_t* d_fold_y,
+                             uint16_t* d_fold_z, double* h_stats4, int reg_kind,
+                             const double* h_K1, const double* h_K2, double E, double eps_reg,
+                             double ll, const void* d_extra, void* d_diag, void* d_minv) {
+  if (!ctx) return fail(-1, "ctx is NULL");
+  int rc = check_mesh(Nx, Ny, Nz, h);
+  if (rc) return rc;
+  if (!gtype_ok(gtype) || !ftype_ok(ftype)) return fail(-1, "unsupported element type");
+  if (!d_diag || !d_minv) return fail(-1, "NULL buffer");
+  const bool fold = d_fold_x != nullptr;
+  // the H whose diagonal is taken: the refolded planes, or the gradients
+  // with the b-mask
+  const void* hg[3] = {fold ? d_fold_x : d_gx, fold ? d_fold_y : d_gy, fold ? d_fold_z : d_gz};
+  const int hgt = fold ? VDVC_BF16 : gtype;
+  const uint8_t* hm = fold ? nullptr : d_bmask;
+  MeshDev M = make_mesh(Nx, Ny, Nz, h);
+  VoxIn in = vox_in(d_gx, d_gy, d_gz, d_bmask);
+  in.fimg = d_f; in.gvals = d_gvals; in.live = d_live;
+  in.mean0 = mean0; in.gmean = gmean; in.kscale = kscale;
+  in.res = d_res;
+  in.fold_out[0] = d_fold_x; in.fold_out[1] = d_fold_y; in.fold_out[2] = d_fold_z;
+  const TilesDev T = tiles_for(M, gtype, in, false);
+  if (gtype == VDVC_F64 || !vdvc::rhs_diag_fused(gtype, ftype, fold, T) ||
+      getenv("VDVC_NO_RHS_DIAG")) {
+    // no fused kernel for this form or plan: the two passes
+    rc = vdvc_residual_rhs(ctx, Nx, Ny, Nz, h, d_gx, d_gy, d_gz, gtype, d_f, ftype, mean0, d_gvals,
+                           gmean, kscale, d_bmask, d_live, d_res, d_b, d_fold_x, d_fold_y,
+                           d_fold_z, h_stats4);
+    if (rc) return rc;
+    return vdvc_jacobi(ctx, Nx, Ny, Nz, h, hg[0], hg[1], hg[2], hgt, hm, reg_kind, h_K1, h_K2, E,
+                       eps_reg, ll, d_extra, d_diag, d_minv);
+  }
+  if (!d_gx || !d_gy || !d_gz || !d_f || !d_gvals || !d_bmask || !d_live || !d_b)
+    return fail(-1, "NULL buffer");
+  if (fold && (!d_fold_y || !d_fold_z)) return fail(-1, "fold planes must all be given");
+  if ((rc = ctx->Pd.ensure((size_t)T.nwaves * T.wave_stride * sizeof(float)))) return rc;
+  if ((rc = ctx->stats.ensure((size_t)T.nwaves * 4 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  in.stats = h_stats4 ? ctx->stats.as<double>() : nullptr;
+  in.pdiag = ctx->Pd.as<float>();
+  // the diagonal's node pass before the statistics' fetch, so that the host
+  // round trip comes last
+  if ((rc = residual_rhs_impl<float>(ctx, M, T, gtype, in, ftype, fold, static_cast<float*>(d_b),
+                                     nullptr)))
+    return rc;
+  if ((rc = jacobi_finish<float>(ctx, M, T, ctx->Pd.as<const float>(), reg_kind, h_K1, h_K2, E,
+                                 eps_reg, ll, static_cast<const float*>(d_extra),
+                                 static_cast<float*>(d_diag), static_cast<float*>(d_minv))))
+    return rc;
+  if (!h_stats4) return 0;
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), T.nwaves, 4, ctx->red.as<double>(),
+                                  ctx->stream));
+  return fetch(ctx, ctx->red.p, 4 * sizeof(double), h_stats4);
 }
 
 int vdvc_residual_out(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_f, int ftype,

@@ -194,12 +194,20 @@ __device__ __forceinline__ void acc_voxel(float gx, float gy, float gz, float wm
 #define VDVC_VEC_MIN_BLOCKS(MODE, GT, E) (MODE == MODE_RHS ? 2 : 3)
 #endif
 
-template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V, int E>
+// DG (residual pass only): the Hessian diagonal of the b-mask's H is taken
+// from the same voxels in the same pass -- per voxel the diagonal mode's
+// operations on g with the mask weight, which equal its operations on the
+// folded g * mask -- into a second slot block set (in.pdiag), so a GN
+// iteration with a dynamic mask reads its gradients once for b, the refold
+// and the Jacobi preconditioner.
+template <int MODE, typename GT, bool HASMASK, typename FT, bool FOLD, int V, int E,
+          bool DG = false>
 __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void vox_vec_kernel(MeshDev M, TilesDev T, VoxIn in,
                                                             const float* __restrict__ v,
                                                             float* __restrict__ P) {
   static_assert(E == 1 || (E == 2 && MODE != MODE_RHS && V % 2 == 0),
                 "paired elements: products and diagonal only");
+  static_assert(!DG || (MODE == MODE_RHS && E == 1), "the diagonal rides on the residual pass");
   // element width of a paired-element lane (h = V / E); unused for E = 1
   constexpr int HE = V / E;
   __shared__ float lds_nodes[MODE == MODE_MATVEC ? VDVC_WAVES_PER_BLOCK * 64 * 12 * (E + 1) : 1];
@@ -262,6 +270,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
   float* res_f = reinterpret_cast<float*>(in.res);
   const float mean0_f = (float)in.mean0, gmean_f = (float)in.gmean, kscale_f = (float)in.kscale;
   float* Pw = P + (size_t)w * T.wave_stride;
+  float* Pd = DG ? in.pdiag + (size_t)w * T.wave_stride : nullptr;
 
   // node-plane accumulators C[cx][cy][cz][k] of the lane's E + 1 node columns
   float C[E + 1][2][2][3];
@@ -273,12 +282,23 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < 3; ++k) C[a][b][c][k] = 0.0f;
+  // DG: the diagonal's node planes
+  constexpr int ND = DG ? E + 1 : 1;
+  float C2[ND][2][2][3];
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) C2[a][b][c][k] = 0.0f;
 
   double st_c = 0.0, st_1 = 0.0, st_2 = 0.0, st_n = 0.0;
   // residual pass: display statistics requested (uniform across the grid)
   const bool stats = MODE == MODE_RHS && in.stats != nullptr;
 
-  auto emit = [&](int kzl) {
+  auto emit = [&](const auto& CC, float* PWb, int kzl) {
     // lanes of one element hold partial sums of that element's two x nodes
     // (a paired-element lane: of its three)
     float a[E + 1][2][3];
@@ -287,7 +307,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a[cx][cy][k] = C[cx][cy][0][k];
+        for (int k = 0; k < 3; ++k) a[cx][cy][k] = CC[cx][cy][0][k];
     for (int o = 1; o < G; o <<= 1) {
 #pragma unroll
       for (int cx = 0; cx < E + 1; ++cx)
@@ -302,7 +322,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
       for (int k = 0; k < 3; ++k) prev[cy][k] = __shfl(a[E][cy][k], src, 64);
-    float* slot = Pw + (size_t)kzl * 2 * T.px * 3;
+    float* slot = PWb + (size_t)kzl * 2 * T.px * 3;
     if ((lane % G) == 0) {
       // the element group's first node column ln (the previous group's last
       // column added in), then a paired lane's middle column
@@ -390,7 +410,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
   // plane state (updated when the flattened row walk enters a new z plane)
   int ez = ez0, tz = 0, ty = 0;
   int nzp = h + (ez == M.Nz - 1 ? 1 : 0);
-  float Z0 = 1.0f, Z1 = 0.0f, Wz = 1.0f, zw0 = 1.0f, zw1 = 0.0f;
+  float Z0 = 1.0f, Z1 = 0.0f, Wz = 1.0f, zw0 = 1.0f, zw1 = 0.0f, zd0 = 1.0f, zd1 = 0.0f;
   int mz = 1;
   long long rowbase = 0, tailbase = 0;
   // u of the plane at the lane's node columns (row 0 of the element row)
@@ -460,6 +480,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
     mz = axis_mult(z, h, M.Nvz - 1);
     zw0 = Z0; zw1 = Z1;
     if (MODE == MODE_DIAG) { zw0 = Z0 * Z0; zw1 = Z1 * Z1; }
+    if (DG) { zd0 = Z0 * Z0; zd1 = Z1 * Z1; }
     rowbase = (long long)z * pl + (long long)y0 * M.pitch;
     tailbase = (long long)z * pl + (long long)y0 * M.pitch + xend;
     if (MODE == MODE_MATVEC) {
@@ -484,13 +505,25 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
     if (++ty < nrows) return;
     ty = 0;
     if (++tz == nzp) {
-      emit(ez - ez0);
+      emit(C, Pw, ez - ez0);
 #pragma unroll
       for (int cx = 0; cx < E + 1; ++cx)
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
           for (int k = 0; k < 3; ++k) { C[cx][cy][0][k] = C[cx][cy][1][k]; C[cx][cy][1][k] = 0.0f; }
+      if constexpr (DG) {
+        emit(C2, Pd, ez - ez0);
+#pragma unroll
+        for (int cx = 0; cx < E + 1; ++cx)
+#pragma unroll
+          for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              C2[cx][cy][0][k] = C2[cx][cy][1][k];
+              C2[cx][cy][1][k] = 0.0f;
+            }
+      }
       ++ez;
       tz = 0;
       nzp = h + (ez == M.Nz - 1 ? 1 : 0);
@@ -539,6 +572,11 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
     for (int e = 0; e < E; ++e)
 #pragma unroll
       for (int k = 0; k < 3; ++k) { S[e][k] = 0.0f; Tt[e][k] = 0.0f; }
+    float S2[ND][3], T2[ND][3];
+#pragma unroll
+    for (int e = 0; e < ND; ++e)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { S2[e][k] = 0.0f; T2[e][k] = 0.0f; }
     // u of this row at the node columns, and its step across each element
     float uc[E + 1][3], dxe[E][3];
 #pragma unroll
@@ -608,6 +646,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         sd = resv[j];
       }
       acc_voxel<MODE>(gx[j], gy[j], gz[j], wm, X1, sd, S[e], Tt[e]);
+      if constexpr (DG) acc_voxel<MODE_DIAG>(gx[j], gy[j], gz[j], wm, X1, 0.0f, S2[e], T2[e]);
     }
     if (tail) {
       // last voxel plane x = Nx*h: t = h of this element, X1 = 1, W_x = 1/2
@@ -648,6 +687,7 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         sd = rv;
       }
       acc_voxel<MODE>(tgx, tgy, tgz, wm, 1.0f, sd, S[E - 1], Tt[E - 1]);
+      if constexpr (DG) acc_voxel<MODE_DIAG>(tgx, tgy, tgz, wm, 1.0f, 0.0f, S2[E - 1], T2[E - 1]);
     }
     // x-restricted row values R[cx][k] (the lanes past the row end have
     // zero node weights), weighted into the node planes
@@ -675,6 +715,28 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
         C[cx][1][0][k] += w10 * R[cx][k];
         C[cx][1][1][k] += w11 * R[cx][k];
       }
+    if constexpr (DG) {
+      // the diagonal mode's row restriction and squared weights
+      float R2[E + 1][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        R2[0][k] = S2[0][k];
+#pragma unroll
+        for (int c = 1; c < E; ++c) R2[c][k] = T2[c - 1][k] + S2[c][k];
+        R2[E][k] = T2[E - 1][k];
+      }
+      const float vy0 = Wr * (1.0f - Y1) * (1.0f - Y1), vy1 = Wr * Y1 * Y1;
+      const float d00 = vy0 * zd0, d01 = vy0 * zd1, d10 = vy1 * zd0, d11 = vy1 * zd1;
+#pragma unroll
+      for (int cx = 0; cx < E + 1; ++cx)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          C2[cx][0][0][k] += d00 * R2[cx][k];
+          C2[cx][0][1][k] += d01 * R2[cx][k];
+          C2[cx][1][0][k] += d10 * R2[cx][k];
+          C2[cx][1][1][k] += d11 * R2[cx][k];
+        }
+    }
   };
 
   element_setup();
@@ -723,7 +785,8 @@ __global__ __launch_bounds__(VDVC_BLOCK, VDVC_VEC_MIN_BLOCKS(MODE, GT, E)) void 
       }
     }
   }
-  emit(ez1 - ez0);
+  emit(C, Pw, ez1 - ez0);
+  if constexpr (DG) emit(C2, Pd, ez1 - ez0);
   if (stats) {
     st_c = wave_sum(st_c);
     st_1 = wave_sum(st_1);
@@ -755,6 +818,15 @@ int vec_width(int h, int gtype) {
   return h < vmax ? h : vmax;
 }
 
+// the residual + diagonal pass exists for the float32 form (float32 image
+// and gradients, no refold) and the mixed form (float16 image, bfloat16
+// gradients, refold), on the vectorised single-element-lane plan
+bool rhs_diag_fused(int gtype, int ftype, bool fold, const TilesDev& T) {
+  return T.vec != 0 && T.epl == 1 &&
+         ((gtype == GT_F32 && ftype == GT_F32 && !fold) ||
+          (gtype == GT_BF16 && ftype == GT_F16 && fold));
+}
+
 int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                         const TilesDev& T, const VoxIn& in, const float* v, float* P,
                         hipStream_t st) {
@@ -771,7 +843,28 @@ int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold,
       default: VV_L(MODE, GT, HM, FT, FO, 1, 1); break; \
     }                                                 \
   } while (0)
+#define VV_LD(MODE, GT, HM, FT, FO, VV)                                                    \
+  hipLaunchKernelGGL((vox_vec_kernel<MODE, GT, HM, FT, FO, VV, 1, true>), g, b, 0, st, M, T, in, \
+                     v, P)
+#define VV_VD(MODE, GT, HM, FT, FO)                    \
+  do {                                                 \
+    switch (T.vec) {                                   \
+      case 8: VV_LD(MODE, GT, HM, FT, FO, 8); break;   \
+      case 4: VV_LD(MODE, GT, HM, FT, FO, 4); break;   \
+      case 2: VV_LD(MODE, GT, HM, FT, FO, 2); break;   \
+      default: VV_LD(MODE, GT, HM, FT, FO, 1); break;  \
+    }                                                  \
+  } while (0)
   if (gtype == GT_F32 && T.vec == 8) return (int)hipErrorInvalidValue;
+  if (in.pdiag) {
+    // the residual pass with the Hessian diagonal (rhs_diag_fused)
+    if (mode != MODE_RHS || !rhs_diag_fused(gtype, ftype, fold, T)) return (int)hipErrorInvalidValue;
+    if (gtype == GT_F32) VV_VD(MODE_RHS, float, true, float, false);
+    else VV_VD(MODE_RHS, __hip_bfloat16, true, __half, true);
+    return (int)hipGetLastError();
+  }
+#undef VV_VD
+#undef VV_LD
   if (T.epl == 2) {
     // paired elements: bfloat16 lanes of 8 (h = 4) or 4 (h = 2) voxels,
     // float32 lanes of 4 (h = 2); products and diagonal only

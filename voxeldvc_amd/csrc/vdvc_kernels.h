@@ -22,6 +22,9 @@ struct VoxIn {
   // array of 64) and kc = c E, each element once (kreg null: none)
   const float* kreg;
   float kc;
+  // residual pass: node partials of the Hessian diagonal (same slot layout
+  // as the product's), taken in the same pass when non-null
+  float* pdiag;
 };
 
 struct SampleIn {
@@ -243,6 +246,7 @@ int launch_copy_volume(const VolCopy& C, hipStream_t s);
 int launch_axpby_x(long long n, double a, const void* x, int xt, double b, void* y, int yt,
                    hipStream_t s);
 int vec_width(int h, int gtype);
+bool rhs_diag_fused(int gtype, int ftype, bool fold, const TilesDev& T);
 int launch_restrict_vec(int mode, int gtype, bool hasmask, int ftype, bool fold, const MeshDev& M,
                         const TilesDev& T, const VoxIn& in, const float* v, float* P,
                         hipStream_t st);

@@ -708,21 +708,26 @@ class _Level:
             # folded storage with a dynamic mask: the residual kernel writes
             # g * b_mask for this iteration's H
             refold = folded and dynamic_mask
-            L.call('vdvc_residual_rhs', c, Nx, Ny, Nz, h, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
+            rhs = (c, Nx, Ny, Nz, h, L.ptr(G[0]), L.ptr(G[1]), L.ptr(G[2]),
                    gt, L.ptr(self.f), self.itype, mean0_f, L.ptr(gvals),
                    rnd(g_mean), rnd(kscale), L.ptr(bmask),
                    L.ptr(mask), None, L.ptr(b),
                    L.ptr(Gw[0]) if refold else None, L.ptr(Gw[1]) if refold else None,
                    L.ptr(Gw[2]) if refold else None, s4 if (disp or log is not None) else None)
+            # H uses the b-mask only with a dynamic mask; otherwise the
+            # frozen-H mask0 (correlate_gpu.py:1042-1053, 1113-1126).  With a
+            # dynamic mask the Jacobi preconditioner of this iteration's H is
+            # taken in the residual pass itself.
+            if dynamic_mask:
+                L.call('vdvc_residual_rhs_jacobi', *rhs, rk[0], rk[1], rk[2], rk[3], rk[4], llf,
+                       L.ptr(hutch), L.ptr(diag), L.ptr(Minv))
+            else:
+                L.call('vdvc_residual_rhs', *rhs)
             last = (rnd(g_mean), rnd(kscale), bmask)
             if l0 is not None:
                 reg.matvec(self.M, U, RU)
                 K.axpby(-llf, RU, 1.0, b)
-            # H uses the b-mask only with a dynamic mask; otherwise the
-            # frozen-H mask0 (correlate_gpu.py:1042-1053, 1113-1126)
             hm = bmask if dynamic_mask else mask0
-            if dynamic_mask:
-                jacobi(bmask)
             Gh, mh = hmask(hm)
             L.call('vdvc_pcg', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]), L.ptr(Gh[2]), gt,
                    L.ptr(mh),
