@@ -424,15 +424,19 @@ __device__ __forceinline__ void tri_fetch(const GI* __restrict__ img, __amdgpu_b
 }
 
 // the same combination in float32 (the mixed-precision form's weights):
-// same order of operations, each rounded to float
+// each x, y and z interpolation one product and one fused multiply-add,
+// written out so that its rounding does not depend on how the compiler
+// contracts the expression where it is inlined
 template <typename GI>
 __device__ __forceinline__ float tri_combine_f32(const TriTaps<GI>& t) {
   const float fx = (float)t.fx, fy = (float)t.fy, fz = (float)t.fz;
   const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
   const float v000 = t.v[0], v100 = t.v[1], v010 = t.v[2], v110 = t.v[3];
   const float v001 = t.v[4], v101 = t.v[5], v011 = t.v[6], v111 = t.v[7];
-  return gz0 * (gy0 * (gx0 * v000 + fx * v100) + fy * (gx0 * v010 + fx * v110)) +
-         fz * (gy0 * (gx0 * v001 + fx * v101) + fy * (gx0 * v011 + fx * v111));
+  const float a00 = fmaf(fx, v100, gx0 * v000), a10 = fmaf(fx, v110, gx0 * v010);
+  const float a01 = fmaf(fx, v101, gx0 * v001), a11 = fmaf(fx, v111, gx0 * v011);
+  const float b0 = fmaf(fy, a10, gy0 * a00), b1 = fmaf(fy, a11, gy0 * a01);
+  return fmaf(fz, b1, gz0 * b0);
 }
 
 // sample_tri's combination, same operations in the same order
