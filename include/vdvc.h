@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define VDVC_ABI_VERSION 11
+#define VDVC_ABI_VERSION 12
 
 enum { VDVC_F32 = 0, VDVC_BF16 = 1, VDVC_F16 = 2, VDVC_F64 = 3 };
 /* image element types of the post-solve stages (the images' own dtypes;
@@ -277,6 +277,13 @@ int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d
  * evaluated one rounded operation at a time from the left; out[3n+1] =
  * out[3n+2] = 0.  Replaces the V construction of correlate_gpu.py:1016-1020. */
 int vdvc_calibration_mode(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, int rtype, void* d_out);
+
+/* h_out2 = (a.b, a.c) in FP64, each equal bit for bit to vdvc_dot of the
+ * pair, in one pass and one host round trip.  Replaces the two dot products
+ * of the ll calibration, H0 = V.HV and L0 = V.LV (engine/correlate_gpu.py:
+ * 1022-1025). */
+int vdvc_dot2(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b,
+              const void* d_c, double* h_out2);
 
 /* The GN update and its convergence norms: U <- U + dU in rtype, then
  * h_out2 = (dU.dU, U.U) in FP64, equal bit for bit to vdvc_axpby(1, dU, 1, U)

@@ -35,7 +35,7 @@ def test_exports_every_declared_symbol(lib):
 
 def test_abi_version(lib):
     from voxeldvc_amd import _lib
-    assert lib.vdvc_abi_version() == _lib.ABI_VERSION == 11
+    assert lib.vdvc_abi_version() == _lib.ABI_VERSION == 12
 
 
 def test_voxel_pitch_matches_host_layout(lib):

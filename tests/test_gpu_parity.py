@@ -244,7 +244,8 @@ def test_residual_rhs_jacobi_matches_two_passes(gpu, shape, form, reg, monkeypat
 @pytest.mark.parametrize('n', [1, 1000, 824875])
 def test_gn_update_matches_axpby_and_dots(gpu, n, rd):
     """vdvc_gn_update equals vdvc_axpby(1, dU, 1, U) followed by the two
-    vdvc_dot norms, bit for bit, and the norms match NumPy's float64 sums."""
+    vdvc_dot norms, bit for bit, and the norms match NumPy's float64 sums;
+    vdvc_dot2 equals its two vdvc_dot products bit for bit."""
     import torch
     from voxeldvc_amd import _lib as L
     from voxeldvc_amd.engine import kernels_dvc as K
@@ -258,6 +259,9 @@ def test_gn_update_matches_axpby_and_dots(gpu, n, rd):
     K.axpby(1.0, dU, 1.0, U2)
     assert torch.equal(U1, U2)
     assert s2[0] == K.dot(dU, dU) and s2[1] == K.dot(U2, U2)
+    d2 = (L._d * 2)()
+    L.call('vdvc_dot2', L.ctx(), n, L.rtype_of(U1), L.ptr(dU), L.ptr(U1), L.ptr(U2), d2)
+    assert d2[0] == K.dot(dU, U1) and d2[1] == K.dot(dU, U2)
     uh, dh = L.to_host(U2).astype(np.float64), L.to_host(dU).astype(np.float64)
     assert abs(s2[0] - dh @ dh) <= 1e-12 * (dh @ dh) and abs(s2[1] - uh @ uh) <= 1e-12 * (uh @ uh)
 

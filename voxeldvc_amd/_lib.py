@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, 'libvdvc.so')
 
 F32, BF16, F16, F64 = 0, 1, 2, 3
 U8, U16, I16, BOOL = 4, 5, 6, 7
-ABI_VERSION = 11
+ABI_VERSION = 12
 REG_NONE, REG_K, REG_EQGAP = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -64,6 +64,7 @@ SIGNATURES = {
                      _vp],
     'vdvc_dot': [_vp, _i64, _i, _vp, _vp, _dp],
     'vdvc_gn_update': [_vp, _i64, _i, _vp, _vp, _dp],
+    'vdvc_dot2': [_vp, _i64, _i, _vp, _vp, _vp, _dp],
     'vdvc_axpby': [_vp, _i64, _i, _d, _vp, _d, _vp],
     'vdvc_calibration_mode': [_vp, _i, _i, _i, _i, _i, _vp],
     'vdvc_mul': [_vp, _i64, _i, _vp, _vp, _vp],

@@ -1273,6 +1273,28 @@ int vdvc_dot(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d
   return fetch(ctx, ctx->red.p, sizeof(double), h_out);
 }
 
+int vdvc_dot2(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_a, const void* d_b,
+              const void* d_c, double* h_out2) {
+  if (!ctx || !d_a || !d_b || !d_c || !h_out2) return fail(-1, "NULL argument");
+  if (n < 0) return fail(-1, "n < 0");
+  if (!rtype_ok(rtype)) return fail(-1, "rtype must be VDVC_F32 or VDVC_F64");
+  int rc;
+  const int nb = 1024;
+  if ((rc = ctx->stats.ensure((size_t)nb * 2 * sizeof(double)))) return rc;
+  if ((rc = ctx->red.ensure(8 * sizeof(double)))) return rc;
+  if (rtype == VDVC_F64)
+    VDVC_LAUNCH(vdvc::launch_dot2(n, static_cast<const double*>(d_a),
+                                  static_cast<const double*>(d_b),
+                                  static_cast<const double*>(d_c), ctx->stats.as<double>(), nb,
+                                  ctx->stream));
+  else
+    VDVC_LAUNCH(vdvc::launch_dot2(n, static_cast<const float*>(d_a),
+                                  static_cast<const float*>(d_b), static_cast<const float*>(d_c),
+                                  ctx->stats.as<double>(), nb, ctx->stream));
+  VDVC_LAUNCH(vdvc::launch_reduce(ctx->stats.as<double>(), nb, 2, ctx->red.as<double>(), ctx->stream));
+  return fetch(ctx, ctx->red.p, 2 * sizeof(double), h_out2);
+}
+
 int vdvc_gn_update(vdvc_ctx* ctx, int64_t n, int rtype, const void* d_dU, void* d_U,
                    double* h_out2) {
   if (!ctx || !d_dU || !d_U || !h_out2) return fail(-1, "NULL argument");

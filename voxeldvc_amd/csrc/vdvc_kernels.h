@@ -302,6 +302,9 @@ int launch_axpby(long long n, double a, const R* x, double b, R* y, hipStream_t 
 template <typename R>
 int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hipStream_t st);
 template <typename R>
+int launch_dot2(long long n, const R* a, const R* b, const R* c, double* part, int nparts,
+                hipStream_t st);
+template <typename R>
 int launch_gn_update(long long n, const R* dU, R* U, double* part, int nparts, hipStream_t st);
 template <typename R>
 int launch_calib_mode(int Nx, int Ny, int Nz, int h, R* out, hipStream_t st);

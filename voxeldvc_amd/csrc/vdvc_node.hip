@@ -798,6 +798,29 @@ __global__ __launch_bounds__(VDVC_BLOCK) void gn_update_kernel(long long n,
   }
 }
 
+// two partial FP64 dot products a.b and a.c in one pass: per value the grid,
+// per-thread order and block reduction of dot_kernel, so each equals the
+// corresponding dot product bit for bit
+template <typename R>
+__global__ __launch_bounds__(VDVC_BLOCK) void dot2_kernel(long long n, const R* __restrict__ a,
+                                                          const R* __restrict__ b,
+                                                          const R* __restrict__ c,
+                                                          double* __restrict__ part) {
+  __shared__ double lds[4 * 2];
+  double v[2] = {0.0, 0.0};
+  for (long long q = (long long)blockIdx.x * VDVC_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * VDVC_BLOCK) {
+    const R x = a[q];
+    v[0] += (double)x * b[q];
+    v[1] += (double)x * c[q];
+  }
+  block_sum_f64<2>(v, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
 // Hutchinson accumulation acc += z * Rz (FP64), z a Rademacher probe
 template <typename R>
 __global__ __launch_bounds__(VDVC_BLOCK) void hutch_acc_kernel(long long n,
@@ -1032,6 +1055,13 @@ int launch_dot(long long n, const R* a, const R* b, double* part, int nparts, hi
 }
 
 template <typename R>
+int launch_dot2(long long n, const R* a, const R* b, const R* c, double* part, int nparts,
+                hipStream_t st) {
+  hipLaunchKernelGGL(dot2_kernel<R>, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, a, b, c, part);
+  return (int)hipGetLastError();
+}
+
+template <typename R>
 int launch_gn_update(long long n, const R* dU, R* U, double* part, int nparts, hipStream_t st) {
   hipLaunchKernelGGL(gn_update_kernel<R>, dim3(nparts), dim3(VDVC_BLOCK), 0, st, n, dU, U, part);
   return (int)hipGetLastError();
@@ -1054,6 +1084,8 @@ int launch_gn_update(long long n, const R* dU, R* U, double* part, int nparts, h
   template int launch_axpby<R>(long long, double, const R*, double, R*, hipStream_t);        \
   template int launch_dot<R>(long long, const R*, const R*, double*, int, hipStream_t);      \
   template int launch_gn_update<R>(long long, const R*, R*, double*, int, hipStream_t);      \
+  template int launch_dot2<R>(long long, const R*, const R*, const R*, double*, int,          \
+                              hipStream_t);                                                  \
   template int launch_calib_mode<R>(int, int, int, int, R*, hipStream_t);
 VDVC_INST(float)
 VDVC_INST(double)

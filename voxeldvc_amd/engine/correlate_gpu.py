@@ -632,6 +632,7 @@ class _Level:
                 raise ValueError("K_ref_elastic is required when l0 is given and "
                                  "reg_type='equilibrium_gap'")
             T = 10 * h
+            s2c = (L._d * 2)()
             Gh, mh = hmask(mask0)
             if reg_type == 'equilibrium_gap':
                 d = L.empty(Ndof, wd)
@@ -652,8 +653,9 @@ class _Level:
                 L.call('vdvc_matvec_H', c, Nx, Ny, Nz, h, L.ptr(Gh[0]), L.ptr(Gh[1]),
                        L.ptr(Gh[2]), gt, L.ptr(mh), L.ptr(Vd), L.ptr(Hv))
                 Lv = reg.matvec(self.M, Vd, L.empty(Ndof, wd))
-                H0 = K.dot(Vd, Hv)
-                L0 = K.dot(Vd, Lv)
+                # H0 = V.HV and L0 = V.LV in one pass (each as its own dot)
+                L.call('vdvc_dot2', c, Ndof, L.rtype_of(Vd), L.ptr(Vd), L.ptr(Hv), L.ptr(Lv), s2c)
+                H0, L0 = s2c[0], s2c[1]
             ll = c_reg * (l0 / T) ** reg_exponent * H0 / L0
             if disp:
                 print('Regularization: %s with lambda = %2.3e' % (reg_type, ll))
