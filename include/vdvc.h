@@ -251,7 +251,9 @@ int vdvc_jacobi(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, 
 
 /* Jacobi-PCG for (H + ll R) x = b from x0 = 0, FP64 dots, stop when
  * sqrt(r.r/(b.b + 1e-30)) < tol, device-resident loop (the host polls the
- * stop flag every check_every iterations).  Replaces pcg
+ * stop flag first after check_every iterations, then after the iterations
+ * the residual's rate since the last poll needs, at most 4 * check_every;
+ * the schedule changes no result).  Replaces pcg
  * (engine/kernels_dvc.py:550-583) with the matvec_full closure of
  * correlate_gpu.py:1122-1126. */
 int vdvc_pcg(vdvc_ctx* ctx, int Nx, int Ny, int Nz, int h, const void* d_gx, const void* d_gy,

@@ -503,7 +503,8 @@ def correlate_gpu(f_pix, g_pix, Nx, Ny, Nz, h, xp=np, g_origin=(0, 0, 0), glt=0.
     """Matrix-free GN DVC correlation on one level; returns (U, res, dU)
     (correlate_gpu.py:828-1140).  Extensions: `storage` (see module doc),
     `log` (list receiving one dict per GN iteration), `pcg_check_every`
-    (PCG iterations between host polls of the device stop flag) and
+    (PCG iterations before the first host poll of the device stop flag;
+    later polls follow the residual's rate) and
     `callback(ik, U)` (called after every GN update with the device U) and
     `refine_fp64` (number of FP32-against-FP64 iterative-refinement steps
     after the GN loop; U is then returned as float64)."""
