@@ -1352,12 +1352,21 @@ __global__ __launch_bounds__(VDVC_BLOCK) void level_image_h2_kernel(MeshDev M,
     __half2* __restrict__ dr = reinterpret_cast<__half2*>(dst + row * dp);
     for (int c0 = 0; c0 < nch; c0 += kCh) {
       ST raw[kCh][2];
+      // a row that starts 8-byte aligned (every other row of an odd-width
+      // float32 volume) takes each pair as one 8-byte load
+      const bool al8 = ((reinterpret_cast<uintptr_t>(sr) & 7) == 0) && sizeof(ST) == 4;
 #pragma unroll
       for (int j = 0; j < kCh; ++j) {
         const int x = (c0 + j) * 128 + 2 * lane;
         const bool ok = c0 + j < nch;
-        raw[j][0] = (ok && x < nx) ? sr[x] : ST(0);
-        raw[j][1] = (ok && x + 1 < nx) ? sr[x + 1] : ST(0);
+        if (al8 && ok && x + 1 < nx) {
+          const float2 q = *reinterpret_cast<const float2*>(sr + x);
+          raw[j][0] = q.x;
+          raw[j][1] = q.y;
+        } else {
+          raw[j][0] = (ok && x < nx) ? sr[x] : ST(0);
+          raw[j][1] = (ok && x + 1 < nx) ? sr[x + 1] : ST(0);
+        }
       }
 #pragma unroll
       for (int j = 0; j < kCh; ++j) {
