@@ -469,7 +469,9 @@ constexpr int kSampleRows = 2;
 // contracts the expression.  F32W: the eight interpolation weights and the
 // combination in float32 (the mixed-precision form) instead of SciPy's
 // float64 order.
-template <typename GI, bool O32, bool F32W>
+// BM: a b-mask is given (statistics over it); without one no b-mask byte is
+// loaded per voxel.
+template <typename GI, bool O32, bool F32W, bool BM>
 __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
     MeshDev M, TilesDev T, SampleIn S, const typename RealOf<GI>::type* __restrict__ U) {
   using R = typename RealOf<GI>::type;
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
         if (act) {
           gvals[vi] = gv;
           mask_out[vi] = mk ? 1 : 0;
-          const bool bm = bmask ? (bmb != 0) : mk;
+          const bool bm = BM ? (bmb != 0) : mk;
           if (bm) {
             const double mlt = (double)(mxz * axis_mult(y, h, M.Nvy - 1));
             const double gd = (double)gv;
@@ -588,11 +590,12 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
           R cx, cy, cz;
           in[q] = coords(tq, cx, cy, cz);
           tri_fetch<O32>(img, rs, S.gnx, S.gny, S.gnz, cx, cy, cz, tp[q]);
-          // mask bytes: unconditional loads (an absent mask reads a byte of
-          // the image instead, ignored) so that they are in flight with the taps
+          // mask bytes: unconditional loads (an absent mask0 reads a byte of
+          // the image instead, ignored) so that they are in flight with the
+          // taps; the b-mask byte only when there is a b-mask
           const long long vq = base + (long long)tq * M.pitch;
           m0[q] = *(mask0 ? mask0 + vq : dummy);
-          bm[q] = *(bmask ? bmask + vq : dummy);
+          bm[q] = BM ? bmask[vq] : 0;
         }
 #pragma unroll
         for (int q = 0; q < kSampleRows; ++q)
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(VDVC_BLOCK) void sample_kernel(
         const bool mk = inb && m0 && (gv > glt);
         gvals[vi] = gv;
         mask_out[vi] = mk ? 1 : 0;
-        const bool bmv = bmask ? (bmask[vi] != 0) : mk;
+        const bool bmv = BM ? (bmask[vi] != 0) : mk;
         if (bmv) {
           const double mlt =
               (double)(mxt * axis_mult(z, h, M.Nvz - 1) * axis_mult(y, h, M.Nvy - 1));
@@ -1251,7 +1254,10 @@ int launch_sample(int gitype, const MeshDev& M, const TilesDev& T, const SampleI
   // 32-bit buffer offsets: float and half images below 2^32 bytes
   const long long nimg = (long long)S.gnx * S.gny * S.gnz;
   const bool o32 = gitype == GT_F32 ? nimg < (1LL << 30) : gitype == GT_F64 ? false : nimg < (1LL << 31);
-#define VDVC_SAMPLE_K(GI, W) (o32 ? sample_kernel<GI, true, W> : sample_kernel<GI, false, W>)
+  const bool bm = S.bmask != nullptr;
+#define VDVC_SAMPLE_K(GI, W)                                                          \
+  (o32 ? (bm ? sample_kernel<GI, true, W, true> : sample_kernel<GI, true, W, false>) \
+       : (bm ? sample_kernel<GI, false, W, true> : sample_kernel<GI, false, W, false>))
   if (gitype == GT_F32) {
     if (f32w)
       hipLaunchKernelGGL(VDVC_SAMPLE_K(float, true), dim3(nblk), dim3(VDVC_BLOCK), 0, st, M, T, S,
